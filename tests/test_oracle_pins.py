@@ -1,0 +1,171 @@
+"""Pins of the CPU oracle against what the mathematics fixes.
+
+The reference ships no tests or fixtures for this path, so every restatement
+in ``oracle/`` is checked here against an independent fact:
+
+* configuration tables: per-type counts and bijectivity;
+* first-order ph-ph block: the singlet eigenvalues must be a subset of an
+  independently built spin-orbital CIS spectrum;
+* zero-coupling limit: with all factors zero the spectrum is the union of the
+  orbital-energy differences;
+* full matrix: the literal element restatement (``elements``) and the
+  density-fitted reformulation (``dfsigma``) agree element by element;
+* Davidson: eigenvalues match ARPACK and the Lowdin-partitioned roots;
+* SIL: the autocorrelation matches exp(-iHt) from a dense eigendecomposition;
+* network: analytic gradients match central finite differences.
+"""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as sla
+
+from adc_ottawa_amd.synth import make_system
+from oracle import csf, davidson, dfsigma, elements, mlp, schur, sil
+
+
+def test_csf_counts_and_bijectivity():
+    for o, v in [(2, 3), (3, 5), (5, 36)]:
+        c = csf.counts(o, v)
+        typ, a, b, k, l = csf.table(o, v)
+        assert typ.size == c["ndim"]
+        for t, name in zip(range(6), ["n1", "t1", "t2", "t3", "t4", "t5"]):
+            assert np.sum(typ == t) == c[name]
+        key = set(zip(typ.tolist(), a.tolist(), b.tolist(), k.tolist(), l.tolist()))
+        assert len(key) == c["ndim"]
+    c = csf.counts(16, 200)
+    assert (c["n1"], c["t2"], c["t3"], c["t4"], c["n2"]) == (3200, 318400, 24000, 2388000, 5121600)
+
+
+def spin_orbital_cis(sys):
+    o, v = sys.nocc, sys.nvirt
+    V = elements.eri(sys.B)
+    e = sys.eps
+    occ = [(p, s) for p in range(o) for s in (0, 1)]
+    vir = [(p, s) for p in range(o, o + v) for s in (0, 1)]
+    exc = [(i, a) for i in occ for a in vir]
+    n = len(exc)
+    A = np.zeros((n, n))
+    for x, (i, a) in enumerate(exc):
+        for y, (j, b) in enumerate(exc):
+            val = 0.0
+            if i == j and a == b:
+                val += e[a[0]] - e[i[0]]
+            if a[1] == i[1] and j[1] == b[1]:
+                val += V[a[0], i[0], j[0], b[0]]
+            if a[1] == b[1] and j[1] == i[1]:
+                val -= V[a[0], b[0], j[0], i[0]]
+            A[x, y] = val
+    return np.linalg.eigvalsh(A)
+
+
+def test_first_order_block_is_singlet_cis():
+    sys = make_system(3, 4, 6, seed=3)
+    blk = elements.build_blocks(sys, second_order=False)
+    singlet = np.linalg.eigvalsh(blk.H11)
+    so = spin_orbital_cis(sys)
+    for w in singlet:
+        assert np.min(np.abs(so - w)) < 1e-12
+
+
+def test_zero_coupling_limit():
+    sys = make_system(3, 5, 4, seed=1)
+    sys.B[:] = 0.0
+    blk = elements.build_blocks(sys)
+    o, v = 3, 5
+    e = sys.eps
+    expect = np.concatenate([(e[o:, None] - e[None, :o]).ravel(), dfsigma.diag2(sys)])
+    got = np.linalg.eigvalsh(blk.dense())
+    assert np.allclose(np.sort(expect), got, atol=1e-14)
+    df = dfsigma.DF(sys)
+    assert np.allclose(df.diag(), blk.diag(), atol=1e-14)
+
+
+@pytest.mark.parametrize("o,v,naux,seed", [(2, 3, 2, 0), (3, 5, 7, 1), (4, 7, 11, 2), (5, 9, 13, 3)])
+def test_literal_vs_density_fitted(o, v, naux, seed):
+    sys = make_system(o, v, naux, seed=seed)
+    blk = elements.build_blocks(sys, apply_minc=False)
+    df = dfsigma.DF(sys)
+    scale = np.abs(blk.H11).max()
+    assert np.abs(blk.H11 - df.H11).max() < 1e-13 * scale
+    assert np.abs(blk.H11 - blk.H11.T).max() < 1e-13 * scale
+    # coupling block column by column through the DF sigma
+    n1 = o * v
+    Hd = blk.dense()
+    Hdf = df.sigma(np.eye(blk.ndim))
+    assert np.abs(Hd - Hdf).max() < 1e-13 * np.abs(Hd).max()
+    assert n1 == blk.n1
+
+
+def test_oracle_eigenvalues_cfg_water():
+    sys = make_system(5, 36, 164, seed=0)
+    blk = elements.build_blocks(sys)
+    op = sla.LinearOperator((blk.ndim, blk.ndim), matvec=blk.sigma, dtype=np.float64)
+    w = np.sort(sla.eigsh(op, k=8, which="SA", tol=1e-13)[0])[:5]
+    nb = schur.count_below(blk.H11, blk.C, blk.D, 5)
+    assert nb >= 1
+    ex = schur.roots(blk.H11, blk.C, blk.D, nb)
+    assert np.abs(ex - w[:nb]).max() < 1e-12
+    r = davidson.davidson(blk.sigma, blk.diag(), 5, blocksize=5, maxvec=40, tol=1e-7,
+                          precon=1, deflate=True, maxiter=400)
+    assert np.abs(r["eigval"] - w).max() < 1e-9
+
+
+@pytest.mark.parametrize("precon,deflate", [(1, False), (2, True)])
+def test_oracle_davidson_small(precon, deflate):
+    sys = make_system(3, 8, 10, seed=4)
+    df = dfsigma.DF(sys)
+    H = df.sigma(np.eye(df.n1 + df.D2.size))
+    w = np.linalg.eigvalsh(H)[:4]
+    r = davidson.davidson(df.sigma, df.diag(), 4, tol=1e-8, precon=precon, deflate=deflate, maxiter=400)
+    assert np.abs(r["eigval"] - w).max() < 1e-10
+    # Ritz vectors are eigenvectors
+    X = r["eigvec"]
+    assert np.abs(H @ X - X * r["eigval"]).max() < 1e-7
+
+
+def test_oracle_sil_exact_propagator():
+    sys = make_system(3, 5, 6, seed=5)
+    df = dfsigma.DF(sys)
+    H = df.sigma(np.eye(df.n1 + df.D2.size))
+    w, U = np.linalg.eigh(H)
+    f = np.zeros(H.shape[0])
+    f[: df.n1] = np.random.default_rng(0).normal(0, 0.1, df.n1)
+    def func(x):
+        return -1j * (H @ x)
+    times, auto = sil.propagate(func, f, tout=0.1, tfinal=4.0, kdim=12, autotol=1e-12)
+    c = U.T @ f
+    exact = np.array([np.sum(c * c * np.exp(-1j * w * t)) for t in times])
+    assert np.abs(auto - exact).max() < 1e-9 * np.sum(c * c)
+    # t = 0 record is |F|^2
+    assert abs(auto[0] - np.dot(f, f)) < 1e-14
+
+
+def test_oracle_mlp_gradients():
+    rng = np.random.default_rng(1)
+    p = mlp.init_params(0)
+    p["w3"] = rng.normal(0, 0.1, mlp.H)
+    n = 37
+    x = mlp.transform(rng.normal(0, 2, n), rng.normal(1, 0.2, n), rng.normal(0, 1, n))
+    cls = rng.integers(0, mlp.NCLASS, n)
+    dn = rng.normal(0, 1, n)
+    en = rng.normal(0, 1, n)
+    g = mlp.backward(p, x, cls, dn, en)
+    h = 1e-6
+    for k in p:
+        flat = p[k].reshape(-1)
+        for idx in rng.choice(flat.size, size=min(5, flat.size), replace=False):
+            old = flat[idx]
+            flat[idx] = old + h
+            lp = mlp.loss(p, x, cls, dn, en)
+            flat[idx] = old - h
+            lm = mlp.loss(p, x, cls, dn, en)
+            flat[idx] = old
+            fd = (lp - lm) / (2 * h)
+            assert abs(fd - g[k].reshape(-1)[idx]) < 1e-6 * max(1.0, abs(fd)), k
+
+
+def test_bf16_rounding():
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 3 * 2 ** -9, -3.14159], dtype=np.float32)
+    r = mlp.bf16_round(x)
+    # ties go to the even significand; bf16 keeps 8 significant bits
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.0 + 2 ** -6 and r[3] == 1.0 + 2 ** -7
+    assert np.all(np.abs(r - x) <= np.abs(x) * 2 ** -8)
