@@ -1,0 +1,221 @@
+// C ABI (include/adc2g.h): argument checks, exception -> error code mapping,
+// host <-> device staging for the host-pointer entry points.
+#include <cstring>
+#include <memory>
+
+#include "handle.hpp"
+#include "solvers.hpp"
+
+struct adc2g_handle {
+  adc2g::Handle h;
+};
+
+namespace adc2g {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+template <class F>
+static int32_t guarded(F&& f) {
+  try {
+    f();
+    return ADC2G_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_last_error("host allocation failed");
+    return ADC2G_ENOMEM;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return ADC2G_EHIP;
+  }
+}
+
+}  // namespace adc2g
+
+using namespace adc2g;
+
+extern "C" {
+
+int32_t adc2g_create(int32_t nocc, int32_t nvirt, int32_t naux, const double* eps, const double* B,
+                     const adc2g_opts* opts, adc2g_handle** out) {
+  return guarded([&] {
+    ARG_CHECK(out != nullptr, "out must not be NULL");
+    *out = nullptr;
+    adc2g_opts o{};
+    o.device = 0;
+    o.build_phph = 1;
+    o.keep_lowprec = 0;
+    if (opts) o = *opts;
+    auto hp = std::make_unique<adc2g_handle>();
+    hp->h.create(nocc, nvirt, naux, eps, B, o);
+    *out = hp.release();
+  });
+}
+
+int32_t adc2g_destroy(adc2g_handle* h) {
+  return guarded([&] {
+    if (!h) return;
+    (void)hipSetDevice(h->h.device);
+    if (h->h.st) (void)hipStreamSynchronize(h->h.st);
+    delete h;
+  });
+}
+
+int32_t adc2g_dims(const adc2g_handle* h, int64_t* ndim, int64_t* n1, int64_t counts[5]) {
+  return guarded([&] {
+    ARG_CHECK(h != nullptr, "NULL handle");
+    if (ndim) *ndim = h->h.d.ndim;
+    if (n1) *n1 = h->h.d.n1;
+    if (counts)
+      for (int t = 0; t < 5; ++t) counts[t] = h->h.d.cnt[t];
+  });
+}
+
+int32_t adc2g_diagonal(adc2g_handle* h, double* hii) {
+  return guarded([&] {
+    ARG_CHECK(h && hii, "NULL argument");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    HIP_CHECK(hipMemcpyAsync(hii, h->h.hii.get(), h->h.d.ndim * sizeof(double), hipMemcpyDeviceToHost, h->h.st));
+    HIP_CHECK(hipStreamSynchronize(h->h.st));
+  });
+}
+
+int32_t adc2g_phph(adc2g_handle* h, double* h11) {
+  return guarded([&] {
+    ARG_CHECK(h && h11, "NULL argument");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    const size_t n = (size_t)h->h.d.n1 * h->h.d.n1;
+    HIP_CHECK(hipMemcpyAsync(h11, h->h.H11.get(), n * sizeof(double), hipMemcpyDeviceToHost, h->h.st));
+    HIP_CHECK(hipStreamSynchronize(h->h.st));
+  });
+}
+
+int32_t adc2g_sigma(adc2g_handle* h, int32_t nvec, const double* X, double* W, int32_t precision) {
+  return guarded([&] {
+    ARG_CHECK(h && X && W, "NULL argument");
+    ARG_CHECK(nvec >= 0, "nvec must be >= 0");
+    Handle& H = h->h;
+    HIP_CHECK(hipSetDevice(H.device));
+    const Prec p = prec_from_int(precision);
+    if (nvec == 0) return;
+    const size_t n = (size_t)H.d.ndim * nvec;
+    DevBuf<double> dX, dW;
+    dX.alloc(n);
+    dW.alloc(n);
+    HIP_CHECK(hipMemcpyAsync(dX.get(), X, n * sizeof(double), hipMemcpyHostToDevice, H.st));
+    H.sigma(dX.get(), H.d.ndim, dW.get(), H.d.ndim, nvec, p, H.st);
+    HIP_CHECK(hipMemcpyAsync(W, dW.get(), n * sizeof(double), hipMemcpyDeviceToHost, H.st));
+    HIP_CHECK(hipStreamSynchronize(H.st));
+  });
+}
+
+int32_t adc2g_sigma_device(adc2g_handle* h, int32_t nvec, const double* dX, double* dW, int32_t precision,
+                           void* stream) {
+  return guarded([&] {
+    ARG_CHECK(h && dX && dW, "NULL argument");
+    ARG_CHECK(nvec >= 0, "nvec must be >= 0");
+    Handle& H = h->h;
+    HIP_CHECK(hipSetDevice(H.device));
+    H.sigma(dX, H.d.ndim, dW, H.d.ndim, nvec, prec_from_int(precision), (hipStream_t)stream);
+  });
+}
+
+int32_t adc2g_matxvec_treal(adc2g_handle* h, const double* v1, double* v2, int32_t precision) {
+  return guarded([&] {
+    ARG_CHECK(h && v1 && v2, "NULL argument");
+    Handle& H = h->h;
+    HIP_CHECK(hipSetDevice(H.device));
+    matxvec_treal_host(H, v1, v2, prec_from_int(precision));
+  });
+}
+
+int32_t adc2g_davidson(adc2g_handle* h, const adc2g_dav_params* p, double* eigval, double* eigvec,
+                       adc2g_dav_stats* stats) {
+  return guarded([&] {
+    ARG_CHECK(h && p && eigval, "NULL argument");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    davidson_run(h->h, *p, eigval, eigvec, stats, nullptr);
+  });
+}
+
+int32_t adc2g_sil_autocorr(adc2g_handle* h, const adc2g_sil_params* p, const double* fvec, double* auto0,
+                           int64_t nrec_cap, int64_t* nrec_out, adc2g_sil_stats* stats) {
+  return guarded([&] {
+    ARG_CHECK(h && p && fvec && auto0, "NULL argument");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    sil_run(h->h, *p, fvec, auto0, nrec_cap, nrec_out, stats);
+  });
+}
+
+int32_t adc2g_precon_harvest(adc2g_handle* h, const adc2g_dav_params* p, int64_t max_records, int64_t* added) {
+  return guarded([&] {
+    ARG_CHECK(h && p, "NULL argument");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    precon_harvest(h->h, *p, max_records, added);
+  });
+}
+
+int32_t adc2g_precon_train(adc2g_handle* h, const adc2g_handle* data, int32_t epochs, int32_t batch, double lr,
+                           uint64_t seed, double* loss_out) {
+  return guarded([&] {
+    ARG_CHECK(h && data, "NULL argument");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    precon_train(h->h, data->h, epochs, batch, lr, seed, loss_out);
+  });
+}
+
+int32_t adc2g_precon_nparams(void) { return precon_nparams(); }
+
+int32_t adc2g_precon_get(const adc2g_handle* h, float* params) {
+  return guarded([&] {
+    ARG_CHECK(h && params, "NULL argument");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    precon_get(h->h, params);
+  });
+}
+
+int32_t adc2g_precon_set(adc2g_handle* h, const float* params) {
+  return guarded([&] {
+    ARG_CHECK(h && params, "NULL argument");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    precon_set(h->h, params);
+  });
+}
+
+int32_t adc2g_precon_eval(adc2g_handle* h, int64_t n, const float* feat, const int32_t* cls, const float* dn,
+                          const float* en, float* m_out, float* grad_out, double* loss_out) {
+  return guarded([&] {
+    ARG_CHECK(h && feat && cls && dn && en, "NULL argument");
+    ARG_CHECK(n > 0, "n must be > 0");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    precon_eval(h->h, n, feat, cls, dn, en, m_out, grad_out, loss_out);
+  });
+}
+
+int32_t adc2g_last_error(char* buf, int32_t len) {
+  if (!buf || len <= 0) return ADC2G_EARG;
+  std::strncpy(buf, g_last_error.c_str(), (size_t)len - 1);
+  buf[len - 1] = '\0';
+  return ADC2G_OK;
+}
+
+int32_t adc2g_sigma_profile(adc2g_handle* h, int32_t enable, double* ms_gemm, double* ms_stream,
+                            double* flops_gemm, double* bytes_stream) {
+  return guarded([&] {
+    ARG_CHECK(h != nullptr, "NULL handle");
+    Handle& H = h->h;
+    H.timing = enable != 0;
+    if (ms_gemm || ms_stream || flops_gemm || bytes_stream) {
+      if (H.last_ms_gemm < 0) throw Error(ADC2G_ESTATE, "no timed sigma-vector yet");
+      if (ms_gemm) *ms_gemm = H.last_ms_gemm;
+      if (ms_stream) *ms_stream = H.last_ms_stream;
+      if (flops_gemm) *flops_gemm = H.last_flops_gemm;
+      if (bytes_stream) *bytes_stream = H.last_bytes_stream;
+    }
+  });
+}
+
+}  // extern "C"

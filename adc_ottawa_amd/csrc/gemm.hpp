@@ -1,0 +1,405 @@
+// MFMA GEMM for gfx950: C(map) = alpha * sum_k A(m,k) B(k,n) + beta * C(map).
+//
+// One kernel template serves every contraction of the sigma-vector, the
+// ph-ph build and the Davidson subspace algebra.  Operands are addressed by
+// element strides, so row-/column-major and the index-permuted views of the
+// density-fitting factors need no copies:
+//     A(m,k) = A[b*sab + m*sam + k*sak],   B(k,n) = B[b*sbb + k*sbk + n*sbn].
+// The output goes through a small set of index maps (MapKind) that write the
+// permuted layouts the next contraction reads.  Split-K writes fp partials
+// that a reduction kernel folds through the same map.
+//
+// Tiling: 256 threads = 4 waves in a 2x2 grid, block tile BM x BN, K tile BK,
+// each wave an (BM/2) x (BN/2) grid of 16x16 MFMA blocks.  Both operand tiles
+// live in LDS K-contiguous (row = m or n, BK elements + pad), double
+// buffered, so one barrier per K tile.  Within a K tile the K index is
+// permuted (lane group g of an MFMA step s takes k = g*KV + s-th slice) so
+// that every lane reads its MFMA operands as one contiguous LDS vector.
+#pragma once
+
+#include "common.hpp"
+
+namespace adc2g {
+
+enum MapKind : int {
+  MAP_STD = 0,    // off = m*scm + n*scn + b*scb
+  MAP_DOWNY = 1,  // DF down-contraction output -> Y_vv and Y_oo layouts
+  MAP_SIG1V = 2,  // (a, k + o*vec) -> sigma1
+  MAP_SIG1O = 3,  // (a*nvec + vec, k) -> sigma1
+  MAP_WUPV = 4,   // (c + v*P, k + o*vec) -> W[P][vec][c][k]
+  MAP_WUPO = 5,   // (c*nvec + vec, k + o*P) -> W[P][vec][c][k]
+};
+
+struct MapParams {
+  int kind = MAP_STD;
+  double alpha = 1.0, beta = 0.0;
+  void* C = nullptr;
+  void* C2 = nullptr;
+  int64_t scm = 0, scn = 0, scb = 0;
+  int64_t q[6] = {0, 0, 0, 0, 0, 0};
+};
+
+struct GemmArgs {
+  int M = 0, N = 0, K = 0, batch = 1, splitk = 1;
+  const void* A = nullptr;
+  int64_t sam = 0, sak = 0, sab = 0;
+  const void* B = nullptr;
+  int64_t sbk = 0, sbn = 0, sbb = 0;
+  MapParams map;
+  int out_prec = 0;         // output element type: 0 double, 1 float, 2 bfloat16
+  void* partial = nullptr;  // split-K workspace (accumulator type), batch*splitk*M*N
+};
+
+// Output offset(s) of element (m, n) of batch b. Returns the number of outputs (1 or 2).
+__device__ __forceinline__ int map_offsets(const MapParams& mp, int64_t m, int64_t n, int64_t b,
+                                           int64_t& o1, int64_t& o2) {
+  switch (mp.kind) {
+    case MAP_DOWNY: {
+      // m = P, n = ck + n1*vec; q = {o, nvec, n1, naux}
+      const int64_t o = mp.q[0], nvec = mp.q[1], n1 = mp.q[2], naux = mp.q[3];
+      const int64_t vec = n / n1, ck = n - vec * n1;
+      const int64_t c = ck / o, k = ck - c * o;
+      o1 = k + o * vec + o * nvec * c + n1 * nvec * m;
+      o2 = k + o * m + o * naux * (c * nvec + vec);
+      return 2;
+    }
+    case MAP_SIG1V: {
+      const int64_t o = mp.q[0], ldw = mp.q[1];
+      const int64_t vec = n / o, k = n - vec * o;
+      o1 = m * o + k + ldw * vec;
+      return 1;
+    }
+    case MAP_SIG1O: {
+      const int64_t o = mp.q[0], ldw = mp.q[1], nvec = mp.q[2];
+      const int64_t a = m / nvec, vec = m - a * nvec;
+      o1 = a * o + n + ldw * vec;
+      return 1;
+    }
+    case MAP_WUPV: {
+      // m = c + v*P, n = k + o*vec; q = {o, v, n1, nvec}
+      const int64_t o = mp.q[0], v = mp.q[1], n1 = mp.q[2], nvec = mp.q[3];
+      const int64_t P = m / v, c = m - P * v;
+      const int64_t vec = n / o, k = n - vec * o;
+      o1 = k + o * c + n1 * vec + n1 * nvec * P;
+      return 1;
+    }
+    case MAP_WUPO: {
+      // m = c*nvec + vec, n = k + o*P
+      const int64_t o = mp.q[0], n1 = mp.q[2], nvec = mp.q[3];
+      const int64_t c = m / nvec, vec = m - c * nvec;
+      const int64_t P = n / o, k = n - P * o;
+      o1 = k + o * c + n1 * vec + n1 * nvec * P;
+      return 1;
+    }
+    default:
+      o1 = m * mp.scm + n * mp.scn + b * mp.scb;
+      return 1;
+  }
+}
+
+template <class T>
+__device__ __forceinline__ T from_double(double x) { return (T)x; }
+template <>
+__device__ __forceinline__ __bf16 from_double<__bf16>(double x) { return (__bf16)(float)x; }
+template <class T>
+__device__ __forceinline__ double to_double(T x) { return (double)x; }
+template <>
+__device__ __forceinline__ double to_double<__bf16>(__bf16 x) { return (double)(float)x; }
+
+template <class OutT>
+__device__ __forceinline__ void map_store(const MapParams& mp, int64_t m, int64_t n, int64_t b, double val) {
+  int64_t o1, o2 = 0;
+  const int cnt = map_offsets(mp, m, n, b, o1, o2);
+  OutT* C = (OutT*)mp.C;
+  double r = mp.alpha * val;
+  if (mp.beta != 0.0) r += mp.beta * to_double(C[o1]);
+  C[o1] = from_double<OutT>(r);
+  if (cnt == 2) {
+    OutT* C2 = (OutT*)mp.C2;
+    double r2 = mp.alpha * val;
+    if (mp.beta != 0.0) r2 += mp.beta * to_double(C2[o2]);
+    C2[o2] = from_double<OutT>(r2);
+  }
+}
+
+// ---- per-type MFMA traits ---------------------------------------------------
+// KV: K elements per lane per MFMA; KM: K per MFMA (= 4*KV).
+template <class T>
+struct Mma;
+
+template <>
+struct Mma<double> {
+  typedef double Acc;
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  typedef double frag;
+  static constexpr int KV = 1;
+  static __device__ __forceinline__ frag load(const double* p) { return *p; }
+  static __device__ __forceinline__ v4 mma(frag a, frag b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // register r of lane l holds D[row][l%16]
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * r + (lane >> 4); }
+};
+
+template <>
+struct Mma<float> {
+  typedef float Acc;
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  typedef float frag;
+  static constexpr int KV = 1;
+  static __device__ __forceinline__ frag load(const float* p) { return *p; }
+  static __device__ __forceinline__ v4 mma(frag a, frag b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+template <>
+struct Mma<__bf16> {
+  typedef float Acc;
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  typedef __bf16 v8 __attribute__((ext_vector_type(8)));
+  typedef v8 frag;
+  static constexpr int KV = 8;
+  static __device__ __forceinline__ frag load(const __bf16* p) { return *(const v8*)p; }
+  static __device__ __forceinline__ v4 mma(frag a, frag b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+// XCD-aware tile order: workgroup w runs on XCD w % 8; give each XCD a
+// contiguous range of the (m-fastest) tile sequence so tiles sharing an
+// operand panel share that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int w, int T) {
+  const int per = T >> 3, rem = T & 7;
+  const int x = w & 7, slot = w >> 3;
+  return x * per + (x < rem ? x : rem) + slot;
+}
+
+template <class T, int BM, int BN, int BK, class OutT>
+__global__ void __launch_bounds__(256, 2)
+gemm_kernel(GemmArgs g) {
+  typedef Mma<T> MT;
+  typedef typename MT::Acc Acc;
+  typedef typename MT::v4 v4;
+  constexpr int KV = MT::KV;
+  constexpr int KM = 4 * KV;           // K per MFMA
+  constexpr int KS = BK / KM;          // MFMA steps per K tile
+  constexpr int PAD = 16 / sizeof(T);  // 16 B of padding per LDS row
+  constexpr int LDR = BK + PAD;
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int EA = BM * BK / 256;    // elements per thread per tile
+  constexpr int EB = BN * BK / 256;
+  static_assert(BK % KM == 0, "BK must be a multiple of the MFMA K");
+
+  __shared__ T sA[2][BM * LDR];
+  __shared__ T sB[2][BN * LDR];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  const int tiles_m = (g.M + BM - 1) / BM;
+  const int tiles_n = (g.N + BN - 1) / BN;
+  const int T_all = tiles_m * tiles_n;
+  const int t = xcd_tile(blockIdx.x, T_all);
+  const int tm = t % tiles_m, tn = t / tiles_m;
+  const int b = blockIdx.y;       // batch
+  const int ks = blockIdx.z;      // split index
+  const int m0 = tm * BM, n0 = tn * BN;
+
+  // K range of this split, in whole K tiles
+  const int ktiles_all = (g.K + BK - 1) / BK;
+  const int kt_per = (ktiles_all + g.splitk - 1) / g.splitk;
+  const int kt_begin = ks * kt_per;
+  const int kt_end = min(ktiles_all, kt_begin + kt_per);
+
+  const T* A = (const T*)g.A + (int64_t)b * g.sab;
+  const T* B = (const T*)g.B + (int64_t)b * g.sbb;
+
+  // load mapping: contiguous-K operands read rows along k, contiguous-M/N
+  // operands read along m/n; both store K-contiguous rows in LDS.
+  const bool a_kc = (g.sak == 1);
+  const bool b_kc = (g.sbk == 1);
+
+  T ra[EA], rb[EB];
+
+  // Per-thread fixed coordinates of the load pattern.  K-contiguous: the
+  // thread keeps column c and walks rows r0 + e*(256/BK).  MN-contiguous: it
+  // keeps row r and walks columns c0 + e*(256/BM).
+  constexpr int RSA = 256 / BK;   // row step, K-contiguous
+  constexpr int CSA = 256 / BM;   // column step, M-contiguous
+  constexpr int CSB = 256 / BN;
+  const int kc_c = tid % BK, kc_r = tid / BK;
+  const int ma_r = tid % BM, ma_c = tid / BM;
+  const int nb_r = tid % BN, nb_c = tid / BN;
+
+  auto load_a = [&](int kt) {
+    const int k0 = kt * BK;
+    if (a_kc) {
+      const int k = k0 + kc_c;
+      const T* p = A + (int64_t)(m0 + kc_r) * g.sam + k;
+#pragma unroll
+      for (int e = 0; e < EA; ++e) {
+        const int m = m0 + kc_r + e * RSA;
+        ra[e] = (m < g.M && k < g.K) ? p[(int64_t)(e * RSA) * g.sam] : (T)0.0f;
+      }
+    } else {
+      const int m = m0 + ma_r;
+      const T* p = A + (int64_t)m * g.sam + (int64_t)(k0 + ma_c) * g.sak;
+#pragma unroll
+      for (int e = 0; e < EA; ++e) {
+        const int k = k0 + ma_c + e * CSA;
+        ra[e] = (m < g.M && k < g.K) ? p[(int64_t)(e * CSA) * g.sak] : (T)0.0f;
+      }
+    }
+  };
+  auto load_b = [&](int kt) {
+    const int k0 = kt * BK;
+    if (b_kc) {
+      const int k = k0 + kc_c;
+      const T* p = B + (int64_t)(n0 + kc_r) * g.sbn + k;
+#pragma unroll
+      for (int e = 0; e < EB; ++e) {
+        const int n = n0 + kc_r + e * RSA;
+        rb[e] = (n < g.N && k < g.K) ? p[(int64_t)(e * RSA) * g.sbn] : (T)0.0f;
+      }
+    } else {
+      const int n = n0 + nb_r;
+      const T* p = B + (int64_t)n * g.sbn + (int64_t)(k0 + nb_c) * g.sbk;
+#pragma unroll
+      for (int e = 0; e < EB; ++e) {
+        const int k = k0 + nb_c + e * CSB;
+        rb[e] = (n < g.N && k < g.K) ? p[(int64_t)(e * CSB) * g.sbk] : (T)0.0f;
+      }
+    }
+  };
+  // LDS column for logical k within the tile: step s, lane group gq, element e
+  // of the lane's vector: k = s*KM + gq*KV + e is stored at gq*(KS*KV) + s*KV + e,
+  // so a lane's operand vector for each step is contiguous.
+  auto kpos = [&](int c) {
+    const int s = c / KM, rem = c % KM, gq = rem / KV, e = rem % KV;
+    return gq * (KS * KV) + s * KV + e;
+  };
+  auto store_a = [&](int buf) {
+    if (a_kc) {
+      const int kp = kpos(kc_c);
+#pragma unroll
+      for (int e = 0; e < EA; ++e) sA[buf][(kc_r + e * RSA) * LDR + kp] = ra[e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < EA; ++e) sA[buf][ma_r * LDR + kpos(ma_c + e * CSA)] = ra[e];
+    }
+  };
+  auto store_b = [&](int buf) {
+    if (b_kc) {
+      const int kp = kpos(kc_c);
+#pragma unroll
+      for (int e = 0; e < EB; ++e) sB[buf][(kc_r + e * RSA) * LDR + kp] = rb[e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < EB; ++e) sB[buf][nb_r * LDR + kpos(nb_c + e * CSB)] = rb[e];
+    }
+  };
+
+  v4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = (v4){0, 0, 0, 0};
+
+  const int gq = lane >> 4, li = lane & 15;
+
+  if (kt_begin < kt_end) {
+    load_a(kt_begin);
+    load_b(kt_begin);
+    store_a(0);
+    store_b(0);
+    __syncthreads();
+    for (int kt = kt_begin; kt < kt_end; ++kt) {
+      const int buf = (kt - kt_begin) & 1;
+      const bool more = (kt + 1 < kt_end);
+      if (more) {
+        load_a(kt + 1);
+        load_b(kt + 1);
+      }
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        typename MT::frag fa[TM], fb[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+          fa[i] = MT::load(&sA[buf][(wm * WM + i * 16 + li) * LDR + gq * (KS * KV) + s * KV]);
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          fb[j] = MT::load(&sB[buf][(wn * WN + j * 16 + li) * LDR + gq * (KS * KV) + s * KV]);
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) acc[i][j] = MT::mma(fa[i], fb[j], acc[i][j]);
+      }
+      if (more) {
+        store_a(buf ^ 1);
+        store_b(buf ^ 1);
+      }
+      __syncthreads();
+    }
+  }
+
+  // epilogue.  Only the plain strided map is applied in-kernel; split-K and
+  // the permuting maps write a row-major partial that the reduction kernel
+  // folds through the map (a small body keeps the loops fully unrolled, so
+  // every index into acc stays compile-time constant).
+  const bool direct = (g.splitk == 1 && g.map.kind == MAP_STD && g.out_prec == 0);
+  Acc* part = (Acc*)g.partial + ((int64_t)b * g.splitk + ks) * (int64_t)g.M * g.N;
+  OutT* C = (OutT*)g.map.C;
+  const double alpha = g.map.alpha, beta = g.map.beta;
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm * WM + i * 16 + MT::row(lane, r);
+        const int n = n0 + wn * WN + j * 16 + li;
+        if (m < g.M && n < g.N) {
+          if (direct) {
+            const int64_t off = (int64_t)m * g.map.scm + (int64_t)n * g.map.scn + (int64_t)b * g.map.scb;
+            double val = alpha * (double)acc[i][j][r];
+            if (beta != 0.0) val += beta * (double)C[off];
+            C[off] = (OutT)val;
+          } else {
+            part[(int64_t)m * g.N + n] = acc[i][j][r];
+          }
+        }
+      }
+}
+
+// Sum the split-K partials and write through the map.
+template <class Acc, class OutT>
+__global__ void splitk_reduce_kernel(GemmArgs g) {
+  const int64_t MN = (int64_t)g.M * g.N;
+  const int64_t total = MN * g.batch;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = idx / MN, mn = idx - b * MN;
+    const Acc* p = (const Acc*)g.partial + b * g.splitk * MN + mn;
+    double s = 0.0;
+    for (int k = 0; k < g.splitk; ++k) s += (double)p[k * MN];
+    map_store<OutT>(g.map, mn / g.N, mn % g.N, b, s);
+  }
+}
+
+// Host launcher. Chooses tile sizes and split-K; `ws` is a workspace for the
+// split-K partials that is grown on demand.
+enum class Prec { F64, F32, BF16 };
+
+struct GemmWorkspace {
+  DevBuf<char> buf;
+};
+
+void gemm(GemmArgs g, Prec prec, GemmWorkspace& ws, hipStream_t st, int splitk_hint = 0);
+
+}  // namespace adc2g

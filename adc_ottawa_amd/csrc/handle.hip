@@ -1,0 +1,310 @@
+// Operator construction (ph-ph block, diagonal, factors) and the sigma-vector.
+//
+// The reference builds the whole matrix once as a COO list on disk
+// (write_fspace_adc2_1, source/adclib/fspacetrial.f90:533-562 ->
+// get_offdiag_adc2_save_omp, source/adclib/get_matrix.f90:705-1186) and then
+// multiplies with it (hxvec_incore, source/eigen/block_davidson.f90:844-883).
+// Here only the dense ph-ph block (n1 x n1) is built; the 1h1p<->2h2p
+// coupling is applied on the fly as density-fitted contractions, and the
+// 2h2p-2h2p block (diagonal, adc_ph.f90:1090-1096) is fused into the
+// pair-to-CSF projection.
+#include "handle.hpp"
+
+#include <algorithm>
+
+namespace adc2g {
+
+Prec prec_from_int(int p) {
+  switch (p) {
+    case ADC2G_PREC_FP64: return Prec::F64;
+    case ADC2G_PREC_FP32: return Prec::F32;
+    case ADC2G_PREC_BF16: return Prec::BF16;
+    default: throw Error(ADC2G_EARG, "unknown precision " + std::to_string(p));
+  }
+}
+
+static size_t prec_size(Prec p) { return p == Prec::F64 ? 8 : (p == Prec::F32 ? 4 : 2); }
+
+Handle::~Handle() {
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (st) (void)hipStreamDestroy(st);
+}
+
+void Handle::create(int o, int v, int naux, const double* eps_h, const double* B_h, const adc2g_opts& opts) {
+  ARG_CHECK(o >= 2 && v >= 2 && naux >= 1, "need nocc >= 2, nvirt >= 2, naux >= 1");
+  ARG_CHECK(eps_h && B_h, "eps and B must not be NULL");
+  device = opts.device;
+  HIP_CHECK(hipSetDevice(device));
+  HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  d.set(o, v, naux);
+  dk = dims_k(d);
+  second_order = opts.build_phph ? 1 : 0;
+  const int nbas = o + v;
+  eps.alloc(nbas);
+  HIP_CHECK(hipMemcpyAsync(eps.get(), eps_h, nbas * sizeof(double), hipMemcpyHostToDevice, st));
+  DevBuf<double> dB;
+  dB.alloc((size_t)naux * nbas * nbas);
+  HIP_CHECK(hipMemcpyAsync(dB.get(), B_h, dB.n * sizeof(double), hipMemcpyHostToDevice, st));
+  Bov.alloc((size_t)naux * d.n1);
+  Bvv.alloc((size_t)naux * v * v);
+  Boo.alloc((size_t)naux * o * o);
+  launch_extract_factors(dB.get(), nbas, dk, Bov.get(), Bvv.get(), Boo.get(), st);
+  build_phph(dB.get(), nbas);
+  HIP_CHECK(hipStreamSynchronize(st));
+  if (opts.keep_lowprec) {
+    ensure_lowprec(Prec::F32);
+    ensure_lowprec(Prec::BF16);
+  }
+  // default chunk: keep the sigma workspace within ~12 GB
+  const size_t budget = (size_t)12 << 30;
+  chunk = 64;
+  while (chunk > 1 && sigma_workspace_bytes(chunk, Prec::F64) > budget) chunk /= 2;
+}
+
+void Handle::ensure_lowprec(Prec p) {
+  const int64_t n = (int64_t)d.naux * d.n1;
+  if (p == Prec::F32 && !Bov32.get()) {
+    Bov32.alloc(n);
+    launch_convert_f32(Bov.get(), Bov32.get(), n, st);
+  }
+  if (p == Prec::BF16 && !Bov16.get()) {
+    Bov16.alloc(n);
+    launch_convert_bf16(Bov.get(), Bov16.get(), n, st);
+  }
+}
+
+// ph-ph block from the factors (first order + CA, CB, CC as products with
+// separable denominators; the algebra is stated in kernels.hip).
+void Handle::build_phph(const double* dB, int nbas) {
+  (void)dB;
+  (void)nbas;
+  const int o = d.o, v = d.v, naux = d.naux;
+  const int64_t n1 = d.n1;
+  DevBuf<double> Mm, Jm;
+  Mm.alloc(n1 * n1);
+  Jm.alloc((size_t)v * v * o * o);
+  H11.alloc(n1 * n1);
+  // M = Bov^T Bov, (ak|ci)
+  {
+    GemmArgs g;
+    g.M = (int)n1; g.N = (int)n1; g.K = naux;
+    g.A = Bov.get(); g.sam = 1; g.sak = n1;
+    g.B = Bov.get(); g.sbk = n1; g.sbn = 1;
+    g.map.C = Mm.get(); g.map.scm = n1; g.map.scn = 1;
+    gemm(g, Prec::F64, gws, st);
+  }
+  // J[(a,a'),(k,k')] = (aa'|kk')
+  {
+    GemmArgs g;
+    g.M = v * v; g.N = o * o; g.K = naux;
+    g.A = Bvv.get(); g.sam = 1; g.sak = (int64_t)v * v;
+    g.B = Boo.get(); g.sbk = (int64_t)o * o; g.sbn = 1;
+    g.map.C = Jm.get(); g.map.scm = (int64_t)o * o; g.map.scn = 1;
+    gemm(g, Prec::F64, gws, st);
+  }
+  DevBuf<double> CA, CB;
+  CA.alloc((size_t)v * v);
+  CB.alloc((size_t)o * o);
+  if (second_order) {
+    {
+      const int64_t K0 = (int64_t)o * o * v;
+      DevBuf<double> X1, X2;
+      X1.alloc((size_t)v * 2 * K0);
+      X2.alloc((size_t)v * 2 * K0);
+      launch_ca_operands(Mm.get(), eps.get(), dk, X1.get(), X2.get(), st);
+      GemmArgs g;
+      g.M = v; g.N = v; g.K = (int)(2 * K0);
+      g.A = X1.get(); g.sam = 2 * K0; g.sak = 1;
+      g.B = X2.get(); g.sbk = 1; g.sbn = 2 * K0;
+      g.map.C = CA.get(); g.map.scm = v; g.map.scn = 1; g.map.alpha = 0.5;
+      gemm(g, Prec::F64, gws, st);
+      HIP_CHECK(hipStreamSynchronize(st));
+    }
+    {
+      const int64_t K0 = (int64_t)v * v * o;
+      DevBuf<double> X1, X2;
+      X1.alloc((size_t)o * 2 * K0);
+      X2.alloc((size_t)o * 2 * K0);
+      launch_cb_operands(Mm.get(), eps.get(), dk, X1.get(), X2.get(), st);
+      GemmArgs g;
+      g.M = o; g.N = o; g.K = (int)(2 * K0);
+      g.A = X1.get(); g.sam = 2 * K0; g.sak = 1;
+      g.B = X2.get(); g.sbk = 1; g.sbn = 2 * K0;
+      g.map.C = CB.get(); g.map.scm = o; g.map.scn = 1; g.map.alpha = -0.5;
+      gemm(g, Prec::F64, gws, st);
+      HIP_CHECK(hipStreamSynchronize(st));
+    }
+    {
+      DevBuf<double> X1, X2;
+      X1.alloc((size_t)n1 * 2 * n1);
+      X2.alloc((size_t)n1 * 2 * n1);
+      launch_cc_operands(Mm.get(), eps.get(), dk, X1.get(), X2.get(), st);
+      GemmArgs g;
+      g.M = (int)n1; g.N = (int)n1; g.K = (int)(2 * n1);
+      g.A = X1.get(); g.sam = 2 * n1; g.sak = 1;
+      g.B = X2.get(); g.sbk = 1; g.sbn = 2 * n1;
+      g.map.C = H11.get(); g.map.scm = n1; g.map.scn = 1; g.map.alpha = -0.5;
+      gemm(g, Prec::F64, gws, st);
+      HIP_CHECK(hipStreamSynchronize(st));
+    }
+  }
+  launch_phph_assemble(H11.get(), Mm.get(), Jm.get(), CA.get(), CB.get(), eps.get(), dk, second_order, st);
+  hii.alloc(d.ndim);
+  launch_diagonal(H11.get(), eps.get(), dk, hii.get(), st);
+  HIP_CHECK(hipStreamSynchronize(st));
+}
+
+size_t Handle::sigma_workspace_bytes(int nc, Prec p) const {
+  const size_t n1 = d.n1, naux = d.naux;
+  size_t b = n1 * n1 * nc * prec_size(p);        // Z / G
+  b += 3 * naux * n1 * nc * sizeof(double);      // Yvv, Yoo, Wup
+  b += naux * n1 * nc * sizeof(double);          // GEMM partials
+  if (p != Prec::F64) b += naux * n1 * nc * prec_size(p);
+  return b;
+}
+
+// W = H X for nvec columns (device pointers, ld = ldx / ldw).
+void Handle::sigma(const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, Prec p, hipStream_t s) {
+  ARG_CHECK(ldx >= d.ndim && ldw >= d.ndim, "leading dimensions must be >= ndim");
+  if (nvec <= 0) return;
+  if (!s) s = st;
+  if (p != Prec::F64) ensure_lowprec(p);
+  const int o = d.o, v = d.v, naux = d.naux;
+  const int64_t n1 = d.n1;
+  const int nc_max = std::min(chunk, nvec);
+  X1c.alloc(n1 * nc_max);
+  Yvv.alloc((size_t)naux * n1 * nc_max);
+  Yoo.alloc((size_t)naux * n1 * nc_max);
+  Wup.alloc((size_t)naux * n1 * nc_max);
+  ZG.alloc((size_t)n1 * n1 * nc_max * prec_size(p));
+  if (p != Prec::F64) Wup_lp.alloc((size_t)naux * n1 * nc_max * prec_size(p));
+  const int out_lp = (p == Prec::F32) ? 1 : (p == Prec::BF16 ? 2 : 0);
+  const void* BovP = (p == Prec::F64) ? (const void*)Bov.get()
+                     : (p == Prec::F32 ? (const void*)Bov32.get() : (const void*)Bov16.get());
+  if (timing) {
+    for (auto& e : ev)
+      if (!e) HIP_CHECK(hipEventCreate(&e));
+  }
+  double flops = 0, bytes = 0;
+  float ms_g = 0, ms_s = 0;
+  for (int c0 = 0; c0 < nvec; c0 += nc_max) {
+    const int nc = std::min(nc_max, nvec - c0);
+    const double* Xc = X + ldx * c0;
+    double* Wc = W + ldw * c0;
+    if (timing) HIP_CHECK(hipEventRecord(ev[0], s));
+    launch_pack_x1<double>(Xc, ldx, nc, dk, X1c.get(), s);
+    switch (p) {
+      case Prec::F64: launch_csf_to_pair<double>(Xc, ldx, nc, dk, (double*)ZG.get(), s); break;
+      case Prec::F32: launch_csf_to_pair<float>(Xc, ldx, nc, dk, (float*)ZG.get(), s); break;
+      case Prec::BF16: launch_csf_to_pair<__bf16>(Xc, ldx, nc, dk, (__bf16*)ZG.get(), s); break;
+    }
+    if (timing) HIP_CHECK(hipEventRecord(ev[1], s));
+    // sigma1 = H11 x1
+    {
+      GemmArgs g;
+      g.M = (int)n1; g.N = nc; g.K = (int)n1;
+      g.A = H11.get(); g.sam = n1; g.sak = 1;
+      g.B = Xc; g.sbk = 1; g.sbn = ldx;
+      g.map.C = Wc; g.map.scm = 1; g.map.scn = ldw;
+      gemm(g, Prec::F64, gws, s);
+    }
+    // down: Y[P,(c,k)] = sum_{bl} B[P,bl] Z[(c,k),(b,l)]  -> Yvv and Yoo layouts
+    {
+      GemmArgs g;
+      g.M = naux; g.N = (int)(n1 * nc); g.K = (int)n1;
+      g.A = BovP; g.sam = n1; g.sak = 1;
+      g.B = ZG.get(); g.sbk = 1; g.sbn = n1;
+      g.map.kind = MAP_DOWNY; g.map.C = Yvv.get(); g.map.C2 = Yoo.get();
+      g.map.q[0] = o; g.map.q[1] = nc; g.map.q[2] = n1; g.map.q[3] = naux;
+      gemm(g, p, gws, s);
+      flops += 2.0 * naux * (double)n1 * n1 * nc;
+    }
+    // sigma1(a,k) += sum_{P,c} B[P,a,c] Y[P,c,k]
+    {
+      GemmArgs g;
+      g.M = v; g.N = o * nc; g.K = v * naux;
+      g.A = Bvv.get(); g.sam = 1; g.sak = v;
+      g.B = Yvv.get(); g.sbk = (int64_t)o * nc; g.sbn = 1;
+      g.map.kind = MAP_SIG1V; g.map.C = Wc; g.map.beta = 1.0;
+      g.map.q[0] = o; g.map.q[1] = ldw;
+      gemm(g, Prec::F64, gws, s);
+    }
+    // sigma1(a,k) -= sum_{P,m} B[P,k,m] Y[P,a,m]
+    {
+      GemmArgs g;
+      g.M = v * nc; g.N = o; g.K = o * naux;
+      g.A = Yoo.get(); g.sam = (int64_t)o * naux; g.sak = 1;
+      g.B = Boo.get(); g.sbk = o; g.sbn = 1;
+      g.map.kind = MAP_SIG1O; g.map.C = Wc; g.map.alpha = -1.0; g.map.beta = 1.0;
+      g.map.q[0] = o; g.map.q[1] = ldw; g.map.q[2] = nc;
+      gemm(g, Prec::F64, gws, s);
+    }
+    // up: W[P,c,k] = sum_a B[P,c,a] x1(a,k) - sum_m B[P,k,m] x1(c,m)
+    void* wup_out = (p == Prec::F64) ? (void*)Wup.get() : (void*)Wup_lp.get();
+    {
+      GemmArgs g;
+      g.M = v * naux; g.N = o * nc; g.K = v;
+      g.A = Bvv.get(); g.sam = v; g.sak = 1;
+      g.B = X1c.get(); g.sbk = (int64_t)o * nc; g.sbn = 1;
+      g.map.kind = MAP_WUPV; g.map.C = wup_out;
+      g.map.q[0] = o; g.map.q[1] = v; g.map.q[2] = n1; g.map.q[3] = nc;
+      g.out_prec = out_lp;
+      gemm(g, Prec::F64, gws, s);
+    }
+    {
+      GemmArgs g;
+      g.M = v * nc; g.N = o * naux; g.K = o;
+      g.A = X1c.get(); g.sam = o; g.sak = 1;
+      g.B = Boo.get(); g.sbk = 1; g.sbn = o;
+      g.map.kind = MAP_WUPO; g.map.C = wup_out; g.map.alpha = -1.0; g.map.beta = 1.0;
+      g.map.q[0] = o; g.map.q[1] = v; g.map.q[2] = n1; g.map.q[3] = nc;
+      g.out_prec = out_lp;
+      gemm(g, Prec::F64, gws, s);
+    }
+    // G[(c,k),(b,l)] = sum_P W[P,(c,k)] B[P,(b,l)]
+    {
+      GemmArgs g;
+      g.M = (int)(n1 * nc); g.N = (int)n1; g.K = naux;
+      g.A = wup_out; g.sam = 1; g.sak = n1 * nc;
+      g.B = BovP; g.sbk = n1; g.sbn = 1;
+      g.map.C = ZG.get(); g.map.scm = n1; g.map.scn = 1;
+      g.out_prec = out_lp;
+      gemm(g, p, gws, s);
+      flops += 2.0 * naux * (double)n1 * n1 * nc;
+    }
+    if (timing) HIP_CHECK(hipEventRecord(ev[2], s));
+    switch (p) {
+      case Prec::F64:
+        launch_pair_to_csf<double>((const double*)ZG.get(), Xc, ldx, Wc, ldw, nc, dk, eps.get(), s);
+        break;
+      case Prec::F32:
+        launch_pair_to_csf<float>((const float*)ZG.get(), Xc, ldx, Wc, ldw, nc, dk, eps.get(), s);
+        break;
+      case Prec::BF16:
+        launch_pair_to_csf<__bf16>((const __bf16*)ZG.get(), Xc, ldx, Wc, ldw, nc, dk, eps.get(), s);
+        break;
+    }
+    if (timing) {
+      HIP_CHECK(hipEventRecord(ev[3], s));
+      HIP_CHECK(hipEventSynchronize(ev[3]));
+      float a = 0, b = 0, c = 0;
+      HIP_CHECK(hipEventElapsedTime(&a, ev[0], ev[1]));
+      HIP_CHECK(hipEventElapsedTime(&b, ev[1], ev[2]));
+      HIP_CHECK(hipEventElapsedTime(&c, ev[2], ev[3]));
+      ms_g += b;
+      ms_s += a + c;
+      // streaming bytes: pack (read x2 once, write Z) + unpack (read G, x2, write sigma2)
+      const double es = (double)prec_size(p);
+      bytes += (double)nc * (2.0 * d.n2 * 8.0 + 2.0 * n1 * n1 * es + d.n2 * 8.0);
+    }
+  }
+  if (timing) {
+    last_ms_gemm = ms_g;
+    last_ms_stream = ms_s;
+    last_flops_gemm = flops;
+    last_bytes_stream = bytes;
+  }
+}
+
+}  // namespace adc2g

@@ -1,0 +1,55 @@
+// The opaque handle: device-resident ADC(2)-s operator and solver state.
+#pragma once
+
+#include "common.hpp"
+#include "gemm.hpp"
+#include "kernels.hpp"
+
+struct adc2g_handle;
+
+namespace adc2g {
+
+struct SigmaTimer;
+struct PreconNet;
+
+struct Handle {
+  Dims d;
+  DimsK dk;
+  int device = 0;
+  hipStream_t st = nullptr;
+  int second_order = 1;
+
+  // operator (FP64, always present)
+  DevBuf<double> eps, Bov, Bvv, Boo, H11, hii;
+  // low-precision copies of the coupling factor (mixed-precision sigma)
+  DevBuf<float> Bov32;
+  DevBuf<__bf16> Bov16;
+
+  // sigma workspaces (grown on demand, sized per vector chunk)
+  DevBuf<double> X1c, Yvv, Yoo, Wup;
+  DevBuf<char> ZG;          // Z then G, in the chunk's GEMM precision
+  DevBuf<char> Wup_lp;      // Wup in low precision for the up GEMM
+  GemmWorkspace gws;
+  int chunk = 32;           // vectors per sigma pass
+
+  // optional device timing of the last sigma (bench)
+  bool timing = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double last_ms_gemm = -1, last_ms_stream = -1, last_flops_gemm = 0, last_bytes_stream = 0;
+
+  // learned preconditioner (network + harvested training records)
+  PreconNet* net = nullptr;
+
+  Handle() = default;
+  ~Handle();
+
+  void create(int o, int v, int naux, const double* eps_h, const double* B_h, const adc2g_opts& opts);
+  void build_phph(const double* dB, int nbas);
+  void ensure_lowprec(Prec p);
+  void sigma(const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, Prec p, hipStream_t s);
+  size_t sigma_workspace_bytes(int nc, Prec p) const;
+};
+
+Prec prec_from_int(int p);
+
+}  // namespace adc2g

@@ -1,0 +1,27 @@
+// Solvers on top of the sigma-vector: block Davidson, SIL propagation and the
+// learned preconditioner.
+#pragma once
+
+#include "handle.hpp"
+
+namespace adc2g {
+
+// v2 = -i H v1 for one complex vector (host arrays, interleaved re/im).
+void matxvec_treal_host(Handle& H, const double* v1, double* v2, Prec p);
+
+struct DavTrace;  // per-iteration log (harvest mode), may be NULL
+void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* eigvec, adc2g_dav_stats* stats,
+                  DavTrace* trace);
+
+void sil_run(Handle& H, const adc2g_sil_params& p, const double* fvec, double* auto0, int64_t nrec_cap,
+             int64_t* nrec_out, adc2g_sil_stats* stats);
+
+int precon_nparams();
+void precon_harvest(Handle& H, const adc2g_dav_params& p, int64_t max_records, int64_t* added);
+void precon_train(Handle& H, const Handle& data, int epochs, int batch, double lr, uint64_t seed, double* loss_out);
+void precon_get(const Handle& H, float* params);
+void precon_set(Handle& H, const float* params);
+void precon_eval(Handle& H, int64_t n, const float* feat, const int32_t* cls, const float* dn, const float* en,
+                 float* m_out, float* grad_out, double* loss_out);
+
+}  // namespace adc2g

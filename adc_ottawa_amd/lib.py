@@ -1,0 +1,250 @@
+"""ctypes binding of the C ABI in include/adc2g.h.
+
+This is the Python-side mirror of the Fortran ``iso_c_binding`` module
+(``adc_ottawa_amd/fortran/adc2g_mod.f90``): the same entry points, argument
+meaning and error behaviour.  It loads the in-tree ``libadc2g.so`` and fails
+loudly when it is missing; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libadc2g.so")
+
+ADC2G_OK = 0
+ERRORS = {1: "SIL illegal integration order", 2: "SIL stepsize underflow",
+          3: "SIL diagonalisation failed", 4: "SIL illegal initial stepsize",
+          10: "bad argument", 11: "HIP failure", 12: "not all vectors have converged",
+          13: "Rayleigh-Ritz eigensolve failed", 14: "device allocation failed",
+          15: "call out of order"}
+
+PREC = {"fp64": 0, "fp32": 1, "bf16": 2}
+PRECON = {"dpr": 1, "olsen": 2, "learned": 3}
+
+
+class Adc2gError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"adc2g error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class Opts(C.Structure):
+    _fields_ = [("device", C.c_int32), ("build_phph", C.c_int32), ("keep_lowprec", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class DavParams(C.Structure):
+    _fields_ = [("nstates", C.c_int32), ("blocksize", C.c_int32), ("maxvec", C.c_int32),
+                ("maxiter", C.c_int32), ("tol", C.c_double), ("precon", C.c_int32),
+                ("deflate", C.c_int32), ("precision", C.c_int32), ("refine", C.c_int32),
+                ("lowprec_tol", C.c_double), ("guess", C.c_int32), ("verbose", C.c_int32)]
+
+
+class DavStats(C.Structure):
+    _fields_ = [("niter", C.c_int32), ("nsigma", C.c_int32), ("nmult_ref", C.c_int64),
+                ("niter_lowprec", C.c_int32), ("nfallback", C.c_int32), ("time_total", C.c_double),
+                ("time_sigma", C.c_double), ("resnorm_max", C.c_double)]
+
+
+class SilParams(C.Structure):
+    _fields_ = [("tout", C.c_double), ("tfinal", C.c_double), ("kdim", C.c_int32),
+                ("precision", C.c_int32), ("autotol", C.c_double)]
+
+
+class SilStats(C.Structure):
+    _fields_ = [("nsteps", C.c_int64), ("nsigma", C.c_int64), ("max_order", C.c_int32),
+                ("reserved", C.c_int32), ("time_total", C.c_double)]
+
+
+# every symbol declared in include/adc2g.h
+SYMBOLS = [
+    "adc2g_create", "adc2g_destroy", "adc2g_dims", "adc2g_diagonal", "adc2g_phph",
+    "adc2g_sigma", "adc2g_sigma_device", "adc2g_matxvec_treal", "adc2g_davidson",
+    "adc2g_sil_autocorr", "adc2g_precon_harvest", "adc2g_precon_train", "adc2g_precon_nparams",
+    "adc2g_precon_get", "adc2g_precon_set", "adc2g_precon_eval", "adc2g_last_error",
+    "adc2g_sigma_profile",
+]
+
+_lib = None
+
+dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def load(path: str = LIBPATH):
+    """Load libadc2g.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is missing: build it with `python -m adc_ottawa_amd.build`")
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    i32, i64 = C.c_int32, C.c_int64
+    lib.adc2g_create.argtypes = [i32, i32, i32, vp, vp, C.POINTER(Opts), C.POINTER(vp)]
+    lib.adc2g_destroy.argtypes = [vp]
+    lib.adc2g_dims.argtypes = [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]
+    lib.adc2g_diagonal.argtypes = [vp, vp]
+    lib.adc2g_phph.argtypes = [vp, vp]
+    lib.adc2g_sigma.argtypes = [vp, i32, vp, vp, i32]
+    lib.adc2g_sigma_device.argtypes = [vp, i32, vp, vp, i32, vp]
+    lib.adc2g_matxvec_treal.argtypes = [vp, vp, vp, i32]
+    lib.adc2g_davidson.argtypes = [vp, C.POINTER(DavParams), vp, vp, C.POINTER(DavStats)]
+    lib.adc2g_sil_autocorr.argtypes = [vp, C.POINTER(SilParams), vp, vp, i64, C.POINTER(i64),
+                                       C.POINTER(SilStats)]
+    lib.adc2g_precon_harvest.argtypes = [vp, C.POINTER(DavParams), i64, C.POINTER(i64)]
+    lib.adc2g_precon_train.argtypes = [vp, vp, i32, i32, C.c_double, C.c_uint64, vp]
+    lib.adc2g_precon_nparams.argtypes = []
+    lib.adc2g_precon_get.argtypes = [vp, vp]
+    lib.adc2g_precon_set.argtypes = [vp, vp]
+    lib.adc2g_precon_eval.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
+    lib.adc2g_last_error.argtypes = [C.c_char_p, i32]
+    lib.adc2g_sigma_profile.argtypes = [vp, i32, vp, vp, vp, vp]
+    for s in SYMBOLS:
+        getattr(lib, s).restype = C.c_int32
+    _lib = lib
+    return lib
+
+
+def check(rc: int):
+    if rc != ADC2G_OK:
+        buf = C.create_string_buffer(1024)
+        _lib.adc2g_last_error(buf, 1024)
+        raise Adc2gError(rc, buf.value.decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Operator:
+    """Device-resident ADC(2)-s operator of one system (``adc2g_create``)."""
+
+    def __init__(self, nocc, nvirt, naux, eps, B, device=0, second_order=True, keep_lowprec=False):
+        lib = load()
+        self._h = C.c_void_p()
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        nbas = nocc + nvirt
+        if eps.size != nbas or B.shape != (naux, nbas, nbas):
+            raise ValueError("eps/B shapes do not match nocc, nvirt, naux")
+        o = Opts(device, 1 if second_order else 0, 1 if keep_lowprec else 0, 0)
+        check(lib.adc2g_create(nocc, nvirt, naux, _ptr(eps), _ptr(B), C.byref(o), C.byref(self._h)))
+        nd, n1 = C.c_int64(), C.c_int64()
+        cnt = (C.c_int64 * 5)()
+        check(lib.adc2g_dims(self._h, C.byref(nd), C.byref(n1), cnt))
+        self.ndim, self.n1, self.counts = nd.value, n1.value, list(cnt)
+        self.nocc, self.nvirt, self.naux = nocc, nvirt, naux
+
+    @classmethod
+    def from_system(cls, sys, **kw):
+        return cls(sys.nocc, sys.nvirt, sys.naux, sys.eps, sys.B, **kw)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            check(_lib.adc2g_destroy(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def diagonal(self):
+        out = np.empty(self.ndim)
+        check(_lib.adc2g_diagonal(self._h, _ptr(out)))
+        return out
+
+    def phph(self):
+        out = np.empty((self.n1, self.n1))
+        check(_lib.adc2g_phph(self._h, _ptr(out)))
+        return out
+
+    def sigma(self, X, precision="fp64"):
+        X = np.asarray(X, dtype=np.float64)
+        one = X.ndim == 1
+        Xf = np.asfortranarray(X.reshape(self.ndim, -1))
+        W = np.empty_like(Xf, order="F")
+        check(_lib.adc2g_sigma(self._h, Xf.shape[1], _ptr(Xf), _ptr(W), PREC[precision]))
+        return W[:, 0] if one else W
+
+    def matxvec_treal(self, v1, precision="fp64"):
+        v1 = np.ascontiguousarray(v1, dtype=np.complex128)
+        v2 = np.empty_like(v1)
+        check(_lib.adc2g_matxvec_treal(self._h, _ptr(v1), _ptr(v2), PREC[precision]))
+        return v2
+
+    def davidson(self, nstates, blocksize=0, maxvec=-1, maxiter=100, tol=1e-7, precon="dpr",
+                 deflate=False, precision="fp64", refine=True, lowprec_tol=1e-4, vectors=True,
+                 verbose=False):
+        p = DavParams(nstates, blocksize, maxvec, maxiter, tol, PRECON[precon], 1 if deflate else 0,
+                      PREC[precision], 1 if refine else 0, lowprec_tol, 0, 1 if verbose else 0)
+        ev = np.empty(nstates)
+        vec = np.empty((nstates, self.ndim)) if vectors else None
+        st = DavStats()
+        check(_lib.adc2g_davidson(self._h, C.byref(p), _ptr(ev),
+                                  _ptr(vec) if vectors else None, C.byref(st)))
+        stats = {f: getattr(st, f) for f, _ in DavStats._fields_}
+        return ev, (vec.T if vectors else None), stats
+
+    def sil_autocorr(self, fvec, tout, tfinal, kdim, autotol, precision="fp64"):
+        fvec = np.ascontiguousarray(fvec, dtype=np.float64)
+        nrec = int(tfinal / (0.5 * tout))
+        out = np.empty(nrec + 1, dtype=np.complex128)
+        p = SilParams(tout, tfinal, kdim, PREC[precision], autotol)
+        n = C.c_int64()
+        st = SilStats()
+        check(_lib.adc2g_sil_autocorr(self._h, C.byref(p), _ptr(fvec), _ptr(out), out.size,
+                                      C.byref(n), C.byref(st)))
+        stats = {f: getattr(st, f) for f, _ in SilStats._fields_}
+        return out[: n.value], stats
+
+    # ---- learned preconditioner
+    def precon_harvest(self, nstates, max_records=1 << 40, **kw):
+        p = DavParams(nstates, kw.get("blocksize", 0), kw.get("maxvec", -1), kw.get("maxiter", 200),
+                      kw.get("tol", 1e-7), 1, 0, 0, 0, 0.0, 0, 0)
+        added = C.c_int64()
+        check(_lib.adc2g_precon_harvest(self._h, C.byref(p), max_records, C.byref(added)))
+        return added.value
+
+    def precon_train(self, data=None, epochs=10, batch=8192, lr=1e-3, seed=0):
+        data = self if data is None else data
+        loss = np.empty(max(1, epochs))
+        check(_lib.adc2g_precon_train(self._h, data.handle, epochs, batch, lr, seed, _ptr(loss)))
+        return loss[:epochs]
+
+    def precon_params(self):
+        out = np.empty(_lib.adc2g_precon_nparams(), dtype=np.float32)
+        check(_lib.adc2g_precon_get(self._h, _ptr(out)))
+        return out
+
+    def set_precon_params(self, params):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        check(_lib.adc2g_precon_set(self._h, _ptr(params)))
+
+    def precon_eval(self, feat, cls, dn, en):
+        n = feat.shape[0]
+        feat = np.ascontiguousarray(feat, dtype=np.float32)
+        cls = np.ascontiguousarray(cls, dtype=np.int32)
+        dn = np.ascontiguousarray(dn, dtype=np.float32)
+        en = np.ascontiguousarray(en, dtype=np.float32)
+        m = np.empty(n, dtype=np.float32)
+        g = np.empty(_lib.adc2g_precon_nparams(), dtype=np.float32)
+        loss = C.c_double()
+        check(_lib.adc2g_precon_eval(self._h, n, _ptr(feat), _ptr(cls), _ptr(dn), _ptr(en), _ptr(m),
+                                     _ptr(g), C.byref(loss)))
+        return m, g, loss.value

@@ -1,0 +1,217 @@
+/*
+ * adc2g.h -- C ABI of the MI355X (gfx950) ADC(2)-s sigma-vector library.
+ *
+ * The library replaces the stored-matrix hot path of ADC_ottawa: the matrix
+ * build and on-disk COO list, the sparse sigma-vector consumed by the block
+ * Davidson solver and the complex sigma-vector consumed by the
+ * short-iterative-Lanczos (SIL) propagator.  Everything lives behind an
+ * opaque handle; device memory is owned by the library and every vector,
+ * subspace and factor stays resident in HBM between calls.
+ *
+ * Conventions (those of the Fortran caller):
+ *  - real(dp) = double, default INTEGER = int32_t, integer*8 = int64_t,
+ *    complex(dp) = two interleaved doubles (re, im);
+ *  - matrices are column-major (leading dimension = number of rows);
+ *  - vectors use the reference's configuration order: the n1 = nocc*nvirt
+ *    1h1p configurations (virtual outer, occupied inner), then the 2h2p
+ *    blocks t1 (a=b,i=j), t2 (a<b,i=j), t3 (a=b,i<j), t4 and t5 (a<b,i<j,
+ *    spin cases I and II), each in the loop order of
+ *    source/adclib/select_fano.f90:539-718;
+ *  - every function returns 0 on success or one of the ADC2G_E* codes; the
+ *    message of the last failure on the calling thread is available from
+ *    adc2g_last_error.
+ *
+ * Functions are not re-entrant per handle.  HIP streams are internal; the
+ * *_device entry points run on the stream given (NULL = the handle's own
+ * stream) and return without synchronising.
+ */
+#ifndef ADC2G_H
+#define ADC2G_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes; 1..4 are the SIL codes of source/propagation/sillib.f90:84-89 */
+#define ADC2G_OK 0
+#define ADC2G_ESIL_ORDER 1      /* illegal integration order */
+#define ADC2G_ESIL_UNDERFLOW 2  /* stepsize underflow */
+#define ADC2G_ESIL_DIAG 3       /* diagonalisation failed */
+#define ADC2G_ESIL_RESTART 4    /* stepsize too large for restart */
+#define ADC2G_EARG 10           /* bad argument */
+#define ADC2G_EHIP 11           /* HIP runtime failure */
+#define ADC2G_ENOCONV 12        /* Davidson: not all vectors have converged */
+#define ADC2G_EEIG 13           /* Rayleigh-Ritz eigensolve failed */
+#define ADC2G_ENOMEM 14         /* device allocation failed */
+#define ADC2G_ESTATE 15         /* call out of order (e.g. no trained network) */
+
+/* arithmetic of the 1h1p<->2h2p coupling contractions */
+#define ADC2G_PREC_FP64 0       /* FP64 MFMA */
+#define ADC2G_PREC_FP32 1       /* FP32 MFMA, FP32 accumulate */
+#define ADC2G_PREC_BF16 2       /* BF16 MFMA, FP32 accumulate */
+
+/* Davidson preconditioners (precon of source/eigen/block_davidson.f90:1121-1125) */
+#define ADC2G_PRE_DPR 1         /* diagonal preconditioned residue */
+#define ADC2G_PRE_OLSEN 2       /* Olsen */
+#define ADC2G_PRE_LEARNED 3     /* MLP correction on top of DPR, DPR fallback */
+
+typedef struct adc2g_handle adc2g_handle;
+
+typedef struct {
+  int32_t device;        /* HIP device ordinal */
+  int32_t build_phph;    /* 1: build the second-order ph-ph block (ADC(2)); 0: first order only */
+  int32_t keep_lowprec;  /* 1: also keep FP32/BF16 copies of the factors for mixed precision */
+  int32_t reserved;
+} adc2g_opts;
+
+/* Davidson parameters (the keys of rddiaginp, source/iomodules/rdinput.f90:1081-1288) */
+typedef struct {
+  int32_t nstates;       /* davstates */
+  int32_t blocksize;     /* dmain; 0 -> max(nstates+5, 2*nstates) */
+  int32_t maxvec;        /* maxsubdim; <0 -> 4*blocksize; rounded down to a multiple of blocksize */
+  int32_t maxiter;       /* maxiter */
+  double tol;            /* davtol: convergence when ||r||_2 < tol */
+  int32_t precon;        /* ADC2G_PRE_* */
+  int32_t deflate;       /* ldfl */
+  int32_t precision;     /* ADC2G_PREC_* of the sigma-vectors in the first phase */
+  int32_t refine;        /* mixed precision: 1 = finish with FP64 sigma-vectors (restarted refinement) */
+  double lowprec_tol;    /* residual at which the low-precision phase hands over to FP64 */
+  int32_t guess;         /* 0 = unit vectors on the smallest |H_jj| (initvec_ondiag) */
+  int32_t verbose;       /* 1: print the iteration table to stdout */
+} adc2g_dav_params;
+
+typedef struct {
+  int32_t niter;         /* Davidson iterations */
+  int32_t nsigma;        /* sigma-vectors actually computed */
+  int64_t nmult_ref;     /* sigma-vectors the reference's hxvec_incore would have computed (sum of currdim) */
+  int32_t niter_lowprec; /* iterations done with low-precision sigma-vectors */
+  int32_t nfallback;     /* learned-preconditioner steps replaced by DPR */
+  double time_total;     /* seconds, wall */
+  double time_sigma;     /* seconds inside sigma-vectors */
+  double resnorm_max;    /* max FP64 residual norm of the returned roots */
+} adc2g_dav_stats;
+
+typedef struct {
+  double tout;           /* time between autocorrelation records (a.u.) */
+  double tfinal;         /* final time; records at i*tout, i = 1..int(tfinal/(tout/2)) */
+  int32_t kdim;          /* krydim: maximum Lanczos order */
+  int32_t precision;     /* ADC2G_PREC_* of the sigma-vectors */
+  double autotol;        /* SIL error tolerance */
+} adc2g_sil_params;
+
+typedef struct {
+  int64_t nsteps;        /* SIL steps taken */
+  int64_t nsigma;        /* complex sigma-vectors (each = 2 real ones) */
+  int32_t max_order;
+  int32_t reserved;
+  double time_total;
+} adc2g_sil_stats;
+
+/* ---- lifetime ---------------------------------------------------------- */
+
+/* Build the on-device ADC(2)-s operator from orbital energies eps[nocc+nvirt]
+ * (occupied first) and density-fitting factors B[naux][nbas][nbas]
+ * (C order: B[(P*nbas + p)*nbas + q], symmetric in p,q) so that
+ * (pq|rs) = sum_P B[P,p,q] B[P,r,s].
+ * Replaces write_fspace_adc2_1 (source/adclib/fspacetrial.f90:533-562) with
+ * get_diag_adc2_save/get_offdiag_adc2_save_omp (source/adclib/get_matrix.f90:267-311,
+ * 705-1186) and the integral globals of source/include/parameters.f90. */
+int32_t adc2g_create(int32_t nocc, int32_t nvirt, int32_t naux, const double* eps,
+                     const double* B, const adc2g_opts* opts, adc2g_handle** out);
+
+/* Free everything the handle owns. */
+int32_t adc2g_destroy(adc2g_handle* h);
+
+/* Sizes: ndim, n1, and the counts of t1..t5 (counts[0..4]); kpq(1..5,0) of
+ * source/adclib/select_fano.f90 with kpq(5,0) repeated for t5. */
+int32_t adc2g_dims(const adc2g_handle* h, int64_t* ndim, int64_t* n1, int64_t counts[5]);
+
+/* Diagonal H_jj of the full matrix in configuration order: the hmlt.dia record
+ * of get_diag_adc2_save (source/adclib/get_matrix.f90:267-311). */
+int32_t adc2g_diagonal(adc2g_handle* h, double* hii);
+
+/* The dense ph-ph block (n1 x n1, column-major): first and second order, as
+ * assembled by get_offdiag_adc2_save_omp (source/adclib/get_matrix.f90:848-884)
+ * plus the diagonal of get_diag_adc2_save. */
+int32_t adc2g_phph(adc2g_handle* h, double* h11);
+
+/* ---- sigma-vectors ----------------------------------------------------- */
+
+/* W = H X for nvec columns, host arrays (ndim x nvec, ld = ndim).
+ * The +H primitive of hxvec_incore (source/eigen/block_davidson.f90:844-883). */
+int32_t adc2g_sigma(adc2g_handle* h, int32_t nvec, const double* X, double* W, int32_t precision);
+
+/* Same with device pointers (ld = ndim) on the given HIP stream. */
+int32_t adc2g_sigma_device(adc2g_handle* h, int32_t nvec, const double* dX, double* dW,
+                           int32_t precision, void* stream);
+
+/* v2 = -i H v1 for one complex vector (interleaved re/im, length ndim):
+ * matxvec_treal (source/propagation/tdselib.f90:323-437). */
+int32_t adc2g_matxvec_treal(adc2g_handle* h, const double* v1, double* v2, int32_t precision);
+
+/* ---- solvers ------------------------------------------------------------ */
+
+/* Block Davidson for the lowest nstates eigenpairs: davdiag_block
+ * (source/eigen/block_davidson.f90:33-145) via master_eig
+ * (source/adclib/adc_common.f90:952-993).  eigval[nstates] ascending,
+ * eigvec (ndim x nstates, may be NULL), stats may be NULL. */
+int32_t adc2g_davidson(adc2g_handle* h, const adc2g_dav_params* p, double* eigval,
+                       double* eigvec, adc2g_dav_stats* stats);
+
+/* SIL wavepacket propagation from fvec (real, ndim): propagate_sillib
+ * (source/propagation/fvecprop.f90:365-564) with SILStep
+ * (source/propagation/sillib.f90:108-393).  auto0 receives nrec+1 complex
+ * values (interleaved), the zeroth-order autocorrelation at t = i*tout,
+ * i = 0..nrec with nrec = int(tfinal/(tout/2)); nrec_out may be NULL. */
+int32_t adc2g_sil_autocorr(adc2g_handle* h, const adc2g_sil_params* p, const double* fvec,
+                           double* auto0, int64_t nrec_cap, int64_t* nrec_out,
+                           adc2g_sil_stats* stats);
+
+/* ---- learned preconditioner -------------------------------------------- */
+
+/* Run FP64 Davidson on this handle's system with the reference DPR step and
+ * append (features, DPR step, exact correction) records of every unconverged
+ * root and iteration to the handle's training set (kept in HBM); at most
+ * max_records are kept per call.  Returns the number added. */
+int32_t adc2g_precon_harvest(adc2g_handle* h, const adc2g_dav_params* p, int64_t max_records,
+                             int64_t* added);
+
+/* Train the network on the records of `data` (a handle that harvested; may
+ * be h itself) with Adam for `epochs` passes of mini-batches of `batch`
+ * records; `loss_out[epochs]` (may be NULL) receives the mean loss per pass. */
+int32_t adc2g_precon_train(adc2g_handle* h, const adc2g_handle* data, int32_t epochs,
+                           int32_t batch, double lr, uint64_t seed, double* loss_out);
+
+/* Copy the network to/from host memory (flat float32 array of
+ * adc2g_precon_nparams() values in the order W1, E, b1, W2, b2, w3, b3). */
+int32_t adc2g_precon_nparams(void);
+int32_t adc2g_precon_get(const adc2g_handle* h, float* params);
+int32_t adc2g_precon_set(adc2g_handle* h, const float* params);
+
+/* Evaluate the network (forward) and its gradient of the mean loss
+ * (backward) on host-supplied records, for checking against a CPU oracle.
+ * feat[n*3] (transformed features), cls[n], dn[n], en[n]; m_out[n] and
+ * grad_out[nparams] may be NULL. */
+int32_t adc2g_precon_eval(adc2g_handle* h, int64_t n, const float* feat, const int32_t* cls,
+                          const float* dn, const float* en, float* m_out, float* grad_out,
+                          double* loss_out);
+
+/* ---- diagnostics ---------------------------------------------------------- */
+
+/* Copy the last error message (NUL-terminated, truncated to len). */
+int32_t adc2g_last_error(char* buf, int32_t len);
+
+/* Average device time (ms) per launch of each kernel class over the last
+ * sigma-vector call, and the bytes/flops each class is credited with:
+ * kinds 0 = coupling GEMMs, 1 = 2h2p diagonal + pair packing.  Used by the
+ * benchmark; returns ADC2G_ESTATE before the first timed sigma. */
+int32_t adc2g_sigma_profile(adc2g_handle* h, int32_t enable, double* ms_gemm,
+                            double* ms_stream, double* flops_gemm, double* bytes_stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ADC2G_H */

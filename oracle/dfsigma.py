@@ -124,13 +124,15 @@ class DF:
         Z = pair_tensor(x2, o, v)                                    # [c,b,k,l]
         Zm = Z.transpose(0, 2, 1, 3).reshape(self.n1, self.n1)      # [(c,k),(b,l)]
         Y = (self.Bov @ Zm.T).reshape(-1, v, o)                     # Y[P,c,k]
-        s = np.einsum("Pac,Pck->ak", self.Bvv, Y) - np.einsum("Pkm,Pam->ak", self.Boo, Y)
+        s = (np.einsum("Pac,Pck->ak", self.Bvv, Y, optimize=True)
+             - np.einsum("Pkm,Pam->ak", self.Boo, Y, optimize=True))
         return s.reshape(self.n1)
 
     def couple_up(self, x1):
         o, v = self.o, self.v
         X = x1.reshape(v, o)
-        W = np.einsum("Pca,ak->Pck", self.Bvv, X) - np.einsum("Pkm,cm->Pck", self.Boo, X)
+        W = (np.einsum("Pca,ak->Pck", self.Bvv, X, optimize=True)
+             - np.einsum("Pkm,cm->Pck", self.Boo, X, optimize=True))
         Gm = W.reshape(-1, self.n1).T @ self.Bov                    # [(c,k),(b,l)]
         G = Gm.reshape(v, o, v, o).transpose(0, 2, 1, 3)            # [c,b,k,l]
         return csf_from_pairs(G, o, v)
@@ -171,7 +173,7 @@ def phph_block(sys) -> np.ndarray:
     n1 = o * v
     Bov = np.ascontiguousarray(B[:, o:, :o]).reshape(B.shape[0], n1)
     M = (Bov.T @ Bov).reshape(v, o, v, o)               # M[a,k,c,i] = (ak|ci)
-    J = np.einsum("Pab,Pkl->akbl", B[:, o:, o:], B[:, :o, :o])
+    J = np.einsum("Pab,Pkl->akbl", B[:, o:, o:], B[:, :o, :o], optimize=True)
     H = (2.0 * M - J).reshape(n1, n1)
     H[np.arange(n1), np.arange(n1)] += (ev[:, None] - eo[None, :]).ravel()
     # CA: K(a;u,v,r) = M[a,u,r,v]
