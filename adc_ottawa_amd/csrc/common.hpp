@@ -90,4 +90,11 @@ struct Dims {
   }
 };
 
+inline int grid_for(int64_t n, int block = 256, int cap = 65536) {
+  int64_t g = (n + block - 1) / block;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
 }  // namespace adc2g

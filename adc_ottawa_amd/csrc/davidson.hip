@@ -1,0 +1,392 @@
+// Block Davidson on the GPU.
+//
+// Semantics follow davdiag_block / run_block_davidson
+// (source/eigen/block_davidson.f90:33-145, 690-774): block size and subspace
+// defaults (:195-205), unit-vector guesses on the smallest |H_jj| (:463-532),
+// Rayleigh-Ritz on the lowest blocksize_curr pairs, r = theta x - W y and
+// ||r||_2 < tol (:1029-1085), the convergence count reset without deflation
+// (:1077), deflation by moving converged roots to the end of the arrays
+// (:1147-1222), the collapse rule currdim > maxvec_curr - blocksize_curr
+// (:1109-1116), DPR/Olsen (:1288-1419) and projection against converged
+// vectors (:1226-1284), eigenpairs returned sorted ascending (:1469-1532).
+//
+// Different by design (same subspaces in exact arithmetic, so the same Ritz
+// values iteration by iteration):
+//  * W is kept: only the new block gets a sigma-vector (the reference
+//    recomputes W for all currdim columns each iteration, nmult += currdim;
+//    that count is reported as nmult_ref).  A collapse keeps W Y.
+//  * The new block is orthonormalised against V by two block classical
+//    Gram-Schmidt passes and within itself through the eigen-decomposition of
+//    its Gram matrix, instead of a Householder QR of the whole subspace
+//    (:1423-1465); numerically dependent directions are dropped.
+//  * Preconditioner denominators below 1e-10 in magnitude are clamped (the
+//    reference divides unguarded).
+// Mixed precision: with precision != FP64 the first phase uses low-precision
+// sigma-vectors without convergence counting; when every residual is below
+// lowprec_tol (or stalls) the solver restarts from the Ritz vectors with FP64
+// sigma-vectors ("restarted refinement"), so convergence is always judged on
+// FP64 residuals.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <numeric>
+
+#include "linalg.hpp"
+#include "precon.hpp"
+#include "solvers.hpp"
+
+namespace adc2g {
+
+namespace {
+
+void gemm_tn(Handle& H, int M, int N, int64_t K, const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
+             int64_t ldc, double alpha, double beta) {
+  GemmArgs g;
+  g.M = M; g.N = N; g.K = (int)K;
+  g.A = A; g.sam = lda; g.sak = 1;
+  g.B = B; g.sbk = 1; g.sbn = ldb;
+  g.map.C = C; g.map.scm = 1; g.map.scn = ldc; g.map.alpha = alpha; g.map.beta = beta;
+  gemm(g, Prec::F64, H.gws, H.st);
+}
+
+void gemm_nn(Handle& H, int64_t M, int N, int K, const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
+             int64_t ldc, double alpha, double beta) {
+  GemmArgs g;
+  g.M = (int)M; g.N = N; g.K = K;
+  g.A = A; g.sam = 1; g.sak = lda;
+  g.B = B; g.sbk = 1; g.sbn = ldb;
+  g.map.C = C; g.map.scm = 1; g.map.scn = ldc; g.map.alpha = alpha; g.map.beta = beta;
+  gemm(g, Prec::F64, H.gws, H.st);
+}
+
+struct Dav {
+  Handle& H;
+  int64_t n;
+  int bs, mv;
+  DevBuf<double> V, W, X, WY, Res, T, Rm, Rt, Zs, thd, part, nrm, small, work, M;
+  DevBuf<int> info;
+  DevBuf<int64_t> idx;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double t_sigma = 0;
+  int nsigma = 0;
+
+  explicit Dav(Handle& h) : H(h) {}
+  ~Dav() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+
+  void alloc() {
+    n = H.d.ndim;
+    V.alloc((size_t)n * mv);
+    W.alloc((size_t)n * mv);
+    X.alloc((size_t)n * bs);
+    WY.alloc((size_t)n * bs);
+    Res.alloc((size_t)n * bs);
+    T.alloc((size_t)n * bs);
+    Rm.alloc((size_t)mv * mv);
+    Rt.alloc((size_t)mv * mv);
+    Zs.alloc((size_t)mv * mv);
+    thd.alloc(mv);
+    part.alloc((size_t)1024 * 2 * bs);
+    nrm.alloc(2 * bs);
+    small.alloc((size_t)mv * std::max(mv, bs) + 16);
+    work.alloc((size_t)4 * mv * mv + 4 * mv);
+    M.alloc((size_t)bs * bs);
+    info.alloc(4);
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+  }
+
+  void sigma(int c0, int nc, Prec p) {
+    HIP_CHECK(hipEventRecord(e0, H.st));
+    H.sigma(V.get() + n * c0, n, W.get() + n * c0, n, nc, p, H.st);
+    HIP_CHECK(hipEventRecord(e1, H.st));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    t_sigma += ms * 1e-3;
+    nsigma += nc;
+  }
+
+  // Rm[0:c1, c0:c1] = V[:, :c1]^T W[:, c0:c1], then mirror.
+  void rayleigh_cols(int c0, int c1) {
+    gemm_tn(H, c1, c1 - c0, n, V.get(), n, W.get() + n * c0, n, Rm.get() + (int64_t)mv * c0, mv, 1.0, 0.0);
+    launch_symmetrize_cols(c0, c1, Rm.get(), mv, H.st);
+  }
+
+  // Remove span(B[:, :nb]) from T[:, :r] (two classical Gram-Schmidt passes).
+  void project(const double* B, int nb, double* Tp, int r) {
+    if (nb <= 0 || r <= 0) return;
+    for (int pass = 0; pass < 2; ++pass) {
+      gemm_tn(H, nb, r, n, B, n, Tp, n, small.get(), nb, 1.0, 0.0);
+      gemm_nn(H, n, r, nb, B, n, small.get(), nb, Tp, n, -1.0, 1.0);
+    }
+  }
+
+  // Orthonormalise T[:, :r] within itself into dst[:, :kept] (dst != T).
+  int orth_within(double* Tp, int r, double* dst) {
+    gemm_tn(H, r, r, n, Tp, n, Tp, n, small.get(), r, 1.0, 0.0);
+    launch_orth_coeffs(r, small.get(), r, M.get(), r, work.get(), info.get(), 1e-12, H.st);
+    int kept = 0;
+    HIP_CHECK(hipMemcpyAsync(&kept, info.get(), sizeof(int), hipMemcpyDeviceToHost, H.st));
+    HIP_CHECK(hipStreamSynchronize(H.st));
+    if (kept < 0) throw Error(ADC2G_EEIG, "Gram-matrix eigensolve failed in block orthonormalisation");
+    if (kept > 0) gemm_nn(H, n, kept, r, Tp, n, M.get(), r, dst, n, 1.0, 0.0);
+    return kept;
+  }
+
+  // New block T[:, :r] -> V[:, c0:c0+kept], orthonormal to V[:, :c0].
+  int append_block(int c0, int r) {
+    project(V.get(), c0, T.get(), r);
+    double* slot = V.get() + n * c0;
+    int k1 = orth_within(T.get(), r, slot);
+    if (k1 == 0) return 0;
+    int k2 = orth_within(slot, k1, T.get());
+    if (k2 > 0) HIP_CHECK(hipMemcpyAsync(slot, T.get(), sizeof(double) * n * k2, hipMemcpyDeviceToDevice, H.st));
+    return k2;
+  }
+
+  void swap_cols(DevBuf<double>& B, int i, int j) {
+    if (i == j) return;
+    double* tmp = T.get() + n * (bs - 1);  // scratch column (T is rebuilt afterwards)
+    HIP_CHECK(hipMemcpyAsync(tmp, B.get() + n * i, sizeof(double) * n, hipMemcpyDeviceToDevice, H.st));
+    HIP_CHECK(hipMemcpyAsync(B.get() + n * i, B.get() + n * j, sizeof(double) * n, hipMemcpyDeviceToDevice, H.st));
+    HIP_CHECK(hipMemcpyAsync(B.get() + n * j, tmp, sizeof(double) * n, hipMemcpyDeviceToDevice, H.st));
+  }
+};
+
+}  // namespace
+
+void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* eigvec, adc2g_dav_stats* stats,
+                  DavTrace* trace) {
+  const auto t0 = std::chrono::steady_clock::now();
+  ARG_CHECK(p.nstates >= 1, "nstates must be >= 1");
+  ARG_CHECK(p.tol > 0, "tol must be > 0");
+  ARG_CHECK(p.precon >= 1 && p.precon <= 3, "precon must be 1 (DPR), 2 (Olsen) or 3 (learned)");
+  if (p.precon == 3 && !precon_ready(H))
+    throw Error(ADC2G_ESTATE, "learned preconditioner requested but no network is loaded");
+  const int nstates = p.nstates;
+  // davinitialise (block_davidson.f90:195-205)
+  int bs = p.blocksize > 0 ? p.blocksize : std::max(nstates + 5, 2 * nstates);
+  int mv = p.maxvec >= 0 ? p.maxvec : 4 * bs;
+  mv = (int)std::floor((double)mv / bs) * bs;
+  ARG_CHECK(bs >= nstates, "blocksize must be >= nstates");
+  ARG_CHECK(mv >= 2 * bs, "maxvec must be at least twice the block size");
+  ARG_CHECK((int64_t)mv <= H.d.ndim, "maxvec exceeds the matrix dimension");
+  ARG_CHECK(mv <= 512, "maxvec > 512 is not supported by the subspace eigensolver");
+
+  Dav D(H);
+  D.bs = bs;
+  D.mv = mv;
+  D.alloc();
+  const int64_t n = D.n;
+  hipStream_t st = H.st;
+  Prec cur = prec_from_int(p.precision);
+  bool lowphase = (cur != Prec::F64);
+  const bool refine = lowphase ? (p.refine != 0) : true;
+  const double lowtol = p.lowprec_tol > 0 ? p.lowprec_tol : 1e-4;
+
+  // initvec_ondiag: unit vectors on the bs smallest |H_jj| (stable order for ties)
+  {
+    std::vector<double> hii(n);
+    HIP_CHECK(hipMemcpyAsync(hii.data(), H.hii.get(), sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    std::vector<int64_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::partial_sort(order.begin(), order.begin() + bs, order.end(), [&](int64_t a, int64_t b) {
+      const double fa = std::fabs(hii[a]), fb = std::fabs(hii[b]);
+      return fa < fb || (fa == fb && a < b);
+    });
+    DevBuf<int64_t> didx;
+    didx.alloc(bs);
+    HIP_CHECK(hipMemcpyAsync(didx.get(), order.data(), sizeof(int64_t) * bs, hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemsetAsync(D.V.get(), 0, sizeof(double) * n * bs, st));
+    launch_unit_vectors(bs, didx.get(), D.V.get(), n, st);
+    HIP_CHECK(hipStreamSynchronize(st));
+  }
+  int currdim = bs, bs_curr = bs, mv_curr = mv, ns_curr = nstates;
+  int nconv = 0, nconv_prev = 0;
+  int64_t nmult_ref = 0;
+  int niter_low = 0, nfallback = 0;
+  D.sigma(0, bs, cur);
+  D.rayleigh_cols(0, bs);
+
+  std::vector<double> theta(bs), norm(bs), stored_norm(bs, 0.0);
+  std::vector<double> prev_norm(bs, INFINITY);
+  std::vector<int> stall(bs, 0);
+  double low_best = INFINITY;
+  int low_stall = 0;
+  bool converged = false;
+  int it = 0;
+  const int maxiter = p.maxiter > 0 ? p.maxiter : 100;
+  for (it = 1; it <= maxiter; ++it) {
+    nmult_ref += currdim;
+    // Rayleigh-Ritz (diagrmat)
+    HIP_CHECK(hipMemcpy2DAsync(D.Rt.get(), sizeof(double) * mv, D.Rm.get(), sizeof(double) * mv,
+                               sizeof(double) * currdim, currdim, hipMemcpyDeviceToDevice, st));
+    launch_syev(currdim, D.Rt.get(), mv, D.thd.get(), D.Zs.get(), mv, D.work.get(), D.info.get(), st);
+    // Ritz vectors and W y (calcritzvec, calcres)
+    gemm_nn(H, n, bs_curr, currdim, D.V.get(), n, D.Zs.get(), mv, D.X.get(), n, 1.0, 0.0);
+    gemm_nn(H, n, bs_curr, currdim, D.W.get(), n, D.Zs.get(), mv, D.WY.get(), n, 1.0, 0.0);
+    const int nb = launch_residual(n, bs_curr, D.thd.get(), D.X.get(), D.WY.get(), D.Res.get(), n, D.part.get(), st);
+    launch_colsum(nb, bs_curr, D.part.get(), D.nrm.get(), st);
+    int einfo = 0;
+    HIP_CHECK(hipMemcpyAsync(&einfo, D.info.get(), sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(theta.data(), D.thd.get(), sizeof(double) * bs_curr, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(norm.data(), D.nrm.get(), sizeof(double) * bs_curr, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    if (einfo != 0) throw Error(ADC2G_EEIG, "Diagonalisation of the Rayleigh matrix failed");
+    for (int i = 0; i < bs_curr; ++i) norm[i] = std::sqrt(norm[i]);
+    if (lowphase) ++niter_low;
+
+    if (p.verbose) {
+      for (int i = 0; i < ns_curr; ++i)
+        std::printf(i == 0 ? "%4d %c  %14.7f  %13.7e  %c\n" : "       %14.7f  %13.7e  %c\n",
+                    i == 0 ? it : 0, lowphase ? 'L' : 'D', theta[i] * 27.2113845, norm[i],
+                    norm[i] < p.tol ? 'y' : 'n');
+      std::fflush(stdout);
+    }
+    if (trace) davtrace_iteration(H, *trace, it, bs_curr, ns_curr, theta.data(), norm.data(), D.X.get(),
+                                  D.Res.get(), n);
+
+    bool newly = false;
+    if (lowphase) {
+      double mx = 0;
+      for (int i = 0; i < nstates; ++i) mx = std::max(mx, norm[i]);
+      if (mx < low_best * 0.8) {
+        low_best = mx;
+        low_stall = 0;
+      } else {
+        ++low_stall;
+      }
+      if (refine && (mx < lowtol || low_stall >= 4)) {
+        // restarted refinement: FP64 sigma-vectors from the current Ritz vectors
+        lowphase = false;
+        cur = Prec::F64;
+        HIP_CHECK(hipMemcpyAsync(D.V.get(), D.X.get(), sizeof(double) * n * bs_curr, hipMemcpyDeviceToDevice, st));
+        currdim = bs_curr;
+        D.sigma(0, currdim, cur);
+        D.rayleigh_cols(0, currdim);
+        std::fill(prev_norm.begin(), prev_norm.end(), INFINITY);
+        continue;
+      }
+      if (!refine) {
+        int c = 0;
+        for (int i = 0; i < ns_curr; ++i)
+          if (norm[i] < p.tol) ++c;
+        if (c == nstates) {
+          converged = true;
+          break;
+        }
+      }
+    } else {
+      // calcres: convergence count
+      if (!p.deflate) nconv = 0;
+      for (int i = 0; i < ns_curr; ++i)
+        if (norm[i] < p.tol) ++nconv;
+      if (nconv == nstates) {
+        converged = true;
+        break;
+      }
+      // deflate (block_davidson.f90:1147-1222)
+      newly = p.deflate && nconv > nconv_prev;
+      if (newly) {
+        std::vector<int> convmap;
+        for (int i = 0; i < bs_curr; ++i)
+          if (norm[i] < p.tol) convmap.push_back(i);
+        for (int i = 1; i <= nconv - nconv_prev; ++i) {
+          const int indx = bs - nconv_prev - i;
+          const int src = convmap[i - 1];
+          D.swap_cols(D.X, indx, src);
+          D.swap_cols(D.Res, indx, src);
+          D.swap_cols(D.WY, indx, src);
+          std::swap(theta[indx], theta[src]);
+          std::swap(norm[indx], norm[src]);
+          stored_norm[indx] = norm[indx];
+        }
+        const int dd = nconv - nconv_prev;
+        bs_curr -= dd;
+        mv_curr -= dd;
+        ns_curr -= dd;
+        currdim -= dd;
+      }
+    }
+    bool collapse = !(currdim <= mv_curr - bs_curr);
+    if (newly) collapse = true;
+    int ilast = currdim;
+    if (collapse) {
+      HIP_CHECK(hipMemcpyAsync(D.V.get(), D.X.get(), sizeof(double) * n * bs_curr, hipMemcpyDeviceToDevice, st));
+      HIP_CHECK(hipMemcpyAsync(D.W.get(), D.WY.get(), sizeof(double) * n * bs_curr, hipMemcpyDeviceToDevice, st));
+      D.rayleigh_cols(0, bs_curr);
+      ilast = bs_curr;
+    }
+    // new directions (dpr / olsen / learned)
+    HIP_CHECK(hipMemcpyAsync(D.thd.get(), theta.data(), sizeof(double) * bs_curr, hipMemcpyHostToDevice, st));
+    if (p.precon == 1) {
+      launch_dpr(n, bs_curr, D.thd.get(), H.hii.get(), D.Res.get(), D.T.get(), n, st);
+    } else if (p.precon == 2) {
+      const int nb2 = launch_olsen_dots(n, bs_curr, D.thd.get(), H.hii.get(), D.X.get(), D.Res.get(), n,
+                                        D.part.get(), st);
+      launch_colsum(nb2, 2 * bs_curr, D.part.get(), D.nrm.get(), st);
+      launch_olsen_step(n, bs_curr, D.thd.get(), H.hii.get(), D.X.get(), D.Res.get(), D.nrm.get(), D.T.get(), n, st);
+    } else {
+      // learned step with DPR fallback for roots whose residual stalls
+      std::vector<int> use_dpr(bs_curr, 0);
+      for (int i = 0; i < bs_curr; ++i) {
+        if (norm[i] > 0.9 * prev_norm[i]) ++stall[i]; else stall[i] = 0;
+        if (stall[i] >= 2) {
+          use_dpr[i] = 1;
+          ++nfallback;
+        }
+      }
+      precon_learned_step(H, n, bs_curr, D.thd.get(), norm.data(), use_dpr.data(), D.Res.get(), D.T.get(), n);
+    }
+    for (int i = 0; i < bs_curr; ++i) prev_norm[i] = norm[i];
+    // projection against the converged (deflated) vectors
+    if (p.deflate && nconv > 0 && !lowphase) D.project(D.X.get() + n * (bs - nconv), nconv, D.T.get(), bs_curr);
+    const int kept = D.append_block(ilast, bs_curr);
+    if (kept == 0) throw Error(ADC2G_ENOCONV, "subspace expansion produced no new directions");
+    D.sigma(ilast, kept, cur);
+    D.rayleigh_cols(ilast, ilast + kept);
+    currdim = ilast + kept;
+    nconv_prev = nconv;
+  }
+  if (!converged) throw Error(ADC2G_ENOCONV, "Not all vectors have converged...");
+
+  // wreigenpairs: deflated roots back after the active ones, ascending order
+  std::vector<int> col(nstates);
+  for (int i = 0; i < ns_curr; ++i) col[i] = i;
+  for (int i = ns_curr; i < nstates; ++i) col[i] = bs - nconv_prev + (i - ns_curr);
+  std::vector<double> ev(nstates), rn(nstates);
+  for (int i = 0; i < nstates; ++i) {
+    ev[i] = theta[col[i]];
+    rn[i] = (i < ns_curr) ? norm[col[i]] : stored_norm[col[i]];
+  }
+  std::vector<int> order(nstates);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ev[a] < ev[b]; });
+  double rmax = 0;
+  for (int i = 0; i < nstates; ++i) {
+    eigval[i] = ev[order[i]];
+    rmax = std::max(rmax, rn[order[i]]);
+  }
+  if (eigvec) {
+    for (int i = 0; i < nstates; ++i)
+      HIP_CHECK(hipMemcpyAsync(eigvec + n * i, D.X.get() + n * col[order[i]], sizeof(double) * n,
+                               hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+  }
+  if (stats) {
+    stats->niter = it;
+    stats->nsigma = D.nsigma;
+    stats->nmult_ref = nmult_ref;
+    stats->niter_lowprec = niter_low;
+    stats->nfallback = nfallback;
+    stats->time_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    stats->time_sigma = D.t_sigma;
+    stats->resnorm_max = rmax;
+  }
+}
+
+}  // namespace adc2g

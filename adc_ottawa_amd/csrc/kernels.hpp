@@ -56,11 +56,5 @@ void launch_phph_assemble(double* H11, const double* Mm, const double* Jm, const
                           const double* CB, const double* eps, const DimsK& d, int second_order,
                           hipStream_t st);
 
-inline int grid_for(int64_t n, int block = 256, int cap = 65536) {
-  int64_t g = (n + block - 1) / block;
-  if (g > cap) g = cap;
-  if (g < 1) g = 1;
-  return (int)g;
-}
 
 }  // namespace adc2g

@@ -7,9 +7,6 @@ void matxvec_treal_host(Handle& H, const double* v1, double* v2, Prec p) {
   (void)H; (void)v1; (void)v2; (void)p;
   throw Error(ADC2G_ESTATE, "not built yet");
 }
-void davidson_run(Handle&, const adc2g_dav_params&, double*, double*, adc2g_dav_stats*, DavTrace*) {
-  throw Error(ADC2G_ESTATE, "not built yet");
-}
 void sil_run(Handle&, const adc2g_sil_params&, const double*, double*, int64_t, int64_t*, adc2g_sil_stats*) {
   throw Error(ADC2G_ESTATE, "not built yet");
 }
@@ -22,4 +19,15 @@ void precon_eval(Handle&, int64_t, const float*, const int32_t*, const float*, c
   throw Error(ADC2G_ESTATE, "not built yet");
 }
 
+}  // namespace adc2g
+
+#include "precon.hpp"
+namespace adc2g {
+bool precon_ready(const Handle&) { return false; }
+void precon_learned_step(Handle&, int64_t, int, const double*, const double*, const int*, const double*, double*,
+                         int64_t) {
+  throw Error(ADC2G_ESTATE, "not built yet");
+}
+void davtrace_iteration(Handle&, DavTrace&, int, int, int, const double*, const double*, const double*,
+                        const double*, int64_t) {}
 }  // namespace adc2g
