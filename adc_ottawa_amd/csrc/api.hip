@@ -202,18 +202,16 @@ int32_t adc2g_last_error(char* buf, int32_t len) {
   return ADC2G_OK;
 }
 
-int32_t adc2g_sigma_profile(adc2g_handle* h, int32_t enable, double* ms_gemm, double* ms_stream,
-                            double* flops_gemm, double* bytes_stream) {
+int32_t adc2g_sigma_profile(adc2g_handle* h, int32_t enable, double* out) {
   return guarded([&] {
     ARG_CHECK(h != nullptr, "NULL handle");
     Handle& H = h->h;
-    H.timing = enable != 0;
-    if (ms_gemm || ms_stream || flops_gemm || bytes_stream) {
-      if (H.last_ms_gemm < 0) throw Error(ADC2G_ESTATE, "no timed sigma-vector yet");
-      if (ms_gemm) *ms_gemm = H.last_ms_gemm;
-      if (ms_stream) *ms_stream = H.last_ms_stream;
-      if (flops_gemm) *flops_gemm = H.last_flops_gemm;
-      if (bytes_stream) *bytes_stream = H.last_bytes_stream;
+    HIP_CHECK(hipSetDevice(H.device));
+    if (out) H.prof_collect(out);
+    H.prof_on = enable != 0;
+    if (!H.prof_on) {
+      H.prof.clear();
+      H.ev_used = 0;
     }
   });
 }

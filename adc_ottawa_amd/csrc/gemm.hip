@@ -9,8 +9,13 @@ template <class T, int BM, int BN, int BK>
 void launch(const GemmArgs& g, hipStream_t st) {
   const int tiles = ((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
   dim3 grid(tiles, g.batch, g.splitk);
-  hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, double>), grid, dim3(256), 0, st, g);
+  if (g.ev_begin) HIP_CHECK(hipEventRecord(g.ev_begin, st));
+  if (g.tag == 1)
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, double, 1>), grid, dim3(256), 0, st, g);
+  else
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, double, 0>), grid, dim3(256), 0, st, g);
   HIP_CHECK(hipGetLastError());
+  if (g.ev_end) HIP_CHECK(hipEventRecord(g.ev_end, st));
 }
 
 template <class T>

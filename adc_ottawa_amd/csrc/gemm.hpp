@@ -48,6 +48,11 @@ struct GemmArgs {
   MapParams map;
   int out_prec = 0;         // output element type: 0 double, 1 float, 2 bfloat16
   void* partial = nullptr;  // split-K workspace (accumulator type), batch*splitk*M*N
+  // host-side only: kernel tag (1 = the DF coupling contractions, compiled as a
+  // separately named kernel so profilers report them apart) and optional
+  // events recorded around the main kernel launch
+  int tag = 0;
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
 };
 
 // Output offset(s) of element (m, n) of batch b. Returns the number of outputs (1 or 2).
@@ -177,7 +182,7 @@ __device__ __forceinline__ int xcd_tile(int w, int T) {
   return x * per + (x < rem ? x : rem) + slot;
 }
 
-template <class T, int BM, int BN, int BK, class OutT>
+template <class T, int BM, int BN, int BK, class OutT, int TAG>
 __global__ void __launch_bounds__(256, 2)
 gemm_kernel(GemmArgs g) {
   typedef Mma<T> MT;

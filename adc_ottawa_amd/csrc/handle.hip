@@ -26,9 +26,32 @@ Prec prec_from_int(int p) {
 static size_t prec_size(Prec p) { return p == Prec::F64 ? 8 : (p == Prec::F32 ? 4 : 2); }
 
 Handle::~Handle() {
-  for (auto& e : ev)
+  for (auto& e : ev_pool)
     if (e) (void)hipEventDestroy(e);
   if (st) (void)hipStreamDestroy(st);
+}
+
+hipEvent_t Handle::prof_event() {
+  if (ev_used == ev_pool.size()) {
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    ev_pool.push_back(e);
+  }
+  return ev_pool[ev_used++];
+}
+
+void Handle::prof_collect(double out[9]) {
+  for (int i = 0; i < 9; ++i) out[i] = 0.0;
+  HIP_CHECK(hipDeviceSynchronize());  // records may sit on a caller's stream
+  for (const auto& r : prof) {
+    float ms = 0.0f;
+    HIP_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+    out[3 * r.cls] += ms;
+    out[3 * r.cls + 1] += 1.0;
+    out[3 * r.cls + 2] += r.work;
+  }
+  prof.clear();
+  ev_used = 0;
 }
 
 void Handle::create(int o, int v, int naux, const double* eps_h, const double* B_h, const adc2g_opts& opts) {
@@ -182,24 +205,26 @@ void Handle::sigma(const double* X, int64_t ldx, double* W, int64_t ldw, int nve
   const int out_lp = (p == Prec::F32) ? 1 : (p == Prec::BF16 ? 2 : 0);
   const void* BovP = (p == Prec::F64) ? (const void*)Bov.get()
                      : (p == Prec::F32 ? (const void*)Bov32.get() : (const void*)Bov16.get());
-  if (timing) {
-    for (auto& e : ev)
-      if (!e) HIP_CHECK(hipEventCreate(&e));
-  }
-  double flops = 0, bytes = 0;
-  float ms_g = 0, ms_s = 0;
+  const double es = (double)prec_size(p);
+  auto span = [&](int cls, double work) -> std::pair<hipEvent_t, hipEvent_t> {
+    if (!prof_on) return {nullptr, nullptr};
+    hipEvent_t a = prof_event(), b = prof_event();
+    prof.push_back({cls, a, b, work});
+    return {a, b};
+  };
   for (int c0 = 0; c0 < nvec; c0 += nc_max) {
     const int nc = std::min(nc_max, nvec - c0);
     const double* Xc = X + ldx * c0;
     double* Wc = W + ldw * c0;
-    if (timing) HIP_CHECK(hipEventRecord(ev[0], s));
     launch_pack_x1<double>(Xc, ldx, nc, dk, X1c.get(), s);
+    auto ep = span(1, (double)nc * (d.n2 * 8.0 + (double)n1 * n1 * es));
+    if (ep.first) HIP_CHECK(hipEventRecord(ep.first, s));
     switch (p) {
       case Prec::F64: launch_csf_to_pair<double>(Xc, ldx, nc, dk, (double*)ZG.get(), s); break;
       case Prec::F32: launch_csf_to_pair<float>(Xc, ldx, nc, dk, (float*)ZG.get(), s); break;
       case Prec::BF16: launch_csf_to_pair<__bf16>(Xc, ldx, nc, dk, (__bf16*)ZG.get(), s); break;
     }
-    if (timing) HIP_CHECK(hipEventRecord(ev[1], s));
+    if (ep.second) HIP_CHECK(hipEventRecord(ep.second, s));
     // sigma1 = H11 x1
     {
       GemmArgs g;
@@ -217,8 +242,10 @@ void Handle::sigma(const double* X, int64_t ldx, double* W, int64_t ldw, int nve
       g.B = ZG.get(); g.sbk = 1; g.sbn = n1;
       g.map.kind = MAP_DOWNY; g.map.C = Yvv.get(); g.map.C2 = Yoo.get();
       g.map.q[0] = o; g.map.q[1] = nc; g.map.q[2] = n1; g.map.q[3] = naux;
+      g.tag = 1;
+      auto e = span(0, 2.0 * naux * (double)n1 * n1 * nc);
+      g.ev_begin = e.first; g.ev_end = e.second;
       gemm(g, p, gws, s);
-      flops += 2.0 * naux * (double)n1 * n1 * nc;
     }
     // sigma1(a,k) += sum_{P,c} B[P,a,c] Y[P,c,k]
     {
@@ -270,10 +297,13 @@ void Handle::sigma(const double* X, int64_t ldx, double* W, int64_t ldw, int nve
       g.B = BovP; g.sbk = n1; g.sbn = 1;
       g.map.C = ZG.get(); g.map.scm = n1; g.map.scn = 1;
       g.out_prec = out_lp;
+      g.tag = 1;
+      auto e = span(0, 2.0 * naux * (double)n1 * n1 * nc);
+      g.ev_begin = e.first; g.ev_end = e.second;
       gemm(g, p, gws, s);
-      flops += 2.0 * naux * (double)n1 * n1 * nc;
     }
-    if (timing) HIP_CHECK(hipEventRecord(ev[2], s));
+    auto eu = span(2, (double)nc * (3.0 * d.n2 * 8.0 + (double)n1 * n1 * es));
+    if (eu.first) HIP_CHECK(hipEventRecord(eu.first, s));
     switch (p) {
       case Prec::F64:
         launch_pair_to_csf<double>((const double*)ZG.get(), Xc, ldx, Wc, ldw, nc, dk, eps.get(), s);
@@ -285,25 +315,7 @@ void Handle::sigma(const double* X, int64_t ldx, double* W, int64_t ldw, int nve
         launch_pair_to_csf<__bf16>((const __bf16*)ZG.get(), Xc, ldx, Wc, ldw, nc, dk, eps.get(), s);
         break;
     }
-    if (timing) {
-      HIP_CHECK(hipEventRecord(ev[3], s));
-      HIP_CHECK(hipEventSynchronize(ev[3]));
-      float a = 0, b = 0, c = 0;
-      HIP_CHECK(hipEventElapsedTime(&a, ev[0], ev[1]));
-      HIP_CHECK(hipEventElapsedTime(&b, ev[1], ev[2]));
-      HIP_CHECK(hipEventElapsedTime(&c, ev[2], ev[3]));
-      ms_g += b;
-      ms_s += a + c;
-      // streaming bytes: pack (read x2 once, write Z) + unpack (read G, x2, write sigma2)
-      const double es = (double)prec_size(p);
-      bytes += (double)nc * (2.0 * d.n2 * 8.0 + 2.0 * n1 * n1 * es + d.n2 * 8.0);
-    }
-  }
-  if (timing) {
-    last_ms_gemm = ms_g;
-    last_ms_stream = ms_s;
-    last_flops_gemm = flops;
-    last_bytes_stream = bytes;
+    if (eu.second) HIP_CHECK(hipEventRecord(eu.second, s));
   }
 }
 

@@ -32,10 +32,21 @@ struct Handle {
   GemmWorkspace gws;
   int chunk = 32;           // vectors per sigma pass
 
-  // optional device timing of the last sigma (bench)
-  bool timing = false;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  double last_ms_gemm = -1, last_ms_stream = -1, last_flops_gemm = 0, last_bytes_stream = 0;
+  // optional per-kernel device timing of sigma-vectors (benchmark): event
+  // pairs around the coupling GEMMs (class 0), the CSF->pair packing (1) and
+  // the pair->CSF projection with the 2h2p diagonal (2), with the algorithmic
+  // flops (class 0) or bytes (1, 2) each launch is credited with.
+  struct ProfRec {
+    int cls;
+    hipEvent_t a, b;
+    double work;
+  };
+  bool prof_on = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<ProfRec> prof;
+  hipEvent_t prof_event();
+  void prof_collect(double out[9]);
 
   // learned preconditioner (network + harvested training records)
   PreconNet* net = nullptr;

@@ -1,0 +1,350 @@
+// Short-iterative-Lanczos wavepacket propagation on the GPU.
+//
+// Follows propagate_sillib (source/propagation/fvecprop.f90:365-564) with
+// SILStep (source/propagation/sillib.f90:108-393) for the path the ADC driver
+// takes (Restart = .true., StdForm = .true., Relax = .false.; PreFactor = -i):
+//  * psi0 = F/|F| (init_wavepacket, fvecprop.f90:338-357), records at
+//    i*tout for i = 1..int(tfinal/intperiod), intperiod = tout/2, with the
+//    t/2 identity a0(2t) = sum_D psi_D(t)^2 |F|^2 (fvecprop.f90:521);
+//  * each step: alpha_0 = <psi|dtpsi>, first Krylov vector and error |beta dt|
+//    (sillib.f90:198-230); then Krylov vectors K_n = (-iH K_{n-1} - alpha
+//    K_{n-1} - cbeta K_{n-2}) / cbeta_n with Err = |Err beta dt / n| until
+//    order == IntOrder, Err <= TolError or beta < 1e-20 (:234-323); the next
+//    diagonal element estimated as the previous one (:327); stepsize reduced
+//    by (TolError/Err)^(1/order) when not converged (:331-334); eigenpairs of
+//    the (order+1) tridiagonal (DSTEQR, :350-361) and
+//    psi <- sum_p [sum_q exp(-i lambda_q dt) U_0q U_pq] K_p (:369-390).
+// Complex vectors are planar on the device (re column, im column), so -iH v
+// is one two-column sigma-vector: (Hv_im, -Hv_re).  Inner products, norms and
+// updates run on the GPU in FP64; the recursion's scalars drive the control
+// flow on the host, as the reference's single-threaded loop does.
+#include <chrono>
+#include <cmath>
+#include <complex>
+
+#include "linalg.hpp"
+#include "solvers.hpp"
+
+namespace adc2g {
+
+namespace {
+
+typedef std::complex<double> cplx;
+
+__device__ double block_sum2(double x, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wv] = x;
+  __syncthreads();
+  double t = 0.0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += sh[i];
+  return t;
+}
+
+// out = (H v)_im, -(H v)_re : turn the two-column sigma of (re, im) into -i H v
+__global__ void minus_i_kernel(int64_t n, const double* __restrict__ Hv, double* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double hr = Hv[i], hi = Hv[n + i];
+    out[i] = hi;
+    out[n + i] = -hr;
+  }
+}
+
+// w <- w - a x - b z  (complex scalars, planar vectors of length n)
+__global__ void caxpy2_kernel(int64_t n, double* __restrict__ w, double ar, double ai, const double* __restrict__ x,
+                              double br, double bi, const double* __restrict__ z) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double wr = w[i], wi = w[n + i];
+    const double xr = x[i], xi = x[n + i];
+    wr -= ar * xr - ai * xi;
+    wi -= ar * xi + ai * xr;
+    if (z) {
+      const double zr = z[i], zi = z[n + i];
+      wr -= br * zr - bi * zi;
+      wi -= br * zi + bi * zr;
+    }
+    w[i] = wr;
+    w[n + i] = wi;
+  }
+}
+
+// x <- c x
+__global__ void cscale_kernel(int64_t n, double* __restrict__ x, double cr, double ci) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double xr = x[i], xi = x[n + i];
+    x[i] = cr * xr - ci * xi;
+    x[n + i] = cr * xi + ci * xr;
+  }
+}
+
+// out = sum_p coef_p K_p (coef complex, np <= 32)
+__global__ void combine_kernel(int64_t n, int np, const double* __restrict__ coef, const double* __restrict__ K,
+                               int64_t stride, double* __restrict__ out) {
+  __shared__ double c[64];
+  if (threadIdx.x < 2 * np) c[threadIdx.x] = coef[threadIdx.x];
+  __syncthreads();
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double sr = 0.0, si = 0.0;
+    for (int p = 0; p < np; ++p) {
+      const double kr = K[p * stride + i], ki = K[p * stride + n + i];
+      sr += c[2 * p] * kr - c[2 * p + 1] * ki;
+      si += c[2 * p] * ki + c[2 * p + 1] * kr;
+    }
+    out[i] = sr;
+    out[n + i] = si;
+  }
+}
+
+// partial sums of psi^2 (not conjugated): re^2 - im^2, 2 re im
+__global__ void square_sum_kernel(int64_t n, const double* __restrict__ x, double* __restrict__ part) {
+  __shared__ double red[16];
+  double sr = 0.0, si = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double a = x[i], b = x[n + i];
+    sr += a * a - b * b;
+    si += 2.0 * a * b;
+  }
+  sr = block_sum2(sr, red);
+  si = block_sum2(si, red);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sr;
+    part[2 * blockIdx.x + 1] = si;
+  }
+}
+
+// coef_p = sum_q exp(-i lam_q dt) U[0,q] U[p,q]   (one thread per p)
+__global__ void expcoef_kernel(int np, const double* __restrict__ lam, const double* __restrict__ U, int ldu,
+                               double dt, double* __restrict__ coef) {
+  const int p = threadIdx.x;
+  if (p >= np) return;
+  double sr = 0.0, si = 0.0;
+  for (int q = 0; q < np; ++q) {
+    const double w = U[0 + q * ldu] * U[p + q * ldu];
+    sr += cos(lam[q] * dt) * w;
+    si += -sin(lam[q] * dt) * w;
+  }
+  coef[2 * p] = sr;
+  coef[2 * p + 1] = si;
+}
+
+struct Sil {
+  Handle& H;
+  int64_t n;
+  int kdim;
+  Prec prec;
+  DevBuf<double> K, psi, hv, tmp, part, sums, tri, U, lam, coef, work;
+  DevBuf<int> info;
+  int64_t nsigma = 0;
+  int nb = 0;
+
+  Sil(Handle& h, int kd, Prec p) : H(h), kdim(kd), prec(p) {
+    n = H.d.ndim;
+    K.alloc((size_t)(kdim + 1) * 2 * n);
+    psi.alloc(2 * n);
+    hv.alloc(2 * n);
+    tmp.alloc(2 * n);
+    nb = reduce_blocks(n);
+    part.alloc(2 * (size_t)nb);
+    sums.alloc(4);
+    tri.alloc((size_t)(kdim + 1) * (kdim + 1));
+    U.alloc((size_t)(kdim + 1) * (kdim + 1));
+    lam.alloc(kdim + 1);
+    coef.alloc(2 * (kdim + 1));
+    work.alloc((size_t)(kdim + 1) * (kdim + 1) + 64);
+    info.alloc(2);
+  }
+
+  double* Kp(int p) { return K.get() + (int64_t)p * 2 * n; }
+
+  // out = -i H v
+  void func(const double* v, double* out) {
+    H.sigma(v, n, hv.get(), n, 2, prec, H.st);
+    hipLaunchKernelGGL(minus_i_kernel, dim3(grid_for(n)), dim3(256), 0, H.st, n, hv.get(), out);
+    HIP_CHECK(hipGetLastError());
+    ++nsigma;
+  }
+
+  cplx cdot(const double* a, const double* b) {
+    const int nbk = launch_cdot_partial(n, a, b, n, part.get(), H.st);
+    launch_colsum(nbk, 2, part.get(), sums.get(), H.st);
+    double h[2];
+    HIP_CHECK(hipMemcpyAsync(h, sums.get(), sizeof(h), hipMemcpyDeviceToHost, H.st));
+    HIP_CHECK(hipStreamSynchronize(H.st));
+    return cplx(h[0], h[1]);
+  }
+
+  cplx square_sum(const double* a) {
+    hipLaunchKernelGGL(square_sum_kernel, dim3(nb), dim3(256), 0, H.st, n, a, part.get());
+    HIP_CHECK(hipGetLastError());
+    launch_colsum(nb, 2, part.get(), sums.get(), H.st);
+    double h[2];
+    HIP_CHECK(hipMemcpyAsync(h, sums.get(), sizeof(h), hipMemcpyDeviceToHost, H.st));
+    HIP_CHECK(hipStreamSynchronize(H.st));
+    return cplx(h[0], h[1]);
+  }
+
+  void axpy2(double* w, cplx a, const double* x, cplx b, const double* z) {
+    hipLaunchKernelGGL(caxpy2_kernel, dim3(grid_for(n)), dim3(256), 0, H.st, n, w, a.real(), a.imag(), x, b.real(),
+                       b.imag(), z);
+    HIP_CHECK(hipGetLastError());
+  }
+
+  void scale(double* x, cplx c) {
+    hipLaunchKernelGGL(cscale_kernel, dim3(grid_for(n)), dim3(256), 0, H.st, n, x, c.real(), c.imag());
+    HIP_CHECK(hipGetLastError());
+  }
+
+  // One SILStep on psi (dtpsi = -iH psi already in Kp(1)); returns stepsize.
+  double step(double intperiod, double tolerror, int& order_out) {
+    const double RelativeMinStep = 1.0e-10;
+    const cplx pref(0.0, -1.0);
+    if (kdim < 2) throw Error(ADC2G_ESIL_ORDER, "IntOrder is too small");
+    double stepsize = intperiod;
+    const double minstep = std::fabs(RelativeMinStep * intperiod);
+    std::vector<double> diag(kdim + 2, 0.0), offd(kdim + 2, 0.0);
+    // Kp(0) = psi, Kp(1) = dtpsi
+    cplx alpha = cdot(Kp(0), Kp(1));
+    diag[0] = (alpha / pref).real();
+    axpy2(Kp(1), alpha, Kp(0), cplx(0, 0), nullptr);
+    double beta = std::sqrt(cdot(Kp(1), Kp(1)).real());
+    offd[1] = beta;
+    cplx cbeta = pref * beta;
+    cplx cinv(0, 0);
+    if (beta < 1.0e-20) {
+      beta = 0.0;
+    } else {
+      cinv = 1.0 / cbeta;
+    }
+    scale(Kp(1), cinv);
+    double error = std::fabs(beta * stepsize);
+    int order = 1;
+    while (true) {
+      if (std::fabs(stepsize) < std::fabs(minstep)) throw Error(ADC2G_ESIL_UNDERFLOW, "Stepsize underflow");
+      ++order;
+      func(Kp(order - 1), Kp(order));
+      alpha = cdot(Kp(order - 1), Kp(order));
+      diag[order - 1] = (alpha / pref).real();
+      axpy2(Kp(order), alpha, Kp(order - 1), cbeta, Kp(order - 2));
+      beta = std::sqrt(cdot(Kp(order), Kp(order)).real());
+      offd[order] = beta;
+      cbeta = pref * beta;
+      if (beta < 1.0e-20) {
+        offd[order] = 0.0;
+        beta = 0.0;
+        cinv = cplx(0, 0);
+      } else {
+        cinv = 1.0 / cbeta;
+      }
+      scale(Kp(order), cinv);
+      error = std::fabs(error * beta * stepsize / (double)order);
+      if (!(order < kdim && error > tolerror && beta > 1.0e-20)) break;
+    }
+    diag[order] = diag[order - 1];
+    if (error > tolerror) stepsize = stepsize * std::pow(tolerror / error, 1.0 / (double)order);
+    // tridiagonal eigenpairs on the device (DSTEQR('I') of the reference)
+    const int np = order + 1;
+    std::vector<double> T((size_t)np * np, 0.0);
+    for (int i = 0; i < np; ++i) {
+      T[i + (size_t)i * np] = diag[i];
+      if (i + 1 < np) {
+        T[i + 1 + (size_t)i * np] = offd[i + 1];
+        T[i + (size_t)(i + 1) * np] = offd[i + 1];
+      }
+    }
+    HIP_CHECK(hipMemcpyAsync(tri.get(), T.data(), sizeof(double) * np * np, hipMemcpyHostToDevice, H.st));
+    launch_syev(np, tri.get(), np, lam.get(), U.get(), np, work.get(), info.get(), H.st);
+    hipLaunchKernelGGL(expcoef_kernel, dim3(1), dim3(64), 0, H.st, np, lam.get(), U.get(), np, stepsize, coef.get());
+    HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(combine_kernel, dim3(grid_for(n)), dim3(256), 0, H.st, n, np, coef.get(), K.get(), 2 * n,
+                       tmp.get());
+    HIP_CHECK(hipGetLastError());
+    int inf = 0;
+    HIP_CHECK(hipMemcpyAsync(&inf, info.get(), sizeof(int), hipMemcpyDeviceToHost, H.st));
+    HIP_CHECK(hipMemcpyAsync(Kp(0), tmp.get(), sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, H.st));
+    HIP_CHECK(hipStreamSynchronize(H.st));
+    if (inf != 0) throw Error(ADC2G_ESIL_DIAG, "Diagonalisation failed");
+    order_out = order;
+    return stepsize;
+  }
+};
+
+}  // namespace
+
+void sil_run(Handle& H, const adc2g_sil_params& p, const double* fvec, double* auto0, int64_t nrec_cap,
+             int64_t* nrec_out, adc2g_sil_stats* stats) {
+  const auto t0 = std::chrono::steady_clock::now();
+  ARG_CHECK(p.tout > 0 && p.tfinal >= 0, "tout must be > 0 and tfinal >= 0");
+  ARG_CHECK(p.kdim >= 1 && p.kdim <= 30, "krydim must be in 1..30");
+  ARG_CHECK(p.autotol > 0, "autotol must be > 0");
+  if (p.kdim < 2) throw Error(ADC2G_ESIL_ORDER, "IntOrder is too small");
+  const double intperiod = 0.5 * p.tout;
+  const int64_t nrec = (int64_t)(p.tfinal / intperiod);
+  ARG_CHECK(nrec_cap >= nrec + 1, "auto0 holds fewer than nrec+1 records");
+  Sil S(H, p.kdim, prec_from_int(p.precision));
+  const int64_t n = S.n;
+  // psi0 = F / |F| (complex, imaginary part zero)
+  std::vector<double> f(fvec, fvec + n);
+  double norm2 = 0.0;
+  for (int64_t i = 0; i < n; ++i) norm2 += f[i] * f[i];
+  ARG_CHECK(norm2 > 0.0, "initial vector is zero");
+  const double norm0 = std::sqrt(norm2);
+  for (auto& x : f) x /= norm0;
+  HIP_CHECK(hipMemcpyAsync(S.Kp(0), f.data(), sizeof(double) * n, hipMemcpyHostToDevice, H.st));
+  HIP_CHECK(hipMemsetAsync(S.Kp(0) + n, 0, sizeof(double) * n, H.st));
+  HIP_CHECK(hipStreamSynchronize(H.st));
+  // t = 0: dot_product(psi0, psi0) * |F|^2
+  const cplx a00 = S.cdot(S.Kp(0), S.Kp(0)) * norm2;
+  auto0[0] = a00.real();
+  auto0[1] = a00.imag();
+  const double tiny = 1e-9;
+  int64_t nsteps = 0;
+  int maxorder = 0;
+  for (int64_t i = 1; i <= nrec; ++i) {
+    double inttime = 0.0;
+    while (true) {
+      const double stepsize = intperiod - inttime;
+      S.func(S.Kp(0), S.Kp(1));
+      int order = 0;
+      const double truestep = S.step(stepsize, p.autotol, order);
+      ++nsteps;
+      maxorder = std::max(maxorder, order);
+      inttime += truestep;
+      if (!(std::fabs(intperiod - inttime) > std::fabs(tiny * intperiod))) break;
+    }
+    const cplx a = S.square_sum(S.Kp(0)) * norm2;
+    auto0[2 * i] = a.real();
+    auto0[2 * i + 1] = a.imag();
+  }
+  if (nrec_out) *nrec_out = nrec + 1;
+  if (stats) {
+    stats->nsteps = nsteps;
+    stats->nsigma = S.nsigma;
+    stats->max_order = maxorder;
+    stats->reserved = 0;
+    stats->time_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+}
+
+void matxvec_treal_host(Handle& H, const double* v1, double* v2, Prec p) {
+  const int64_t n = H.d.ndim;
+  std::vector<double> planar(2 * n);
+  for (int64_t i = 0; i < n; ++i) {
+    planar[i] = v1[2 * i];
+    planar[n + i] = v1[2 * i + 1];
+  }
+  DevBuf<double> dv, dh;
+  dv.alloc(2 * n);
+  dh.alloc(2 * n);
+  HIP_CHECK(hipMemcpyAsync(dv.get(), planar.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice, H.st));
+  H.sigma(dv.get(), n, dh.get(), n, 2, p, H.st);
+  HIP_CHECK(hipMemcpyAsync(planar.data(), dh.get(), sizeof(double) * 2 * n, hipMemcpyDeviceToHost, H.st));
+  HIP_CHECK(hipStreamSynchronize(H.st));
+  // -i (Hre + i Him) = Him - i Hre
+  for (int64_t i = 0; i < n; ++i) {
+    v2[2 * i] = planar[n + i];
+    v2[2 * i + 1] = -planar[i];
+  }
+}
+
+}  // namespace adc2g

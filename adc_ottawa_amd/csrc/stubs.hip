@@ -3,13 +3,6 @@
 
 namespace adc2g {
 
-void matxvec_treal_host(Handle& H, const double* v1, double* v2, Prec p) {
-  (void)H; (void)v1; (void)v2; (void)p;
-  throw Error(ADC2G_ESTATE, "not built yet");
-}
-void sil_run(Handle&, const adc2g_sil_params&, const double*, double*, int64_t, int64_t*, adc2g_sil_stats*) {
-  throw Error(ADC2G_ESTATE, "not built yet");
-}
 int precon_nparams() { return 0; }
 void precon_harvest(Handle&, const adc2g_dav_params&, int64_t, int64_t*) { throw Error(ADC2G_ESTATE, "not built yet"); }
 void precon_train(Handle&, const Handle&, int, int, double, uint64_t, double*) { throw Error(ADC2G_ESTATE, "not built yet"); }

@@ -102,7 +102,7 @@ def load(path: str = LIBPATH):
     lib.adc2g_precon_set.argtypes = [vp, vp]
     lib.adc2g_precon_eval.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
     lib.adc2g_last_error.argtypes = [C.c_char_p, i32]
-    lib.adc2g_sigma_profile.argtypes = [vp, i32, vp, vp, vp, vp]
+    lib.adc2g_sigma_profile.argtypes = [vp, i32, vp]
     for s in SYMBOLS:
         getattr(lib, s).restype = C.c_int32
     _lib = lib
@@ -212,6 +212,19 @@ class Operator:
                                       C.byref(n), C.byref(st)))
         stats = {f: getattr(st, f) for f, _ in SilStats._fields_}
         return out[: n.value], stats
+
+    def sigma_device(self, nvec, dX, dW, precision="fp64", stream=None):
+        """W = H X on device pointers (ints), asynchronous on ``stream``."""
+        check(_lib.adc2g_sigma_device(self._h, nvec, C.c_void_p(dX), C.c_void_p(dW), PREC[precision],
+                                      C.c_void_p(stream) if stream else None))
+
+    def profile(self, enable=True):
+        """Collect the per-kernel timing recorded so far and (re)arm it.
+        Returns {class: (ms, launches, work)} for coupling/pack/unpack."""
+        out = np.zeros(9)
+        check(_lib.adc2g_sigma_profile(self._h, 1 if enable else 0, _ptr(out)))
+        names = ("coupling_gemm", "csf_to_pair", "pair_to_csf_diag")
+        return {n: tuple(out[3 * i:3 * i + 3]) for i, n in enumerate(names)}
 
     # ---- learned preconditioner
     def precon_harvest(self, nstates, max_records=1 << 40, **kw):

@@ -203,12 +203,16 @@ int32_t adc2g_precon_eval(adc2g_handle* h, int64_t n, const float* feat, const i
 /* Copy the last error message (NUL-terminated, truncated to len). */
 int32_t adc2g_last_error(char* buf, int32_t len);
 
-/* Average device time (ms) per launch of each kernel class over the last
- * sigma-vector call, and the bytes/flops each class is credited with:
- * kinds 0 = coupling GEMMs, 1 = 2h2p diagonal + pair packing.  Used by the
- * benchmark; returns ADC2G_ESTATE before the first timed sigma. */
-int32_t adc2g_sigma_profile(adc2g_handle* h, int32_t enable, double* ms_gemm,
-                            double* ms_stream, double* flops_gemm, double* bytes_stream);
+/* Per-kernel device timing of sigma-vectors (no counterpart in the
+ * reference; its timers are wall clocks around whole solver calls,
+ * source/utilities/timingmod.f90:37-55).  enable = 1 starts recording HIP
+ * events around the coupling contractions (class 0), the CSF -> pair-tensor
+ * packing (class 1) and the pair -> CSF projection fused with the 2h2p
+ * diagonal (class 2) of every following sigma-vector.  If out is not NULL it
+ * receives, for classes 0..2, {milliseconds, launches, work} (9 doubles; work
+ * in algorithmic flops for class 0 and algorithmic HBM bytes for 1 and 2) of
+ * everything recorded since the previous call, and the records are reset. */
+int32_t adc2g_sigma_profile(adc2g_handle* h, int32_t enable, double* out);
 
 #ifdef __cplusplus
 }
