@@ -26,6 +26,7 @@ Prec prec_from_int(int p) {
 static size_t prec_size(Prec p) { return p == Prec::F64 ? 8 : (p == Prec::F32 ? 4 : 2); }
 
 Handle::~Handle() {
+  precon_net_free(*this);
   for (auto& e : ev_pool)
     if (e) (void)hipEventDestroy(e);
   if (st) (void)hipStreamDestroy(st);

@@ -62,5 +62,6 @@ struct Handle {
 };
 
 Prec prec_from_int(int p);
+void precon_net_free(Handle& H);
 
 }  // namespace adc2g
