@@ -54,8 +54,34 @@ def build(jobs: int = 8, verbose: bool = True) -> str:
     return LIB
 
 
+
+
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(HERE, "adc2g_driver")
+FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+
+
+def build_fortran(verbose: bool = True) -> str:
+    """Compile the iso_c_binding module and the Fortran driver against the
+    in-tree libadc2g.so (rpath to the package directory)."""
+    os.makedirs(BUILD, exist_ok=True)
+    mod = os.path.join(FORTRAN_DIR, "adc2g_mod.f90")
+    drv = os.path.join(FORTRAN_DIR, "adc2g_driver.f90")
+    cmd = [FC, "-O2", "-J", BUILD, "-o", DRIVER, mod, drv, "-L" + HERE, "-ladc2g",
+           "-Wl,-rpath," + HERE]
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=BUILD)
+    if r.returncode != 0:
+        raise RuntimeError(f"Fortran build failed:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {DRIVER}", file=sys.stderr)
+    return DRIVER
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=min(8, os.cpu_count() or 1))
+    ap.add_argument("--no-fortran", action="store_true")
     a = ap.parse_args()
     build(a.jobs)
+    if not a.no_fortran:
+        build_fortran()

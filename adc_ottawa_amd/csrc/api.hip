@@ -167,6 +167,14 @@ int32_t adc2g_precon_train(adc2g_handle* h, const adc2g_handle* data, int32_t ep
   });
 }
 
+int32_t adc2g_precon_absorb(adc2g_handle* dst, const adc2g_handle* src, int64_t* nrec) {
+  return guarded([&] {
+    ARG_CHECK(dst && src, "NULL argument");
+    HIP_CHECK(hipSetDevice(dst->h.device));
+    precon_absorb(dst->h, src->h, nrec);
+  });
+}
+
 int32_t adc2g_precon_nparams(void) { return precon_nparams(); }
 
 int32_t adc2g_precon_get(const adc2g_handle* h, float* params) {

@@ -33,6 +33,7 @@
 
 #include "linalg.hpp"
 #include "precon.hpp"
+#include "skinny.hpp"
 #include "solvers.hpp"
 
 namespace adc2g {
@@ -51,6 +52,7 @@ void gemm_tn(Handle& H, int M, int N, int64_t K, const double* A, int64_t lda, c
 
 void gemm_nn(Handle& H, int64_t M, int N, int K, const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
              int64_t ldc, double alpha, double beta) {
+  if (M > 4096 && skinny_nn(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, H.st)) return;
   GemmArgs g;
   g.M = (int)M; g.N = N; g.K = K;
   g.A = A; g.sam = 1; g.sak = lda;

@@ -30,6 +30,7 @@ struct Handle {
   DevBuf<char> ZG;          // Z then G, in the chunk's GEMM precision
   DevBuf<char> Wup_lp;      // Wup in low precision for the up GEMM
   GemmWorkspace gws;
+  DevBuf<double> skinny_ws;  // per-block partials of the tall-skinny products
   int chunk = 32;           // vectors per sigma pass
 
   // optional per-kernel device timing of sigma-vectors (benchmark): event

@@ -15,7 +15,7 @@ namespace adc2g {
 
 namespace {
 
-constexpr int SYEV_THREADS = 256;
+constexpr int SYEV_THREADS = 512;
 constexpr int SYEV_NMAX = 512;
 
 __device__ double block_sum(double x, double* sh) {
@@ -72,12 +72,11 @@ __device__ void syev_dev(int n, double* A, int lda, double* w, double* Z, int ld
     if (tid == 0) S.e[k] = alpha;
     __syncthreads();
     const double* v = A + (int64_t)k * lda;  // v[i] for i in k+1..n-1
-    // p = A22 v (column j of the symmetric trailing block)
-    for (int j = k + 1 + tid; j < n; j += nt) {
+    // p = A22 v, one row per lane (coalesced over the column-major storage)
+    for (int i = k + 1 + tid; i < n; i += nt) {
       double s = 0.0;
-      const double* col = A + (int64_t)j * lda;
-      for (int i = k + 1; i < n; ++i) s += col[i] * v[i];
-      Zw[j] = s;
+      for (int j = k + 1; j < n; ++j) s += A[i + (int64_t)j * lda] * v[j];
+      Zw[i] = s;
     }
     __syncthreads();
     part = 0.0;
@@ -165,12 +164,16 @@ __device__ void syev_dev(int n, double* A, int lda, double* w, double* Z, int ld
       __syncthreads();  // everyone has read the control words before lane 0 rewrites them
       if (!go) break;
       for (int r = tid; r < n; r += nt) {
+        // the element of column i+1 is carried in a register down the sweep,
+        // so the loads of column i do not wait on the previous store
+        double carry = Zw[r + (int64_t)(hi + 1) * n];
         for (int i = hi; i >= lo; --i) {
           const double c = S.rc[i], s = S.rs[i];
-          const double zi = Zw[r + (int64_t)i * n], zi1 = Zw[r + (int64_t)(i + 1) * n];
-          Zw[r + (int64_t)(i + 1) * n] = s * zi + c * zi1;
-          Zw[r + (int64_t)i * n] = c * zi - s * zi1;
+          const double zi = Zw[r + (int64_t)i * n];
+          Zw[r + (int64_t)(i + 1) * n] = s * zi + c * carry;
+          carry = c * zi - s * carry;
         }
+        Zw[r + (int64_t)lo * n] = carry;
       }
       __syncthreads();
     }
@@ -182,18 +185,24 @@ __device__ void syev_dev(int n, double* A, int lda, double* w, double* Z, int ld
     __syncthreads();
     return;
   }
-  // ---- back transformation: Zw <- H_0 ... H_{n-3} Zw (one column per lane)
-  for (int j = tid; j < n; j += nt) {
-    double* z = Zw + (int64_t)j * n;
-    for (int k = n - 3; k >= 0; --k) {
-      const double* v = A + (int64_t)k * lda;
-      double t = 0.0;
-      for (int i = k + 1; i < n; ++i) t += v[i] * z[i];
-      t *= 2.0;
-      if (t != 0.0)
-        for (int i = k + 1; i < n; ++i) z[i] -= t * v[i];
+  // ---- back transformation: Zw <- H_0 ... H_{n-3} Zw (one column per wave,
+  // rows split over the lanes)
+  {
+    const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
+    for (int j = wid; j < n; j += nw) {
+      double* z = Zw + (int64_t)j * n;
+      for (int k = n - 3; k >= 0; --k) {
+        const double* v = A + (int64_t)k * lda;
+        double t = 0.0;
+        for (int i = k + 1 + lane; i < n; i += 64) t += v[i] * z[i];
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        t *= 2.0;
+        if (t != 0.0)
+          for (int i = k + 1 + lane; i < n; i += 64) z[i] -= t * v[i];
+      }
     }
   }
+  __syncthreads();
   // ---- ascending order
   if (tid == 0) {
     for (int i = 0; i < n; ++i) S.perm[i] = i;

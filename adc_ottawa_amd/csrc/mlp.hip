@@ -12,7 +12,7 @@
 //   t_j = d_j (1 + m),  d_j = r_j / (theta_i - H_jj)  (the DPR step)
 // Training minimises mean (dn (1 + m) - en)^2 over harvested records, dn and
 // en being the DPR step and the exact correction x - x*/(x.x*) towards the
-// converged eigenvector x*, both divided by |x - x*/(x.x*)|.
+// converged eigenvector x*, both divided by the RMS element of x - x*/(x.x*).
 //
 // Weights are kept as fp32 master copies (Adam) and used rounded to bf16.
 // Training runs one 64-lane wave per record stream in fp32 with the bf16
@@ -324,11 +324,12 @@ struct SampleArgs {
   const double* theta;   // per root (device)
   const double* rnorm;   // per root
   const double* s;       // x . x*
-  const double* enorm;   // |x - x*/s|
+  const double* enorm;   // RMS element of x - x*/s
   const int* active;     // root sampled?
   const double* hii;
   const double* X;
   const double* Xs;      // converged eigenvectors (ld)
+  const int* match;      // column of Xs for each root
   const double* Res;
   float* feat;
   int* cls;
@@ -353,7 +354,7 @@ __global__ void sample_kernel(SampleArgs a) {
     const double th = a.theta[r];
     const double rj = a.Res[j + a.ld * r];
     const double dj = rj / guard_den2(th - a.hii[j]);
-    const double ej = (a.X[j + a.ld * r] - a.Xs[j + a.ld * r] / a.s[r]) / a.enorm[r];
+    const double ej = (a.X[j + a.ld * r] - a.Xs[j + a.ld * a.match[r]] / a.s[r]) / a.enorm[r];
     a.feat[3 * out] = asinhf((float)(a.hii[j] - th));
     a.feat[3 * out + 1] = (float)th;
     a.feat[3 * out + 2] = asinhf((float)(rj / a.rnorm[r]));
@@ -416,8 +417,8 @@ struct DavTrace {
   uint64_t seed = 1;
   double tol = 1e-7;
   DevBuf<double> dbuf;       // theta, rnorm, s, enorm per root
-  DevBuf<int> active;
-  DevBuf<double> part;
+  DevBuf<int> active, match;
+  DevBuf<double> part, ovl;
 };
 
 static PreconNet& net_of(Handle& H) {
@@ -448,6 +449,21 @@ void precon_set(Handle& H, const float* params) {
   HIP_CHECK(hipStreamSynchronize(H.st));
   N.adam_t = 0;
   N.have = true;
+}
+
+void precon_absorb(Handle& dst, const Handle& src, int64_t* nrec) {
+  PreconNet& D = net_of(dst);
+  const PreconNet* S = src.net;
+  const int64_t m = S ? S->nrec : 0;
+  if (m > 0) {
+    D.reserve(D.nrec + m);
+    HIP_CHECK(hipMemcpy(D.feat.get() + 3 * D.nrec, S->feat.get(), sizeof(float) * 3 * m, hipMemcpyDeviceToDevice));
+    HIP_CHECK(hipMemcpy(D.dn.get() + D.nrec, S->dn.get(), sizeof(float) * m, hipMemcpyDeviceToDevice));
+    HIP_CHECK(hipMemcpy(D.en.get() + D.nrec, S->en.get(), sizeof(float) * m, hipMemcpyDeviceToDevice));
+    HIP_CHECK(hipMemcpy(D.cls.get() + D.nrec, S->cls.get(), sizeof(int) * m, hipMemcpyDeviceToDevice));
+    D.nrec += m;
+  }
+  if (nrec) *nrec = D.nrec;
 }
 
 void precon_get(const Handle& H, float* params) {
@@ -569,39 +585,51 @@ void precon_learned_step(Handle& H, int64_t n, int nc, const double* theta_d, co
 
 void davtrace_iteration(Handle& H, DavTrace& tr, int it, int bs_curr, int ns_curr, const double* theta_h,
                         const double* norm_h, const double* X, const double* Res, int64_t n) {
-  (void)it;
-  if (tr.added >= tr.max_records) return;
-  const int nr = std::min(ns_curr, tr.nstates);
   (void)bs_curr;
+  if (tr.added >= tr.max_records) return;
+  const int nr = ns_curr;  // active (not yet deflated) roots
   if (nr <= 0) return;
   PreconNet& N = net_of(H);
+  const int ns = tr.nstates;
   tr.dbuf.alloc(4 * (size_t)nr);
   tr.active.alloc(nr);
-  const int nb = reduce_blocks(n);
-  tr.part.alloc((size_t)nb * nr);
-  std::vector<double> h(4 * (size_t)nr);
-  std::vector<int> act(nr);
-  // s_i = x_i . x*_i and |x_i - x*_i / s_i|^2 = |x|^2 - 2 (x.x*)/s + |x*|^2/s^2 = 1/s^2 - 1 for unit vectors
-  hipLaunchKernelGGL(coldot_kernel, dim3(nb), dim3(256), 0, H.st, n, nr, X, n, tr.Xs.get(), n, tr.part.get());
-  HIP_CHECK(hipGetLastError());
-  launch_colsum(nb, nr, tr.part.get(), tr.dbuf.get() + 2 * nr, H.st);
-  std::vector<double> sdot(nr);
-  HIP_CHECK(hipMemcpyAsync(sdot.data(), tr.dbuf.get() + 2 * nr, sizeof(double) * nr, hipMemcpyDeviceToHost, H.st));
+  tr.match.alloc(nr);
+  tr.ovl.alloc((size_t)nr * ns);
+  // overlaps O = X^T X* (nr x ns): each active root is matched to the
+  // converged eigenvector it overlaps most (deflation reorders the roots)
+  {
+    GemmArgs g;
+    g.M = nr; g.N = ns; g.K = (int)n;
+    g.A = X; g.sam = n; g.sak = 1;
+    g.B = tr.Xs.get(); g.sbk = 1; g.sbn = n;
+    g.map.C = tr.ovl.get(); g.map.scm = 1; g.map.scn = nr;
+    gemm(g, Prec::F64, H.gws, H.st);
+  }
+  std::vector<double> O((size_t)nr * ns);
+  HIP_CHECK(hipMemcpyAsync(O.data(), tr.ovl.get(), sizeof(double) * O.size(), hipMemcpyDeviceToHost, H.st));
   HIP_CHECK(hipStreamSynchronize(H.st));
+  std::vector<double> h(4 * (size_t)nr);
+  std::vector<int> act(nr), mt(nr);
   int nact = 0;
   for (int i = 0; i < nr; ++i) {
-    const double s = sdot[i];
+    int best = 0;
+    for (int j = 1; j < ns; ++j)
+      if (std::fabs(O[i + (size_t)j * nr]) > std::fabs(O[i + (size_t)best * nr])) best = j;
+    const double s = O[i + (size_t)best * nr];
+    mt[i] = best;
+    // |x - x*/s|^2 = 1/s^2 - 1 for unit x, x*; targets are divided by its RMS
     const double e2 = (s != 0.0) ? (1.0 / (s * s) - 1.0) : 0.0;
-    act[i] = (norm_h[i] >= tr.tol && std::fabs(s) > 1e-3 && e2 > 1e-24) ? 1 : 0;
+    act[i] = (norm_h[i] >= tr.tol && std::fabs(s) > 0.5 && e2 > 1e-24) ? 1 : 0;
     nact += act[i];
     h[i] = theta_h[i];
     h[nr + i] = norm_h[i];
     h[2 * nr + i] = s;
-    h[3 * nr + i] = std::sqrt(std::max(e2, 1e-300));
+    h[3 * nr + i] = std::sqrt(std::max(e2, 1e-300) / (double)n);
   }
   if (nact == 0) return;
   HIP_CHECK(hipMemcpyAsync(tr.dbuf.get(), h.data(), sizeof(double) * 4 * nr, hipMemcpyHostToDevice, H.st));
   HIP_CHECK(hipMemcpyAsync(tr.active.get(), act.data(), sizeof(int) * nr, hipMemcpyHostToDevice, H.st));
+  HIP_CHECK(hipMemcpyAsync(tr.match.get(), mt.data(), sizeof(int) * nr, hipMemcpyHostToDevice, H.st));
   const int64_t cnt = (int64_t)nr * tr.per_root;
   N.reserve(N.nrec + cnt);
   SampleArgs a;
@@ -610,7 +638,7 @@ void davtrace_iteration(Handle& H, DavTrace& tr, int it, int bs_curr, int ns_cur
   a.seed = tr.seed + (uint64_t)it * 0x9E3779B97F4A7C15ull;
   a.theta = tr.dbuf.get(); a.rnorm = tr.dbuf.get() + nr; a.s = tr.dbuf.get() + 2 * nr;
   a.enorm = tr.dbuf.get() + 3 * nr; a.active = tr.active.get();
-  a.hii = H.hii.get(); a.X = X; a.Xs = tr.Xs.get(); a.Res = Res;
+  a.hii = H.hii.get(); a.X = X; a.Xs = tr.Xs.get(); a.match = tr.match.get(); a.Res = Res;
   a.feat = N.feat.get(); a.cls = N.cls.get(); a.dn = N.dn.get(); a.en = N.en.get();
   a.base = N.nrec;
   hipLaunchKernelGGL(sample_kernel, dim3(grid_for(cnt)), dim3(256), 0, H.st, a);
@@ -646,10 +674,10 @@ void davtrace_iteration(Handle& H, DavTrace& tr, int it, int bs_curr, int ns_cur
 }
 
 void precon_harvest(Handle& H, const adc2g_dav_params& p, int64_t max_records, int64_t* added) {
-  // pass 1: FP64 DPR Davidson without deflation to convergence -> x*
+  // pass 1: FP64 DPR Davidson (with deflation) to convergence -> x*
   adc2g_dav_params q = p;
   q.precon = ADC2G_PRE_DPR;
-  q.deflate = 0;
+  q.deflate = 1;
   q.precision = ADC2G_PREC_FP64;
   q.verbose = 0;
   const int64_t n = H.d.ndim;

@@ -20,6 +20,7 @@ int precon_nparams();
 void precon_harvest(Handle& H, const adc2g_dav_params& p, int64_t max_records, int64_t* added);
 void precon_train(Handle& H, const Handle& data, int epochs, int batch, double lr, uint64_t seed, double* loss_out);
 void precon_get(const Handle& H, float* params);
+void precon_absorb(Handle& dst, const Handle& src, int64_t* nrec);
 void precon_set(Handle& H, const float* params);
 void precon_eval(Handle& H, int64_t n, const float* feat, const int32_t* cls, const float* dn, const float* en,
                  float* m_out, float* grad_out, double* loss_out);

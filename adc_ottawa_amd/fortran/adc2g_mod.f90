@@ -1,0 +1,195 @@
+!=======================================================================
+! adc2g_mod: iso_c_binding interface to libadc2g (include/adc2g.h).
+!
+! The Fortran side of the drop-in.  A caller that today does
+!   call write_fspace_adc2_1(ndim,kpq,noffdel,'i')   ! fspacetrial.f90:533
+!   call master_eig(ndim,noffdel,'i')                 ! adc_common.f90:952
+! builds an operator handle from its orbital energies and factors instead,
+! runs adc2g_davidson, and writes the eigenpair file with
+! adc2g_write_davstates in the record layout of wreigenpairs
+! (source/eigen/block_davidson.f90:1515-1523), so readdavvc
+! (source/adclib/adc_common.f90:999-1052) reads it unchanged.  Errors are
+! turned into a message for error_control (source/iomodules/iomod.f90:94-112)
+! through adc2g_errmsg.
+!=======================================================================
+module adc2g_mod
+
+  use iso_c_binding
+  implicit none
+
+  integer(c_int32_t), parameter :: ADC2G_OK = 0
+  integer(c_int32_t), parameter :: ADC2G_PREC_FP64 = 0, ADC2G_PREC_FP32 = 1, ADC2G_PREC_BF16 = 2
+  integer(c_int32_t), parameter :: ADC2G_PRE_DPR = 1, ADC2G_PRE_OLSEN = 2, ADC2G_PRE_LEARNED = 3
+  real(c_double), parameter :: adc2g_eh2ev = 27.2113845d0       ! parameters.f90:192
+  real(c_double), parameter :: adc2g_au2fs = 1.0d0/41.34137333656d0 ! fvecprop.f90:19
+
+  type, bind(c) :: adc2g_opts
+     integer(c_int32_t) :: device = 0
+     integer(c_int32_t) :: build_phph = 1
+     integer(c_int32_t) :: keep_lowprec = 0
+     integer(c_int32_t) :: reserved = 0
+  end type adc2g_opts
+
+  type, bind(c) :: adc2g_dav_params
+     integer(c_int32_t) :: nstates
+     integer(c_int32_t) :: blocksize = 0
+     integer(c_int32_t) :: maxvec = -1
+     integer(c_int32_t) :: maxiter = 100
+     real(c_double)     :: tol = 1.0d-7
+     integer(c_int32_t) :: precon = 2
+     integer(c_int32_t) :: deflate = 1
+     integer(c_int32_t) :: precision = 0
+     integer(c_int32_t) :: refine = 1
+     real(c_double)     :: lowprec_tol = 1.0d-4
+     integer(c_int32_t) :: guess = 0
+     integer(c_int32_t) :: verbose = 0
+  end type adc2g_dav_params
+
+  type, bind(c) :: adc2g_dav_stats
+     integer(c_int32_t) :: niter, nsigma
+     integer(c_int64_t) :: nmult_ref
+     integer(c_int32_t) :: niter_lowprec, nfallback
+     real(c_double)     :: time_total, time_sigma, resnorm_max
+  end type adc2g_dav_stats
+
+  type, bind(c) :: adc2g_sil_params
+     real(c_double)     :: tout, tfinal
+     integer(c_int32_t) :: kdim = 10
+     integer(c_int32_t) :: precision = 0
+     real(c_double)     :: autotol = 1.0d-5
+  end type adc2g_sil_params
+
+  type, bind(c) :: adc2g_sil_stats
+     integer(c_int64_t) :: nsteps, nsigma
+     integer(c_int32_t) :: max_order, reserved
+     real(c_double)     :: time_total
+  end type adc2g_sil_stats
+
+  interface
+     integer(c_int32_t) function adc2g_create(nocc, nvirt, naux, eps, B, opts, h) bind(c, name='adc2g_create')
+       import :: c_int32_t, c_double, c_ptr, adc2g_opts
+       integer(c_int32_t), value :: nocc, nvirt, naux
+       real(c_double), intent(in) :: eps(*), B(*)
+       type(adc2g_opts), intent(in) :: opts
+       type(c_ptr), intent(out) :: h
+     end function adc2g_create
+
+     integer(c_int32_t) function adc2g_destroy(h) bind(c, name='adc2g_destroy')
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: h
+     end function adc2g_destroy
+
+     integer(c_int32_t) function adc2g_dims(h, ndim, n1, counts) bind(c, name='adc2g_dims')
+       import :: c_int32_t, c_int64_t, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int64_t), intent(out) :: ndim, n1, counts(5)
+     end function adc2g_dims
+
+     integer(c_int32_t) function adc2g_diagonal(h, hii) bind(c, name='adc2g_diagonal')
+       import :: c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: hii(*)
+     end function adc2g_diagonal
+
+     integer(c_int32_t) function adc2g_sigma(h, nvec, X, W, precision) bind(c, name='adc2g_sigma')
+       import :: c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: nvec, precision
+       real(c_double), intent(in) :: X(*)
+       real(c_double), intent(out) :: W(*)
+     end function adc2g_sigma
+
+     integer(c_int32_t) function adc2g_matxvec_treal(h, v1, v2, precision) bind(c, name='adc2g_matxvec_treal')
+       import :: c_int32_t, c_double_complex, c_ptr
+       type(c_ptr), value :: h
+       complex(c_double_complex), intent(in) :: v1(*)
+       complex(c_double_complex), intent(out) :: v2(*)
+       integer(c_int32_t), value :: precision
+     end function adc2g_matxvec_treal
+
+     integer(c_int32_t) function adc2g_davidson(h, p, eigval, eigvec, stats) bind(c, name='adc2g_davidson')
+       import :: c_int32_t, c_double, c_ptr, adc2g_dav_params, adc2g_dav_stats
+       type(c_ptr), value :: h
+       type(adc2g_dav_params), intent(in) :: p
+       real(c_double), intent(out) :: eigval(*), eigvec(*)
+       type(adc2g_dav_stats), intent(out) :: stats
+     end function adc2g_davidson
+
+     integer(c_int32_t) function adc2g_sil_autocorr(h, p, fvec, auto0, nrec_cap, nrec_out, stats) &
+          bind(c, name='adc2g_sil_autocorr')
+       import :: c_int32_t, c_int64_t, c_double, c_double_complex, c_ptr, adc2g_sil_params, adc2g_sil_stats
+       type(c_ptr), value :: h
+       type(adc2g_sil_params), intent(in) :: p
+       real(c_double), intent(in) :: fvec(*)
+       complex(c_double_complex), intent(out) :: auto0(*)
+       integer(c_int64_t), value :: nrec_cap
+       integer(c_int64_t), intent(out) :: nrec_out
+       type(adc2g_sil_stats), intent(out) :: stats
+     end function adc2g_sil_autocorr
+
+     integer(c_int32_t) function adc2g_precon_nparams() bind(c, name='adc2g_precon_nparams')
+       import :: c_int32_t
+     end function adc2g_precon_nparams
+
+     integer(c_int32_t) function adc2g_precon_set(h, params) bind(c, name='adc2g_precon_set')
+       import :: c_int32_t, c_float, c_ptr
+       type(c_ptr), value :: h
+       real(c_float), intent(in) :: params(*)
+     end function adc2g_precon_set
+
+     integer(c_int32_t) function adc2g_last_error(buf, length) bind(c, name='adc2g_last_error')
+       import :: c_int32_t, c_char
+       character(kind=c_char), intent(out) :: buf(*)
+       integer(c_int32_t), value :: length
+     end function adc2g_last_error
+  end interface
+
+contains
+
+  ! Last error as a Fortran string (for errmsg / error_control).
+  function adc2g_errmsg() result(msg)
+    character(len=256) :: msg
+    character(kind=c_char) :: buf(256)
+    integer :: i, rc
+    rc = adc2g_last_error(buf, 256)
+    msg = ' '
+    do i = 1, 256
+       if (buf(i) == c_null_char) exit
+       msg(i:i) = buf(i)
+    end do
+  end function adc2g_errmsg
+
+  ! Eigenpair file in the record layout of wreigenpairs: one unformatted
+  ! sequential record (i, E_i, vec_i(1:ndim)) per state.
+  subroutine adc2g_write_davstates(fname, ndim, nstates, eigval, eigvec)
+    character(len=*), intent(in) :: fname
+    integer(c_int64_t), intent(in) :: ndim
+    integer, intent(in) :: nstates
+    real(c_double), intent(in) :: eigval(nstates), eigvec(ndim, nstates)
+    integer :: unit, i
+    open(newunit=unit, file=fname, status='unknown', access='sequential', form='unformatted')
+    do i = 1, nstates
+       write(unit) i, eigval(i), eigvec(:, i)
+    end do
+    close(unit)
+  end subroutine adc2g_write_davstates
+
+  ! Autocorrelation file in the format of open_autofiles/wrauto
+  ! (source/propagation/fvecprop.f90:281-304, 585-598).
+  subroutine adc2g_write_auto(fname, nrec, tout, auto0)
+    character(len=*), intent(in) :: fname
+    integer(c_int64_t), intent(in) :: nrec
+    real(c_double), intent(in) :: tout
+    complex(c_double_complex), intent(in) :: auto0(nrec)
+    integer :: unit
+    integer(c_int64_t) :: i
+    open(newunit=unit, file=fname, form='formatted', status='unknown')
+    write(unit, '(a)') '#    time[fs]         Re(autocorrel)         Im(autocorrel)         Abs(autocorrel)'
+    do i = 1, nrec
+       write(unit, '(F15.8,4x,3(2x,E21.14))') real(i - 1, c_double) * tout * adc2g_au2fs, &
+            real(auto0(i)), aimag(auto0(i)), abs(auto0(i))
+    end do
+    close(unit)
+  end subroutine adc2g_write_auto
+
+end module adc2g_mod

@@ -66,7 +66,7 @@ SYMBOLS = [
     "adc2g_sigma", "adc2g_sigma_device", "adc2g_matxvec_treal", "adc2g_davidson",
     "adc2g_sil_autocorr", "adc2g_precon_harvest", "adc2g_precon_train", "adc2g_precon_nparams",
     "adc2g_precon_get", "adc2g_precon_set", "adc2g_precon_eval", "adc2g_last_error",
-    "adc2g_sigma_profile",
+    "adc2g_sigma_profile", "adc2g_precon_absorb",
 ]
 
 _lib = None
@@ -103,6 +103,7 @@ def load(path: str = LIBPATH):
     lib.adc2g_precon_eval.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
     lib.adc2g_last_error.argtypes = [C.c_char_p, i32]
     lib.adc2g_sigma_profile.argtypes = [vp, i32, vp]
+    lib.adc2g_precon_absorb.argtypes = [vp, vp, C.POINTER(i64)]
     for s in SYMBOLS:
         getattr(lib, s).restype = C.c_int32
     _lib = lib
@@ -233,6 +234,12 @@ class Operator:
         added = C.c_int64()
         check(_lib.adc2g_precon_harvest(self._h, C.byref(p), max_records, C.byref(added)))
         return added.value
+
+    def absorb(self, other):
+        """Append other's harvested records to this handle's; returns the count."""
+        n = C.c_int64()
+        check(_lib.adc2g_precon_absorb(self._h, other.handle, C.byref(n)))
+        return n.value
 
     def precon_train(self, data=None, epochs=10, batch=8192, lr=1e-3, seed=0):
         data = self if data is None else data

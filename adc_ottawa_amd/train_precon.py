@@ -1,0 +1,97 @@
+"""Train the learned Davidson preconditioner on the GPU.
+
+    python -m adc_ottawa_amd.train_precon [--systems 40] [--epochs 20] [--out weights.npy]
+
+1. Harvest: for each seeded synthetic system (o, v drawn from the given
+   ranges, naux = 4 (o + v), seeds 1000, 1001, ...) run the FP64 Davidson with
+   the diagonal (DPR) step and deflation twice -- once to convergence for the
+   eigenvectors x*, once replaying the same trajectory -- and keep, for every
+   unconverged root (matched to the x* it overlaps most) and iteration, 16
+   sampled elements (half 1h1p, half 2h2p) of (features, DPR step, exact
+   correction x - x*/(x.x*)), divided by the RMS element of the correction.
+   Records stay in HBM on one data handle.
+2. Train: Adam on mini-batches with the library's own forward/backward
+   kernels (adc2g_precon_train).
+3. Save the fp32 master weights (4865 values) as .npy.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+from adc_ottawa_amd import lib  # noqa: E402
+from adc_ottawa_amd.synth import make_system  # noqa: E402
+
+DEFAULT_OUT = os.path.join(HERE, "precon_weights.npy")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def harvest(nsys, orange, vrange, nroots, seed0=1000, max_per_system=60000):
+    rng = np.random.default_rng(12345)
+    data = None
+    total = 0
+    for i in range(nsys):
+        o = int(rng.integers(orange[0], orange[1] + 1))
+        v = int(rng.integers(vrange[0], vrange[1] + 1))
+        seed = seed0 + i
+        s = make_system(o, v, 4 * (o + v), seed=seed)
+        op = lib.Operator.from_system(s)
+        t0 = time.perf_counter()
+        try:
+            added = op.precon_harvest(nroots, max_records=max_per_system, blocksize=nroots,
+                                      maxvec=10 * nroots, tol=1e-7, maxiter=600)
+        except lib.Adc2gError as e:
+            log(f"system {i} (o={o} v={v} seed={seed}) skipped: {e}")
+            op.close()
+            continue
+        total += added
+        log(f"system {i}: o={o} v={v} ndim={op.ndim} seed={seed}: {added} records "
+            f"({time.perf_counter() - t0:.1f}s), total {total}")
+        if data is None:
+            data = op
+        else:
+            # move this system's records onto the data handle: train from each
+            # handle in turn is equivalent, but one pool keeps batches mixed
+            data.absorb(op)
+            op.close()
+    return data, total
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--systems", type=int, default=40)
+    ap.add_argument("--omin", type=int, default=8)
+    ap.add_argument("--omax", type=int, default=12)
+    ap.add_argument("--vmin", type=int, default=80)
+    ap.add_argument("--vmax", type=int, default=120)
+    ap.add_argument("--roots", type=int, default=10)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--out", default=DEFAULT_OUT)
+    a = ap.parse_args()
+    t0 = time.perf_counter()
+    data, total = harvest(a.systems, (a.omin, a.omax), (a.vmin, a.vmax), a.roots)
+    log(f"harvested {total} records in {time.perf_counter() - t0:.1f}s")
+    t1 = time.perf_counter()
+    loss = data.precon_train(epochs=a.epochs, batch=a.batch, lr=a.lr, seed=7)
+    log(f"trained {a.epochs} epochs in {time.perf_counter() - t1:.1f}s; loss per epoch: "
+        + " ".join(f"{x:.4f}" for x in loss))
+    w = data.precon_params()
+    np.save(a.out, w)
+    log(f"saved {w.size} parameters to {a.out}")
+    data.close()
+
+
+if __name__ == "__main__":
+    main()

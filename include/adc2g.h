@@ -184,6 +184,10 @@ int32_t adc2g_precon_harvest(adc2g_handle* h, const adc2g_dav_params* p, int64_t
 int32_t adc2g_precon_train(adc2g_handle* h, const adc2g_handle* data, int32_t epochs,
                            int32_t batch, double lr, uint64_t seed, double* loss_out);
 
+/* Append the training records harvested on src to those of dst (both stay
+ * valid); *nrec (may be NULL) receives dst's record count afterwards. */
+int32_t adc2g_precon_absorb(adc2g_handle* dst, const adc2g_handle* src, int64_t* nrec);
+
 /* Copy the network to/from host memory (flat float32 array of
  * adc2g_precon_nparams() values in the order W1, E, b1, W2, b2, w3, b3). */
 int32_t adc2g_precon_nparams(void);

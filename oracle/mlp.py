@@ -14,8 +14,9 @@ Network (per vector element j of root i):
   correction t_j = d_j * (1 + m), d_j = r_j / (theta_i - H_jj) (the DPR step)
 with the input transform g' = asinh(g), theta' = theta, rhat' = asinh(rhat).
 
-Training pairs (per element): features, dn_j = d_j/|e|, en_j = e_j/|e| with
-e = x - x*/(x.x*) the exact correction towards the converged eigenvector x*;
+Training pairs (per element): features, dn_j = d_j/rms(e), en_j = e_j/rms(e)
+with e = x - x*/(x.x*) the exact correction towards the converged eigenvector
+x* and rms(e) its root-mean-square element;
 loss = mean_j (dn_j (1 + m_j) - en_j)^2.
 
 Weights are stored in bfloat16 and every product accumulates in fp32 on the
