@@ -337,6 +337,10 @@ void launch_pack_x1(const double* X, int64_t ldx, int nvec, const DimsK& d, T* X
 
 template <class T>
 void launch_csf_to_pair(const double* X, int64_t ldx, int nvec, const DimsK& d, T* Z, hipStream_t st) {
+  if (d.o <= 64) {
+    launch_pack_tiled<T>(X, ldx, nvec, d, Z, st);
+    return;
+  }
   hipLaunchKernelGGL((csf_to_pair_kernel<T>), dim3(grid_for(d.n1 * d.n1 * nvec)), dim3(256), 0, st, X, ldx,
                      nvec, d, Z);
   HIP_CHECK(hipGetLastError());
@@ -345,6 +349,10 @@ void launch_csf_to_pair(const double* X, int64_t ldx, int nvec, const DimsK& d, 
 template <class T>
 void launch_pair_to_csf(const T* G, const double* X, int64_t ldx, double* W, int64_t ldw, int nvec,
                         const DimsK& d, const double* eps, hipStream_t st) {
+  if (d.o <= 64) {
+    launch_unpack_tiled<T>(G, X, ldx, W, ldw, nvec, d, eps, st);
+    return;
+  }
   const int64_t units = (d.off5 - d.off1) * nvec;
   hipLaunchKernelGGL((pair_to_csf_kernel<T>), dim3(grid_for(units)), dim3(256), 0, st, G, X, ldx, W, ldw, nvec,
                      d, eps);

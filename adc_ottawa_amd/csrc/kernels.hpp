@@ -38,6 +38,12 @@ void launch_csf_to_pair(const double* X, int64_t ldx, int nvec, const DimsK& d, 
 template <class T>
 void launch_pair_to_csf(const T* G, const double* X, int64_t ldx, double* W, int64_t ldw, int nvec,
                         const DimsK& d, const double* eps, hipStream_t st);
+// Tiled versions of the two maps above (pairs.hip), used for nocc <= 64.
+template <class T>
+void launch_pack_tiled(const double* X, int64_t ldx, int nvec, const DimsK& d, T* Z, hipStream_t st);
+template <class T>
+void launch_unpack_tiled(const T* G, const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, const DimsK& d,
+                         const double* eps, hipStream_t st);
 // hii[j] for the whole vector: diag(H11) and the 2h2p orbital-energy sums
 void launch_diagonal(const double* H11, const double* eps, const DimsK& d, double* hii, hipStream_t st);
 
