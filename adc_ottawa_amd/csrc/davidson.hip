@@ -15,10 +15,15 @@
 //  * W is kept: only the new block gets a sigma-vector (the reference
 //    recomputes W for all currdim columns each iteration, nmult += currdim;
 //    that count is reported as nmult_ref).  A collapse keeps W Y.
-//  * The new block is orthonormalised against V by two block classical
-//    Gram-Schmidt passes and within itself through the eigen-decomposition of
-//    its Gram matrix, instead of a Householder QR of the whole subspace
-//    (:1423-1465); numerically dependent directions are dropped.
+//  * The new block is orthonormalised against V (and the locked vectors) by
+//    block classical Gram-Schmidt -- a second pass only when the DGKS test
+//    finds that a column lost more than half its squared norm -- and within
+//    itself through the eigen-decomposition of its Gram matrix (repeated when
+//    that matrix is ill-conditioned), instead of a Householder QR of the
+//    whole subspace (:1423-1465); numerically dependent directions are
+//    dropped.
+//  * Rayleigh-Ritz computes only the bs_curr lowest eigenpairs of the
+//    subspace matrix (the reference's dsyev computes all of them).
 //  * Preconditioner denominators below 1e-10 in magnitude are clamped (the
 //    reference divides unguarded).
 // Mixed precision: with precision != FP64 the first phase uses low-precision
@@ -42,6 +47,7 @@ namespace {
 
 void gemm_tn(Handle& H, int M, int N, int64_t K, const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
              int64_t ldc, double alpha, double beta) {
+  if (K > 4096 && skinny_tn(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, H.skinny_ws, H.st)) return;
   GemmArgs g;
   g.M = M; g.N = N; g.K = (int)K;
   g.A = A; g.sam = lda; g.sak = 1;
@@ -65,12 +71,13 @@ struct Dav {
   Handle& H;
   int64_t n;
   int bs, mv;
-  DevBuf<double> V, W, X, WY, Res, T, Rm, Rt, Zs, thd, part, nrm, small, work, M;
+  DevBuf<double> V, W, X, WY, Res, T, Rm, Rt, Zs, thd, part, nrm, small, work, M, S1, S2, Gm, cond;
   DevBuf<int> info;
   DevBuf<int64_t> idx;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   double t_sigma = 0;
   int nsigma = 0;
+  int n_second_pass = 0;
 
   explicit Dav(Handle& h) : H(h) {}
   ~Dav() {
@@ -95,6 +102,10 @@ struct Dav {
     small.alloc((size_t)mv * std::max(mv, bs) + 16);
     work.alloc((size_t)4 * mv * mv + 4 * mv);
     M.alloc((size_t)bs * bs);
+    S1.alloc((size_t)mv * bs);
+    S2.alloc((size_t)bs * bs);
+    Gm.alloc((size_t)bs * bs);
+    cond.alloc(1);
     info.alloc(4);
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
@@ -117,36 +128,58 @@ struct Dav {
     launch_symmetrize_cols(c0, c1, Rm.get(), mv, H.st);
   }
 
-  // Remove span(B[:, :nb]) from T[:, :r] (two classical Gram-Schmidt passes).
-  void project(const double* B, int nb, double* Tp, int r) {
-    if (nb <= 0 || r <= 0) return;
-    for (int pass = 0; pass < 2; ++pass) {
-      gemm_tn(H, nb, r, n, B, n, Tp, n, small.get(), nb, 1.0, 0.0);
-      gemm_nn(H, n, r, nb, B, n, small.get(), nb, Tp, n, -1.0, 1.0);
-    }
+  // One block classical Gram-Schmidt pass of T[:, :r] against the orthonormal
+  // columns of V1[:, :nb1] and V2[:, :nb2]; coefficients kept in S1 / S2.
+  void cgs_pass(const double* V1, int nb1, const double* V2, int nb2, double* Tp, int r) {
+    if (nb1 > 0) gemm_tn(H, nb1, r, n, V1, n, Tp, n, S1.get(), mv, 1.0, 0.0);
+    if (nb2 > 0) gemm_tn(H, nb2, r, n, V2, n, Tp, n, S2.get(), bs, 1.0, 0.0);
+    if (nb1 > 0) gemm_nn(H, n, r, nb1, V1, n, S1.get(), mv, Tp, n, -1.0, 1.0);
+    if (nb2 > 0) gemm_nn(H, n, r, nb2, V2, n, S2.get(), bs, Tp, n, -1.0, 1.0);
   }
 
-  // Orthonormalise T[:, :r] within itself into dst[:, :kept] (dst != T).
-  int orth_within(double* Tp, int r, double* dst) {
-    gemm_tn(H, r, r, n, Tp, n, Tp, n, small.get(), r, 1.0, 0.0);
-    launch_orth_coeffs(r, small.get(), r, M.get(), r, work.get(), info.get(), 1e-12, H.st);
+  // Orthonormalise Tp[:, :r] within itself into dst (dst != Tp) from its Gram
+  // matrix Gm (r x r, ld r); returns the number of directions kept and the
+  // condition number of the kept part of the Gram matrix.
+  int orth_from_gram(double* Tp, int r, double* dst, double* gram_cond) {
+    launch_orth_coeffs(r, Gm.get(), r, M.get(), r, work.get(), info.get(), 1e-12, H.st, cond.get());
     int kept = 0;
     HIP_CHECK(hipMemcpyAsync(&kept, info.get(), sizeof(int), hipMemcpyDeviceToHost, H.st));
+    HIP_CHECK(hipMemcpyAsync(gram_cond, cond.get(), sizeof(double), hipMemcpyDeviceToHost, H.st));
     HIP_CHECK(hipStreamSynchronize(H.st));
     if (kept < 0) throw Error(ADC2G_EEIG, "Gram-matrix eigensolve failed in block orthonormalisation");
     if (kept > 0) gemm_nn(H, n, kept, r, Tp, n, M.get(), r, dst, n, 1.0, 0.0);
     return kept;
   }
 
-  // New block T[:, :r] -> V[:, c0:c0+kept], orthonormal to V[:, :c0].
-  int append_block(int c0, int r) {
-    project(V.get(), c0, T.get(), r);
+  // New block T[:, :r] -> V[:, c0:c0+kept], orthonormal to V[:, :c0] and to
+  // the locked vectors Xc[:, :ncv].  One block CGS pass, a second only when
+  // the DGKS test asks for it; within the block, orthonormalisation through
+  // the Gram matrix, repeated when that matrix was ill-conditioned.
+  int append_block(int c0, int r, const double* Xc, int ncv) {
+    cgs_pass(V.get(), c0, Xc, ncv, T.get(), r);
+    gemm_tn(H, r, r, n, T.get(), n, T.get(), n, Gm.get(), r, 1.0, 0.0);
+    if (c0 + ncv > 0) {
+      launch_dgks_check(r, c0, S1.get(), mv, ncv, S2.get(), bs, Gm.get(), r, info.get() + 1, H.st);
+      int need = 0;
+      HIP_CHECK(hipMemcpyAsync(&need, info.get() + 1, sizeof(int), hipMemcpyDeviceToHost, H.st));
+      HIP_CHECK(hipStreamSynchronize(H.st));
+      if (need) {
+        ++n_second_pass;
+        cgs_pass(V.get(), c0, Xc, ncv, T.get(), r);
+        gemm_tn(H, r, r, n, T.get(), n, T.get(), n, Gm.get(), r, 1.0, 0.0);
+      }
+    }
     double* slot = V.get() + n * c0;
-    int k1 = orth_within(T.get(), r, slot);
+    double gc = 0.0;
+    const int k1 = orth_from_gram(T.get(), r, slot, &gc);
     if (k1 == 0) return 0;
-    int k2 = orth_within(slot, k1, T.get());
-    if (k2 > 0) HIP_CHECK(hipMemcpyAsync(slot, T.get(), sizeof(double) * n * k2, hipMemcpyDeviceToDevice, H.st));
-    return k2;
+    if (!(gc < 1e3)) {
+      gemm_tn(H, k1, k1, n, slot, n, slot, n, Gm.get(), k1, 1.0, 0.0);
+      const int k2 = orth_from_gram(slot, k1, T.get(), &gc);
+      if (k2 > 0) HIP_CHECK(hipMemcpyAsync(slot, T.get(), sizeof(double) * n * k2, hipMemcpyDeviceToDevice, H.st));
+      return k2;
+    }
+    return k1;
   }
 
   void swap_cols(DevBuf<double>& B, int i, int j) {
@@ -227,7 +260,7 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
     // Rayleigh-Ritz (diagrmat)
     HIP_CHECK(hipMemcpy2DAsync(D.Rt.get(), sizeof(double) * mv, D.Rm.get(), sizeof(double) * mv,
                                sizeof(double) * currdim, currdim, hipMemcpyDeviceToDevice, st));
-    launch_syev(currdim, D.Rt.get(), mv, D.thd.get(), D.Zs.get(), mv, D.work.get(), D.info.get(), st);
+    launch_syevx(currdim, bs_curr, D.Rt.get(), mv, D.thd.get(), D.Zs.get(), mv, D.work.get(), D.info.get(), st);
     // Ritz vectors and W y (calcritzvec, calcres)
     gemm_nn(H, n, bs_curr, currdim, D.V.get(), n, D.Zs.get(), mv, D.X.get(), n, 1.0, 0.0);
     gemm_nn(H, n, bs_curr, currdim, D.W.get(), n, D.Zs.get(), mv, D.WY.get(), n, 1.0, 0.0);
@@ -345,9 +378,9 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
       precon_learned_step(H, n, bs_curr, D.thd.get(), norm.data(), use_dpr.data(), D.Res.get(), D.T.get(), n);
     }
     for (int i = 0; i < bs_curr; ++i) prev_norm[i] = norm[i];
-    // projection against the converged (deflated) vectors
-    if (p.deflate && nconv > 0 && !lowphase) D.project(D.X.get() + n * (bs - nconv), nconv, D.T.get(), bs_curr);
-    const int kept = D.append_block(ilast, bs_curr);
+    // new block, orthogonal to the subspace and to the converged (deflated) vectors
+    const bool lock = p.deflate && nconv > 0 && !lowphase;
+    const int kept = D.append_block(ilast, bs_curr, lock ? D.X.get() + n * (bs - nconv) : nullptr, lock ? nconv : 0);
     if (kept == 0) throw Error(ADC2G_ENOCONV, "subspace expansion produced no new directions");
     D.sigma(ilast, kept, cur);
     D.rayleigh_cols(ilast, ilast + kept);

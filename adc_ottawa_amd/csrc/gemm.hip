@@ -54,6 +54,17 @@ void dispatch(GemmArgs g, GemmWorkspace& ws, hipStream_t st, int splitk_hint) {
     launch<T, 128, 128, BK>(g, st);
   if (need_partial) {
     const int64_t total = (int64_t)g.M * g.N * g.batch;
+    if (split >= 8 && total <= (int64_t)1 << 22) {
+      const int wb = (int)((total + 63) / 64);
+      if (g.out_prec == 1)
+        hipLaunchKernelGGL((splitk_reduce_wide_kernel<Acc, float>), dim3(wb), dim3(256), 0, st, g);
+      else if (g.out_prec == 2)
+        hipLaunchKernelGGL((splitk_reduce_wide_kernel<Acc, __bf16>), dim3(wb), dim3(256), 0, st, g);
+      else
+        hipLaunchKernelGGL((splitk_reduce_wide_kernel<Acc, double>), dim3(wb), dim3(256), 0, st, g);
+      HIP_CHECK(hipGetLastError());
+      return;
+    }
     const int blocks = (int)std::min<int64_t>((total + 255) / 256, 4096);
     if (g.out_prec == 1)
       hipLaunchKernelGGL((splitk_reduce_kernel<Acc, float>), dim3(blocks), dim3(256), 0, st, g);

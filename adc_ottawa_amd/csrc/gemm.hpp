@@ -397,6 +397,31 @@ __global__ void splitk_reduce_kernel(GemmArgs g) {
   }
 }
 
+// Split-K reduction for many splits and few outputs: 64 consecutive outputs
+// per workgroup (one per lane), the splits dealt round-robin to the 4 waves and
+// the 4 wave sums added in a fixed order.
+template <class Acc, class OutT>
+__global__ void __launch_bounds__(256) splitk_reduce_wide_kernel(GemmArgs g) {
+  __shared__ double red[4][64];
+  const int64_t MN = (int64_t)g.M * g.N;
+  const int64_t total = MN * g.batch;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t idx = (int64_t)blockIdx.x * 64 + lane;
+  double s = 0.0;
+  if (idx < total) {
+    const int64_t b = idx / MN, mn = idx - b * MN;
+    const Acc* p = (const Acc*)g.partial + b * g.splitk * MN + mn;
+    for (int k = wv; k < g.splitk; k += 4) s += (double)p[k * MN];
+  }
+  red[wv][lane] = s;
+  __syncthreads();
+  if (wv == 0 && idx < total) {
+    s = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    const int64_t b = idx / MN, mn = idx - b * MN;
+    map_store<OutT>(g.map, mn / g.N, mn % g.N, b, s);
+  }
+}
+
 // Host launcher. Chooses tile sizes and split-K; `ws` is a workspace for the
 // split-K partials that is grown on demand.
 enum class Prec { F64, F32, BF16 };

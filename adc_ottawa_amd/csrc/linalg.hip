@@ -37,12 +37,13 @@ struct SyevShared {
   int lo, hi, go, info;
 };
 
-// Eigen-decomposition of the symmetric n x n matrix A (destroyed); eigenvalues
-// ascending into w, eigenvectors into the columns of Z; Zw is n*n scratch.
-__device__ void syev_dev(int n, double* A, int lda, double* w, double* Z, int ldz, double* Zw, int* info,
-                         SyevShared& S) {
+// Householder reduction of the symmetric n x n matrix A to tridiagonal form.
+// On exit S.d / S.e hold the diagonal and the off-diagonal (e[n-1] = 0) and
+// column k of A below row k holds reflector k (unit 2-norm, H_k = I - 2 v v^T).
+// Zw needs 2n doubles; S.rc is used as scratch for the matrix-vector product.
+__device__ void tridiag_dev(int n, double* A, int lda, double* Zw, SyevShared& S) {
   const int tid = threadIdx.x, nt = blockDim.x;
-  // ---- Householder tridiagonalisation (reflector k stored in column k below row k)
+  const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
   for (int k = 0; k + 2 < n; ++k) {
     const int m = n - k - 1;
     double part = 0.0;
@@ -72,11 +73,29 @@ __device__ void syev_dev(int n, double* A, int lda, double* w, double* Z, int ld
     if (tid == 0) S.e[k] = alpha;
     __syncthreads();
     const double* v = A + (int64_t)k * lda;  // v[i] for i in k+1..n-1
-    // p = A22 v, one row per lane (coalesced over the column-major storage)
-    for (int i = k + 1 + tid; i < n; i += nt) {
-      double s = 0.0;
-      for (int j = k + 1; j < n; ++j) s += A[i + (int64_t)j * lda] * v[j];
-      Zw[i] = s;
+    // p = A22 v: the m rows are split into chunks of 64 (one row per lane,
+    // coalesced over the column-major storage) and the columns into ns
+    // slices, one (chunk, slice) per wave; partial sums meet in S.rc.
+    {
+      const int rch = (m + 63) >> 6;
+      const int ns = max(1, nw / rch);
+      const int per = (m + ns - 1) / ns;
+      for (int task = wv; task < rch * ns; task += nw) {
+        const int rc = task % rch, sl = task / rch;
+        const int i = k + 1 + rc * 64 + lane;
+        const int j0 = k + 1 + sl * per, j1 = min(n, j0 + per);
+        if (i < n) {
+          double s = 0.0;
+          for (int j = j0; j < j1; ++j) s += A[i + (int64_t)j * lda] * v[j];
+          S.rc[sl * m + (i - k - 1)] = s;
+        }
+      }
+      __syncthreads();
+      for (int i = k + 1 + tid; i < n; i += nt) {
+        double s = 0.0;
+        for (int sl = 0; sl < ns; ++sl) s += S.rc[sl * m + (i - k - 1)];
+        Zw[i] = s;
+      }
     }
     __syncthreads();
     part = 0.0;
@@ -96,8 +115,17 @@ __device__ void syev_dev(int n, double* A, int lda, double* w, double* Z, int ld
   if (tid == 0) {
     if (n >= 2) S.e[n - 2] = A[n - 1 + (int64_t)(n - 2) * lda];
     S.e[n - 1] = 0.0;
-    S.info = 0;
   }
+  __syncthreads();
+}
+
+// Eigen-decomposition of the symmetric n x n matrix A (destroyed); eigenvalues
+// ascending into w, eigenvectors into the columns of Z; Zw is n*n scratch.
+__device__ void syev_dev(int n, double* A, int lda, double* w, double* Z, int ldz, double* Zw, int* info,
+                         SyevShared& S) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  tridiag_dev(n, A, lda, Zw, S);
+  if (tid == 0) S.info = 0;
   for (int64_t idx = tid; idx < (int64_t)n * n; idx += nt) {
     const int i = (int)(idx % n), j = (int)(idx / n);
     Zw[idx] = (i == j) ? 1.0 : 0.0;
@@ -231,11 +259,258 @@ __global__ void __launch_bounds__(SYEV_THREADS) syev_kernel(int n, double* A, in
   syev_dev(n, A, lda, w, Z, ldz, work, info, S);
 }
 
+// ---- lowest k eigenpairs: bisection + inverse iteration on the tridiagonal
+
+struct BisectShared {
+  double lo[SYEV_NMAX], hi[SYEV_NMAX], mu[SYEV_NMAX];
+  double xs[SYEV_THREADS];
+  int cs[SYEV_THREADS];
+  double red[3][16];
+};
+
+// Number of eigenvalues of the tridiagonal (d, e2 = e^2) below x (Sturm count).
+__device__ int sturm_count(int n, const double* d, const double* e2, double x, double pivmin) {
+  double q = d[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  int c = q < 0.0;
+  for (int i = 1; i < n; ++i) {
+    q = (d[i] - x) - e2[i - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    c += q < 0.0;
+  }
+  return c;
+}
+
+__device__ __forceinline__ double unit_rand(uint64_t a, uint64_t b) {
+  uint64_t z = a * 0x9E3779B97F4A7C15ull + b * 0xBF58476D1CE4E5B9ull + 0x94D049BB133111EBull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;  // [-1, 1)
+}
+
+// Lowest k eigenpairs of the symmetric n x n matrix A (destroyed), eigenvalues
+// ascending in w[0..k), eigenvectors in Z[:, 0..k).  Householder reduction,
+// then on the tridiagonal T:
+//  * eigenvalue j by multisection: nt/k lanes evaluate Sturm counts at
+//    interior points of its bracket each round, until the bracket is within
+//    eps (2 |lambda| + ||T||);
+//  * eigenvector j by inverse iteration on lane j: LU of T - mu_j I with
+//    partial pivoting (shifts of coincident eigenvalues separated by
+//    10 eps ||T||), three solves from a fixed pseudo-random start, then
+//    two modified Gram-Schmidt sweeps over the k vectors (one wave);
+//  * back-transformation of the k vectors with the stored reflectors.
+// work: 2n + 6nk doubles.
+__device__ void syevx_dev(int n, int k, double* A, int lda, double* w, double* Z, int ldz, double* work, int* info,
+                          SyevShared& S, BisectShared& B) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+  tridiag_dev(n, A, lda, work, S);
+  const double* d = S.d;
+  double* e2 = S.rc;
+  for (int i = tid; i < n; i += nt) e2[i] = S.e[i] * S.e[i];
+  // Gershgorin interval, ||T|| and the largest e^2
+  double glo = INFINITY, ghi = -INFINITY, em = 0.0;
+  int bad = 0;
+  for (int i = tid; i < n; i += nt) {
+    const double r = (i > 0 ? fabs(S.e[i - 1]) : 0.0) + fabs(S.e[i]);
+    glo = fmin(glo, d[i] - r);
+    ghi = fmax(ghi, d[i] + r);
+    em = fmax(em, S.e[i] * S.e[i]);
+    if (!isfinite(d[i]) || !isfinite(S.e[i])) bad = 1;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    glo = fmin(glo, __shfl_xor(glo, off, 64));
+    ghi = fmax(ghi, __shfl_xor(ghi, off, 64));
+    em = fmax(em, __shfl_xor(em, off, 64));
+  }
+  if (lane == 0) {
+    B.red[0][wv] = glo;
+    B.red[1][wv] = ghi;
+    B.red[2][wv] = em;
+  }
+  bad = __syncthreads_or(bad);
+  if (bad) {
+    if (tid == 0) info[0] = 1;
+    return;
+  }
+  glo = B.red[0][0];
+  ghi = B.red[1][0];
+  em = B.red[2][0];
+  for (int i = 1; i < nw; ++i) {
+    glo = fmin(glo, B.red[0][i]);
+    ghi = fmax(ghi, B.red[1][i]);
+    em = fmax(em, B.red[2][i]);
+  }
+  const double tnorm = fmax(fmax(fabs(glo), fabs(ghi)), DBL_MIN);
+  const double pivmin = DBL_MIN * fmax(1.0, em);
+  const double fudge = 2.0 * tnorm * DBL_EPSILON * n + 2.0 * pivmin;
+  glo -= fudge;
+  ghi += fudge;
+  for (int j = tid; j < k; j += nt) {
+    B.lo[j] = glo;
+    B.hi[j] = ghi;
+  }
+  __syncthreads();
+  // ---- multisection
+  const int G = max(1, nt / k);
+  for (int round = 0; round < 96; ++round) {
+    const int j = tid / G, g = tid - j * G;
+    if (j < k) {
+      const double lo = B.lo[j], hi = B.hi[j];
+      const double x = lo + (hi - lo) * (double)(g + 1) / (double)(G + 1);
+      B.xs[tid] = x;
+      B.cs[tid] = sturm_count(n, d, e2, x, pivmin);
+    }
+    __syncthreads();
+    int open = 0;
+    for (int jj = tid; jj < k; jj += nt) {
+      double lo = B.lo[jj], hi = B.hi[jj];
+      for (int gg = 0; gg < G; ++gg) {
+        const double x = B.xs[jj * G + gg];
+        if (B.cs[jj * G + gg] <= jj) lo = fmax(lo, x);
+        else hi = fmin(hi, x);
+      }
+      B.lo[jj] = lo;
+      B.hi[jj] = hi;
+      const double tol = DBL_EPSILON * (2.0 * fmax(fabs(lo), fabs(hi)) + tnorm);
+      const double mid = 0.5 * (lo + hi);
+      if (hi - lo > tol && mid > lo && mid < hi) open = 1;
+    }
+    if (!__syncthreads_or(open)) break;
+  }
+  for (int j = tid; j < k; j += nt) w[j] = 0.5 * (B.lo[j] + B.hi[j]);
+  if (tid == 0) {
+    const double pertol = 10.0 * DBL_EPSILON * tnorm;
+    double prev = -INFINITY;
+    for (int j = 0; j < k; ++j) {
+      double m = 0.5 * (B.lo[j] + B.hi[j]);
+      if (j > 0 && m - prev < pertol) m = prev + pertol;
+      B.mu[j] = m;
+      prev = m;
+    }
+  }
+  __syncthreads();
+  // ---- inverse iteration, one lane per eigenvalue; arrays [i * k + j]
+  {
+    double* U1 = work + 2 * n;
+    double* U2 = U1 + (int64_t)n * k;
+    double* U3 = U2 + (int64_t)n * k;
+    double* ML = U3 + (int64_t)n * k;
+    double* PV = ML + (int64_t)n * k;
+    double* Y = PV + (int64_t)n * k;
+    const double tiny = DBL_EPSILON * tnorm;
+    for (int j = tid; j < k; j += nt) {
+      const double mu = B.mu[j];
+      double cu = d[0] - mu, cv = (n > 1) ? S.e[0] : 0.0;
+      for (int i = 0; i + 1 < n; ++i) {
+        const double a1 = d[i + 1] - mu, b1 = S.e[i], c1 = (i + 2 < n) ? S.e[i + 1] : 0.0;
+        double m;
+        if (fabs(cu) >= fabs(b1)) {
+          m = (cu != 0.0) ? b1 / cu : 0.0;
+          U1[(int64_t)i * k + j] = cu;
+          U2[(int64_t)i * k + j] = cv;
+          U3[(int64_t)i * k + j] = 0.0;
+          PV[(int64_t)i * k + j] = 0.0;
+          cu = a1 - m * cv;
+          cv = c1;
+        } else {
+          m = cu / b1;
+          U1[(int64_t)i * k + j] = b1;
+          U2[(int64_t)i * k + j] = a1;
+          U3[(int64_t)i * k + j] = c1;
+          PV[(int64_t)i * k + j] = 1.0;
+          const double nu = cv - m * a1;
+          cv = -m * c1;
+          cu = nu;
+        }
+        ML[(int64_t)i * k + j] = m;
+      }
+      U1[(int64_t)(n - 1) * k + j] = cu;
+      for (int i = 0; i < n; ++i) {
+        double& u = U1[(int64_t)i * k + j];
+        if (fabs(u) < tiny) u = (u < 0.0) ? -tiny : tiny;
+      }
+      for (int i = 0; i < n; ++i) Y[(int64_t)i * k + j] = unit_rand(j + 1, i);
+      for (int itr = 0; itr < 3; ++itr) {
+        for (int i = 0; i + 1 < n; ++i) {
+          double bi = Y[(int64_t)i * k + j], bn = Y[(int64_t)(i + 1) * k + j];
+          if (PV[(int64_t)i * k + j] != 0.0) {
+            const double t = bi;
+            bi = bn;
+            bn = t;
+          }
+          Y[(int64_t)i * k + j] = bi;
+          Y[(int64_t)(i + 1) * k + j] = bn - ML[(int64_t)i * k + j] * bi;
+        }
+        double y2 = 0.0, y1 = 0.0, ymax = 0.0;
+        for (int i = n - 1; i >= 0; --i) {
+          const double y = (Y[(int64_t)i * k + j] - U2[(int64_t)i * k + j] * y1 - U3[(int64_t)i * k + j] * y2) /
+                           U1[(int64_t)i * k + j];
+          Y[(int64_t)i * k + j] = y;
+          y2 = y1;
+          y1 = y;
+          ymax = fmax(ymax, fabs(y));
+        }
+        const double sc = (ymax > 0.0 && isfinite(ymax)) ? 1.0 / ymax : 0.0;
+        for (int i = 0; i < n; ++i) Y[(int64_t)i * k + j] *= sc;
+      }
+      for (int i = 0; i < n; ++i) Z[i + (int64_t)j * ldz] = Y[(int64_t)i * k + j];
+    }
+  }
+  __syncthreads();
+  // ---- two modified Gram-Schmidt sweeps (wave 0)
+  if (wv == 0) {
+    for (int j = 0; j < k; ++j) {
+      double* zj = Z + (int64_t)j * ldz;
+      for (int pass = 0; pass < 2; ++pass) {
+        for (int i = 0; i < j; ++i) {
+          const double* zi = Z + (int64_t)i * ldz;
+          double t = 0.0;
+          for (int r = lane; r < n; r += 64) t += zi[r] * zj[r];
+          for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+          for (int r = lane; r < n; r += 64) zj[r] -= t * zi[r];
+        }
+      }
+      double t = 0.0;
+      for (int r = lane; r < n; r += 64) t += zj[r] * zj[r];
+      for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+      const double sc = (t > 0.0) ? 1.0 / sqrt(t) : 0.0;
+      for (int r = lane; r < n; r += 64) zj[r] *= sc;
+    }
+  }
+  __syncthreads();
+  // ---- back transformation of the k vectors (one column per wave)
+  for (int j = wv; j < k; j += nw) {
+    double* z = Z + (int64_t)j * ldz;
+    for (int kk = n - 3; kk >= 0; --kk) {
+      const double* v = A + (int64_t)kk * lda;
+      double t = 0.0;
+      for (int i = kk + 1 + lane; i < n; i += 64) t += v[i] * z[i];
+      for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+      t *= 2.0;
+      if (t != 0.0)
+        for (int i = kk + 1 + lane; i < n; i += 64) z[i] -= t * v[i];
+    }
+  }
+  if (tid == 0) info[0] = 0;
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(SYEV_THREADS) syevx_kernel(int n, int k, double* A, int lda, double* w, double* Z,
+                                                              int ldz, double* work, int* info) {
+  __shared__ SyevShared S;
+  __shared__ BisectShared B;
+  syevx_dev(n, k, A, lda, w, Z, ldz, work, info, S, B);
+}
+
 // M = D U_kept Lambda^{-1/2} for the scaled Gram matrix D G D (D = diag(G)^{-1/2}).
 // work: r*r (copy) + r*r (U) + r (lambda) + r*r (syev scratch).
 __global__ void __launch_bounds__(SYEV_THREADS) orth_coeffs_kernel(int r, const double* G, int ldg, double* M,
-                                                                    int ldm, double* work, int* kept, double tol) {
+                                                                    int ldm, double* work, int* kept, double tol,
+                                                                    double* cond) {
   __shared__ SyevShared S;
+  __shared__ BisectShared Bs;
   __shared__ double dinv[SYEV_NMAX];
   __shared__ int sel[SYEV_NMAX];
   __shared__ int nsel;
@@ -255,15 +530,20 @@ __global__ void __launch_bounds__(SYEV_THREADS) orth_coeffs_kernel(int r, const 
   }
   __syncthreads();
   __shared__ int sinfo;
-  syev_dev(r, Gc, r, lam, U, r, Zw, &sinfo, S);
+  syevx_dev(r, r, Gc, r, lam, U, r, Zw, &sinfo, S, Bs);
   __syncthreads();
   if (tid == 0) {
     int c = 0;
+    double lmin = 0.0;
     if (sinfo == 0) {
       const double lmax = lam[r - 1];
       // largest eigenvalues first, so the best-conditioned directions come first
       for (int j = r - 1; j >= 0; --j)
-        if (lam[j] > tol * lmax && lam[j] > 0.0) sel[c++] = j;
+        if (lam[j] > tol * lmax && lam[j] > 0.0) {
+          sel[c++] = j;
+          lmin = lam[j];
+        }
+      if (cond) cond[0] = (c > 0) ? lmax / lmin : INFINITY;
     }
     nsel = c;
     kept[0] = (sinfo == 0) ? c : -1;
@@ -275,6 +555,24 @@ __global__ void __launch_bounds__(SYEV_THREADS) orth_coeffs_kernel(int r, const 
     const int j = sel[t];
     M[i + (int64_t)t * ldm] = dinv[i] * U[i + (int64_t)j * r] / sqrt(lam[j]);
   }
+}
+
+// need[0] = 1 when some column lost more than half its squared norm to the
+// projection: before^2 = |S1[:,j]|^2 + |S2[:,j]|^2 + G[j,j], after^2 = G[j,j].
+__global__ void dgks_check_kernel(int r, int nb1, const double* __restrict__ S1, int ld1, int nb2,
+                                  const double* __restrict__ S2, int ld2, const double* __restrict__ G, int ldg,
+                                  int* __restrict__ need) {
+  int bad = 0;
+  for (int j = threadIdx.x; j < r; j += blockDim.x) {
+    double b = 0.0;
+    for (int i = 0; i < nb1; ++i) b += S1[i + (int64_t)j * ld1] * S1[i + (int64_t)j * ld1];
+    for (int i = 0; i < nb2; ++i) b += S2[i + (int64_t)j * ld2] * S2[i + (int64_t)j * ld2];
+    const double a = G[j + (int64_t)j * ldg];
+    b += a;
+    if (!(a >= 0.5 * b)) bad = 1;
+  }
+  bad = __syncthreads_or(bad);
+  if (threadIdx.x == 0) need[0] = bad;
 }
 
 __global__ void residual_kernel(int64_t n, int nc, const double* __restrict__ theta, const double* __restrict__ X,
@@ -297,12 +595,14 @@ __global__ void residual_kernel(int64_t n, int nc, const double* __restrict__ th
   }
 }
 
+// One workgroup per column; fixed summation order.
 __global__ void colsum_kernel(int nblk, int nc, const double* __restrict__ part, double* __restrict__ out) {
-  for (int c = threadIdx.x; c < nc; c += blockDim.x) {
-    double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * nc + c];
-    out[c] = s;
-  }
+  __shared__ double red[16];
+  const int c = blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[(int64_t)b * nc + c];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) out[c] = s;
 }
 
 __device__ __forceinline__ double guard_den(double den) {
@@ -402,6 +702,14 @@ void launch_syev(int n, double* A, int lda, double* w, double* Z, int ldz, doubl
   HIP_CHECK(hipGetLastError());
 }
 
+void launch_syevx(int n, int k, double* A, int lda, double* w, double* Z, int ldz, double* work, int* info,
+                  hipStream_t st) {
+  if (n > SYEV_NMAX) throw Error(ADC2G_EARG, "syevx: n > " + std::to_string(SYEV_NMAX));
+  if (k < 1 || k > n) throw Error(ADC2G_EARG, "syevx: need 1 <= k <= n");
+  hipLaunchKernelGGL(syevx_kernel, dim3(1), dim3(SYEV_THREADS), 0, st, n, k, A, lda, w, Z, ldz, work, info);
+  HIP_CHECK(hipGetLastError());
+}
+
 int launch_residual(int64_t n, int nc, const double* theta, const double* X, const double* WY, double* Res,
                     int64_t ld, double* part, hipStream_t st) {
   const int nb = reduce_blocks(n);
@@ -411,7 +719,8 @@ int launch_residual(int64_t n, int nc, const double* theta, const double* X, con
 }
 
 void launch_colsum(int nblk, int nc, const double* part, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(256), 0, st, nblk, nc, part, out);
+  if (nc <= 0) return;
+  hipLaunchKernelGGL(colsum_kernel, dim3(nc), dim3(256), 0, st, nblk, nc, part, out);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -436,15 +745,22 @@ void launch_olsen_step(int64_t n, int nc, const double* theta, const double* hii
   HIP_CHECK(hipGetLastError());
 }
 
+void launch_dgks_check(int r, int nb1, const double* S1, int ld1, int nb2, const double* S2, int ld2, const double* G,
+                       int ldg, int* need, hipStream_t st) {
+  hipLaunchKernelGGL(dgks_check_kernel, dim3(1), dim3(256), 0, st, r, nb1, S1, ld1, nb2, S2, ld2, G, ldg, need);
+  HIP_CHECK(hipGetLastError());
+}
+
 void launch_unit_vectors(int nc, const int64_t* idx, double* V, int64_t ld, hipStream_t st) {
   hipLaunchKernelGGL(unit_vectors_kernel, dim3((nc + 255) / 256), dim3(256), 0, st, nc, idx, V, ld);
   HIP_CHECK(hipGetLastError());
 }
 
 void launch_orth_coeffs(int r, const double* G, int ldg, double* M, int ldm, double* work, int* kept, double tol,
-                        hipStream_t st) {
+                        hipStream_t st, double* cond) {
   if (r > SYEV_NMAX) throw Error(ADC2G_EARG, "orth_coeffs: block too large");
-  hipLaunchKernelGGL(orth_coeffs_kernel, dim3(1), dim3(SYEV_THREADS), 0, st, r, G, ldg, M, ldm, work, kept, tol);
+  hipLaunchKernelGGL(orth_coeffs_kernel, dim3(1), dim3(SYEV_THREADS), 0, st, r, G, ldg, M, ldm, work, kept, tol,
+                     cond);
   HIP_CHECK(hipGetLastError());
 }
 

@@ -11,6 +11,12 @@ namespace adc2g {
 void launch_syev(int n, double* A, int lda, double* w, double* Z, int ldz, double* work, int* info,
                  hipStream_t st);
 
+// Lowest k eigenpairs of the symmetric n x n matrix A (destroyed; one
+// workgroup): eigenvalues ascending in w[0..k), eigenvectors in Z[:, 0..k).
+// work needs 2n + 6nk doubles.  info[0] = 0 on success, 1 on non-finite input.
+void launch_syevx(int n, int k, double* A, int lda, double* w, double* Z, int ldz, double* work, int* info,
+                  hipStream_t st);
+
 // Res[:,i] = theta[i] X[:,i] - WY[:,i]; part[blk][i] = partial sums of Res^2
 // (nblk = returned grid size), then colsum -> norms2[i].
 int launch_residual(int64_t n, int nc, const double* theta, const double* X, const double* WY, double* Res,
@@ -35,8 +41,15 @@ void launch_unit_vectors(int nc, const int64_t* idx, double* V, int64_t ld, hipS
 // of T, build M (r x r_kept, ld = ldm) with T M orthonormal; drops directions
 // with relative eigenvalue below tol.  Returns r_kept through kept[0].
 // Runs on one workgroup; scratch: work (see .hip).
+// cond (optional, device) receives lambda_max / lambda_min of the kept directions.
 void launch_orth_coeffs(int r, const double* G, int ldg, double* M, int ldm, double* work, int* kept,
-                        double tol, hipStream_t st);
+                        double tol, hipStream_t st, double* cond = nullptr);
+
+// Block Gram-Schmidt "twice is enough" test (Kahan-Parlett / DGKS): after one
+// projection T <- T - V1 S1 - V2 S2 of an orthonormal [V1 V2], need[0] = 1 if
+// some column j kept less than half its squared norm (G = T^T T afterwards).
+void launch_dgks_check(int r, int nb1, const double* S1, int ld1, int nb2, const double* S2, int ld2, const double* G,
+                       int ldg, int* need, hipStream_t st);
 
 // Copy the upper block of a symmetric matrix to the lower: R[j + ld*i] = R[i + ld*j]
 // for i in [c0, c1), j < c0... (rows c0..c1 of the new columns)

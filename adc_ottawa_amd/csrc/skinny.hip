@@ -4,9 +4,8 @@
 //              lane, B staged in LDS (broadcast reads), accumulators in
 //              registers.
 //   skinny_tn: C (M x N) = alpha A^T B + beta C, A (K x M), B (K x N) with K =
-//              ndim rows: each block reduces a K-chunk into a partial M x N
-//              through LDS tiles of 16 rows, a second kernel sums the
-//              partials in a fixed order (deterministic).
+//              ndim rows: FP64 MFMA over K-splits (see tn_mfma_kernel), then a
+//              fixed-order sum of the split partials (deterministic).
 #include "skinny.hpp"
 
 namespace adc2g {
@@ -56,81 +55,110 @@ __global__ void __launch_bounds__(NN_THREADS) skinny_nn_kernel(int64_t M, int N,
   }
 }
 
-constexpr int TN_THREADS = 256;
-constexpr int TN_RB = 16;  // rows per LDS tile
+// C = A^T B with K (= ndim) rows: each workgroup takes 64 columns of A (one
+// 16-column tile per wave) and all N <= 16*NT columns of B over one K-split,
+// streams K in chunks of 64 rows through LDS (each column run is 512
+// contiguous bytes) and accumulates with FP64 MFMA 16x16x4.  The splits of one
+// column block share an XCD (blockIdx -> XCD is round-robin), so B is re-read
+// from that XCD's L2.  Partials go to part[split][m][n]; tn_reduce sums them
+// in a fixed order.
+constexpr int TN_KC = 64;
+constexpr int TN_MB = 64;
 
-// partial[blk][m*N + n] = sum over the block's rows of A[k][m] B[k][n]
-__global__ void __launch_bounds__(TN_THREADS) skinny_tn_kernel(int64_t K, int M, int N, const double* __restrict__ A,
-                                                               int64_t lda, const double* __restrict__ B,
-                                                               int64_t ldb, int64_t rows_per_block,
-                                                               double* __restrict__ part) {
-  extern __shared__ double sm[];
-  double* sA = sm;                   // [TN_RB][M]
-  double* sB = sm + TN_RB * M;       // [TN_RB][N]
-  const int mb = (M + 3) / 4, nb = (N + 3) / 4;
-  const int tiles = mb * nb;
-  // each lane owns up to 4 tiles of 4x4 outputs
-  double acc[4][16];
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+template <int NT>
+__global__ void __launch_bounds__(256) tn_mfma_kernel(int64_t K, int M, int N, const double* __restrict__ A,
+                                                      int64_t lda, const double* __restrict__ B, int64_t ldb,
+                                                      int mblocks, int splits, int64_t rows_per_split,
+                                                      double* __restrict__ part) {
+  __shared__ double sA[TN_MB][TN_KC + 1];
+  __shared__ double sB[16 * NT][TN_KC + 1];
+  const int L = blockIdx.x;
+  const int xcd = L & 7, local = L >> 3;
+  const int mb = local % mblocks;
+  const int split = (local / mblocks) * 8 + xcd;
+  if (split >= splits) return;  // whole workgroup, before any barrier
+  const int m0 = mb * TN_MB;
+  const int64_t r0 = (int64_t)split * rows_per_split;
+  const int64_t r1 = min(K, r0 + rows_per_split);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double ra[16], rb[4 * NT];
+  auto load = [&](int64_t k0) {
+    const int64_t k = k0 + lane;
+    const bool ok = k < r1;
 #pragma unroll
-  for (int q = 0; q < 4; ++q)
-#pragma unroll
-    for (int e = 0; e < 16; ++e) acc[q][e] = 0.0;
-  const int64_t r0 = blockIdx.x * rows_per_block;
-  const int64_t r1 = min(K, r0 + rows_per_block);
-  for (int64_t k0 = r0; k0 < r1; k0 += TN_RB) {
-    const int kr = (int)min((int64_t)TN_RB, r1 - k0);
-    __syncthreads();
-    for (int t = threadIdx.x; t < TN_RB * M; t += TN_THREADS) {
-      const int k = t % TN_RB, m = t / TN_RB;
-      sA[k * M + m] = (k < kr) ? A[(k0 + k) + lda * m] : 0.0;
+    for (int c = 0; c < 16; ++c) {
+      const int m = m0 + wv + 4 * c;
+      ra[c] = (ok && m < M) ? A[k + lda * m] : 0.0;
     }
-    for (int t = threadIdx.x; t < TN_RB * N; t += TN_THREADS) {
-      const int k = t % TN_RB, n = t / TN_RB;
-      sB[k * N + n] = (k < kr) ? B[(k0 + k) + ldb * n] : 0.0;
+#pragma unroll
+    for (int c = 0; c < 4 * NT; ++c) {
+      const int n = wv + 4 * c;
+      rb[c] = (ok && n < N) ? B[k + ldb * n] : 0.0;
     }
+  };
+  dbl4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+  load(r0);
+  const int row = lane & 15, kg = lane >> 4;
+  for (int64_t k0 = r0; k0 < r1; k0 += TN_KC) {
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int tile = threadIdx.x + q * TN_THREADS;
-      if (tile < tiles) {
-      const int m0 = (tile / nb) * 4, n0 = (tile % nb) * 4;
-#pragma unroll 4
-      for (int k = 0; k < TN_RB; ++k) {
-        double a[4], b[4];
+    for (int c = 0; c < 16; ++c) sA[wv + 4 * c][lane] = ra[c];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          a[u] = (m0 + u < M) ? sA[k * M + m0 + u] : 0.0;
-          b[u] = (n0 + u < N) ? sB[k * N + n0 + u] : 0.0;
-        }
+    for (int c = 0; c < 4 * NT; ++c) sB[wv + 4 * c][lane] = rb[c];
+    __syncthreads();
+    if (k0 + TN_KC < r1) load(k0 + TN_KC);
+    if (m0 + 16 * wv >= M) continue;  // this wave's column tile is empty
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+    for (int ks = 0; ks < TN_KC / 4; ++ks) {
+      const double a = sA[16 * wv + row][4 * ks + kg];
 #pragma unroll
-          for (int w = 0; w < 4; ++w) acc[q][u * 4 + w] += a[u] * b[w];
-      }
+      for (int t = 0; t < NT; ++t) {
+        const double b = sB[16 * t + row][4 * ks + kg];
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
       }
     }
   }
-  double* p = part + (int64_t)blockIdx.x * M * N;
+  // D[i][j] of a 16x16 tile sits at lane j + 16 (i % 4), register i / 4
+  double* p = part + (int64_t)split * M * N;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int tile = threadIdx.x + q * TN_THREADS;
-    if (tile < tiles) {
-      const int m0 = (tile / nb) * 4, n0 = (tile % nb) * 4;
+  for (int t = 0; t < NT; ++t) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int w = 0; w < 4; ++w)
-          if (m0 + u < M && n0 + w < N) p[(m0 + u) * N + n0 + w] = acc[q][u * 4 + w];
+    for (int r = 0; r < 4; ++r) {
+      const int m = m0 + 16 * wv + 4 * r + kg;
+      const int n = 16 * t + row;
+      if (m < M && n < N) p[(int64_t)m * N + n] = acc[t][r];
     }
   }
 }
 
-__global__ void skinny_tn_reduce(int nblk, int M, int N, const double* __restrict__ part, double* __restrict__ C,
-                                 int64_t ldc, double alpha, double beta) {
+// C[m + ldc n] = alpha sum_s part[s][m][n] + beta C: 64 outputs per
+// workgroup, splits dealt round-robin to the 16 waves, fixed summation order.
+__global__ void __launch_bounds__(1024) tn_reduce_kernel(int splits, int M, int N, const double* __restrict__ part,
+                                                         double* __restrict__ C, int64_t ldc, double alpha,
+                                                         double beta) {
+  __shared__ double red[16][64];
   const int total = M * N;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int t = blockIdx.x * 64 + lane;
+  double s0 = 0.0, s1 = 0.0;
+  if (t < total) {
+    int b = wv;
+    for (; b + 16 < splits; b += 32) {
+      s0 += part[(int64_t)b * total + t];
+      s1 += part[(int64_t)(b + 16) * total + t];
+    }
+    if (b < splits) s0 += part[(int64_t)b * total + t];
+  }
+  red[wv][lane] = s0 + s1;
+  __syncthreads();
+  if (wv == 0 && t < total) {
     double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * total + t];
+#pragma unroll
+    for (int w = 0; w < 16; ++w) s += red[w][lane];
     const int m = t / N, n = t % N;
     double* c = C + m + ldc * n;
     *c = alpha * s + (beta != 0.0 ? beta * *c : 0.0);
@@ -161,20 +189,29 @@ bool skinny_nn(int64_t M, int N, int K, const double* A, int64_t lda, const doub
 bool skinny_tn(int M, int N, int64_t K, const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
                int64_t ldc, double alpha, double beta, DevBuf<double>& ws, hipStream_t st) {
   if (M <= 0 || N <= 0) return true;
-  const int tiles = ((M + 3) / 4) * ((N + 3) / 4);
-  if (tiles > 4 * TN_THREADS) return false;
-  const size_t lds = sizeof(double) * TN_RB * (size_t)(M + N);
-  if (lds > 64 * 1024) return false;
-  int nblk = 1024;
-  int64_t rpb = (K + nblk - 1) / nblk;
-  rpb = ((rpb + TN_RB - 1) / TN_RB) * TN_RB;
-  nblk = (int)((K + rpb - 1) / rpb);
-  ws.alloc((size_t)nblk * M * N);
-  hipLaunchKernelGGL(skinny_tn_kernel, dim3(nblk), dim3(TN_THREADS), lds, st, K, M, N, A, lda, B, ldb, rpb, ws.get());
+  if (N > 64 || K < 4 * TN_KC) return false;
+  const int mblocks = (M + TN_MB - 1) / TN_MB;
+  int splits = (1024 + mblocks - 1) / mblocks;
+  splits = (int)std::min<int64_t>(splits, K / (4 * TN_KC));
+  splits = std::max(splits, 1);
+  int64_t rps = (K + splits - 1) / splits;
+  rps = ((rps + TN_KC - 1) / TN_KC) * TN_KC;
+  splits = (int)((K + rps - 1) / rps);
+  ws.alloc((size_t)splits * M * N);
+  const int grid = ((splits + 7) / 8) * 8 * mblocks;
+  const int nt = (N + 15) / 16;
+#define L(NT)                                                                                                  \
+  hipLaunchKernelGGL((tn_mfma_kernel<NT>), dim3(grid), dim3(256), 0, st, K, M, N, A, lda, B, ldb, mblocks, splits, \
+                     rps, ws.get())
+  if (nt == 1) L(1);
+  else if (nt == 2) L(2);
+  else if (nt == 3) L(3);
+  else L(4);
+#undef L
   HIP_CHECK(hipGetLastError());
   const int tot = M * N;
-  hipLaunchKernelGGL(skinny_tn_reduce, dim3((tot + 255) / 256), dim3(256), 0, st, nblk, M, N, ws.get(), C, ldc, alpha,
-                     beta);
+  hipLaunchKernelGGL(tn_reduce_kernel, dim3((tot + 63) / 64), dim3(1024), 0, st, splits, M, N, ws.get(), C, ldc,
+                     alpha, beta);
   HIP_CHECK(hipGetLastError());
   return true;
 }

@@ -74,6 +74,26 @@ class Hip:
     def free(self, p):
         self.h.hipFree(C.c_void_p(p))
 
+    def d2d(self, dst, src, nbytes):
+        rc = self.h.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), nbytes, 3)  # hipMemcpyDeviceToDevice
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy D2D failed: {rc}")
+
+
+def copy_calibration(hip, nbytes=1 << 30, reps=3):
+    """Plain device-to-device copies of a known size (runtime blit kernel).
+
+    Under ``rocprofv3 --pmc FETCH_SIZE`` / ``WRITE_SIZE`` the counters read for
+    these dispatches, against the nbytes read and written, give the factor that
+    turns the counters of the sigma kernels into HBM bytes."""
+    a, b = hip.malloc(nbytes), hip.malloc(nbytes)
+    for _ in range(reps):
+        hip.d2d(b, a, nbytes)
+    hip.sync()
+    hip.free(a)
+    hip.free(b)
+    log(f"copy calibration: {reps} device copies of {nbytes} bytes")
+
 
 def timed_sigma(op, hip, dX, dW, nvec, prec, steps, warmup):
     for _ in range(warmup):
@@ -187,6 +207,8 @@ def main():
     ap.add_argument("--skip-davidson", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-lowprec", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run device copies of 1 GiB (scale factor for rocprofv3 --pmc byte counters)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -229,6 +251,8 @@ def main():
                 f"{r2['achieved']} TFLOP/s")
     hip.free(dX)
     hip.free(dW)
+    if args.copy_calibration:
+        copy_calibration(hip)
 
     cpu = None
     if not args.skip_cpu:
