@@ -5,15 +5,47 @@ namespace adc2g {
 
 namespace {
 
+// Output element type of the direct map epilogues: the down contraction writes
+// Y in the GEMM precision, the W build writes FP64 (FP64 GEMMs) or FP32.
+template <class T> struct WOut { typedef float type; };
+template <> struct WOut<double> { typedef double type; };
+
+template <class T> constexpr int prec_code() { return sizeof(T) == 8 ? 0 : (sizeof(T) == 4 ? 1 : 2); }
+
+// Maps with a direct (no split-K) epilogue instantiation, given the output type.
+template <class T>
+bool direct_map(const GemmArgs& g) {
+  switch (g.map.kind) {
+    case MAP_STD: return true;
+    case MAP_DOWNY: return g.out_prec == prec_code<T>();
+    case MAP_WUPV:
+    case MAP_WUPO: return g.out_prec == (sizeof(T) == 8 ? 0 : 1);
+    default: return false;
+  }
+}
+
+template <class T, int BM, int BN, int BK, class OutT, int TAG, int MK>
+void launch_k(const GemmArgs& g, dim3 grid, hipStream_t st) {
+  if (g.vec)
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, OutT, TAG, true, MK>), grid, dim3(256), 0, st, g);
+  else
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, OutT, TAG, false, MK>), grid, dim3(256), 0, st, g);
+}
+
 template <class T, int BM, int BN, int BK>
 void launch(const GemmArgs& g, hipStream_t st) {
   const int tiles = ((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
   dim3 grid(tiles, g.batch, g.splitk);
+  const int mk = g.direct ? g.map.kind : MAP_STD;
   if (g.ev_begin) HIP_CHECK(hipEventRecord(g.ev_begin, st));
-  if (g.tag == 1)
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, double, 1>), grid, dim3(256), 0, st, g);
-  else
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, double, 0>), grid, dim3(256), 0, st, g);
+  switch (mk) {
+    case MAP_DOWNY: launch_k<T, BM, BN, BK, T, 1, MAP_DOWNY>(g, grid, st); break;
+    case MAP_WUPV: launch_k<T, BM, BN, BK, typename WOut<T>::type, 0, MAP_WUPV>(g, grid, st); break;
+    case MAP_WUPO: launch_k<T, BM, BN, BK, typename WOut<T>::type, 0, MAP_WUPO>(g, grid, st); break;
+    default:
+      if (g.tag == 1) launch_k<T, BM, BN, BK, double, 1, MAP_STD>(g, grid, st);
+      else launch_k<T, BM, BN, BK, double, 0, MAP_STD>(g, grid, st);
+  }
   HIP_CHECK(hipGetLastError());
   if (g.ev_end) HIP_CHECK(hipEventRecord(g.ev_end, st));
 }
@@ -42,7 +74,17 @@ void dispatch(GemmArgs g, GemmWorkspace& ws, hipStream_t st, int splitk_hint) {
     split = (ktiles + per - 1) / per;
   }
   g.splitk = split;
-  const bool need_partial = (split > 1 || g.map.kind != MAP_STD || g.out_prec != 0);
+  auto aligned = [](const void* p, int64_t stride, int64_t bstride) {
+    return ((uintptr_t)p % 16 == 0) && ((stride * (int64_t)sizeof(T)) % 16 == 0) &&
+           ((bstride * (int64_t)sizeof(T)) % 16 == 0);
+  };
+  constexpr int EV = 16 / sizeof(T);
+  g.vec = (g.sak == 1 && g.sbk == 1 && g.K % EV == 0 && aligned(g.A, g.sam, g.sab) && aligned(g.B, g.sbn, g.sbb))
+              ? 1 : 0;
+  // without split-K the map is applied in-kernel when an instantiation has it,
+  // otherwise by the reduction kernel from the partial
+  g.direct = (split == 1 && direct_map<T>(g)) ? 1 : 0;
+  const bool need_partial = !g.direct;
   if (need_partial) {
     const size_t need = (size_t)g.batch * split * (size_t)g.M * g.N * sizeof(Acc);
     ws.buf.alloc(need);
@@ -54,7 +96,7 @@ void dispatch(GemmArgs g, GemmWorkspace& ws, hipStream_t st, int splitk_hint) {
     launch<T, 128, 128, BK>(g, st);
   if (need_partial) {
     const int64_t total = (int64_t)g.M * g.N * g.batch;
-    if (split >= 8 && total <= (int64_t)1 << 22) {
+    if (split >= 8 && total <= (int64_t)1 << 22) {  // many splits, few outputs
       const int wb = (int)((total + 63) / 64);
       if (g.out_prec == 1)
         hipLaunchKernelGGL((splitk_reduce_wide_kernel<Acc, float>), dim3(wb), dim3(256), 0, st, g);

@@ -47,6 +47,11 @@ struct GemmArgs {
   int64_t sbk = 0, sbn = 0, sbb = 0;
   MapParams map;
   int out_prec = 0;         // output element type: 0 double, 1 float, 2 bfloat16
+  // set by the launcher: both operands' rows are K-contiguous, 16-byte aligned
+  // and K is a whole number of 16-byte chunks, so tiles load with 16-byte
+  // vectors (the VEC instantiation)
+  int vec = 0;
+  int direct = 0;           // set by the launcher: apply the map in the kernel epilogue
   void* partial = nullptr;  // split-K workspace (accumulator type), batch*splitk*M*N
   // host-side only: kernel tag (1 = the DF coupling contractions, compiled as a
   // separately named kernel so profilers report them apart) and optional
@@ -127,6 +132,41 @@ __device__ __forceinline__ void map_store(const MapParams& mp, int64_t m, int64_
   }
 }
 
+// map_store for a map kind fixed at compile time (the direct epilogue of the
+// GEMM kernel instantiated for that map).
+template <int MK, class OutT>
+__device__ __forceinline__ void map_store_k(const MapParams& mp, int64_t m, int64_t n, int64_t b, double val) {
+  int64_t o1 = 0, o2 = -1;
+  if constexpr (MK == MAP_DOWNY) {
+    const int64_t o = mp.q[0], nvec = mp.q[1], n1 = mp.q[2], naux = mp.q[3];
+    const int64_t vec = n / n1, ck = n - vec * n1;
+    const int64_t c = ck / o, k = ck - c * o;
+    o1 = k + o * vec + o * nvec * c + n1 * nvec * m;
+    o2 = k + o * m + o * naux * (c * nvec + vec);
+  } else if constexpr (MK == MAP_WUPV) {
+    const int64_t o = mp.q[0], v = mp.q[1], n1 = mp.q[2], nvec = mp.q[3];
+    const int64_t P = m / v, c = m - P * v;
+    const int64_t vec = n / o, k = n - vec * o;
+    o1 = k + o * c + n1 * vec + n1 * nvec * P;
+  } else {
+    static_assert(MK == MAP_WUPO, "map kind without a direct epilogue");
+    const int64_t o = mp.q[0], n1 = mp.q[2], nvec = mp.q[3];
+    const int64_t c = m / nvec, vec = m - c * nvec;
+    const int64_t P = n / o, k = n - P * o;
+    o1 = k + o * c + n1 * vec + n1 * nvec * P;
+  }
+  OutT* C = (OutT*)mp.C;
+  double r = mp.alpha * val;
+  if (mp.beta != 0.0) r += mp.beta * to_double(C[o1]);
+  C[o1] = from_double<OutT>(r);
+  if (o2 >= 0) {
+    OutT* C2 = (OutT*)mp.C2;
+    double r2 = mp.alpha * val;
+    if (mp.beta != 0.0) r2 += mp.beta * to_double(C2[o2]);
+    C2[o2] = from_double<OutT>(r2);
+  }
+}
+
 // ---- per-type MFMA traits ---------------------------------------------------
 // KV: K elements per lane per MFMA; KM: K per MFMA (= 4*KV).
 template <class T>
@@ -182,7 +222,7 @@ __device__ __forceinline__ int xcd_tile(int w, int T) {
   return x * per + (x < rem ? x : rem) + slot;
 }
 
-template <class T, int BM, int BN, int BK, class OutT, int TAG>
+template <class T, int BM, int BN, int BK, class OutT, int TAG, bool VEC, int MK>
 __global__ void __launch_bounds__(256, 2)
 gemm_kernel(GemmArgs g) {
   typedef Mma<T> MT;
@@ -199,8 +239,8 @@ gemm_kernel(GemmArgs g) {
   constexpr int EB = BN * BK / 256;
   static_assert(BK % KM == 0, "BK must be a multiple of the MFMA K");
 
-  __shared__ T sA[2][BM * LDR];
-  __shared__ T sB[2][BN * LDR];
+  __shared__ __attribute__((aligned(16))) T sA[2][BM * LDR];
+  __shared__ __attribute__((aligned(16))) T sB[2][BN * LDR];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -230,7 +270,7 @@ gemm_kernel(GemmArgs g) {
   const bool a_kc = (g.sak == 1);
   const bool b_kc = (g.sbk == 1);
 
-  T ra[EA], rb[EB];
+  T ra[VEC ? 1 : EA], rb[VEC ? 1 : EB];
 
   // Per-thread fixed coordinates of the load pattern.  K-contiguous: the
   // thread keeps column c and walks rows r0 + e*(256/BK).  MN-contiguous: it
@@ -242,9 +282,40 @@ gemm_kernel(GemmArgs g) {
   const int ma_r = tid % BM, ma_c = tid / BM;
   const int nb_r = tid % BN, nb_c = tid / BN;
 
+  // 16-byte vector loads of K-contiguous operands: each row of a K tile is
+  // BK*sizeof(T) = 128 bytes = CPR chunks; thread t takes chunk t % CPR of
+  // rows t / CPR + e * RPP.
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  constexpr int EV = 16 / sizeof(T);
+  constexpr int CPR = BK / EV;
+  constexpr int RPP = 256 / CPR;
+  constexpr int VA = (BM + RPP - 1) / RPP, VB = (BN + RPP - 1) / RPP;
+  static_assert(BK % EV == 0 && 256 % CPR == 0, "vector load tiling");
+  const int vc = tid % CPR, vr = tid / CPR;
+  u32x4 va[VEC ? VA : 1], vb[VEC ? VB : 1];
+  auto vload_a = [&](int kt) {
+    const int k = kt * BK + vc * EV;
+#pragma unroll
+    for (int e = 0; e < VA; ++e) {
+      const int m = m0 + vr + e * RPP;
+      va[e] = (vr + e * RPP < BM && m < g.M && k < g.K) ? *(const u32x4*)(A + (int64_t)m * g.sam + k)
+                                                         : (u32x4){0u, 0u, 0u, 0u};
+    }
+  };
+  auto vload_b = [&](int kt) {
+    const int k = kt * BK + vc * EV;
+#pragma unroll
+    for (int e = 0; e < VB; ++e) {
+      const int n = n0 + vr + e * RPP;
+      vb[e] = (vr + e * RPP < BN && n < g.N && k < g.K) ? *(const u32x4*)(B + (int64_t)n * g.sbn + k)
+                                                         : (u32x4){0u, 0u, 0u, 0u};
+    }
+  };
   auto load_a = [&](int kt) {
     const int k0 = kt * BK;
-    if (a_kc) {
+    if constexpr (VEC) {
+      vload_a(kt);
+    } else if (a_kc) {
       const int k = k0 + kc_c;
       const T* p = A + (int64_t)(m0 + kc_r) * g.sam + k;
 #pragma unroll
@@ -264,7 +335,9 @@ gemm_kernel(GemmArgs g) {
   };
   auto load_b = [&](int kt) {
     const int k0 = kt * BK;
-    if (b_kc) {
+    if constexpr (VEC) {
+      vload_b(kt);
+    } else if (b_kc) {
       const int k = k0 + kc_c;
       const T* p = B + (int64_t)(n0 + kc_r) * g.sbn + k;
 #pragma unroll
@@ -289,8 +362,22 @@ gemm_kernel(GemmArgs g) {
     const int s = c / KM, rem = c % KM, gq = rem / KV, e = rem % KV;
     return gq * (KS * KV) + s * KV + e;
   };
+  // chunk -> LDS: element j of the chunk is logical k = vc*EV + j
+  auto vstore = [&](T* base, const u32x4& v) {
+    if constexpr (EV == KV) {
+      *(u32x4*)(base + kpos(vc * EV)) = v;  // the chunk is one lane operand vector
+    } else {
+      const T* el = (const T*)&v;
+#pragma unroll
+      for (int j = 0; j < EV; ++j) base[kpos(vc * EV + j)] = el[j];
+    }
+  };
   auto store_a = [&](int buf) {
-    if (a_kc) {
+    if constexpr (VEC) {
+#pragma unroll
+      for (int e = 0; e < VA; ++e)
+        if (vr + e * RPP < BM) vstore(&sA[buf][(vr + e * RPP) * LDR], va[e]);
+    } else if (a_kc) {
       const int kp = kpos(kc_c);
 #pragma unroll
       for (int e = 0; e < EA; ++e) sA[buf][(kc_r + e * RSA) * LDR + kp] = ra[e];
@@ -300,7 +387,11 @@ gemm_kernel(GemmArgs g) {
     }
   };
   auto store_b = [&](int buf) {
-    if (b_kc) {
+    if constexpr (VEC) {
+#pragma unroll
+      for (int e = 0; e < VB; ++e)
+        if (vr + e * RPP < BN) vstore(&sB[buf][(vr + e * RPP) * LDR], vb[e]);
+    } else if (b_kc) {
       const int kp = kpos(kc_c);
 #pragma unroll
       for (int e = 0; e < EB; ++e) sB[buf][(kc_r + e * RSA) * LDR + kp] = rb[e];
@@ -353,11 +444,11 @@ gemm_kernel(GemmArgs g) {
     }
   }
 
-  // epilogue.  Only the plain strided map is applied in-kernel; split-K and
-  // the permuting maps write a row-major partial that the reduction kernel
-  // folds through the map (a small body keeps the loops fully unrolled, so
-  // every index into acc stays compile-time constant).
-  const bool direct = (g.splitk == 1 && g.map.kind == MAP_STD && g.out_prec == 0);
+  // epilogue.  Without split-K the tile goes straight through the map this
+  // instantiation was compiled for (MK); split-K writes a row-major partial
+  // that the reduction kernel folds through the map.  The loops stay fully
+  // unrolled, so every index into acc is a compile-time constant.
+  const bool direct = (g.direct != 0);
   Acc* part = (Acc*)g.partial + ((int64_t)b * g.splitk + ks) * (int64_t)g.M * g.N;
   OutT* C = (OutT*)g.map.C;
   const double alpha = g.map.alpha, beta = g.map.beta;
@@ -370,11 +461,29 @@ gemm_kernel(GemmArgs g) {
         const int m = m0 + wm * WM + i * 16 + MT::row(lane, r);
         const int n = n0 + wn * WN + j * 16 + li;
         if (m < g.M && n < g.N) {
-          if (direct) {
+          if constexpr (MK != MAP_STD) {
+            if (direct) {
+              map_store_k<MK, OutT>(g.map, m, n, b, (double)acc[i][j][r]);
+            } else {
+              part[(int64_t)m * g.N + n] = acc[i][j][r];
+            }
+          } else if (direct) {
             const int64_t off = (int64_t)m * g.map.scm + (int64_t)n * g.map.scn + (int64_t)b * g.map.scb;
-            double val = alpha * (double)acc[i][j][r];
-            if (beta != 0.0) val += beta * (double)C[off];
-            C[off] = (OutT)val;
+            if (g.out_prec == 0) {
+              double val = alpha * (double)acc[i][j][r];
+              if (beta != 0.0) val += beta * (double)C[off];
+              C[off] = (OutT)val;
+            } else if (g.out_prec == 1) {
+              float* Cf = (float*)g.map.C;
+              double val = alpha * (double)acc[i][j][r];
+              if (beta != 0.0) val += beta * (double)Cf[off];
+              Cf[off] = (float)val;
+            } else {
+              __bf16* Ch = (__bf16*)g.map.C;
+              double val = alpha * (double)acc[i][j][r];
+              if (beta != 0.0) val += beta * (double)(float)Ch[off];
+              Ch[off] = (__bf16)(float)val;
+            }
           } else {
             part[(int64_t)m * g.N + n] = acc[i][j][r];
           }

@@ -76,6 +76,7 @@ void Handle::create(int o, int v, int naux, const double* eps_h, const double* B
   launch_extract_factors(dB.get(), nbas, dk, Bov.get(), Bvv.get(), Boo.get(), st);
   build_phph(dB.get(), nbas);
   HIP_CHECK(hipStreamSynchronize(st));
+  ensure_lowprec(Prec::F64);
   if (opts.keep_lowprec) {
     ensure_lowprec(Prec::F32);
     ensure_lowprec(Prec::BF16);
@@ -86,17 +87,44 @@ void Handle::create(int o, int v, int naux, const double* eps_h, const double* B
   while (chunk > 1 && sigma_workspace_bytes(chunk, Prec::F64) > budget) chunk /= 2;
 }
 
+static int prec_index(Prec p) { return p == Prec::F64 ? 0 : (p == Prec::F32 ? 1 : 2); }
+
+// Build the factor set of precision p (once).
 void Handle::ensure_lowprec(Prec p) {
-  const int64_t n = (int64_t)d.naux * d.n1;
-  if (p == Prec::F32 && !Bov32.get()) {
-    Bov32.alloc(n);
-    launch_convert_f32(Bov.get(), Bov32.get(), n, st);
+  FactorSet& f = fs[prec_index(p)];
+  if (f.ready) return;
+  const int64_t n1 = d.n1, naux = d.naux;
+  const int64_t nov = naux * n1, nvv = (int64_t)naux * d.v * d.v, noo = (int64_t)naux * d.o * d.o;
+  if (p == Prec::F64) {
+    f.BovT.alloc(sizeof(double) * nov);
+    launch_transpose<double, double>(Bov.get(), naux, n1, (double*)f.BovT.get(), st);
+  } else if (p == Prec::F32) {
+    f.Bov.alloc(sizeof(float) * nov);
+    f.BovT.alloc(sizeof(float) * nov);
+    f.Bvv.alloc(sizeof(float) * nvv);
+    f.Boo.alloc(sizeof(float) * noo);
+    launch_convert_f32(Bov.get(), (float*)f.Bov.get(), nov, st);
+    launch_transpose<double, float>(Bov.get(), naux, n1, (float*)f.BovT.get(), st);
+    launch_convert_f32(Bvv.get(), (float*)f.Bvv.get(), nvv, st);
+    launch_convert_f32(Boo.get(), (float*)f.Boo.get(), noo, st);
+  } else {
+    f.Bov.alloc(sizeof(__bf16) * nov);
+    f.BovT.alloc(sizeof(__bf16) * nov);
+    f.Bvv.alloc(sizeof(__bf16) * nvv);
+    f.Boo.alloc(sizeof(__bf16) * noo);
+    launch_convert_bf16(Bov.get(), (__bf16*)f.Bov.get(), nov, st);
+    launch_transpose<double, __bf16>(Bov.get(), naux, n1, (__bf16*)f.BovT.get(), st);
+    launch_convert_bf16(Bvv.get(), (__bf16*)f.Bvv.get(), nvv, st);
+    launch_convert_bf16(Boo.get(), (__bf16*)f.Boo.get(), noo, st);
   }
-  if (p == Prec::BF16 && !Bov16.get()) {
-    Bov16.alloc(n);
-    launch_convert_bf16(Bov.get(), Bov16.get(), n, st);
-  }
+  HIP_CHECK(hipStreamSynchronize(st));
+  f.ready = true;
 }
+
+const void* Handle::fBov(Prec p) const { return p == Prec::F64 ? (const void*)Bov.get() : fs[prec_index(p)].Bov.get(); }
+const void* Handle::fBovT(Prec p) const { return fs[prec_index(p)].BovT.get(); }
+const void* Handle::fBvv(Prec p) const { return p == Prec::F64 ? (const void*)Bvv.get() : fs[prec_index(p)].Bvv.get(); }
+const void* Handle::fBoo(Prec p) const { return p == Prec::F64 ? (const void*)Boo.get() : fs[prec_index(p)].Boo.get(); }
 
 // ph-ph block from the factors (first order + CA, CB, CC as products with
 // separable denominators; the algebra is stated in kernels.hip).
@@ -180,33 +208,51 @@ void Handle::build_phph(const double* dB, int nbas) {
 }
 
 size_t Handle::sigma_workspace_bytes(int nc, Prec p) const {
-  const size_t n1 = d.n1, naux = d.naux;
-  size_t b = n1 * n1 * nc * prec_size(p);        // Z / G
-  b += 3 * naux * n1 * nc * sizeof(double);      // Yvv, Yoo, Wup
+  const size_t n1 = d.n1, naux = d.naux, es = prec_size(p);
+  size_t b = n1 * n1 * nc * es;                  // Z / G
+  b += 2 * naux * n1 * nc * es;                  // Yvv, Yoo
+  b += naux * n1 * nc * (p == Prec::F64 ? 8 : 4);  // W
+  b += naux * n1 * nc * es;                      // W transposed
   b += naux * n1 * nc * sizeof(double);          // GEMM partials
-  if (p != Prec::F64) b += naux * n1 * nc * prec_size(p);
   return b;
 }
 
 // W = H X for nvec columns (device pointers, ld = ldx / ldw).
+//
+// Per chunk of nc vectors, with T the GEMM precision p (the 1h1p-1h1p block
+// H11 x1 always FP64; every coupling contraction in T with FP32 accumulation
+// for FP32/BF16):
+//   Z    = pairs(x2)                              [vec][(c,k)][(b,l)]   (pack)
+//   s1   = H11 x1                                                      (FP64)
+//   Y    = Bov . Z   -> Yvv [(c)][(P)]..., Yoo                         (down)
+//   s1  += Bvv . Yvv - Yoo . Boo
+//   W    = Bvv . x1 - x1 . Boo                    [P][vec][(c,k)]
+//   Wt   = W^T                                    [(vec,c,k)][P]
+//   G    = Wt . BovT^T                            [vec][(c,k)][(b,l)]   (up)
+//   s2   = projection of G onto the CSFs + D x2                        (unpack)
 void Handle::sigma(const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, Prec p, hipStream_t s) {
   ARG_CHECK(ldx >= d.ndim && ldw >= d.ndim, "leading dimensions must be >= ndim");
   if (nvec <= 0) return;
   if (!s) s = st;
-  if (p != Prec::F64) ensure_lowprec(p);
+  ensure_lowprec(p);
   const int o = d.o, v = d.v, naux = d.naux;
   const int64_t n1 = d.n1;
   const int nc_max = std::min(chunk, nvec);
-  X1c.alloc(n1 * nc_max);
-  Yvv.alloc((size_t)naux * n1 * nc_max);
-  Yoo.alloc((size_t)naux * n1 * nc_max);
-  Wup.alloc((size_t)naux * n1 * nc_max);
-  ZG.alloc((size_t)n1 * n1 * nc_max * prec_size(p));
-  if (p != Prec::F64) Wup_lp.alloc((size_t)naux * n1 * nc_max * prec_size(p));
-  const int out_lp = (p == Prec::F32) ? 1 : (p == Prec::BF16 ? 2 : 0);
-  const void* BovP = (p == Prec::F64) ? (const void*)Bov.get()
-                     : (p == Prec::F32 ? (const void*)Bov32.get() : (const void*)Bov16.get());
-  const double es = (double)prec_size(p);
+  const size_t es = prec_size(p);
+  const size_t wes = (p == Prec::F64) ? 8 : 4;  // W accumulator type
+  const int out_lp = (p == Prec::F64) ? 0 : (p == Prec::F32 ? 1 : 2);
+  const int out_w = (p == Prec::F64) ? 0 : 1;
+  X1c.alloc(n1 * nc_max * es);
+  Yvv.alloc((size_t)naux * n1 * nc_max * es);
+  Yoo.alloc((size_t)naux * n1 * nc_max * es);
+  Wup.alloc((size_t)naux * n1 * nc_max * wes);
+  WupT.alloc((size_t)naux * n1 * nc_max * es);
+  ZG.alloc((size_t)n1 * n1 * nc_max * es);
+  const void* BovP = fBov(p);
+  const void* BovTP = fBovT(p);
+  const void* BvvP = fBvv(p);
+  const void* BooP = fBoo(p);
+  const double esd = (double)es;
   auto span = [&](int cls, double work) -> std::pair<hipEvent_t, hipEvent_t> {
     if (!prof_on) return {nullptr, nullptr};
     hipEvent_t a = prof_event(), b = prof_event();
@@ -217,16 +263,24 @@ void Handle::sigma(const double* X, int64_t ldx, double* W, int64_t ldw, int nve
     const int nc = std::min(nc_max, nvec - c0);
     const double* Xc = X + ldx * c0;
     double* Wc = W + ldw * c0;
-    launch_pack_x1<double>(Xc, ldx, nc, dk, X1c.get(), s);
-    auto ep = span(1, (double)nc * (d.n2 * 8.0 + (double)n1 * n1 * es));
+    auto ep = span(1, (double)nc * (d.n2 * 8.0 + (double)n1 * n1 * esd));
     if (ep.first) HIP_CHECK(hipEventRecord(ep.first, s));
     switch (p) {
-      case Prec::F64: launch_csf_to_pair<double>(Xc, ldx, nc, dk, (double*)ZG.get(), s); break;
-      case Prec::F32: launch_csf_to_pair<float>(Xc, ldx, nc, dk, (float*)ZG.get(), s); break;
-      case Prec::BF16: launch_csf_to_pair<__bf16>(Xc, ldx, nc, dk, (__bf16*)ZG.get(), s); break;
+      case Prec::F64:
+        launch_csf_to_pair<double>(Xc, ldx, nc, dk, (double*)ZG.get(), s);
+        launch_pack_x1<double>(Xc, ldx, nc, dk, (double*)X1c.get(), s);
+        break;
+      case Prec::F32:
+        launch_csf_to_pair<float>(Xc, ldx, nc, dk, (float*)ZG.get(), s);
+        launch_pack_x1<float>(Xc, ldx, nc, dk, (float*)X1c.get(), s);
+        break;
+      case Prec::BF16:
+        launch_csf_to_pair<__bf16>(Xc, ldx, nc, dk, (__bf16*)ZG.get(), s);
+        launch_pack_x1<__bf16>(Xc, ldx, nc, dk, (__bf16*)X1c.get(), s);
+        break;
     }
     if (ep.second) HIP_CHECK(hipEventRecord(ep.second, s));
-    // sigma1 = H11 x1
+    // sigma1 = H11 x1 (FP64)
     {
       GemmArgs g;
       g.M = (int)n1; g.N = nc; g.K = (int)n1;
@@ -243,6 +297,7 @@ void Handle::sigma(const double* X, int64_t ldx, double* W, int64_t ldw, int nve
       g.B = ZG.get(); g.sbk = 1; g.sbn = n1;
       g.map.kind = MAP_DOWNY; g.map.C = Yvv.get(); g.map.C2 = Yoo.get();
       g.map.q[0] = o; g.map.q[1] = nc; g.map.q[2] = n1; g.map.q[3] = naux;
+      g.out_prec = out_lp;
       g.tag = 1;
       auto e = span(0, 2.0 * naux * (double)n1 * n1 * nc);
       g.ev_begin = e.first; g.ev_end = e.second;
@@ -252,50 +307,61 @@ void Handle::sigma(const double* X, int64_t ldx, double* W, int64_t ldw, int nve
     {
       GemmArgs g;
       g.M = v; g.N = o * nc; g.K = v * naux;
-      g.A = Bvv.get(); g.sam = 1; g.sak = v;
+      g.A = BvvP; g.sam = 1; g.sak = v;
       g.B = Yvv.get(); g.sbk = (int64_t)o * nc; g.sbn = 1;
       g.map.kind = MAP_SIG1V; g.map.C = Wc; g.map.beta = 1.0;
       g.map.q[0] = o; g.map.q[1] = ldw;
-      gemm(g, Prec::F64, gws, s);
+      gemm(g, p, gws, s);
     }
     // sigma1(a,k) -= sum_{P,m} B[P,k,m] Y[P,a,m]
     {
       GemmArgs g;
       g.M = v * nc; g.N = o; g.K = o * naux;
       g.A = Yoo.get(); g.sam = (int64_t)o * naux; g.sak = 1;
-      g.B = Boo.get(); g.sbk = o; g.sbn = 1;
+      g.B = BooP; g.sbk = o; g.sbn = 1;
       g.map.kind = MAP_SIG1O; g.map.C = Wc; g.map.alpha = -1.0; g.map.beta = 1.0;
       g.map.q[0] = o; g.map.q[1] = ldw; g.map.q[2] = nc;
-      gemm(g, Prec::F64, gws, s);
+      gemm(g, p, gws, s);
     }
-    // up: W[P,c,k] = sum_a B[P,c,a] x1(a,k) - sum_m B[P,k,m] x1(c,m)
-    void* wup_out = (p == Prec::F64) ? (void*)Wup.get() : (void*)Wup_lp.get();
+    // W[P,vec,c,k] = sum_a B[P,c,a] x1(a,k) - sum_m B[P,k,m] x1(c,m)
     {
       GemmArgs g;
       g.M = v * naux; g.N = o * nc; g.K = v;
-      g.A = Bvv.get(); g.sam = v; g.sak = 1;
+      g.A = BvvP; g.sam = v; g.sak = 1;
       g.B = X1c.get(); g.sbk = (int64_t)o * nc; g.sbn = 1;
-      g.map.kind = MAP_WUPV; g.map.C = wup_out;
+      g.map.kind = MAP_WUPV; g.map.C = Wup.get();
       g.map.q[0] = o; g.map.q[1] = v; g.map.q[2] = n1; g.map.q[3] = nc;
-      g.out_prec = out_lp;
-      gemm(g, Prec::F64, gws, s);
+      g.out_prec = out_w;
+      gemm(g, p, gws, s);
     }
     {
       GemmArgs g;
       g.M = v * nc; g.N = o * naux; g.K = o;
       g.A = X1c.get(); g.sam = o; g.sak = 1;
-      g.B = Boo.get(); g.sbk = 1; g.sbn = o;
-      g.map.kind = MAP_WUPO; g.map.C = wup_out; g.map.alpha = -1.0; g.map.beta = 1.0;
+      g.B = BooP; g.sbk = 1; g.sbn = o;
+      g.map.kind = MAP_WUPO; g.map.C = Wup.get(); g.map.alpha = -1.0; g.map.beta = 1.0;
       g.map.q[0] = o; g.map.q[1] = v; g.map.q[2] = n1; g.map.q[3] = nc;
-      g.out_prec = out_lp;
-      gemm(g, Prec::F64, gws, s);
+      g.out_prec = out_w;
+      gemm(g, p, gws, s);
     }
-    // G[(c,k),(b,l)] = sum_P W[P,(c,k)] B[P,(b,l)]
+    // Wt[(vec,c,k)][P] = W[P][(vec,c,k)] in the GEMM precision
+    switch (p) {
+      case Prec::F64:
+        launch_transpose<double, double>((const double*)Wup.get(), naux, n1 * nc, (double*)WupT.get(), s);
+        break;
+      case Prec::F32:
+        launch_transpose<float, float>((const float*)Wup.get(), naux, n1 * nc, (float*)WupT.get(), s);
+        break;
+      case Prec::BF16:
+        launch_transpose<float, __bf16>((const float*)Wup.get(), naux, n1 * nc, (__bf16*)WupT.get(), s);
+        break;
+    }
+    // up: G[(vec,c,k),(b,l)] = sum_P Wt[(vec,c,k),P] BovT[(b,l),P]  (both K-contiguous)
     {
       GemmArgs g;
       g.M = (int)(n1 * nc); g.N = (int)n1; g.K = naux;
-      g.A = wup_out; g.sam = 1; g.sak = n1 * nc;
-      g.B = BovP; g.sbk = n1; g.sbn = 1;
+      g.A = WupT.get(); g.sam = naux; g.sak = 1;
+      g.B = BovTP; g.sbk = 1; g.sbn = naux;
       g.map.C = ZG.get(); g.map.scm = n1; g.map.scn = 1;
       g.out_prec = out_lp;
       g.tag = 1;
@@ -303,7 +369,7 @@ void Handle::sigma(const double* X, int64_t ldx, double* W, int64_t ldw, int nve
       g.ev_begin = e.first; g.ev_end = e.second;
       gemm(g, p, gws, s);
     }
-    auto eu = span(2, (double)nc * (3.0 * d.n2 * 8.0 + (double)n1 * n1 * es));
+    auto eu = span(2, (double)nc * (3.0 * d.n2 * 8.0 + (double)n1 * n1 * esd));
     if (eu.first) HIP_CHECK(hipEventRecord(eu.first, s));
     switch (p) {
       case Prec::F64:

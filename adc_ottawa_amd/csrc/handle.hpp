@@ -21,14 +21,26 @@ struct Handle {
 
   // operator (FP64, always present)
   DevBuf<double> eps, Bov, Bvv, Boo, H11, hii;
-  // low-precision copies of the coupling factor (mixed-precision sigma)
-  DevBuf<float> Bov32;
-  DevBuf<__bf16> Bov16;
+  // coupling factors in each GEMM precision (0 FP64, 1 FP32, 2 BF16): Bov
+  // [P][(a,k)], its transpose BovT [(a,k)][P] (K-contiguous operand of the up
+  // contraction), Bvv [P][a][b], Boo [P][k][l].  The FP64 set holds only BovT
+  // and otherwise uses the buffers above.
+  struct FactorSet {
+    DevBuf<char> Bov, BovT, Bvv, Boo;
+    bool ready = false;
+  };
+  FactorSet fs[3];
+  const void* fBov(Prec p) const;
+  const void* fBovT(Prec p) const;
+  const void* fBvv(Prec p) const;
+  const void* fBoo(Prec p) const;
 
-  // sigma workspaces (grown on demand, sized per vector chunk)
-  DevBuf<double> X1c, Yvv, Yoo, Wup;
+  // sigma workspaces (grown on demand, sized per vector chunk), typed by the
+  // chunk's GEMM precision
+  DevBuf<char> X1c, Yvv, Yoo;
+  DevBuf<char> Wup;         // W[P][vec][(c,k)] (accumulator type: FP64, or FP32 for FP32/BF16)
+  DevBuf<char> WupT;        // W transposed, [(vec,c,k)][P], GEMM precision
   DevBuf<char> ZG;          // Z then G, in the chunk's GEMM precision
-  DevBuf<char> Wup_lp;      // Wup in low precision for the up GEMM
   GemmWorkspace gws;
   DevBuf<double> skinny_ws;  // per-block partials of the tall-skinny products
   int chunk = 32;           // vectors per sigma pass

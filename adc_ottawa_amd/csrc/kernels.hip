@@ -69,6 +69,62 @@ __global__ void convert_bf16_kernel(const double* __restrict__ s, __bf16* __rest
     d[i] = (__bf16)(float)s[i];
 }
 
+// dst[c * R + r] = src[r * C + c] through 32 x 32 LDS tiles, converting the
+// element type on the way.
+template <class S, class D>
+__global__ void __launch_bounds__(256) transpose_kernel(const S* __restrict__ src, int64_t R, int64_t C,
+                                                        D* __restrict__ dst) {
+  __shared__ double tile[32][33];
+  const int64_t tiles_c = (C + 31) / 32;
+  const int64_t r0 = (blockIdx.x / tiles_c) * 32, c0 = (blockIdx.x % tiles_c) * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int j = ty; j < 32; j += 8) {
+    const int64_t r = r0 + j, c = c0 + tx;
+    tile[j][tx] = (r < R && c < C) ? (double)src[r * C + c] : 0.0;
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    const int64_t c = c0 + j, r = r0 + tx;
+    if (r < R && c < C) dst[c * R + r] = (D)tile[tx][j];
+  }
+}
+
+template <>
+__global__ void __launch_bounds__(256) transpose_kernel<double, __bf16>(const double* __restrict__ src, int64_t R,
+                                                                       int64_t C, __bf16* __restrict__ dst) {
+  __shared__ double tile[32][33];
+  const int64_t tiles_c = (C + 31) / 32;
+  const int64_t r0 = (blockIdx.x / tiles_c) * 32, c0 = (blockIdx.x % tiles_c) * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int j = ty; j < 32; j += 8) {
+    const int64_t r = r0 + j, c = c0 + tx;
+    tile[j][tx] = (r < R && c < C) ? src[r * C + c] : 0.0;
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    const int64_t c = c0 + j, r = r0 + tx;
+    if (r < R && c < C) dst[c * R + r] = (__bf16)(float)tile[tx][j];
+  }
+}
+
+template <>
+__global__ void __launch_bounds__(256) transpose_kernel<float, __bf16>(const float* __restrict__ src, int64_t R,
+                                                                      int64_t C, __bf16* __restrict__ dst) {
+  __shared__ double tile[32][33];
+  const int64_t tiles_c = (C + 31) / 32;
+  const int64_t r0 = (blockIdx.x / tiles_c) * 32, c0 = (blockIdx.x % tiles_c) * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int j = ty; j < 32; j += 8) {
+    const int64_t r = r0 + j, c = c0 + tx;
+    tile[j][tx] = (r < R && c < C) ? (double)src[r * C + c] : 0.0;
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    const int64_t c = c0 + j, r = r0 + tx;
+    if (r < R && c < C) dst[c * R + r] = (__bf16)(float)tile[tx][j];
+  }
+}
+
 template <class T>
 __global__ void pack_x1_kernel(const double* __restrict__ X, int64_t ldx, int nvec, DimsK d, T* __restrict__ X1c) {
   const int64_t total = d.n1 * nvec;
@@ -328,6 +384,20 @@ void launch_convert_bf16(const double* src, __bf16* dst, int64_t n, hipStream_t 
   hipLaunchKernelGGL(convert_bf16_kernel, dim3(grid_for(n)), dim3(256), 0, st, src, dst, n);
   HIP_CHECK(hipGetLastError());
 }
+
+template <class S, class D>
+void launch_transpose(const S* src, int64_t R, int64_t C, D* dst, hipStream_t st) {
+  const int64_t tiles = ((R + 31) / 32) * ((C + 31) / 32);
+  if (tiles <= 0) return;
+  if (tiles > 0x7fffffff) throw Error(ADC2G_EARG, "transpose too large");
+  hipLaunchKernelGGL((transpose_kernel<S, D>), dim3((unsigned)tiles), dim3(256), 0, st, src, R, C, dst);
+  HIP_CHECK(hipGetLastError());
+}
+template void launch_transpose<double, double>(const double*, int64_t, int64_t, double*, hipStream_t);
+template void launch_transpose<double, float>(const double*, int64_t, int64_t, float*, hipStream_t);
+template void launch_transpose<double, __bf16>(const double*, int64_t, int64_t, __bf16*, hipStream_t);
+template void launch_transpose<float, float>(const float*, int64_t, int64_t, float*, hipStream_t);
+template void launch_transpose<float, __bf16>(const float*, int64_t, int64_t, __bf16*, hipStream_t);
 
 template <class T>
 void launch_pack_x1(const double* X, int64_t ldx, int nvec, const DimsK& d, T* X1c, hipStream_t st) {

@@ -26,6 +26,9 @@ void launch_extract_factors(const double* B, int nbas, const DimsK& d, double* B
                             double* Boo, hipStream_t st);
 void launch_convert_f32(const double* src, float* dst, int64_t n, hipStream_t st);
 void launch_convert_bf16(const double* src, __bf16* dst, int64_t n, hipStream_t st);
+// dst[c * R + r] = (D) src[r * C + c]  (R x C row-major -> C x R row-major)
+template <class S, class D>
+void launch_transpose(const S* src, int64_t R, int64_t C, D* dst, hipStream_t st);
 
 // ---- sigma-vector streaming kernels
 // X1c[a][vec][k] <- X[a*o+k + ldx*vec]
