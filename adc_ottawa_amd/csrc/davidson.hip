@@ -71,7 +71,7 @@ struct Dav {
   Handle& H;
   int64_t n;
   int bs, mv;
-  DevBuf<double> V, W, X, WY, Res, T, Rm, Rt, Zs, thd, part, nrm, small, work, M, S1, S2, Gm, cond;
+  DevBuf<double> V, W, X, WY, Res, T, Rm, Rt, Zs, thd, part, nrm, small, work, M, S1, S2, Gm, cond, rpart, tmpcol;
   DevBuf<int> info;
   DevBuf<int64_t> idx;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -184,7 +184,8 @@ struct Dav {
 
   void swap_cols(DevBuf<double>& B, int i, int j) {
     if (i == j) return;
-    double* tmp = T.get() + n * (bs - 1);  // scratch column (T is rebuilt afterwards)
+    tmpcol.alloc(n);
+    double* tmp = tmpcol.get();
     HIP_CHECK(hipMemcpyAsync(tmp, B.get() + n * i, sizeof(double) * n, hipMemcpyDeviceToDevice, H.st));
     HIP_CHECK(hipMemcpyAsync(B.get() + n * i, B.get() + n * j, sizeof(double) * n, hipMemcpyDeviceToDevice, H.st));
     HIP_CHECK(hipMemcpyAsync(B.get() + n * j, tmp, sizeof(double) * n, hipMemcpyDeviceToDevice, H.st));
@@ -261,11 +262,20 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
     HIP_CHECK(hipMemcpy2DAsync(D.Rt.get(), sizeof(double) * mv, D.Rm.get(), sizeof(double) * mv,
                                sizeof(double) * currdim, currdim, hipMemcpyDeviceToDevice, st));
     launch_syevx(currdim, bs_curr, D.Rt.get(), mv, D.thd.get(), D.Zs.get(), mv, D.work.get(), D.info.get(), st);
-    // Ritz vectors and W y (calcritzvec, calcres)
-    gemm_nn(H, n, bs_curr, currdim, D.V.get(), n, D.Zs.get(), mv, D.X.get(), n, 1.0, 0.0);
-    gemm_nn(H, n, bs_curr, currdim, D.W.get(), n, D.Zs.get(), mv, D.WY.get(), n, 1.0, 0.0);
-    const int nb = launch_residual(n, bs_curr, D.thd.get(), D.X.get(), D.WY.get(), D.Res.get(), n, D.part.get(), st);
-    launch_colsum(nb, bs_curr, D.part.get(), D.nrm.get(), st);
+    // Ritz vectors and W y (calcritzvec, calcres), fused with the residuals
+    // and, for DPR, the correction step
+    const bool fused_dpr = (p.precon == 1);
+    int nb = ritz_fused(n, currdim, bs_curr, D.V.get(), D.W.get(), n, D.Zs.get(), mv, D.thd.get(), H.hii.get(),
+                        fused_dpr, D.X.get(), D.WY.get(), D.Res.get(), D.T.get(), D.rpart, st);
+    const bool have_t = fused_dpr && nb > 0;
+    if (nb == 0) {
+      gemm_nn(H, n, bs_curr, currdim, D.V.get(), n, D.Zs.get(), mv, D.X.get(), n, 1.0, 0.0);
+      gemm_nn(H, n, bs_curr, currdim, D.W.get(), n, D.Zs.get(), mv, D.WY.get(), n, 1.0, 0.0);
+      nb = launch_residual(n, bs_curr, D.thd.get(), D.X.get(), D.WY.get(), D.Res.get(), n, D.part.get(), st);
+      launch_colsum(nb, bs_curr, D.part.get(), D.nrm.get(), st);
+    } else {
+      launch_colsum(nb, bs_curr, D.rpart.get(), D.nrm.get(), st);
+    }
     int einfo = 0;
     HIP_CHECK(hipMemcpyAsync(&einfo, D.info.get(), sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipMemcpyAsync(theta.data(), D.thd.get(), sizeof(double) * bs_curr, hipMemcpyDeviceToHost, st));
@@ -336,6 +346,7 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
           D.swap_cols(D.X, indx, src);
           D.swap_cols(D.Res, indx, src);
           D.swap_cols(D.WY, indx, src);
+          if (have_t) D.swap_cols(D.T, indx, src);
           std::swap(theta[indx], theta[src]);
           std::swap(norm[indx], norm[src]);
           stored_norm[indx] = norm[indx];
@@ -359,7 +370,7 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
     // new directions (dpr / olsen / learned)
     HIP_CHECK(hipMemcpyAsync(D.thd.get(), theta.data(), sizeof(double) * bs_curr, hipMemcpyHostToDevice, st));
     if (p.precon == 1) {
-      launch_dpr(n, bs_curr, D.thd.get(), H.hii.get(), D.Res.get(), D.T.get(), n, st);
+      if (!have_t) launch_dpr(n, bs_curr, D.thd.get(), H.hii.get(), D.Res.get(), D.T.get(), n, st);
     } else if (p.precon == 2) {
       const int nb2 = launch_olsen_dots(n, bs_curr, D.thd.get(), H.hii.get(), D.X.get(), D.Res.get(), n,
                                         D.part.get(), st);

@@ -31,7 +31,7 @@ __device__ double block_sum(double x, double* sh) {
 }
 
 struct SyevShared {
-  double d[SYEV_NMAX], e[SYEV_NMAX], rc[SYEV_NMAX], rs[SYEV_NMAX];
+  double d[SYEV_NMAX], e[SYEV_NMAX], rc[SYEV_NMAX], rs[SYEV_NMAX], qv[SYEV_NMAX];
   double red[16];
   int perm[SYEV_NMAX];
   int lo, hi, go, info;
@@ -40,42 +40,54 @@ struct SyevShared {
 // Householder reduction of the symmetric n x n matrix A to tridiagonal form.
 // On exit S.d / S.e hold the diagonal and the off-diagonal (e[n-1] = 0) and
 // column k of A below row k holds reflector k (unit 2-norm, H_k = I - 2 v v^T).
-// Zw needs 2n doubles; S.rc is used as scratch for the matrix-vector product.
+// Four barriers per column: wave 0 forms the reflector (v in S.rs), all waves
+// form p = A22 v (partials in S.rc), wave 0 forms q = 2 (p - (v.p) v) (S.qv),
+// all waves apply A22 <- A22 - v q^T - q v^T.  Zw is unused (kept for the
+// callers' workspace layout).
 __device__ void tridiag_dev(int n, double* A, int lda, double* Zw, SyevShared& S) {
+  (void)Zw;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+  double* vv = S.rs;
+  double* qv = S.qv;
   for (int k = 0; k + 2 < n; ++k) {
     const int m = n - k - 1;
-    double part = 0.0;
-    for (int i = k + 2 + tid; i < n; i += nt) {
-      const double x = A[i + (int64_t)k * lda];
-      part += x * x;
-    }
-    const double x0 = A[k + 1 + (int64_t)k * lda];
-    const double tail = block_sum(part, S.red);
-    if (tail == 0.0) {
-      if (tid == 0) {
-        S.e[k] = x0;
-        A[k + 1 + (int64_t)k * lda] = 0.0;  // no reflection: zero reflector
+    double* col = A + (int64_t)k * lda;
+    if (wv == 0) {
+      double part = 0.0;
+      for (int i = k + 2 + lane; i < n; i += 64) part += col[i] * col[i];
+      for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+      const double x0 = col[k + 1];
+      if (part == 0.0) {
+        for (int i = k + 1 + lane; i < n; i += 64) vv[i] = 0.0;
+        if (lane == 0) {
+          S.e[k] = x0;
+          col[k + 1] = 0.0;  // no reflection: zero reflector
+          S.go = 0;
+        }
+      } else {
+        const double alpha = -copysign(sqrt(part + x0 * x0), x0);
+        const double v0 = x0 - alpha;
+        const double inv = 1.0 / sqrt(part + v0 * v0);
+        for (int i = k + 1 + lane; i < n; i += 64) {
+          const double x = (i == k + 1 ? v0 : col[i]) * inv;
+          vv[i] = x;
+          col[i] = x;
+        }
+        if (lane == 0) {
+          S.e[k] = alpha;
+          S.go = 1;
+        }
       }
-      __syncthreads();
+    }
+    __syncthreads();
+    if (!S.go) {
+      __syncthreads();  // everyone has read S.go before wave 0 rewrites it
       continue;
     }
-    const double sig = tail + x0 * x0;
-    const double alpha = -copysign(sqrt(sig), x0);
-    const double v0 = x0 - alpha;
-    const double inv = 1.0 / sqrt(tail + v0 * v0);
-    __syncthreads();
-    for (int i = k + 1 + tid; i < n; i += nt) {
-      const double x = (i == k + 1) ? v0 : A[i + (int64_t)k * lda];
-      A[i + (int64_t)k * lda] = x * inv;
-    }
-    if (tid == 0) S.e[k] = alpha;
-    __syncthreads();
-    const double* v = A + (int64_t)k * lda;  // v[i] for i in k+1..n-1
-    // p = A22 v: the m rows are split into chunks of 64 (one row per lane,
-    // coalesced over the column-major storage) and the columns into ns
-    // slices, one (chunk, slice) per wave; partial sums meet in S.rc.
+    // p = A22 v: the m rows in chunks of 64 (one row per lane, coalesced over
+    // the column-major storage), the columns in ns slices, one (chunk, slice)
+    // per wave
     {
       const int rch = (m + 63) >> 6;
       const int ns = max(1, nw / rch);
@@ -86,28 +98,28 @@ __device__ void tridiag_dev(int n, double* A, int lda, double* Zw, SyevShared& S
         const int j0 = k + 1 + sl * per, j1 = min(n, j0 + per);
         if (i < n) {
           double s = 0.0;
-          for (int j = j0; j < j1; ++j) s += A[i + (int64_t)j * lda] * v[j];
+          for (int j = j0; j < j1; ++j) s += A[i + (int64_t)j * lda] * vv[j];
           S.rc[sl * m + (i - k - 1)] = s;
         }
       }
       __syncthreads();
-      for (int i = k + 1 + tid; i < n; i += nt) {
-        double s = 0.0;
-        for (int sl = 0; sl < ns; ++sl) s += S.rc[sl * m + (i - k - 1)];
-        Zw[i] = s;
+      if (wv == 0) {
+        double kv = 0.0;
+        for (int i = k + 1 + lane; i < n; i += 64) {
+          double s = 0.0;
+          for (int sl = 0; sl < ns; ++sl) s += S.rc[sl * m + (i - k - 1)];
+          qv[i] = s;  // p for now
+          kv += s * vv[i];
+        }
+        for (int off = 32; off > 0; off >>= 1) kv += __shfl_xor(kv, off, 64);
+        for (int i = k + 1 + lane; i < n; i += 64) qv[i] = 2.0 * (qv[i] - kv * vv[i]);
       }
     }
     __syncthreads();
-    part = 0.0;
-    for (int j = k + 1 + tid; j < n; j += nt) part += v[j] * Zw[j];
-    const double Kv = block_sum(part, S.red);
-    for (int j = k + 1 + tid; j < n; j += nt) Zw[n + j] = 2.0 * (Zw[j] - Kv * v[j]);  // q
-    __syncthreads();
-    const double* q = Zw + n;
     const int64_t mm = (int64_t)m * m;
     for (int64_t idx = tid; idx < mm; idx += nt) {
       const int i = k + 1 + (int)(idx % m), j = k + 1 + (int)(idx / m);
-      A[i + (int64_t)j * lda] -= v[i] * q[j] + q[i] * v[j];
+      A[i + (int64_t)j * lda] -= vv[i] * qv[j] + qv[i] * vv[j];
     }
     __syncthreads();
   }

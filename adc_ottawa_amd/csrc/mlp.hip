@@ -10,9 +10,11 @@
 //   h2 = silu(W2 h1 + b2)                      (64)
 //   m  = w3 . h2 + b3
 //   t_j = d_j (1 + m),  d_j = r_j / (theta_i - H_jj)  (the DPR step)
-// Training minimises mean (dn (1 + m) - en)^2 over harvested records, dn and
-// en being the DPR step and the exact correction x - x*/(x.x*) towards the
-// converged eigenvector x*, both divided by the RMS element of x - x*/(x.x*).
+// Training minimises mean (asinh(dn (1 + m)) - asinh(en))^2 over harvested
+// records, dn and en being the DPR step and the exact correction x - x*/(x.x*)
+// towards the converged eigenvector x*, both divided by the RMS element of
+// x - x*/(x.x*); the asinh keeps the few elements with a near-zero DPR
+// denominator from dominating the mean.
 //
 // Weights are kept as fp32 master copies (Adam) and used rounded to bf16.
 // Training runs one 64-lane wave per record stream in fp32 with the bf16
@@ -86,9 +88,10 @@ __global__ void __launch_bounds__(64) fwdbwd_kernel(const float* __restrict__ P,
     const float h2 = silu(z2);
     const float m = wave_sum(w3 * h2) + b3;
     if (mout && j == 0) mout[rec] = m;
-    const float r = d * (1.0f + m) - e;
+    const float t = d * (1.0f + m);
+    const float r = asinhf(t) - asinhf(e);
     if (j == 0) loss += (double)r * r;
-    const float gm = 2.0f * r * d * inv_n;
+    const float gm = 2.0f * r * d * inv_n / sqrtf(1.0f + t * t);
     gw3 += gm * h2;
     if (j == 0) gb3 += gm;
     const float gz2 = gm * w3 * dsilu(z2);

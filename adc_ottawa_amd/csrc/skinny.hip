@@ -55,6 +55,80 @@ __global__ void __launch_bounds__(NN_THREADS) skinny_nn_kernel(int64_t M, int N,
   }
 }
 
+// Ritz vectors, W y, residuals and (optionally) the DPR step in one pass over
+// the subspace: X = V Z, WY = W Z, Res = X diag(theta) - WY,
+// T = Res / guard(theta - hii); one row per lane, Z staged in LDS, per-block
+// partial sums of Res^2 per column into part[block][col].
+template <int NMAX>
+__global__ void __launch_bounds__(NN_THREADS) ritz_kernel(int64_t n, int c, int nb, const double* __restrict__ V,
+                                                          const double* __restrict__ W, int64_t ld,
+                                                          const double* __restrict__ Z, int ldz,
+                                                          const double* __restrict__ theta,
+                                                          const double* __restrict__ hii, int dpr,
+                                                          double* __restrict__ X, double* __restrict__ WY,
+                                                          double* __restrict__ Res, double* __restrict__ T,
+                                                          double* __restrict__ part) {
+  __shared__ double sZ[NN_KC * NMAX];
+  __shared__ double red[NN_THREADS / 64][NMAX];
+  const int64_t i = blockIdx.x * (int64_t)NN_THREADS + threadIdx.x;
+  const bool row = i < n;
+  double ax[NMAX], aw[NMAX];
+#pragma unroll
+  for (int q = 0; q < NMAX; ++q) ax[q] = aw[q] = 0.0;
+  for (int k0 = 0; k0 < c; k0 += NN_KC) {
+    const int kc = min(NN_KC, c - k0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < kc * NMAX; t += NN_THREADS) {
+      const int k = t / NMAX, q = t % NMAX;
+      sZ[t] = (q < nb) ? Z[(k0 + k) + (int64_t)ldz * q] : 0.0;
+    }
+    __syncthreads();
+    if (row) {
+      const double* v = V + i + ld * k0;
+      const double* w = W + i + ld * k0;
+#pragma unroll 2
+      for (int k = 0; k < kc; ++k) {
+        const double vv = v[ld * k], ww = w[ld * k];
+        const double* z = sZ + k * NMAX;
+#pragma unroll
+        for (int q = 0; q < NMAX; ++q) {
+          ax[q] += vv * z[q];
+          aw[q] += ww * z[q];
+        }
+      }
+    }
+  }
+  const double h = row ? hii[i] : 0.0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NMAX; ++q) {
+    double sq = 0.0;
+    if (q < nb) {
+      const double th = theta[q];
+      const double r = th * ax[q] - aw[q];
+      if (row) {
+        X[i + ld * q] = ax[q];
+        WY[i + ld * q] = aw[q];
+        Res[i + ld * q] = r;
+        if (dpr) {
+          double den = th - h;
+          if (fabs(den) < 1e-10) den = den < 0.0 ? -1e-10 : 1e-10;
+          T[i + ld * q] = r / den;
+        }
+        sq = r * r;
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
+    if (lane == 0) red[wv][q] = sq;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < nb; q += NN_THREADS) {
+    double s = 0.0;
+    for (int w = 0; w < NN_THREADS / 64; ++w) s += red[w][q];
+    part[(int64_t)blockIdx.x * nb + q] = s;
+  }
+}
+
 // C = A^T B with K (= ndim) rows: each workgroup takes 64 columns of A (one
 // 16-column tile per wave) and all N <= 16*NT columns of B over one K-split,
 // streams K in chunks of 64 rows through LDS (each column run is 512
@@ -184,6 +258,24 @@ bool skinny_nn(int64_t M, int N, int K, const double* A, int64_t lda, const doub
 #undef L
   HIP_CHECK(hipGetLastError());
   return true;
+}
+
+int ritz_fused(int64_t n, int c, int nb, const double* V, const double* W, int64_t ld, const double* Z, int ldz,
+               const double* theta, const double* hii, bool dpr, double* X, double* WY, double* Res, double* T,
+               DevBuf<double>& part, hipStream_t st) {
+  if (nb <= 0 || nb > 32 || c <= 0) return 0;
+  const int grid = (int)((n + NN_THREADS - 1) / NN_THREADS);
+  part.alloc((size_t)grid * nb);
+#define L(NM)                                                                                                       \
+  hipLaunchKernelGGL((ritz_kernel<NM>), dim3(grid), dim3(NN_THREADS), 0, st, n, c, nb, V, W, ld, Z, ldz, theta, hii, \
+                     dpr ? 1 : 0, X, WY, Res, T, part.get())
+  if (nb <= 8) L(8);
+  else if (nb <= 16) L(16);
+  else if (nb <= 24) L(24);
+  else L(32);
+#undef L
+  HIP_CHECK(hipGetLastError());
+  return grid;
 }
 
 bool skinny_tn(int M, int N, int64_t K, const double* A, int64_t lda, const double* B, int64_t ldb, double* C,

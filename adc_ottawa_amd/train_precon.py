@@ -1,6 +1,6 @@
 """Train the learned Davidson preconditioner on the GPU.
 
-    python -m adc_ottawa_amd.train_precon [--systems 40] [--epochs 20] [--out weights.npy]
+    python -m adc_ottawa_amd.train_precon [--systems 120] [--epochs 30] [--out weights.npy]
 
 1. Harvest: for each seeded synthetic system (o, v drawn from the given
    ranges, naux = 4 (o + v), seeds 1000, 1001, ...) run the FP64 Davidson with
@@ -8,8 +8,9 @@
    eigenvectors x*, once replaying the same trajectory -- and keep, for every
    unconverged root (matched to the x* it overlaps most) and iteration, 16
    sampled elements (half 1h1p, half 2h2p) of (features, DPR step, exact
-   correction x - x*/(x.x*)), divided by the RMS element of the correction.
-   Records stay in HBM on one data handle.
+   correction x - x*/(x.x*)), divided by the RMS element of the correction;
+   at most --per-system records per system.  Records stay in HBM on one data
+   handle.
 2. Train: Adam on mini-batches with the library's own forward/backward
    kernels (adc2g_precon_train).
 3. Save the fp32 master weights (4865 values) as .npy.
@@ -69,19 +70,20 @@ def harvest(nsys, orange, vrange, nroots, seed0=1000, max_per_system=60000):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--systems", type=int, default=40)
+    ap.add_argument("--systems", type=int, default=120)
     ap.add_argument("--omin", type=int, default=8)
-    ap.add_argument("--omax", type=int, default=12)
+    ap.add_argument("--omax", type=int, default=16)
     ap.add_argument("--vmin", type=int, default=80)
-    ap.add_argument("--vmax", type=int, default=120)
+    ap.add_argument("--vmax", type=int, default=200)
     ap.add_argument("--roots", type=int, default=10)
-    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--per-system", type=int, default=25000)
+    ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", default=DEFAULT_OUT)
     a = ap.parse_args()
     t0 = time.perf_counter()
-    data, total = harvest(a.systems, (a.omin, a.omax), (a.vmin, a.vmax), a.roots)
+    data, total = harvest(a.systems, (a.omin, a.omax), (a.vmin, a.vmax), a.roots, max_per_system=a.per_system)
     log(f"harvested {total} records in {time.perf_counter() - t0:.1f}s")
     t1 = time.perf_counter()
     loss = data.precon_train(epochs=a.epochs, batch=a.batch, lr=a.lr, seed=7)

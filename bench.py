@@ -39,6 +39,20 @@ METRIC = "sigma-vector GB/s & time-to-20-roots (≤1e-7 Eh): learned vs diagonal
 PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # dense MFMA TFLOP/s, MI355X
 HBM_PEAK = 8000.0  # GB/s
 WEIGHTS = os.path.join(ROOT, "adc_ottawa_amd", "precon_weights.npy")
+# HBM bytes per launch of the coupling contractions, measured with rocprofv3
+# FETCH_SIZE / WRITE_SIZE counter runs of this benchmark (scaled by a copy of
+# known size); written next to the rocprof summaries it comes from.
+TRAFFIC = os.path.join(ROOT, "profiles", "coupling_gemm_traffic.json")
+
+
+def measured_traffic(prec, workload):
+    try:
+        with open(TRAFFIC) as f:
+            t = json.load(f)
+        rec = t.get(workload, {}).get(prec)
+        return None if rec is None else float(rec["bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def log(*a):
@@ -109,12 +123,12 @@ def timed_sigma(op, hip, dX, dW, nvec, prec, steps, warmup):
     return (t1 - t0), prof
 
 
-def roofline_of(prof, prec):
+def roofline_of(prof, prec, workload):
     ms, launches, flops = prof["coupling_gemm"]
     achieved = flops / (ms * 1e-3) / 1e12
-    return {"kernel": "gemm_kernel<%s,128,128,*,tag=1> (DF coupling: down + up)" % prec,
+    return {"kernel": "gemm_kernel<%s,128,128,*,tag=1> (DF coupling: down + up, mean per launch)" % prec,
             "bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK[prec], "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK[prec], 4), "traffic": None,
+            "frac": round(achieved / PEAK[prec], 4), "traffic": measured_traffic(prec, workload),
             "launches": int(launches), "avg_launch_ms": round(ms / max(launches, 1), 4),
             "flops_per_launch": flops / max(launches, 1)}
 
@@ -170,7 +184,8 @@ def davidson_compare(op, nroots):
     for name, kw in runs:
         t0 = time.perf_counter()
         try:
-            ev, _, st = op.davidson(nroots, blocksize=20, maxvec=200, tol=1e-7, maxiter=400, vectors=False, **kw)
+            ev, _, st = op.davidson(nroots, blocksize=20, maxvec=200, tol=1e-7, maxiter=1000, deflate=True,
+                                    vectors=False, **kw)
         except lib.Adc2gError as e:
             res[name] = {"error": str(e)}
             log(f"davidson {name}: {e}")
@@ -235,7 +250,7 @@ def main():
     secs, prof = timed_sigma(op, hip, dX, dW, nvec, "fp64", args.steps, args.warmup)
     ms_per_step = secs / args.steps * 1e3
     value = nvec * args.steps / secs
-    roof = roofline_of(prof, "fp64")
+    roof = roofline_of(prof, "fp64", args.workload)
     streams = stream_stats(prof)
     log(f"fp64: {ms_per_step:.3f} ms/step, {value:.1f} sigma-vectors/s, coupling {roof['achieved']} TFLOP/s")
     extra = {"fp64": {"ms_per_step": round(ms_per_step, 4), "sigma_vectors_per_s": round(value, 2),
@@ -243,7 +258,7 @@ def main():
     if not args.skip_lowprec:
         for prec in ("fp32", "bf16"):
             s2, p2 = timed_sigma(op, hip, dX, dW, nvec, prec, max(2, args.steps // 2), 1)
-            r2 = roofline_of(p2, prec)
+            r2 = roofline_of(p2, prec, args.workload)
             v2 = nvec * max(2, args.steps // 2) / s2
             extra[prec] = {"ms_per_step": round(s2 / max(2, args.steps // 2) * 1e3, 4),
                            "sigma_vectors_per_s": round(v2, 2), "coupling": r2, "streaming": stream_stats(p2)}

@@ -17,7 +17,10 @@ with the input transform g' = asinh(g), theta' = theta, rhat' = asinh(rhat).
 Training pairs (per element): features, dn_j = d_j/rms(e), en_j = e_j/rms(e)
 with e = x - x*/(x.x*) the exact correction towards the converged eigenvector
 x* and rms(e) its root-mean-square element;
-loss = mean_j (dn_j (1 + m_j) - en_j)^2.
+loss = mean_j (asinh(dn_j (1 + m_j)) - asinh(en_j))^2
+(compared through asinh, so that the rare elements with a near-zero DPR
+denominator, whose dn is orders of magnitude above the rest, do not dominate
+the mean).
 
 Weights are stored in bfloat16 and every product accumulates in fp32 on the
 GPU; this oracle evaluates the same bf16-rounded weights in float64.
@@ -81,7 +84,7 @@ def forward(p, x, cls):
 
 def loss(p, x, cls, dn, en):
     m, _ = forward(p, x, cls)
-    r = dn * (1.0 + m) - en
+    r = np.arcsinh(dn * (1.0 + m)) - np.arcsinh(en)
     return np.mean(r * r)
 
 
@@ -89,8 +92,9 @@ def backward(p, x, cls, dn, en):
     """Gradients of ``loss`` w.r.t. every parameter."""
     m, (x, cls, z1, h1, z2, h2) = forward(p, x, cls)
     n = x.shape[0]
-    r = dn * (1.0 + m) - en
-    gm = 2.0 * r * dn / n
+    t = dn * (1.0 + m)
+    r = np.arcsinh(t) - np.arcsinh(en)
+    gm = 2.0 * r * dn / np.sqrt(1.0 + t * t) / n
     g = {"w3": h2.T @ gm, "b3": np.array([gm.sum()])}
     gz2 = np.outer(gm, p["w3"]) * dsilu(z2)
     g["W2"] = gz2.T @ h1
