@@ -28,9 +28,10 @@
 //    reference divides unguarded).
 // Mixed precision: with precision != FP64 the first phase uses low-precision
 // sigma-vectors without convergence counting; when every residual is below
-// lowprec_tol (or stalls) the solver restarts from the Ritz vectors with FP64
-// sigma-vectors ("restarted refinement"), so convergence is always judged on
-// FP64 residuals.
+// lowprec_tol, or the largest one has not improved by 10% for 40 iterations,
+// the solver restarts from the Ritz vectors with FP64 sigma-vectors
+// ("restarted refinement"), so convergence is always judged on FP64
+// residuals.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -299,13 +300,16 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
     if (lowphase) {
       double mx = 0;
       for (int i = 0; i < nstates; ++i) mx = std::max(mx, norm[i]);
-      if (mx < low_best * 0.8) {
+      // the low-precision phase ends at lowprec_tol, or when the largest
+      // residual has not improved by 10% over its best for 40 iterations
+      // (the noise floor of the low-precision sigma)
+      if (mx < low_best * 0.9) {
         low_best = mx;
         low_stall = 0;
       } else {
         ++low_stall;
       }
-      if (refine && (mx < lowtol || low_stall >= 4)) {
+      if (refine && (mx < lowtol || low_stall >= 40)) {
         // restarted refinement: FP64 sigma-vectors from the current Ritz vectors
         lowphase = false;
         cur = Prec::F64;

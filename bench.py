@@ -170,16 +170,18 @@ def cpu_baseline(sys_, op, nvec):
             "seconds_per_block_scaled": round(t_full, 2), "stored_elements_full": int(coo.full_nnz)}
 
 
-def davidson_compare(op, nroots):
+def davidson_compare(op, nroots, only=None):
     res = {}
     runs = [("fp64_dpr", dict(precon="dpr", precision="fp64")),
-            ("bf16_dpr_refined", dict(precon="dpr", precision="bf16", lowprec_tol=1e-3)),
-            ("fp32_dpr_refined", dict(precon="dpr", precision="fp32", lowprec_tol=1e-5))]
+            ("bf16_dpr_refined", dict(precon="dpr", precision="bf16", lowprec_tol=5e-2)),
+            ("fp32_dpr_refined", dict(precon="dpr", precision="fp32", lowprec_tol=1e-4))]
     have_net = os.path.exists(WEIGHTS)
     if have_net:
         op.set_precon_params(np.load(WEIGHTS))
         runs += [("fp64_learned", dict(precon="learned", precision="fp64")),
-                 ("bf16_learned_refined", dict(precon="learned", precision="bf16", lowprec_tol=1e-3))]
+                 ("bf16_learned_refined", dict(precon="learned", precision="bf16", lowprec_tol=5e-2))]
+    if only:
+        runs = [r for r in runs if r[0] in only]
     ref = None
     for name, kw in runs:
         t0 = time.perf_counter()
@@ -222,6 +224,8 @@ def main():
     ap.add_argument("--skip-davidson", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-lowprec", action="store_true")
+    ap.add_argument("--dav-runs", default="",
+                    help="comma-separated subset of the Davidson comparison runs (default: all)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run device copies of 1 GiB (scale factor for rocprofv3 --pmc byte counters)")
     args = ap.parse_args()
@@ -277,7 +281,7 @@ def main():
             log(f"cpu baseline failed: {e}")
     dav = None
     if not args.skip_davidson:
-        dav = davidson_compare(op, 20)
+        dav = davidson_compare(op, 20, [r for r in args.dav_runs.split(",") if r])
 
     n1 = op.n1
     bytes_per_sigma = 8.0 * (2 * op.ndim) + 8.0 * 2 * n1 * n1  # read x, write sigma, pair tensors
