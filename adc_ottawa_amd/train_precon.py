@@ -14,6 +14,10 @@
 2. Train: Adam on mini-batches with the library's own forward/backward
    kernels (adc2g_precon_train).
 3. Save the fp32 master weights (4865 values) as .npy.
+
+    python -m adc_ottawa_amd.train_precon --evaluate 7001,7002
+
+compares the saved network with the DPR step on unseen benzene-sized systems.
 """
 from __future__ import annotations
 
@@ -68,6 +72,30 @@ def harvest(nsys, orange, vrange, nroots, seed0=1000, max_per_system=60000):
     return data, total
 
 
+def evaluate(weights, seeds, nroots=20):
+    """Time to nroots roots (1e-7, block nroots, subspace 10*nroots, deflation)
+    with the DPR step and with the network, FP64 and BF16 + FP64 refinement,
+    on the benzene-sized system (o=16, v=200, naux=864) of each seed."""
+    import json
+    for seed in seeds:
+        s = make_system(16, 200, 864, seed=seed)
+        with lib.Operator.from_system(s) as op:
+            op.set_precon_params(weights)
+            ref = None
+            for name, kw in (("fp64_dpr", dict(precon="dpr")), ("fp64_learned", dict(precon="learned")),
+                             ("bf16_dpr", dict(precon="dpr", precision="bf16", lowprec_tol=5e-2)),
+                             ("bf16_learned", dict(precon="learned", precision="bf16", lowprec_tol=5e-2))):
+                t0 = time.perf_counter()
+                ev, _, st = op.davidson(nroots, blocksize=nroots, maxvec=10 * nroots, tol=1e-7, maxiter=1500,
+                                        deflate=True, vectors=False, **kw)
+                if ref is None:
+                    ref = ev
+                print(json.dumps(dict(seed=seed, run=name, seconds=round(time.perf_counter() - t0, 3),
+                                      iterations=st["niter"], sigma_vectors=st["nsigma"],
+                                      fallbacks=st["nfallback"], max_err_vs_fp64_dpr=float(abs(ev - ref).max()))),
+                      flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--systems", type=int, default=120)
@@ -81,7 +109,12 @@ def main():
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--evaluate", default="",
+                    help="only evaluate the weights in --out on these comma-separated seeds (o=16, v=200)")
     a = ap.parse_args()
+    if a.evaluate:
+        evaluate(np.load(a.out), [int(x) for x in a.evaluate.split(",")])
+        return
     t0 = time.perf_counter()
     data, total = harvest(a.systems, (a.omin, a.omax), (a.vmin, a.vmax), a.roots, max_per_system=a.per_system)
     log(f"harvested {total} records in {time.perf_counter() - t0:.1f}s")
