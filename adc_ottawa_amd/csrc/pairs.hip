@@ -42,25 +42,29 @@ __global__ void __launch_bounds__(PT) pack_tiled_kernel(const double* __restrict
   extern __shared__ double sm[];
   const int o = d.o, v = d.v, tid = threadIdx.x;
   const int a = blockIdx.x;
-  const int64_t vec = blockIdx.y;
+  const int chunk = blockIdx.y;
+  const int64_t vec = blockIdx.z;
+  const int b0 = a + 1 + chunk * BB;
+  if (b0 >= v && chunk > 0) return;  // whole workgroup, before any barrier
   const int64_t n1 = d.n1, npo = d.npo, npv = d.npv;
   const double* x = X + ldx * vec;
   T* z = Z + vec * n1 * n1;
   double* xs4 = sm;                 // [npo][BB]
   double* xs5 = xs4 + npo * BB;     // [npo][BB]
   double* xs2 = xs5 + npo * BB;     // [BB][o]
-  // diagonal tile b = a
-  for (int t = tid; t < o * o; t += PT) {
-    const int k = t / o, l = t - k * o;
-    double val;
-    if (k == l) val = -SQRT2 * x[d.off1 + (int64_t)k * v + a];
-    else val = x[d.off3 + pidx(min(k, l), max(k, l), o) * v + a];
-    z[((int64_t)a * o + k) * n1 + (int64_t)a * o + l] = cvt<T>(val);
+  // diagonal tile b = a (first chunk)
+  if (chunk == 0) {
+    for (int t = tid; t < o * o; t += PT) {
+      const int k = t / o, l = t - k * o;
+      double val;
+      if (k == l) val = -SQRT2 * x[d.off1 + (int64_t)k * v + a];
+      else val = x[d.off3 + pidx(min(k, l), max(k, l), o) * v + a];
+      z[((int64_t)a * o + k) * n1 + (int64_t)a * o + l] = cvt<T>(val);
+    }
   }
-  for (int b0 = a + 1; b0 < v; b0 += BB) {
+  if (b0 < v) {
     const int nb = min(BB, v - b0);
     const int64_t p0 = pidx(a, b0, v);  // pv(a, b0); pv(a, b0 + bb) = p0 + bb
-    __syncthreads();
     for (int t = tid; t < npo * nb; t += PT) {
       const int q = t / nb, bb = t - q * nb;
       const int64_t J = (int64_t)q * npv + p0 + bb;
@@ -72,33 +76,28 @@ __global__ void __launch_bounds__(PT) pack_tiled_kernel(const double* __restrict
       xs2[bb * o + k] = -x[d.off2 + (p0 + bb) * o + k];
     }
     __syncthreads();
-    // Z(a, b, k, l): rows a*o + k, columns b*o + l for b in the chunk
-    for (int t = tid; t < o * nb * o; t += PT) {
-      const int k = t / (nb * o);
-      const int rem = t - k * nb * o;
-      const int bb = rem / o, l = rem - bb * o;
-      double val;
-      if (k == l) {
-        val = xs2[bb * o + k];
-      } else {
-        const int q = (int)pidx(min(k, l), max(k, l), o);
-        val = (k < l) ? xs4[q * BB + bb] + xs5[q * BB + bb] : xs4[q * BB + bb] - xs5[q * BB + bb];
+    // Z(a, b, k, l) (rows a*o + k, columns b*o + l) and Z(b, a, k, l) (rows
+    // b*o + k, columns a*o + l): each thread keeps one (k, l) and walks the b
+    // of the chunk, so the index decoding is done once per thread
+    for (int kl = tid; kl < o * o; kl += PT) {
+      const int k = kl / o, l = kl - k * o;
+      const bool diag = (k == l);
+      const int q = diag ? 0 : (int)pidx(min(k, l), max(k, l), o);
+      const double s_ab = (k < l) ? 1.0 : -1.0;  // sign of x5 in Z(a,b,k,l); Z(b,a,k,l) takes the other
+      T* zab = z + ((int64_t)a * o + k) * n1 + (int64_t)b0 * o + l;
+      T* zba = z + ((int64_t)b0 * o + k) * n1 + (int64_t)a * o + l;
+      for (int bb = 0; bb < nb; ++bb) {
+        double vab, vba;
+        if (diag) {
+          vab = vba = xs2[bb * o + k];
+        } else {
+          const double p4 = xs4[q * BB + bb], p5 = xs5[q * BB + bb];
+          vab = p4 + s_ab * p5;
+          vba = p4 - s_ab * p5;
+        }
+        zab[(int64_t)bb * o] = cvt<T>(vab);
+        zba[(int64_t)bb * o * n1] = cvt<T>(vba);
       }
-      z[((int64_t)a * o + k) * n1 + (int64_t)(b0 + bb) * o + l] = cvt<T>(val);
-    }
-    // Z(b, a, k, l): rows b*o + k, columns a*o + l
-    for (int t = tid; t < nb * o * o; t += PT) {
-      const int bb = t / (o * o);
-      const int rem = t - bb * o * o;
-      const int k = rem / o, l = rem - k * o;
-      double val;
-      if (k == l) {
-        val = xs2[bb * o + k];
-      } else {
-        const int q = (int)pidx(min(k, l), max(k, l), o);
-        val = (k < l) ? xs4[q * BB + bb] - xs5[q * BB + bb] : xs4[q * BB + bb] + xs5[q * BB + bb];
-      }
-      z[((int64_t)(b0 + bb) * o + k) * n1 + (int64_t)a * o + l] = cvt<T>(val);
     }
   }
 }
@@ -110,7 +109,10 @@ __global__ void __launch_bounds__(PT) unpack_tiled_kernel(const T* __restrict__ 
   extern __shared__ double sm[];
   const int o = d.o, v = d.v, tid = threadIdx.x;
   const int a = blockIdx.x;
-  const int64_t vec = blockIdx.y;
+  const int chunk = blockIdx.y;
+  const int64_t vec = blockIdx.z;
+  const int b0 = a + 1 + chunk * BB;
+  if (b0 >= v && chunk > 0) return;  // whole workgroup, before any barrier
   const int64_t n1 = d.n1, npo = d.npo, npv = d.npv;
   const T* g = G + vec * n1 * n1;
   const double* x = X + ldx * vec;
@@ -119,35 +121,39 @@ __global__ void __launch_bounds__(PT) unpack_tiled_kernel(const T* __restrict__ 
   double* tab = sm;            // [BB][o][o]  G[(a,k),(b,l)]
   double* tba = sm + BB * oo;  // [BB][o][o]  G[(b,k),(a,l)]
   const double ea = eps[o + a];
-  // diagonal tile b = a: t1 and t3
-  for (int t = tid; t < oo; t += PT) tab[t] = dbl(g[((int64_t)a * o + t / o) * n1 + (int64_t)a * o + t % o]);
-  __syncthreads();
-  for (int k = tid; k < o; k += PT) {
-    const int64_t J = d.off1 + (int64_t)k * v + a;
-    w[J] = -SQRT2 * tab[k * o + k] + (2.0 * ea - 2.0 * eps[k]) * x[J];
-  }
-  for (int q = tid; q < npo; q += PT) {
-    // decode q -> (k, l) by walking the rows (npo is small)
-    int k = 0;
-    int64_t base = 0;
-    while (base + (o - 1 - k) <= q) {
-      base += o - 1 - k;
-      ++k;
+  // diagonal tile b = a: t1 and t3 (first chunk)
+  if (chunk == 0) {
+    for (int t = tid; t < oo; t += PT) tab[t] = dbl(g[((int64_t)a * o + t / o) * n1 + (int64_t)a * o + t % o]);
+    __syncthreads();
+    for (int k = tid; k < o; k += PT) {
+      const int64_t J = d.off1 + (int64_t)k * v + a;
+      w[J] = -SQRT2 * tab[k * o + k] + (2.0 * ea - 2.0 * eps[k]) * x[J];
     }
-    const int l = (int)(q - base) + k + 1;
-    const int64_t J = d.off3 + (int64_t)q * v + a;
-    w[J] = tab[k * o + l] + tab[l * o + k] + (2.0 * ea - eps[k] - eps[l]) * x[J];
+    for (int q = tid; q < npo; q += PT) {
+      // decode q -> (k, l) by walking the rows (npo is small)
+      int k = 0;
+      int64_t base = 0;
+      while (base + (o - 1 - k) <= q) {
+        base += o - 1 - k;
+        ++k;
+      }
+      const int l = (int)(q - base) + k + 1;
+      const int64_t J = d.off3 + (int64_t)q * v + a;
+      w[J] = tab[k * o + l] + tab[l * o + k] + (2.0 * ea - eps[k] - eps[l]) * x[J];
+    }
   }
-  for (int b0 = a + 1; b0 < v; b0 += BB) {
+  if (b0 < v) {
     const int nb = min(BB, v - b0);
     const int64_t p0 = pidx(a, b0, v);
     __syncthreads();
-    for (int t = tid; t < nb * oo; t += PT) {
-      const int bb = t / oo, rem = t - bb * oo;
-      const int k = rem / o, l = rem - k * o;
-      const int b = b0 + bb;
-      tab[t] = dbl(g[((int64_t)a * o + k) * n1 + (int64_t)b * o + l]);
-      tba[t] = dbl(g[((int64_t)b * o + k) * n1 + (int64_t)a * o + l]);
+    for (int kl = tid; kl < oo; kl += PT) {
+      const int k = kl / o, l = kl - k * o;
+      const T* gab = g + ((int64_t)a * o + k) * n1 + (int64_t)b0 * o + l;
+      const T* gba = g + ((int64_t)b0 * o + k) * n1 + (int64_t)a * o + l;
+      for (int bb = 0; bb < nb; ++bb) {
+        tab[bb * oo + kl] = dbl(gab[(int64_t)bb * o]);
+        tba[bb * oo + kl] = dbl(gba[(int64_t)bb * o * n1]);
+      }
     }
     __syncthreads();
     // t2 (a<b, k=l): contiguous over (b, k)
@@ -157,7 +163,8 @@ __global__ void __launch_bounds__(PT) unpack_tiled_kernel(const T* __restrict__ 
       const double D = ea + eps[o + b0 + bb] - 2.0 * eps[k];
       w[J] = -(tab[bb * oo + k * o + k] + tba[bb * oo + k * o + k]) + D * x[J];
     }
-    // t4, t5 (a<b, k<l): contiguous over b for each occupied pair q
+    // t4, t5 (a<b, k<l): contiguous over b for each occupied pair q; the
+    // pair (k, l) of q is decoded once per thread and step of q
     for (int t = tid; t < npo * nb; t += PT) {
       const int q = t / nb, bb = t - q * nb;
       int k = 0;
@@ -195,7 +202,8 @@ template <class T>
 void launch_pack_tiled(const double* X, int64_t ldx, int nvec, const DimsK& d, T* Z, hipStream_t st) {
   const int BB = pack_chunk(d);
   const size_t lds = sizeof(double) * (size_t)BB * (2 * d.npo + d.o);
-  hipLaunchKernelGGL((pack_tiled_kernel<T>), dim3(d.v, nvec), dim3(PT), lds, st, X, ldx, d, Z, BB);
+  const int chunks = (d.v - 1 + BB - 1) / BB;
+  hipLaunchKernelGGL((pack_tiled_kernel<T>), dim3(d.v, std::max(chunks, 1), nvec), dim3(PT), lds, st, X, ldx, d, Z, BB);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -204,7 +212,9 @@ void launch_unpack_tiled(const T* G, const double* X, int64_t ldx, double* W, in
                          const double* eps, hipStream_t st) {
   const int BB = unpack_chunk(d);
   const size_t lds = sizeof(double) * (size_t)2 * BB * d.o * d.o;
-  hipLaunchKernelGGL((unpack_tiled_kernel<T>), dim3(d.v, nvec), dim3(PT), lds, st, G, X, ldx, W, ldw, d, eps, BB);
+  const int chunks = (d.v - 1 + BB - 1) / BB;
+  hipLaunchKernelGGL((unpack_tiled_kernel<T>), dim3(d.v, std::max(chunks, 1), nvec), dim3(PT), lds, st, G, X, ldx, W,
+                     ldw, d, eps, BB);
   HIP_CHECK(hipGetLastError());
 }
 
