@@ -214,6 +214,34 @@ def davidson_compare(op, nroots, only=None):
     return res
 
 
+def sil_compare(op, sz, tfinal, seed=0):
+    """SIL propagation of the benzene-sized Hamiltonian: initial state the
+    seeded synthetic dipole vector (1h1p part, N(0, 0.1^2)) normalised, Lanczos
+    order 20, output interval 0.1 a.u., autocorrelation every interval; FP64
+    and FP32-MFMA sigma-vectors (FP64 recursion and exponential in both)."""
+    from adc_ottawa_amd.synth import unit_dipole
+    f = np.zeros(op.ndim)
+    f[: op.n1] = unit_dipole(sz["nocc"], sz["nvirt"], seed=seed)
+    f /= np.linalg.norm(f)
+    out, ref = {}, None
+    for prec in ("fp64", "fp32"):
+        t0 = time.perf_counter()
+        auto, st = op.sil_autocorr(f, tout=0.1, tfinal=tfinal, kdim=20, autotol=1e-8, precision=prec)
+        wall = time.perf_counter() - t0
+        rec = {"seconds": round(wall, 3), "records": int(auto.size), "lanczos_steps": int(st["nsteps"]),
+               "sigma_vectors": int(st["nsigma"]), "max_order": int(st["max_order"])}
+        if ref is None:
+            ref = auto
+        else:
+            n = min(ref.size, auto.size)
+            rec["max_abs_err_vs_fp64"] = float(np.abs(auto[:n] - ref[:n]).max())
+        out[prec] = rec
+        log(f"sil {prec}: {wall:.2f}s, {st['nsteps']} Lanczos steps, {st['nsigma']} sigma-vectors, "
+            f"{auto.size} autocorrelation records")
+    out["tfinal_au"] = tfinal
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,6 +252,8 @@ def main():
     ap.add_argument("--skip-davidson", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-lowprec", action="store_true")
+    ap.add_argument("--sil", type=float, default=0.0,
+                    help="also propagate with SIL to this final time (a.u.), FP64 and FP32 sigma")
     ap.add_argument("--dav-runs", default="",
                     help="comma-separated subset of the Davidson comparison runs (default: all)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -283,6 +313,8 @@ def main():
     if not args.skip_davidson:
         dav = davidson_compare(op, 20, [r for r in args.dav_runs.split(",") if r])
 
+    sil = sil_compare(op, sz, args.sil) if args.sil > 0 else None
+
     n1 = op.n1
     bytes_per_sigma = 8.0 * (2 * op.ndim) + 8.0 * 2 * n1 * n1  # read x, write sigma, pair tensors
     out = {
@@ -306,6 +338,7 @@ def main():
         "sigma_GBps_effective": round(bytes_per_sigma * value / 1e9, 1),
         "detail": extra,
         "davidson_20_roots": dav,
+        "sil_autocorrelation": sil,
         "operator_build_seconds": round(t_build, 3),
     }
     op.close()
