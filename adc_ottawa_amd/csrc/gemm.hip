@@ -24,12 +24,17 @@ bool direct_map(const GemmArgs& g) {
   }
 }
 
+// Threads per workgroup: FP64 128x128 tiles run 8 waves (each 64x32) so that
+// two workgroups per CU keep four waves per SIMD; the rest run 4 waves.
+template <class T, int BM> constexpr int threads_for() { return (sizeof(T) == 8 && BM == 128) ? 512 : 256; }
+
 template <class T, int BM, int BN, int BK, class OutT, int TAG, int MK>
 void launch_k(const GemmArgs& g, dim3 grid, hipStream_t st) {
+  constexpr int NT = threads_for<T, BM>();
   if (g.vec)
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, OutT, TAG, true, MK>), grid, dim3(256), 0, st, g);
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, OutT, TAG, true, MK, NT>), grid, dim3(NT), 0, st, g);
   else
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, OutT, TAG, false, MK>), grid, dim3(256), 0, st, g);
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, OutT, TAG, false, MK, NT>), grid, dim3(NT), 0, st, g);
 }
 
 template <class T, int BM, int BN, int BK>

@@ -9,7 +9,7 @@
 // permuted layouts the next contraction reads.  Split-K writes fp partials
 // that a reduction kernel folds through the same map.
 //
-// Tiling: 256 threads = 4 waves in a 2x2 grid, block tile BM x BN, K tile BK,
+// Tiling: NT threads = NT/64 waves in a 2 x (NT/128) grid, block tile BM x BN, K tile BK,
 // each wave an (BM/2) x (BN/2) grid of 16x16 MFMA blocks.  Both operand tiles
 // live in LDS K-contiguous (row = m or n, BK elements + pad), double
 // buffered, so one barrier per K tile.  Within a K tile the K index is
@@ -222,8 +222,8 @@ __device__ __forceinline__ int xcd_tile(int w, int T) {
   return x * per + (x < rem ? x : rem) + slot;
 }
 
-template <class T, int BM, int BN, int BK, class OutT, int TAG, bool VEC, int MK>
-__global__ void __launch_bounds__(256, 2)
+template <class T, int BM, int BN, int BK, class OutT, int TAG, bool VEC, int MK, int NT>
+__global__ void __launch_bounds__(NT, NT / 128)
 gemm_kernel(GemmArgs g) {
   typedef Mma<T> MT;
   typedef typename MT::Acc Acc;
@@ -233,10 +233,12 @@ gemm_kernel(GemmArgs g) {
   constexpr int KS = BK / KM;          // MFMA steps per K tile
   constexpr int PAD = 16 / sizeof(T);  // 16 B of padding per LDS row
   constexpr int LDR = BK + PAD;
-  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int WGN = NT / 128;        // waves along N (2 x WGN wave grid)
+  constexpr int WM = BM / 2, WN = BN / WGN;
   constexpr int TM = WM / 16, TN = WN / 16;
-  constexpr int EA = BM * BK / 256;    // elements per thread per tile
-  constexpr int EB = BN * BK / 256;
+  constexpr int EA = BM * BK / NT;     // elements per thread per tile
+  constexpr int EB = BN * BK / NT;
+  static_assert(TN >= 1 && EA >= 1 && EB >= 1, "tile too small for the thread count");
   static_assert(BK % KM == 0, "BK must be a multiple of the MFMA K");
 
   __shared__ __attribute__((aligned(16))) T sA[2][BM * LDR];
@@ -245,7 +247,7 @@ gemm_kernel(GemmArgs g) {
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / WGN, wn = wave % WGN;
 
   const int tiles_m = (g.M + BM - 1) / BM;
   const int tiles_n = (g.N + BN - 1) / BN;
@@ -275,9 +277,9 @@ gemm_kernel(GemmArgs g) {
   // Per-thread fixed coordinates of the load pattern.  K-contiguous: the
   // thread keeps column c and walks rows r0 + e*(256/BK).  MN-contiguous: it
   // keeps row r and walks columns c0 + e*(256/BM).
-  constexpr int RSA = 256 / BK;   // row step, K-contiguous
-  constexpr int CSA = 256 / BM;   // column step, M-contiguous
-  constexpr int CSB = 256 / BN;
+  constexpr int RSA = NT / BK;    // row step, K-contiguous
+  constexpr int CSA = NT / BM;    // column step, M-contiguous
+  constexpr int CSB = NT / BN;
   const int kc_c = tid % BK, kc_r = tid / BK;
   const int ma_r = tid % BM, ma_c = tid / BM;
   const int nb_r = tid % BN, nb_c = tid / BN;
@@ -288,9 +290,9 @@ gemm_kernel(GemmArgs g) {
   typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
   constexpr int EV = 16 / sizeof(T);
   constexpr int CPR = BK / EV;
-  constexpr int RPP = 256 / CPR;
+  constexpr int RPP = NT / CPR;
   constexpr int VA = (BM + RPP - 1) / RPP, VB = (BN + RPP - 1) / RPP;
-  static_assert(BK % EV == 0 && 256 % CPR == 0, "vector load tiling");
+  static_assert(BK % EV == 0 && NT % CPR == 0, "vector load tiling");
   const int vc = tid % CPR, vr = tid / CPR;
   u32x4 va[VEC ? VA : 1], vb[VEC ? VB : 1];
   auto vload_a = [&](int kt) {
