@@ -132,39 +132,62 @@ __device__ __forceinline__ void map_store(const MapParams& mp, int64_t m, int64_
   }
 }
 
-// map_store for a map kind fixed at compile time (the direct epilogue of the
-// GEMM kernel instantiated for that map).
-template <int MK, class OutT>
-__device__ __forceinline__ void map_store_k(const MapParams& mp, int64_t m, int64_t n, int64_t b, double val) {
-  int64_t o1 = 0, o2 = -1;
+// The permuting maps are separable: offset = mpart(m) + npart(n) (two outputs
+// for MAP_DOWNY), so the direct epilogue decodes each row and each column of
+// a tile once, with 32-bit divisions, and stores with additions only.
+template <int MK>
+__device__ __forceinline__ void map_npart(const MapParams& mp, uint32_t n, int64_t& g1, int64_t& g2) {
   if constexpr (MK == MAP_DOWNY) {
-    const int64_t o = mp.q[0], nvec = mp.q[1], n1 = mp.q[2], naux = mp.q[3];
-    const int64_t vec = n / n1, ck = n - vec * n1;
-    const int64_t c = ck / o, k = ck - c * o;
-    o1 = k + o * vec + o * nvec * c + n1 * nvec * m;
-    o2 = k + o * m + o * naux * (c * nvec + vec);
+    // n = ck + n1*vec, ck = c*o + k:  o1 = k + o*vec + o*nvec*c (+ n1*nvec*m),
+    //                                 o2 = k + o*naux*(c*nvec + vec) (+ o*m)
+    const uint32_t o = (uint32_t)mp.q[0], nvec = (uint32_t)mp.q[1], n1 = (uint32_t)mp.q[2];
+    const int64_t naux = mp.q[3];
+    const uint32_t vec = n / n1, ck = n - vec * n1, c = ck / o, k = ck - c * o;
+    g1 = (int64_t)k + (int64_t)o * vec + (int64_t)o * nvec * c;
+    g2 = (int64_t)k + (int64_t)o * naux * ((int64_t)c * nvec + vec);
   } else if constexpr (MK == MAP_WUPV) {
-    const int64_t o = mp.q[0], v = mp.q[1], n1 = mp.q[2], nvec = mp.q[3];
-    const int64_t P = m / v, c = m - P * v;
-    const int64_t vec = n / o, k = n - vec * o;
-    o1 = k + o * c + n1 * vec + n1 * nvec * P;
+    // n = k + o*vec -> k + n1*vec
+    const uint32_t o = (uint32_t)mp.q[0];
+    const int64_t n1 = mp.q[2];
+    const uint32_t vec = n / o, k = n - vec * o;
+    g1 = (int64_t)k + n1 * vec;
+    g2 = 0;
   } else {
     static_assert(MK == MAP_WUPO, "map kind without a direct epilogue");
-    const int64_t o = mp.q[0], n1 = mp.q[2], nvec = mp.q[3];
-    const int64_t c = m / nvec, vec = m - c * nvec;
-    const int64_t P = n / o, k = n - P * o;
-    o1 = k + o * c + n1 * vec + n1 * nvec * P;
+    // n = k + o*P -> k + n1*nvec*P
+    const uint32_t o = (uint32_t)mp.q[0];
+    const int64_t n1 = mp.q[2], nvec = mp.q[3];
+    const uint32_t P = n / o, k = n - P * o;
+    g1 = (int64_t)k + n1 * nvec * P;
+    g2 = 0;
   }
-  OutT* C = (OutT*)mp.C;
-  double r = mp.alpha * val;
-  if (mp.beta != 0.0) r += mp.beta * to_double(C[o1]);
-  C[o1] = from_double<OutT>(r);
-  if (o2 >= 0) {
-    OutT* C2 = (OutT*)mp.C2;
-    double r2 = mp.alpha * val;
-    if (mp.beta != 0.0) r2 += mp.beta * to_double(C2[o2]);
-    C2[o2] = from_double<OutT>(r2);
+}
+
+template <int MK>
+__device__ __forceinline__ void map_mpart(const MapParams& mp, uint32_t m, int64_t& f1, int64_t& f2) {
+  if constexpr (MK == MAP_DOWNY) {
+    f1 = mp.q[2] * mp.q[1] * (int64_t)m;  // n1*nvec*m
+    f2 = mp.q[0] * (int64_t)m;            // o*m
+  } else if constexpr (MK == MAP_WUPV) {
+    // m = c + v*P -> o*c + n1*nvec*P
+    const uint32_t v = (uint32_t)mp.q[1];
+    const uint32_t P = m / v, c = m - P * v;
+    f1 = mp.q[0] * (int64_t)c + mp.q[2] * mp.q[3] * (int64_t)P;
+    f2 = 0;
+  } else {
+    // m = c*nvec + vec -> o*c + n1*vec
+    const uint32_t nvec = (uint32_t)mp.q[3];
+    const uint32_t c = m / nvec, vec = m - c * nvec;
+    f1 = mp.q[0] * (int64_t)c + mp.q[2] * (int64_t)vec;
+    f2 = 0;
   }
+}
+
+template <class OutT>
+__device__ __forceinline__ void store_scaled(OutT* C, int64_t off, double alpha, double beta, double val) {
+  double r = alpha * val;
+  if (beta != 0.0) r += beta * to_double(C[off]);
+  C[off] = from_double<OutT>(r);
 }
 
 // ---- per-type MFMA traits ---------------------------------------------------
@@ -454,6 +477,36 @@ gemm_kernel(GemmArgs g) {
   Acc* part = (Acc*)g.partial + ((int64_t)b * g.splitk + ks) * (int64_t)g.M * g.N;
   OutT* C = (OutT*)g.map.C;
   const double alpha = g.map.alpha, beta = g.map.beta;
+  if constexpr (MK != MAP_STD) {
+    if (direct) {
+      OutT* C2 = (OutT*)g.map.C2;
+      int64_t gn1[TN], gn2[TN];
+      bool nv[TN];
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int n = n0 + wn * WN + j * 16 + li;
+        nv[j] = n < g.N;
+        map_npart<MK>(g.map, nv[j] ? (uint32_t)n : 0u, gn1[j], gn2[j]);
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = m0 + wm * WM + i * 16 + MT::row(lane, r);
+          if (m >= g.M) continue;
+          int64_t f1, f2;
+          map_mpart<MK>(g.map, (uint32_t)m, f1, f2);
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            if (!nv[j]) continue;
+            const double val = (double)acc[i][j][r];
+            store_scaled<OutT>(C, f1 + gn1[j], alpha, beta, val);
+            if constexpr (MK == MAP_DOWNY) store_scaled<OutT>(C2, f2 + gn2[j], alpha, beta, val);
+          }
+        }
+      return;
+    }
+  }
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -463,13 +516,7 @@ gemm_kernel(GemmArgs g) {
         const int m = m0 + wm * WM + i * 16 + MT::row(lane, r);
         const int n = n0 + wn * WN + j * 16 + li;
         if (m < g.M && n < g.N) {
-          if constexpr (MK != MAP_STD) {
-            if (direct) {
-              map_store_k<MK, OutT>(g.map, m, n, b, (double)acc[i][j][r]);
-            } else {
-              part[(int64_t)m * g.N + n] = acc[i][j][r];
-            }
-          } else if (direct) {
+          if (MK == MAP_STD && direct) {
             const int64_t off = (int64_t)m * g.map.scm + (int64_t)n * g.map.scn + (int64_t)b * g.map.scb;
             if (g.out_prec == 0) {
               double val = alpha * (double)acc[i][j][r];
