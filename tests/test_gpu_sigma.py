@@ -115,3 +115,18 @@ def test_sigma_benzene_vs_df_oracle(benzene):
     ref = df.sigma(x)
     assert np.abs(w - ref).max() < 1e-11 * np.abs(ref).max()
     assert np.abs(op.diagonal() - df.diag()).max() < 1e-11 * np.abs(df.diag()).max()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("bf16", 3e-2)])
+def test_sigma_many_occupied(prec, tol):
+    """nocc > 64 takes the element-wise CSF<->pair kernels instead of the
+    LDS-tiled ones; checked against the density-fitted oracle."""
+    print(f"[case] sigma o=66 v=4 naux=30 {prec}", flush=True)
+    sys = make_system(66, 4, 30, seed=6)
+    df = dfsigma.DF(sys)
+    with lib.Operator.from_system(sys) as op:
+        rng = np.random.default_rng(13)
+        X = rng.standard_normal((op.ndim, 3))
+        W = op.sigma(X, precision=prec)
+    ref = df.sigma(X)
+    assert np.abs(W - ref).max() < tol * np.abs(ref).max()
