@@ -49,9 +49,10 @@ __global__ void __launch_bounds__(PT) pack_tiled_kernel(const double* __restrict
   const int64_t n1 = d.n1, npo = d.npo, npv = d.npv;
   const double* x = X + ldx * vec;
   T* z = Z + vec * n1 * n1;
-  double* xs4 = sm;                 // [npo][BB]
-  double* xs5 = xs4 + npo * BB;     // [npo][BB]
-  double* xs2 = xs5 + npo * BB;     // [BB][o]
+  const int BP = BB + 1;            // padded run length (LDS banks differ across q)
+  double* xs4 = sm;                 // [npo][BP]
+  double* xs5 = xs4 + npo * BP;     // [npo][BP]
+  double* xs2 = xs5 + npo * BP;     // [BB][o]
   // diagonal tile b = a (first chunk)
   if (chunk == 0) {
     for (int t = tid; t < o * o; t += PT) {
@@ -68,8 +69,8 @@ __global__ void __launch_bounds__(PT) pack_tiled_kernel(const double* __restrict
     for (int t = tid; t < npo * nb; t += PT) {
       const int q = t / nb, bb = t - q * nb;
       const int64_t J = (int64_t)q * npv + p0 + bb;
-      xs4[q * BB + bb] = INV_SQRT2 * x[d.off4 + J];
-      xs5[q * BB + bb] = SQRT3_2 * x[d.off5 + J];
+      xs4[q * BP + bb] = INV_SQRT2 * x[d.off4 + J];
+      xs5[q * BP + bb] = SQRT3_2 * x[d.off5 + J];
     }
     for (int t = tid; t < nb * o; t += PT) {
       const int bb = t / o, k = t - bb * o;
@@ -91,7 +92,7 @@ __global__ void __launch_bounds__(PT) pack_tiled_kernel(const double* __restrict
         if (diag) {
           vab = vba = xs2[bb * o + k];
         } else {
-          const double p4 = xs4[q * BB + bb], p5 = xs5[q * BB + bb];
+          const double p4 = xs4[q * BP + bb], p5 = xs5[q * BP + bb];
           vab = p4 + s_ab * p5;
           vba = p4 - s_ab * p5;
         }
@@ -105,7 +106,7 @@ __global__ void __launch_bounds__(PT) pack_tiled_kernel(const double* __restrict
 template <class T>
 __global__ void __launch_bounds__(PT) unpack_tiled_kernel(const T* __restrict__ G, const double* __restrict__ X,
                                                            int64_t ldx, double* __restrict__ W, int64_t ldw, DimsK d,
-                                                           const double* __restrict__ eps, int BB) {
+                                                           const double* __restrict__ eps, int BB, int op) {
   extern __shared__ double sm[];
   const int o = d.o, v = d.v, tid = threadIdx.x;
   const int a = blockIdx.x;
@@ -118,8 +119,10 @@ __global__ void __launch_bounds__(PT) unpack_tiled_kernel(const T* __restrict__ 
   const double* x = X + ldx * vec;
   double* w = W + ldw * vec;
   const int oo = o * o;
-  double* tab = sm;            // [BB][o][o]  G[(a,k),(b,l)]
-  double* tba = sm + BB * oo;  // [BB][o][o]  G[(b,k),(a,l)]
+  // op: tile stride, o*o + 1 when that fits the LDS budget (LDS banks then
+  // differ across b)
+  double* tab = sm;            // [BB][o*o + 1]  G[(a,k),(b,l)]
+  double* tba = sm + BB * op;  // [BB][o*o + 1]  G[(b,k),(a,l)]
   const double ea = eps[o + a];
   // diagonal tile b = a: t1 and t3 (first chunk)
   if (chunk == 0) {
@@ -151,8 +154,8 @@ __global__ void __launch_bounds__(PT) unpack_tiled_kernel(const T* __restrict__ 
       const T* gab = g + ((int64_t)a * o + k) * n1 + (int64_t)b0 * o + l;
       const T* gba = g + ((int64_t)b0 * o + k) * n1 + (int64_t)a * o + l;
       for (int bb = 0; bb < nb; ++bb) {
-        tab[bb * oo + kl] = dbl(gab[(int64_t)bb * o]);
-        tba[bb * oo + kl] = dbl(gba[(int64_t)bb * o * n1]);
+        tab[bb * op + kl] = dbl(gab[(int64_t)bb * o]);
+        tba[bb * op + kl] = dbl(gba[(int64_t)bb * o * n1]);
       }
     }
     __syncthreads();
@@ -161,7 +164,7 @@ __global__ void __launch_bounds__(PT) unpack_tiled_kernel(const T* __restrict__ 
       const int bb = t / o, k = t - bb * o;
       const int64_t J = d.off2 + (p0 + bb) * o + k;
       const double D = ea + eps[o + b0 + bb] - 2.0 * eps[k];
-      w[J] = -(tab[bb * oo + k * o + k] + tba[bb * oo + k * o + k]) + D * x[J];
+      w[J] = -(tab[bb * op + k * o + k] + tba[bb * op + k * o + k]) + D * x[J];
     }
     // t4, t5 (a<b, k<l): contiguous over b for each occupied pair q; the
     // pair (k, l) of q is decoded once per thread and step of q
@@ -174,8 +177,8 @@ __global__ void __launch_bounds__(PT) unpack_tiled_kernel(const T* __restrict__ 
         ++k;
       }
       const int l = (int)(q - base) + k + 1;
-      const double* A = tab + bb * oo;
-      const double* Bt = tba + bb * oo;
+      const double* A = tab + bb * op;
+      const double* Bt = tba + bb * op;
       const double g1 = A[k * o + l], g3 = A[l * o + k], g2 = Bt[k * o + l], g4 = Bt[l * o + k];
       const double D = ea + eps[o + b0 + bb] - eps[k] - eps[l];
       const int64_t J = (int64_t)q * npv + p0 + bb;
@@ -201,7 +204,7 @@ int unpack_chunk(const DimsK& d) {
 template <class T>
 void launch_pack_tiled(const double* X, int64_t ldx, int nvec, const DimsK& d, T* Z, hipStream_t st) {
   const int BB = pack_chunk(d);
-  const size_t lds = sizeof(double) * (size_t)BB * (2 * d.npo + d.o);
+  const size_t lds = sizeof(double) * ((size_t)(BB + 1) * 2 * d.npo + (size_t)BB * d.o);
   const int chunks = (d.v - 1 + BB - 1) / BB;
   hipLaunchKernelGGL((pack_tiled_kernel<T>), dim3(d.v, std::max(chunks, 1), nvec), dim3(PT), lds, st, X, ldx, d, Z, BB);
   HIP_CHECK(hipGetLastError());
@@ -211,10 +214,12 @@ template <class T>
 void launch_unpack_tiled(const T* G, const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, const DimsK& d,
                          const double* eps, hipStream_t st) {
   const int BB = unpack_chunk(d);
-  const size_t lds = sizeof(double) * (size_t)2 * BB * d.o * d.o;
+  int op = d.o * d.o + 1;
+  if ((size_t)2 * BB * op * sizeof(double) > 65536) op = d.o * d.o;
+  const size_t lds = sizeof(double) * (size_t)2 * BB * op;
   const int chunks = (d.v - 1 + BB - 1) / BB;
   hipLaunchKernelGGL((unpack_tiled_kernel<T>), dim3(d.v, std::max(chunks, 1), nvec), dim3(PT), lds, st, G, X, ldx, W,
-                     ldw, d, eps, BB);
+                     ldw, d, eps, BB, op);
   HIP_CHECK(hipGetLastError());
 }
 

@@ -214,7 +214,7 @@ def davidson_compare(op, nroots, only=None):
     return res
 
 
-def sil_compare(op, sz, tfinal, seed=0):
+def sil_compare(op, sz, tfinal, autotol, seed=0):
     """SIL propagation of the benzene-sized Hamiltonian: initial state the
     seeded synthetic dipole vector (1h1p part, N(0, 0.1^2)) normalised, Lanczos
     order 20, output interval 0.1 a.u., autocorrelation every interval; FP64
@@ -226,7 +226,7 @@ def sil_compare(op, sz, tfinal, seed=0):
     out, ref = {}, None
     for prec in ("fp64", "fp32"):
         t0 = time.perf_counter()
-        auto, st = op.sil_autocorr(f, tout=0.1, tfinal=tfinal, kdim=20, autotol=1e-8, precision=prec)
+        auto, st = op.sil_autocorr(f, tout=0.1, tfinal=tfinal, kdim=20, autotol=autotol, precision=prec)
         wall = time.perf_counter() - t0
         rec = {"seconds": round(wall, 3), "records": int(auto.size), "lanczos_steps": int(st["nsteps"]),
                "sigma_vectors": int(st["nsigma"]), "max_order": int(st["max_order"])}
@@ -239,6 +239,7 @@ def sil_compare(op, sz, tfinal, seed=0):
         log(f"sil {prec}: {wall:.2f}s, {st['nsteps']} Lanczos steps, {st['nsigma']} sigma-vectors, "
             f"{auto.size} autocorrelation records")
     out["tfinal_au"] = tfinal
+    out["autotol"] = autotol
     return out
 
 
@@ -254,6 +255,7 @@ def main():
     ap.add_argument("--skip-lowprec", action="store_true")
     ap.add_argument("--sil", type=float, default=0.0,
                     help="also propagate with SIL to this final time (a.u.), FP64 and FP32 sigma")
+    ap.add_argument("--sil-tol", type=float, default=1e-10, help="SIL error tolerance (autotol)")
     ap.add_argument("--dav-runs", default="",
                     help="comma-separated subset of the Davidson comparison runs (default: all)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -313,7 +315,7 @@ def main():
     if not args.skip_davidson:
         dav = davidson_compare(op, 20, [r for r in args.dav_runs.split(",") if r])
 
-    sil = sil_compare(op, sz, args.sil) if args.sil > 0 else None
+    sil = sil_compare(op, sz, args.sil, args.sil_tol) if args.sil > 0 else None
 
     n1 = op.n1
     bytes_per_sigma = 8.0 * (2 * op.ndim) + 8.0 * 2 * n1 * n1  # read x, write sigma, pair tensors
