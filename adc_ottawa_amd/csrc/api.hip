@@ -74,6 +74,13 @@ int32_t adc2g_dims(const adc2g_handle* h, int64_t* ndim, int64_t* n1, int64_t co
   });
 }
 
+int32_t adc2g_coupling_form(const adc2g_handle* h, int32_t* form) {
+  return guarded([&] {
+    ARG_CHECK(h && form, "NULL argument");
+    *form = h->h.form == Handle::FORM_INT ? ADC2G_COUPLING_INT : ADC2G_COUPLING_DF;
+  });
+}
+
 int32_t adc2g_diagonal(adc2g_handle* h, double* hii) {
   return guarded([&] {
     ARG_CHECK(h && hii, "NULL argument");

@@ -26,11 +26,13 @@ bool direct_map(const GemmArgs& g) {
 
 // Threads per workgroup: FP64 128x128 tiles run 8 waves (each 64x32) so that
 // two workgroups per CU keep four waves per SIMD; the rest run 4 waves.
-template <class T, int BM> constexpr int threads_for() { return (sizeof(T) == 8 && BM == 128) ? 512 : 256; }
+template <class T, int BM, int BN> constexpr int threads_for() {
+  return (sizeof(T) == 8 && BM == 128 && BN == 128) ? 512 : 256;
+}
 
 template <class T, int BM, int BN, int BK, class OutT, int TAG, int MK>
 void launch_k(const GemmArgs& g, dim3 grid, hipStream_t st) {
-  constexpr int NT = threads_for<T, BM>();
+  constexpr int NT = threads_for<T, BM, BN>();
   if (g.vec)
     hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, OutT, TAG, true, MK, NT>), grid, dim3(NT), 0, st, g);
   else
@@ -59,9 +61,33 @@ template <class T>
 void dispatch(GemmArgs g, GemmWorkspace& ws, hipStream_t st, int splitk_hint) {
   typedef typename Mma<T>::Acc Acc;
   constexpr int BK = (sizeof(T) == 8) ? 16 : (sizeof(T) == 4 ? 32 : 64);
-  const bool small = (g.M <= 64 || g.N <= 64);
-  const int BMs = small ? 64 : 128;
-  const int tiles = ((g.M + BMs - 1) / BMs) * ((g.N + BMs - 1) / BMs) * g.batch;
+  // tile shape: 128x32 for N <= 32, 64x64 when M or N is small; otherwise
+  // the shape with the least padded output (within 2%), larger tiles first
+  static const int SBM[6] = {0, 128, 64, 128, 128, 64};
+  static const int SBN[6] = {0, 128, 64, 32, 64, 128};
+  int shape = g.tile;
+  if (shape <= 0) {
+    auto padded = [&](int sh) {
+      const int bm = SBM[sh], bn = SBN[sh];
+      return (double)((g.M + bm - 1) / bm * bm) * (double)((g.N + bn - 1) / bn * bn);
+    };
+    if (g.N <= 32 && g.M > 64) {
+      shape = 3;
+    } else if (g.M <= 64 || g.N <= 64) {
+      shape = 2;
+    } else {
+      const int order[4] = {1, 4, 5, 2};
+      double best = padded(1);
+      for (int sh : order) best = std::min(best, padded(sh));
+      for (int sh : order)
+        if (padded(sh) <= 1.02 * best) {
+          shape = sh;
+          break;
+        }
+    }
+  }
+  const int BMs = SBM[shape], BNs = SBN[shape];
+  const int tiles = ((g.M + BMs - 1) / BMs) * ((g.N + BNs - 1) / BNs) * g.batch;
   const int ktiles = (g.K + BK - 1) / BK;
   int split = splitk_hint;
   if (split <= 0) {
@@ -88,20 +114,27 @@ void dispatch(GemmArgs g, GemmWorkspace& ws, hipStream_t st, int splitk_hint) {
               ? 1 : 0;
   // without split-K the map is applied in-kernel when an instantiation has it,
   // otherwise by the reduction kernel from the partial
-  g.direct = (split == 1 && direct_map<T>(g)) ? 1 : 0;
+  g.direct = (split == 1 && !g.reduce_batch && direct_map<T>(g)) ? 1 : 0;
   const bool need_partial = !g.direct;
   if (need_partial) {
     const size_t need = (size_t)g.batch * split * (size_t)g.M * g.N * sizeof(Acc);
     ws.buf.alloc(need);
     g.partial = ws.buf.get();
   }
-  if (small)
-    launch<T, 64, 64, BK>(g, st);
-  else
-    launch<T, 128, 128, BK>(g, st);
+  switch (shape) {
+    case 2: launch<T, 64, 64, BK>(g, st); break;
+    case 3: launch<T, 128, 32, BK>(g, st); break;
+    case 4: launch<T, 128, 64, BK>(g, st); break;
+    case 5: launch<T, 64, 128, BK>(g, st); break;
+    default: launch<T, 128, 128, BK>(g, st);
+  }
   if (need_partial) {
+    if (g.reduce_batch) {  // the batch partials are further splits of one sum
+      g.splitk *= g.batch;
+      g.batch = 1;
+    }
     const int64_t total = (int64_t)g.M * g.N * g.batch;
-    if (split >= 8 && total <= (int64_t)1 << 22) {  // many splits, few outputs
+    if (g.splitk >= 8 && total <= (int64_t)1 << 22) {  // many splits, few outputs
       const int wb = (int)((total + 63) / 64);
       if (g.out_prec == 1)
         hipLaunchKernelGGL((splitk_reduce_wide_kernel<Acc, float>), dim3(wb), dim3(256), 0, st, g);

@@ -52,6 +52,12 @@ struct GemmArgs {
   // vectors (the VEC instantiation)
   int vec = 0;
   int direct = 0;           // set by the launcher: apply the map in the kernel epilogue
+  // 1: the batches are terms of one sum (per-batch partials, reduced over
+  // batches and splits into a single M x N output through the map)
+  int reduce_batch = 0;
+  // tile shape: 0 chosen by the launcher, 1 = 128x128, 2 = 64x64, 3 = 128x32,
+  // 4 = 128x64, 5 = 64x128
+  int tile = 0;
   void* partial = nullptr;  // split-K workspace (accumulator type), batch*splitk*M*N
   // host-side only: kernel tag (1 = the DF coupling contractions, compiled as a
   // separately named kernel so profilers report them apart) and optional
@@ -236,9 +242,10 @@ struct Mma<__bf16> {
   static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-// XCD-aware tile order: workgroup w runs on XCD w % 8; give each XCD a
-// contiguous range of the (m-fastest) tile sequence so tiles sharing an
-// operand panel share that XCD's L2.
+// XCD-aware work order: the w-th workgroup dispatched (flattened over the
+// grid) runs on XCD w % 8; give each XCD a contiguous range of the work
+// sequence (tiles m-fastest, then batches, then K splits) so that tiles
+// sharing an operand panel, and all tiles of one batch, share that XCD's L2.
 __device__ __forceinline__ int xcd_tile(int w, int T) {
   const int per = T >> 3, rem = T & 7;
   const int x = w & 7, slot = w >> 3;
@@ -275,10 +282,12 @@ gemm_kernel(GemmArgs g) {
   const int tiles_m = (g.M + BM - 1) / BM;
   const int tiles_n = (g.N + BN - 1) / BN;
   const int T_all = tiles_m * tiles_n;
-  const int t = xcd_tile(blockIdx.x, T_all);
+  const int w = xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
+                         gridDim.x * gridDim.y * gridDim.z);
+  const int t = w % T_all;
+  const int b = (w / T_all) % gridDim.y;  // batch
+  const int ks = w / (T_all * gridDim.y);  // split index
   const int tm = t % tiles_m, tn = t / tiles_m;
-  const int b = blockIdx.y;       // batch
-  const int ks = blockIdx.z;      // split index
   const int m0 = tm * BM, n0 = tn * BN;
 
   // K range of this split, in whole K tiles

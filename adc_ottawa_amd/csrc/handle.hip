@@ -5,9 +5,11 @@
 // get_offdiag_adc2_save_omp, source/adclib/get_matrix.f90:705-1186) and then
 // multiplies with it (hxvec_incore, source/eigen/block_davidson.f90:844-883).
 // Here only the dense ph-ph block (n1 x n1) is built; the 1h1p<->2h2p
-// coupling is applied on the fly as density-fitted contractions, and the
-// 2h2p-2h2p block (diagonal, adc_ph.f90:1090-1096) is fused into the
-// pair-to-CSF projection.
+// coupling is applied on the fly as MFMA contractions of the pair tensor of
+// the 2h2p amplitudes, either with the density-fitting factors (DF form, sums
+// over naux) or with the (vv|ov) and (oo|ov) blocks assembled once from them
+// (INT form, sums over v and o), and the 2h2p-2h2p block (diagonal,
+// adc_ph.f90:1090-1096) is fused into the pair-to-CSF projection.
 #include "handle.hpp"
 
 #include <algorithm>
@@ -41,8 +43,8 @@ hipEvent_t Handle::prof_event() {
   return ev_pool[ev_used++];
 }
 
-void Handle::prof_collect(double out[9]) {
-  for (int i = 0; i < 9; ++i) out[i] = 0.0;
+void Handle::prof_collect(double out[12]) {
+  for (int i = 0; i < 12; ++i) out[i] = 0.0;
   HIP_CHECK(hipDeviceSynchronize());  // records may sit on a caller's stream
   for (const auto& r : prof) {
     float ms = 0.0f;
@@ -58,6 +60,7 @@ void Handle::prof_collect(double out[9]) {
 void Handle::create(int o, int v, int naux, const double* eps_h, const double* B_h, const adc2g_opts& opts) {
   ARG_CHECK(o >= 2 && v >= 2 && naux >= 1, "need nocc >= 2, nvirt >= 2, naux >= 1");
   ARG_CHECK(eps_h && B_h, "eps and B must not be NULL");
+  ARG_CHECK(opts.coupling >= ADC2G_COUPLING_AUTO && opts.coupling <= ADC2G_COUPLING_INT, "unknown coupling form");
   device = opts.device;
   HIP_CHECK(hipSetDevice(device));
   HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -76,6 +79,18 @@ void Handle::create(int o, int v, int naux, const double* eps_h, const double* B
   launch_extract_factors(dB.get(), nbas, dk, Bov.get(), Bvv.get(), Boo.get(), st);
   build_phph(dB.get(), nbas);
   HIP_CHECK(hipStreamSynchronize(st));
+  // coupling form: the integral blocks need (v + o)/naux of the flops of the
+  // factor contractions, for v*v*n1 + o*o*n1 stored elements (twice: both
+  // operand layouts) per precision
+  size_t need = integral_bytes(Prec::F64);
+  if (opts.keep_lowprec) need += integral_bytes(Prec::F32) + integral_bytes(Prec::BF16);
+  const size_t int_budget = (size_t)96 << 30;
+  if (opts.coupling == ADC2G_COUPLING_INT)
+    form = FORM_INT;
+  else if (opts.coupling == ADC2G_COUPLING_DF)
+    form = FORM_DF;
+  else
+    form = ((double)(o + v) < 0.6 * naux && need <= int_budget) ? FORM_INT : FORM_DF;
   ensure_lowprec(Prec::F64);
   if (opts.keep_lowprec) {
     ensure_lowprec(Prec::F32);
@@ -89,8 +104,79 @@ void Handle::create(int o, int v, int naux, const double* eps_h, const double* B
 
 static int prec_index(Prec p) { return p == Prec::F64 ? 0 : (p == Prec::F32 ? 1 : 2); }
 
-// Build the factor set of precision p (once).
+size_t Handle::integral_bytes(Prec p) const {
+  return prec_size(p) * 2 * ((size_t)d.v * d.v + (size_t)d.o * d.o) * (size_t)d.n1;
+}
+
+// FP64 integral blocks of the INT form, assembled from the factors by GEMMs:
+// V[(a,c)][bl] = sum_P Bvv[P][a][c] Bov[P][bl], O[(k,m)][bl] = sum_P
+// Boo[P][k][m] Bov[P][bl]; Vt and Ot are the same numbers in the layouts the
+// up contractions read.
+void Handle::build_integrals() {
+  IntSet& s = is[0];
+  if (s.ready) return;
+  const int o = d.o, v = d.v, naux = d.naux;
+  const int64_t n1 = d.n1;
+  const int64_t nV = (int64_t)v * v * n1, nO = (int64_t)o * o * n1;
+  s.V.alloc(sizeof(double) * nV);
+  s.Vt.alloc(sizeof(double) * nV);
+  s.O.alloc(sizeof(double) * nO);
+  s.Ot.alloc(sizeof(double) * nO);
+  for (int which = 0; which < 2; ++which) {
+    GemmArgs g;
+    const int r = which == 0 ? v : o;
+    g.M = r * r; g.N = (int)n1; g.K = naux;
+    g.A = which == 0 ? (const void*)Bvv.get() : (const void*)Boo.get();
+    g.sam = 1; g.sak = (int64_t)r * r;
+    g.B = Bov.get(); g.sbk = n1; g.sbn = 1;
+    g.map.C = which == 0 ? s.V.get() : s.O.get();
+    g.map.scm = n1; g.map.scn = 1;
+    gemm(g, Prec::F64, gws, st);
+  }
+  const double* V = (const double*)s.V.get();
+  for (int c = 0; c < v; ++c)
+    launch_transpose<double, double>(V + (int64_t)c * v * n1, v, n1, (double*)s.Vt.get() + (int64_t)c * n1 * v, st);
+  launch_swap01<double, double>((const double*)s.O.get(), o, o, n1, (double*)s.Ot.get(), st);
+  HIP_CHECK(hipStreamSynchronize(st));
+  s.ready = true;
+}
+
+// Build the coupling operands of precision p (once): the factor set (DF form)
+// or the integral blocks (INT form, rounded from the FP64 ones).
 void Handle::ensure_lowprec(Prec p) {
+  if (form == FORM_INT) {
+    build_integrals();
+    IntSet& s = is[prec_index(p)];
+    if (s.ready) return;
+    const IntSet& f = is[0];
+    const int o = d.o, v = d.v;
+    const int64_t n1 = d.n1;
+    const int64_t nV = (int64_t)v * v * n1, nO = (int64_t)o * o * n1;
+    const size_t es = prec_size(p);
+    s.V.alloc(es * nV);
+    s.Vt.alloc(es * nV);
+    s.O.alloc(es * nO);
+    s.Ot.alloc(es * nO);
+    const double* V = (const double*)f.V.get();
+    const double* O = (const double*)f.O.get();
+    if (p == Prec::F32) {
+      launch_convert_f32(V, (float*)s.V.get(), nV, st);
+      launch_convert_f32(O, (float*)s.O.get(), nO, st);
+      for (int c = 0; c < v; ++c)
+        launch_transpose<double, float>(V + (int64_t)c * v * n1, v, n1, (float*)s.Vt.get() + (int64_t)c * n1 * v, st);
+      launch_swap01<double, float>(O, o, o, n1, (float*)s.Ot.get(), st);
+    } else {
+      launch_convert_bf16(V, (__bf16*)s.V.get(), nV, st);
+      launch_convert_bf16(O, (__bf16*)s.O.get(), nO, st);
+      for (int c = 0; c < v; ++c)
+        launch_transpose<double, __bf16>(V + (int64_t)c * v * n1, v, n1, (__bf16*)s.Vt.get() + (int64_t)c * n1 * v,
+                                         st);
+      launch_swap01<double, __bf16>(O, o, o, n1, (__bf16*)s.Ot.get(), st);
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
+    s.ready = true;
+    return;
+  }
   FactorSet& f = fs[prec_index(p)];
   if (f.ready) return;
   const int64_t n1 = d.n1, naux = d.naux;
@@ -209,7 +295,12 @@ void Handle::build_phph(const double* dB, int nbas) {
 
 size_t Handle::sigma_workspace_bytes(int nc, Prec p) const {
   const size_t n1 = d.n1, naux = d.naux, es = prec_size(p);
-  size_t b = n1 * n1 * nc * es;                  // Z / G
+  size_t b = n1 * n1 * nc * es;  // Z / G
+  if (form == FORM_INT) {
+    b += 2 * n1 * nc * es;                         // x1 in two layouts
+    b += (size_t)d.v * d.v * d.o * nc * sizeof(double);  // per-c partials of the down contraction
+    return b;
+  }
   b += 2 * naux * n1 * nc * es;                  // Yvv, Yoo
   b += naux * n1 * nc * (p == Prec::F64 ? 8 : 4);  // W
   b += naux * n1 * nc * es;                      // W transposed
@@ -217,11 +308,161 @@ size_t Handle::sigma_workspace_bytes(int nc, Prec p) const {
   return b;
 }
 
-// W = H X for nvec columns (device pointers, ld = ldx / ldw).
-//
-// Per chunk of nc vectors, with T the GEMM precision p (the 1h1p-1h1p block
-// H11 x1 always FP64; every coupling contraction in T with FP32 accumulation
-// for FP32/BF16):
+// W = H X for nvec columns (device pointers, ld = ldx / ldw), in chunks of at
+// most `chunk` vectors.  The 1h1p-1h1p block H11 x1 is always FP64; every
+// coupling contraction runs in the GEMM precision p (FP32 accumulation for
+// FP32/BF16), the 2h2p diagonal in FP64 inside the pair->CSF projection.
+void Handle::sigma(const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, Prec p, hipStream_t s) {
+  ARG_CHECK(ldx >= d.ndim && ldw >= d.ndim, "leading dimensions must be >= ndim");
+  if (nvec <= 0) return;
+  if (!s) s = st;
+  ensure_lowprec(p);
+  const int nc_max = std::min(chunk, nvec);
+  const size_t es = prec_size(p);
+  const size_t n1 = d.n1, naux = d.naux;
+  X1c.alloc(n1 * nc_max * es);
+  ZG.alloc(n1 * n1 * nc_max * es);
+  if (form == FORM_INT) {
+    X1t.alloc(n1 * nc_max * es);
+  } else {
+    const size_t wes = (p == Prec::F64) ? 8 : 4;  // W accumulator type
+    Yvv.alloc(naux * n1 * nc_max * es);
+    Yoo.alloc(naux * n1 * nc_max * es);
+    Wup.alloc(naux * n1 * nc_max * wes);
+    WupT.alloc(naux * n1 * nc_max * es);
+  }
+  for (int c0 = 0; c0 < nvec; c0 += nc_max) {
+    const int nc = std::min(nc_max, nvec - c0);
+    if (form == FORM_INT)
+      sigma_int(X + ldx * c0, ldx, W + ldw * c0, ldw, nc, p, s);
+    else
+      sigma_df(X + ldx * c0, ldx, W + ldw * c0, ldw, nc, p, s);
+  }
+}
+
+namespace {
+
+template <class T>
+void pack_chunk_t(const double* Xc, int64_t ldx, int nc, const DimsK& dk, void* Z, void* X1c, void* X1t,
+                  const PairLayout& lay, hipStream_t s) {
+  launch_csf_to_pair<T>(Xc, ldx, nc, dk, (T*)Z, lay, s);
+  launch_pack_x1<T>(Xc, ldx, nc, dk, (T*)X1c, s);
+  if (X1t) launch_pack_x1t<T>(Xc, ldx, nc, dk, (T*)X1t, s);
+}
+
+void pack_chunk(Prec p, const double* Xc, int64_t ldx, int nc, const DimsK& dk, void* Z, void* X1c, void* X1t,
+                const PairLayout& lay, hipStream_t s) {
+  switch (p) {
+    case Prec::F64: pack_chunk_t<double>(Xc, ldx, nc, dk, Z, X1c, X1t, lay, s); break;
+    case Prec::F32: pack_chunk_t<float>(Xc, ldx, nc, dk, Z, X1c, X1t, lay, s); break;
+    case Prec::BF16: pack_chunk_t<__bf16>(Xc, ldx, nc, dk, Z, X1c, X1t, lay, s); break;
+  }
+}
+
+void unpack_chunk(Prec p, const void* G, const double* Xc, int64_t ldx, double* Wc, int64_t ldw, int nc,
+                  const DimsK& dk, const double* eps, const PairLayout& lay, hipStream_t s) {
+  switch (p) {
+    case Prec::F64: launch_pair_to_csf<double>((const double*)G, Xc, ldx, Wc, ldw, nc, dk, eps, lay, s); break;
+    case Prec::F32: launch_pair_to_csf<float>((const float*)G, Xc, ldx, Wc, ldw, nc, dk, eps, lay, s); break;
+    case Prec::BF16: launch_pair_to_csf<__bf16>((const __bf16*)G, Xc, ldx, Wc, ldw, nc, dk, eps, lay, s); break;
+  }
+}
+
+}  // namespace
+
+// INT form, one chunk of nc vectors.  With the pair tensor stored
+// virtual-major, Z[c][vec][k][(b,l)] = Z(c,b,k,l) of vector vec:
+//   s1(a,k)   = H11 x1                                              (FP64)
+//   s1(a,k)  += sum_c sum_bl V[a][c][bl] Z[c][vec][k][bl]           (down, batched over c, summed)
+//   s1(a,k)  -= sum_{m,bl} Z[a][vec][m][bl] O[k][m][bl]             (down, occupied)
+//   G         = -sum_m x1(c,m) Ot[m][k][bl]   [c][vec][k][bl]        (up, occupied)
+//   G        += sum_a x1(a,k) Vt[c][bl][a]                           (up, batched over c)
+//   s2        = projection of G onto the CSFs + D x2                 (unpack)
+// which is sigma1(a,k) = sum_{P,c} B[P,a,c] Y[P,c,k] - sum_{P,m} B[P,k,m]
+// Y[P,a,m] and G = sum_P W[P,c,k] B[P,l,b] of the DF form with the sum over P
+// taken first.
+void Handle::sigma_int(const double* Xc, int64_t ldx, double* Wc, int64_t ldw, int nc, Prec p, hipStream_t s) {
+  const int o = d.o, v = d.v;
+  const int64_t n1 = d.n1;
+  const int out_lp = (p == Prec::F64) ? 0 : (p == Prec::F32 ? 1 : 2);
+  const IntSet& I = is[prec_index(p)];
+  const PairLayout lay = pair_layout_virt_major(dk, nc);
+  const double esd = (double)prec_size(p);
+  auto span = [&](int cls, double work) -> std::pair<hipEvent_t, hipEvent_t> {
+    if (!prof_on) return {nullptr, nullptr};
+    hipEvent_t a = prof_event(), b = prof_event();
+    prof.push_back({cls, a, b, work});
+    return {a, b};
+  };
+  const double flops_v = 2.0 * v * (double)n1 * n1 * nc;  // each virtual contraction
+  const double flops_o = 2.0 * o * (double)n1 * n1 * nc;  // each occupied contraction
+  auto ep = span(1, (double)nc * (d.n2 * 8.0 + (double)n1 * n1 * esd));
+  if (ep.first) HIP_CHECK(hipEventRecord(ep.first, s));
+  pack_chunk(p, Xc, ldx, nc, dk, ZG.get(), X1c.get(), X1t.get(), lay, s);
+  if (ep.second) HIP_CHECK(hipEventRecord(ep.second, s));
+  {  // sigma1 = H11 x1 (FP64)
+    GemmArgs g;
+    g.M = (int)n1; g.N = nc; g.K = (int)n1;
+    g.A = H11.get(); g.sam = n1; g.sak = 1;
+    g.B = Xc; g.sbk = 1; g.sbn = ldx;
+    g.map.C = Wc; g.map.scm = 1; g.map.scn = ldw;
+    gemm(g, Prec::F64, gws, s);
+  }
+  {  // sigma1(a, k + o*vec) += sum_{c,bl} V[a][c][bl] Z[c][vec][k][bl]
+    GemmArgs g;
+    g.M = v; g.N = o * nc; g.K = (int)n1; g.batch = v; g.reduce_batch = 1;
+    g.A = I.V.get(); g.sam = (int64_t)v * n1; g.sak = 1; g.sab = n1;
+    g.B = ZG.get(); g.sbn = n1; g.sbk = 1; g.sbb = (int64_t)nc * o * n1;
+    g.map.kind = MAP_SIG1V; g.map.C = Wc; g.map.beta = 1.0;
+    g.map.q[0] = o; g.map.q[1] = ldw;
+    g.tag = 1;
+    auto e = span(0, flops_v);
+    g.ev_begin = e.first; g.ev_end = e.second;
+    gemm(g, p, gws, s);
+  }
+  {  // sigma1(a*nc + vec, k) -= sum_{m,bl} Z[a][vec][m][bl] O[k][m][bl]
+    GemmArgs g;
+    g.M = v * nc; g.N = o; g.K = (int)(o * n1);
+    g.A = ZG.get(); g.sam = (int64_t)o * n1; g.sak = 1;
+    g.B = I.O.get(); g.sbn = (int64_t)o * n1; g.sbk = 1;
+    g.map.kind = MAP_SIG1O; g.map.C = Wc; g.map.alpha = -1.0; g.map.beta = 1.0;
+    g.map.q[0] = o; g.map.q[1] = ldw; g.map.q[2] = nc;
+    auto e = span(3, flops_o);
+    g.ev_begin = e.first; g.ev_end = e.second;
+    gemm(g, p, gws, s);
+  }
+  {  // G[(c*nc + vec)][(k,bl)] = -sum_m x1(c,m) Ot[m][k][bl]   (overwrites Z)
+    GemmArgs g;
+    g.M = v * nc; g.N = (int)(o * n1); g.K = o;
+    g.A = X1c.get(); g.sam = o; g.sak = 1;
+    g.B = I.Ot.get(); g.sbk = (int64_t)o * n1; g.sbn = 1;
+    g.map.C = ZG.get(); g.map.scm = (int64_t)o * n1; g.map.scn = 1; g.map.alpha = -1.0;
+    g.out_prec = out_lp;
+    auto e = span(3, flops_o);
+    g.ev_begin = e.first; g.ev_end = e.second;
+    gemm(g, p, gws, s);
+  }
+  {  // G[c][(vec,k)][bl] += sum_a x1t[(vec,k)][a] Vt[c][bl][a]
+    GemmArgs g;
+    g.tile = 2;  // 64x64: K = v is short and the epilogue reads and writes G; measured
+                 // ahead of 64x128 and 128x128 at o=16, v=200, 20 vectors
+    g.M = o * nc; g.N = (int)n1; g.K = v; g.batch = v;
+    g.A = X1t.get(); g.sam = v; g.sak = 1; g.sab = 0;
+    g.B = I.Vt.get(); g.sbn = v; g.sbk = 1; g.sbb = n1 * v;
+    g.map.C = ZG.get(); g.map.scm = n1; g.map.scn = 1; g.map.scb = (int64_t)nc * o * n1; g.map.beta = 1.0;
+    g.out_prec = out_lp;
+    g.tag = 1;
+    auto e = span(0, flops_v);
+    g.ev_begin = e.first; g.ev_end = e.second;
+    gemm(g, p, gws, s);
+  }
+  auto eu = span(2, (double)nc * (3.0 * d.n2 * 8.0 + (double)n1 * n1 * esd));
+  if (eu.first) HIP_CHECK(hipEventRecord(eu.first, s));
+  unpack_chunk(p, ZG.get(), Xc, ldx, Wc, ldw, nc, dk, eps.get(), lay, s);
+  if (eu.second) HIP_CHECK(hipEventRecord(eu.second, s));
+}
+
+// DF form, one chunk of nc vectors (T the GEMM precision p):
 //   Z    = pairs(x2)                              [vec][(c,k)][(b,l)]   (pack)
 //   s1   = H11 x1                                                      (FP64)
 //   Y    = Bov . Z   -> Yvv [(c)][(P)]..., Yoo                         (down)
@@ -230,160 +471,124 @@ size_t Handle::sigma_workspace_bytes(int nc, Prec p) const {
 //   Wt   = W^T                                    [(vec,c,k)][P]
 //   G    = Wt . BovT^T                            [vec][(c,k)][(b,l)]   (up)
 //   s2   = projection of G onto the CSFs + D x2                        (unpack)
-void Handle::sigma(const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, Prec p, hipStream_t s) {
-  ARG_CHECK(ldx >= d.ndim && ldw >= d.ndim, "leading dimensions must be >= ndim");
-  if (nvec <= 0) return;
-  if (!s) s = st;
-  ensure_lowprec(p);
+void Handle::sigma_df(const double* Xc, int64_t ldx, double* Wc, int64_t ldw, int nc, Prec p, hipStream_t s) {
   const int o = d.o, v = d.v, naux = d.naux;
   const int64_t n1 = d.n1;
-  const int nc_max = std::min(chunk, nvec);
-  const size_t es = prec_size(p);
-  const size_t wes = (p == Prec::F64) ? 8 : 4;  // W accumulator type
   const int out_lp = (p == Prec::F64) ? 0 : (p == Prec::F32 ? 1 : 2);
   const int out_w = (p == Prec::F64) ? 0 : 1;
-  X1c.alloc(n1 * nc_max * es);
-  Yvv.alloc((size_t)naux * n1 * nc_max * es);
-  Yoo.alloc((size_t)naux * n1 * nc_max * es);
-  Wup.alloc((size_t)naux * n1 * nc_max * wes);
-  WupT.alloc((size_t)naux * n1 * nc_max * es);
-  ZG.alloc((size_t)n1 * n1 * nc_max * es);
+  const PairLayout lay = pair_layout_vec_major(dk);
   const void* BovP = fBov(p);
   const void* BovTP = fBovT(p);
   const void* BvvP = fBvv(p);
   const void* BooP = fBoo(p);
-  const double esd = (double)es;
+  const double esd = (double)prec_size(p);
   auto span = [&](int cls, double work) -> std::pair<hipEvent_t, hipEvent_t> {
     if (!prof_on) return {nullptr, nullptr};
     hipEvent_t a = prof_event(), b = prof_event();
     prof.push_back({cls, a, b, work});
     return {a, b};
   };
-  for (int c0 = 0; c0 < nvec; c0 += nc_max) {
-    const int nc = std::min(nc_max, nvec - c0);
-    const double* Xc = X + ldx * c0;
-    double* Wc = W + ldw * c0;
-    auto ep = span(1, (double)nc * (d.n2 * 8.0 + (double)n1 * n1 * esd));
-    if (ep.first) HIP_CHECK(hipEventRecord(ep.first, s));
-    switch (p) {
-      case Prec::F64:
-        launch_csf_to_pair<double>(Xc, ldx, nc, dk, (double*)ZG.get(), s);
-        launch_pack_x1<double>(Xc, ldx, nc, dk, (double*)X1c.get(), s);
-        break;
-      case Prec::F32:
-        launch_csf_to_pair<float>(Xc, ldx, nc, dk, (float*)ZG.get(), s);
-        launch_pack_x1<float>(Xc, ldx, nc, dk, (float*)X1c.get(), s);
-        break;
-      case Prec::BF16:
-        launch_csf_to_pair<__bf16>(Xc, ldx, nc, dk, (__bf16*)ZG.get(), s);
-        launch_pack_x1<__bf16>(Xc, ldx, nc, dk, (__bf16*)X1c.get(), s);
-        break;
-    }
-    if (ep.second) HIP_CHECK(hipEventRecord(ep.second, s));
-    // sigma1 = H11 x1 (FP64)
-    {
-      GemmArgs g;
-      g.M = (int)n1; g.N = nc; g.K = (int)n1;
-      g.A = H11.get(); g.sam = n1; g.sak = 1;
-      g.B = Xc; g.sbk = 1; g.sbn = ldx;
-      g.map.C = Wc; g.map.scm = 1; g.map.scn = ldw;
-      gemm(g, Prec::F64, gws, s);
-    }
-    // down: Y[P,(c,k)] = sum_{bl} B[P,bl] Z[(c,k),(b,l)]  -> Yvv and Yoo layouts
-    {
-      GemmArgs g;
-      g.M = naux; g.N = (int)(n1 * nc); g.K = (int)n1;
-      g.A = BovP; g.sam = n1; g.sak = 1;
-      g.B = ZG.get(); g.sbk = 1; g.sbn = n1;
-      g.map.kind = MAP_DOWNY; g.map.C = Yvv.get(); g.map.C2 = Yoo.get();
-      g.map.q[0] = o; g.map.q[1] = nc; g.map.q[2] = n1; g.map.q[3] = naux;
-      g.out_prec = out_lp;
-      g.tag = 1;
-      auto e = span(0, 2.0 * naux * (double)n1 * n1 * nc);
-      g.ev_begin = e.first; g.ev_end = e.second;
-      gemm(g, p, gws, s);
-    }
-    // sigma1(a,k) += sum_{P,c} B[P,a,c] Y[P,c,k]
-    {
-      GemmArgs g;
-      g.M = v; g.N = o * nc; g.K = v * naux;
-      g.A = BvvP; g.sam = 1; g.sak = v;
-      g.B = Yvv.get(); g.sbk = (int64_t)o * nc; g.sbn = 1;
-      g.map.kind = MAP_SIG1V; g.map.C = Wc; g.map.beta = 1.0;
-      g.map.q[0] = o; g.map.q[1] = ldw;
-      gemm(g, p, gws, s);
-    }
-    // sigma1(a,k) -= sum_{P,m} B[P,k,m] Y[P,a,m]
-    {
-      GemmArgs g;
-      g.M = v * nc; g.N = o; g.K = o * naux;
-      g.A = Yoo.get(); g.sam = (int64_t)o * naux; g.sak = 1;
-      g.B = BooP; g.sbk = o; g.sbn = 1;
-      g.map.kind = MAP_SIG1O; g.map.C = Wc; g.map.alpha = -1.0; g.map.beta = 1.0;
-      g.map.q[0] = o; g.map.q[1] = ldw; g.map.q[2] = nc;
-      gemm(g, p, gws, s);
-    }
-    // W[P,vec,c,k] = sum_a B[P,c,a] x1(a,k) - sum_m B[P,k,m] x1(c,m)
-    {
-      GemmArgs g;
-      g.M = v * naux; g.N = o * nc; g.K = v;
-      g.A = BvvP; g.sam = v; g.sak = 1;
-      g.B = X1c.get(); g.sbk = (int64_t)o * nc; g.sbn = 1;
-      g.map.kind = MAP_WUPV; g.map.C = Wup.get();
-      g.map.q[0] = o; g.map.q[1] = v; g.map.q[2] = n1; g.map.q[3] = nc;
-      g.out_prec = out_w;
-      gemm(g, p, gws, s);
-    }
-    {
-      GemmArgs g;
-      g.M = v * nc; g.N = o * naux; g.K = o;
-      g.A = X1c.get(); g.sam = o; g.sak = 1;
-      g.B = BooP; g.sbk = 1; g.sbn = o;
-      g.map.kind = MAP_WUPO; g.map.C = Wup.get(); g.map.alpha = -1.0; g.map.beta = 1.0;
-      g.map.q[0] = o; g.map.q[1] = v; g.map.q[2] = n1; g.map.q[3] = nc;
-      g.out_prec = out_w;
-      gemm(g, p, gws, s);
-    }
-    // Wt[(vec,c,k)][P] = W[P][(vec,c,k)] in the GEMM precision
-    switch (p) {
-      case Prec::F64:
-        launch_transpose<double, double>((const double*)Wup.get(), naux, n1 * nc, (double*)WupT.get(), s);
-        break;
-      case Prec::F32:
-        launch_transpose<float, float>((const float*)Wup.get(), naux, n1 * nc, (float*)WupT.get(), s);
-        break;
-      case Prec::BF16:
-        launch_transpose<float, __bf16>((const float*)Wup.get(), naux, n1 * nc, (__bf16*)WupT.get(), s);
-        break;
-    }
-    // up: G[(vec,c,k),(b,l)] = sum_P Wt[(vec,c,k),P] BovT[(b,l),P]  (both K-contiguous)
-    {
-      GemmArgs g;
-      g.M = (int)(n1 * nc); g.N = (int)n1; g.K = naux;
-      g.A = WupT.get(); g.sam = naux; g.sak = 1;
-      g.B = BovTP; g.sbk = 1; g.sbn = naux;
-      g.map.C = ZG.get(); g.map.scm = n1; g.map.scn = 1;
-      g.out_prec = out_lp;
-      g.tag = 1;
-      auto e = span(0, 2.0 * naux * (double)n1 * n1 * nc);
-      g.ev_begin = e.first; g.ev_end = e.second;
-      gemm(g, p, gws, s);
-    }
-    auto eu = span(2, (double)nc * (3.0 * d.n2 * 8.0 + (double)n1 * n1 * esd));
-    if (eu.first) HIP_CHECK(hipEventRecord(eu.first, s));
-    switch (p) {
-      case Prec::F64:
-        launch_pair_to_csf<double>((const double*)ZG.get(), Xc, ldx, Wc, ldw, nc, dk, eps.get(), s);
-        break;
-      case Prec::F32:
-        launch_pair_to_csf<float>((const float*)ZG.get(), Xc, ldx, Wc, ldw, nc, dk, eps.get(), s);
-        break;
-      case Prec::BF16:
-        launch_pair_to_csf<__bf16>((const __bf16*)ZG.get(), Xc, ldx, Wc, ldw, nc, dk, eps.get(), s);
-        break;
-    }
-    if (eu.second) HIP_CHECK(hipEventRecord(eu.second, s));
+  const double flops_w = 2.0 * (v + o) * (double)naux * n1 * nc;  // each of the sigma1 and W contraction pairs
+  auto ep = span(1, (double)nc * (d.n2 * 8.0 + (double)n1 * n1 * esd));
+  if (ep.first) HIP_CHECK(hipEventRecord(ep.first, s));
+  pack_chunk(p, Xc, ldx, nc, dk, ZG.get(), X1c.get(), nullptr, lay, s);
+  if (ep.second) HIP_CHECK(hipEventRecord(ep.second, s));
+  // sigma1 = H11 x1 (FP64)
+  {
+    GemmArgs g;
+    g.M = (int)n1; g.N = nc; g.K = (int)n1;
+    g.A = H11.get(); g.sam = n1; g.sak = 1;
+    g.B = Xc; g.sbk = 1; g.sbn = ldx;
+    g.map.C = Wc; g.map.scm = 1; g.map.scn = ldw;
+    gemm(g, Prec::F64, gws, s);
   }
+  // down: Y[P,(c,k)] = sum_{bl} B[P,bl] Z[(c,k),(b,l)]  -> Yvv and Yoo layouts
+  {
+    GemmArgs g;
+    g.M = naux; g.N = (int)(n1 * nc); g.K = (int)n1;
+    g.A = BovP; g.sam = n1; g.sak = 1;
+    g.B = ZG.get(); g.sbk = 1; g.sbn = n1;
+    g.map.kind = MAP_DOWNY; g.map.C = Yvv.get(); g.map.C2 = Yoo.get();
+    g.map.q[0] = o; g.map.q[1] = nc; g.map.q[2] = n1; g.map.q[3] = naux;
+    g.out_prec = out_lp;
+    g.tag = 1;
+    auto e = span(0, 2.0 * naux * (double)n1 * n1 * nc);
+    g.ev_begin = e.first; g.ev_end = e.second;
+    gemm(g, p, gws, s);
+  }
+  auto e1 = span(3, 2.0 * flops_w);
+  if (e1.first) HIP_CHECK(hipEventRecord(e1.first, s));
+  // sigma1(a,k) += sum_{P,c} B[P,a,c] Y[P,c,k]
+  {
+    GemmArgs g;
+    g.M = v; g.N = o * nc; g.K = v * naux;
+    g.A = BvvP; g.sam = 1; g.sak = v;
+    g.B = Yvv.get(); g.sbk = (int64_t)o * nc; g.sbn = 1;
+    g.map.kind = MAP_SIG1V; g.map.C = Wc; g.map.beta = 1.0;
+    g.map.q[0] = o; g.map.q[1] = ldw;
+    gemm(g, p, gws, s);
+  }
+  // sigma1(a,k) -= sum_{P,m} B[P,k,m] Y[P,a,m]
+  {
+    GemmArgs g;
+    g.M = v * nc; g.N = o; g.K = o * naux;
+    g.A = Yoo.get(); g.sam = (int64_t)o * naux; g.sak = 1;
+    g.B = BooP; g.sbk = o; g.sbn = 1;
+    g.map.kind = MAP_SIG1O; g.map.C = Wc; g.map.alpha = -1.0; g.map.beta = 1.0;
+    g.map.q[0] = o; g.map.q[1] = ldw; g.map.q[2] = nc;
+    gemm(g, p, gws, s);
+  }
+  // W[P,vec,c,k] = sum_a B[P,c,a] x1(a,k) - sum_m B[P,k,m] x1(c,m)
+  {
+    GemmArgs g;
+    g.M = v * naux; g.N = o * nc; g.K = v;
+    g.A = BvvP; g.sam = v; g.sak = 1;
+    g.B = X1c.get(); g.sbk = (int64_t)o * nc; g.sbn = 1;
+    g.map.kind = MAP_WUPV; g.map.C = Wup.get();
+    g.map.q[0] = o; g.map.q[1] = v; g.map.q[2] = n1; g.map.q[3] = nc;
+    g.out_prec = out_w;
+    gemm(g, p, gws, s);
+  }
+  {
+    GemmArgs g;
+    g.M = v * nc; g.N = o * naux; g.K = o;
+    g.A = X1c.get(); g.sam = o; g.sak = 1;
+    g.B = BooP; g.sbk = 1; g.sbn = o;
+    g.map.kind = MAP_WUPO; g.map.C = Wup.get(); g.map.alpha = -1.0; g.map.beta = 1.0;
+    g.map.q[0] = o; g.map.q[1] = v; g.map.q[2] = n1; g.map.q[3] = nc;
+    g.out_prec = out_w;
+    gemm(g, p, gws, s);
+  }
+  // Wt[(vec,c,k)][P] = W[P][(vec,c,k)] in the GEMM precision
+  switch (p) {
+    case Prec::F64:
+      launch_transpose<double, double>((const double*)Wup.get(), naux, n1 * nc, (double*)WupT.get(), s);
+      break;
+    case Prec::F32:
+      launch_transpose<float, float>((const float*)Wup.get(), naux, n1 * nc, (float*)WupT.get(), s);
+      break;
+    case Prec::BF16:
+      launch_transpose<float, __bf16>((const float*)Wup.get(), naux, n1 * nc, (__bf16*)WupT.get(), s);
+      break;
+  }
+  if (e1.second) HIP_CHECK(hipEventRecord(e1.second, s));
+  // up: G[(vec,c,k),(b,l)] = sum_P Wt[(vec,c,k),P] BovT[(b,l),P]  (both K-contiguous)
+  {
+    GemmArgs g;
+    g.M = (int)(n1 * nc); g.N = (int)n1; g.K = naux;
+    g.A = WupT.get(); g.sam = naux; g.sak = 1;
+    g.B = BovTP; g.sbk = 1; g.sbn = naux;
+    g.map.C = ZG.get(); g.map.scm = n1; g.map.scn = 1;
+    g.out_prec = out_lp;
+    g.tag = 1;
+    auto e = span(0, 2.0 * naux * (double)n1 * n1 * nc);
+    g.ev_begin = e.first; g.ev_end = e.second;
+    gemm(g, p, gws, s);
+  }
+  auto eu = span(2, (double)nc * (3.0 * d.n2 * 8.0 + (double)n1 * n1 * esd));
+  if (eu.first) HIP_CHECK(hipEventRecord(eu.first, s));
+  unpack_chunk(p, ZG.get(), Xc, ldx, Wc, ldw, nc, dk, eps.get(), lay, s);
+  if (eu.second) HIP_CHECK(hipEventRecord(eu.second, s));
 }
 
 }  // namespace adc2g

@@ -30,6 +30,20 @@ struct Handle {
     bool ready = false;
   };
   FactorSet fs[3];
+  // coupling form: FORM_DF contracts the amplitudes with the factors (sums
+  // over naux), FORM_INT with the integral blocks assembled once from them
+  // (sums over v and o):
+  //   V [a][c][(b,l)] = (ac|bl),  Vt [c][(b,l)][a] = (ca|bl),
+  //   O [k][m][(b,l)] = (km|bl),  Ot [m][k][(b,l)] = (km|bl).
+  enum { FORM_DF = 0, FORM_INT = 1 };
+  int form = FORM_DF;
+  struct IntSet {
+    DevBuf<char> V, Vt, O, Ot;
+    bool ready = false;
+  };
+  IntSet is[3];
+  void build_integrals();
+  size_t integral_bytes(Prec p) const;
   const void* fBov(Prec p) const;
   const void* fBovT(Prec p) const;
   const void* fBvv(Prec p) const;
@@ -37,7 +51,7 @@ struct Handle {
 
   // sigma workspaces (grown on demand, sized per vector chunk), typed by the
   // chunk's GEMM precision
-  DevBuf<char> X1c, Yvv, Yoo;
+  DevBuf<char> X1c, X1t, Yvv, Yoo;
   DevBuf<char> Wup;         // W[P][vec][(c,k)] (accumulator type: FP64, or FP32 for FP32/BF16)
   DevBuf<char> WupT;        // W transposed, [(vec,c,k)][P], GEMM precision
   DevBuf<char> ZG;          // Z then G, in the chunk's GEMM precision
@@ -46,9 +60,11 @@ struct Handle {
   int chunk = 32;           // vectors per sigma pass
 
   // optional per-kernel device timing of sigma-vectors (benchmark): event
-  // pairs around the coupling GEMMs (class 0), the CSF->pair packing (1) and
-  // the pair->CSF projection with the 2h2p diagonal (2), with the algorithmic
-  // flops (class 0) or bytes (1, 2) each launch is credited with.
+  // pairs around the main coupling GEMMs (class 0: the virtual-index
+  // contractions, or the naux contractions of the DF form), the CSF->pair
+  // packing (1), the pair->CSF projection with the 2h2p diagonal (2) and the
+  // remaining coupling GEMMs (3), with the algorithmic flops (classes 0, 3)
+  // or bytes (1, 2) each launch is credited with.
   struct ProfRec {
     int cls;
     hipEvent_t a, b;
@@ -59,7 +75,7 @@ struct Handle {
   size_t ev_used = 0;
   std::vector<ProfRec> prof;
   hipEvent_t prof_event();
-  void prof_collect(double out[9]);
+  void prof_collect(double out[12]);
 
   // learned preconditioner (network + harvested training records)
   PreconNet* net = nullptr;
@@ -71,6 +87,8 @@ struct Handle {
   void build_phph(const double* dB, int nbas);
   void ensure_lowprec(Prec p);
   void sigma(const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, Prec p, hipStream_t s);
+  void sigma_df(const double* X, int64_t ldx, double* W, int64_t ldw, int nc, Prec p, hipStream_t s);
+  void sigma_int(const double* X, int64_t ldx, double* W, int64_t ldw, int nc, Prec p, hipStream_t s);
   size_t sigma_workspace_bytes(int nc, Prec p) const;
 };
 

@@ -125,6 +125,17 @@ __global__ void __launch_bounds__(256) transpose_kernel<float, __bf16>(const flo
   }
 }
 
+template <class S, class D>
+__global__ void swap01_kernel(const S* __restrict__ src, int64_t A, int64_t B, int64_t C, D* __restrict__ dst) {
+  const int64_t total = A * B * C;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = idx % C, ji = idx / C;
+    const int64_t i = ji % A, j = ji / A;
+    dst[idx] = (D)src[(i * B + j) * C + c];
+  }
+}
+
 template <class T>
 __global__ void pack_x1_kernel(const double* __restrict__ X, int64_t ldx, int nvec, DimsK d, T* __restrict__ X1c) {
   const int64_t total = d.n1 * nvec;
@@ -139,8 +150,21 @@ __global__ void pack_x1_kernel(const double* __restrict__ X, int64_t ldx, int nv
 }
 
 template <class T>
+__global__ void pack_x1t_kernel(const double* __restrict__ X, int64_t ldx, int nvec, DimsK d, T* __restrict__ X1t) {
+  const int64_t total = d.n1 * nvec;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    // output order: vec outer, k, a inner
+    const int64_t a = idx % d.v;
+    const int64_t rest = idx / d.v;
+    const int64_t k = rest % d.o, vec = rest / d.o;
+    X1t[idx] = (T)X[a * d.o + k + ldx * vec];
+  }
+}
+
+template <class T>
 __global__ void csf_to_pair_kernel(const double* __restrict__ X, int64_t ldx, int nvec, DimsK d,
-                                   T* __restrict__ Z) {
+                                   T* __restrict__ Z, PairLayout lay) {
   const int64_t n1 = d.n1, nn = n1 * n1;
   const int o = d.o, v = d.v;
   const int64_t total = nn * nvec;
@@ -172,28 +196,28 @@ __global__ void csf_to_pair_kernel(const double* __restrict__ X, int64_t ldx, in
         val = ((c < b) == (k < l)) ? (s4 + s5) : (s4 - s5);
       }
     }
-    Z[idx] = (T)val;
+    Z[(int64_t)c * lay.sc + vec * lay.sv + (int64_t)k * n1 + bl] = (T)val;
   }
 }
 
 template <class T>
 __global__ void pair_to_csf_kernel(const T* __restrict__ G, const double* __restrict__ X, int64_t ldx,
                                    double* __restrict__ W, int64_t ldw, int nvec, DimsK d,
-                                   const double* __restrict__ eps) {
+                                   const double* __restrict__ eps, PairLayout lay) {
   const int o = d.o, v = d.v;
-  const int64_t n1 = d.n1, nn = n1 * n1;
+  const int64_t n1 = d.n1;
   const int64_t units = d.off5 - d.off1;  // t1..t4 entries; a t4 unit also writes its t5 twin
   const int64_t total = units * nvec;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
        idx += (int64_t)gridDim.x * blockDim.x) {
     const int64_t vec = idx / units;
     const int64_t u = idx - vec * units;
-    const T* g = G + vec * nn;
+    const T* g = G + vec * lay.sv;
     const double* x = X + ldx * vec;
     double* w = W + ldw * vec;
     const int64_t J = d.off1 + u;
     auto gm = [&](int c, int k, int b, int l) -> double {
-      return (double)g[((int64_t)c * o + k) * n1 + (int64_t)b * o + l];
+      return (double)g[(int64_t)c * lay.sc + (int64_t)k * n1 + (int64_t)b * o + l];
     };
     if (J < d.off2) {  // t1
       const int64_t t = J - d.off1;
@@ -399,6 +423,15 @@ template void launch_transpose<double, __bf16>(const double*, int64_t, int64_t, 
 template void launch_transpose<float, float>(const float*, int64_t, int64_t, float*, hipStream_t);
 template void launch_transpose<float, __bf16>(const float*, int64_t, int64_t, __bf16*, hipStream_t);
 
+template <class S, class D>
+void launch_swap01(const S* src, int64_t A, int64_t B, int64_t C, D* dst, hipStream_t st) {
+  hipLaunchKernelGGL((swap01_kernel<S, D>), dim3(grid_for(A * B * C)), dim3(256), 0, st, src, A, B, C, dst);
+  HIP_CHECK(hipGetLastError());
+}
+template void launch_swap01<double, double>(const double*, int64_t, int64_t, int64_t, double*, hipStream_t);
+template void launch_swap01<double, float>(const double*, int64_t, int64_t, int64_t, float*, hipStream_t);
+template void launch_swap01<double, __bf16>(const double*, int64_t, int64_t, int64_t, __bf16*, hipStream_t);
+
 template <class T>
 void launch_pack_x1(const double* X, int64_t ldx, int nvec, const DimsK& d, T* X1c, hipStream_t st) {
   hipLaunchKernelGGL((pack_x1_kernel<T>), dim3(grid_for(d.n1 * nvec)), dim3(256), 0, st, X, ldx, nvec, d, X1c);
@@ -406,26 +439,33 @@ void launch_pack_x1(const double* X, int64_t ldx, int nvec, const DimsK& d, T* X
 }
 
 template <class T>
-void launch_csf_to_pair(const double* X, int64_t ldx, int nvec, const DimsK& d, T* Z, hipStream_t st) {
+void launch_pack_x1t(const double* X, int64_t ldx, int nvec, const DimsK& d, T* X1t, hipStream_t st) {
+  hipLaunchKernelGGL((pack_x1t_kernel<T>), dim3(grid_for(d.n1 * nvec)), dim3(256), 0, st, X, ldx, nvec, d, X1t);
+  HIP_CHECK(hipGetLastError());
+}
+
+template <class T>
+void launch_csf_to_pair(const double* X, int64_t ldx, int nvec, const DimsK& d, T* Z, const PairLayout& lay,
+                        hipStream_t st) {
   if (d.o <= 64) {
-    launch_pack_tiled<T>(X, ldx, nvec, d, Z, st);
+    launch_pack_tiled<T>(X, ldx, nvec, d, Z, lay, st);
     return;
   }
   hipLaunchKernelGGL((csf_to_pair_kernel<T>), dim3(grid_for(d.n1 * d.n1 * nvec)), dim3(256), 0, st, X, ldx,
-                     nvec, d, Z);
+                     nvec, d, Z, lay);
   HIP_CHECK(hipGetLastError());
 }
 
 template <class T>
 void launch_pair_to_csf(const T* G, const double* X, int64_t ldx, double* W, int64_t ldw, int nvec,
-                        const DimsK& d, const double* eps, hipStream_t st) {
+                        const DimsK& d, const double* eps, const PairLayout& lay, hipStream_t st) {
   if (d.o <= 64) {
-    launch_unpack_tiled<T>(G, X, ldx, W, ldw, nvec, d, eps, st);
+    launch_unpack_tiled<T>(G, X, ldx, W, ldw, nvec, d, eps, lay, st);
     return;
   }
   const int64_t units = (d.off5 - d.off1) * nvec;
   hipLaunchKernelGGL((pair_to_csf_kernel<T>), dim3(grid_for(units)), dim3(256), 0, st, G, X, ldx, W, ldw, nvec,
-                     d, eps);
+                     d, eps, lay);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -463,9 +503,11 @@ void launch_phph_assemble(double* H11, const double* Mm, const double* Jm, const
 
 #define INST(T)                                                                                             \
   template void launch_pack_x1<T>(const double*, int64_t, int, const DimsK&, T*, hipStream_t);               \
-  template void launch_csf_to_pair<T>(const double*, int64_t, int, const DimsK&, T*, hipStream_t);           \
+  template void launch_pack_x1t<T>(const double*, int64_t, int, const DimsK&, T*, hipStream_t);              \
+  template void launch_csf_to_pair<T>(const double*, int64_t, int, const DimsK&, T*, const PairLayout&,       \
+                                      hipStream_t);                                                           \
   template void launch_pair_to_csf<T>(const T*, const double*, int64_t, double*, int64_t, int, const DimsK&, \
-                                      const double*, hipStream_t);
+                                      const double*, const PairLayout&, hipStream_t);
 INST(double)
 INST(float)
 INST(__bf16)

@@ -21,6 +21,18 @@ inline DimsK dims_k(const Dims& d) {
   return k;
 }
 
+// Storage of the pair tensors Z / G of a chunk of nc vectors: row (c, k) of
+// vector vec starts at c*sc + vec*sv + k*n1 and column (b, l) is b*o + l.
+//   vector-major  [vec][c][k][(b,l)]: sc = o*n1,    sv = n1*n1
+//   virtual-major [c][vec][k][(b,l)]: sc = nc*o*n1, sv = o*n1
+struct PairLayout {
+  int64_t sc, sv;
+};
+inline PairLayout pair_layout_vec_major(const DimsK& d) { return {(int64_t)d.o * d.n1, d.n1 * d.n1}; }
+inline PairLayout pair_layout_virt_major(const DimsK& d, int nc) {
+  return {(int64_t)nc * d.o * d.n1, (int64_t)d.o * d.n1};
+}
+
 // ---- factor extraction: B[P][p][q] (nbas^2 per P) -> Bov[P][a*o+k], Bvv[P][a][b], Boo[P][k][l]
 void launch_extract_factors(const double* B, int nbas, const DimsK& d, double* Bov, double* Bvv,
                             double* Boo, hipStream_t st);
@@ -29,24 +41,32 @@ void launch_convert_bf16(const double* src, __bf16* dst, int64_t n, hipStream_t 
 // dst[c * R + r] = (D) src[r * C + c]  (R x C row-major -> C x R row-major)
 template <class S, class D>
 void launch_transpose(const S* src, int64_t R, int64_t C, D* dst, hipStream_t st);
+// dst[(j*A + i)*C + c] = (D) src[(i*B + j)*C + c]  ([A][B][C] -> [B][A][C])
+template <class S, class D>
+void launch_swap01(const S* src, int64_t A, int64_t B, int64_t C, D* dst, hipStream_t st);
 
 // ---- sigma-vector streaming kernels
 // X1c[a][vec][k] <- X[a*o+k + ldx*vec]
 template <class T>
 void launch_pack_x1(const double* X, int64_t ldx, int nvec, const DimsK& d, T* X1c, hipStream_t st);
-// Z_vec[ck][bl] (n1 x n1 per vector) from the 2h2p amplitudes of X
+// X1t[(vec,k)][a] <- X[a*o+k + ldx*vec]
 template <class T>
-void launch_csf_to_pair(const double* X, int64_t ldx, int nvec, const DimsK& d, T* Z, hipStream_t st);
+void launch_pack_x1t(const double* X, int64_t ldx, int nvec, const DimsK& d, T* X1t, hipStream_t st);
+// Z[(c,k),(b,l)] (n1 x n1 per vector, stored as `lay`) from the 2h2p amplitudes of X
+template <class T>
+void launch_csf_to_pair(const double* X, int64_t ldx, int nvec, const DimsK& d, T* Z, const PairLayout& lay,
+                        hipStream_t st);
 // W[n1 + J] = (projection of G onto CSF J) + D_J * X[n1 + J], for every 2h2p J
 template <class T>
 void launch_pair_to_csf(const T* G, const double* X, int64_t ldx, double* W, int64_t ldw, int nvec,
-                        const DimsK& d, const double* eps, hipStream_t st);
+                        const DimsK& d, const double* eps, const PairLayout& lay, hipStream_t st);
 // Tiled versions of the two maps above (pairs.hip), used for nocc <= 64.
 template <class T>
-void launch_pack_tiled(const double* X, int64_t ldx, int nvec, const DimsK& d, T* Z, hipStream_t st);
+void launch_pack_tiled(const double* X, int64_t ldx, int nvec, const DimsK& d, T* Z, const PairLayout& lay,
+                       hipStream_t st);
 template <class T>
 void launch_unpack_tiled(const T* G, const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, const DimsK& d,
-                         const double* eps, hipStream_t st);
+                         const double* eps, const PairLayout& lay, hipStream_t st);
 // hii[j] for the whole vector: diag(H11) and the 2h2p orbital-energy sums
 void launch_diagonal(const double* H11, const double* eps, const DimsK& d, double* hii, hipStream_t st);
 

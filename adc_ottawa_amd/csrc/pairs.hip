@@ -38,7 +38,7 @@ __device__ __forceinline__ double dbl<__bf16>(__bf16 x) { return (double)(float)
 
 template <class T>
 __global__ void __launch_bounds__(PT) pack_tiled_kernel(const double* __restrict__ X, int64_t ldx, DimsK d,
-                                                         T* __restrict__ Z, int BB) {
+                                                         T* __restrict__ Z, PairLayout lay, int BB) {
   extern __shared__ double sm[];
   const int o = d.o, v = d.v, tid = threadIdx.x;
   const int a = blockIdx.x;
@@ -48,7 +48,8 @@ __global__ void __launch_bounds__(PT) pack_tiled_kernel(const double* __restrict
   if (b0 >= v && chunk > 0) return;  // whole workgroup, before any barrier
   const int64_t n1 = d.n1, npo = d.npo, npv = d.npv;
   const double* x = X + ldx * vec;
-  T* z = Z + vec * n1 * n1;
+  T* z = Z + vec * lay.sv;
+  const int64_t sc = lay.sc;
   const int BP = BB + 1;            // padded run length (LDS banks differ across q)
   double* xs4 = sm;                 // [npo][BP]
   double* xs5 = xs4 + npo * BP;     // [npo][BP]
@@ -60,7 +61,7 @@ __global__ void __launch_bounds__(PT) pack_tiled_kernel(const double* __restrict
       double val;
       if (k == l) val = -SQRT2 * x[d.off1 + (int64_t)k * v + a];
       else val = x[d.off3 + pidx(min(k, l), max(k, l), o) * v + a];
-      z[((int64_t)a * o + k) * n1 + (int64_t)a * o + l] = cvt<T>(val);
+      z[(int64_t)a * sc + (int64_t)k * n1 + (int64_t)a * o + l] = cvt<T>(val);
     }
   }
   if (b0 < v) {
@@ -85,8 +86,8 @@ __global__ void __launch_bounds__(PT) pack_tiled_kernel(const double* __restrict
       const bool diag = (k == l);
       const int q = diag ? 0 : (int)pidx(min(k, l), max(k, l), o);
       const double s_ab = (k < l) ? 1.0 : -1.0;  // sign of x5 in Z(a,b,k,l); Z(b,a,k,l) takes the other
-      T* zab = z + ((int64_t)a * o + k) * n1 + (int64_t)b0 * o + l;
-      T* zba = z + ((int64_t)b0 * o + k) * n1 + (int64_t)a * o + l;
+      T* zab = z + (int64_t)a * sc + (int64_t)k * n1 + (int64_t)b0 * o + l;
+      T* zba = z + (int64_t)b0 * sc + (int64_t)k * n1 + (int64_t)a * o + l;
       for (int bb = 0; bb < nb; ++bb) {
         double vab, vba;
         if (diag) {
@@ -97,7 +98,7 @@ __global__ void __launch_bounds__(PT) pack_tiled_kernel(const double* __restrict
           vba = p4 - s_ab * p5;
         }
         zab[(int64_t)bb * o] = cvt<T>(vab);
-        zba[(int64_t)bb * o * n1] = cvt<T>(vba);
+        zba[(int64_t)bb * sc] = cvt<T>(vba);
       }
     }
   }
@@ -106,7 +107,8 @@ __global__ void __launch_bounds__(PT) pack_tiled_kernel(const double* __restrict
 template <class T>
 __global__ void __launch_bounds__(PT) unpack_tiled_kernel(const T* __restrict__ G, const double* __restrict__ X,
                                                            int64_t ldx, double* __restrict__ W, int64_t ldw, DimsK d,
-                                                           const double* __restrict__ eps, int BB, int op) {
+                                                           const double* __restrict__ eps, PairLayout lay, int BB,
+                                                           int op) {
   extern __shared__ double sm[];
   const int o = d.o, v = d.v, tid = threadIdx.x;
   const int a = blockIdx.x;
@@ -115,7 +117,8 @@ __global__ void __launch_bounds__(PT) unpack_tiled_kernel(const T* __restrict__ 
   const int b0 = a + 1 + chunk * BB;
   if (b0 >= v && chunk > 0) return;  // whole workgroup, before any barrier
   const int64_t n1 = d.n1, npo = d.npo, npv = d.npv;
-  const T* g = G + vec * n1 * n1;
+  const T* g = G + vec * lay.sv;
+  const int64_t sc = lay.sc;
   const double* x = X + ldx * vec;
   double* w = W + ldw * vec;
   const int oo = o * o;
@@ -126,7 +129,7 @@ __global__ void __launch_bounds__(PT) unpack_tiled_kernel(const T* __restrict__ 
   const double ea = eps[o + a];
   // diagonal tile b = a: t1 and t3 (first chunk)
   if (chunk == 0) {
-    for (int t = tid; t < oo; t += PT) tab[t] = dbl(g[((int64_t)a * o + t / o) * n1 + (int64_t)a * o + t % o]);
+    for (int t = tid; t < oo; t += PT) tab[t] = dbl(g[(int64_t)a * sc + (int64_t)(t / o) * n1 + (int64_t)a * o + t % o]);
     __syncthreads();
     for (int k = tid; k < o; k += PT) {
       const int64_t J = d.off1 + (int64_t)k * v + a;
@@ -151,11 +154,11 @@ __global__ void __launch_bounds__(PT) unpack_tiled_kernel(const T* __restrict__ 
     __syncthreads();
     for (int kl = tid; kl < oo; kl += PT) {
       const int k = kl / o, l = kl - k * o;
-      const T* gab = g + ((int64_t)a * o + k) * n1 + (int64_t)b0 * o + l;
-      const T* gba = g + ((int64_t)b0 * o + k) * n1 + (int64_t)a * o + l;
+      const T* gab = g + (int64_t)a * sc + (int64_t)k * n1 + (int64_t)b0 * o + l;
+      const T* gba = g + (int64_t)b0 * sc + (int64_t)k * n1 + (int64_t)a * o + l;
       for (int bb = 0; bb < nb; ++bb) {
         tab[bb * op + kl] = dbl(gab[(int64_t)bb * o]);
-        tba[bb * op + kl] = dbl(gba[(int64_t)bb * o * n1]);
+        tba[bb * op + kl] = dbl(gba[(int64_t)bb * sc]);
       }
     }
     __syncthreads();
@@ -202,31 +205,33 @@ int unpack_chunk(const DimsK& d) {
 }
 
 template <class T>
-void launch_pack_tiled(const double* X, int64_t ldx, int nvec, const DimsK& d, T* Z, hipStream_t st) {
+void launch_pack_tiled(const double* X, int64_t ldx, int nvec, const DimsK& d, T* Z, const PairLayout& lay,
+                       hipStream_t st) {
   const int BB = pack_chunk(d);
   const size_t lds = sizeof(double) * ((size_t)(BB + 1) * 2 * d.npo + (size_t)BB * d.o);
   const int chunks = (d.v - 1 + BB - 1) / BB;
-  hipLaunchKernelGGL((pack_tiled_kernel<T>), dim3(d.v, std::max(chunks, 1), nvec), dim3(PT), lds, st, X, ldx, d, Z, BB);
+  hipLaunchKernelGGL((pack_tiled_kernel<T>), dim3(d.v, std::max(chunks, 1), nvec), dim3(PT), lds, st, X, ldx, d, Z, lay, BB);
   HIP_CHECK(hipGetLastError());
 }
 
 template <class T>
 void launch_unpack_tiled(const T* G, const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, const DimsK& d,
-                         const double* eps, hipStream_t st) {
+                         const double* eps, const PairLayout& lay, hipStream_t st) {
   const int BB = unpack_chunk(d);
   int op = d.o * d.o + 1;
   if ((size_t)2 * BB * op * sizeof(double) > 65536) op = d.o * d.o;
   const size_t lds = sizeof(double) * (size_t)2 * BB * op;
   const int chunks = (d.v - 1 + BB - 1) / BB;
   hipLaunchKernelGGL((unpack_tiled_kernel<T>), dim3(d.v, std::max(chunks, 1), nvec), dim3(PT), lds, st, G, X, ldx, W,
-                     ldw, d, eps, BB, op);
+                     ldw, d, eps, lay, BB, op);
   HIP_CHECK(hipGetLastError());
 }
 
 #define INST(T)                                                                                           \
-  template void launch_pack_tiled<T>(const double*, int64_t, int, const DimsK&, T*, hipStream_t);          \
+  template void launch_pack_tiled<T>(const double*, int64_t, int, const DimsK&, T*, const PairLayout&,      \
+                                     hipStream_t);                                                          \
   template void launch_unpack_tiled<T>(const T*, const double*, int64_t, double*, int64_t, int, const DimsK&, \
-                                       const double*, hipStream_t);
+                                       const double*, const PairLayout&, hipStream_t);
 INST(double)
 INST(float)
 INST(__bf16)

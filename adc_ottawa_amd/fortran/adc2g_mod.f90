@@ -20,6 +20,7 @@ module adc2g_mod
   integer(c_int32_t), parameter :: ADC2G_OK = 0
   integer(c_int32_t), parameter :: ADC2G_PREC_FP64 = 0, ADC2G_PREC_FP32 = 1, ADC2G_PREC_BF16 = 2
   integer(c_int32_t), parameter :: ADC2G_PRE_DPR = 1, ADC2G_PRE_OLSEN = 2, ADC2G_PRE_LEARNED = 3
+  integer(c_int32_t), parameter :: ADC2G_COUPLING_AUTO = 0, ADC2G_COUPLING_DF = 1, ADC2G_COUPLING_INT = 2
   real(c_double), parameter :: adc2g_eh2ev = 27.2113845d0       ! parameters.f90:192
   real(c_double), parameter :: adc2g_au2fs = 1.0d0/41.34137333656d0 ! fvecprop.f90:19
 
@@ -27,7 +28,7 @@ module adc2g_mod
      integer(c_int32_t) :: device = 0
      integer(c_int32_t) :: build_phph = 1
      integer(c_int32_t) :: keep_lowprec = 0
-     integer(c_int32_t) :: reserved = 0
+     integer(c_int32_t) :: coupling = 0        ! ADC2G_COUPLING_*
   end type adc2g_opts
 
   type, bind(c) :: adc2g_dav_params
@@ -84,6 +85,12 @@ module adc2g_mod
        type(c_ptr), value :: h
        integer(c_int64_t), intent(out) :: ndim, n1, counts(5)
      end function adc2g_dims
+
+     integer(c_int32_t) function adc2g_coupling_form(h, form) bind(c, name='adc2g_coupling_form')
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int32_t), intent(out) :: form
+     end function adc2g_coupling_form
 
      integer(c_int32_t) function adc2g_diagonal(h, hii) bind(c, name='adc2g_diagonal')
        import :: c_int32_t, c_double, c_ptr

@@ -34,7 +34,10 @@ class Adc2gError(RuntimeError):
 
 class Opts(C.Structure):
     _fields_ = [("device", C.c_int32), ("build_phph", C.c_int32), ("keep_lowprec", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("coupling", C.c_int32)]
+
+
+COUPLING = {"auto": 0, "df": 1, "int": 2}
 
 
 class DavParams(C.Structure):
@@ -66,7 +69,7 @@ SYMBOLS = [
     "adc2g_sigma", "adc2g_sigma_device", "adc2g_matxvec_treal", "adc2g_davidson",
     "adc2g_sil_autocorr", "adc2g_precon_harvest", "adc2g_precon_train", "adc2g_precon_nparams",
     "adc2g_precon_get", "adc2g_precon_set", "adc2g_precon_eval", "adc2g_last_error",
-    "adc2g_sigma_profile", "adc2g_precon_absorb",
+    "adc2g_sigma_profile", "adc2g_precon_absorb", "adc2g_coupling_form",
 ]
 
 _lib = None
@@ -103,6 +106,7 @@ def load(path: str = LIBPATH):
     lib.adc2g_precon_eval.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
     lib.adc2g_last_error.argtypes = [C.c_char_p, i32]
     lib.adc2g_sigma_profile.argtypes = [vp, i32, vp]
+    lib.adc2g_coupling_form.argtypes = [vp, vp]
     lib.adc2g_precon_absorb.argtypes = [vp, vp, C.POINTER(i64)]
     for s in SYMBOLS:
         getattr(lib, s).restype = C.c_int32
@@ -124,7 +128,8 @@ def _ptr(a: np.ndarray):
 class Operator:
     """Device-resident ADC(2)-s operator of one system (``adc2g_create``)."""
 
-    def __init__(self, nocc, nvirt, naux, eps, B, device=0, second_order=True, keep_lowprec=False):
+    def __init__(self, nocc, nvirt, naux, eps, B, device=0, second_order=True, keep_lowprec=False,
+                 coupling="auto"):
         lib = load()
         self._h = C.c_void_p()
         eps = np.ascontiguousarray(eps, dtype=np.float64)
@@ -132,13 +137,16 @@ class Operator:
         nbas = nocc + nvirt
         if eps.size != nbas or B.shape != (naux, nbas, nbas):
             raise ValueError("eps/B shapes do not match nocc, nvirt, naux")
-        o = Opts(device, 1 if second_order else 0, 1 if keep_lowprec else 0, 0)
+        o = Opts(device, 1 if second_order else 0, 1 if keep_lowprec else 0, COUPLING[coupling])
         check(lib.adc2g_create(nocc, nvirt, naux, _ptr(eps), _ptr(B), C.byref(o), C.byref(self._h)))
         nd, n1 = C.c_int64(), C.c_int64()
         cnt = (C.c_int64 * 5)()
         check(lib.adc2g_dims(self._h, C.byref(nd), C.byref(n1), cnt))
         self.ndim, self.n1, self.counts = nd.value, n1.value, list(cnt)
         self.nocc, self.nvirt, self.naux = nocc, nvirt, naux
+        form = C.c_int32()
+        check(lib.adc2g_coupling_form(self._h, C.byref(form)))
+        self.coupling = {1: "df", 2: "int"}[form.value]
 
     @classmethod
     def from_system(cls, sys, **kw):
@@ -222,9 +230,9 @@ class Operator:
     def profile(self, enable=True):
         """Collect the per-kernel timing recorded so far and (re)arm it.
         Returns {class: (ms, launches, work)} for coupling/pack/unpack."""
-        out = np.zeros(9)
+        out = np.zeros(12)
         check(_lib.adc2g_sigma_profile(self._h, 1 if enable else 0, _ptr(out)))
-        names = ("coupling_gemm", "csf_to_pair", "pair_to_csf_diag")
+        names = ("coupling_gemm", "csf_to_pair", "pair_to_csf_diag", "coupling_minor")
         return {n: tuple(out[3 * i:3 * i + 3]) for i, n in enumerate(names)}
 
     # ---- learned preconditioner

@@ -7,9 +7,9 @@ A step is one sigma-vector block product W = H X of the benzene-sized
 synthetic system (o=16, v=200, naux=864, ndim 5,124,800) on a block of 20
 resident FP64 vectors (the Davidson block size of that workload).  Inputs are
 in HBM before the timed region; the JSON line reports the whole-job
-sigma-vectors/s, the roofline of the dominant kernel (the density-fitted
-coupling contractions, FP64 MFMA) measured with HIP events over the timed
-region, and a CPU baseline: the reference's stored-matrix sigma-vector
+sigma-vectors/s, the roofline of the dominant kernel (the virtual-index
+coupling contractions, FP64 MFMA; the naux contractions with --coupling df)
+measured with HIP events over the timed region, and a CPU baseline: the reference's stored-matrix sigma-vector
 (COO + hxvec_incore) restated in C/OpenMP and timed on the host on a sample of
 matrix rows, scaled to the full matrix.
 
@@ -45,11 +45,11 @@ WEIGHTS = os.path.join(ROOT, "adc_ottawa_amd", "precon_weights.npy")
 TRAFFIC = os.path.join(ROOT, "profiles", "coupling_gemm_traffic.json")
 
 
-def measured_traffic(prec, workload):
+def measured_traffic(prec, workload, form):
     try:
         with open(TRAFFIC) as f:
             t = json.load(f)
-        rec = t.get(workload, {}).get(prec)
+        rec = t.get(workload, {}).get(form, {}).get(prec)
         return None if rec is None else float(rec["bytes_per_launch"])
     except (OSError, ValueError, KeyError):
         return None
@@ -123,14 +123,26 @@ def timed_sigma(op, hip, dX, dW, nvec, prec, steps, warmup):
     return (t1 - t0), prof
 
 
-def roofline_of(prof, prec, workload):
+KERNEL_DESC = {
+    "int": "gemm_kernel<%s,*,tag=1> (virtual-index coupling contractions with the (vv|ov) block: "
+           "down + up, mean per launch)",
+    "df": "gemm_kernel<%s,128,128,*,tag=1> (DF coupling contractions over naux: down + up, mean per launch)",
+}
+
+
+def roofline_of(prof, prec, workload, form):
     ms, launches, flops = prof["coupling_gemm"]
     achieved = flops / (ms * 1e-3) / 1e12
-    return {"kernel": "gemm_kernel<%s,128,128,*,tag=1> (DF coupling: down + up, mean per launch)" % prec,
+    return {"kernel": KERNEL_DESC[form] % prec,
             "bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK[prec], "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK[prec], 4), "traffic": measured_traffic(prec, workload),
+            "frac": round(achieved / PEAK[prec], 4), "traffic": measured_traffic(prec, workload, form),
             "launches": int(launches), "avg_launch_ms": round(ms / max(launches, 1), 4),
             "flops_per_launch": flops / max(launches, 1)}
+
+
+def minor_stats(prof):
+    ms, n, flops = prof["coupling_minor"]
+    return {"ms": round(ms, 4), "launches": int(n), "TFLOPs": round(flops / max(ms, 1e-9) * 1e-9, 2)}
 
 
 def stream_stats(prof):
@@ -250,6 +262,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="benzene", choices=sorted(SIZES))
     ap.add_argument("--nvec", type=int, default=20)
+    ap.add_argument("--coupling", default="auto", choices=("auto", "df", "int"),
+                    help="coupling form of the operator (auto: the library's choice)")
     ap.add_argument("--skip-davidson", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-lowprec", action="store_true")
@@ -272,9 +286,11 @@ def main():
     sys_ = make_system(sz["nocc"], sz["nvirt"], sz["naux"], seed=0)
     tg = time.perf_counter() - t0
     t0 = time.perf_counter()
-    op = lib.Operator.from_system(sys_, keep_lowprec=True)
+    op = lib.Operator.from_system(sys_, keep_lowprec=True, coupling=args.coupling)
     t_build = time.perf_counter() - t0
-    log(f"system generated in {tg:.1f}s; operator built on GPU in {t_build:.2f}s (ndim {op.ndim})")
+    form = op.coupling
+    log(f"system generated in {tg:.1f}s; operator built on GPU in {t_build:.2f}s (ndim {op.ndim}, "
+        f"coupling form {form})")
     hip = Hip()
     nvec = args.nvec
     nbytes = op.ndim * nvec * 8
@@ -286,18 +302,20 @@ def main():
     secs, prof = timed_sigma(op, hip, dX, dW, nvec, "fp64", args.steps, args.warmup)
     ms_per_step = secs / args.steps * 1e3
     value = nvec * args.steps / secs
-    roof = roofline_of(prof, "fp64", args.workload)
+    roof = roofline_of(prof, "fp64", args.workload, form)
     streams = stream_stats(prof)
-    log(f"fp64: {ms_per_step:.3f} ms/step, {value:.1f} sigma-vectors/s, coupling {roof['achieved']} TFLOP/s")
+    log(f"fp64: {ms_per_step:.3f} ms/step, {value:.1f} sigma-vectors/s, coupling {roof['achieved']} TFLOP/s, "
+        f"minor {minor_stats(prof)}, streaming {streams}")
     extra = {"fp64": {"ms_per_step": round(ms_per_step, 4), "sigma_vectors_per_s": round(value, 2),
-                      "coupling": roof, "streaming": streams}}
+                      "coupling": roof, "coupling_minor": minor_stats(prof), "streaming": streams}}
     if not args.skip_lowprec:
         for prec in ("fp32", "bf16"):
             s2, p2 = timed_sigma(op, hip, dX, dW, nvec, prec, max(2, args.steps // 2), 1)
-            r2 = roofline_of(p2, prec, args.workload)
+            r2 = roofline_of(p2, prec, args.workload, form)
             v2 = nvec * max(2, args.steps // 2) / s2
             extra[prec] = {"ms_per_step": round(s2 / max(2, args.steps // 2) * 1e3, 4),
-                           "sigma_vectors_per_s": round(v2, 2), "coupling": r2, "streaming": stream_stats(p2)}
+                           "sigma_vectors_per_s": round(v2, 2), "coupling": r2,
+                           "coupling_minor": minor_stats(p2), "streaming": stream_stats(p2)}
             log(f"{prec}: {extra[prec]['ms_per_step']} ms/step, {v2:.1f} sigma-vectors/s, coupling "
                 f"{r2['achieved']} TFLOP/s")
     hip.free(dX)
@@ -334,7 +352,7 @@ def main():
         "data": "synthetic (seeded orbital energies and density-fitting factors, seed 0)",
         "config": {"workload": f"{args.workload}: ADC(2)-s singlet o={sz['nocc']} v={sz['nvirt']} "
                                f"naux={sz['naux']}, ndim={op.ndim}, block of {nvec} vectors per step",
-                   "nvec": nvec, "ndim": op.ndim, "precision": "fp64"},
+                   "nvec": nvec, "ndim": op.ndim, "precision": "fp64", "coupling_form": form},
         "roofline": {k: roof[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
         "cpu_baseline": cpu,
         "sigma_GBps_effective": round(bytes_per_sigma * value / 1e9, 1),

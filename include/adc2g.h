@@ -57,13 +57,18 @@ extern "C" {
 #define ADC2G_PRE_OLSEN 2       /* Olsen */
 #define ADC2G_PRE_LEARNED 3     /* MLP correction on top of DPR, DPR fallback */
 
+/* Evaluation of the 1h1p<->2h2p coupling (same matrix either way) */
+#define ADC2G_COUPLING_AUTO 0   /* INT when it needs fewer flops and its integrals fit, else DF */
+#define ADC2G_COUPLING_DF 1     /* contractions with the factors B[P,p,q] (sums over naux) */
+#define ADC2G_COUPLING_INT 2    /* contractions with (vv|ov), (oo|ov) assembled once from the factors */
+
 typedef struct adc2g_handle adc2g_handle;
 
 typedef struct {
   int32_t device;        /* HIP device ordinal */
   int32_t build_phph;    /* 1: build the second-order ph-ph block (ADC(2)); 0: first order only */
-  int32_t keep_lowprec;  /* 1: also keep FP32/BF16 copies of the factors for mixed precision */
-  int32_t reserved;
+  int32_t keep_lowprec;  /* 1: also keep FP32/BF16 copies of the coupling operands for mixed precision */
+  int32_t coupling;      /* ADC2G_COUPLING_* */
 } adc2g_opts;
 
 /* Davidson parameters (the keys of rddiaginp, source/iomodules/rdinput.f90:1081-1288) */
@@ -127,6 +132,10 @@ int32_t adc2g_destroy(adc2g_handle* h);
 /* Sizes: ndim, n1, and the counts of t1..t5 (counts[0..4]); kpq(1..5,0) of
  * source/adclib/select_fano.f90 with kpq(5,0) repeated for t5. */
 int32_t adc2g_dims(const adc2g_handle* h, int64_t* ndim, int64_t* n1, int64_t counts[5]);
+
+/* Coupling form the handle evaluates (ADC2G_COUPLING_DF or _INT; no
+ * counterpart in the reference, which stores the matrix elements). */
+int32_t adc2g_coupling_form(const adc2g_handle* h, int32_t* form);
 
 /* Diagonal H_jj of the full matrix in configuration order: the hmlt.dia record
  * of get_diag_adc2_save (source/adclib/get_matrix.f90:267-311). */
@@ -210,12 +219,14 @@ int32_t adc2g_last_error(char* buf, int32_t len);
 /* Per-kernel device timing of sigma-vectors (no counterpart in the
  * reference; its timers are wall clocks around whole solver calls,
  * source/utilities/timingmod.f90:37-55).  enable = 1 starts recording HIP
- * events around the coupling contractions (class 0), the CSF -> pair-tensor
- * packing (class 1) and the pair -> CSF projection fused with the 2h2p
- * diagonal (class 2) of every following sigma-vector.  If out is not NULL it
- * receives, for classes 0..2, {milliseconds, launches, work} (9 doubles; work
- * in algorithmic flops for class 0 and algorithmic HBM bytes for 1 and 2) of
- * everything recorded since the previous call, and the records are reset. */
+ * events around the main coupling contractions (class 0: the sums over the
+ * virtual index, or over naux in the DF form), the CSF -> pair-tensor packing
+ * (class 1), the pair -> CSF projection fused with the 2h2p diagonal (class 2)
+ * and the remaining coupling contractions (class 3) of every following
+ * sigma-vector.  If out is not NULL it receives, for classes 0..3,
+ * {milliseconds, launches, work} (12 doubles; work in algorithmic flops for
+ * classes 0 and 3 and algorithmic HBM bytes for 1 and 2) of everything
+ * recorded since the previous call, and the records are reset. */
 int32_t adc2g_sigma_profile(adc2g_handle* h, int32_t enable, double* out);
 
 #ifdef __cplusplus

@@ -5,6 +5,8 @@ The oracle's literal restatement of the reference's element functions
 workload (o=16, v=200, naux=864, ndim 5.1e6) the density-fitted oracle
 restatement (pinned against the literal one in test_oracle_pins) and
 size-independent identities (symmetry <y,Hx> = <Hy,x>, linearity) are used.
+Both coupling forms are checked: contractions with the density-fitting
+factors ("df") and with the integral blocks assembled from them ("int").
 FP64 tolerance: 1e-11 relative to max|sigma|; FP32 coupling 2e-6; BF16 3e-2.
 """
 import numpy as np
@@ -16,16 +18,19 @@ from oracle import dfsigma, elements
 
 pytestmark = pytest.mark.gpu
 
-SMALL = [(2, 3, 2, 0), (3, 5, 7, 1), (4, 7, 11, 2), (5, 9, 13, 3), (5, 36, 164, 0)]
+SMALL = [(2, 3, 2, 0), (3, 5, 7, 1), (4, 7, 11, 2), (5, 9, 13, 3), (5, 36, 164, 0), (6, 16, 40, 4)]
+FORMS = ("df", "int")
 
 
-@pytest.fixture(scope="module", params=SMALL, ids=lambda p: "o%d_v%d_naux%d" % p[:3])
+@pytest.fixture(scope="module", params=[(c, f) for c in SMALL for f in FORMS],
+                ids=lambda p: "o%d_v%d_naux%d-%s" % (p[0][:3] + (p[1],)))
 def pair(request):
-    o, v, naux, seed = request.param
-    print(f"[case] sigma o={o} v={v} naux={naux} seed={seed}", flush=True)
+    (o, v, naux, seed), form = request.param
+    print(f"[case] sigma o={o} v={v} naux={naux} seed={seed} coupling={form}", flush=True)
     sys = make_system(o, v, naux, seed=seed)
     blk = elements.build_blocks(sys, apply_minc=False)
-    op = lib.Operator.from_system(sys)
+    op = lib.Operator.from_system(sys, coupling=form)
+    assert op.coupling == form
     yield sys, blk, op
     op.close()
 
@@ -84,12 +89,41 @@ def test_sigma_unit_vectors_are_columns(pair):
 
 
 @pytest.fixture(scope="module")
-def benzene():
-    print("[case] sigma benzene-size o=16 v=200 naux=864", flush=True)
-    sys = make_system(16, 200, 864, seed=0)
-    op = lib.Operator.from_system(sys)
-    yield sys, op
+def benzene_sys():
+    return make_system(16, 200, 864, seed=0)
+
+
+@pytest.fixture(scope="module", params=FORMS)
+def benzene(request, benzene_sys):
+    print(f"[case] sigma benzene-size o=16 v=200 naux=864 coupling={request.param}", flush=True)
+    op = lib.Operator.from_system(benzene_sys, coupling=request.param)
+    yield benzene_sys, op
     op.close()
+
+
+def test_benzene_auto_form_is_int(benzene_sys):
+    """At o=16, v=200, naux=864 the integral form needs (v+o)/naux = 1/4 of
+    the flops and fits easily, so the automatic choice takes it."""
+    with lib.Operator.from_system(benzene_sys, second_order=False) as op:
+        assert op.coupling == "int"
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6), ("bf16", 3e-2)])
+def test_sigma_benzene_forms_agree(benzene_sys, prec, tol):
+    """The two coupling forms evaluate the same matrix: FP64 to rounding,
+    FP32/BF16 each within its tolerance of the other's FP64 result."""
+    ops = {f: lib.Operator.from_system(benzene_sys, second_order=False, coupling=f) for f in FORMS}
+    try:
+        X = np.random.default_rng(21).standard_normal((ops["df"].ndim, 20))
+        out = {f: (op.sigma(X), op.sigma(X, precision=prec)) for f, op in ops.items()}
+    finally:
+        for op in ops.values():
+            op.close()
+    ref = out["df"][0]
+    scale = np.abs(ref).max()
+    assert np.abs(out["int"][0] - ref).max() < 1e-12 * scale
+    for form in FORMS:
+        assert np.abs(out[form][1] - ref).max() < tol * scale
 
 
 def test_sigma_benzene_symmetry_linearity(benzene):
@@ -117,14 +151,15 @@ def test_sigma_benzene_vs_df_oracle(benzene):
     assert np.abs(op.diagonal() - df.diag()).max() < 1e-11 * np.abs(df.diag()).max()
 
 
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("bf16", 3e-2)])
-def test_sigma_many_occupied(prec, tol):
+def test_sigma_many_occupied(prec, tol, form):
     """nocc > 64 takes the element-wise CSF<->pair kernels instead of the
     LDS-tiled ones; checked against the density-fitted oracle."""
-    print(f"[case] sigma o=66 v=4 naux=30 {prec}", flush=True)
+    print(f"[case] sigma o=66 v=4 naux=30 {prec} coupling={form}", flush=True)
     sys = make_system(66, 4, 30, seed=6)
     df = dfsigma.DF(sys)
-    with lib.Operator.from_system(sys) as op:
+    with lib.Operator.from_system(sys, coupling=form) as op:
         rng = np.random.default_rng(13)
         X = rng.standard_normal((op.ndim, 3))
         W = op.sigma(X, precision=prec)
