@@ -18,7 +18,10 @@ from oracle import dfsigma, elements
 
 pytestmark = pytest.mark.gpu
 
-SMALL = [(2, 3, 2, 0), (3, 5, 7, 1), (4, 7, 11, 2), (5, 9, 13, 3), (5, 36, 164, 0), (6, 16, 40, 4)]
+# odd nocc and v a multiple of 8 (16-byte vector loads of bf16 rows) next to
+# sizes that take the scalar GEMM loads
+SMALL = [(2, 3, 2, 0), (3, 5, 7, 1), (4, 7, 11, 2), (5, 9, 13, 3), (5, 36, 164, 0), (6, 16, 40, 4), (3, 8, 10, 5),
+         (7, 24, 30, 6)]
 FORMS = ("df", "int")
 
 
