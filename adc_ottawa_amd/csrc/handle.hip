@@ -377,6 +377,7 @@ void unpack_chunk(Prec p, const void* G, const double* Xc, int64_t ldx, double* 
 //   s1(a,k)  -= sum_{m,bl} Z[a][vec][m][bl] O[k][m][bl]             (down, occupied)
 //   G         = -sum_m x1(c,m) Ot[m][k][bl]   [c][vec][k][bl]        (up, occupied)
 //   G        += sum_a x1(a,k) Vt[c][bl][a]                           (up, batched over c)
+//              (BF16: the two up terms in the opposite order)
 //   s2        = projection of G onto the CSFs + D x2                 (unpack)
 // which is sigma1(a,k) = sum_{P,c} B[P,a,c] Y[P,c,k] - sum_{P,m} B[P,k,m]
 // Y[P,a,m] and G = sum_P W[P,c,k] B[P,l,b] of the DF form with the sum over P
@@ -431,7 +432,31 @@ void Handle::sigma_int(const double* Xc, int64_t ldx, double* Wc, int64_t ldw, i
     g.ev_begin = e.first; g.ev_end = e.second;
     gemm(g, p, gws, s);
   }
-  {  // G[(c*nc + vec)][(k,bl)] = -sum_m x1(c,m) Ot[m][k][bl]   (overwrites Z)
+  // occupied up term: streaming kernel (nocc <= 32) or GEMM.  BF16 writes G
+  // with the virtual contraction first and adds this term afterwards (no G
+  // read in the GEMM epilogue); FP64/FP32 write this term first and let the
+  // GEMM epilogue add to it (measured faster each way at o=16, v=200).
+  auto occ_kernel = [&](bool accumulate) -> bool {
+    auto e = span(3, flops_o);
+    if (e.first) HIP_CHECK(hipEventRecord(e.first, s));
+    bool ok = false;
+    switch (p) {
+      case Prec::F64:
+        ok = launch_occ_up<double>(Xc, ldx, nc, dk, (const double*)I.Ot.get(), (double*)ZG.get(), accumulate, s);
+        break;
+      case Prec::F32:
+        ok = launch_occ_up<float>(Xc, ldx, nc, dk, (const float*)I.Ot.get(), (float*)ZG.get(), accumulate, s);
+        break;
+      case Prec::BF16:
+        ok = launch_occ_up<__bf16>(Xc, ldx, nc, dk, (const __bf16*)I.Ot.get(), (__bf16*)ZG.get(), accumulate, s);
+        break;
+    }
+    if (e.second) HIP_CHECK(hipEventRecord(e.second, s));
+    if (!ok && prof_on) prof.pop_back();  // nothing was launched between the two events
+    return ok;
+  };
+  const bool occ_last = (p == Prec::BF16) && d.o <= 32;
+  if (!occ_last && !occ_kernel(false)) {  // G[(c*nc + vec)][(k,bl)] = -sum_m x1(c,m) Ot[m][k][bl]   (overwrites Z)
     GemmArgs g;
     g.M = v * nc; g.N = (int)(o * n1); g.K = o;
     g.A = X1c.get(); g.sam = o; g.sak = 1;
@@ -449,13 +474,15 @@ void Handle::sigma_int(const double* Xc, int64_t ldx, double* Wc, int64_t ldw, i
     g.M = o * nc; g.N = (int)n1; g.K = v; g.batch = v;
     g.A = X1t.get(); g.sam = v; g.sak = 1; g.sab = 0;
     g.B = I.Vt.get(); g.sbn = v; g.sbk = 1; g.sbb = n1 * v;
-    g.map.C = ZG.get(); g.map.scm = n1; g.map.scn = 1; g.map.scb = (int64_t)nc * o * n1; g.map.beta = 1.0;
+    g.map.C = ZG.get(); g.map.scm = n1; g.map.scn = 1; g.map.scb = (int64_t)nc * o * n1;
+    g.map.beta = occ_last ? 0.0 : 1.0;
     g.out_prec = out_lp;
     g.tag = 1;
     auto e = span(0, flops_v);
     g.ev_begin = e.first; g.ev_end = e.second;
     gemm(g, p, gws, s);
   }
+  if (occ_last) occ_kernel(true);
   auto eu = span(2, (double)nc * (3.0 * d.n2 * 8.0 + (double)n1 * n1 * esd));
   if (eu.first) HIP_CHECK(hipEventRecord(eu.first, s));
   unpack_chunk(p, ZG.get(), Xc, ldx, Wc, ldw, nc, dk, eps.get(), lay, s);

@@ -150,6 +150,45 @@ __global__ void pack_x1_kernel(const double* __restrict__ X, int64_t ldx, int nv
 }
 
 template <class T>
+__device__ __forceinline__ double dbl_of(T x) { return (double)x; }
+template <>
+__device__ __forceinline__ double dbl_of<__bf16>(__bf16 x) { return (double)(float)x; }
+
+// G[(c*nc + vec)][n] (+)= -sum_m x1(c,m;vec) Ot[m][n], n < o*n1: one column
+// per thread with its o values of Ot in registers, RB rows per workgroup with
+// their x1 values staged in LDS (read as broadcasts).
+constexpr int OCC_RB = 32;
+template <class T, int OMAX>
+__global__ void __launch_bounds__(256) occ_up_kernel(const double* __restrict__ X, int64_t ldx, int nc, DimsK d,
+                                                     const T* __restrict__ Ot, T* __restrict__ G, int accumulate) {
+  __shared__ double xs[OCC_RB * OMAX];
+  const int o = d.o;
+  const int64_t ncol = (int64_t)o * d.n1;
+  const int64_t rows = (int64_t)d.v * nc;
+  const int64_t r0 = (int64_t)blockIdx.y * OCC_RB;
+  const int nr = (int)min((int64_t)OCC_RB, rows - r0);
+  for (int t = threadIdx.x; t < nr * o; t += blockDim.x) {
+    const int r = t / o, m = t - r * o;
+    const int64_t mu = r0 + r, c = mu / nc, vec = mu - c * nc;
+    xs[r * OMAX + m] = X[c * o + m + ldx * vec];
+  }
+  __syncthreads();
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= ncol) return;
+  double ot[OMAX];
+#pragma unroll
+  for (int m = 0; m < OMAX; ++m) ot[m] = (m < o) ? dbl_of(Ot[(int64_t)m * ncol + n]) : 0.0;
+  for (int r = 0; r < nr; ++r) {
+    double acc = 0.0;
+#pragma unroll
+    for (int m = 0; m < OMAX; ++m)
+      if (m < o) acc += xs[r * OMAX + m] * ot[m];
+    T* g = G + (r0 + r) * ncol + n;
+    *g = accumulate ? (T)(dbl_of(*g) - acc) : (T)(-acc);
+  }
+}
+
+template <class T>
 __global__ void pack_x1t_kernel(const double* __restrict__ X, int64_t ldx, int nvec, DimsK d, T* __restrict__ X1t) {
   const int64_t total = d.n1 * nvec;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
@@ -439,6 +478,20 @@ void launch_pack_x1(const double* X, int64_t ldx, int nvec, const DimsK& d, T* X
 }
 
 template <class T>
+bool launch_occ_up(const double* X, int64_t ldx, int nc, const DimsK& d, const T* Ot, T* G, bool accumulate,
+                   hipStream_t st) {
+  if (d.o > 32) return false;
+  const int64_t ncol = (int64_t)d.o * d.n1, rows = (int64_t)d.v * nc;
+  dim3 grid((unsigned)((ncol + 255) / 256), (unsigned)((rows + OCC_RB - 1) / OCC_RB));
+  if (d.o <= 16)
+    hipLaunchKernelGGL((occ_up_kernel<T, 16>), grid, dim3(256), 0, st, X, ldx, nc, d, Ot, G, accumulate ? 1 : 0);
+  else
+    hipLaunchKernelGGL((occ_up_kernel<T, 32>), grid, dim3(256), 0, st, X, ldx, nc, d, Ot, G, accumulate ? 1 : 0);
+  HIP_CHECK(hipGetLastError());
+  return true;
+}
+
+template <class T>
 void launch_pack_x1t(const double* X, int64_t ldx, int nvec, const DimsK& d, T* X1t, hipStream_t st) {
   hipLaunchKernelGGL((pack_x1t_kernel<T>), dim3(grid_for(d.n1 * nvec)), dim3(256), 0, st, X, ldx, nvec, d, X1t);
   HIP_CHECK(hipGetLastError());
@@ -504,6 +557,7 @@ void launch_phph_assemble(double* H11, const double* Mm, const double* Jm, const
 #define INST(T)                                                                                             \
   template void launch_pack_x1<T>(const double*, int64_t, int, const DimsK&, T*, hipStream_t);               \
   template void launch_pack_x1t<T>(const double*, int64_t, int, const DimsK&, T*, hipStream_t);              \
+  template bool launch_occ_up<T>(const double*, int64_t, int, const DimsK&, const T*, T*, bool, hipStream_t);    \
   template void launch_csf_to_pair<T>(const double*, int64_t, int, const DimsK&, T*, const PairLayout&,       \
                                       hipStream_t);                                                           \
   template void launch_pair_to_csf<T>(const T*, const double*, int64_t, double*, int64_t, int, const DimsK&, \

@@ -52,6 +52,12 @@ void launch_pack_x1(const double* X, int64_t ldx, int nvec, const DimsK& d, T* X
 // X1t[(vec,k)][a] <- X[a*o+k + ldx*vec]
 template <class T>
 void launch_pack_x1t(const double* X, int64_t ldx, int nvec, const DimsK& d, T* X1t, hipStream_t st);
+// Occupied term of the INT up contraction, virtual-major pair layout:
+// G[(c*nc + vec)][(k,bl)] = -sum_m x1(c,m;vec) Ot[m][(k,bl)], or added to G
+// when accumulate.  Returns false (nothing launched) for nocc > 32.
+template <class T>
+bool launch_occ_up(const double* X, int64_t ldx, int nc, const DimsK& d, const T* Ot, T* G, bool accumulate,
+                   hipStream_t st);
 // Z[(c,k),(b,l)] (n1 x n1 per vector, stored as `lay`) from the 2h2p amplitudes of X
 template <class T>
 void launch_csf_to_pair(const double* X, int64_t ldx, int nvec, const DimsK& d, T* Z, const PairLayout& lay,
