@@ -17,6 +17,8 @@ namespace {
 
 constexpr int SYEV_THREADS = 512;
 constexpr int SYEV_NMAX = 512;
+constexpr int SYEVX_ZCAP = 10240;  // doubles of LDS for the eigenvector block (80 KB)
+constexpr int ORTH_ZCAP = 4096;    // the same for the Gram-matrix eigensolve (blocks up to 64)
 
 __device__ double block_sum(double x, double* sh) {
   for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
@@ -116,10 +118,10 @@ __device__ void tridiag_dev(int n, double* A, int lda, double* Zw, SyevShared& S
       }
     }
     __syncthreads();
-    const int64_t mm = (int64_t)m * m;
-    for (int64_t idx = tid; idx < mm; idx += nt) {
-      const int i = k + 1 + (int)(idx % m), j = k + 1 + (int)(idx / m);
-      A[i + (int64_t)j * lda] -= vv[i] * qv[j] + qv[i] * vv[j];
+    for (int j = k + 1 + wv; j < n; j += nw) {
+      const double vj = vv[j], qj = qv[j];
+      double* Aj = A + (int64_t)j * lda;
+      for (int i = k + 1 + lane; i < n; i += 64) Aj[i] -= vv[i] * qj + qv[i] * vj;
     }
     __syncthreads();
   }
@@ -313,8 +315,12 @@ __device__ __forceinline__ double unit_rand(uint64_t a, uint64_t b) {
 //    two modified Gram-Schmidt sweeps over the k vectors (one wave);
 //  * back-transformation of the k vectors with the stored reflectors.
 // work: 2n + 6nk doubles.
+// zl / zcap: LDS for the k eigenvectors ([i * k + j]) through inverse
+// iteration, Gram-Schmidt and back-transformation when k * n <= zcap (their
+// steps are serial chains, so the latency of each access is what counts);
+// otherwise they run on Z in global memory.
 __device__ void syevx_dev(int n, int k, double* A, int lda, double* w, double* Z, int ldz, double* work, int* info,
-                          SyevShared& S, BisectShared& B) {
+                          SyevShared& S, BisectShared& B, double* zl = nullptr, int zcap = 0) {
   const int tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
   tridiag_dev(n, A, lda, work, S);
@@ -410,7 +416,7 @@ __device__ void syevx_dev(int n, int k, double* A, int lda, double* w, double* Z
     double* U3 = U2 + (int64_t)n * k;
     double* ML = U3 + (int64_t)n * k;
     double* PV = ML + (int64_t)n * k;
-    double* Y = PV + (int64_t)n * k;
+    double* Y = (zl && (int64_t)n * k <= zcap) ? zl : PV + (int64_t)n * k;
     const double tiny = DBL_EPSILON * tnorm;
     for (int j = tid; j < k; j += nt) {
       const double mu = B.mu[j];
@@ -467,44 +473,51 @@ __device__ void syevx_dev(int n, int k, double* A, int lda, double* w, double* Z
         const double sc = (ymax > 0.0 && isfinite(ymax)) ? 1.0 / ymax : 0.0;
         for (int i = 0; i < n; ++i) Y[(int64_t)i * k + j] *= sc;
       }
-      for (int i = 0; i < n; ++i) Z[i + (int64_t)j * ldz] = Y[(int64_t)i * k + j];
+      if (Y != zl)
+        for (int i = 0; i < n; ++i) Z[i + (int64_t)j * ldz] = Y[(int64_t)i * k + j];
     }
   }
   __syncthreads();
+  // element (i, j) of the eigenvector block: LDS or Z
+  const bool in_lds = zl && (int64_t)n * k <= zcap;
+  auto zat = [&](int i, int j) -> double& { return in_lds ? zl[(int64_t)i * k + j] : Z[i + (int64_t)j * ldz]; };
   // ---- two modified Gram-Schmidt sweeps (wave 0)
   if (wv == 0) {
     for (int j = 0; j < k; ++j) {
-      double* zj = Z + (int64_t)j * ldz;
       for (int pass = 0; pass < 2; ++pass) {
         for (int i = 0; i < j; ++i) {
-          const double* zi = Z + (int64_t)i * ldz;
           double t = 0.0;
-          for (int r = lane; r < n; r += 64) t += zi[r] * zj[r];
+          for (int r = lane; r < n; r += 64) t += zat(r, i) * zat(r, j);
           for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
-          for (int r = lane; r < n; r += 64) zj[r] -= t * zi[r];
+          for (int r = lane; r < n; r += 64) zat(r, j) -= t * zat(r, i);
         }
       }
       double t = 0.0;
-      for (int r = lane; r < n; r += 64) t += zj[r] * zj[r];
+      for (int r = lane; r < n; r += 64) t += zat(r, j) * zat(r, j);
       for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
       const double sc = (t > 0.0) ? 1.0 / sqrt(t) : 0.0;
-      for (int r = lane; r < n; r += 64) zj[r] *= sc;
+      for (int r = lane; r < n; r += 64) zat(r, j) *= sc;
     }
   }
   __syncthreads();
   // ---- back transformation of the k vectors (one column per wave)
   for (int j = wv; j < k; j += nw) {
-    double* z = Z + (int64_t)j * ldz;
     for (int kk = n - 3; kk >= 0; --kk) {
       const double* v = A + (int64_t)kk * lda;
       double t = 0.0;
-      for (int i = kk + 1 + lane; i < n; i += 64) t += v[i] * z[i];
+      for (int i = kk + 1 + lane; i < n; i += 64) t += v[i] * zat(i, j);
       for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
       t *= 2.0;
       if (t != 0.0)
-        for (int i = kk + 1 + lane; i < n; i += 64) z[i] -= t * v[i];
+        for (int i = kk + 1 + lane; i < n; i += 64) zat(i, j) -= t * v[i];
     }
   }
+  __syncthreads();
+  if (in_lds)
+    for (int64_t idx = tid; idx < (int64_t)n * k; idx += nt) {
+      const int i = (int)(idx % n), j = (int)(idx / n);
+      Z[i + (int64_t)j * ldz] = zl[(int64_t)i * k + j];
+    }
   if (tid == 0) info[0] = 0;
   __syncthreads();
 }
@@ -513,7 +526,8 @@ __global__ void __launch_bounds__(SYEV_THREADS) syevx_kernel(int n, int k, doubl
                                                               int ldz, double* work, int* info) {
   __shared__ SyevShared S;
   __shared__ BisectShared B;
-  syevx_dev(n, k, A, lda, w, Z, ldz, work, info, S, B);
+  __shared__ double zl[SYEVX_ZCAP];
+  syevx_dev(n, k, A, lda, w, Z, ldz, work, info, S, B, zl, SYEVX_ZCAP);
 }
 
 // M = D U_kept Lambda^{-1/2} for the scaled Gram matrix D G D (D = diag(G)^{-1/2}).
@@ -542,7 +556,8 @@ __global__ void __launch_bounds__(SYEV_THREADS) orth_coeffs_kernel(int r, const 
   }
   __syncthreads();
   __shared__ int sinfo;
-  syevx_dev(r, r, Gc, r, lam, U, r, Zw, &sinfo, S, Bs);
+  __shared__ double zl[ORTH_ZCAP];
+  syevx_dev(r, r, Gc, r, lam, U, r, Zw, &sinfo, S, Bs, zl, ORTH_ZCAP);
   __syncthreads();
   if (tid == 0) {
     int c = 0;
