@@ -182,7 +182,7 @@ def cpu_baseline(sys_, op, nvec):
             "seconds_per_block_scaled": round(t_full, 2), "stored_elements_full": int(coo.full_nnz)}
 
 
-def davidson_compare(op, nroots, only=None):
+def davidson_compare(op, nroots, only=None, blocksize=20, maxvec=200):
     res = {}
     runs = [("fp64_dpr", dict(precon="dpr", precision="fp64")),
             ("bf16_dpr_refined", dict(precon="dpr", precision="bf16", lowprec_tol=5e-2)),
@@ -198,8 +198,8 @@ def davidson_compare(op, nroots, only=None):
     for name, kw in runs:
         t0 = time.perf_counter()
         try:
-            ev, _, st = op.davidson(nroots, blocksize=20, maxvec=200, tol=1e-7, maxiter=1000, deflate=True,
-                                    vectors=False, **kw)
+            ev, _, st = op.davidson(nroots, blocksize=blocksize, maxvec=maxvec, tol=1e-7, maxiter=1000,
+                                    deflate=True, vectors=False, **kw)
         except lib.Adc2gError as e:
             res[name] = {"error": str(e)}
             log(f"davidson {name}: {e}")
@@ -223,6 +223,7 @@ def davidson_compare(op, nroots, only=None):
             if "seconds" in res[name]:
                 res[name]["speedup_vs_fp64_dpr"] = round(base / res[name]["seconds"], 3)
     res["learned_weights"] = have_net
+    res["roots"], res["blocksize"], res["maxvec"] = nroots, blocksize, maxvec
     return res
 
 
@@ -270,6 +271,9 @@ def main():
     ap.add_argument("--sil", type=float, default=0.0,
                     help="also propagate with SIL to this final time (a.u.), FP64 and FP32 sigma")
     ap.add_argument("--sil-tol", type=float, default=1e-10, help="SIL error tolerance (autotol)")
+    ap.add_argument("--dav-roots", type=int, default=20, help="Davidson comparison: roots (block >= roots)")
+    ap.add_argument("--dav-block", type=int, default=20, help="Davidson comparison: block size")
+    ap.add_argument("--dav-maxvec", type=int, default=200, help="Davidson comparison: maximum subspace")
     ap.add_argument("--dav-runs", default="",
                     help="comma-separated subset of the Davidson comparison runs (default: all)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -331,7 +335,8 @@ def main():
             log(f"cpu baseline failed: {e}")
     dav = None
     if not args.skip_davidson:
-        dav = davidson_compare(op, 20, [r for r in args.dav_runs.split(",") if r])
+        dav = davidson_compare(op, args.dav_roots, [r for r in args.dav_runs.split(",") if r],
+                               blocksize=args.dav_block, maxvec=args.dav_maxvec)
 
     sil = sil_compare(op, sz, args.sil, args.sil_tol) if args.sil > 0 else None
 
