@@ -315,6 +315,11 @@ void sil_run(Handle& H, const adc2g_sil_params& p, const double* fvec, double* a
     const cplx a = S.square_sum(S.Kp(0)) * norm2;
     auto0[2 * i] = a.real();
     auto0[2 * i + 1] = a.imag();
+    if (p.verbose && (i % 100 == 0 || i == nrec)) {
+      std::fprintf(stderr, "sil: record %lld of %lld (t = %.2f a.u.), %lld steps, %lld sigma-vectors\n", (long long)i,
+                   (long long)nrec, i * intperiod, (long long)nsteps, (long long)S.nsigma);
+      std::fflush(stderr);
+    }
   }
   if (nrec_out) *nrec_out = nrec + 1;
   if (stats) {

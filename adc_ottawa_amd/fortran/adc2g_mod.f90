@@ -58,6 +58,8 @@ module adc2g_mod
      integer(c_int32_t) :: kdim = 10
      integer(c_int32_t) :: precision = 0
      real(c_double)     :: autotol = 1.0d-5
+     integer(c_int32_t) :: verbose = 0
+     integer(c_int32_t) :: reserved = 0
   end type adc2g_sil_params
 
   type, bind(c) :: adc2g_sil_stats

@@ -55,7 +55,8 @@ class DavStats(C.Structure):
 
 class SilParams(C.Structure):
     _fields_ = [("tout", C.c_double), ("tfinal", C.c_double), ("kdim", C.c_int32),
-                ("precision", C.c_int32), ("autotol", C.c_double)]
+                ("precision", C.c_int32), ("autotol", C.c_double), ("verbose", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class SilStats(C.Structure):
@@ -210,11 +211,11 @@ class Operator:
         stats = {f: getattr(st, f) for f, _ in DavStats._fields_}
         return ev, (vec.T if vectors else None), stats
 
-    def sil_autocorr(self, fvec, tout, tfinal, kdim, autotol, precision="fp64"):
+    def sil_autocorr(self, fvec, tout, tfinal, kdim, autotol, precision="fp64", verbose=False):
         fvec = np.ascontiguousarray(fvec, dtype=np.float64)
         nrec = int(tfinal / (0.5 * tout))
         out = np.empty(nrec + 1, dtype=np.complex128)
-        p = SilParams(tout, tfinal, kdim, PREC[precision], autotol)
+        p = SilParams(tout, tfinal, kdim, PREC[precision], autotol, 1 if verbose else 0, 0)
         n = C.c_int64()
         st = SilStats()
         check(_lib.adc2g_sil_autocorr(self._h, C.byref(p), _ptr(fvec), _ptr(out), out.size,

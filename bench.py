@@ -239,7 +239,8 @@ def sil_compare(op, sz, tfinal, autotol, seed=0):
     out, ref = {}, None
     for prec in ("fp64", "fp32"):
         t0 = time.perf_counter()
-        auto, st = op.sil_autocorr(f, tout=0.1, tfinal=tfinal, kdim=20, autotol=autotol, precision=prec)
+        auto, st = op.sil_autocorr(f, tout=0.1, tfinal=tfinal, kdim=20, autotol=autotol, precision=prec,
+                                   verbose=True)
         wall = time.perf_counter() - t0
         rec = {"seconds": round(wall, 3), "records": int(auto.size), "lanczos_steps": int(st["nsteps"]),
                "sigma_vectors": int(st["nsigma"]), "max_order": int(st["max_order"])}

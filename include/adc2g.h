@@ -104,6 +104,8 @@ typedef struct {
   int32_t kdim;          /* krydim: maximum Lanczos order */
   int32_t precision;     /* ADC2G_PREC_* of the sigma-vectors */
   double autotol;        /* SIL error tolerance */
+  int32_t verbose;       /* 1: a progress line on stderr every 100 records */
+  int32_t reserved;
 } adc2g_sil_params;
 
 typedef struct {
