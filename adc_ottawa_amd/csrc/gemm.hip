@@ -63,9 +63,11 @@ void dispatch(GemmArgs g, GemmWorkspace& ws, hipStream_t st, int splitk_hint) {
   constexpr int BK = (sizeof(T) == 8) ? 16 : (sizeof(T) == 4 ? 32 : 64);
   // tile shape: 128x32 for N <= 32, 64x64 when M or N is small; otherwise
   // the shape with the least padded output (within 2%), larger tiles first
-  static const int SBM[6] = {0, 128, 64, 128, 128, 64};
-  static const int SBN[6] = {0, 128, 64, 32, 64, 128};
+  static const int SBM[7] = {0, 128, 64, 128, 128, 64, 64};
+  static const int SBN[7] = {0, 128, 64, 32, 64, 128, 208};
   int shape = g.tile;
+  // the 208-column tile loads its operands along K only
+  if (shape == 6 && !(g.sak == 1 && g.sbk == 1)) shape = 0;
   if (shape <= 0) {
     auto padded = [&](int sh) {
       const int bm = SBM[sh], bn = SBN[sh];
@@ -126,6 +128,7 @@ void dispatch(GemmArgs g, GemmWorkspace& ws, hipStream_t st, int splitk_hint) {
     case 3: launch<T, 128, 32, BK>(g, st); break;
     case 4: launch<T, 128, 64, BK>(g, st); break;
     case 5: launch<T, 64, 128, BK>(g, st); break;
+    case 6: launch<T, 64, 208, BK>(g, st); break;
     default: launch<T, 128, 128, BK>(g, st);
   }
   if (need_partial) {

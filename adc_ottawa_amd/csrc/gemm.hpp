@@ -28,6 +28,7 @@ enum MapKind : int {
   MAP_SIG1O = 3,  // (a*nvec + vec, k) -> sigma1
   MAP_WUPV = 4,   // (c + v*P, k + o*vec) -> W[P][vec][c][k]
   MAP_WUPO = 5,   // (c*nvec + vec, k + o*P) -> W[P][vec][c][k]
+  MAP_SIG1T = 6,  // (k + o*vec, a) -> sigma1 (the transpose of MAP_SIG1V)
 };
 
 struct MapParams {
@@ -56,7 +57,7 @@ struct GemmArgs {
   // batches and splits into a single M x N output through the map)
   int reduce_batch = 0;
   // tile shape: 0 chosen by the launcher, 1 = 128x128, 2 = 64x64, 3 = 128x32,
-  // 4 = 128x64, 5 = 64x128
+  // 4 = 128x64, 5 = 64x128, 6 = 64x208 (N = 200-ish; both operands K-contiguous)
   int tile = 0;
   void* partial = nullptr;  // split-K workspace (accumulator type), batch*splitk*M*N
   // host-side only: kernel tag (1 = the DF coupling contractions, compiled as a
@@ -83,6 +84,12 @@ __device__ __forceinline__ int map_offsets(const MapParams& mp, int64_t m, int64
       const int64_t o = mp.q[0], ldw = mp.q[1];
       const int64_t vec = n / o, k = n - vec * o;
       o1 = m * o + k + ldw * vec;
+      return 1;
+    }
+    case MAP_SIG1T: {
+      const int64_t o = mp.q[0], ldw = mp.q[1];
+      const int64_t vec = m / o, k = m - vec * o;
+      o1 = n * o + k + ldw * vec;
       return 1;
     }
     case MAP_SIG1O: {
@@ -264,8 +271,14 @@ gemm_kernel(GemmArgs g) {
   constexpr int PAD = 16 / sizeof(T);  // 16 B of padding per LDS row
   constexpr int LDR = BK + PAD;
   constexpr int WGN = NT / 128;        // waves along N (2 x WGN wave grid)
-  constexpr int WM = BM / 2, WN = BN / WGN;
-  constexpr int TM = WM / 16, TN = WN / 16;
+  // 16-column blocks per wave; when they do not divide evenly (BN = 208:
+  // 7 + 6) the last wave takes fewer, tn_eff below
+  constexpr int NB16 = BN / 16;
+  constexpr int TN = (NB16 + WGN - 1) / WGN;
+  constexpr bool EVEN = (NB16 % WGN == 0);
+  constexpr int WM = BM / 2, WN = TN * 16;
+  constexpr int TM = WM / 16;
+  static_assert(BN % 16 == 0 && BM % 32 == 0, "tile shape");
   constexpr int EA = BM * BK / NT;     // elements per thread per tile
   constexpr int EB = BN * BK / NT;
   static_assert(TN >= 1 && EA >= 1 && EB >= 1, "tile too small for the thread count");
@@ -278,6 +291,7 @@ gemm_kernel(GemmArgs g) {
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave / WGN, wn = wave % WGN;
+  const int tn_eff = EVEN ? TN : min(TN, NB16 - wn * TN);  // wave-uniform
 
   const int tiles_m = (g.M + BM - 1) / BM;
   const int tiles_n = (g.N + BN - 1) / BN;
@@ -464,11 +478,13 @@ gemm_kernel(GemmArgs g) {
           fa[i] = MT::load(&sA[buf][(wm * WM + i * 16 + li) * LDR + gq * (KS * KV) + s * KV]);
 #pragma unroll
         for (int j = 0; j < TN; ++j)
-          fb[j] = MT::load(&sB[buf][(wn * WN + j * 16 + li) * LDR + gq * (KS * KV) + s * KV]);
+          if (EVEN || j < tn_eff)
+            fb[j] = MT::load(&sB[buf][(wn * WN + j * 16 + li) * LDR + gq * (KS * KV) + s * KV]);
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
-          for (int j = 0; j < TN; ++j) acc[i][j] = MT::mma(fa[i], fb[j], acc[i][j]);
+          for (int j = 0; j < TN; ++j)
+            if (EVEN || j < tn_eff) acc[i][j] = MT::mma(fa[i], fb[j], acc[i][j]);
       }
       if (more) {
         store_a(buf ^ 1);
@@ -494,7 +510,7 @@ gemm_kernel(GemmArgs g) {
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
         const int n = n0 + wn * WN + j * 16 + li;
-        nv[j] = n < g.N;
+        nv[j] = n < g.N && (EVEN || j < tn_eff);
         map_npart<MK>(g.map, nv[j] ? (uint32_t)n : 0u, gn1[j], gn2[j]);
       }
 #pragma unroll
@@ -524,7 +540,7 @@ gemm_kernel(GemmArgs g) {
       for (int r = 0; r < 4; ++r) {
         const int m = m0 + wm * WM + i * 16 + MT::row(lane, r);
         const int n = n0 + wn * WN + j * 16 + li;
-        if (m < g.M && n < g.N) {
+        if (m < g.M && n < g.N && (EVEN || j < tn_eff)) {
           if (MK == MAP_STD && direct) {
             const int64_t off = (int64_t)m * g.map.scm + (int64_t)n * g.map.scn + (int64_t)b * g.map.scb;
             if (g.out_prec == 0) {

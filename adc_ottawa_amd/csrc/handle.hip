@@ -409,13 +409,15 @@ void Handle::sigma_int(const double* Xc, int64_t ldx, double* Wc, int64_t ldw, i
     g.map.C = Wc; g.map.scm = 1; g.map.scn = ldw;
     gemm(g, Prec::F64, gws, s);
   }
-  {  // sigma1(a, k + o*vec) += sum_{c,bl} V[a][c][bl] Z[c][vec][k][bl]
+  {  // sigma1(a, k + o*vec) += sum_{c,bl} Z[c][vec][k][bl] V[a][c][bl], computed as its
+     // transpose (rows (vec,k), columns a) so that v = 200 fills a 208-column tile
     GemmArgs g;
-    g.M = v; g.N = o * nc; g.K = (int)n1; g.batch = v; g.reduce_batch = 1;
-    g.A = I.V.get(); g.sam = (int64_t)v * n1; g.sak = 1; g.sab = n1;
-    g.B = ZG.get(); g.sbn = n1; g.sbk = 1; g.sbb = (int64_t)nc * o * n1;
-    g.map.kind = MAP_SIG1V; g.map.C = Wc; g.map.beta = 1.0;
+    g.M = o * nc; g.N = v; g.K = (int)n1; g.batch = v; g.reduce_batch = 1;
+    g.A = ZG.get(); g.sam = n1; g.sak = 1; g.sab = (int64_t)nc * o * n1;
+    g.B = I.V.get(); g.sbn = (int64_t)v * n1; g.sbk = 1; g.sbb = n1;
+    g.map.kind = MAP_SIG1T; g.map.C = Wc; g.map.beta = 1.0;
     g.map.q[0] = o; g.map.q[1] = ldw;
+    if (v > 192 && v <= 208) g.tile = 6;
     g.tag = 1;
     auto e = span(0, flops_v);
     g.ev_begin = e.first; g.ev_end = e.second;

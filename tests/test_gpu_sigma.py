@@ -168,3 +168,19 @@ def test_sigma_many_occupied(prec, tol, form):
         W = op.sigma(X, precision=prec)
     ref = df.sigma(X)
     assert np.abs(W - ref).max() < tol * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("fp32", 2e-6), ("bf16", 3e-2)])
+def test_sigma_virtual_200(prec, tol, form):
+    """v = 200 takes the 208-column GEMM tile of the INT down contraction
+    (uneven split of the 13 column blocks over the waves); nocc = 2 keeps the
+    density-fitted oracle fast.  9 vectors: a partial last vector group."""
+    print(f"[case] sigma o=2 v=200 naux=20 {prec} coupling={form}", flush=True)
+    sys = make_system(2, 200, 20, seed=8)
+    df = dfsigma.DF(sys)
+    with lib.Operator.from_system(sys, coupling=form) as op:
+        X = np.random.default_rng(14).standard_normal((op.ndim, 9))
+        W = op.sigma(X, precision=prec)
+    ref = df.sigma(X)
+    assert np.abs(W - ref).max() < tol * np.abs(ref).max()
