@@ -217,6 +217,15 @@ int32_t adc2g_last_error(char* buf, int32_t len) {
   return ADC2G_OK;
 }
 
+int32_t adc2g_cheby_auto(adc2g_handle* h, const double* bounds, const double* q0, int32_t order,
+                         int32_t precision, double* auto_out) {
+  return guarded([&] {
+    ARG_CHECK(h && bounds && q0 && auto_out, "NULL argument");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    cheby_run(h->h, bounds, q0, order, prec_from_int(precision), auto_out);
+  });
+}
+
 int32_t adc2g_sigma_profile(adc2g_handle* h, int32_t enable, double* out) {
   return guarded([&] {
     ARG_CHECK(h != nullptr, "NULL handle");

@@ -1,0 +1,118 @@
+// Chebyshev order-domain autocorrelation function of the ADC(2)-s matrix
+// (chebyshev_auto, source/propagation/chebyspec.f90:245-319), with the matrix
+// scaled to [-1, 1] by the spectral bounds as in matxvec_chebyshev_incore
+// (source/propagation/tdselib.f90:1316-1398):
+//   H_norm = (2/dE) H - (2 Emin/dE + 1) I,  dE = Emax - Emin,
+//   q_1 = H_norm q_0,  q_k = 2 H_norm q_{k-1} - q_{k-2}   (chebyshev_recursion, tdselib.f90:1228-1257),
+//   C_k = q_0.q_k for k = 0..K and, for k > K/2, C_2k = 2 q_k.q_k - C_0 and
+//   C_{2k-1} = 2 q_{k-1}.q_k - C_1 (K = order, 2K+1 values).
+// One sigma-vector per order; the scaling, the recursion and the three dot
+// products of the order are one streaming kernel; the dot products stay on
+// the device until the end (no host round trip per order).
+#include <cmath>
+#include <vector>
+
+#include "linalg.hpp"
+#include "solvers.hpp"
+
+namespace adc2g {
+
+namespace {
+
+constexpr int CB_THREADS = 256;
+
+// qk = c (a w + b qkm1) - d qkm2; part[blk*3 + {0,1,2}] = partial q0.qk, qk.qk, qkm1.qk
+__global__ void __launch_bounds__(CB_THREADS) cheby_step_kernel(int64_t n, double a, double b, double c, double d,
+                                                                const double* __restrict__ w,
+                                                                const double* __restrict__ qkm1,
+                                                                const double* __restrict__ qkm2,
+                                                                const double* __restrict__ q0,
+                                                                double* __restrict__ qk, double* __restrict__ part) {
+  __shared__ double red[3][CB_THREADS / 64];
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  const bool three = (d != 0.0);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double p = qkm1[i];
+    double x = c * (a * w[i] + b * p);
+    if (three) x -= d * qkm2[i];
+    qk[i] = x;
+    s0 += q0[i] * x;
+    s1 += x * x;
+    s2 += p * x;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s0 += __shfl_down(s0, off, 64);
+    s1 += __shfl_down(s1, off, 64);
+    s2 += __shfl_down(s2, off, 64);
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][wv] = s0;
+    red[1][wv] = s1;
+    red[2][wv] = s2;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double t = 0.0;
+    for (int i = 0; i < CB_THREADS / 64; ++i) t += red[threadIdx.x][i];
+    part[(int64_t)blockIdx.x * 3 + threadIdx.x] = t;
+  }
+}
+
+}  // namespace
+
+void cheby_run(Handle& H, const double bounds[2], const double* q0_h, int order, Prec p, double* auto_out) {
+  ARG_CHECK(order >= 1, "order must be >= 1");
+  ARG_CHECK(std::isfinite(bounds[0]) && std::isfinite(bounds[1]) && bounds[1] > bounds[0],
+            "bounds must be finite with Emax > Emin");
+  const int64_t n = H.d.ndim;
+  const double de = bounds[1] - bounds[0];
+  const double a = 2.0 / de, b = -(2.0 * bounds[0] / de + 1.0);
+  hipStream_t st = H.st;
+  DevBuf<double> q0, qa, qb, qc, w, part, dots;
+  q0.alloc(n);
+  qa.alloc(n);
+  qb.alloc(n);
+  qc.alloc(n);
+  w.alloc(n);
+  const int nblk = reduce_blocks(n);
+  part.alloc((size_t)nblk * 3);
+  dots.alloc((size_t)3 * (order + 1));
+  HIP_CHECK(hipMemcpyAsync(q0.get(), q0_h, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipMemcpyAsync(qa.get(), q0_h, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  // order 0: the three dots of q0 with itself (C_0 from the first)
+  hipLaunchKernelGGL(cheby_step_kernel, dim3(nblk), dim3(CB_THREADS), 0, st, n, 0.0, 1.0, 1.0, 0.0, qa.get(),
+                     qa.get(), qa.get(), q0.get(), qc.get(), part.get());
+  HIP_CHECK(hipGetLastError());
+  launch_colsum(nblk, 3, part.get(), dots.get(), st);
+  double* qkm2 = qc.get();  // unused at k = 1
+  double* qkm1 = qa.get();
+  double* qk = qb.get();
+  for (int k = 1; k <= order; ++k) {
+    H.sigma(qkm1, n, w.get(), n, 1, p, st);
+    const double c = (k > 1) ? 2.0 : 1.0, d = (k > 1) ? 1.0 : 0.0;
+    hipLaunchKernelGGL(cheby_step_kernel, dim3(nblk), dim3(CB_THREADS), 0, st, n, a, b, c, d, w.get(), qkm1, qkm2,
+                       q0.get(), qk, part.get());
+    HIP_CHECK(hipGetLastError());
+    launch_colsum(nblk, 3, part.get(), dots.get() + 3 * k, st);
+    double* t = qkm2;
+    qkm2 = qkm1;
+    qkm1 = qk;
+    qk = t;
+  }
+  std::vector<double> h((size_t)3 * (order + 1));
+  HIP_CHECK(hipMemcpyAsync(h.data(), dots.get(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+  // the assignments in the reference's order (a doubled value C_{2k-1} with
+  // 2k-1 <= K is overwritten by the direct one of a later order)
+  auto_out[0] = h[0];
+  for (int k = 1; k <= order; ++k) {
+    auto_out[k] = h[3 * k];
+    if (k > order / 2) {
+      auto_out[2 * k] = 2.0 * h[3 * k + 1] - auto_out[0];
+      auto_out[2 * k - 1] = 2.0 * h[3 * k + 2] - auto_out[1];
+    }
+  }
+}
+
+}  // namespace adc2g

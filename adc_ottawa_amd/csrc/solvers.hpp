@@ -16,6 +16,10 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
 void sil_run(Handle& H, const adc2g_sil_params& p, const double* fvec, double* auto0, int64_t nrec_cap,
              int64_t* nrec_out, adc2g_sil_stats* stats);
 
+// Chebyshev order-domain autocorrelation C_0..C_2K (auto_out, host) of the
+// host vector q0 with the spectrum scaled by bounds[0..1] = (Emin, Emax).
+void cheby_run(Handle& H, const double bounds[2], const double* q0, int order, Prec p, double* auto_out);
+
 int precon_nparams();
 void precon_harvest(Handle& H, const adc2g_dav_params& p, int64_t max_records, int64_t* added);
 void precon_train(Handle& H, const Handle& data, int epochs, int batch, double lr, uint64_t seed, double* loss_out);

@@ -136,6 +136,16 @@ module adc2g_mod
        type(adc2g_sil_stats), intent(out) :: stats
      end function adc2g_sil_autocorr
 
+     integer(c_int32_t) function adc2g_cheby_auto(h, bounds, q0, order, precision, auto_out) &
+          bind(c, name='adc2g_cheby_auto')
+       import :: c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: bounds(2)
+       real(c_double), intent(in) :: q0(*)
+       integer(c_int32_t), value :: order, precision
+       real(c_double), intent(out) :: auto_out(*)
+     end function adc2g_cheby_auto
+
      integer(c_int32_t) function adc2g_precon_nparams() bind(c, name='adc2g_precon_nparams')
        import :: c_int32_t
      end function adc2g_precon_nparams
@@ -200,5 +210,22 @@ contains
     end do
     close(unit)
   end subroutine adc2g_write_auto
+
+  ! the 'chebyauto' file of write_chebyshev_auto (chebyspec.f90:323-355):
+  ! C_0 .. C_2K with the spectral bounds in the header
+  subroutine adc2g_write_chebyauto(fname, bounds, order, auto_out)
+    character(len=*), intent(in) :: fname
+    real(c_double), intent(in) :: bounds(2)
+    integer(c_int32_t), intent(in) :: order
+    real(c_double), intent(in) :: auto_out(0:2*order)
+    integer :: unit, k
+    open(newunit=unit, file=fname, form='formatted', status='unknown')
+    write(unit, '(a,2(2x,E21.14),/)') '#    Spectral bounds:', bounds(1), bounds(2)
+    write(unit, '(a)') '#    Order [k]    C_k'
+    do k = 0, 2 * order
+       write(unit, '(i6,11x,E21.14)') k, auto_out(k)
+    end do
+    close(unit)
+  end subroutine adc2g_write_chebyauto
 
 end module adc2g_mod

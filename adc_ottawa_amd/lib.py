@@ -70,7 +70,7 @@ SYMBOLS = [
     "adc2g_sigma", "adc2g_sigma_device", "adc2g_matxvec_treal", "adc2g_davidson",
     "adc2g_sil_autocorr", "adc2g_precon_harvest", "adc2g_precon_train", "adc2g_precon_nparams",
     "adc2g_precon_get", "adc2g_precon_set", "adc2g_precon_eval", "adc2g_last_error",
-    "adc2g_sigma_profile", "adc2g_precon_absorb", "adc2g_coupling_form",
+    "adc2g_sigma_profile", "adc2g_precon_absorb", "adc2g_coupling_form", "adc2g_cheby_auto",
 ]
 
 _lib = None
@@ -108,6 +108,7 @@ def load(path: str = LIBPATH):
     lib.adc2g_last_error.argtypes = [C.c_char_p, i32]
     lib.adc2g_sigma_profile.argtypes = [vp, i32, vp]
     lib.adc2g_coupling_form.argtypes = [vp, vp]
+    lib.adc2g_cheby_auto.argtypes = [vp, vp, vp, i32, i32, vp]
     lib.adc2g_precon_absorb.argtypes = [vp, vp, C.POINTER(i64)]
     for s in SYMBOLS:
         getattr(lib, s).restype = C.c_int32
@@ -222,6 +223,17 @@ class Operator:
                                       C.byref(n), C.byref(st)))
         stats = {f: getattr(st, f) for f, _ in SilStats._fields_}
         return out[: n.value], stats
+
+    def cheby_auto(self, q0, bounds, order, precision="fp64"):
+        """Chebyshev order-domain autocorrelation C_0..C_2K (K = order) of q0
+        with the spectrum scaled by bounds = (Emin, Emax)."""
+        q0 = np.ascontiguousarray(q0, dtype=np.float64)
+        if q0.size != self.ndim:
+            raise ValueError("q0 must have ndim entries")
+        b = np.ascontiguousarray(bounds, dtype=np.float64)
+        out = np.empty(2 * order + 1)
+        check(_lib.adc2g_cheby_auto(self._h, _ptr(b), _ptr(q0), int(order), PREC[precision], _ptr(out)))
+        return out
 
     def sigma_device(self, nvec, dX, dW, precision="fp64", stream=None):
         """W = H X on device pointers (ints), asynchronous on ``stream``."""

@@ -257,6 +257,51 @@ def sil_compare(op, sz, tfinal, autotol, seed=0):
     return out
 
 
+def lanczos_extremes(op, steps=30, seed=1):
+    """Extreme Ritz values of a short Lanczos run with the GPU sigma-vector."""
+    rng = np.random.default_rng(seed)
+    q = rng.standard_normal(op.ndim)
+    q /= np.linalg.norm(q)
+    qp, b = np.zeros(op.ndim), 0.0
+    alpha, beta = [], []
+    for _ in range(steps):
+        w = op.sigma(q) - b * qp
+        a = q @ w
+        w -= a * q
+        b = np.linalg.norm(w)
+        alpha.append(a)
+        beta.append(b)
+        qp, q = q, w / b
+    ev = np.linalg.eigvalsh(np.diag(alpha) + np.diag(beta[:-1], 1) + np.diag(beta[:-1], -1))
+    return float(ev[0]), float(ev[-1])
+
+
+def cheby_measure(op, sz, order, seed=0):
+    """Chebyshev order-domain autocorrelation (adc2g_cheby_auto) of the
+    synthetic dipole vector, bounds from a padded 30-step Lanczos run; FP64 and
+    FP32 sigma-vectors."""
+    from adc_ottawa_amd.synth import unit_dipole
+    lo, hi = lanczos_extremes(op)
+    pad = 0.05 * (hi - lo)
+    bounds = (lo - pad, hi + pad)
+    q0 = np.zeros(op.ndim)
+    q0[: op.n1] = unit_dipole(sz["nocc"], sz["nvirt"], seed=seed)
+    q0 /= np.linalg.norm(q0)
+    out, ref = {"bounds_Eh": bounds, "order": order}, None
+    for prec in ("fp64", "fp32"):
+        t0 = time.perf_counter()
+        auto = op.cheby_auto(q0, bounds, order, precision=prec)
+        wall = time.perf_counter() - t0
+        rec = {"seconds": round(wall, 3), "ms_per_order": round(wall / order * 1e3, 3)}
+        if ref is None:
+            ref = auto
+        else:
+            rec["max_abs_err_vs_fp64"] = float(np.abs(auto - ref).max())
+        out[prec] = rec
+        log(f"cheby {prec}: {wall:.2f}s for {order} orders ({2 * order + 1} autocorrelation values)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,6 +317,8 @@ def main():
     ap.add_argument("--sil", type=float, default=0.0,
                     help="also propagate with SIL to this final time (a.u.), FP64 and FP32 sigma")
     ap.add_argument("--sil-tol", type=float, default=1e-10, help="SIL error tolerance (autotol)")
+    ap.add_argument("--cheby", type=int, default=0,
+                    help="also run the Chebyshev order-domain autocorrelation to this order")
     ap.add_argument("--dav-roots", type=int, default=20, help="Davidson comparison: roots (block >= roots)")
     ap.add_argument("--dav-block", type=int, default=20, help="Davidson comparison: block size")
     ap.add_argument("--dav-maxvec", type=int, default=200, help="Davidson comparison: maximum subspace")
@@ -340,6 +387,7 @@ def main():
                                blocksize=args.dav_block, maxvec=args.dav_maxvec)
 
     sil = sil_compare(op, sz, args.sil, args.sil_tol) if args.sil > 0 else None
+    cheb = cheby_measure(op, sz, args.cheby) if args.cheby > 0 else None
 
     n1 = op.n1
     bytes_per_sigma = 8.0 * (2 * op.ndim) + 8.0 * 2 * n1 * n1  # read x, write sigma, pair tensors
@@ -365,6 +413,7 @@ def main():
         "detail": extra,
         "davidson_20_roots": dav,
         "sil_autocorrelation": sil,
+        "chebyshev_autocorrelation": cheb,
         "operator_build_seconds": round(t_build, 3),
     }
     op.close()

@@ -180,6 +180,20 @@ int32_t adc2g_sil_autocorr(adc2g_handle* h, const adc2g_sil_params* p, const dou
                            double* auto0, int64_t nrec_cap, int64_t* nrec_out,
                            adc2g_sil_stats* stats);
 
+/* Chebyshev order-domain autocorrelation function (chebyshev_auto,
+ * source/propagation/chebyspec.f90:245-319, with the scaled product of
+ * matxvec_chebyshev_incore, source/propagation/tdselib.f90:1316-1398):
+ * H_norm = (2/dE) H - (2 Emin/dE + 1), dE = Emax - Emin, bounds = {Emin, Emax}
+ * (the reference estimates them with spectral_bounds and widens them by
+ * 0.9 / 1.1, which assumes positive energies; here the caller passes bounds
+ * that contain the spectrum); q_1 = H_norm q0, q_k = 2 H_norm q_{k-1} - q_{k-2};
+ * auto_out[k] = q0.q_k for k <= order and, for k > order/2,
+ * auto_out[2k] = 2 q_k.q_k - auto_out[0], auto_out[2k-1] = 2 q_{k-1}.q_k - auto_out[1]:
+ * 2*order + 1 doubles.  q0: host vector of ndim.  precision: ADC2G_PREC_* of
+ * the sigma-vectors (recursion and dot products in FP64). */
+int32_t adc2g_cheby_auto(adc2g_handle* h, const double* bounds, const double* q0, int32_t order,
+                         int32_t precision, double* auto_out);
+
 /* ---- learned preconditioner -------------------------------------------- */
 
 /* Run FP64 Davidson on this handle's system with the reference DPR step and

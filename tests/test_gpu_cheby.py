@@ -1,0 +1,95 @@
+"""Chebyshev order-domain autocorrelation (adc2g_cheby_auto) against the oracle.
+
+Small systems: the GPU values against the oracle restatement of
+chebyshev_auto (oracle/cheby.py, pinned by the closed form in
+test_oracle_pins) and against the closed form itself, for both coupling
+forms; FP64 tolerance 1e-10 of C_0, FP32 sigma-vectors 1e-4 of C_0.  At the
+benzene size (ndim 5.1e6): the doubled orders of an order-K run (C_2k = 2 q_k.q_k
+- C_0) against the directly computed C_k of an order-2K run (1e-9 of C_0), and
+|C_k| <= C_0 (T_k is bounded by 1 when the bounds contain the spectrum).
+"""
+import numpy as np
+import pytest
+
+from adc_ottawa_amd import lib
+from adc_ottawa_amd.synth import make_system
+from oracle import cheby, elements
+
+pytestmark = pytest.mark.gpu
+
+FORMS = ("df", "int")
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("o,v,naux,seed", [(3, 8, 10, 5), (4, 12, 20, 6)])
+def test_cheby_small_vs_oracle(o, v, naux, seed, form):
+    print(f"[case] cheby o={o} v={v} naux={naux} coupling={form}", flush=True)
+    sys = make_system(o, v, naux, seed=seed)
+    H = elements.build_blocks(sys, apply_minc=False).dense()
+    lam = np.linalg.eigvalsh(H)
+    span = lam[-1] - lam[0]
+    bounds = (lam[0] - 0.02 * span, lam[-1] + 0.02 * span)
+    q0 = np.random.default_rng(seed).standard_normal(H.shape[0])
+    order = 30
+    ref = cheby.chebyshev_auto(lambda x: H @ x, q0, bounds, order)
+    exact = cheby.closed_form(H, q0, bounds, 2 * order + 1)
+    with lib.Operator.from_system(sys, coupling=form) as op:
+        got = op.cheby_auto(q0, bounds, order)
+        got32 = op.cheby_auto(q0, bounds, order, precision="fp32")
+    c0 = ref[0]
+    assert np.abs(got - ref).max() < 1e-10 * c0
+    assert np.abs(got - exact).max() < 1e-10 * c0
+    assert np.abs(got32 - ref).max() < 1e-4 * c0
+
+
+def test_cheby_bad_arguments():
+    sys = make_system(3, 8, 10, seed=5)
+    with lib.Operator.from_system(sys) as op:
+        q0 = np.ones(op.ndim)
+        with pytest.raises(lib.Adc2gError):
+            op.cheby_auto(q0, (1.0, 1.0), 4)
+        with pytest.raises(lib.Adc2gError):
+            op.cheby_auto(q0, (0.0, 1.0), 0)
+
+
+def _lanczos_extremes(op, n, steps, seed):
+    """Extreme Ritz values of a short Lanczos run on the GPU sigma (inside the
+    spectrum; the caller pads them)."""
+    rng = np.random.default_rng(seed)
+    q = rng.standard_normal(n)
+    q /= np.linalg.norm(q)
+    qp = np.zeros(n)
+    alpha, beta = [], []
+    b = 0.0
+    for _ in range(steps):
+        w = op.sigma(q) - b * qp
+        a = q @ w
+        w -= a * q
+        b = np.linalg.norm(w)
+        alpha.append(a)
+        beta.append(b)
+        qp, q = q, w / b
+    T = np.diag(alpha) + np.diag(beta[:-1], 1) + np.diag(beta[:-1], -1)
+    ev = np.linalg.eigvalsh(T)
+    return ev[0], ev[-1]
+
+
+def test_cheby_benzene_doubling_and_bound():
+    print("[case] cheby benzene-size o=16 v=200 naux=864", flush=True)
+    sys = make_system(16, 200, 864, seed=0)
+    with lib.Operator.from_system(sys) as op:
+        lo, hi = _lanczos_extremes(op, op.ndim, 30, seed=1)
+        pad = 0.05 * (hi - lo)
+        bounds = (lo - pad, hi + pad)
+        q0 = np.zeros(op.ndim)
+        q0[: op.n1] = np.random.default_rng(2).normal(0.0, 0.1, op.n1)
+        q0 /= np.linalg.norm(q0)
+        k = 16
+        a1 = op.cheby_auto(q0, bounds, k)
+        a2 = op.cheby_auto(q0, bounds, 2 * k)
+    c0 = a1[0]
+    assert abs(c0 - 1.0) < 1e-12
+    # orders k+1..2k: doubled in the first run, direct in the second
+    assert np.abs(a1[k + 1:] - a2[k + 1:2 * k + 1]).max() < 1e-9 * c0
+    assert np.abs(a1[: k + 1] - a2[: k + 1]).max() < 1e-12 * c0
+    assert np.abs(a2).max() <= c0 * (1.0 + 1e-9)

@@ -19,7 +19,7 @@ import pytest
 import scipy.sparse.linalg as sla
 
 from adc_ottawa_amd.synth import make_system
-from oracle import csf, davidson, dfsigma, elements, mlp, schur, sil
+from oracle import cheby, csf, davidson, dfsigma, elements, mlp, schur, sil
 
 
 def test_csf_counts_and_bijectivity():
@@ -137,6 +137,25 @@ def test_oracle_sil_exact_propagator():
     assert np.abs(auto - exact).max() < 1e-9 * np.sum(c * c)
     # t = 0 record is |F|^2
     assert abs(auto[0] - np.dot(f, f)) < 1e-14
+
+
+def test_oracle_chebyshev_closed_form():
+    """The Chebyshev recursion restatement against sum_n (v_n.q0)^2 T_k(x_n)
+    from a dense eigendecomposition, on every order including the doubled ones
+    (C_2k = 2 q_k.q_k - C_0); a wrong recursion coefficient, scaling or
+    doubling formula fails it."""
+    sys = make_system(3, 6, 9, seed=4)
+    blk = elements.build_blocks(sys, apply_minc=False)
+    H = blk.dense()
+    lam = np.linalg.eigvalsh(H)
+    bounds = (lam[0] - 0.05 * (lam[-1] - lam[0]), lam[-1] + 0.05 * (lam[-1] - lam[0]))
+    q0 = np.random.default_rng(3).standard_normal(H.shape[0])
+    for order in (1, 7, 20):
+        got = cheby.chebyshev_auto(lambda x: H @ x, q0, bounds, order)
+        ref = cheby.closed_form(H, q0, bounds, 2 * order + 1)
+        assert np.abs(got - ref).max() < 1e-10 * np.abs(ref[0])
+    with pytest.raises(ValueError):
+        cheby.closed_form(H, q0, (lam[0], 0.5 * (lam[0] + lam[-1])), 3)
 
 
 def test_oracle_mlp_gradients():
