@@ -45,9 +45,10 @@ void launch_unit_vectors(int nc, const int64_t* idx, double* V, int64_t ld, hipS
 void launch_orth_coeffs(int r, const double* G, int ldg, double* M, int ldm, double* work, int* kept,
                         double tol, hipStream_t st, double* cond = nullptr);
 
-// Block Gram-Schmidt "twice is enough" test (Kahan-Parlett / DGKS): after one
+// Block Gram-Schmidt re-orthogonalisation test (DGKS form): after one
 // projection T <- T - V1 S1 - V2 S2 of an orthonormal [V1 V2], need[0] = 1 if
-// some column j kept less than half its squared norm (G = T^T T afterwards).
+// some column j kept less than 1e-6 of its squared norm (G = T^T T
+// afterwards); above that, one pass leaves it orthogonal to ~1e-13.
 void launch_dgks_check(int r, int nb1, const double* S1, int ld1, int nb2, const double* S2, int ld2, const double* G,
                        int ldg, int* need, hipStream_t st);
 
