@@ -226,6 +226,16 @@ int32_t adc2g_cheby_auto(adc2g_handle* h, const double* bounds, const double* q0
   });
 }
 
+int32_t adc2g_block_lanczos(adc2g_handle* h, const adc2g_lanczos_params* p, const double* q0,
+                            double* eigval, double* tmat, const double* travec, double* tranmom,
+                            double* ritz, adc2g_lanczos_stats* stats) {
+  return guarded([&] {
+    ARG_CHECK(h && p && q0 && eigval, "NULL argument");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    lanczos_run(h->h, *p, q0, eigval, tmat, travec, tranmom, ritz, stats);
+  });
+}
+
 int32_t adc2g_sigma_profile(adc2g_handle* h, int32_t enable, double* out) {
   return guarded([&] {
     ARG_CHECK(h != nullptr, "NULL handle");

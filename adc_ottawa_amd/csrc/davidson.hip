@@ -44,8 +44,6 @@
 
 namespace adc2g {
 
-namespace {
-
 void gemm_tn(Handle& H, int M, int N, int64_t K, const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
              int64_t ldc, double alpha, double beta) {
   if (K > 4096 && skinny_tn(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, H.skinny_ws, H.st)) return;
@@ -67,6 +65,8 @@ void gemm_nn(Handle& H, int64_t M, int N, int K, const double* A, int64_t lda, c
   g.map.C = C; g.map.scm = 1; g.map.scn = ldc; g.map.alpha = alpha; g.map.beta = beta;
   gemm(g, Prec::F64, H.gws, H.st);
 }
+
+namespace {
 
 struct Dav {
   Handle& H;

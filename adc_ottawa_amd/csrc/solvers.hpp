@@ -6,6 +6,15 @@
 
 namespace adc2g {
 
+// FP64 products on the handle's stream: C = alpha A^T B + beta C (A: K x M,
+// B: K x N) and C = alpha A B + beta C (A: M x K, B: K x N); the tall-skinny
+// kernels when K (tn) or M (nn) is large and N <= 64, the general GEMM
+// otherwise.  C must not alias A or B.
+void gemm_tn(Handle& H, int M, int N, int64_t K, const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
+             int64_t ldc, double alpha, double beta);
+void gemm_nn(Handle& H, int64_t M, int N, int K, const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
+             int64_t ldc, double alpha, double beta);
+
 // v2 = -i H v1 for one complex vector (host arrays, interleaved re/im).
 void matxvec_treal_host(Handle& H, const double* v1, double* v2, Prec p);
 
@@ -19,6 +28,11 @@ void sil_run(Handle& H, const adc2g_sil_params& p, const double* fvec, double* a
 // Chebyshev order-domain autocorrelation C_0..C_2K (auto_out, host) of the
 // host vector q0 with the spectrum scaled by bounds[0..1] = (Emin, Emax).
 void cheby_run(Handle& H, const double bounds[2], const double* q0, int order, Prec p, double* auto_out);
+
+// Block Lanczos pseudospectrum (orthotype 0) from the host block q0
+// (ndim x blocksize); outputs as adc2g_block_lanczos documents them.
+void lanczos_run(Handle& H, const adc2g_lanczos_params& p, const double* q0, double* eigval, double* tmat,
+                 const double* travec, double* tranmom, double* ritz, adc2g_lanczos_stats* stats);
 
 int precon_nparams();
 void precon_harvest(Handle& H, const adc2g_dav_params& p, int64_t max_records, int64_t* added);

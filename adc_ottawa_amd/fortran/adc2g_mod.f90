@@ -68,6 +68,20 @@ module adc2g_mod
      real(c_double)     :: time_total
   end type adc2g_sil_stats
 
+  ! block Lanczos (lancdiag_block, source/eigen/block_lanczos.f90:28-81):
+  ! blocksize = lmain, ncycles = iter of the lanczos section
+  type, bind(c) :: adc2g_lanczos_params
+     integer(c_int32_t) :: blocksize = 0
+     integer(c_int32_t) :: ncycles = 0
+     integer(c_int32_t) :: precision = 0
+     integer(c_int32_t) :: check_orth = 0
+  end type adc2g_lanczos_params
+
+  type, bind(c) :: adc2g_lanczos_stats
+     integer(c_int32_t) :: nvec, nsigma
+     real(c_double)     :: time_sigma, time_tdiag, time_ritz, time_total, orth_loss
+  end type adc2g_lanczos_stats
+
   interface
      integer(c_int32_t) function adc2g_create(nocc, nvirt, naux, eps, B, opts, h) bind(c, name='adc2g_create')
        import :: c_int32_t, c_double, c_ptr, adc2g_opts
@@ -146,6 +160,18 @@ module adc2g_mod
        real(c_double), intent(out) :: auto_out(*)
      end function adc2g_cheby_auto
 
+     ! tmat, travec, tranmom, ritz: c_loc of the array or c_null_ptr
+     integer(c_int32_t) function adc2g_block_lanczos(h, p, q0, eigval, tmat, travec, tranmom, &
+          ritz, stats) bind(c, name='adc2g_block_lanczos')
+       import :: c_int32_t, c_double, c_ptr, adc2g_lanczos_params, adc2g_lanczos_stats
+       type(c_ptr), value :: h
+       type(adc2g_lanczos_params), intent(in) :: p
+       real(c_double), intent(in) :: q0(*)
+       real(c_double), intent(out) :: eigval(*)
+       type(c_ptr), value :: tmat, travec, tranmom, ritz
+       type(adc2g_lanczos_stats), intent(out) :: stats
+     end function adc2g_block_lanczos
+
      integer(c_int32_t) function adc2g_precon_nparams() bind(c, name='adc2g_precon_nparams')
        import :: c_int32_t
      end function adc2g_precon_nparams
@@ -179,7 +205,9 @@ contains
   end function adc2g_errmsg
 
   ! Eigenpair file in the record layout of wreigenpairs: one unformatted
-  ! sequential record (i, E_i, vec_i(1:ndim)) per state.
+  ! sequential record (i, E_i, vec_i(1:ndim)) per state.  The Lanczos-state
+  ! file of ritzvecs_incore (source/eigen/block_lanczos.f90:2297-2299, lancname)
+  ! has the same records: call it with the Ritz values and vectors.
   subroutine adc2g_write_davstates(fname, ndim, nstates, eigval, eigvec)
     character(len=*), intent(in) :: fname
     integer(c_int64_t), intent(in) :: ndim

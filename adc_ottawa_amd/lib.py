@@ -20,7 +20,7 @@ ERRORS = {1: "SIL illegal integration order", 2: "SIL stepsize underflow",
           3: "SIL diagonalisation failed", 4: "SIL illegal initial stepsize",
           10: "bad argument", 11: "HIP failure", 12: "not all vectors have converged",
           13: "Rayleigh-Ritz eigensolve failed", 14: "device allocation failed",
-          15: "call out of order"}
+          15: "call out of order", 16: "block Lanczos breakdown"}
 
 PREC = {"fp64": 0, "fp32": 1, "bf16": 2}
 PRECON = {"dpr": 1, "olsen": 2, "learned": 3}
@@ -64,6 +64,17 @@ class SilStats(C.Structure):
                 ("reserved", C.c_int32), ("time_total", C.c_double)]
 
 
+class LanczosParams(C.Structure):
+    _fields_ = [("blocksize", C.c_int32), ("ncycles", C.c_int32), ("precision", C.c_int32),
+                ("check_orth", C.c_int32)]
+
+
+class LanczosStats(C.Structure):
+    _fields_ = [("nvec", C.c_int32), ("nsigma", C.c_int32), ("time_sigma", C.c_double),
+                ("time_tdiag", C.c_double), ("time_ritz", C.c_double), ("time_total", C.c_double),
+                ("orth_loss", C.c_double)]
+
+
 # every symbol declared in include/adc2g.h
 SYMBOLS = [
     "adc2g_create", "adc2g_destroy", "adc2g_dims", "adc2g_diagonal", "adc2g_phph",
@@ -71,6 +82,7 @@ SYMBOLS = [
     "adc2g_sil_autocorr", "adc2g_precon_harvest", "adc2g_precon_train", "adc2g_precon_nparams",
     "adc2g_precon_get", "adc2g_precon_set", "adc2g_precon_eval", "adc2g_last_error",
     "adc2g_sigma_profile", "adc2g_precon_absorb", "adc2g_coupling_form", "adc2g_cheby_auto",
+    "adc2g_block_lanczos",
 ]
 
 _lib = None
@@ -110,6 +122,8 @@ def load(path: str = LIBPATH):
     lib.adc2g_coupling_form.argtypes = [vp, vp]
     lib.adc2g_cheby_auto.argtypes = [vp, vp, vp, i32, i32, vp]
     lib.adc2g_precon_absorb.argtypes = [vp, vp, C.POINTER(i64)]
+    lib.adc2g_block_lanczos.argtypes = [vp, C.POINTER(LanczosParams), vp, vp, vp, vp, vp, vp,
+                                        C.POINTER(LanczosStats)]
     for s in SYMBOLS:
         getattr(lib, s).restype = C.c_int32
     _lib = lib
@@ -234,6 +248,35 @@ class Operator:
         out = np.empty(2 * order + 1)
         check(_lib.adc2g_cheby_auto(self._h, _ptr(b), _ptr(q0), int(order), PREC[precision], _ptr(out)))
         return out
+
+    def block_lanczos(self, q0, ncycles, precision="fp64", travec=None, tmat=False, ritz=False,
+                      check_orth=False):
+        """Block Lanczos pseudospectrum from the orthonormal block q0 (ndim x b).
+
+        Returns (eigval, extras, stats); extras holds "tranmom" (travec given),
+        "tmat" and "ritz" (ndim x nvec) when asked for."""
+        q0 = np.asfortranarray(np.asarray(q0, dtype=np.float64).reshape(self.ndim, -1))
+        b = q0.shape[1]
+        nvec = b * int(ncycles)
+        p = LanczosParams(b, int(ncycles), PREC[precision], 1 if check_orth else 0)
+        ev = np.empty(nvec)
+        extras = {}
+        tm_out = f = None
+        if travec is not None:
+            f = np.ascontiguousarray(travec, dtype=np.float64)
+            if f.size != self.ndim:
+                raise ValueError("travec must have ndim entries")
+            tm_out = extras["tranmom"] = np.empty(nvec)
+        t_out = extras["tmat"] = np.empty((nvec, nvec), order="F") if tmat else None
+        r_out = extras["ritz"] = np.empty((self.ndim, nvec), order="F") if ritz else None
+        st = LanczosStats()
+        check(_lib.adc2g_block_lanczos(
+            self._h, C.byref(p), _ptr(q0), _ptr(ev), _ptr(t_out) if tmat else None,
+            _ptr(f) if f is not None else None, _ptr(tm_out) if f is not None else None,
+            _ptr(r_out) if ritz else None, C.byref(st)))
+        extras = {k: v for k, v in extras.items() if v is not None}
+        stats = {k: getattr(st, k) for k, _ in LanczosStats._fields_}
+        return ev, extras, stats
 
     def sigma_device(self, nvec, dX, dW, precision="fp64", stream=None):
         """W = H X on device pointers (ints), asynchronous on ``stream``."""

@@ -302,6 +302,37 @@ def cheby_measure(op, sz, order, seed=0):
     return out
 
 
+def lanczos_measure(op, sz, spec, seed=0):
+    """Block Lanczos pseudospectrum (adc2g_block_lanczos): blocksize x ncycles
+    from unit vectors on the 1h1p configurations with the largest synthetic
+    transition dipoles (init_vec_unit), transition moments with that dipole
+    vector; FP64 and FP32 sigma-vectors."""
+    from adc_ottawa_amd.synth import unit_dipole
+    b, m = (int(x) for x in spec.split(","))
+    mu = np.zeros(op.ndim)
+    mu[: op.n1] = unit_dipole(sz["nocc"], sz["nvirt"], seed=seed)
+    idx = np.argsort(-np.abs(mu[: op.n1]), kind="stable")[:b]
+    q0 = np.zeros((op.ndim, b), order="F")
+    q0[idx, np.arange(b)] = 1.0
+    out, ref = {"blocksize": b, "ncycles": m, "nvec": b * m}, None
+    for prec in ("fp64", "fp32"):
+        t0 = time.perf_counter()
+        ev, ex, st = op.block_lanczos(q0, m, precision=prec, travec=mu, check_orth=(prec == "fp64"))
+        wall = time.perf_counter() - t0
+        rec = {"seconds": round(wall, 3), "sigma_seconds": round(st["time_sigma"], 3),
+               "tdiag_seconds": round(st["time_tdiag"], 3), "ritz_seconds": round(st["time_ritz"], 3),
+               "lowest_ritz_Eh": [float(x) for x in ev[:3]]}
+        if prec == "fp64":
+            rec["orth_loss"] = float(st["orth_loss"])
+            ref = ev
+        else:
+            rec["max_abs_ritz_diff_vs_fp64_lowest20"] = float(np.abs(ev[:20] - ref[:20]).max())
+        out[prec] = rec
+        log(f"lanczos {prec}: {wall:.2f}s for {b}x{m} (sigma {st['time_sigma']:.2f}s, "
+            f"T eig {st['time_tdiag']:.2f}s, Ritz {st['time_ritz']:.2f}s)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -319,6 +350,8 @@ def main():
     ap.add_argument("--sil-tol", type=float, default=1e-10, help="SIL error tolerance (autotol)")
     ap.add_argument("--cheby", type=int, default=0,
                     help="also run the Chebyshev order-domain autocorrelation to this order")
+    ap.add_argument("--lanczos", default="",
+                    help="also run the block Lanczos pseudospectrum, 'blocksize,ncycles' (e.g. 50,20)")
     ap.add_argument("--dav-roots", type=int, default=20, help="Davidson comparison: roots (block >= roots)")
     ap.add_argument("--dav-block", type=int, default=20, help="Davidson comparison: block size")
     ap.add_argument("--dav-maxvec", type=int, default=200, help="Davidson comparison: maximum subspace")
@@ -388,6 +421,7 @@ def main():
 
     sil = sil_compare(op, sz, args.sil, args.sil_tol) if args.sil > 0 else None
     cheb = cheby_measure(op, sz, args.cheby) if args.cheby > 0 else None
+    lanc = lanczos_measure(op, sz, args.lanczos) if args.lanczos else None
 
     n1 = op.n1
     bytes_per_sigma = 8.0 * (2 * op.ndim) + 8.0 * 2 * n1 * n1  # read x, write sigma, pair tensors
@@ -414,6 +448,7 @@ def main():
         "davidson_20_roots": dav,
         "sil_autocorrelation": sil,
         "chebyshev_autocorrelation": cheb,
+        "block_lanczos": lanc,
         "operator_build_seconds": round(t_build, 3),
     }
     op.close()

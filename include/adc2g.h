@@ -46,6 +46,7 @@ extern "C" {
 #define ADC2G_EEIG 13           /* Rayleigh-Ritz eigensolve failed */
 #define ADC2G_ENOMEM 14         /* device allocation failed */
 #define ADC2G_ESTATE 15         /* call out of order (e.g. no trained network) */
+#define ADC2G_EBREAK 16         /* block Lanczos: the residual block lost rank */
 
 /* arithmetic of the 1h1p<->2h2p coupling contractions */
 #define ADC2G_PREC_FP64 0       /* FP64 MFMA */
@@ -115,6 +116,23 @@ typedef struct {
   int32_t reserved;
   double time_total;
 } adc2g_sil_stats;
+
+typedef struct {
+  int32_t blocksize;     /* lmain: vectors per Lanczos block */
+  int32_t ncycles;       /* iter: blocks generated; nvec = blocksize * ncycles */
+  int32_t precision;     /* ADC2G_PREC_* of the sigma-vectors */
+  int32_t check_orth;    /* 1: stats->orth_loss = max |L^T L - I| over the Lanczos vectors */
+} adc2g_lanczos_params;
+
+typedef struct {
+  int32_t nvec;          /* Lanczos states = blocksize * ncycles */
+  int32_t nsigma;        /* sigma-vectors applied */
+  double time_sigma;     /* s in the sigma-vectors (device events) */
+  double time_tdiag;     /* s in the eigen-decomposition of T (host) */
+  double time_ritz;      /* s forming the Ritz vectors and transition moments */
+  double time_total;     /* s, whole call */
+  double orth_loss;      /* max |L^T L - I| when check_orth, else -1 */
+} adc2g_lanczos_stats;
 
 /* ---- lifetime ---------------------------------------------------------- */
 
@@ -193,6 +211,35 @@ int32_t adc2g_sil_autocorr(adc2g_handle* h, const adc2g_sil_params* p, const dou
  * the sigma-vectors (recursion and dot products in FP64). */
 int32_t adc2g_cheby_auto(adc2g_handle* h, const double* bounds, const double* q0, int32_t order,
                          int32_t precision, double* auto_out);
+
+/* Block Lanczos pseudospectrum: run_block_lanczos
+ * (source/eigen/block_lanczos.f90:947-1256) in its default form, orthotype 0
+ * (no re-orthogonalisation), with hxq_incore (:1325-1361) replaced by the
+ * sigma-vector of the block; the partial re-orthogonalisation variants
+ * (orthotype 1, 2: pro, mpro, :1474-1845) are not provided.
+ *   U = H Q_j - Q_{j-1} B_{j-1}^T,  A_j = Q_j^T U,  R = U - Q_j A_j = Q_{j+1} B_j
+ * with B_j upper triangular.  The reference factors R with dgeqrf/dorgqr; here a
+ * shifted Cholesky QR with two refinement passes gives the factor with a
+ * positive diagonal, so Q_{j+1} and B_j differ from the reference's by the
+ * signs of their columns and rows.  T is then D T_ref D for a diagonal D of
+ * +-1: the same eigenvalues, Ritz vectors and |transition moments|.
+ * q0: host block (ndim x blocksize) of orthonormal columns (the initial
+ * vectors of init_vec, :128-197; rejected if max |q0^T q0 - I| > 1e-8).
+ * Outputs (host; every one but eigval may be NULL):
+ *   eigval[nvec]     eigenvalues of T ascending (dsbev, diagmat_banded :2006-2052);
+ *   tmat[nvec*nvec]  T, block tridiagonal, diagonal blocks from the upper
+ *                    triangle of A_j (the fill order of :1194-1217);
+ *   tranmom[nvec]    travec . r_n for the normalised Ritz vectors r_n (the
+ *                    products get_tranmom_1 forms, source/adclib/fspacetrial.f90:1076-1156;
+ *                    travec and tranmom are both NULL or both given);
+ *   ritz[ndim*nvec]  the normalised Ritz vectors r_n = L y_n / |L y_n|
+ *                    (ritzvecs_incore :2237-2300), in the order of eigval.
+ * The Lanczos vectors stay in HBM (8 ndim nvec bytes); the eigen-decomposition
+ * of T runs on the host, as dsbev does in the reference.  Returns ADC2G_EBREAK
+ * when a residual block is rank deficient (the Krylov space is exhausted). */
+int32_t adc2g_block_lanczos(adc2g_handle* h, const adc2g_lanczos_params* p, const double* q0,
+                            double* eigval, double* tmat, const double* travec, double* tranmom,
+                            double* ritz, adc2g_lanczos_stats* stats);
 
 /* ---- learned preconditioner -------------------------------------------- */
 

@@ -1,0 +1,109 @@
+"""Block Lanczos pseudospectrum (adc2g_block_lanczos) against the oracle.
+
+Small systems, both coupling forms: eigenvalues of T, |T| elementwise (the
+GPU factor has a positive diagonal where the reference's Householder QR
+takes signs, so T differs by a diagonal +-1 similarity), |transition
+moments| and the Ritz vectors up to sign against oracle/lanczos.py (pinned
+in test_oracle_pins); FP64 tolerances 1e-10 relative for T and its
+eigenvalues, 1e-8 for moments and vectors; FP32 sigma-vectors 1e-5 for the
+eigenvalues.  At the benzene size (ndim 5.1e6, 20 x 5 Lanczos vectors):
+orthonormality of the Lanczos vectors, the band structure of T and exact
+block Gauss quadrature sum_n (f.r_n)^2 theta_n^k = f^T H^k f (k <= 3) with
+f^T H^k f from repeated FP64 sigma-vectors.
+"""
+import numpy as np
+import pytest
+
+from adc_ottawa_amd import lib
+from adc_ottawa_amd.synth import make_system
+from oracle import elements, lanczos
+
+pytestmark = pytest.mark.gpu
+
+FORMS = ("df", "int")
+
+
+def _start_block(n, n1, b, seed):
+    """Unit vectors on the b 1h1p configurations with the largest |mu|
+    (init_vec_unit with stvc_lbl chosen by the transition dipoles) and a
+    synthetic dipole vector mu on the 1h1p block."""
+    mu = np.zeros(n)
+    mu[:n1] = np.random.default_rng(seed).normal(0.0, 0.1, n1)
+    idx = np.argsort(-np.abs(mu[:n1]), kind="stable")[:b]
+    q0 = np.zeros((n, b))
+    q0[idx, np.arange(b)] = 1.0
+    return q0, mu
+
+
+def _match_up_to_sign(a, b):
+    d = np.minimum(np.abs(a - b).max(axis=0), np.abs(a + b).max(axis=0))
+    return d.max()
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("o,v,naux,seed,b,m", [(3, 8, 10, 5, 3, 6), (4, 12, 20, 6, 4, 6)])
+def test_lanczos_small_vs_oracle(o, v, naux, seed, b, m, form):
+    print(f"[case] lanczos o={o} v={v} naux={naux} b={b} m={m} coupling={form}", flush=True)
+    sys = make_system(o, v, naux, seed=seed)
+    H = elements.build_blocks(sys, apply_minc=False).dense()
+    n = H.shape[0]
+    n1 = o * v
+    q0, mu = _start_block(n, n1, b, seed)
+    ref = lanczos.block_lanczos(lambda X: H @ X, q0, m)
+    tm_ref = lanczos.tranmom(ref["ritz"], mu)
+    with lib.Operator.from_system(sys, coupling=form) as op:
+        ev, ex, st = op.block_lanczos(q0, m, travec=mu, tmat=True, ritz=True, check_orth=True)
+        ev32, _, _ = op.block_lanczos(q0, m, precision="fp32")
+    scale = np.abs(ref["eigval"]).max()
+    assert st["nvec"] == b * m and st["nsigma"] == b * m
+    assert st["orth_loss"] < 1e-10
+    assert np.abs(ev - ref["eigval"]).max() < 1e-10 * scale
+    assert np.abs(np.abs(ex["tmat"]) - np.abs(ref["T"])).max() < 1e-10 * scale
+    assert np.abs(np.abs(ex["tranmom"]) - np.abs(tm_ref)).max() < 1e-8 * np.linalg.norm(mu)
+    assert _match_up_to_sign(ex["ritz"], ref["ritz"]) < 1e-8
+    assert np.abs(ev32 - ref["eigval"]).max() < 1e-5 * scale
+
+
+def test_lanczos_bad_arguments():
+    sys = make_system(3, 8, 10, seed=5)
+    with lib.Operator.from_system(sys) as op:
+        n = op.ndim
+        q0, _ = _start_block(n, op.n1, 2, 0)
+        with pytest.raises(lib.Adc2gError):
+            op.block_lanczos(2.0 * q0, 3)          # not orthonormal
+        with pytest.raises(lib.Adc2gError):
+            op.block_lanczos(q0, 0)                # no cycles
+        with pytest.raises(lib.Adc2gError):
+            op.block_lanczos(q0, n // 2 + 1)       # more vectors than ndim
+        # the handle is still usable after the failures
+        ev, _, _ = op.block_lanczos(q0, 3)
+        assert np.all(np.isfinite(ev))
+
+
+def test_lanczos_benzene_quadrature():
+    print("[case] lanczos benzene-size o=16 v=200 naux=864 b=20 m=5", flush=True)
+    sys = make_system(16, 200, 864, seed=0)
+    b, m = 20, 5
+    with lib.Operator.from_system(sys) as op:
+        n = op.ndim
+        q0, _ = _start_block(n, op.n1, b, 7)
+        c = np.random.default_rng(8).normal(size=b)
+        f = q0 @ c
+        ev, ex, st = op.block_lanczos(q0, m, travec=f, tmat=True, check_orth=True)
+        hk = f.copy()
+        moments = []
+        for _ in range(4):
+            moments.append(f @ hk)
+            hk = op.sigma(hk)
+    print(f"[lanczos benzene] orth_loss={st['orth_loss']:.2e} sigma {st['time_sigma']:.3f}s "
+          f"T-eig {st['time_tdiag']:.3f}s total {st['time_total']:.3f}s", flush=True)
+    assert st["orth_loss"] < 1e-9
+    T = ex["tmat"]
+    nv = b * m
+    i, j = np.indices((nv, nv))
+    assert np.abs(T[np.abs(i - j) > b]).max() == 0.0
+    assert np.abs(T - T.T).max() == 0.0
+    tm = ex["tranmom"]
+    for k, ref in enumerate(moments):
+        got = (tm ** 2 * ev ** k).sum()
+        assert abs(got - ref) < 1e-9 * (tm ** 2 * np.abs(ev) ** k).sum(), k

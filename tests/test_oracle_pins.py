@@ -12,6 +12,9 @@ in ``oracle/`` is checked here against an independent fact:
   density-fitted reformulation (``dfsigma``) agree element by element;
 * Davidson: eigenvalues match ARPACK and the Lowdin-partitioned roots;
 * SIL: the autocorrelation matches exp(-iHt) from a dense eigendecomposition;
+* Chebyshev: the order-domain autocorrelation matches sum_n (v_n.q0)^2 T_k(x_n);
+* block Lanczos: the three-term recurrence, orthonormality, exact block Gauss
+  quadrature and the full-Krylov spectrum;
 * network: analytic gradients match central finite differences.
 """
 import numpy as np
@@ -19,7 +22,7 @@ import pytest
 import scipy.sparse.linalg as sla
 
 from adc_ottawa_amd.synth import make_system
-from oracle import cheby, csf, davidson, dfsigma, elements, mlp, schur, sil
+from oracle import cheby, csf, davidson, dfsigma, elements, lanczos, mlp, schur, sil
 
 
 def test_csf_counts_and_bijectivity():
@@ -156,6 +159,59 @@ def test_oracle_chebyshev_closed_form():
         assert np.abs(got - ref).max() < 1e-10 * np.abs(ref[0])
     with pytest.raises(ValueError):
         cheby.closed_form(H, q0, (lam[0], 0.5 * (lam[0] + lam[-1])), 3)
+
+
+def _unit_block(n, idx):
+    q = np.zeros((n, len(idx)))
+    q[idx, np.arange(len(idx))] = 1.0
+    return q
+
+
+def test_oracle_block_lanczos_recurrence_and_quadrature():
+    """The block Lanczos restatement against what the mathematics fixes:
+    H L = L T + Q_{m+1} B_m E_m^T to rounding (a transposed B_{j-1}, a wrong
+    sign or a wrong diagonal block breaks it); L^T L = I over a few cycles;
+    exact block Gauss quadrature sum_n (f.r_n)^2 theta_n^k = f^T H^k f for
+    f in span(Q_1) and k < 2m (a wrong Ritz normalisation or a wrong
+    eigenvector breaks it)."""
+    sys = make_system(4, 12, 20, seed=6)
+    H = elements.build_blocks(sys, apply_minc=False).dense()
+    n = H.shape[0]
+    b, m = 4, 3
+    q0 = _unit_block(n, [0, 5, 17, 30])
+    out = lanczos.block_lanczos(lambda X: H @ X, q0, m)
+    L, T = out["L"], out["T"]
+    E = np.zeros((b * m, b))
+    E[-b:, :] = np.eye(b)
+    resid = H @ L - L @ T - out["Qnext"] @ out["Bnext"] @ E.T
+    hn = np.abs(H).max()
+    assert np.abs(resid).max() < 1e-11 * hn * n ** 0.5
+    assert np.abs(L.T @ L - np.eye(b * m)).max() < 1e-11
+    assert np.allclose(np.tril(out["Bnext"], -1), 0.0)
+    f = q0 @ np.array([0.3, -1.1, 0.7, 0.2])
+    tm = lanczos.tranmom(out["ritz"], f)
+    th = out["eigval"]
+    hk = f.copy()
+    for k in range(2 * m):
+        lhs = (tm ** 2 * th ** k).sum()
+        rhs = f @ hk
+        assert abs(lhs - rhs) < 1e-9 * (np.abs(tm) ** 2 * np.abs(th) ** k).sum()
+        hk = H @ hk
+
+
+def test_oracle_block_lanczos_full_krylov():
+    """With blocksize * ncycles = ndim the Krylov space is the whole space and
+    the spectrum of T is the spectrum of H."""
+    rng = np.random.default_rng(2)
+    n, b = 24, 3
+    G = rng.standard_normal((n, n))
+    H = G + G.T
+    q0, _ = np.linalg.qr(rng.standard_normal((n, b)))
+    out = lanczos.block_lanczos(lambda X: H @ X, q0, n // b)
+    assert np.abs(out["eigval"] - np.linalg.eigvalsh(H)).max() < 1e-9 * np.abs(H).max()
+    # the Ritz vectors are then eigenvectors of H
+    R = out["ritz"]
+    assert np.abs(H @ R - R * out["eigval"]).max() < 1e-8 * np.abs(H).max()
 
 
 def test_oracle_mlp_gradients():
