@@ -1,0 +1,186 @@
+"""Imaginary-time relaxation with the XSIL (Lanczos-Liu) integrator.  Test oracle only.
+
+Restates xsil_rlx (source/propagation/relaxation.f90:139-309) with its helpers.
+States s = 1..nstates are relaxed one after another from the initial vectors.
+Each timestep of a state does the following.
+
+  * collapse_subspace (:1163-1203): when the variational subspace is full,
+    drop the oldest timestep's krydim + 1 vectors and its rows/columns.
+  * get_lancvecs_current (:1207-1326): the first vector is psi,
+    orthogonalised against the converged states 1..s-1 one after another and
+    normalised.  Then krydim Lanczos steps follow
+        r = H q - beta_{j-1} v,  alpha_j = q.r,  r -= alpha_j q,  beta_j = |r|
+    and the (krydim+1)-th vector r / beta_krydim, which only enters the
+    matrix elements.  Within the block H is tridiagonal (alpha, beta) and S = I.
+  * subspace_matrices (:1330-1381): between a vector i of an earlier timestep
+    and a vector j of the current one,
+        S_ij = l_i.l_j,
+        H_ij = beta_{k-1} l_i.l_{j-1} + alpha_k l_i.l_j + beta_k l_i.l_{j+1},
+    with k the position of j in its block and no beta_0 term.
+  * solve_subspace_tdse (:1385-1491) and canonical_ortho (:1495-1606): S is
+    diagonalised and eigenvalues |sigma| <= 1e-6 are dropped (X = U sigma^-1/2).
+    H' = X^T H X is diagonalised, and c' = exp(-H' dtau) X^+ e_first with
+    X^+ = sigma^1/2 U^T is normalised.  psi = sum_i (X c')_i l_i over the
+    non-extra vectors.
+  * Then psi is normalised.  The reference's loop before that subtracts
+    (psi.vec_conv(:,s)) vec_conv(:,s); vec_conv(:,s) is zero until state s
+    converges, so the loop changes nothing.
+  * residual_1vec (:1610-1651): E = psi.H psi, residual = |H psi - E psi|.
+    Convergence when residual <= eps.
+
+Pinned in tests/test_oracle_pins.py by the following checks.
+  * Each of the first timesteps propagates psi to exp(-H dtau) psi /
+    |exp(-H dtau) psi| (scipy.linalg.expm), to the Krylov truncation error.
+  * The subspace matrices equal L^T H L and L^T L.
+  * The relaxed energies and vectors are the lowest eigenpairs of a dense
+    eigendecomposition.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def _canonical(S, Hs, first, tol=1e-6):
+    sig, U = np.linalg.eigh(S)
+    keep = np.abs(sig) > tol
+    nnull = int((~keep).sum())
+    # the reference counts |sigma| > tol and takes the last nlin columns
+    nlin = int(keep.sum())
+    U = U[:, nnull:]
+    s = sig[nnull:]
+    X = U / np.sqrt(s)
+    Xinv = (U * np.sqrt(s)).T
+    H1 = X.T @ Hs @ X
+    return X, H1, Xinv[:, first], nlin
+
+
+def relax(matvec, guess: np.ndarray, krydim=10, maxdim=-1, maxiter=100, step=10.0, eps=1e-7,
+          trace=False):
+    """``matvec(x)`` returns H x; guess: (n, nstates) initial vectors.
+
+    Returns energies, vectors (n, nstates), residuals, iteration counts and,
+    with ``trace``, per state the list of (energy, residual) of every step and
+    the propagated vectors."""
+    guess = np.asarray(guess, dtype=np.float64)
+    n, nstates = guess.shape
+    if maxdim < 0:
+        maxdim = 2 * krydim
+    if maxdim % krydim != 0:
+        raise ValueError("maxvec must be an integer multiple of krydim")
+    maxvec = maxdim + maxdim // krydim
+    nsteps_cap = maxvec // (krydim + 1)
+    ener = np.zeros(nstates)
+    vecs = np.zeros((n, nstates))
+    resid = np.zeros(nstates)
+    iters = np.zeros(nstates, dtype=int)
+    conv = []
+    traces = []
+    for s in range(nstates):
+        lanc = np.zeros((n, maxvec))
+        subh = np.zeros((maxvec, maxvec))
+        subs = np.zeros((maxvec, maxvec))
+        alph = np.zeros((krydim, nsteps_cap))
+        beth = np.zeros((krydim, nsteps_cap))
+        vec0 = guess[:, s].copy()
+        ncurr = 0
+        hpsi = matvec(vec0)
+        e = vec0 @ hpsi
+        res = np.linalg.norm(hpsi - e * vec0)
+        tr = {"steps": [(e, res)], "psi": []}
+        converged = False
+        for it in range(1, maxiter + 1):
+            if ncurr * (krydim + 1) == maxvec:
+                kk = krydim + 1
+                lanc[:, : maxvec - kk] = lanc[:, kk:].copy()
+                subs[: maxvec - krydim, : maxvec - krydim] = subs[krydim:, krydim:].copy()
+                subh[: maxvec - krydim, : maxvec - krydim] = subh[krydim:, krydim:].copy()
+                alph[:, : nsteps_cap - 1] = alph[:, 1:].copy()
+                beth[:, : nsteps_cap - 1] = beth[:, 1:].copy()
+                ncurr -= 1
+            ncurr += 1
+            # get_lancvecs_current
+            k1 = (ncurr - 1) * (krydim + 1)
+            l0 = vec0.copy()
+            for c in conv:
+                l0 = l0 - (c @ l0) * c
+            l0 = l0 / np.sqrt(l0 @ l0)
+            lanc[:, k1] = l0
+            q = l0
+            r = matvec(q)
+            alpha = np.zeros(krydim)
+            beta = np.zeros(krydim)
+            alpha[0] = q @ r
+            r = r - alpha[0] * q
+            beta[0] = np.sqrt(r @ r)
+            for j1 in range(1, krydim):
+                v = q
+                q = r / beta[j1 - 1]
+                lanc[:, k1 + j1] = q
+                r = matvec(q) - beta[j1 - 1] * v
+                alpha[j1] = q @ r
+                r = r - alpha[j1] * q
+                beta[j1] = np.sqrt(r @ r)
+            lanc[:, k1 + krydim] = r / beta[krydim - 1]
+            b0 = (ncurr - 1) * krydim
+            for i in range(krydim):
+                subh[b0 + i, b0 + i] = alpha[i]
+                subs[b0 + i, b0 + i] = 1.0
+            for i in range(krydim - 1):
+                subh[b0 + i, b0 + i + 1] = beta[i]
+                subh[b0 + i + 1, b0 + i] = beta[i]
+            alph[:, ncurr - 1] = alpha
+            beth[:, ncurr - 1] = beta
+            # subspace_matrices
+            for nprev in range(ncurr - 1):
+                for ii in range(krydim):
+                    i = nprev * (krydim + 1) + ii
+                    i1 = nprev * krydim + ii
+                    for k in range(krydim):
+                        j = k1 + k
+                        j1 = b0 + k
+                        hij = alpha[k] * (lanc[:, i] @ lanc[:, j]) + beta[k] * (lanc[:, i] @ lanc[:, j + 1])
+                        if k > 0:
+                            hij += beta[k - 1] * (lanc[:, i] @ lanc[:, j - 1])
+                        subh[i1, j1] = subh[j1, i1] = hij
+                        subs[i1, j1] = subs[j1, i1] = lanc[:, i] @ lanc[:, j]
+            # solve_subspace_tdse
+            nvec = ncurr * krydim
+            X, H1, c0, _ = _canonical(subs[:nvec, :nvec], subh[:nvec, :nvec], b0)
+            lam, V = np.linalg.eigh(H1)
+            c1 = (V * np.exp(-lam * step)) @ (V.T @ c0)
+            c1 /= np.sqrt(c1 @ c1)
+            coeff = X @ c1
+            cols = [bk * (krydim + 1) + i for bk in range(ncurr) for i in range(krydim)]
+            psi = lanc[:, cols] @ coeff
+            psi = psi / np.sqrt(psi @ psi)
+            hpsi = matvec(psi)
+            e = psi @ hpsi
+            res = np.sqrt(((hpsi - e * psi) ** 2).sum())
+            vec0 = psi
+            tr["steps"].append((e, res))
+            if trace:
+                tr["psi"].append(psi.copy())
+            if res <= eps:
+                converged = True
+                iters[s] = it
+                break
+        if not converged:
+            iters[s] = maxiter
+        ener[s], vecs[:, s], resid[s] = e, vec0, res
+        conv.append(vec0.copy())
+        traces.append(tr)
+        if not converged:
+            break
+    out = {"energies": ener, "vectors": vecs, "residuals": resid, "iterations": iters}
+    if trace:
+        out["trace"] = traces
+    return out
+
+
+def ondiag_guess(hii: np.ndarray, nstates: int) -> np.ndarray:
+    """Unit vectors on the nstates smallest |H_jj| (initvec_ondiag,
+    source/propagation/relaxation.f90:875-944; ties in index order)."""
+    idx = np.argsort(np.abs(hii), kind="stable")[:nstates]
+    g = np.zeros((hii.size, nstates))
+    g[idx, np.arange(nstates)] = 1.0
+    return g

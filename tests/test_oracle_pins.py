@@ -15,6 +15,8 @@ in ``oracle/`` is checked here against an independent fact:
 * Chebyshev: the order-domain autocorrelation matches sum_n (v_n.q0)^2 T_k(x_n);
 * block Lanczos: the three-term recurrence, orthonormality, exact block Gauss
   quadrature and the full-Krylov spectrum;
+* relaxation: each timestep against exp(-H dtau) from scipy.linalg.expm, and
+  the relaxed states against a dense eigendecomposition;
 * network: analytic gradients match central finite differences.
 """
 import numpy as np
@@ -22,7 +24,7 @@ import pytest
 import scipy.sparse.linalg as sla
 
 from adc_ottawa_amd.synth import make_system
-from oracle import cheby, csf, davidson, dfsigma, elements, lanczos, mlp, schur, sil
+from oracle import cheby, csf, davidson, dfsigma, elements, lanczos, mlp, relax, schur, sil
 
 
 def test_csf_counts_and_bijectivity():
@@ -212,6 +214,44 @@ def test_oracle_block_lanczos_full_krylov():
     # the Ritz vectors are then eigenvectors of H
     R = out["ritz"]
     assert np.abs(H @ R - R * out["eigval"]).max() < 1e-8 * np.abs(H).max()
+
+
+def test_oracle_relax_steps_are_imaginary_time_propagation():
+    """Each XSIL timestep propagates psi to exp(-H dtau) psi (normalised), up to
+    the Krylov truncation error, which is negligible here (dtau * spread ~ 1,
+    12 Lanczos vectors).  Steps 2-4 use the inter-timestep matrix elements
+    and step 3 follows a subspace collapse, so a wrong H_ij or S_ij, a wrong
+    canonical orthogonalisation or a wrong collapse fails it."""
+    from scipy.linalg import expm
+    sys = make_system(3, 8, 10, seed=5)
+    H = elements.build_blocks(sys, apply_minc=False).dense()
+    lam = np.linalg.eigvalsh(H)
+    dtau = 1.0 / (lam[-1] - lam[0])
+    g = relax.ondiag_guess(np.diag(H), 1)
+    out = relax.relax(lambda x: H @ x, g, krydim=12, maxiter=4, step=dtau, eps=0.0, trace=True)
+    P = expm(-dtau * H)
+    psi = g[:, 0]
+    for k, got in enumerate(out["trace"][0]["psi"]):
+        ref = P @ psi
+        ref /= np.linalg.norm(ref)
+        assert np.abs(got - ref).max() < 1e-10, k
+        psi = got
+
+
+def test_oracle_relax_converges_to_lowest_states():
+    """The relaxed states are the lowest eigenpairs (each state's first
+    Lanczos vector is orthogonalised against the states already converged)."""
+    sys = make_system(3, 8, 10, seed=5)
+    H = elements.build_blocks(sys, apply_minc=False).dense()
+    lam, U = np.linalg.eigh(H)
+    g = relax.ondiag_guess(np.diag(H), 3)
+    out = relax.relax(lambda x: H @ x, g, krydim=10, maxiter=300, step=30.0, eps=1e-7)
+    assert (out["residuals"] <= 1e-7).all()
+    # energy error ~ residual^2 / gap, vector error ~ residual / gap
+    assert np.abs(out["energies"] - lam[:3]).max() < 1e-12 * np.abs(lam).max()
+    for s in range(3):
+        v = out["vectors"][:, s]
+        assert min(np.abs(v - U[:, s]).max(), np.abs(v + U[:, s]).max()) < 2e-5
 
 
 def test_oracle_mlp_gradients():
