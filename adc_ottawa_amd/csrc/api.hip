@@ -236,6 +236,15 @@ int32_t adc2g_block_lanczos(adc2g_handle* h, const adc2g_lanczos_params* p, cons
   });
 }
 
+int32_t adc2g_relax(adc2g_handle* h, const adc2g_relax_params* p, const double* guess, double* eigval,
+                    double* eigvec, double* residual, int32_t* niter, adc2g_relax_stats* stats) {
+  return guarded([&] {
+    ARG_CHECK(h && p && eigval, "NULL argument");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    relax_run(h->h, *p, guess, eigval, eigvec, residual, niter, stats);
+  });
+}
+
 int32_t adc2g_sigma_profile(adc2g_handle* h, int32_t enable, double* out) {
   return guarded([&] {
     ARG_CHECK(h != nullptr, "NULL handle");

@@ -122,6 +122,8 @@ struct SpinBarrier {
   std::atomic<int> count_{0}, gen_{0};
 };
 
+}  // namespace
+
 // Eigen-decomposition of the symmetric n x n matrix A (destroyed): eigenvalues
 // ascending in w, eigenvectors in the columns of Z.
 //  1. Householder reduction to tridiagonal form: reflector k (unit 2-norm)
@@ -316,6 +318,8 @@ bool host_syev(int n, double* A, double* w, double* Z) {
   std::copy(Zs.begin(), Zs.end(), Z);
   return true;
 }
+
+namespace {
 
 // ---- device kernels of the Ritz stage
 

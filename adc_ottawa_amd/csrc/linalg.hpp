@@ -60,6 +60,11 @@ void launch_symmetrize_cols(int c0, int c1, double* R, int ld, hipStream_t st);
 // out[0] = sum(a_re*b_re + a_im*b_im), out[1] = sum(a_re*b_im - a_im*b_re) (i.e. <a|b>)
 int launch_cdot_partial(int64_t n, const double* a, const double* b, int64_t ld, double* part, hipStream_t st);
 
+// Host eigen-decomposition of the symmetric n x n matrix A (column-major,
+// destroyed) on up to 16 threads: eigenvalues ascending in w, eigenvectors
+// in the columns of Z (n x n).  False if the QL iteration does not converge.
+bool host_syev(int n, double* A, double* w, double* Z);
+
 inline int reduce_blocks(int64_t n) {
   int64_t b = (n + 1023) / 1024;
   if (b > 1024) b = 1024;

@@ -1,0 +1,436 @@
+// Imaginary-time relaxation to the lowest eigenstates on the GPU
+// sigma-vector: relaxation (source/propagation/relaxation.f90:36-133) with
+// its default integrator, XSIL (xsil_rlx, :139-309).
+//
+// States are relaxed one after another.  One timestep of a state:
+//  * collapse (:1163-1203): when the variational subspace holds maxdim/krydim
+//    timesteps, the oldest block of krydim + 1 vectors and its rows and
+//    columns of the subspace matrices are dropped;
+//  * Lanczos block (get_lancvecs_current, :1207-1326): the first vector is
+//    psi, orthogonalised against the converged states one after another and
+//    normalised; krydim Lanczos steps r = H q - beta v, alpha = q.r,
+//    r -= alpha q, beta = |r|, and the (krydim+1)-th vector r / beta.  The
+//    steps run back to back on the stream: alpha and beta^2 stay in device
+//    memory and the kernels that need them read them there;
+//  * matrix elements with the earlier blocks (subspace_matrices,
+//    :1330-1381): S = L_prev^T L_cur in one tall-skinny product, and H_ij from
+//    the Lanczos relation, H l_j = beta_{k-1} l_{j-1} + alpha_k l_j + beta_k l_{j+1};
+//  * propagation (solve_subspace_tdse, :1385-1491, canonical_ortho,
+//    :1495-1606): on the host, the overlap is diagonalised, eigenvalues
+//    |sigma| <= 1e-6 are dropped, and H' = X^T H X is diagonalised.  Then
+//    c = X exp(-H' dtau) X^+ e_first, normalised, and psi = L c on the
+//    device.  The propagated state is normalised; the reference's
+//    orthogonalisation before that subtracts the projection on
+//    vec_conv(:,s), which is zero until state s converges, so only the
+//    normalisation is kept;
+//  * E = psi.H psi and |H psi - E psi| (residual_1vec, :1610-1651).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <numeric>
+#include <vector>
+
+#include "linalg.hpp"
+#include "solvers.hpp"
+
+namespace adc2g {
+
+namespace {
+
+constexpr int RL_THREADS = 256;
+
+__device__ __forceinline__ void rl_block_partial(double s, double* part) {
+  __shared__ double red[RL_THREADS / 64];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < RL_THREADS / 64; ++i) t += red[i];
+    part[blockIdx.x] = t;
+  }
+}
+
+#define RL_LOOP(i) \
+  for (int64_t i = blockIdx.x * (int64_t)RL_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RL_THREADS)
+
+// part[blk] = partial a.b
+__global__ void __launch_bounds__(RL_THREADS) rl_dot_kernel(int64_t n, const double* __restrict__ a,
+                                                            const double* __restrict__ b, double* part) {
+  double s = 0.0;
+  RL_LOOP(i) s += a[i] * b[i];
+  rl_block_partial(s, part);
+}
+
+// y -= coef[0] x
+__global__ void __launch_bounds__(RL_THREADS) rl_axmy_kernel(int64_t n, const double* __restrict__ coef,
+                                                             const double* __restrict__ x, double* __restrict__ y) {
+  const double c = coef[0];
+  RL_LOOP(i) y[i] -= c * x[i];
+}
+
+// out = r / sqrt(b2[0]) (out may be r)
+__global__ void __launch_bounds__(RL_THREADS) rl_scale_kernel(int64_t n, const double* r, const double* b2,
+                                                              double* out) {
+  const double inv = 1.0 / sqrt(b2[0]);
+  RL_LOOP(i) out[i] = r[i] * inv;
+}
+
+// Lanczos step, first half: r -= sqrt(b2[0]) v (when b2 is given); part = partial q.r
+__global__ void __launch_bounds__(RL_THREADS) rl_lanc_a_kernel(int64_t n, double* __restrict__ r,
+                                                               const double* __restrict__ v, const double* b2,
+                                                               const double* __restrict__ q, double* part) {
+  const double beta = b2 ? sqrt(b2[0]) : 0.0;
+  double s = 0.0;
+  RL_LOOP(i) {
+    double x = r[i];
+    if (b2) x -= beta * v[i];
+    r[i] = x;
+    s += q[i] * x;
+  }
+  rl_block_partial(s, part);
+}
+
+// second half: r -= alpha[0] q; part = partial r.r
+__global__ void __launch_bounds__(RL_THREADS) rl_lanc_b_kernel(int64_t n, double* __restrict__ r,
+                                                               const double* __restrict__ q, const double* alpha,
+                                                               double* part) {
+  const double a = alpha[0];
+  double s = 0.0;
+  RL_LOOP(i) {
+    const double x = r[i] - a * q[i];
+    r[i] = x;
+    s += x * x;
+  }
+  rl_block_partial(s, part);
+}
+
+// part = partial (hpsi - E[0] psi)^2
+__global__ void __launch_bounds__(RL_THREADS) rl_res_kernel(int64_t n, const double* __restrict__ hpsi,
+                                                            const double* __restrict__ psi, const double* E,
+                                                            double* part) {
+  const double e = E[0];
+  double s = 0.0;
+  RL_LOOP(i) {
+    const double x = hpsi[i] - e * psi[i];
+    s += x * x;
+  }
+  rl_block_partial(s, part);
+}
+
+#undef RL_LOOP
+
+// sigma-vector time from event pairs read after the stream has been synchronised
+struct SigmaClock {
+  std::vector<hipEvent_t> ev;
+  size_t used = 0;
+  double seconds = 0.0;
+  int64_t count = 0;
+  ~SigmaClock() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+  hipEvent_t next() {
+    if (used == ev.size()) {
+      hipEvent_t e;
+      HIP_CHECK(hipEventCreate(&e));
+      ev.push_back(e);
+    }
+    return ev[used++];
+  }
+  // after a stream synchronisation
+  void collect() {
+    for (size_t i = 0; i + 1 < used; i += 2) {
+      float ms = 0.0f;
+      HIP_CHECK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+      seconds += ms * 1e-3;
+    }
+    used = 0;
+  }
+};
+
+}  // namespace
+
+void relax_run(Handle& H, const adc2g_relax_params& p, const double* guess_h, double* eigval, double* eigvec,
+               double* resid_out, int32_t* niter_out, adc2g_relax_stats* stats) {
+  using clk = std::chrono::steady_clock;
+  const auto t_start = clk::now();
+  const int64_t n = H.d.ndim;
+  const int ns = p.nstates, kd = p.krydim;
+  ARG_CHECK(ns >= 1 && kd >= 1 && p.maxiter >= 1, "nstates, krydim and maxiter must be >= 1");
+  ARG_CHECK(std::isfinite(p.step) && p.step > 0.0, "step must be finite and positive");
+  ARG_CHECK(p.tol >= 0.0, "tol must be >= 0");
+  ARG_CHECK((int64_t)kd + 1 <= n && (int64_t)ns <= n, "krydim + 1 and nstates must not exceed ndim");
+  const int maxdim = p.maxdim <= 0 ? 2 * kd : p.maxdim;
+  ARG_CHECK(maxdim % kd == 0, "maxdim must be an integer multiple of krydim");
+  const int maxvec = maxdim + maxdim / kd;  // stored vectors, krydim + 1 per timestep
+  const int kb = kd + 1;
+  const Prec pr = prec_from_int(p.precision);
+  hipStream_t st = H.st;
+  const int nblk = reduce_blocks(n);
+
+  DevBuf<double> L, r, psi, hpsi, guess, conv, part, sc, coef, Sx;
+  L.alloc((size_t)n * maxvec);
+  r.alloc(n);
+  psi.alloc(n);
+  hpsi.alloc(n);
+  guess.alloc((size_t)n * ns);
+  conv.alloc((size_t)n * ns);
+  part.alloc(nblk);
+  // device scalars: alpha[kd], beta^2[kd], then 8 work slots
+  sc.alloc((size_t)2 * kd + 8);
+  coef.alloc(maxvec);
+  Sx.alloc((size_t)maxvec * kb);
+  double* alpha_d = sc.get();
+  double* b2_d = sc.get() + kd;
+  double* w_d = sc.get() + 2 * kd;
+
+  if (guess_h) {
+    HIP_CHECK(hipMemcpyAsync(guess.get(), guess_h, sizeof(double) * n * ns, hipMemcpyHostToDevice, st));
+  } else {
+    // initvec_ondiag: unit vectors on the ns smallest |H_jj| (ties in index order)
+    std::vector<double> hii(n);
+    HIP_CHECK(hipMemcpyAsync(hii.data(), H.hii.get(), sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    std::vector<int64_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::partial_sort(order.begin(), order.begin() + ns, order.end(), [&](int64_t a, int64_t b) {
+      const double fa = std::fabs(hii[a]), fb = std::fabs(hii[b]);
+      return fa < fb || (fa == fb && a < b);
+    });
+    DevBuf<int64_t> didx;
+    didx.alloc(ns);
+    HIP_CHECK(hipMemcpyAsync(didx.get(), order.data(), sizeof(int64_t) * ns, hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemsetAsync(guess.get(), 0, sizeof(double) * n * ns, st));
+    launch_unit_vectors(ns, didx.get(), guess.get(), n, st);
+    HIP_CHECK(hipStreamSynchronize(st));
+  }
+
+  SigmaClock clock;
+  auto sigma1 = [&](const double* x, double* w) {
+    HIP_CHECK(hipEventRecord(clock.next(), st));
+    H.sigma(x, n, w, n, 1, pr, st);
+    HIP_CHECK(hipEventRecord(clock.next(), st));
+    ++clock.count;
+  };
+  auto dot = [&](const double* a, const double* b, double* out_d) {
+    hipLaunchKernelGGL(rl_dot_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, a, b, part.get());
+    HIP_CHECK(hipGetLastError());
+    launch_colsum(nblk, 1, part.get(), out_d, st);
+  };
+  auto normalise = [&](double* x, double* slot_d) {
+    dot(x, x, slot_d);
+    hipLaunchKernelGGL(rl_scale_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, x, slot_d, x);
+    HIP_CHECK(hipGetLastError());
+  };
+  // E = x.Hx and |Hx - E x| (one sigma-vector, one download)
+  auto residual = [&](const double* x, double& E, double& res) {
+    sigma1(x, hpsi.get());
+    dot(x, hpsi.get(), w_d + 3);
+    hipLaunchKernelGGL(rl_res_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, hpsi.get(), x, w_d + 3, part.get());
+    HIP_CHECK(hipGetLastError());
+    launch_colsum(nblk, 1, part.get(), w_d + 4, st);
+    double h[2];
+    HIP_CHECK(hipMemcpyAsync(h, w_d + 3, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    clock.collect();
+    E = h[0];
+    res = std::sqrt(h[1]);
+  };
+
+  // host copies of the subspace matrices (column-major, ld = maxvec)
+  std::vector<double> subh((size_t)maxvec * maxvec), subs((size_t)maxvec * maxvec), tmp((size_t)maxvec * maxvec);
+  std::vector<double> alpha(kd), beta(kd), b2h(kd), sx((size_t)maxvec * kb);
+  auto SH = [&](std::vector<double>& m, int i, int j) -> double& { return m[i + (size_t)maxvec * j]; };
+
+  int nconv = 0;
+  int status = ADC2G_OK;
+  std::string fail_msg;
+  for (int s = 0; s < ns; ++s) {
+    std::fill(subh.begin(), subh.end(), 0.0);
+    std::fill(subs.begin(), subs.end(), 0.0);
+    int ncurr = 0;
+    HIP_CHECK(hipMemcpyAsync(psi.get(), guess.get() + (size_t)n * s, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                             st));
+    double E = 0.0, res = 0.0;
+    residual(psi.get(), E, res);
+    if (p.verbose) std::fprintf(stderr, "relax state %d step %5d  E = %.12f  residual = %.4e\n", s + 1, 0, E, res);
+    int it_done = p.maxiter;
+    bool converged = false;
+    for (int it = 1; it <= p.maxiter; ++it) {
+      if (ncurr * kb == maxvec) {  // collapse_subspace
+        for (int blk = 1; blk < maxvec / kb; ++blk)
+          HIP_CHECK(hipMemcpyAsync(L.get() + (size_t)n * kb * (blk - 1), L.get() + (size_t)n * kb * blk,
+                                   sizeof(double) * n * kb, hipMemcpyDeviceToDevice, st));
+        const int keep = maxvec - kd;
+        for (auto* m : {&subs, &subh}) {
+          for (int j = 0; j < keep; ++j)
+            for (int i = 0; i < keep; ++i) tmp[i + (size_t)keep * j] = SH(*m, kd + i, kd + j);
+          for (int j = 0; j < keep; ++j)
+            for (int i = 0; i < keep; ++i) SH(*m, i, j) = tmp[i + (size_t)keep * j];
+        }
+        --ncurr;
+      }
+      ++ncurr;
+      const int k1 = (ncurr - 1) * kb;  // first vector of this timestep in L
+      const int b0 = (ncurr - 1) * kd;  // its row/column in the subspace matrices
+      double* Lk = L.get() + (size_t)n * k1;
+      // first vector: psi orthogonalised against the converged states, normalised
+      HIP_CHECK(hipMemcpyAsync(Lk, psi.get(), sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+      for (int c = 0; c < nconv; ++c) {
+        const double* vc = conv.get() + (size_t)n * c;
+        dot(vc, Lk, w_d);
+        hipLaunchKernelGGL(rl_axmy_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, w_d, vc, Lk);
+        HIP_CHECK(hipGetLastError());
+      }
+      normalise(Lk, w_d + 1);
+      // krydim Lanczos steps and the extra vector
+      for (int j = 0; j < kd; ++j) {
+        const double* q = Lk + (size_t)n * j;
+        sigma1(q, r.get());
+        hipLaunchKernelGGL(rl_lanc_a_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, r.get(),
+                           j ? q - n : nullptr, j ? b2_d + j - 1 : nullptr, q, part.get());
+        HIP_CHECK(hipGetLastError());
+        launch_colsum(nblk, 1, part.get(), alpha_d + j, st);
+        hipLaunchKernelGGL(rl_lanc_b_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, r.get(), q, alpha_d + j,
+                           part.get());
+        HIP_CHECK(hipGetLastError());
+        launch_colsum(nblk, 1, part.get(), b2_d + j, st);
+        hipLaunchKernelGGL(rl_scale_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, r.get(), b2_d + j,
+                           Lk + (size_t)n * (j + 1));
+        HIP_CHECK(hipGetLastError());
+      }
+      if (ncurr > 1) gemm_tn(H, k1, kb, n, L.get(), n, Lk, n, Sx.get(), k1, 1.0, 0.0);
+      HIP_CHECK(hipMemcpyAsync(alpha.data(), alpha_d, sizeof(double) * kd, hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipMemcpyAsync(b2h.data(), b2_d, sizeof(double) * kd, hipMemcpyDeviceToHost, st));
+      if (ncurr > 1)
+        HIP_CHECK(hipMemcpyAsync(sx.data(), Sx.get(), sizeof(double) * k1 * kb, hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipStreamSynchronize(st));
+      clock.collect();
+      for (int j = 0; j < kd; ++j) {
+        if (!(b2h[j] > 0.0) || !std::isfinite(b2h[j]) || !std::isfinite(alpha[j]))
+          throw Error(ADC2G_EBREAK, "relaxation: Lanczos breakdown at state " + std::to_string(s + 1) +
+                                        ", step " + std::to_string(it));
+        beta[j] = std::sqrt(b2h[j]);
+      }
+      for (int i = 0; i < kd; ++i) {
+        SH(subh, b0 + i, b0 + i) = alpha[i];
+        SH(subs, b0 + i, b0 + i) = 1.0;
+      }
+      for (int i = 0; i + 1 < kd; ++i) {
+        SH(subh, b0 + i, b0 + i + 1) = beta[i];
+        SH(subh, b0 + i + 1, b0 + i) = beta[i];
+      }
+      // matrix elements with the vectors of the earlier timesteps
+      for (int np = 0; np + 1 < ncurr; ++np)
+        for (int ii = 0; ii < kd; ++ii) {
+          const int i = np * kb + ii, i1 = np * kd + ii;
+          auto ov = [&](int col) { return sx[i + (size_t)k1 * col]; };  // l_i . l_{k1+col}
+          for (int k = 0; k < kd; ++k) {
+            const int j1 = b0 + k;
+            const double hij = (k == 0) ? alpha[k] * ov(k) + beta[k] * ov(k + 1)
+                                        : beta[k - 1] * ov(k - 1) + alpha[k] * ov(k) + beta[k] * ov(k + 1);
+            SH(subh, i1, j1) = hij;
+            SH(subh, j1, i1) = hij;
+            SH(subs, i1, j1) = ov(k);
+            SH(subs, j1, i1) = ov(k);
+          }
+        }
+      // canonical orthogonalisation and propagation in the subspace (host)
+      const int nv = ncurr * kd;
+      std::vector<double> S((size_t)nv * nv), Hs((size_t)nv * nv), sig(nv), U((size_t)nv * nv);
+      for (int j = 0; j < nv; ++j)
+        for (int i = 0; i < nv; ++i) {
+          S[i + (size_t)nv * j] = SH(subs, i, j);
+          Hs[i + (size_t)nv * j] = SH(subh, i, j);
+        }
+      if (!host_syev(nv, S.data(), sig.data(), U.data()))
+        throw Error(ADC2G_EEIG, "relaxation: diagonalisation of the subspace overlap matrix failed");
+      int nlin = 0;
+      for (int i = 0; i < nv; ++i)
+        if (std::fabs(sig[i]) > 1e-6) ++nlin;
+      const int nnull = nv - nlin;
+      std::vector<double> X((size_t)nv * nlin), c0(nlin), T1((size_t)nv * nlin), H1((size_t)nlin * nlin);
+      for (int c = 0; c < nlin; ++c) {
+        const double sv = sig[nnull + c], rs = std::sqrt(sv);
+        for (int i = 0; i < nv; ++i) X[i + (size_t)nv * c] = U[i + (size_t)nv * (nnull + c)] / rs;
+        c0[c] = U[b0 + (size_t)nv * (nnull + c)] * rs;
+      }
+      // H1 = X^T (Hs X)
+      for (int c = 0; c < nlin; ++c)
+        for (int i = 0; i < nv; ++i) {
+          double t = 0.0;
+          for (int k = 0; k < nv; ++k) t += Hs[i + (size_t)nv * k] * X[k + (size_t)nv * c];
+          T1[i + (size_t)nv * c] = t;
+        }
+      for (int c = 0; c < nlin; ++c)
+        for (int rr = 0; rr < nlin; ++rr) {
+          double t = 0.0;
+          for (int k = 0; k < nv; ++k) t += X[k + (size_t)nv * rr] * T1[k + (size_t)nv * c];
+          H1[rr + (size_t)nlin * c] = t;
+        }
+      std::vector<double> lam(nlin), V((size_t)nlin * nlin);
+      if (!host_syev(nlin, H1.data(), lam.data(), V.data()))
+        throw Error(ADC2G_EEIG, "relaxation: diagonalisation of the subspace Hamiltonian failed");
+      // c1 = V exp(-lam dtau) V^T c0, normalised; coefficients X c1
+      std::vector<double> y(nlin), c1(nlin, 0.0);
+      for (int k = 0; k < nlin; ++k) {
+        double t = 0.0;
+        for (int i = 0; i < nlin; ++i) t += V[i + (size_t)nlin * k] * c0[i];
+        y[k] = std::exp(-lam[k] * p.step) * t;
+      }
+      for (int k = 0; k < nlin; ++k)
+        for (int i = 0; i < nlin; ++i) c1[i] += V[i + (size_t)nlin * k] * y[k];
+      double nrm = 0.0;
+      for (int i = 0; i < nlin; ++i) nrm += c1[i] * c1[i];
+      nrm = std::sqrt(nrm);
+      for (int i = 0; i < nlin; ++i) c1[i] /= nrm;
+      // coefficients over the stored vectors (zero on the extra vector of each block)
+      const int ncols = ncurr * kb;
+      std::vector<double> cf(ncols, 0.0);
+      for (int blk = 0; blk < ncurr; ++blk)
+        for (int i = 0; i < kd; ++i) {
+          double t = 0.0;
+          for (int c = 0; c < nlin; ++c) t += X[(blk * kd + i) + (size_t)nv * c] * c1[c];
+          cf[blk * kb + i] = t;
+        }
+      HIP_CHECK(hipMemcpyAsync(coef.get(), cf.data(), sizeof(double) * ncols, hipMemcpyHostToDevice, st));
+      gemm_nn(H, n, 1, ncols, L.get(), n, coef.get(), ncols, psi.get(), n, 1.0, 0.0);
+      normalise(psi.get(), w_d + 2);
+      residual(psi.get(), E, res);
+      if (!std::isfinite(E) || !std::isfinite(res))
+        throw Error(ADC2G_EBREAK, "relaxation: non-finite energy at state " + std::to_string(s + 1));
+      if (p.verbose)
+        std::fprintf(stderr, "relax state %d step %5d  E = %.12f  residual = %.4e\n", s + 1, it, E, res);
+      if (res <= p.tol) {
+        converged = true;
+        it_done = it;
+        break;
+      }
+    }
+    eigval[s] = E;
+    if (resid_out) resid_out[s] = res;
+    if (niter_out) niter_out[s] = it_done;
+    if (eigvec)
+      HIP_CHECK(hipMemcpyAsync(eigvec + (size_t)n * s, psi.get(), sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(conv.get() + (size_t)n * s, psi.get(), sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    if (!converged) {
+      status = ADC2G_ENOCONV;
+      fail_msg = "Convergence not reached for state: " + std::to_string(s + 1);
+      break;
+    }
+    ++nconv;
+  }
+  if (stats) {
+    stats->nconv = nconv;
+    stats->reserved = 0;
+    stats->nsigma = clock.count;
+    stats->time_sigma = clock.seconds;
+    stats->time_total = std::chrono::duration<double>(clk::now() - t_start).count();
+  }
+  if (status != ADC2G_OK) throw Error(status, fail_msg);
+}
+
+}  // namespace adc2g

@@ -34,6 +34,11 @@ void cheby_run(Handle& H, const double bounds[2], const double* q0, int order, P
 void lanczos_run(Handle& H, const adc2g_lanczos_params& p, const double* q0, double* eigval, double* tmat,
                  const double* travec, double* tranmom, double* ritz, adc2g_lanczos_stats* stats);
 
+// Imaginary-time relaxation (XSIL) of p.nstates states; outputs as
+// adc2g_relax documents them (guess may be NULL).
+void relax_run(Handle& H, const adc2g_relax_params& p, const double* guess, double* eigval, double* eigvec,
+               double* residual, int32_t* niter, adc2g_relax_stats* stats);
+
 int precon_nparams();
 void precon_harvest(Handle& H, const adc2g_dav_params& p, int64_t max_records, int64_t* added);
 void precon_train(Handle& H, const Handle& data, int epochs, int batch, double lr, uint64_t seed, double* loss_out);

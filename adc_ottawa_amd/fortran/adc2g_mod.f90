@@ -77,6 +77,26 @@ module adc2g_mod
      integer(c_int32_t) :: check_orth = 0
   end type adc2g_lanczos_params
 
+  ! relaxation (source/propagation/relaxation.f90:36-133, XSIL integrator):
+  ! nstates = davstates, krydim = kdim, maxdim = maxsubdim, step = stepsize,
+  ! tol = davtol of the relaxation input
+  type, bind(c) :: adc2g_relax_params
+     integer(c_int32_t) :: nstates = 0
+     integer(c_int32_t) :: krydim = 10
+     integer(c_int32_t) :: maxdim = -1
+     integer(c_int32_t) :: maxiter = 100
+     real(c_double)     :: step = 10.0d0
+     real(c_double)     :: tol = 1.0d-7
+     integer(c_int32_t) :: precision = 0
+     integer(c_int32_t) :: verbose = 0
+  end type adc2g_relax_params
+
+  type, bind(c) :: adc2g_relax_stats
+     integer(c_int32_t) :: nconv, reserved
+     integer(c_int64_t) :: nsigma
+     real(c_double)     :: time_sigma, time_total
+  end type adc2g_relax_stats
+
   type, bind(c) :: adc2g_lanczos_stats
      integer(c_int32_t) :: nvec, nsigma
      real(c_double)     :: time_sigma, time_tdiag, time_ritz, time_total, orth_loss
@@ -171,6 +191,18 @@ module adc2g_mod
        type(c_ptr), value :: tmat, travec, tranmom, ritz
        type(adc2g_lanczos_stats), intent(out) :: stats
      end function adc2g_block_lanczos
+
+     ! guess, eigvec, residual, niter: c_loc of the array or c_null_ptr
+     integer(c_int32_t) function adc2g_relax(h, p, guess, eigval, eigvec, residual, niter, stats) &
+          bind(c, name='adc2g_relax')
+       import :: c_int32_t, c_double, c_ptr, adc2g_relax_params, adc2g_relax_stats
+       type(c_ptr), value :: h
+       type(adc2g_relax_params), intent(in) :: p
+       type(c_ptr), value :: guess
+       real(c_double), intent(out) :: eigval(*)
+       type(c_ptr), value :: eigvec, residual, niter
+       type(adc2g_relax_stats), intent(out) :: stats
+     end function adc2g_relax
 
      integer(c_int32_t) function adc2g_precon_nparams() bind(c, name='adc2g_precon_nparams')
        import :: c_int32_t

@@ -75,6 +75,17 @@ class LanczosStats(C.Structure):
                 ("orth_loss", C.c_double)]
 
 
+class RelaxParams(C.Structure):
+    _fields_ = [("nstates", C.c_int32), ("krydim", C.c_int32), ("maxdim", C.c_int32),
+                ("maxiter", C.c_int32), ("step", C.c_double), ("tol", C.c_double),
+                ("precision", C.c_int32), ("verbose", C.c_int32)]
+
+
+class RelaxStats(C.Structure):
+    _fields_ = [("nconv", C.c_int32), ("reserved", C.c_int32), ("nsigma", C.c_int64),
+                ("time_sigma", C.c_double), ("time_total", C.c_double)]
+
+
 # every symbol declared in include/adc2g.h
 SYMBOLS = [
     "adc2g_create", "adc2g_destroy", "adc2g_dims", "adc2g_diagonal", "adc2g_phph",
@@ -82,7 +93,7 @@ SYMBOLS = [
     "adc2g_sil_autocorr", "adc2g_precon_harvest", "adc2g_precon_train", "adc2g_precon_nparams",
     "adc2g_precon_get", "adc2g_precon_set", "adc2g_precon_eval", "adc2g_last_error",
     "adc2g_sigma_profile", "adc2g_precon_absorb", "adc2g_coupling_form", "adc2g_cheby_auto",
-    "adc2g_block_lanczos",
+    "adc2g_block_lanczos", "adc2g_relax",
 ]
 
 _lib = None
@@ -122,6 +133,7 @@ def load(path: str = LIBPATH):
     lib.adc2g_coupling_form.argtypes = [vp, vp]
     lib.adc2g_cheby_auto.argtypes = [vp, vp, vp, i32, i32, vp]
     lib.adc2g_precon_absorb.argtypes = [vp, vp, C.POINTER(i64)]
+    lib.adc2g_relax.argtypes = [vp, C.POINTER(RelaxParams), vp, vp, vp, vp, vp, C.POINTER(RelaxStats)]
     lib.adc2g_block_lanczos.argtypes = [vp, C.POINTER(LanczosParams), vp, vp, vp, vp, vp, vp,
                                         C.POINTER(LanczosStats)]
     for s in SYMBOLS:
@@ -277,6 +289,26 @@ class Operator:
         extras = {k: v for k, v in extras.items() if v is not None}
         stats = {k: getattr(st, k) for k, _ in LanczosStats._fields_}
         return ev, extras, stats
+
+    def relax(self, nstates, guess=None, krydim=10, maxdim=-1, maxiter=100, step=10.0, tol=1e-7,
+              precision="fp64", vectors=True, verbose=False):
+        """Imaginary-time relaxation (XSIL) of the nstates lowest states.
+
+        guess: (ndim, nstates) or None (unit vectors on the smallest |H_jj|).
+        Returns (energies, vectors or None, residuals, iterations, stats)."""
+        g = None
+        if guess is not None:
+            g = np.asfortranarray(np.asarray(guess, dtype=np.float64).reshape(self.ndim, nstates))
+        p = RelaxParams(nstates, krydim, maxdim, maxiter, step, tol, PREC[precision], 1 if verbose else 0)
+        ev = np.empty(nstates)
+        vec = np.empty((self.ndim, nstates), order="F") if vectors else None
+        res = np.empty(nstates)
+        nit = np.empty(nstates, dtype=np.int32)
+        st = RelaxStats()
+        check(_lib.adc2g_relax(self._h, C.byref(p), _ptr(g) if g is not None else None, _ptr(ev),
+                               _ptr(vec) if vectors else None, _ptr(res), _ptr(nit), C.byref(st)))
+        stats = {k: getattr(st, k) for k, _ in RelaxStats._fields_}
+        return ev, vec, res, nit, stats
 
     def sigma_device(self, nvec, dX, dW, precision="fp64", stream=None):
         """W = H X on device pointers (ints), asynchronous on ``stream``."""

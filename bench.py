@@ -333,6 +333,21 @@ def lanczos_measure(op, sz, spec, seed=0):
     return out
 
 
+def relax_measure(op, tol):
+    """Imaginary-time relaxation (adc2g_relax, XSIL, krydim 10, step 30 a.u.)
+    of the lowest state from the unit vector on the smallest |H_jj|, FP64."""
+    t0 = time.perf_counter()
+    ev, _, res, nit, st = op.relax(1, step=30.0, maxiter=400, tol=tol, vectors=False)
+    wall = time.perf_counter() - t0
+    out = {"tol": tol, "step_au": 30.0, "krydim": 10, "seconds": round(wall, 3),
+           "timesteps": int(nit[0]), "sigma_vectors": int(st["nsigma"]),
+           "sigma_seconds": round(st["time_sigma"], 3),
+           "ms_per_timestep": round(wall / max(1, int(nit[0])) * 1e3, 3),
+           "energy_Eh": float(ev[0]), "residual": float(res[0])}
+    log(f"relax fp64: {wall:.2f}s, {int(nit[0])} timesteps to residual {res[0]:.2e}, E = {ev[0]:.8f}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -352,6 +367,8 @@ def main():
                     help="also run the Chebyshev order-domain autocorrelation to this order")
     ap.add_argument("--lanczos", default="",
                     help="also run the block Lanczos pseudospectrum, 'blocksize,ncycles' (e.g. 50,20)")
+    ap.add_argument("--relax", type=float, default=0.0,
+                    help="also relax the lowest state in imaginary time to this residual (e.g. 1e-3)")
     ap.add_argument("--dav-roots", type=int, default=20, help="Davidson comparison: roots (block >= roots)")
     ap.add_argument("--dav-block", type=int, default=20, help="Davidson comparison: block size")
     ap.add_argument("--dav-maxvec", type=int, default=200, help="Davidson comparison: maximum subspace")
@@ -422,6 +439,7 @@ def main():
     sil = sil_compare(op, sz, args.sil, args.sil_tol) if args.sil > 0 else None
     cheb = cheby_measure(op, sz, args.cheby) if args.cheby > 0 else None
     lanc = lanczos_measure(op, sz, args.lanczos) if args.lanczos else None
+    rlx = relax_measure(op, args.relax) if args.relax > 0 else None
 
     n1 = op.n1
     bytes_per_sigma = 8.0 * (2 * op.ndim) + 8.0 * 2 * n1 * n1  # read x, write sigma, pair tensors
@@ -449,6 +467,7 @@ def main():
         "sil_autocorrelation": sil,
         "chebyshev_autocorrelation": cheb,
         "block_lanczos": lanc,
+        "relaxation": rlx,
         "operator_build_seconds": round(t_build, 3),
     }
     op.close()

@@ -134,6 +134,25 @@ typedef struct {
   double orth_loss;      /* max |L^T L - I| when check_orth, else -1 */
 } adc2g_lanczos_stats;
 
+typedef struct {
+  int32_t nstates;       /* davstates: states relaxed one after another */
+  int32_t krydim;        /* kdim: Lanczos vectors per timestep */
+  int32_t maxdim;        /* maxsubdim: variational subspace, a multiple of krydim (<= 0: 2 krydim) */
+  int32_t maxiter;       /* maxiter: timesteps per state */
+  double step;           /* stepsize: imaginary timestep (a.u.) */
+  double tol;            /* davtol: convergence when |H psi - E psi| <= tol */
+  int32_t precision;     /* ADC2G_PREC_* of the sigma-vectors */
+  int32_t verbose;       /* 1: one line per timestep on stderr (wrtable_1vec) */
+} adc2g_relax_params;
+
+typedef struct {
+  int32_t nconv;         /* states converged */
+  int32_t reserved;
+  int64_t nsigma;        /* sigma-vectors applied (nmult) */
+  double time_sigma;     /* s in the sigma-vectors (device events) */
+  double time_total;     /* s, whole call */
+} adc2g_relax_stats;
+
 /* ---- lifetime ---------------------------------------------------------- */
 
 /* Build the on-device ADC(2)-s operator from orbital energies eps[nocc+nvirt]
@@ -240,6 +259,27 @@ int32_t adc2g_cheby_auto(adc2g_handle* h, const double* bounds, const double* q0
 int32_t adc2g_block_lanczos(adc2g_handle* h, const adc2g_lanczos_params* p, const double* q0,
                             double* eigval, double* tmat, const double* travec, double* tranmom,
                             double* ritz, adc2g_lanczos_stats* stats);
+
+/* Imaginary-time relaxation to the lowest eigenstates: relaxation
+ * (source/propagation/relaxation.f90:36-133) with its default integrator,
+ * XSIL (xsil_rlx, :139-309).  For each state s in turn: per timestep, krydim
+ * Lanczos vectors from psi, whose first vector is orthogonalised against the
+ * states already converged (get_lancvecs_current, :1207-1326).  psi is
+ * propagated by exp(-H step) in the variational subspace of the last
+ * maxdim/krydim timesteps (subspace_matrices, :1330-1381;
+ * solve_subspace_tdse, :1385-1491; Lowdin canonical orthogonalisation that
+ * drops overlap eigenvalues |sigma| <= 1e-6, canonical_ortho, :1495-1606;
+ * subspace collapse, :1163-1203) and normalised.  It stops at
+ * |H psi - E psi| <= tol (residual_1vec, :1610-1651).
+ * guess: host ndim x nstates initial vectors, or NULL for unit vectors on the
+ * nstates smallest |H_jj| (initvec_ondiag, :875-944).  eigval[nstates];
+ * eigvec (ndim x nstates), residual[nstates], niter[nstates] and stats may
+ * be NULL.  Returns ADC2G_ENOCONV if a state does not converge within maxiter
+ * steps (the reference stops there; the outputs hold the states up to and
+ * including that one), ADC2G_EBREAK if a Lanczos vector cannot be formed. */
+int32_t adc2g_relax(adc2g_handle* h, const adc2g_relax_params* p, const double* guess,
+                    double* eigval, double* eigvec, double* residual, int32_t* niter,
+                    adc2g_relax_stats* stats);
 
 /* ---- learned preconditioner -------------------------------------------- */
 

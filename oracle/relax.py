@@ -31,7 +31,8 @@ Each timestep of a state does the following.
 Pinned in tests/test_oracle_pins.py by the following checks.
   * Each of the first timesteps propagates psi to exp(-H dtau) psi /
     |exp(-H dtau) psi| (scipy.linalg.expm), to the Krylov truncation error.
-  * The subspace matrices equal L^T H L and L^T L.
+  * Steps 2-4 of that check go through the inter-timestep matrix elements
+    and a subspace collapse, so the same check covers them.
   * The relaxed energies and vectors are the lowest eigenpairs of a dense
     eigendecomposition.
 """
@@ -138,11 +139,14 @@ def relax(matvec, guess: np.ndarray, krydim=10, maxdim=-1, maxiter=100, step=10.
                     for k in range(krydim):
                         j = k1 + k
                         j1 = b0 + k
-                        hij = alpha[k] * (lanc[:, i] @ lanc[:, j]) + beta[k] * (lanc[:, i] @ lanc[:, j + 1])
-                        if k > 0:
-                            hij += beta[k - 1] * (lanc[:, i] @ lanc[:, j - 1])
+                        sj = lanc[:, i] @ lanc[:, j]
+                        sj1 = lanc[:, i] @ lanc[:, j + 1]
+                        if k == 0:
+                            hij = alpha[k] * sj + beta[k] * sj1
+                        else:
+                            hij = beta[k - 1] * (lanc[:, i] @ lanc[:, j - 1]) + alpha[k] * sj + beta[k] * sj1
                         subh[i1, j1] = subh[j1, i1] = hij
-                        subs[i1, j1] = subs[j1, i1] = lanc[:, i] @ lanc[:, j]
+                        subs[i1, j1] = subs[j1, i1] = sj
             # solve_subspace_tdse
             nvec = ncurr * krydim
             X, H1, c0, _ = _canonical(subs[:nvec, :nvec], subh[:nvec, :nvec], b0)
