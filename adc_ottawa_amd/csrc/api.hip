@@ -245,6 +245,15 @@ int32_t adc2g_relax(adc2g_handle* h, const adc2g_relax_params* p, const double* 
   });
 }
 
+int32_t adc2g_write_hmlt(adc2g_handle* h, const char* dia_path, const char* off_path, double minc,
+                         int32_t maxbl, int64_t* noffd, int32_t* nbuf) {
+  return guarded([&] {
+    ARG_CHECK(h != nullptr, "NULL handle");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    hmlt_write(h->h, dia_path, off_path, minc, maxbl > 0 ? maxbl : 655360, noffd, nbuf);
+  });
+}
+
 int32_t adc2g_sigma_profile(adc2g_handle* h, int32_t enable, double* out) {
   return guarded([&] {
     ARG_CHECK(h != nullptr, "NULL handle");

@@ -39,6 +39,11 @@ void lanczos_run(Handle& H, const adc2g_lanczos_params& p, const double* q0, dou
 void relax_run(Handle& H, const adc2g_relax_params& p, const double* guess, double* eigval, double* eigvec,
                double* residual, int32_t* niter, adc2g_relax_stats* stats);
 
+// The hmlt.dia / hmlt.off files of the matrix (upper triangle, |h_ij| > minc,
+// records of maxbl elements); noffd and nbuf may be NULL.
+void hmlt_write(Handle& H, const char* dia_path, const char* off_path, double minc, int maxbl, int64_t* noffd,
+                int32_t* nbuf);
+
 int precon_nparams();
 void precon_harvest(Handle& H, const adc2g_dav_params& p, int64_t max_records, int64_t* added);
 void precon_train(Handle& H, const Handle& data, int epochs, int batch, double lr, uint64_t seed, double* loss_out);

@@ -204,6 +204,18 @@ module adc2g_mod
        type(adc2g_relax_stats), intent(out) :: stats
      end function adc2g_relax
 
+     ! dia_path, off_path: NUL-terminated (trim(name)//c_null_char)
+     integer(c_int32_t) function adc2g_write_hmlt(h, dia_path, off_path, minc, maxbl, noffd, nbuf) &
+          bind(c, name='adc2g_write_hmlt')
+       import :: c_int32_t, c_int64_t, c_double, c_char, c_ptr
+       type(c_ptr), value :: h
+       character(kind=c_char), intent(in) :: dia_path(*), off_path(*)
+       real(c_double), value :: minc
+       integer(c_int32_t), value :: maxbl
+       integer(c_int64_t), intent(out) :: noffd
+       integer(c_int32_t), intent(out) :: nbuf
+     end function adc2g_write_hmlt
+
      integer(c_int32_t) function adc2g_precon_nparams() bind(c, name='adc2g_precon_nparams')
        import :: c_int32_t
      end function adc2g_precon_nparams

@@ -93,7 +93,7 @@ SYMBOLS = [
     "adc2g_sil_autocorr", "adc2g_precon_harvest", "adc2g_precon_train", "adc2g_precon_nparams",
     "adc2g_precon_get", "adc2g_precon_set", "adc2g_precon_eval", "adc2g_last_error",
     "adc2g_sigma_profile", "adc2g_precon_absorb", "adc2g_coupling_form", "adc2g_cheby_auto",
-    "adc2g_block_lanczos", "adc2g_relax",
+    "adc2g_block_lanczos", "adc2g_relax", "adc2g_write_hmlt",
 ]
 
 _lib = None
@@ -133,6 +133,8 @@ def load(path: str = LIBPATH):
     lib.adc2g_coupling_form.argtypes = [vp, vp]
     lib.adc2g_cheby_auto.argtypes = [vp, vp, vp, i32, i32, vp]
     lib.adc2g_precon_absorb.argtypes = [vp, vp, C.POINTER(i64)]
+    lib.adc2g_write_hmlt.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_double, i32, C.POINTER(i64),
+                                     C.POINTER(i32)]
     lib.adc2g_relax.argtypes = [vp, C.POINTER(RelaxParams), vp, vp, vp, vp, vp, C.POINTER(RelaxStats)]
     lib.adc2g_block_lanczos.argtypes = [vp, C.POINTER(LanczosParams), vp, vp, vp, vp, vp, vp,
                                         C.POINTER(LanczosStats)]
@@ -309,6 +311,13 @@ class Operator:
                                _ptr(vec) if vectors else None, _ptr(res), _ptr(nit), C.byref(st)))
         stats = {k: getattr(st, k) for k, _ in RelaxStats._fields_}
         return ev, vec, res, nit, stats
+
+    def write_hmlt(self, dia_path, off_path, minc=1e-12, maxbl=0):
+        """hmlt.dia / hmlt.off files of the matrix; returns (noffd, nbuf)."""
+        nz, nb = C.c_int64(), C.c_int32()
+        check(_lib.adc2g_write_hmlt(self._h, os.fsencode(dia_path), os.fsencode(off_path), float(minc),
+                                    int(maxbl), C.byref(nz), C.byref(nb)))
+        return nz.value, nb.value
 
     def sigma_device(self, nvec, dX, dW, precision="fp64", stream=None):
         """W = H X on device pointers (ints), asynchronous on ``stream``."""

@@ -281,6 +281,23 @@ int32_t adc2g_relax(adc2g_handle* h, const adc2g_relax_params* p, const double* 
                     double* eigval, double* eigvec, double* residual, int32_t* niter,
                     adc2g_relax_stats* stats);
 
+/* ---- on-disk matrix ------------------------------------------------------ */
+
+/* Write the matrix in the reference's on-disk format so that its readers
+ * (rdham_on / rdham_off, source/eigen/block_davidson.f90:600-686) can load it:
+ * dia_path gets the records (maxbl, nbuf) and H_jj (wrtdg,
+ * source/adclib/filetools.f90:24-34); off_path gets nbuf records
+ * (buff(maxbl), oi(maxbl), oj(maxbl), nlim) (wrtoffdg, :36-44) holding the
+ * upper triangle i < j (1-based) with |h_ij| > minc (the reference keeps
+ * |h| > minc = 1e-12), full records and a last, possibly empty one, as
+ * get_offdiag_adc2_save_omp (source/adclib/get_matrix.f90:705-1186) writes
+ * them.  maxbl <= 0 means 655360 (buf_size, source/include/parameters.f90:425).
+ * The columns are formed by FP64 sigma-vectors of unit vectors, so the cost
+ * is that of ndim sigma-vectors: meant for systems whose stored matrix the
+ * reference itself could hold.  noffd (elements written) and nbuf may be NULL. */
+int32_t adc2g_write_hmlt(adc2g_handle* h, const char* dia_path, const char* off_path, double minc,
+                         int32_t maxbl, int64_t* noffd, int32_t* nbuf);
+
 /* ---- learned preconditioner -------------------------------------------- */
 
 /* Run FP64 Davidson on this handle's system with the reference DPR step and
