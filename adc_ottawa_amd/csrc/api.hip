@@ -144,7 +144,20 @@ int32_t adc2g_davidson(adc2g_handle* h, const adc2g_dav_params* p, double* eigva
   return guarded([&] {
     ARG_CHECK(h && p && eigval, "NULL argument");
     HIP_CHECK(hipSetDevice(h->h.device));
+    ARG_CHECK(p->guess != ADC2G_GUESS_VECTORS, "guess = ADC2G_GUESS_VECTORS needs adc2g_davidson_vectors");
     davidson_run(h->h, *p, eigval, eigvec, stats, nullptr);
+  });
+}
+
+int32_t adc2g_davidson_vectors(adc2g_handle* h, const adc2g_dav_params* p, const double* guess, double* eigval,
+                               double* eigvec, adc2g_dav_stats* stats) {
+  return guarded([&] {
+    ARG_CHECK(h && p && guess && eigval, "NULL argument");
+    ARG_CHECK(p->blocksize > 0, "blocksize must be given with an initial block");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    adc2g_dav_params q = *p;
+    q.guess = ADC2G_GUESS_VECTORS;
+    davidson_run(h->h, q, eigval, eigvec, stats, nullptr, guess);
   });
 }
 

@@ -68,6 +68,94 @@ void gemm_nn(Handle& H, int64_t M, int N, int K, const double* A, int64_t lda, c
 
 namespace {
 
+// rows idx[0..nr) of the nc columns of W -> out[r + ldo c]
+__global__ void gather_rows_kernel(int nr, int nc, const int64_t* __restrict__ idx, const double* __restrict__ W,
+                                   int64_t ldw, double* __restrict__ out, int ldo) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nr * nc) return;
+  const int r = (int)(t % nr), c = (int)(t / nr);
+  out[r + (int64_t)ldo * c] = W[idx[r] + ldw * c];
+}
+
+// V[idx[r] + ldv c] = U[r + ldu c] for the nc columns (V zeroed before)
+__global__ void scatter_rows_kernel(int nr, int nc, const int64_t* __restrict__ idx, const double* __restrict__ U,
+                                    int ldu, double* __restrict__ V, int64_t ldv) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nr * nc) return;
+  const int r = (int)(t % nr), c = (int)(t / nr);
+  V[idx[r] + ldv * c] = U[r + (int64_t)ldu * c];
+}
+
+// initvec_subdiag (source/eigen/block_davidson.f90:293-459): the subdim ISs
+// of lowest H_jj (ascending; subdim grows while the next H_jj is within 1e-6
+// of the last, so a degenerate group is not split), the stored matrix on
+// them (diagonal H_jj, off-diagonal elements |h| > 1e-12 taken from the upper
+// triangle i < j as the reference stores them), its lowest bs eigenvectors
+// scattered into V (n x bs, ld n).  The columns come from FP64 sigma-vectors
+// of unit vectors on the selected ISs.
+void subdiag_guess(Handle& H, int subdim, int bs, double* V) {
+  const int64_t n = H.d.ndim;
+  hipStream_t st = H.st;
+  std::vector<double> hii(n);
+  HIP_CHECK(hipMemcpyAsync(hii.data(), H.hii.get(), sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+  std::vector<int64_t> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return hii[a] < hii[b]; });
+  int64_t sd = std::min<int64_t>(subdim, n);
+  while (sd < n && std::fabs(hii[order[sd - 1]] - hii[order[sd]]) < 1e-6) ++sd;
+  ARG_CHECK(sd <= 4096, "the sub-diagonalisation guess is limited to 4096 ISs");
+  ARG_CHECK(bs <= sd, "blocksize exceeds the sub-diagonalisation dimension");
+  const int m = (int)sd;
+  DevBuf<int64_t> idx;
+  idx.alloc(m);
+  HIP_CHECK(hipMemcpyAsync(idx.get(), order.data(), sizeof(int64_t) * m, hipMemcpyHostToDevice, st));
+  const int cb = std::min(m, 128);
+  DevBuf<double> X, W, Hs;
+  X.alloc((size_t)n * cb);
+  W.alloc((size_t)n * cb);
+  Hs.alloc((size_t)m * m);
+  for (int c0 = 0; c0 < m; c0 += cb) {
+    const int nc = std::min(cb, m - c0);
+    HIP_CHECK(hipMemsetAsync(X.get(), 0, sizeof(double) * n * nc, st));
+    launch_unit_vectors(nc, idx.get() + c0, X.get(), n, st);
+    H.sigma(X.get(), n, W.get(), n, nc, Prec::F64, st);
+    const int64_t tot = (int64_t)m * nc;
+    hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, m, nc, idx.get(),
+                       W.get(), n, Hs.get() + (size_t)m * c0, m);
+    HIP_CHECK(hipGetLastError());
+  }
+  std::vector<double> hc((size_t)m * m);
+  HIP_CHECK(hipMemcpyAsync(hc.data(), Hs.get(), sizeof(double) * hc.size(), hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+  // hc[p + m q] = H[order[p], order[q]]; the stored element of a pair is the
+  // one with the smaller full-space index as its row
+  std::vector<double> A((size_t)m * m);
+  for (int q = 0; q < m; ++q)
+    for (int pr = 0; pr < m; ++pr) {
+      double h;
+      if (pr == q) {
+        h = hii[order[pr]];
+      } else {
+        h = order[pr] < order[q] ? hc[pr + (size_t)m * q] : hc[q + (size_t)m * pr];
+        if (std::fabs(h) <= 1e-12) h = 0.0;
+      }
+      A[pr + (size_t)m * q] = h;
+    }
+  std::vector<double> w(m), U((size_t)m * m);
+  if (!host_syev(m, A.data(), w.data(), U.data()))
+    throw Error(ADC2G_EEIG, "the diagonalisation of the subspace Hamiltonian failed");
+  DevBuf<double> Ud;
+  Ud.alloc((size_t)m * bs);
+  HIP_CHECK(hipMemcpyAsync(Ud.get(), U.data(), sizeof(double) * m * bs, hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipMemsetAsync(V, 0, sizeof(double) * n * bs, st));
+  const int64_t tot = (int64_t)m * bs;
+  hipLaunchKernelGGL(scatter_rows_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, m, bs, idx.get(),
+                     Ud.get(), m, V, n);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(st));
+}
+
 struct Dav {
   Handle& H;
   int64_t n;
@@ -196,7 +284,7 @@ struct Dav {
 }  // namespace
 
 void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* eigvec, adc2g_dav_stats* stats,
-                  DavTrace* trace) {
+                  DavTrace* trace, const double* guess) {
   const auto t0 = std::chrono::steady_clock::now();
   ARG_CHECK(p.nstates >= 1, "nstates must be >= 1");
   ARG_CHECK(p.tol > 0, "tol must be > 0");
@@ -224,8 +312,26 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
   const bool refine = lowphase ? (p.refine != 0) : true;
   const double lowtol = p.lowprec_tol > 0 ? p.lowprec_tol : 1e-4;
 
-  // initvec_ondiag: unit vectors on the bs smallest |H_jj| (stable order for ties)
-  {
+  ARG_CHECK(p.guess >= ADC2G_GUESS_ONDIAG && p.guess <= ADC2G_GUESS_VECTORS, "guess must be 0, 1 or 2");
+  ARG_CHECK((p.guess == ADC2G_GUESS_VECTORS) == (guess != nullptr), "initial block missing or unexpected");
+  if (p.guess == ADC2G_GUESS_VECTORS) {
+    // initvec_adc1: the caller's block (e.g. ADC(1) eigenvectors)
+    HIP_CHECK(hipMemcpyAsync(D.V.get(), guess, sizeof(double) * n * bs, hipMemcpyHostToDevice, st));
+    DevBuf<double> G;
+    G.alloc((size_t)bs * bs);
+    gemm_tn(H, bs, bs, n, D.V.get(), n, D.V.get(), n, G.get(), bs, 1.0, 0.0);
+    std::vector<double> g((size_t)bs * bs);
+    HIP_CHECK(hipMemcpyAsync(g.data(), G.get(), sizeof(double) * g.size(), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    double dev = 0.0;
+    for (int j = 0; j < bs; ++j)
+      for (int i = 0; i < bs; ++i) dev = std::max(dev, std::fabs(g[i + (size_t)bs * j] - (i == j ? 1.0 : 0.0)));
+    ARG_CHECK(dev <= 1e-8, "the initial block must have orthonormal columns (max |G - I| = " + std::to_string(dev) +
+                               ")");
+  } else if (p.guess == ADC2G_GUESS_SUBDIAG) {
+    subdiag_guess(H, p.subdim > 0 ? p.subdim : 800, bs, D.V.get());
+  } else {
+    // initvec_ondiag: unit vectors on the bs smallest |H_jj| (stable order for ties)
     std::vector<double> hii(n);
     HIP_CHECK(hipMemcpyAsync(hii.data(), H.hii.get(), sizeof(double) * n, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));

@@ -19,8 +19,9 @@ void gemm_nn(Handle& H, int64_t M, int N, int K, const double* A, int64_t lda, c
 void matxvec_treal_host(Handle& H, const double* v1, double* v2, Prec p);
 
 struct DavTrace;  // per-iteration log (harvest mode), may be NULL
+// guess: the caller's initial block when p.guess == ADC2G_GUESS_VECTORS (else NULL)
 void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* eigvec, adc2g_dav_stats* stats,
-                  DavTrace* trace);
+                  DavTrace* trace, const double* guess = nullptr);
 
 void sil_run(Handle& H, const adc2g_sil_params& p, const double* fvec, double* auto0, int64_t nrec_cap,
              int64_t* nrec_out, adc2g_sil_stats* stats);

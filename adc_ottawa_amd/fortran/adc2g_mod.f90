@@ -21,6 +21,7 @@ module adc2g_mod
   integer(c_int32_t), parameter :: ADC2G_PREC_FP64 = 0, ADC2G_PREC_FP32 = 1, ADC2G_PREC_BF16 = 2
   integer(c_int32_t), parameter :: ADC2G_PRE_DPR = 1, ADC2G_PRE_OLSEN = 2, ADC2G_PRE_LEARNED = 3
   integer(c_int32_t), parameter :: ADC2G_COUPLING_AUTO = 0, ADC2G_COUPLING_DF = 1, ADC2G_COUPLING_INT = 2
+  integer(c_int32_t), parameter :: ADC2G_GUESS_ONDIAG = 0, ADC2G_GUESS_SUBDIAG = 1, ADC2G_GUESS_VECTORS = 2
   real(c_double), parameter :: adc2g_eh2ev = 27.2113845d0       ! parameters.f90:192
   real(c_double), parameter :: adc2g_au2fs = 1.0d0/41.34137333656d0 ! fvecprop.f90:19
 
@@ -42,8 +43,10 @@ module adc2g_mod
      integer(c_int32_t) :: precision = 0
      integer(c_int32_t) :: refine = 1
      real(c_double)     :: lowprec_tol = 1.0d-4
-     integer(c_int32_t) :: guess = 0
+     integer(c_int32_t) :: guess = 0        ! ADC2G_GUESS_*
      integer(c_int32_t) :: verbose = 0
+     integer(c_int32_t) :: subdim = 800     ! guessdim of the sub-diagonalisation guess
+     integer(c_int32_t) :: reserved = 0
   end type adc2g_dav_params
 
   type, bind(c) :: adc2g_dav_stats
@@ -215,6 +218,17 @@ module adc2g_mod
        integer(c_int64_t), intent(out) :: noffd
        integer(c_int32_t), intent(out) :: nbuf
      end function adc2g_write_hmlt
+
+     ! initial vectors given (initvec_adc1): guess(ndim, blocksize)
+     integer(c_int32_t) function adc2g_davidson_vectors(h, p, guess, eigval, eigvec, stats) &
+          bind(c, name='adc2g_davidson_vectors')
+       import :: c_int32_t, c_double, c_ptr, adc2g_dav_params, adc2g_dav_stats
+       type(c_ptr), value :: h
+       type(adc2g_dav_params), intent(in) :: p
+       real(c_double), intent(in) :: guess(*)
+       real(c_double), intent(out) :: eigval(*), eigvec(*)
+       type(adc2g_dav_stats), intent(out) :: stats
+     end function adc2g_davidson_vectors
 
      integer(c_int32_t) function adc2g_precon_nparams() bind(c, name='adc2g_precon_nparams')
        import :: c_int32_t

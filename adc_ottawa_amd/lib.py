@@ -44,7 +44,11 @@ class DavParams(C.Structure):
     _fields_ = [("nstates", C.c_int32), ("blocksize", C.c_int32), ("maxvec", C.c_int32),
                 ("maxiter", C.c_int32), ("tol", C.c_double), ("precon", C.c_int32),
                 ("deflate", C.c_int32), ("precision", C.c_int32), ("refine", C.c_int32),
-                ("lowprec_tol", C.c_double), ("guess", C.c_int32), ("verbose", C.c_int32)]
+                ("lowprec_tol", C.c_double), ("guess", C.c_int32), ("verbose", C.c_int32),
+                ("subdim", C.c_int32), ("reserved", C.c_int32)]
+
+
+GUESS = {"ondiag": 0, "subdiag": 1, "vectors": 2}
 
 
 class DavStats(C.Structure):
@@ -93,7 +97,7 @@ SYMBOLS = [
     "adc2g_sil_autocorr", "adc2g_precon_harvest", "adc2g_precon_train", "adc2g_precon_nparams",
     "adc2g_precon_get", "adc2g_precon_set", "adc2g_precon_eval", "adc2g_last_error",
     "adc2g_sigma_profile", "adc2g_precon_absorb", "adc2g_coupling_form", "adc2g_cheby_auto",
-    "adc2g_block_lanczos", "adc2g_relax", "adc2g_write_hmlt",
+    "adc2g_block_lanczos", "adc2g_relax", "adc2g_write_hmlt", "adc2g_davidson_vectors",
 ]
 
 _lib = None
@@ -135,6 +139,7 @@ def load(path: str = LIBPATH):
     lib.adc2g_precon_absorb.argtypes = [vp, vp, C.POINTER(i64)]
     lib.adc2g_write_hmlt.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_double, i32, C.POINTER(i64),
                                      C.POINTER(i32)]
+    lib.adc2g_davidson_vectors.argtypes = [vp, C.POINTER(DavParams), vp, vp, vp, C.POINTER(DavStats)]
     lib.adc2g_relax.argtypes = [vp, C.POINTER(RelaxParams), vp, vp, vp, vp, vp, C.POINTER(RelaxStats)]
     lib.adc2g_block_lanczos.argtypes = [vp, C.POINTER(LanczosParams), vp, vp, vp, vp, vp, vp,
                                         C.POINTER(LanczosStats)]
@@ -229,14 +234,29 @@ class Operator:
 
     def davidson(self, nstates, blocksize=0, maxvec=-1, maxiter=100, tol=1e-7, precon="dpr",
                  deflate=False, precision="fp64", refine=True, lowprec_tol=1e-4, vectors=True,
-                 verbose=False):
-        p = DavParams(nstates, blocksize, maxvec, maxiter, tol, PRECON[precon], 1 if deflate else 0,
-                      PREC[precision], 1 if refine else 0, lowprec_tol, 0, 1 if verbose else 0)
+                 verbose=False, guess="ondiag", subdim=800):
+        """Block Davidson.  guess: "ondiag", "subdiag" (with subdim) or an
+        (ndim, blocksize) array of orthonormal initial vectors."""
+        g = None
+        if not isinstance(guess, str):
+            g = np.asfortranarray(np.asarray(guess, dtype=np.float64))
+            if g.shape[0] != self.ndim:
+                raise ValueError("guess must have ndim rows")
+            blocksize = g.shape[1]
+            guess = "vectors"
+        p = DavParams(nstates=nstates, blocksize=blocksize, maxvec=maxvec, maxiter=maxiter, tol=tol,
+                      precon=PRECON[precon], deflate=1 if deflate else 0, precision=PREC[precision],
+                      refine=1 if refine else 0, lowprec_tol=lowprec_tol, guess=GUESS[guess],
+                      verbose=1 if verbose else 0, subdim=subdim)
         ev = np.empty(nstates)
         vec = np.empty((nstates, self.ndim)) if vectors else None
         st = DavStats()
-        check(_lib.adc2g_davidson(self._h, C.byref(p), _ptr(ev),
-                                  _ptr(vec) if vectors else None, C.byref(st)))
+        if g is not None:
+            check(_lib.adc2g_davidson_vectors(self._h, C.byref(p), _ptr(g), _ptr(ev),
+                                              _ptr(vec) if vectors else None, C.byref(st)))
+        else:
+            check(_lib.adc2g_davidson(self._h, C.byref(p), _ptr(ev),
+                                      _ptr(vec) if vectors else None, C.byref(st)))
         stats = {f: getattr(st, f) for f, _ in DavStats._fields_}
         return ev, (vec.T if vectors else None), stats
 
@@ -334,8 +354,8 @@ class Operator:
 
     # ---- learned preconditioner
     def precon_harvest(self, nstates, max_records=1 << 40, **kw):
-        p = DavParams(nstates, kw.get("blocksize", 0), kw.get("maxvec", -1), kw.get("maxiter", 200),
-                      kw.get("tol", 1e-7), 1, 0, 0, 0, 0.0, 0, 0)
+        p = DavParams(nstates=nstates, blocksize=kw.get("blocksize", 0), maxvec=kw.get("maxvec", -1),
+                      maxiter=kw.get("maxiter", 200), tol=kw.get("tol", 1e-7), precon=1)
         added = C.c_int64()
         check(_lib.adc2g_precon_harvest(self._h, C.byref(p), max_records, C.byref(added)))
         return added.value

@@ -186,7 +186,8 @@ def davidson_compare(op, nroots, only=None, blocksize=20, maxvec=200):
     res = {}
     runs = [("fp64_dpr", dict(precon="dpr", precision="fp64")),
             ("bf16_dpr_refined", dict(precon="dpr", precision="bf16", lowprec_tol=5e-2)),
-            ("fp32_dpr_refined", dict(precon="dpr", precision="fp32", lowprec_tol=1e-4))]
+            ("fp32_dpr_refined", dict(precon="dpr", precision="fp32", lowprec_tol=1e-4)),
+            ("fp64_dpr_subdiag_guess", dict(precon="dpr", precision="fp64", guess="subdiag", subdim=800))]
     have_net = os.path.exists(WEIGHTS)
     if have_net:
         op.set_precon_params(np.load(WEIGHTS))

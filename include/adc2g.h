@@ -84,9 +84,18 @@ typedef struct {
   int32_t precision;     /* ADC2G_PREC_* of the sigma-vectors in the first phase */
   int32_t refine;        /* mixed precision: 1 = finish with FP64 sigma-vectors (restarted refinement) */
   double lowprec_tol;    /* residual at which the low-precision phase hands over to FP64 */
-  int32_t guess;         /* 0 = unit vectors on the smallest |H_jj| (initvec_ondiag) */
+  int32_t guess;         /* ADC2G_GUESS_*: initial vectors (davinitialise, block_davidson.f90:149-242) */
   int32_t verbose;       /* 1: print the iteration table to stdout */
+  int32_t subdim;        /* guessdim: ISs of the sub-diagonalisation guess (<= 0: 800) */
+  int32_t reserved;
 } adc2g_dav_params;
+
+/* Davidson initial vectors */
+#define ADC2G_GUESS_ONDIAG 0   /* unit vectors on the smallest |H_jj| (initvec_ondiag, :463-532) */
+#define ADC2G_GUESS_SUBDIAG 1  /* lowest eigenvectors of H on the subdim ISs of lowest H_jj
+                                  (initvec_subdiag, :293-459) */
+#define ADC2G_GUESS_VECTORS 2  /* caller's block, e.g. ADC(1) eigenvectors (initvec_adc1,
+                                  :246-289): adc2g_davidson_vectors */
 
 typedef struct {
   int32_t niter;         /* Davidson iterations */
@@ -207,6 +216,14 @@ int32_t adc2g_matxvec_treal(adc2g_handle* h, const double* v1, double* v2, int32
  * eigvec (ndim x nstates, may be NULL), stats may be NULL. */
 int32_t adc2g_davidson(adc2g_handle* h, const adc2g_dav_params* p, double* eigval,
                        double* eigvec, adc2g_dav_stats* stats);
+
+/* The same with the initial vectors given (p->guess = ADC2G_GUESS_VECTORS):
+ * guess is a host block of ndim x blocksize orthonormal columns (rejected if
+ * max |G^T G - I| > 1e-8), e.g. the ADC(1) eigenvectors that initvec_adc1
+ * (source/eigen/block_davidson.f90:246-289) reads from SCRATCH/adc1_vecs,
+ * zero on the 2h2p configurations; blocksize must be given (> 0). */
+int32_t adc2g_davidson_vectors(adc2g_handle* h, const adc2g_dav_params* p, const double* guess,
+                               double* eigval, double* eigvec, adc2g_dav_stats* stats);
 
 /* SIL wavepacket propagation from fvec (real, ndim): propagate_sillib
  * (source/propagation/fvecprop.f90:365-564) with SILStep

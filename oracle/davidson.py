@@ -7,7 +7,8 @@ Follows the reference step by step (1-based Fortran ranges in the comments):
 * guess ``initvec_ondiag`` (:463-532): unit vectors on the ``blocksize``
   smallest |H_jj| (ascending sort; ties broken by index here, the reference's
   heapsort ``dsortindxa1`` (source/utilities/misc.f90:492-591) leaves tie
-  order unspecified);
+  order unspecified); ``subdiag_guess`` restates ``initvec_subdiag``
+  (:293-459); a given block stands for ``initvec_adc1`` (:246-289);
 * loop ``run_block_davidson`` (:690-774): sigma of all ``currdim`` columns
   (``hxvec_incore`` :844-883, nmult += currdim), Rayleigh matrix (:954-967),
   ``dsyev`` (:971-1008), Ritz vectors (:1012-1025), residuals
@@ -43,6 +44,30 @@ def guard(den):
     den = np.asarray(den, dtype=np.float64)
     small = np.abs(den) < DEN_MIN
     return np.where(small, np.where(den < 0, -DEN_MIN, DEN_MIN), den)
+
+
+def subdiag_guess(hii, hsub, blocksize, subdim=800, minc=1e-12):
+    """initvec_subdiag (source/eigen/block_davidson.f90:293-459).
+
+    The subdim ISs of lowest H_jj (ascending), extended while the next H_jj
+    is within 1e-6 of the last.  ``hsub(sel)`` returns H[sel][:, sel].  The
+    diagonal is taken from hii, and off-diagonal elements |h| <= minc are
+    dropped, as the stored matrix drops them.  The lowest ``blocksize``
+    eigenvectors of that block are scattered into an (ndim, blocksize) array."""
+    hii = np.asarray(hii, dtype=np.float64)
+    n = hii.size
+    order = np.argsort(hii, kind="stable")
+    sd = min(subdim, n)
+    while sd < n and abs(hii[order[sd - 1]] - hii[order[sd]]) < 1e-6:
+        sd += 1
+    sel = order[:sd]
+    A = np.array(hsub(sel), dtype=np.float64)
+    A[np.abs(A) <= minc] = 0.0
+    A[np.arange(sd), np.arange(sd)] = hii[sel]
+    _, U = np.linalg.eigh(A)
+    V = np.zeros((n, blocksize))
+    V[sel, :] = U[:, :blocksize]
+    return V
 
 
 def davidson(sigma, hii, nstates, blocksize=0, maxvec=-1, tol=1e-7, maxiter=100,

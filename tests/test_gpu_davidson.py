@@ -91,3 +91,63 @@ def test_bad_arguments(water):
     with pytest.raises(lib.Adc2gError) as e:
         op.davidson(5, blocksize=3)
     assert e.value.code == 10
+
+
+def test_subdiag_guess_small_full_space(small):
+    """initvec_subdiag with subdim = ndim: the guess is the exact lowest
+    eigenvectors, so the first iteration converges."""
+    sys, df, w, op = small
+    ev, X, st = op.davidson(4, blocksize=4, tol=1e-8, guess="subdiag", subdim=op.ndim)
+    assert st["niter"] == 1
+    assert np.abs(ev - w[:4]).max() < 1e-10
+
+
+@pytest.mark.parametrize("precon", ["dpr", "olsen"])
+def test_water_subdiag_guess_vs_oracle(water, precon):
+    sys, blk, w, op = water
+    hii = blk.diag()
+
+    def hsub(sel):
+        E = np.zeros((blk.ndim, sel.size))
+        E[sel, np.arange(sel.size)] = 1.0
+        return blk.sigma(E)[sel, :]
+
+    ev, X, st = op.davidson(5, blocksize=5, maxvec=40, tol=1e-7, precon=precon, guess="subdiag",
+                            subdim=200, maxiter=500)
+    print("subdiag guess: per-root |E - E_arpack|", np.abs(ev - w), "iterations", st["niter"])
+    assert np.abs(ev - w).max() < 1e-8
+    assert residual_norms(blk.sigma, ev, X).max() < 1e-7 * 1.01
+    g = odav.subdiag_guess(hii, hsub, 5, subdim=200)
+    ref = odav.davidson(blk.sigma, hii, 5, blocksize=5, maxvec=40, tol=1e-7,
+                        precon=1 if precon == "dpr" else 2, deflate=False, maxiter=500, guess=g)
+    assert np.abs(ref["eigval"] - ev).max() < 1e-8
+    assert abs(ref["niter"] - st["niter"]) <= 3
+
+
+def test_water_adc1_guess_vs_oracle(water):
+    """Initial vectors given by the caller (initvec_adc1): the ADC(1)
+    eigenvectors, from the first-order ph-ph block, zero on the 2h2p part."""
+    sys, blk, w, op = water
+    b1 = elements.build_blocks(sys, second_order=False)
+    _, U = np.linalg.eigh(b1.H11)
+    g = np.zeros((blk.ndim, 5))
+    g[: U.shape[0], :] = U[:, :5]
+    ev, X, st = op.davidson(5, maxvec=40, tol=1e-7, precon="dpr", guess=g, maxiter=500)
+    print("ADC(1) guess: per-root |E - E_arpack|", np.abs(ev - w), "iterations", st["niter"])
+    assert np.abs(ev - w).max() < 1e-8
+    ref = odav.davidson(blk.sigma, blk.diag(), 5, blocksize=5, maxvec=40, tol=1e-7, precon=1,
+                        deflate=False, maxiter=500, guess=g)
+    assert np.abs(ref["eigval"] - ev).max() < 1e-8
+    assert abs(ref["niter"] - st["niter"]) <= 3
+
+
+def test_guess_errors(water):
+    sys, blk, w, op = water
+    g = np.zeros((blk.ndim, 5))
+    g[:5, :] = 2.0 * np.eye(5)                     # not orthonormal
+    with pytest.raises(lib.Adc2gError) as e:
+        op.davidson(5, maxvec=40, guess=g)
+    assert e.value.code == 10
+    with pytest.raises(lib.Adc2gError) as e:
+        op.davidson(5, blocksize=5, maxvec=40, guess="subdiag", subdim=3)   # blocksize > subdim
+    assert e.value.code == 10

@@ -216,6 +216,35 @@ def test_oracle_block_lanczos_full_krylov():
     assert np.abs(H @ R - R * out["eigval"]).max() < 1e-8 * np.abs(H).max()
 
 
+def test_oracle_subdiag_guess():
+    """initvec_subdiag: with subdim = ndim the guess vectors are the exact
+    lowest eigenvectors, so the Davidson converges in the first iteration;
+    a tie at the boundary extends the selection over the whole degenerate
+    group; the vectors live on the ISs of lowest H_jj."""
+    sys = make_system(3, 8, 10, seed=4)
+    H = elements.build_blocks(sys, apply_minc=True).dense()
+    hii = np.diag(H).copy()
+    n = H.shape[0]
+    hs = lambda sel: H[np.ix_(sel, sel)]
+    lam, U = np.linalg.eigh(H)
+    V = davidson.subdiag_guess(hii, hs, 4, subdim=n)
+    for c in range(4):
+        assert min(np.abs(V[:, c] - U[:, c]).max(), np.abs(V[:, c] + U[:, c]).max()) < 1e-10
+    out = davidson.davidson(lambda X: H @ X, hii, 4, blocksize=4, tol=1e-8, guess=V)
+    assert out["niter"] == 1 and np.abs(out["eigval"] - lam[:4]).max() < 1e-12
+    # support: the 30 lowest H_jj
+    V = davidson.subdiag_guess(hii, hs, 4, subdim=30)
+    sel = np.argsort(hii, kind="stable")[:30]
+    assert np.abs(np.delete(V, sel, axis=0)).max() == 0.0
+    # a degenerate group at the boundary is kept whole
+    h2 = hii.copy()
+    o = np.argsort(h2, kind="stable")
+    h2[o[30]] = h2[o[29]] + 1e-8
+    V = davidson.subdiag_guess(h2, hs, 4, subdim=30)
+    assert np.abs(V[o[30]]).max() > 0.0
+    assert np.abs(np.delete(V, o[:31], axis=0)).max() == 0.0
+
+
 def test_oracle_relax_steps_are_imaginary_time_propagation():
     """Each XSIL timestep propagates psi to exp(-H dtau) psi (normalised), up to
     the Krylov truncation error, which is negligible here (dtau * spread ~ 1,
