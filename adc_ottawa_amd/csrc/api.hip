@@ -170,6 +170,16 @@ int32_t adc2g_sil_autocorr(adc2g_handle* h, const adc2g_sil_params* p, const dou
   });
 }
 
+int32_t adc2g_sil_autocorr_orders(adc2g_handle* h, const adc2g_sil_params* p, const double* fvec, int32_t autoord,
+                                  double* auto0, double* auto1, double* auto2, int64_t nrec_cap, int64_t* nrec_out,
+                                  adc2g_sil_stats* stats) {
+  return guarded([&] {
+    ARG_CHECK(h && p && fvec && auto0, "NULL argument");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    sil_run(h->h, *p, fvec, auto0, nrec_cap, nrec_out, stats, autoord, auto1, auto2);
+  });
+}
+
 int32_t adc2g_precon_harvest(adc2g_handle* h, const adc2g_dav_params* p, int64_t max_records, int64_t* added) {
   return guarded([&] {
     ARG_CHECK(h && p, "NULL argument");

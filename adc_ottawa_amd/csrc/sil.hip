@@ -113,6 +113,24 @@ __global__ void square_sum_kernel(int64_t n, const double* __restrict__ x, doubl
   }
 }
 
+// part[blk] = partial sum_k a_k b_k (complex, unconjugated; planar re/im)
+__global__ void bilinear_kernel(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                                double* __restrict__ part) {
+  __shared__ double red[16];
+  double sr = 0.0, si = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double a = x[i], b = x[n + i], c = y[i], d = y[n + i];
+    sr += a * c - b * d;
+    si += a * d + b * c;
+  }
+  sr = block_sum2(sr, red);
+  si = block_sum2(si, red);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sr;
+    part[2 * blockIdx.x + 1] = si;
+  }
+}
+
 // coef_p = sum_q exp(-i lam_q dt) U[0,q] U[p,q]   (one thread per p)
 __global__ void expcoef_kernel(int np, const double* __restrict__ lam, const double* __restrict__ U, int ldu,
                                double dt, double* __restrict__ coef) {
@@ -172,6 +190,22 @@ struct Sil {
     HIP_CHECK(hipMemcpyAsync(h, sums.get(), sizeof(h), hipMemcpyDeviceToHost, H.st));
     HIP_CHECK(hipStreamSynchronize(H.st));
     return cplx(h[0], h[1]);
+  }
+
+  cplx bilinear(const double* a, const double* b) {
+    hipLaunchKernelGGL(bilinear_kernel, dim3(nb), dim3(256), 0, H.st, n, a, b, part.get());
+    HIP_CHECK(hipGetLastError());
+    launch_colsum(nb, 2, part.get(), sums.get(), H.st);
+    double h[2];
+    HIP_CHECK(hipMemcpyAsync(h, sums.get(), sizeof(h), hipMemcpyDeviceToHost, H.st));
+    HIP_CHECK(hipStreamSynchronize(H.st));
+    return cplx(h[0], h[1]);
+  }
+
+  // out = H v (complex, planar): the i * matxvec_treal of the reference
+  void hmul(const double* v, double* out) {
+    H.sigma(v, n, out, n, 2, prec, H.st);
+    ++nsigma;
   }
 
   cplx square_sum(const double* a) {
@@ -272,7 +306,7 @@ struct Sil {
 }  // namespace
 
 void sil_run(Handle& H, const adc2g_sil_params& p, const double* fvec, double* auto0, int64_t nrec_cap,
-             int64_t* nrec_out, adc2g_sil_stats* stats) {
+             int64_t* nrec_out, adc2g_sil_stats* stats, int autoord, double* auto1, double* auto2) {
   const auto t0 = std::chrono::steady_clock::now();
   ARG_CHECK(p.tout > 0 && p.tfinal >= 0, "tout must be > 0 and tfinal >= 0");
   ARG_CHECK(p.kdim >= 1 && p.kdim <= 30, "krydim must be in 1..30");
@@ -281,7 +315,12 @@ void sil_run(Handle& H, const adc2g_sil_params& p, const double* fvec, double* a
   const double intperiod = 0.5 * p.tout;
   const int64_t nrec = (int64_t)(p.tfinal / intperiod);
   ARG_CHECK(nrec_cap >= nrec + 1, "auto0 holds fewer than nrec+1 records");
+  ARG_CHECK(autoord >= 0 && autoord <= 2, "autoord must be 0, 1 or 2");
+  ARG_CHECK((autoord < 1 || auto1) && (autoord < 2 || auto2), "missing output array for the requested order");
   Sil S(H, p.kdim, prec_from_int(p.precision));
+  DevBuf<double> hpsi, h2psi;
+  if (autoord >= 1) hpsi.alloc(2 * (size_t)S.n);
+  if (autoord >= 2) h2psi.alloc(2 * (size_t)S.n);
   const int64_t n = S.n;
   // psi0 = F / |F| (complex, imaginary part zero)
   std::vector<double> f(fvec, fvec + n);
@@ -297,6 +336,20 @@ void sil_run(Handle& H, const adc2g_sil_params& p, const double* fvec, double* a
   const cplx a00 = S.cdot(S.Kp(0), S.Kp(0)) * norm2;
   auto0[0] = a00.real();
   auto0[1] = a00.imag();
+  // first and second order at t = 0: dot_product(psi0, H^k psi0) |F|^2
+  // (propagate_sillib, fvecprop.f90:456-471)
+  if (autoord >= 1) {
+    S.hmul(S.Kp(0), hpsi.get());
+    const cplx a = S.cdot(S.Kp(0), hpsi.get()) * norm2;
+    auto1[0] = a.real();
+    auto1[1] = a.imag();
+  }
+  if (autoord >= 2) {
+    S.hmul(hpsi.get(), h2psi.get());
+    const cplx a = S.cdot(S.Kp(0), h2psi.get()) * norm2;
+    auto2[0] = a.real();
+    auto2[1] = a.imag();
+  }
   const double tiny = 1e-9;
   int64_t nsteps = 0;
   int maxorder = 0;
@@ -315,6 +368,19 @@ void sil_run(Handle& H, const adc2g_sil_params& p, const double* fvec, double* a
     const cplx a = S.square_sum(S.Kp(0)) * norm2;
     auto0[2 * i] = a.real();
     auto0[2 * i + 1] = a.imag();
+    // orders 1 and 2 with the t/2 trick: psi^T H^k psi |F|^2 (fvecprop.f90:526-541)
+    if (autoord >= 1) {
+      S.hmul(S.Kp(0), hpsi.get());
+      const cplx a1 = S.bilinear(S.Kp(0), hpsi.get()) * norm2;
+      auto1[2 * i] = a1.real();
+      auto1[2 * i + 1] = a1.imag();
+    }
+    if (autoord >= 2) {
+      S.hmul(hpsi.get(), h2psi.get());
+      const cplx a2 = S.bilinear(S.Kp(0), h2psi.get()) * norm2;
+      auto2[2 * i] = a2.real();
+      auto2[2 * i + 1] = a2.imag();
+    }
     if (p.verbose && (i % 100 == 0 || i == nrec)) {
       std::fprintf(stderr, "sil: record %lld of %lld (t = %.2f a.u.), %lld steps, %lld sigma-vectors\n", (long long)i,
                    (long long)nrec, i * intperiod, (long long)nsteps, (long long)S.nsigma);

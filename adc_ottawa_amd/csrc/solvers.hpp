@@ -23,8 +23,10 @@ struct DavTrace;  // per-iteration log (harvest mode), may be NULL
 void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* eigvec, adc2g_dav_stats* stats,
                   DavTrace* trace, const double* guess = nullptr);
 
+// autoord 1, 2 also fill auto1 / auto2 (nrec+1 complex values each)
 void sil_run(Handle& H, const adc2g_sil_params& p, const double* fvec, double* auto0, int64_t nrec_cap,
-             int64_t* nrec_out, adc2g_sil_stats* stats);
+             int64_t* nrec_out, adc2g_sil_stats* stats, int autoord = 0, double* auto1 = nullptr,
+             double* auto2 = nullptr);
 
 // Chebyshev order-domain autocorrelation C_0..C_2K (auto_out, host) of the
 // host vector q0 with the spectrum scaled by bounds[0..1] = (Emin, Emax).

@@ -173,6 +173,21 @@ module adc2g_mod
        type(adc2g_sil_stats), intent(out) :: stats
      end function adc2g_sil_autocorr
 
+     ! auto1, auto2: c_loc of complex(c_double_complex) arrays (nrec+1) or c_null_ptr
+     integer(c_int32_t) function adc2g_sil_autocorr_orders(h, p, fvec, autoord, auto0, auto1, auto2, &
+          nrec_cap, nrec_out, stats) bind(c, name='adc2g_sil_autocorr_orders')
+       import :: c_int32_t, c_int64_t, c_double, c_double_complex, c_ptr, adc2g_sil_params, adc2g_sil_stats
+       type(c_ptr), value :: h
+       type(adc2g_sil_params), intent(in) :: p
+       real(c_double), intent(in) :: fvec(*)
+       integer(c_int32_t), value :: autoord
+       complex(c_double_complex), intent(out) :: auto0(*)
+       type(c_ptr), value :: auto1, auto2
+       integer(c_int64_t), value :: nrec_cap
+       integer(c_int64_t), intent(out) :: nrec_out
+       type(adc2g_sil_stats), intent(out) :: stats
+     end function adc2g_sil_autocorr_orders
+
      integer(c_int32_t) function adc2g_cheby_auto(h, bounds, q0, order, precision, auto_out) &
           bind(c, name='adc2g_cheby_auto')
        import :: c_int32_t, c_double, c_ptr

@@ -98,6 +98,7 @@ SYMBOLS = [
     "adc2g_precon_get", "adc2g_precon_set", "adc2g_precon_eval", "adc2g_last_error",
     "adc2g_sigma_profile", "adc2g_precon_absorb", "adc2g_coupling_form", "adc2g_cheby_auto",
     "adc2g_block_lanczos", "adc2g_relax", "adc2g_write_hmlt", "adc2g_davidson_vectors",
+    "adc2g_sil_autocorr_orders",
 ]
 
 _lib = None
@@ -140,6 +141,8 @@ def load(path: str = LIBPATH):
     lib.adc2g_write_hmlt.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_double, i32, C.POINTER(i64),
                                      C.POINTER(i32)]
     lib.adc2g_davidson_vectors.argtypes = [vp, C.POINTER(DavParams), vp, vp, vp, C.POINTER(DavStats)]
+    lib.adc2g_sil_autocorr_orders.argtypes = [vp, C.POINTER(SilParams), vp, i32, vp, vp, vp, i64,
+                                              C.POINTER(i64), C.POINTER(SilStats)]
     lib.adc2g_relax.argtypes = [vp, C.POINTER(RelaxParams), vp, vp, vp, vp, vp, C.POINTER(RelaxStats)]
     lib.adc2g_block_lanczos.argtypes = [vp, C.POINTER(LanczosParams), vp, vp, vp, vp, vp, vp,
                                         C.POINTER(LanczosStats)]
@@ -271,6 +274,22 @@ class Operator:
                                       C.byref(n), C.byref(st)))
         stats = {f: getattr(st, f) for f, _ in SilStats._fields_}
         return out[: n.value], stats
+
+    def sil_autocorr_orders(self, fvec, tout, tfinal, kdim, autotol, autoord, precision="fp64",
+                            verbose=False):
+        """SIL autocorrelation functions of orders 0..autoord; returns
+        ([auto0, auto1, ...], stats)."""
+        fvec = np.ascontiguousarray(fvec, dtype=np.float64)
+        nrec = int(tfinal / (0.5 * tout))
+        outs = [np.empty(nrec + 1, dtype=np.complex128) for _ in range(autoord + 1)]
+        ptrs = [_ptr(o) for o in outs] + [None] * (2 - autoord)
+        p = SilParams(tout, tfinal, kdim, PREC[precision], autotol, 1 if verbose else 0, 0)
+        n = C.c_int64()
+        st = SilStats()
+        check(_lib.adc2g_sil_autocorr_orders(self._h, C.byref(p), _ptr(fvec), int(autoord), ptrs[0], ptrs[1],
+                                             ptrs[2], nrec + 1, C.byref(n), C.byref(st)))
+        stats = {f: getattr(st, f) for f, _ in SilStats._fields_}
+        return [o[: n.value] for o in outs], stats
 
     def cheby_auto(self, q0, bounds, order, precision="fp64"):
         """Chebyshev order-domain autocorrelation C_0..C_2K (K = order) of q0

@@ -234,6 +234,17 @@ int32_t adc2g_sil_autocorr(adc2g_handle* h, const adc2g_sil_params* p, const dou
                            double* auto0, int64_t nrec_cap, int64_t* nrec_out,
                            adc2g_sil_stats* stats);
 
+/* The same with the higher-order autocorrelation functions of autoord
+ * (propagate_sillib, source/propagation/fvecprop.f90:421-541): for
+ * autoord >= 1, auto1 receives psi(t/2)^T H psi(t/2) |F|^2 (dot_product(psi0, H psi0) |F|^2
+ * at t = 0), and for autoord = 2, auto2 receives the same with H^2; one
+ * (autoord 1) or two (autoord 2) more complex sigma-vectors per record.
+ * auto1 / auto2 hold nrec+1 complex values each and may be NULL when not
+ * requested; autoord = 0 is adc2g_sil_autocorr. */
+int32_t adc2g_sil_autocorr_orders(adc2g_handle* h, const adc2g_sil_params* p, const double* fvec,
+                                  int32_t autoord, double* auto0, double* auto1, double* auto2,
+                                  int64_t nrec_cap, int64_t* nrec_out, adc2g_sil_stats* stats);
+
 /* Chebyshev order-domain autocorrelation function (chebyshev_auto,
  * source/propagation/chebyspec.f90:245-319, with the scaled product of
  * matxvec_chebyshev_incore, source/propagation/tdselib.f90:1316-1398):

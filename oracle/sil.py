@@ -101,8 +101,14 @@ def silstep(psi, dtpsi, func, intperiod, intorder, tolerror):
     return out, stepsize, trueorder
 
 
-def propagate(func, fvec, tout, tfinal, kdim, autotol, record_orders=False):
-    """Return (times, auto0) with auto0[0] at t=0 and auto0[i] at i*tout."""
+def propagate(func, fvec, tout, tfinal, kdim, autotol, record_orders=False, autoord=0):
+    """Return (times, auto0) with auto0[0] at t=0 and auto0[i] at i*tout.
+
+    With autoord = 1 or 2, auto0 is replaced by the list [auto0, auto1, ...]
+    of propagate_sillib (source/propagation/fvecprop.f90:421-541):
+    auto_k[0] = dot_product(psi0, H^k psi0) |F|^2 (conjugating psi0) and
+    auto_k[i] = sum(psi H^k psi) |F|^2 (no conjugation: the t/2 trick), with
+    H v = i func(v)."""
     norm0 = np.sqrt(np.vdot(fvec, fvec).real)
     psi = fvec.astype(np.complex128) / norm0
     intperiod = 0.5 * tout
@@ -110,6 +116,11 @@ def propagate(func, fvec, tout, tfinal, kdim, autotol, record_orders=False):
     nrec = int(tfinal / intperiod)
     auto = np.empty(nrec + 1, dtype=np.complex128)
     auto[0] = np.vdot(psi, psi) * norm0 ** 2
+    higher = [np.empty(nrec + 1, dtype=np.complex128) for _ in range(autoord)]
+    hk = psi
+    for k in range(autoord):
+        hk = 1j * func(hk)
+        higher[k][0] = np.vdot(psi, hk) * norm0 ** 2
     orders = []
     for i in range(1, nrec + 1):
         inttime = 0.0
@@ -122,7 +133,13 @@ def propagate(func, fvec, tout, tfinal, kdim, autotol, record_orders=False):
             if not (abs(intperiod - inttime) > abs(tiny * intperiod)):
                 break
         auto[i] = np.sum(psi * psi) * norm0 ** 2
+        hk = psi
+        for k in range(autoord):
+            hk = 1j * func(hk)
+            higher[k][i] = np.sum(psi * hk) * norm0 ** 2
     times = np.arange(nrec + 1) * tout
+    if autoord:
+        auto = [auto] + higher
     if record_orders:
         return times, auto, orders
     return times, auto

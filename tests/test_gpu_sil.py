@@ -77,3 +77,24 @@ def test_sil_bad_order(small):
     with pytest.raises(lib.Adc2gError) as e:
         op.sil_autocorr(f, tout=0.1, tfinal=0.2, kdim=1, autotol=1e-8)
     assert e.value.code == 1
+
+
+@pytest.mark.parametrize("autoord", [1, 2])
+def test_sil_higher_orders_vs_oracle_and_exact(small, autoord):
+    """Orders 1 and 2 of propagate_sillib against the oracle (1e-10 relative
+    to sum c_n^2 |E_n|^k) and the closed form sum_n c_n^2 E_n^k exp(-i E_n t)."""
+    sys, H, op, f = small
+    autos, st = op.sil_autocorr_orders(f, tout=0.1, tfinal=3.0, kdim=12, autotol=1e-12, autoord=autoord)
+    times, ref = sil.propagate(lambda x: -1j * (H @ x), f, tout=0.1, tfinal=3.0, kdim=12, autotol=1e-12,
+                               autoord=autoord)
+    w, U = np.linalg.eigh(H)
+    c = U.T @ f
+    assert len(autos) == autoord + 1
+    for k in range(autoord + 1):
+        scale = np.sum(c * c * np.abs(w) ** k)
+        assert np.abs(autos[k] - ref[k]).max() < 1e-10 * scale, k
+        exact = np.array([np.sum(c * c * w ** k * np.exp(-1j * w * t)) for t in times])
+        assert np.abs(autos[k] - exact).max() < 1e-8 * scale, k
+    # the zeroth order equals the plain entry point's
+    auto0, _ = op.sil_autocorr(f, tout=0.1, tfinal=3.0, kdim=12, autotol=1e-12)
+    assert np.abs(autos[0] - auto0).max() == 0.0

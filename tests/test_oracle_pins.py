@@ -144,6 +144,25 @@ def test_oracle_sil_exact_propagator():
     assert abs(auto[0] - np.dot(f, f)) < 1e-14
 
 
+def test_oracle_sil_higher_orders_exact():
+    """Autocorrelation functions of orders 1 and 2 against the closed form
+    sum_n c_n^2 E_n^k exp(-i E_n t) (the t/2 trick holds for H^k because H is
+    real symmetric); a missing factor i, a conjugated bilinear form or a
+    wrong power fails it."""
+    sys = make_system(3, 5, 6, seed=5)
+    df = dfsigma.DF(sys)
+    H = df.sigma(np.eye(df.n1 + df.D2.size))
+    w, U = np.linalg.eigh(H)
+    f = np.zeros(H.shape[0])
+    f[: df.n1] = np.random.default_rng(0).normal(0, 0.1, df.n1)
+    times, autos = sil.propagate(lambda x: -1j * (H @ x), f, tout=0.1, tfinal=4.0, kdim=12,
+                                 autotol=1e-12, autoord=2)
+    c = U.T @ f
+    for k in (0, 1, 2):
+        exact = np.array([np.sum(c * c * w ** k * np.exp(-1j * w * t)) for t in times])
+        assert np.abs(autos[k] - exact).max() < 1e-9 * np.sum(c * c * np.abs(w) ** k), k
+
+
 def test_oracle_chebyshev_closed_form():
     """The Chebyshev recursion restatement against sum_n (v_n.q0)^2 T_k(x_n)
     from a dense eigendecomposition, on every order including the doubled ones
