@@ -13,6 +13,17 @@
 !   &adc2g nocc=5 nvirt=36 naux=164 eps_file='eps.bin' b_file='B.bin'
 !          nstates=5 blocksize=5 maxvec=40 tol=1d-7 precon=1 davname='davstates'
 !          sil=.true. fvec_file='f.bin' tout=0.1d0 tfinal=2d0 krydim=20 autotol=1d-10 /
+! Further keys:
+!   solver = 1 Davidson (davdiag_block) | 2 relaxation (XSIL, relaxation.f90:36-133)
+!            with stepsize, rlx_krydim and the same nstates / maxiter / tol;
+!   guess = 0 unit vectors | 1 sub-diagonalisation (subdim ISs) for Davidson;
+!   autoord = 0..2: SIL autocorrelation orders, files auto_file, 'auto1', 'auto2';
+!   lanczos = .true.: block Lanczos (lancdiag_block) with lmain unit vectors on
+!            the 1h1p entries of largest |fvec| and ncycles blocks, Lanczos states
+!            written to lancname, transition moments fvec.r_n printed;
+!   chebyord > 0: Chebyshev autocorrelation of fvec with the bounds
+!            cheby_emin / cheby_emax, file 'chebyauto';
+!   hmlt = .true.: the matrix files 'hmlt.diai' / 'hmlt.offi'.
 !=======================================================================
 program adc2g_driver
 
@@ -22,12 +33,17 @@ program adc2g_driver
 
   integer :: nocc = 0, nvirt = 0, naux = 0, nstates = 5, blocksize = 0, maxvec = -1, maxiter = 300
   integer :: precon = 2, precision = 0, krydim = 10
-  logical :: deflate = .true., sil = .false.
+  integer :: solver = 1, guess = 0, subdim = 800, rlx_krydim = 10, autoord = 0
+  integer :: lmain = 0, ncycles = 0, chebyord = 0
+  logical :: deflate = .true., sil = .false., lanczos = .false., hmlt = .false.
   real(c_double) :: tol = 1.0d-7, tout = 0.1d0, tfinal = 1.0d0, autotol = 1.0d-5
+  real(c_double) :: stepsize = 10.0d0, cheby_emin = 0.0d0, cheby_emax = 1.0d0
   character(len=256) :: eps_file = 'eps.bin', b_file = 'B.bin', davname = 'davstates'
-  character(len=256) :: fvec_file = 'fvec.bin', auto_file = 'auto'
+  character(len=256) :: fvec_file = 'fvec.bin', auto_file = 'auto', lancname = 'lancstates'
   namelist /adc2g/ nocc, nvirt, naux, eps_file, b_file, nstates, blocksize, maxvec, maxiter, tol, &
-       precon, precision, deflate, davname, sil, fvec_file, auto_file, tout, tfinal, krydim, autotol
+       precon, precision, deflate, davname, sil, fvec_file, auto_file, tout, tfinal, krydim, autotol, &
+       solver, guess, subdim, stepsize, rlx_krydim, autoord, lanczos, lmain, ncycles, lancname, &
+       chebyord, cheby_emin, cheby_emax, hmlt
 
   type(c_ptr) :: h
   type(adc2g_opts) :: opts
@@ -35,10 +51,19 @@ program adc2g_driver
   type(adc2g_dav_stats) :: ds
   type(adc2g_sil_params) :: sp
   type(adc2g_sil_stats) :: ss
-  real(c_double), allocatable :: eps(:), B(:), eigval(:), eigvec(:,:), fvec(:)
+  type(adc2g_relax_params) :: rp
+  type(adc2g_relax_stats) :: rs
+  type(adc2g_lanczos_params) :: lp
+  type(adc2g_lanczos_stats) :: ls
+  real(c_double), allocatable :: eps(:), B(:), eigval(:), fvec(:), q0(:,:), cauto(:)
+  real(c_double), allocatable, target :: eigvec(:,:), lener(:), ritz(:,:), tmom(:), fvec_t(:)
   complex(c_double_complex), allocatable :: auto0(:)
-  integer(c_int64_t) :: ndim, n1, counts(5), nrec, nout
-  integer :: nbas, rc, i, u
+  complex(c_double_complex), allocatable, target :: auto1(:), auto2(:)
+  integer(c_int64_t) :: ndim, n1, counts(5), nrec, nout, noffd
+  integer(c_int32_t) :: nbuf
+  integer :: nbas, rc, i, u, nvec, k
+  logical, allocatable :: used(:)
+  real(c_double) :: bounds(2)
 
   read(*, nml=adc2g)
   nbas = nocc + nvirt
@@ -57,39 +82,105 @@ program adc2g_driver
   call check(rc)
   write(*, '(a,i12,a,i8,a,5i10)') ' ADC(2)-s space: ndim =', ndim, '  1h1p =', n1, '  2h2p types =', counts
 
-  dp%nstates = nstates
-  dp%blocksize = blocksize
-  dp%maxvec = maxvec
-  dp%maxiter = maxiter
-  dp%tol = tol
-  dp%precon = precon
-  dp%deflate = merge(1, 0, deflate)
-  dp%precision = precision
   allocate(eigval(nstates), eigvec(ndim, nstates))
-  rc = adc2g_davidson(h, dp, eigval, eigvec, ds)
-  call check(rc)
-  write(*, '(a,i6,a,i8,a,f10.3,a)') ' Davidson: iterations', ds%niter, ', sigma-vectors', ds%nsigma, &
-       ', time', ds%time_total, ' s'
+  if (solver == 2) then
+     ! master_eig with solver = 2: relaxation (adc_common.f90:952-993)
+     rp%nstates = nstates
+     rp%krydim = rlx_krydim
+     rp%maxdim = -1
+     rp%maxiter = maxiter
+     rp%step = stepsize
+     rp%tol = tol
+     rp%precision = precision
+     rc = adc2g_relax(h, rp, c_null_ptr, eigval, c_loc(eigvec), c_null_ptr, c_null_ptr, rs)
+     call check(rc)
+     write(*, '(a,i8,a,f10.3,a)') ' Relaxation: sigma-vectors', rs%nsigma, ', time', rs%time_total, ' s'
+  else
+     dp%nstates = nstates
+     dp%blocksize = blocksize
+     dp%maxvec = maxvec
+     dp%maxiter = maxiter
+     dp%tol = tol
+     dp%precon = precon
+     dp%deflate = merge(1, 0, deflate)
+     dp%precision = precision
+     dp%guess = guess
+     dp%subdim = subdim
+     rc = adc2g_davidson(h, dp, eigval, eigvec, ds)
+     call check(rc)
+     write(*, '(a,i6,a,i8,a,f10.3,a)') ' Davidson: iterations', ds%niter, ', sigma-vectors', ds%nsigma, &
+          ', time', ds%time_total, ' s'
+  end if
   do i = 1, nstates
      write(*, '(i4,2x,f18.12,2x,f14.7)') i, eigval(i), eigval(i) * adc2g_eh2ev
   end do
   call adc2g_write_davstates(trim(davname), ndim, nstates, eigval, eigvec)
 
-  if (sil) then
+  if (hmlt) then
+     rc = adc2g_write_hmlt(h, 'hmlt.diai'//c_null_char, 'hmlt.offi'//c_null_char, 1.0d-12, 0_c_int32_t, &
+          noffd, nbuf)
+     call check(rc)
+     write(*, '(a,i12,a,i6)') ' hmlt: off-diagonal elements', noffd, ', records', nbuf
+  end if
+
+  if (sil .or. lanczos .or. chebyord > 0) then
      allocate(fvec(ndim))
      open(newunit=u, file=trim(fvec_file), access='stream', form='unformatted', status='old')
      read(u) fvec
      close(u)
+  end if
+
+  if (lanczos) then
+     ! init_vec_unit: unit vectors on the lmain 1h1p entries of largest |fvec|
+     nvec = lmain * ncycles
+     allocate(q0(ndim, lmain), lener(nvec), ritz(ndim, nvec), tmom(nvec), fvec_t(ndim), used(n1))
+     q0 = 0.0d0
+     used = .false.
+     do i = 1, lmain
+        k = maxloc(abs(fvec(1:n1)), dim=1, mask=.not. used)
+        used(k) = .true.
+        q0(k, i) = 1.0d0
+     end do
+     fvec_t = fvec
+     lp%blocksize = lmain
+     lp%ncycles = ncycles
+     lp%precision = precision
+     rc = adc2g_block_lanczos(h, lp, q0, lener, c_null_ptr, c_loc(fvec_t), c_loc(tmom), c_loc(ritz), ls)
+     call check(rc)
+     call adc2g_write_davstates(trim(lancname), ndim, nvec, lener, ritz)
+     write(*, '(a,i6,a,f10.3,a)') ' Block Lanczos: states', nvec, ', time', ls%time_total, ' s'
+     do i = 1, min(nvec, 5)
+        write(*, '(i4,2x,f18.12,2x,es16.8)') i, lener(i), tmom(i)**2
+     end do
+  end if
+
+  if (chebyord > 0) then
+     bounds = (/ cheby_emin, cheby_emax /)
+     allocate(cauto(0:2 * chebyord))
+     rc = adc2g_cheby_auto(h, bounds, fvec, int(chebyord, c_int32_t), int(precision, c_int32_t), cauto)
+     call check(rc)
+     call adc2g_write_chebyauto('chebyauto', bounds, int(chebyord, c_int32_t), cauto)
+     write(*, '(a,i6)') ' Chebyshev: orders', chebyord
+  end if
+
+  if (sil) then
      sp%tout = tout
      sp%tfinal = tfinal
      sp%kdim = krydim
      sp%autotol = autotol
      sp%precision = precision
      nrec = int(tfinal / (0.5d0 * tout), c_int64_t) + 1
-     allocate(auto0(nrec))
-     rc = adc2g_sil_autocorr(h, sp, fvec, auto0, nrec, nout, ss)
+     allocate(auto0(nrec), auto1(nrec), auto2(nrec))
+     if (autoord == 0) then
+        rc = adc2g_sil_autocorr(h, sp, fvec, auto0, nrec, nout, ss)
+     else
+        rc = adc2g_sil_autocorr_orders(h, sp, fvec, int(autoord, c_int32_t), auto0, c_loc(auto1), &
+             c_loc(auto2), nrec, nout, ss)
+     end if
      call check(rc)
      call adc2g_write_auto(trim(auto_file), nout, tout, auto0)
+     if (autoord >= 1) call adc2g_write_auto('auto1', nout, tout, auto1)
+     if (autoord >= 2) call adc2g_write_auto('auto2', nout, tout, auto2)
      write(*, '(a,i10,a,i10)') ' SIL: steps', ss%nsteps, ', complex sigma-vectors', ss%nsigma
   end if
 

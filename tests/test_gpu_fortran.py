@@ -67,3 +67,79 @@ def test_fortran_driver(tmp_path):
     assert data.shape == (ref.size, 4)
     assert np.allclose(data[:, 0], np.arange(ref.size) * 0.1 / 41.34137333656, atol=5e-9)
     assert np.abs(data[:, 1] + 1j * data[:, 2] - ref).max() < 1e-10 * np.dot(f, f) + 1e-13
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built")
+def test_fortran_driver_consumers(tmp_path):
+    """The Fortran host program driving every other consumer through the C
+    ABI.  It covers:
+      * relaxation (solver = 2) against the exact lowest eigenpairs;
+      * the block-Lanczos state file against the oracle;
+      * SIL autocorrelation orders 0-2;
+      * the Chebyshev file;
+      * the hmlt matrix files."""
+    from oracle import cheby, elements, lanczos
+    print("[case] fortran driver consumers o=3 v=7 naux=9", flush=True)
+    o, v, naux = 3, 7, 9
+    sys_ = make_system(o, v, naux, seed=11)
+    sys_.eps.astype("<f8").tofile(tmp_path / "eps.bin")
+    sys_.B.astype("<f8").tofile(tmp_path / "B.bin")
+    df = dfsigma.DF(sys_)
+    n1 = df.n1
+    ndim = n1 + df.D2.size
+    f = np.zeros(ndim)
+    f[:n1] = unit_dipole(o, v, seed=2)
+    f.astype("<f8").tofile(tmp_path / "f.bin")
+    H = df.sigma(np.eye(ndim))
+    w, U = np.linalg.eigh(H)
+    span = w[-1] - w[0]
+    emin, emax = w[0] - 0.02 * span, w[-1] + 0.02 * span
+    nml = (f"&adc2g nocc={o} nvirt={v} naux={naux} eps_file='eps.bin' b_file='B.bin' nstates=2 "
+           f"solver=2 stepsize=100d0 maxiter=300 tol=1d-7 davname='rlxstates' "
+           f"sil=.true. fvec_file='f.bin' auto_file='auto' tout=0.1d0 tfinal=1.0d0 krydim=12 "
+           f"autotol=1d-12 autoord=2 lanczos=.true. lmain=3 ncycles=4 lancname='lancstates' "
+           f"chebyord=20 cheby_emin={emin:.17e} cheby_emax={emax:.17e} hmlt=.true. /\n")
+    nml = nml.replace("e-", "d-").replace("e+", "d+")
+    r = subprocess.run([DRIVER], input=nml, capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    # relaxation: the two lowest eigenpairs
+    recs = read_davstates(tmp_path / "rlxstates", ndim)
+    ev = np.array([x[1] for x in recs])
+    assert np.abs(ev - w[:2]).max() < 1e-10 * np.abs(w).max()
+    # block Lanczos states: the oracle from the same start block
+    idx = np.argsort(-np.abs(f[:n1]), kind="stable")[:3]
+    q0 = np.zeros((ndim, 3))
+    q0[idx, np.arange(3)] = 1.0
+    ref = lanczos.block_lanczos(lambda X: H @ X, q0, 4)
+    recs = read_davstates(tmp_path / "lancstates", ndim)
+    assert [x[0] for x in recs] == list(range(1, 13))
+    le = np.array([x[1] for x in recs])
+    assert np.abs(le - ref["eigval"]).max() < 1e-10 * np.abs(ref["eigval"]).max()
+    R = np.stack([x[2] for x in recs], axis=1)
+    d = np.minimum(np.abs(R - ref["ritz"]).max(axis=0), np.abs(R + ref["ritz"]).max(axis=0))
+    assert d.max() < 1e-8
+    # SIL orders 0..2
+    _, autos = sil.propagate(lambda x: -1j * (H @ x), f, tout=0.1, tfinal=1.0, kdim=12, autotol=1e-12,
+                             autoord=2)
+    c = U.T @ f
+    for k, name in enumerate(("auto", "auto1", "auto2")):
+        lines = open(tmp_path / name).read().splitlines()[1:]
+        data = np.array([[float(t) for t in ln.split()] for ln in lines])
+        scale = np.sum(c * c * np.abs(w) ** k)
+        assert np.abs(data[:, 1] + 1j * data[:, 2] - autos[k]).max() < 1e-10 * scale + 1e-13, name
+    # Chebyshev autocorrelation
+    cref = cheby.chebyshev_auto(lambda x: H @ x, f, (emin, emax), 20)
+    vals = [ln.split() for ln in open(tmp_path / "chebyauto").read().splitlines()
+            if ln.strip() and not ln.startswith("#")]
+    ck = np.array([float(x[1]) for x in vals])
+    assert ck.size == 41
+    assert np.abs(ck - cref).max() < 1e-10 * cref[0]
+    assert np.isclose(cref[0], np.dot(f, f))
+    # matrix files: the upper triangle of the filtered restatement
+    Hm = np.triu(elements.build_blocks(sys_, apply_minc=True).dense(), 1)
+    data = open(tmp_path / "hmlt.offi", "rb").read()
+    assert len(data) > 0 and (tmp_path / "hmlt.diai").stat().st_size == 8 + 8 + 8 * ndim + 8
+    near = (np.abs(np.abs(np.triu(H, 1)) - 1e-12) < 1e-12).sum()
+    nrec_elems = int(np.frombuffer(data, np.int32, 1, len(data) - 8)[0])
+    assert abs(nrec_elems - np.count_nonzero(Hm)) <= near
