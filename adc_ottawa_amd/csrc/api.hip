@@ -55,9 +55,23 @@ int32_t adc2g_create(int32_t nocc, int32_t nvirt, int32_t naux, const double* ep
   });
 }
 
+int32_t adc2g_create_cvs(adc2g_handle* full, int32_t ncore, const int32_t* core, adc2g_handle** out) {
+  return guarded([&] {
+    ARG_CHECK(full && out, "NULL argument");
+    *out = nullptr;
+    HIP_CHECK(hipSetDevice(full->h.device));
+    auto hp = std::make_unique<adc2g_handle>();
+    hp->h.create_cvs(full->h, ncore, core);
+    *out = hp.release();
+  });
+}
+
 int32_t adc2g_destroy(adc2g_handle* h) {
   return guarded([&] {
     if (!h) return;
+    if (h->h.nviews > 0)
+      throw Error(ADC2G_ESTATE, "the operator still has " + std::to_string(h->h.nviews) +
+                                    " CVS view(s); destroy them first");
     (void)hipSetDevice(h->h.device);
     if (h->h.st) (void)hipStreamSynchronize(h->h.st);
     delete h;
@@ -93,6 +107,7 @@ int32_t adc2g_diagonal(adc2g_handle* h, double* hii) {
 int32_t adc2g_phph(adc2g_handle* h, double* h11) {
   return guarded([&] {
     ARG_CHECK(h && h11, "NULL argument");
+    ARG_CHECK(!h->h.parent, "the ph-ph block belongs to the full operator, not to a CVS view");
     HIP_CHECK(hipSetDevice(h->h.device));
     const size_t n = (size_t)h->h.d.n1 * h->h.d.n1;
     HIP_CHECK(hipMemcpyAsync(h11, h->h.H11.get(), n * sizeof(double), hipMemcpyDeviceToHost, h->h.st));
@@ -183,6 +198,7 @@ int32_t adc2g_sil_autocorr_orders(adc2g_handle* h, const adc2g_sil_params* p, co
 int32_t adc2g_precon_harvest(adc2g_handle* h, const adc2g_dav_params* p, int64_t max_records, int64_t* added) {
   return guarded([&] {
     ARG_CHECK(h && p, "NULL argument");
+    ARG_CHECK(!h->h.parent, "the learned preconditioner works on the full operator only");
     HIP_CHECK(hipSetDevice(h->h.device));
     precon_harvest(h->h, *p, max_records, added);
   });
@@ -192,6 +208,7 @@ int32_t adc2g_precon_train(adc2g_handle* h, const adc2g_handle* data, int32_t ep
                            uint64_t seed, double* loss_out) {
   return guarded([&] {
     ARG_CHECK(h && data, "NULL argument");
+    ARG_CHECK(!h->h.parent && !data->h.parent, "the learned preconditioner works on the full operator only");
     HIP_CHECK(hipSetDevice(h->h.device));
     precon_train(h->h, data->h, epochs, batch, lr, seed, loss_out);
   });

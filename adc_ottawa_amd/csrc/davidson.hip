@@ -289,6 +289,7 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
   ARG_CHECK(p.nstates >= 1, "nstates must be >= 1");
   ARG_CHECK(p.tol > 0, "tol must be > 0");
   ARG_CHECK(p.precon >= 1 && p.precon <= 3, "precon must be 1 (DPR), 2 (Olsen) or 3 (learned)");
+  ARG_CHECK(!(p.precon == 3 && H.parent), "the learned preconditioner works on the full operator only");
   if (p.precon == 3 && !precon_ready(H))
     throw Error(ADC2G_ESTATE, "learned preconditioner requested but no network is loaded");
   const int nstates = p.nstates;

@@ -28,6 +28,7 @@ Prec prec_from_int(int p) {
 static size_t prec_size(Prec p) { return p == Prec::F64 ? 8 : (p == Prec::F32 ? 4 : 2); }
 
 Handle::~Handle() {
+  if (parent) --parent->nviews;
   precon_net_free(*this);
   for (auto& e : ev_pool)
     if (e) (void)hipEventDestroy(e);
@@ -315,6 +316,10 @@ size_t Handle::sigma_workspace_bytes(int nc, Prec p) const {
 void Handle::sigma(const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, Prec p, hipStream_t s) {
   ARG_CHECK(ldx >= d.ndim && ldw >= d.ndim, "leading dimensions must be >= ndim");
   if (nvec <= 0) return;
+  if (parent) {
+    sigma_view(X, ldx, W, ldw, nvec, p, s);
+    return;
+  }
   if (!s) s = st;
   ensure_lowprec(p);
   const int nc_max = std::min(chunk, nvec);
@@ -618,6 +623,133 @@ void Handle::sigma_df(const double* Xc, int64_t ldx, double* Wc, int64_t ldw, in
   if (eu.first) HIP_CHECK(hipEventRecord(eu.first, s));
   unpack_chunk(p, ZG.get(), Xc, ldx, Wc, ldw, nc, dk, eps.get(), lay, s);
   if (eu.second) HIP_CHECK(hipEventRecord(eu.second, s));
+}
+
+}  // namespace adc2g
+
+// ---- sub-space views (CVS)
+
+namespace adc2g {
+
+namespace {
+
+// Xf[f + nf c] = X[inv[f] + ldx c] for the configurations of the view, 0 elsewhere
+__global__ void view_scatter_kernel(int64_t nf, int nc, const int64_t* __restrict__ inv, const double* __restrict__ X,
+                                    int64_t ldx, double* __restrict__ Xf) {
+  const int64_t total = nf * nc;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t f = t % nf, c = t / nf;
+    const int64_t r = inv[f];
+    Xf[t] = r >= 0 ? X[r + ldx * c] : 0.0;
+  }
+}
+
+// W[r + ldw c] = Wf[map[r] + nf c]
+__global__ void view_gather_kernel(int64_t nv, int nc, const int64_t* __restrict__ map, const double* __restrict__ Wf,
+                                   int64_t nf, double* __restrict__ W, int64_t ldw) {
+  const int64_t total = nv * nc;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = t % nv, c = t / nv;
+    W[r + ldw * c] = Wf[map[r] + nf * c];
+  }
+}
+
+}  // namespace
+
+// CVS configuration space (select_atom_is_cvs, source/adclib/select_fano.f90:112-142;
+// select_atom_d_cvs, :903-1027): the 1h1p (a, i) with i a core orbital (a outer,
+// i inner), no t1 or t2, and the t3, t4, t5 configurations with exactly one of
+// i < j a core orbital, each in the loop order of the full space.  The CVS
+// matrix elements are those of the full matrix (get_offdiag_adc2_save_cvs,
+// source/adclib/get_matrix.f90:1323-1726, calls the same element functions), so
+// the view is the principal sub-matrix on these configurations, in the order of
+// the full space.
+void Handle::create_cvs(Handle& full, int ncore, const int32_t* core) {
+  ARG_CHECK(!full.parent, "a view cannot be made from another view");
+  const int o = full.d.o, v = full.d.v;
+  ARG_CHECK(ncore >= 1 && ncore < o && core, "need 1 <= ncore < nocc core orbitals");
+  std::vector<char> isc(o, 0);
+  for (int c = 0; c < ncore; ++c) {
+    ARG_CHECK(core[c] >= 1 && core[c] <= o, "core orbitals are occupied orbitals 1..nocc");
+    ARG_CHECK(!isc[core[c] - 1], "core orbital listed twice");
+    isc[core[c] - 1] = 1;
+  }
+  const Dims& F = full.d;
+  std::vector<int64_t> map;
+  map.reserve((size_t)F.ndim / 4);
+  // 1h1p: idx = a o + i
+  for (int a = 0; a < v; ++a)
+    for (int i = 0; i < o; ++i)
+      if (isc[i]) map.push_back((int64_t)a * o + i);
+  vcnt[0] = vcnt[1] = 0;
+  // t3 (a = b, i < j): (i, j) outer, a inner
+  int64_t pij = 0;
+  int64_t c3 = 0, c4 = 0;
+  for (int i = 0; i < o; ++i)
+    for (int j = i + 1; j < o; ++j, ++pij)
+      if (isc[i] + isc[j] == 1)
+        for (int a = 0; a < v; ++a, ++c3) map.push_back(F.off[2] + pij * v + a);
+  // t4 then t5 (a < b, i < j): (i, j) outer, (a, b) inner
+  for (int t = 3; t <= 4; ++t) {
+    pij = 0;
+    for (int i = 0; i < o; ++i)
+      for (int j = i + 1; j < o; ++j, ++pij)
+        if (isc[i] + isc[j] == 1)
+          for (int64_t pab = 0; pab < F.npv; ++pab) {
+            map.push_back(F.off[t] + pij * F.npv + pab);
+            if (t == 3) ++c4;
+          }
+  }
+  vcnt[2] = c3;
+  vcnt[3] = vcnt[4] = c4;
+  parent = &full;
+  ++full.nviews;
+  device = full.device;
+  form = full.form;
+  HIP_CHECK(hipSetDevice(device));
+  HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  const int64_t nv = (int64_t)map.size();
+  d = Dims();
+  d.o = o;
+  d.v = v;
+  d.naux = F.naux;
+  d.n1 = (int64_t)ncore * v;
+  d.ndim = nv;
+  d.n2 = nv - d.n1;
+  for (int t = 0; t < 5; ++t) d.cnt[t] = vcnt[t];
+  d.off[0] = d.n1;
+  for (int t = 0; t < 5; ++t) d.off[t + 1] = d.off[t] + d.cnt[t];
+  std::vector<int64_t> inv(F.ndim, -1);
+  for (int64_t r = 0; r < nv; ++r) inv[map[r]] = r;
+  vmap.alloc(nv);
+  vinv.alloc(F.ndim);
+  HIP_CHECK(hipMemcpyAsync(vmap.get(), map.data(), sizeof(int64_t) * nv, hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipMemcpyAsync(vinv.get(), inv.data(), sizeof(int64_t) * F.ndim, hipMemcpyHostToDevice, st));
+  // the diagonal of the view
+  hii.alloc(nv);
+  HIP_CHECK(hipStreamSynchronize(full.st));
+  hipLaunchKernelGGL(view_gather_kernel, dim3(grid_for(nv)), dim3(256), 0, st, nv, 1, vmap.get(), full.hii.get(),
+                     F.ndim, hii.get(), nv);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(st));
+}
+
+void Handle::sigma_view(const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, Prec p, hipStream_t s) {
+  if (!s) s = st;
+  const int64_t nf = parent->d.ndim, nv = d.ndim;
+  const int cw = std::min(nvec, parent->chunk);
+  vX.alloc((size_t)nf * cw);
+  vW.alloc((size_t)nf * cw);
+  for (int c0 = 0; c0 < nvec; c0 += cw) {
+    const int nc = std::min(cw, nvec - c0);
+    hipLaunchKernelGGL(view_scatter_kernel, dim3(grid_for(nf * nc)), dim3(256), 0, s, nf, nc, vinv.get(),
+                       X + ldx * c0, ldx, vX.get());
+    HIP_CHECK(hipGetLastError());
+    parent->sigma(vX.get(), nf, vW.get(), nf, nc, p, s);
+    hipLaunchKernelGGL(view_gather_kernel, dim3(grid_for(nv * nc)), dim3(256), 0, s, nv, nc, vmap.get(), vW.get(), nf,
+                       W + ldw * c0, ldw);
+    HIP_CHECK(hipGetLastError());
+  }
 }
 
 }  // namespace adc2g

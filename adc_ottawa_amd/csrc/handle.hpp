@@ -80,6 +80,18 @@ struct Handle {
   // learned preconditioner (network + harvested training records)
   PreconNet* net = nullptr;
 
+  // sub-space view (CVS): this handle's vectors are the configurations
+  // vmap[0..d.ndim) of the parent's space, in that order; sigma scatters into
+  // the parent's space, applies the parent's sigma-vector and gathers.  d of
+  // a view holds only ndim, n1 and the type counts (vcnt).
+  Handle* parent = nullptr;
+  int nviews = 0;                 // views alive on this handle
+  DevBuf<int64_t> vmap, vinv;     // view -> parent index, parent -> view index or -1
+  DevBuf<double> vX, vW;          // parent-size staging (columns of a chunk)
+  int64_t vcnt[5] = {0, 0, 0, 0, 0};
+  void create_cvs(Handle& full, int ncore, const int32_t* core);
+  void sigma_view(const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, Prec p, hipStream_t s);
+
   Handle() = default;
   ~Handle();
 

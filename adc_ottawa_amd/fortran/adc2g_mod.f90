@@ -245,6 +245,15 @@ module adc2g_mod
        type(adc2g_dav_stats), intent(out) :: stats
      end function adc2g_davidson_vectors
 
+     ! CVS view on the core orbitals core(1:ncore) (1-based occupied indices)
+     integer(c_int32_t) function adc2g_create_cvs(full, ncore, core, out) bind(c, name='adc2g_create_cvs')
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: full
+       integer(c_int32_t), value :: ncore
+       integer(c_int32_t), intent(in) :: core(*)
+       type(c_ptr), intent(out) :: out
+     end function adc2g_create_cvs
+
      integer(c_int32_t) function adc2g_precon_nparams() bind(c, name='adc2g_precon_nparams')
        import :: c_int32_t
      end function adc2g_precon_nparams

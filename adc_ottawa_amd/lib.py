@@ -98,7 +98,7 @@ SYMBOLS = [
     "adc2g_precon_get", "adc2g_precon_set", "adc2g_precon_eval", "adc2g_last_error",
     "adc2g_sigma_profile", "adc2g_precon_absorb", "adc2g_coupling_form", "adc2g_cheby_auto",
     "adc2g_block_lanczos", "adc2g_relax", "adc2g_write_hmlt", "adc2g_davidson_vectors",
-    "adc2g_sil_autocorr_orders",
+    "adc2g_sil_autocorr_orders", "adc2g_create_cvs",
 ]
 
 _lib = None
@@ -141,6 +141,7 @@ def load(path: str = LIBPATH):
     lib.adc2g_write_hmlt.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_double, i32, C.POINTER(i64),
                                      C.POINTER(i32)]
     lib.adc2g_davidson_vectors.argtypes = [vp, C.POINTER(DavParams), vp, vp, vp, C.POINTER(DavStats)]
+    lib.adc2g_create_cvs.argtypes = [vp, i32, vp, C.POINTER(vp)]
     lib.adc2g_sil_autocorr_orders.argtypes = [vp, C.POINTER(SilParams), vp, i32, vp, vp, vp, i64,
                                               C.POINTER(i64), C.POINTER(SilStats)]
     lib.adc2g_relax.argtypes = [vp, C.POINTER(RelaxParams), vp, vp, vp, vp, vp, C.POINTER(RelaxStats)]
@@ -358,6 +359,11 @@ class Operator:
                                     int(maxbl), C.byref(nz), C.byref(nb)))
         return nz.value, nb.value
 
+    def cvs(self, core):
+        """CVS view on the core orbitals ``core`` (1-based occupied indices);
+        close it before this operator."""
+        return CvsOperator(self, core)
+
     def sigma_device(self, nvec, dX, dW, precision="fp64", stream=None):
         """W = H X on device pointers (ints), asynchronous on ``stream``."""
         check(_lib.adc2g_sigma_device(self._h, nvec, C.c_void_p(dX), C.c_void_p(dW), PREC[precision],
@@ -412,3 +418,22 @@ class Operator:
         check(_lib.adc2g_precon_eval(self._h, n, _ptr(feat), _ptr(cls), _ptr(dn), _ptr(en), _ptr(m),
                                      _ptr(g), C.byref(loss)))
         return m, g, loss.value
+
+
+class CvsOperator(Operator):
+    """CVS view of an Operator (``adc2g_create_cvs``): the principal sub-matrix
+    on the core-hole configurations, with the same solver methods."""
+
+    def __init__(self, parent, core):
+        lib = load()
+        self._h = C.c_void_p()
+        core = np.ascontiguousarray(core, dtype=np.int32)
+        check(lib.adc2g_create_cvs(parent.handle, core.size, _ptr(core), C.byref(self._h)))
+        self._parent = parent
+        nd, n1 = C.c_int64(), C.c_int64()
+        cnt = (C.c_int64 * 5)()
+        check(lib.adc2g_dims(self._h, C.byref(nd), C.byref(n1), cnt))
+        self.ndim, self.n1, self.counts = nd.value, n1.value, list(cnt)
+        self.nocc, self.nvirt, self.naux = parent.nocc, parent.nvirt, parent.naux
+        self.coupling = parent.coupling
+        self.core = [int(c) for c in core]

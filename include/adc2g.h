@@ -174,8 +174,27 @@ typedef struct {
 int32_t adc2g_create(int32_t nocc, int32_t nvirt, int32_t naux, const double* eps,
                      const double* B, const adc2g_opts* opts, adc2g_handle** out);
 
-/* Free everything the handle owns. */
+/* Free everything the handle owns.  An operator with CVS views alive is not
+ * destroyed (ADC2G_ESTATE): destroy the views first. */
 int32_t adc2g_destroy(adc2g_handle* h);
+
+/* CVS view of an operator (core-valence separation; select_atom_is_cvs,
+ * source/adclib/select_fano.f90:112-142, select_atom_d_cvs, :903-1027, with the
+ * element functions of get_offdiag_adc2_save_cvs, source/adclib/get_matrix.f90:1323-1726).
+ * The configurations are:
+ *  - the 1h1p (a, i) with i one of the ncore core orbitals (1-based occupied
+ *    indices in core[]), a outer, i inner;
+ *  - no a=b,i=j or a!=b,i=j configurations;
+ *  - the t3, t4 and t5 configurations in which exactly one of i < j is a core
+ *    orbital, each in the order of the full space.
+ * The CVS matrix is the principal sub-matrix of the full one on these
+ * configurations.  The view's sigma-vector scatters into the full space,
+ * applies the full sigma-vector and gathers.  Every solver entry point
+ * (Davidson, relaxation, SIL, Chebyshev, block Lanczos) and adc2g_dims,
+ * adc2g_diagonal, adc2g_sigma* and adc2g_write_hmlt accept the view; the ph-ph
+ * block and the learned preconditioner belong to the full operator.  The
+ * full operator must outlive the view. */
+int32_t adc2g_create_cvs(adc2g_handle* full, int32_t ncore, const int32_t* core, adc2g_handle** out);
 
 /* Sizes: ndim, n1, and the counts of t1..t5 (counts[0..4]); kpq(1..5,0) of
  * source/adclib/select_fano.f90 with kpq(5,0) repeated for t5. */

@@ -71,3 +71,31 @@ def table(o: int, v: int):
     rows.extend((5,) + r[1:] for r in t4)
     arr = np.array(rows, dtype=np.int64)
     return arr[:, 0], arr[:, 1], arr[:, 2], arr[:, 3], arr[:, 4]
+
+
+def cvs_indices(o: int, v: int, core) -> np.ndarray:
+    """Full-space indices of the CVS configurations, in the CVS order.
+
+    The loops follow select_atom_is_cvs (source/adclib/select_fano.f90:112-142)
+    and select_atom_d_cvs (:903-1027).  1h1p: a outer, core hole i inner.  No
+    t1 or t2.  t3: pairs i < j with exactly one core orbital, a inner.  t4,
+    then t5: the same pairs, (a, b) inner.  ``core`` holds 0-based occupied
+    indices.
+    """
+    c = counts(o, v)
+    isc = np.zeros(o, dtype=int)
+    isc[np.asarray(core, dtype=int)] = 1
+    npv = v * (v - 1) // 2
+    off2 = c["n1"] + c["t1"] + c["t2"]
+    off3 = off2 + c["t3"]
+    off4 = off3 + c["t4"]
+    out = [a * o + i for a in range(v) for i in range(o) if isc[i]]
+    pairs = [(i, j) for i in range(o) for j in range(i + 1, o)]
+    for p, (i, j) in enumerate(pairs):
+        if isc[i] + isc[j] == 1:
+            out.extend(off2 + p * v + a for a in range(v))
+    for off in (off3, off4):
+        for p, (i, j) in enumerate(pairs):
+            if isc[i] + isc[j] == 1:
+                out.extend(off + p * npv + q for q in range(npv))
+    return np.asarray(out, dtype=np.int64)

@@ -235,6 +235,27 @@ def test_oracle_block_lanczos_full_krylov():
     assert np.abs(H @ R - R * out["eigval"]).max() < 1e-8 * np.abs(H).max()
 
 
+def test_oracle_cvs_indices():
+    """The CVS loop restatement against a brute-force filter of the full
+    configuration table: 1h1p with a core hole, t3/t4/t5 with exactly one
+    core hole among i < j, nothing of t1/t2, in ascending full-space order
+    (the CVS loops nest like the full ones).  Counts: n1 = nc v, t3 = nc (o-nc) v,
+    t4 = t5 = nc (o-nc) v (v-1)/2."""
+    for o, v, core in [(4, 5, [0]), (5, 6, [1, 3]), (3, 4, [0, 1])]:
+        typ, a, b, k, l = csf.table(o, v)
+        isc = np.zeros(o, dtype=bool)
+        isc[core] = True
+        keep = np.where(typ == 0, isc[k], False)
+        two = (typ >= 3) & (isc[k].astype(int) + isc[l].astype(int) == 1)
+        ref = np.flatnonzero(keep | two)
+        got = csf.cvs_indices(o, v, core)
+        assert np.array_equal(got, ref)
+        nc = len(core)
+        assert (typ[got] == 0).sum() == nc * v
+        assert (typ[got] == 3).sum() == nc * (o - nc) * v
+        assert (typ[got] == 4).sum() == (typ[got] == 5).sum() == nc * (o - nc) * v * (v - 1) // 2
+
+
 def test_oracle_subdiag_guess():
     """initvec_subdiag: with subdim = ndim the guess vectors are the exact
     lowest eigenvectors, so the Davidson converges in the first iteration;
