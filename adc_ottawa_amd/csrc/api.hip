@@ -294,6 +294,14 @@ int32_t adc2g_write_hmlt(adc2g_handle* h, const char* dia_path, const char* off_
   });
 }
 
+int32_t adc2g_mp2(adc2g_handle* h, double* e_mp2, double* d2, double* rho) {
+  return guarded([&] {
+    ARG_CHECK(h != nullptr, "NULL handle");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    mp2_run(h->h, e_mp2, d2, rho);
+  });
+}
+
 int32_t adc2g_sigma_profile(adc2g_handle* h, int32_t enable, double* out) {
   return guarded([&] {
     ARG_CHECK(h != nullptr, "NULL handle");

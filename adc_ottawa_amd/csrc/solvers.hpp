@@ -47,6 +47,10 @@ void relax_run(Handle& H, const adc2g_relax_params& p, const double* guess, doub
 void hmlt_write(Handle& H, const char* dia_path, const char* off_path, double minc, int maxbl, int64_t* noffd,
                 int32_t* nbuf);
 
+// MP2 energy, D2 diagnostic and ground-state density (rho: nbas x nbas
+// column-major); any output may be NULL.
+void mp2_run(Handle& H, double* e_mp2, double* d2, double* rho);
+
 int precon_nparams();
 void precon_harvest(Handle& H, const adc2g_dav_params& p, int64_t max_records, int64_t* added);
 void precon_train(Handle& H, const Handle& data, int epochs, int batch, double lr, uint64_t seed, double* loss_out);

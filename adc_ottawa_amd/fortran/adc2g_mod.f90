@@ -254,6 +254,14 @@ module adc2g_mod
        type(c_ptr), intent(out) :: out
      end function adc2g_create_cvs
 
+     ! MP2 energy, D2 diagnostic and density (rho: c_loc of nbas x nbas or c_null_ptr)
+     integer(c_int32_t) function adc2g_mp2(h, e_mp2, d2, rho) bind(c, name='adc2g_mp2')
+       import :: c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: e_mp2, d2
+       type(c_ptr), value :: rho
+     end function adc2g_mp2
+
      integer(c_int32_t) function adc2g_precon_nparams() bind(c, name='adc2g_precon_nparams')
        import :: c_int32_t
      end function adc2g_precon_nparams

@@ -98,7 +98,7 @@ SYMBOLS = [
     "adc2g_precon_get", "adc2g_precon_set", "adc2g_precon_eval", "adc2g_last_error",
     "adc2g_sigma_profile", "adc2g_precon_absorb", "adc2g_coupling_form", "adc2g_cheby_auto",
     "adc2g_block_lanczos", "adc2g_relax", "adc2g_write_hmlt", "adc2g_davidson_vectors",
-    "adc2g_sil_autocorr_orders", "adc2g_create_cvs",
+    "adc2g_sil_autocorr_orders", "adc2g_create_cvs", "adc2g_mp2",
 ]
 
 _lib = None
@@ -142,6 +142,7 @@ def load(path: str = LIBPATH):
                                      C.POINTER(i32)]
     lib.adc2g_davidson_vectors.argtypes = [vp, C.POINTER(DavParams), vp, vp, vp, C.POINTER(DavStats)]
     lib.adc2g_create_cvs.argtypes = [vp, i32, vp, C.POINTER(vp)]
+    lib.adc2g_mp2.argtypes = [vp, vp, vp, vp]
     lib.adc2g_sil_autocorr_orders.argtypes = [vp, C.POINTER(SilParams), vp, i32, vp, vp, vp, i64,
                                               C.POINTER(i64), C.POINTER(SilStats)]
     lib.adc2g_relax.argtypes = [vp, C.POINTER(RelaxParams), vp, vp, vp, vp, vp, C.POINTER(RelaxStats)]
@@ -358,6 +359,14 @@ class Operator:
         check(_lib.adc2g_write_hmlt(self._h, os.fsencode(dia_path), os.fsencode(off_path), float(minc),
                                     int(maxbl), C.byref(nz), C.byref(nb)))
         return nz.value, nb.value
+
+    def mp2(self, density=True):
+        """(E_MP2, D2, rho or None) of mp2_ener, mp2_d2 and rho_mp2."""
+        e, d = C.c_double(), C.c_double()
+        nb = self.nocc + self.nvirt
+        rho = np.empty((nb, nb), order="F") if density else None
+        check(_lib.adc2g_mp2(self._h, C.byref(e), C.byref(d), _ptr(rho) if density else None))
+        return e.value, d.value, rho
 
     def cvs(self, core):
         """CVS view on the core orbitals ``core`` (1-based occupied indices);

@@ -328,6 +328,19 @@ int32_t adc2g_relax(adc2g_handle* h, const adc2g_relax_params* p, const double* 
                     double* eigval, double* eigvec, double* residual, int32_t* niter,
                     adc2g_relax_stats* stats);
 
+/* ---- MP2 ground state ----------------------------------------------------- */
+
+/* MP2 quantities of the operator's system (closed shell, no frozen core):
+ *  - e_mp2: the correlation energy of mp2_ener (source/adclib/mp2.f90:68-154);
+ *  - d2: the D2 diagnostic of mp2_d2 (:161-276);
+ *  - rho: the nbas x nbas column-major ground-state density of rho_mp2
+ *    (:301-413, rhogs2_uu/oo/ou :421-596), assembled as the reference's loops
+ *    leave it: 2 + oo(i,i) on the occupied diagonal, oo(i,j) + oo(j,i) on the
+ *    occupied off-diagonal, uu and ou assigned once and mirrored.
+ * Any output may be NULL.  rho needs the (vv|ov) block in memory
+ * (8 v^2 o v bytes).  Not available on a CVS view. */
+int32_t adc2g_mp2(adc2g_handle* h, double* e_mp2, double* d2, double* rho);
+
 /* ---- on-disk matrix ------------------------------------------------------ */
 
 /* Write the matrix in the reference's on-disk format so that its readers

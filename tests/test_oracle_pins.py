@@ -24,7 +24,7 @@ import pytest
 import scipy.sparse.linalg as sla
 
 from adc_ottawa_amd.synth import make_system
-from oracle import cheby, csf, davidson, dfsigma, elements, lanczos, mlp, relax, schur, sil
+from oracle import cheby, csf, davidson, dfsigma, elements, lanczos, mlp, mp2, relax, schur, sil
 
 
 def test_csf_counts_and_bijectivity():
@@ -254,6 +254,41 @@ def test_oracle_cvs_indices():
         assert (typ[got] == 0).sum() == nc * v
         assert (typ[got] == 3).sum() == nc * (o - nc) * v
         assert (typ[got] == 4).sum() == (typ[got] == 5).sum() == nc * (o - nc) * v * (v - 1) // 2
+
+
+def test_oracle_mp2():
+    """MP2 energy against the spin-orbital formula 1/4 sum |<ij||ab>|^2 / D;
+    d2 against the largest singular values of the amplitude matricisations;
+    the density's oo/uu blocks against the closed-shell textbook forms and
+    its trace (the second-order correction is traceless); the assembly
+    doubles the off-diagonal oo elements as the reference's loop does."""
+    sys = make_system(3, 5, 7, seed=2)
+    o, v, e = sys.nocc, sys.nvirt, sys.eps
+    V = mp2.integrals(sys)
+    occ = [(p, s) for p in range(o) for s in (0, 1)]
+    vir = [(p, s) for p in range(o, o + v) for s in (0, 1)]
+
+    def asym(p, q, r, s):
+        a = V[p[0], r[0], q[0], s[0]] if (p[1] == r[1] and q[1] == s[1]) else 0.0
+        b = V[p[0], s[0], q[0], r[0]] if (p[1] == s[1] and q[1] == r[1]) else 0.0
+        return a - b
+
+    E = sum(0.25 * asym(i, j, a, b) ** 2 / (e[i[0]] + e[j[0]] - e[a[0]] - e[b[0]])
+            for i in occ for j in occ for a in vir for b in vir)
+    assert abs(mp2.energy(sys, V) - E) < 1e-14 * abs(E)
+    t = mp2.amplitudes(sys, V)
+    so = np.linalg.svd(t.reshape(o, -1), compute_uv=False)[0]
+    sv = np.linalg.svd(t.transpose(2, 0, 1, 3).reshape(v, -1), compute_uv=False)[0]
+    assert abs(mp2.d2(sys, V) - max(so, sv)) < 1e-13
+    rho, blk = mp2.density(sys, V)
+    Poo = -2.0 * np.einsum("ikab,jkab->ij", t, 2.0 * t - t.transpose(0, 1, 3, 2))
+    Puu = 2.0 * np.einsum("ijac,ijbc->ab", t, 2.0 * t - t.transpose(0, 1, 3, 2))
+    assert np.abs(blk["oo"] - Poo).max() < 1e-13
+    assert np.abs(blk["uu"] - Puu).max() < 1e-13
+    assert abs(np.trace(blk["oo"]) + np.trace(blk["uu"])) < 1e-14
+    off = ~np.eye(o, dtype=bool)
+    assert np.abs(rho[:o, :o][off] - 2.0 * blk["oo"][off]).max() < 1e-15
+    assert np.abs(np.diag(rho)[:o] - 2.0 - np.diag(blk["oo"])).max() < 1e-15
 
 
 def test_oracle_subdiag_guess():
