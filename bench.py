@@ -349,6 +349,19 @@ def relax_measure(op, tol):
     return out
 
 
+def mp2_measure(op):
+    """MP2 correlation energy, D2 diagnostic and ground-state density
+    (adc2g_mp2) of the workload's system, FP64."""
+    t0 = time.perf_counter()
+    e, d2, _ = op.mp2(density=False)
+    t1 = time.perf_counter()
+    _, _, rho = op.mp2(density=True)
+    t2 = time.perf_counter()
+    log(f"mp2: energy + D2 {t1 - t0:.2f}s, with the density {t2 - t1:.2f}s")
+    return {"energy_seconds": round(t1 - t0, 3), "with_density_seconds": round(t2 - t1, 3),
+            "e_mp2_Eh": float(e), "d2": float(d2), "density_trace": float(np.trace(rho))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -364,12 +377,13 @@ def main():
     ap.add_argument("--sil", type=float, default=0.0,
                     help="also propagate with SIL to this final time (a.u.), FP64 and FP32 sigma")
     ap.add_argument("--sil-tol", type=float, default=1e-10, help="SIL error tolerance (autotol)")
-    ap.add_argument("--cheby", type=int, default=0,
-                    help="also run the Chebyshev order-domain autocorrelation to this order")
-    ap.add_argument("--lanczos", default="",
-                    help="also run the block Lanczos pseudospectrum, 'blocksize,ncycles' (e.g. 50,20)")
-    ap.add_argument("--relax", type=float, default=0.0,
-                    help="also relax the lowest state in imaginary time to this residual (e.g. 1e-3)")
+    ap.add_argument("--cheby", type=int, default=500,
+                    help="Chebyshev order-domain autocorrelation to this order (0 skips)")
+    ap.add_argument("--lanczos", default="50,20",
+                    help="block Lanczos pseudospectrum, 'blocksize,ncycles' (empty skips)")
+    ap.add_argument("--relax", type=float, default=1e-3,
+                    help="relax the lowest state in imaginary time to this residual (0 skips)")
+    ap.add_argument("--skip-mp2", action="store_true", help="skip the MP2 energy/density timing")
     ap.add_argument("--dav-roots", type=int, default=20, help="Davidson comparison: roots (block >= roots)")
     ap.add_argument("--dav-block", type=int, default=20, help="Davidson comparison: block size")
     ap.add_argument("--dav-maxvec", type=int, default=200, help="Davidson comparison: maximum subspace")
@@ -438,9 +452,19 @@ def main():
                                blocksize=args.dav_block, maxvec=args.dav_maxvec)
 
     sil = sil_compare(op, sz, args.sil, args.sil_tol) if args.sil > 0 else None
-    cheb = cheby_measure(op, sz, args.cheby) if args.cheby > 0 else None
-    lanc = lanczos_measure(op, sz, args.lanczos) if args.lanczos else None
-    rlx = relax_measure(op, args.relax) if args.relax > 0 else None
+    def guarded(name, fn):
+        """Run one consumer measurement; a failure is reported in the JSON
+        line instead of ending the benchmark."""
+        try:
+            return fn()
+        except Exception as e:  # noqa: BLE001 - reported, never fatal
+            log(f"{name} failed: {e}")
+            return {"error": str(e)}
+
+    cheb = guarded("cheby", lambda: cheby_measure(op, sz, args.cheby)) if args.cheby > 0 else None
+    lanc = guarded("lanczos", lambda: lanczos_measure(op, sz, args.lanczos)) if args.lanczos else None
+    rlx = guarded("relax", lambda: relax_measure(op, args.relax)) if args.relax > 0 else None
+    mp2r = guarded("mp2", lambda: mp2_measure(op)) if not args.skip_mp2 else None
 
     n1 = op.n1
     bytes_per_sigma = 8.0 * (2 * op.ndim) + 8.0 * 2 * n1 * n1  # read x, write sigma, pair tensors
@@ -469,6 +493,7 @@ def main():
         "chebyshev_autocorrelation": cheb,
         "block_lanczos": lanc,
         "relaxation": rlx,
+        "mp2": mp2r,
         "operator_build_seconds": round(t_build, 3),
     }
     op.close()
