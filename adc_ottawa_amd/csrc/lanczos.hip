@@ -1,7 +1,8 @@
 // Block Lanczos pseudospectrum on the GPU sigma-vector: run_block_lanczos
-// (source/eigen/block_lanczos.f90:947-1256) in its default form, orthotype 0
-// (no re-orthogonalisation), with hxq_incore (:1325-1361) replaced by the
-// sigma-vector of the block.  Per cycle j (B_0 = 0):
+// (source/eigen/block_lanczos.f90:947-1256) without re-orthogonalisation
+// (orthotype 0, the default) or with partial re-orthogonalisation
+// (orthotype 1: pro, update_omega, pro_mgs, localro, :1430-1706), with
+// hxq_incore (:1325-1361) replaced by the sigma-vector of the block.  Per cycle j (B_0 = 0):
 //   U = H Q_j - Q_{j-1} B_{j-1}^T,  A_j = Q_j^T U,  R = U - Q_j A_j,
 //   R = Q_{j+1} B_j  (B_j upper triangular).
 // All Lanczos vectors stay in HBM (L, ndim x nvec); the products with them
@@ -392,11 +393,20 @@ void lanczos_run(Handle& H, const adc2g_lanczos_params& p, const double* q0_h, d
   const int nvec = b * m;
   hipStream_t st = H.st;
 
-  DevBuf<double> L, U, X2, small;
+  ARG_CHECK(p.orthotype == 0 || p.orthotype == 1, "orthotype must be 0 or 1");
+  const bool pro = p.orthotype == 1;
+  DevBuf<double> L, U, X2, small, Pw, part, sums, scale;
   L.alloc((size_t)n * nvec);
   U.alloc((size_t)n * b);
   X2.alloc((size_t)n * b);
   small.alloc((size_t)b * b);
+  const int nblk = reduce_blocks(n);
+  if (pro) {
+    Pw.alloc((size_t)n * b);
+    part.alloc((size_t)nblk * 2 * b);
+    sums.alloc((size_t)2 * b);
+    scale.alloc(b);
+  }
   HIP_CHECK(hipMemcpyAsync(L.get(), q0_h, sizeof(double) * n * b, hipMemcpyHostToDevice, st));
 
   std::vector<double> G((size_t)b * b), A((size_t)b * b), Bprev((size_t)b * b, 0.0), Bt((size_t)b * b);
@@ -422,6 +432,45 @@ void lanczos_run(Handle& H, const adc2g_lanczos_params& p, const double* q0_h, d
     ARG_CHECK(dev <= 1e-8, "q0 must have orthonormal columns (max |q0^T q0 - I| = " + std::to_string(dev) + ")");
   }
 
+  // partial re-orthogonalisation (orthotype 1): the omega recurrence on the
+  // host, the block operations on the device
+  const double epmach = DBL_EPSILON;  // machine_precision (:927-943) returns 2^-52
+  const double om_eps = epmach * b * std::sqrt((double)n), orthlim = std::sqrt(epmach);
+  std::vector<double> om((size_t)(m + 2) * (m + 2), 0.0), anorm(m + 2, 0.0), bnorm(m + 2, 0.0);
+  std::vector<std::vector<double>> A_all, B_all;
+  auto OM = [&](int r, int c) -> double& { return om[r + (size_t)(m + 2) * c]; };
+  // singular values of a b x b matrix: {largest, smallest}
+  auto svd_ends = [&](const std::vector<double>& M) {
+    std::vector<double> MtM((size_t)b * b), w(b), Z((size_t)b * b);
+    for (int c = 0; c < b; ++c)
+      for (int r = 0; r < b; ++r) {
+        double t = 0.0;
+        for (int k = 0; k < b; ++k) t += M[k + (size_t)b * r] * M[k + (size_t)b * c];
+        MtM[r + (size_t)b * c] = t;
+      }
+    if (!host_syev(b, MtM.data(), w.data(), Z.data())) throw Error(ADC2G_EEIG, "block Lanczos: SVD failed");
+    return std::make_pair(std::sqrt(std::max(w[b - 1], 0.0)), std::sqrt(std::max(w[0], 0.0)));
+  };
+  // Q <- Q - S C with C = S^T Q (b columns each)
+  auto project_out = [&](const double* S, double* Q) {
+    gemm_tn(H, b, b, n, S, n, Q, n, small.get(), b, 1.0, 0.0);
+    gemm_nn(H, n, b, b, S, n, small.get(), b, Q, n, -1.0, 1.0);
+  };
+  // columns of Q scaled to unit 2-norm
+  auto normalise_cols = [&](double* Q) {
+    hipLaunchKernelGGL(ritz_stats_kernel, dim3(nblk, b), dim3(LZ_THREADS), 0, st, n, b, Q, n, nullptr, part.get());
+    HIP_CHECK(hipGetLastError());
+    launch_colsum(nblk, 2 * b, part.get(), sums.get(), st);
+    std::vector<double> h((size_t)2 * b), sc(b);
+    HIP_CHECK(hipMemcpyAsync(h.data(), sums.get(), sizeof(double) * 2 * b, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    for (int c = 0; c < b; ++c) sc[c] = 1.0 / std::sqrt(h[2 * c]);
+    HIP_CHECK(hipMemcpyAsync(scale.get(), sc.data(), sizeof(double) * b, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(scale_cols_kernel, dim3(nblk, b), dim3(LZ_THREADS), 0, st, n, Q, n, scale.get());
+    HIP_CHECK(hipGetLastError());
+  };
+  int npro = 0;
+
   Timer tm;
   double t_sigma = 0.0;
   const double u = 0.5 * DBL_EPSILON;
@@ -435,7 +484,7 @@ void lanczos_run(Handle& H, const adc2g_lanczos_params& p, const double* q0_h, d
       for (int r = 0; r < b; ++r)
         for (int c = 0; c < b; ++c) Bt[r + (size_t)b * c] = Bprev[c + (size_t)b * r];
       upload_small(Bt);
-      gemm_nn(H, n, b, b, Qj - (size_t)n * b, n, small.get(), b, U.get(), n, -1.0, 1.0);
+      gemm_nn(H, n, b, b, pro ? Pw.get() : Qj - (size_t)n * b, n, small.get(), b, U.get(), n, -1.0, 1.0);
     }
     // A_j = Q_j^T U
     gemm_tn(H, b, b, n, Qj, n, U.get(), n, small.get(), b, 1.0, 0.0);
@@ -483,6 +532,53 @@ void lanczos_run(Handle& H, const adc2g_lanczos_params& p, const double* q0_h, d
     // B_j = R3 R2 R1
     upper_mul(b, R2.data(), R1.data(), W1.data());
     upper_mul(b, R3.data(), W1.data(), Bprev.data());
+    if (pro) {
+      // the working Q_j (the stored block keeps its value for the Ritz vectors)
+      HIP_CHECK(hipMemcpyAsync(Pw.get(), Qj, sizeof(double) * n * b, hipMemcpyDeviceToDevice, st));
+      A_all.push_back(A);
+      B_all.push_back(Bprev);
+      const int jj = j + 1;  // the reference's 1-based cycle
+      bool lro = false;
+      if (jj > 1 && jj < m) {
+        // update_omega (:1530-1639)
+        if (jj == 2) {
+          anorm[1] = svd_ends(A_all[0]).first;
+          bnorm[1] = svd_ends(B_all[0]).first;
+          const auto s2 = svd_ends(B_all[1]);
+          bnorm[2] = s2.first;
+          OM(1, 1) = OM(2, 2) = om_eps;
+          OM(1, 2) = OM(2, 1) = om_eps / s2.second;
+        }
+        anorm[jj] = svd_ends(A_all[jj - 1]).first;
+        const auto sb = svd_ends(B_all[jj - 1]);
+        bnorm[jj] = sb.first;
+        const double inv = 1.0 / sb.second;
+        OM(jj, jj + 1) = OM(jj + 1, jj) = OM(jj + 1, jj + 1) = om_eps;
+        for (int k = 1; k < jj; ++k) {
+          double x = bnorm[k + 1] * OM(jj, k + 1);
+          if (k > 1) x += bnorm[k] * OM(jj, k - 1);
+          x += bnorm[jj] * OM(jj - 1, k) + (anorm[jj] + anorm[k]) * OM(jj, k);
+          OM(jj + 1, k) = OM(k, jj + 1) = inv * x;
+        }
+        for (int k = 1; k < jj; ++k)
+          if (OM(jj + 1, k) >= orthlim) lro = true;
+        if (lro) {
+          // pro_mgs (:1643-1706): Q_j and Q_{j+1} against the stored blocks 1..j-1,
+          // Q_j normalised, then localro
+          ++npro;
+          for (int k = 0; k + 1 < jj; ++k) {
+            const double* Lk = L.get() + (size_t)n * b * k;
+            project_out(Lk, Pw.get());
+            project_out(Lk, Qn);
+          }
+          normalise_cols(Pw.get());
+          for (int k = 1; k <= jj; ++k) OM(jj + 1, k) = OM(jj, k) = om_eps;
+        }
+      }
+      // localro (:1430-1470): Q_{j+1} against Q_j, normalised
+      project_out(Pw.get(), Qn);
+      normalise_cols(Qn);
+    }
     for (int c = 0; c < b; ++c)
       for (int r = 0; r < b; ++r) {
         const double v = Bprev[r + (size_t)b * c];
@@ -564,6 +660,8 @@ void lanczos_run(Handle& H, const adc2g_lanczos_params& p, const double* q0_h, d
     stats->time_ritz = time_ritz;
     stats->time_total = std::chrono::duration<double>(clk::now() - t_start).count();
     stats->orth_loss = orth_loss;
+    stats->npro = npro;
+    stats->reserved = 0;
   }
 }
 

@@ -78,6 +78,8 @@ module adc2g_mod
      integer(c_int32_t) :: ncycles = 0
      integer(c_int32_t) :: precision = 0
      integer(c_int32_t) :: check_orth = 0
+     integer(c_int32_t) :: orthotype = 0      ! 0 none, 1 partial re-orthogonalisation
+     integer(c_int32_t) :: reserved = 0
   end type adc2g_lanczos_params
 
   ! relaxation (source/propagation/relaxation.f90:36-133, XSIL integrator):
@@ -101,7 +103,7 @@ module adc2g_mod
   end type adc2g_relax_stats
 
   type, bind(c) :: adc2g_lanczos_stats
-     integer(c_int32_t) :: nvec, nsigma
+     integer(c_int32_t) :: nvec, nsigma, npro, reserved
      real(c_double)     :: time_sigma, time_tdiag, time_ritz, time_total, orth_loss
   end type adc2g_lanczos_stats
 

@@ -70,11 +70,12 @@ class SilStats(C.Structure):
 
 class LanczosParams(C.Structure):
     _fields_ = [("blocksize", C.c_int32), ("ncycles", C.c_int32), ("precision", C.c_int32),
-                ("check_orth", C.c_int32)]
+                ("check_orth", C.c_int32), ("orthotype", C.c_int32), ("reserved", C.c_int32)]
 
 
 class LanczosStats(C.Structure):
-    _fields_ = [("nvec", C.c_int32), ("nsigma", C.c_int32), ("time_sigma", C.c_double),
+    _fields_ = [("nvec", C.c_int32), ("nsigma", C.c_int32), ("npro", C.c_int32), ("reserved", C.c_int32),
+                ("time_sigma", C.c_double),
                 ("time_tdiag", C.c_double), ("time_ritz", C.c_double), ("time_total", C.c_double),
                 ("orth_loss", C.c_double)]
 
@@ -305,7 +306,7 @@ class Operator:
         return out
 
     def block_lanczos(self, q0, ncycles, precision="fp64", travec=None, tmat=False, ritz=False,
-                      check_orth=False):
+                      check_orth=False, orthotype=0):
         """Block Lanczos pseudospectrum from the orthonormal block q0 (ndim x b).
 
         Returns (eigval, extras, stats); extras holds "tranmom" (travec given),
@@ -313,7 +314,7 @@ class Operator:
         q0 = np.asfortranarray(np.asarray(q0, dtype=np.float64).reshape(self.ndim, -1))
         b = q0.shape[1]
         nvec = b * int(ncycles)
-        p = LanczosParams(b, int(ncycles), PREC[precision], 1 if check_orth else 0)
+        p = LanczosParams(b, int(ncycles), PREC[precision], 1 if check_orth else 0, int(orthotype), 0)
         ev = np.empty(nvec)
         extras = {}
         tm_out = f = None

@@ -131,11 +131,15 @@ typedef struct {
   int32_t ncycles;       /* iter: blocks generated; nvec = blocksize * ncycles */
   int32_t precision;     /* ADC2G_PREC_* of the sigma-vectors */
   int32_t check_orth;    /* 1: stats->orth_loss = max |L^T L - I| over the Lanczos vectors */
+  int32_t orthotype;     /* 0: none (default); 1: partial re-orthogonalisation (pro) */
+  int32_t reserved;
 } adc2g_lanczos_params;
 
 typedef struct {
   int32_t nvec;          /* Lanczos states = blocksize * ncycles */
   int32_t nsigma;        /* sigma-vectors applied */
+  int32_t npro;          /* partial re-orthogonalisations (orthotype 1) */
+  int32_t reserved;
   double time_sigma;     /* s in the sigma-vectors (device events) */
   double time_tdiag;     /* s in the eigen-decomposition of T (host) */
   double time_ritz;      /* s forming the Ritz vectors and transition moments */
@@ -279,10 +283,13 @@ int32_t adc2g_cheby_auto(adc2g_handle* h, const double* bounds, const double* q0
                          int32_t precision, double* auto_out);
 
 /* Block Lanczos pseudospectrum: run_block_lanczos
- * (source/eigen/block_lanczos.f90:947-1256) in its default form, orthotype 0
- * (no re-orthogonalisation), with hxq_incore (:1325-1361) replaced by the
- * sigma-vector of the block; the partial re-orthogonalisation variants
- * (orthotype 1, 2: pro, mpro, :1474-1845) are not provided.
+ * (source/eigen/block_lanczos.f90:947-1256) with hxq_incore (:1325-1361)
+ * replaced by the sigma-vector of the block.  orthotype 0 (default) does no
+ * re-orthogonalisation.  orthotype 1 adds partial re-orthogonalisation: pro
+ * with the omega recurrence (:1474-1639), pro_mgs against the stored blocks
+ * (:1643-1706) and localro (:1430-1470), with block Gram-Schmidt steps in
+ * place of the vector-by-vector ones.  orthotype 2 (mpro, :1710-1845) is not
+ * provided.
  *   U = H Q_j - Q_{j-1} B_{j-1}^T,  A_j = Q_j^T U,  R = U - Q_j A_j = Q_{j+1} B_j
  * with B_j upper triangular.  The reference factors R with dgeqrf/dorgqr; here a
  * shifted Cholesky QR with two refinement passes gives the factor with a

@@ -30,12 +30,34 @@ from __future__ import annotations
 import numpy as np
 
 
-def block_lanczos(matvec, q0: np.ndarray, ncycles: int) -> dict:
+EPMACH = 2.0 ** -52   # machine_precision (block_lanczos.f90:927-943)
+
+
+def _localro(q1, q2):
+    """localro (:1430-1470): Q_{j+1} orthogonalised against the vectors of Q_j
+    one after another, then its columns normalised."""
+    q2 = q2.copy()
+    for m in range(q1.shape[1]):
+        q2 -= np.outer(q1[:, m], q2.T @ q1[:, m])
+    return q2 / np.sqrt((q2 * q2).sum(axis=0))
+
+
+def block_lanczos(matvec, q0: np.ndarray, ncycles: int, orthotype: int = 0) -> dict:
     """``matvec(X)`` returns H X for an (n, b) block; q0 holds orthonormal columns.
 
+    orthotype 1 adds partial re-orthogonalisation (pro, :1474-1526).  The
+    omega recurrence (update_omega, :1530-1639) estimates the loss of
+    orthogonality from the spectral norms of A_j and B_j and the smallest
+    singular value of B_j.  When an estimate reaches sqrt(eps), pro_mgs
+    (:1643-1706) orthogonalises the working Q_j and Q_{j+1} against the
+    stored blocks 1..j-1 vector by vector, normalises Q_j and applies localro.
+    Otherwise localro alone runs (:1192).  The stored Lanczos vectors (Ritz
+    vectors, later re-orthogonalisations) are the blocks as they were at the
+    start of their cycle; the recurrence uses the modified working Q_j.
+
     Returns T, eigval (ascending), Y (eigenvectors of T), ritz (normalised
-    L Y), L (the Lanczos vectors), and Qnext / Bnext (the block and the
-    triangle of the last cycle's QR, which T leaves out).
+    L Y), L (the Lanczos vectors), Qnext / Bnext (the block and the triangle
+    of the last cycle's QR, which T leaves out) and npro (re-orthogonalisations).
     """
     q0 = np.asarray(q0, dtype=np.float64)
     n, b = q0.shape
@@ -45,6 +67,13 @@ def block_lanczos(matvec, q0: np.ndarray, ncycles: int) -> dict:
     q_prev = np.zeros((n, b))
     b_prev = np.zeros((b, b))
     q_cur = q0.copy()
+    eps = EPMACH * b * np.sqrt(float(n))
+    orthlim = np.sqrt(EPMACH)
+    om = np.zeros((ncycles + 2, ncycles + 2))
+    anorm = np.zeros(ncycles + 1)
+    bnorm = np.zeros(ncycles + 1)
+    amats, bmats = [], []
+    npro = 0
     for j in range(ncycles):
         c0 = j * b
         L[:, c0:c0 + b] = q_cur
@@ -52,15 +81,57 @@ def block_lanczos(matvec, q0: np.ndarray, ncycles: int) -> dict:
         a = q_cur.T @ u
         r = u - q_cur @ a
         q_next, b_next = np.linalg.qr(r)
+        amats.append(a)
+        bmats.append(b_next)
+        q_prev, q_cur = q_cur.copy(), q_next
+        jj = j + 1                                   # the reference's 1-based cycle
+        lro = False
+        if orthotype == 1 and 1 < jj < ncycles:
+            sv = lambda m: np.linalg.svd(m, compute_uv=False)
+            if jj == 2:
+                anorm[1] = sv(amats[0])[0]
+                bnorm[1] = sv(bmats[0])[0]
+                s2 = sv(bmats[1])
+                bnorm[2] = s2[0]
+                om[1, 1] = om[2, 2] = eps
+                om[1, 2] = om[2, 1] = eps / s2[-1]
+            anorm[jj] = sv(amats[jj - 1])[0]
+            sb = sv(bmats[jj - 1])
+            bnorm[jj] = sb[0]
+            inv = 1.0 / sb[-1]
+            om[jj, jj + 1] = om[jj + 1, jj] = om[jj + 1, jj + 1] = eps
+            for k in range(1, jj):
+                x = bnorm[k + 1] * om[jj, k + 1] + bnorm[jj] * om[jj - 1, k] + (anorm[jj] + anorm[k]) * om[jj, k]
+                if k > 1:
+                    x += bnorm[k] * om[jj, k - 1]
+                om[jj + 1, k] = om[k, jj + 1] = inv * x
+            if any(om[jj + 1, k] >= orthlim for k in range(1, jj)):
+                lro = True
+                npro += 1
+                for k in range(1, jj):
+                    for m in range(b):
+                        l = L[:, (k - 1) * b + m]
+                        d1 = q_prev.T @ l
+                        d2 = q_cur.T @ l
+                        q_prev -= np.outer(l, d1)
+                        q_cur -= np.outer(l, d2)
+                q_prev /= np.sqrt((q_prev * q_prev).sum(axis=0))
+                q_cur = _localro(q_prev, q_cur)
+                for k in range(1, jj + 1):
+                    om[jj + 1, k] = eps
+                    om[jj, k] = eps
+        if orthotype == 1 and not lro:
+            q_cur = _localro(q_prev, q_cur)
         T[c0:c0 + b, c0:c0 + b] = np.triu(a) + np.triu(a, 1).T
         if j + 1 < ncycles:
             T[c0 + b:c0 + 2 * b, c0:c0 + b] = b_next
             T[c0:c0 + b, c0 + b:c0 + 2 * b] = b_next.T
-        q_prev, q_cur, b_prev = q_cur, q_next, b_next
+        b_prev = b_next
     lam, Y = np.linalg.eigh(T, UPLO="L")
     ritz = L @ Y
     ritz /= np.sqrt((ritz * ritz).sum(axis=0))
-    return {"T": T, "eigval": lam, "Y": Y, "ritz": ritz, "L": L, "Qnext": q_cur, "Bnext": b_prev}
+    return {"T": T, "eigval": lam, "Y": Y, "ritz": ritz, "L": L, "Qnext": q_cur, "Bnext": b_prev,
+            "npro": npro}
 
 
 def tranmom(ritz: np.ndarray, travec: np.ndarray) -> np.ndarray:

@@ -107,3 +107,31 @@ def test_lanczos_benzene_quadrature():
     for k, ref in enumerate(moments):
         got = (tm ** 2 * ev ** k).sum()
         assert abs(got - ref) < 1e-9 * (tm ** 2 * np.abs(ev) ** k).sum(), k
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_lanczos_partial_reorthogonalisation_vs_oracle(form):
+    """orthotype 1 on a run that loses orthogonality without it: the same
+    number of re-orthogonalisations as the oracle, orthogonal stored vectors
+    and every eigenvalue of T to 1e-10 relative (the extreme ones equal to
+    the exact spectrum, the top one once); orthotype 0 on the same run loses
+    orthogonality (max |L^T L - I| > 1e-2)."""
+    print(f"[case] lanczos pro o=3 v=8 naux=10 b=3 m=40 coupling={form}", flush=True)
+    sys = make_system(3, 8, 10, seed=5)
+    H = elements.build_blocks(sys, apply_minc=False).dense()
+    n = H.shape[0]
+    q0 = np.zeros((n, 3))
+    q0[[0, 5, 11], [0, 1, 2]] = 1.0
+    ref = lanczos.block_lanczos(lambda X: H @ X, q0, 40, orthotype=1)
+    lam = np.linalg.eigvalsh(H)
+    with lib.Operator.from_system(sys, coupling=form) as op:
+        ev, _, st = op.block_lanczos(q0, 40, orthotype=1, check_orth=True)
+        ev0, _, st0 = op.block_lanczos(q0, 40, orthotype=0, check_orth=True)
+    scale = np.abs(lam).max()
+    assert st["npro"] == ref["npro"] > 0
+    assert st["orth_loss"] < 1e-10 and st0["orth_loss"] > 1e-2
+    assert np.abs(ev - ref["eigval"]).max() < 1e-10 * scale
+    assert np.abs(ev[-3:] - lam[-3:]).max() < 1e-10 * scale
+    # with re-orthogonalisation the top eigenvalue appears once (no ghost copy)
+    assert (np.abs(ev - lam[-1]) < 1e-6).sum() == 1
+    assert ev0.size == ev.size

@@ -220,6 +220,29 @@ def test_oracle_block_lanczos_recurrence_and_quadrature():
         hk = H @ hk
 
 
+def test_oracle_block_lanczos_partial_reorthogonalisation():
+    """orthotype 1 on a run long enough to lose orthogonality without it.
+    Without it, max |L^T L - I| is O(1) and a ghost copy of a converged
+    eigenvalue appears; with it, the stored vectors stay orthogonal to well
+    below sqrt(eps), re-orthogonalisation happens, and every extreme
+    eigenvalue appears once and matches eigh.  A wrong omega recurrence that
+    never triggers, or a pro_mgs that misses a block, fails it."""
+    sys = make_system(3, 8, 10, seed=5)
+    H = elements.build_blocks(sys, apply_minc=False).dense()
+    n = H.shape[0]
+    lam = np.linalg.eigvalsh(H)
+    q0 = _unit_block(n, [0, 5, 11])
+    out0 = lanczos.block_lanczos(lambda X: H @ X, q0, 40, orthotype=0)
+    out1 = lanczos.block_lanczos(lambda X: H @ X, q0, 40, orthotype=1)
+    L0, L1 = out0["L"], out1["L"]
+    assert np.abs(L0.T @ L0 - np.eye(L0.shape[1])).max() > 1e-2
+    assert out1["npro"] > 0
+    assert np.abs(L1.T @ L1 - np.eye(L1.shape[1])).max() < 1e-10
+    ev = out1["eigval"]
+    assert np.abs(ev[:3] - lam[:3]).max() < 1e-10 and np.abs(ev[-3:] - lam[-3:]).max() < 1e-10
+    assert (np.abs(ev - lam[-1]) < 1e-6).sum() == 1
+
+
 def test_oracle_block_lanczos_full_krylov():
     """With blocksize * ncycles = ndim the Krylov space is the whole space and
     the spectrum of T is the spectrum of H."""
