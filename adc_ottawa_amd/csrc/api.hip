@@ -302,6 +302,14 @@ int32_t adc2g_mp2(adc2g_handle* h, double* e_mp2, double* d2, double* rho) {
   });
 }
 
+int32_t adc2g_spectral_bounds(adc2g_handle* h, int32_t ncycles, double* bounds) {
+  return guarded([&] {
+    ARG_CHECK(h && bounds, "NULL argument");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    spectral_bounds_davlanc(h->h, ncycles, bounds);
+  });
+}
+
 int32_t adc2g_sigma_profile(adc2g_handle* h, int32_t enable, double* out) {
   return guarded([&] {
     ARG_CHECK(h != nullptr, "NULL handle");

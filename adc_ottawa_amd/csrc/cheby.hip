@@ -115,4 +115,48 @@ void cheby_run(Handle& H, const double bounds[2], const double* q0_h, int order,
   }
 }
 
+// spectral_bounds_davlanc (source/propagation/specbounds.f90:85-160): the
+// lower bound is the lowest root of block Davidson (davstates 1, dmain 5,
+// maxsubdim 20, davtol 1e-8, the final-space defaults precon_f = 2 (Olsen),
+// ldfl_f = true, maxiter_f = 100, sub-diagonalisation guess with guessdim_f =
+// 800; source/adclib/defaults.f90:84-95).  The upper bound is the highest
+// eigenvalue of block Lanczos (lmain 2, ncycles 100, orthotype 1) from
+// init_vec_singlesubspace (lancguess 9).  The caller of spectral_bounds widens
+// them (chebyspec.f90:82-83 multiplies by 0.9 and 1.1); here they are
+// returned as estimated.
+void spectral_bounds_davlanc(Handle& H, int ncycles, double bounds[2]) {
+  const int64_t n = H.d.ndim;
+  adc2g_dav_params dp{};
+  dp.nstates = 1;
+  dp.blocksize = 5;
+  dp.maxvec = 20;
+  dp.maxiter = 100;
+  dp.tol = 1e-8;
+  dp.precon = 2;
+  dp.deflate = 1;
+  dp.precision = 0;
+  dp.refine = 1;
+  dp.guess = ADC2G_GUESS_SUBDIAG;
+  dp.subdim = 800;
+  double lo = 0.0;
+  davidson_run(H, dp, &lo, nullptr, nullptr, nullptr);
+  const int m = ncycles > 0 ? ncycles : 100;
+  DevBuf<double> g;
+  g.alloc((size_t)n * 2);
+  subdiag_guess(H, 800, 2, g.get(), true);
+  std::vector<double> q0((size_t)n * 2);
+  HIP_CHECK(hipMemcpyAsync(q0.data(), g.get(), sizeof(double) * q0.size(), hipMemcpyDeviceToHost, H.st));
+  HIP_CHECK(hipStreamSynchronize(H.st));
+  adc2g_lanczos_params lp{};
+  lp.blocksize = 2;
+  lp.ncycles = m;
+  lp.precision = 0;
+  lp.check_orth = 0;
+  lp.orthotype = 1;
+  std::vector<double> ev((size_t)2 * m);
+  lanczos_run(H, lp, q0.data(), ev.data(), nullptr, nullptr, nullptr, nullptr, nullptr);
+  bounds[0] = lo;
+  bounds[1] = ev.back();
+}
+
 }  // namespace adc2g

@@ -86,14 +86,18 @@ __global__ void scatter_rows_kernel(int nr, int nc, const int64_t* __restrict__ 
   V[idx[r] + ldv * c] = U[r + (int64_t)ldu * c];
 }
 
+}  // namespace
+
 // initvec_subdiag (source/eigen/block_davidson.f90:293-459): the subdim ISs
 // of lowest H_jj (ascending; subdim grows while the next H_jj is within 1e-6
 // of the last, so a degenerate group is not split), the stored matrix on
 // them (diagonal H_jj, off-diagonal elements |h| > 1e-12 taken from the upper
 // triangle i < j as the reference stores them), its lowest bs eigenvectors
 // scattered into V (n x bs, ld n).  The columns come from FP64 sigma-vectors
-// of unit vectors on the selected ISs.
-void subdiag_guess(Handle& H, int subdim, int bs, double* V) {
+// of unit vectors on the selected ISs.  With descending, the ISs of highest
+// H_jj (init_vec_singlesubspace, source/eigen/block_lanczos.f90:748-923, which
+// also keeps the lowest eigenvectors of that block).
+void subdiag_guess(Handle& H, int subdim, int bs, double* V, bool descending) {
   const int64_t n = H.d.ndim;
   hipStream_t st = H.st;
   std::vector<double> hii(n);
@@ -101,7 +105,10 @@ void subdiag_guess(Handle& H, int subdim, int bs, double* V) {
   HIP_CHECK(hipStreamSynchronize(st));
   std::vector<int64_t> order(n);
   std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return hii[a] < hii[b]; });
+  if (descending)
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return hii[a] > hii[b]; });
+  else
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return hii[a] < hii[b]; });
   int64_t sd = std::min<int64_t>(subdim, n);
   while (sd < n && std::fabs(hii[order[sd - 1]] - hii[order[sd]]) < 1e-6) ++sd;
   ARG_CHECK(sd <= 4096, "the sub-diagonalisation guess is limited to 4096 ISs");
@@ -155,6 +162,8 @@ void subdiag_guess(Handle& H, int subdim, int bs, double* V) {
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(st));
 }
+
+namespace {
 
 struct Dav {
   Handle& H;
@@ -330,7 +339,7 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
     ARG_CHECK(dev <= 1e-8, "the initial block must have orthonormal columns (max |G - I| = " + std::to_string(dev) +
                                ")");
   } else if (p.guess == ADC2G_GUESS_SUBDIAG) {
-    subdiag_guess(H, p.subdim > 0 ? p.subdim : 800, bs, D.V.get());
+    subdiag_guess(H, p.subdim > 0 ? p.subdim : 800, bs, D.V.get(), false);
   } else {
     // initvec_ondiag: unit vectors on the bs smallest |H_jj| (stable order for ties)
     std::vector<double> hii(n);

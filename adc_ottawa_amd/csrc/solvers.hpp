@@ -51,6 +51,14 @@ void hmlt_write(Handle& H, const char* dia_path, const char* off_path, double mi
 // column-major); any output may be NULL.
 void mp2_run(Handle& H, double* e_mp2, double* d2, double* rho);
 
+// Sub-space diagonalisation guess: the lowest bs eigenvectors of the stored
+// matrix on the subdim ISs of lowest (descending: highest) H_jj, into V
+// (ndim x bs, device).
+void subdiag_guess(Handle& H, int subdim, int bs, double* V, bool descending);
+
+// Spectral bounds of spectral_bounds_davlanc (source/propagation/specbounds.f90:85-160).
+void spectral_bounds_davlanc(Handle& H, int ncycles, double bounds[2]);
+
 int precon_nparams();
 void precon_harvest(Handle& H, const adc2g_dav_params& p, int64_t max_records, int64_t* added);
 void precon_train(Handle& H, const Handle& data, int epochs, int batch, double lr, uint64_t seed, double* loss_out);

@@ -264,6 +264,14 @@ module adc2g_mod
        type(c_ptr), value :: rho
      end function adc2g_mp2
 
+     ! spectral_bounds 'davlanc' (ncycles 0 = 100); widen by 0.9 / 1.1 as chebyspec.f90:82-83
+     integer(c_int32_t) function adc2g_spectral_bounds(h, ncycles, bounds) bind(c, name='adc2g_spectral_bounds')
+       import :: c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: ncycles
+       real(c_double), intent(out) :: bounds(2)
+     end function adc2g_spectral_bounds
+
      integer(c_int32_t) function adc2g_precon_nparams() bind(c, name='adc2g_precon_nparams')
        import :: c_int32_t
      end function adc2g_precon_nparams

@@ -100,6 +100,7 @@ SYMBOLS = [
     "adc2g_sigma_profile", "adc2g_precon_absorb", "adc2g_coupling_form", "adc2g_cheby_auto",
     "adc2g_block_lanczos", "adc2g_relax", "adc2g_write_hmlt", "adc2g_davidson_vectors",
     "adc2g_sil_autocorr_orders", "adc2g_create_cvs", "adc2g_mp2",
+    "adc2g_spectral_bounds",
 ]
 
 _lib = None
@@ -144,6 +145,7 @@ def load(path: str = LIBPATH):
     lib.adc2g_davidson_vectors.argtypes = [vp, C.POINTER(DavParams), vp, vp, vp, C.POINTER(DavStats)]
     lib.adc2g_create_cvs.argtypes = [vp, i32, vp, C.POINTER(vp)]
     lib.adc2g_mp2.argtypes = [vp, vp, vp, vp]
+    lib.adc2g_spectral_bounds.argtypes = [vp, i32, vp]
     lib.adc2g_sil_autocorr_orders.argtypes = [vp, C.POINTER(SilParams), vp, i32, vp, vp, vp, i64,
                                               C.POINTER(i64), C.POINTER(SilStats)]
     lib.adc2g_relax.argtypes = [vp, C.POINTER(RelaxParams), vp, vp, vp, vp, vp, C.POINTER(RelaxStats)]
@@ -368,6 +370,12 @@ class Operator:
         rho = np.empty((nb, nb), order="F") if density else None
         check(_lib.adc2g_mp2(self._h, C.byref(e), C.byref(d), _ptr(rho) if density else None))
         return e.value, d.value, rho
+
+    def spectral_bounds(self, ncycles=0):
+        """(Emin, Emax) as spectral_bounds 'davlanc' estimates them."""
+        b = np.empty(2)
+        check(_lib.adc2g_spectral_bounds(self._h, int(ncycles), _ptr(b)))
+        return float(b[0]), float(b[1])
 
     def cvs(self, core):
         """CVS view on the core orbitals ``core`` (1-based occupied indices);

@@ -285,10 +285,19 @@ def cheby_measure(op, sz, order, seed=0):
     lo, hi = lanczos_extremes(op)
     pad = 0.05 * (hi - lo)
     bounds = (lo - pad, hi + pad)
+    t0 = time.perf_counter()
+    try:
+        dl = op.spectral_bounds()
+        log(f"spectral bounds (davlanc): {dl[0]:.6f} .. {dl[1]:.3f} Eh in {time.perf_counter() - t0:.2f}s")
+    except lib.Adc2gError as e:   # reported; the padded Lanczos bounds are used either way
+        dl = str(e)
+        log(f"spectral bounds (davlanc): {e}")
+    t_dl = time.perf_counter() - t0
     q0 = np.zeros(op.ndim)
     q0[: op.n1] = unit_dipole(sz["nocc"], sz["nvirt"], seed=seed)
     q0 /= np.linalg.norm(q0)
-    out, ref = {"bounds_Eh": bounds, "order": order}, None
+    out, ref = {"bounds_Eh": bounds, "order": order, "davlanc_bounds_Eh": dl,
+                "davlanc_seconds": round(t_dl, 3)}, None
     for prec in ("fp64", "fp32"):
         t0 = time.perf_counter()
         auto = op.cheby_auto(q0, bounds, order, precision=prec)

@@ -282,6 +282,20 @@ int32_t adc2g_sil_autocorr_orders(adc2g_handle* h, const adc2g_sil_params* p, co
 int32_t adc2g_cheby_auto(adc2g_handle* h, const double* bounds, const double* q0, int32_t order,
                          int32_t precision, double* auto_out);
 
+/* Spectral bounds of the matrix as spectral_bounds(..., 'davlanc')
+ * (source/propagation/specbounds.f90:85-160) estimates them:
+ *  - bounds[0] is the lowest root of block Davidson (1 root, block 5,
+ *    subspace 20, tol 1e-8, Olsen, deflation, sub-diagonalisation guess on 800
+ *    ISs);
+ *  - bounds[1] is the highest eigenvalue of block Lanczos (block 2, ncycles
+ *    cycles, 0 meaning the reference's 100, partial re-orthogonalisation)
+ *    started from the lowest eigenvectors of the 800 ISs of highest H_jj
+ *    (init_vec_singlesubspace, source/eigen/block_lanczos.f90:748-923).
+ * The reference's Chebyshev driver then multiplies them by 0.9 and 1.1
+ * (chebyspec.f90:82-83), which contains the spectrum only for positive
+ * bounds; here they are returned unwidened. */
+int32_t adc2g_spectral_bounds(adc2g_handle* h, int32_t ncycles, double* bounds);
+
 /* Block Lanczos pseudospectrum: run_block_lanczos
  * (source/eigen/block_lanczos.f90:947-1256) with hxq_incore (:1325-1361)
  * replaced by the sigma-vector of the block.  orthotype 0 (default) does no

@@ -93,3 +93,20 @@ def test_cheby_benzene_doubling_and_bound():
     assert np.abs(a1[k + 1:] - a2[k + 1:2 * k + 1]).max() < 1e-9 * c0
     assert np.abs(a1[: k + 1] - a2[: k + 1]).max() < 1e-12 * c0
     assert np.abs(a2).max() <= c0 * (1.0 + 1e-9)
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_spectral_bounds_davlanc(form):
+    """spectral_bounds 'davlanc': the lowest Davidson root (converged to
+    1e-8, so 1e-9 relative) and the highest block-Lanczos eigenvalue (2 x 100,
+    partial re-orthogonalisation; converged to about 1e-9 relative here, bar
+    1e-7) equal the extreme eigenvalues of the dense matrix."""
+    print(f"[case] spectral bounds o=3 v=8 naux=10 coupling={form}", flush=True)
+    sys = make_system(3, 8, 10, seed=5)
+    H = elements.build_blocks(sys, apply_minc=False).dense()
+    lam = np.linalg.eigvalsh(H)
+    with lib.Operator.from_system(sys, coupling=form) as op:
+        lo, hi = op.spectral_bounds()
+    scale = np.abs(lam).max()
+    assert abs(lo - lam[0]) < 1e-9 * scale
+    assert abs(hi - lam[-1]) < 1e-7 * scale
