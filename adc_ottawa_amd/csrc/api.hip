@@ -302,11 +302,27 @@ int32_t adc2g_mp2(adc2g_handle* h, double* e_mp2, double* d2, double* rho) {
   });
 }
 
-int32_t adc2g_spectral_bounds(adc2g_handle* h, int32_t ncycles, double* bounds) {
+int32_t adc2g_spectral_bounds(adc2g_handle* h, int32_t method, int32_t ncycles, double* bounds) {
   return guarded([&] {
     ARG_CHECK(h && bounds, "NULL argument");
     HIP_CHECK(hipSetDevice(h->h.device));
-    spectral_bounds_davlanc(h->h, ncycles, bounds);
+    spectral_bounds_run(h->h, method, ncycles, bounds);
+  });
+}
+
+int32_t adc2g_power(adc2g_handle* h, int32_t subdim, int32_t maxit, double thrsh, double* eval, double* vec,
+                    int32_t* niter) {
+  return guarded([&] {
+    ARG_CHECK(h && eval, "NULL argument");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    int it = 0;
+    try {
+      power_run(h->h, subdim, maxit > 0 ? maxit : 350, thrsh > 0.0 ? thrsh : 1e-8, eval, vec, &it);
+    } catch (...) {
+      if (niter) *niter = it;
+      throw;
+    }
+    if (niter) *niter = it;
   });
 }
 

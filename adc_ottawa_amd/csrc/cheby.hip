@@ -115,21 +115,115 @@ void cheby_run(Handle& H, const double bounds[2], const double* q0_h, int order,
   }
 }
 
-// spectral_bounds_davlanc (source/propagation/specbounds.f90:85-160): the
-// lower bound is the lowest root of block Davidson (davstates 1, dmain 5,
-// maxsubdim 20, davtol 1e-8, the final-space defaults precon_f = 2 (Olsen),
-// ldfl_f = true, maxiter_f = 100, sub-diagonalisation guess with guessdim_f =
-// 800; source/adclib/defaults.f90:84-95).  The upper bound is the highest
-// eigenvalue of block Lanczos (lmain 2, ncycles 100, orthotype 1) from
-// init_vec_singlesubspace (lancguess 9).  The caller of spectral_bounds widens
+// spectral_bounds (source/propagation/specbounds.f90): the lower bound is the
+// lowest root of block Davidson (davstates 1, davtol 1e-8, dmain/maxsubdim 5/20
+// for 'davlanc' (:85-160) and 2/10 for 'davpower' (:166-238), the final-space
+// defaults precon_f = 2 (Olsen), ldfl_f = true, maxiter_f = 100 and a
+// sub-diagonalisation guess with guessdim_f = 800; defaults.f90:84-95).  The
+// upper bound is the highest eigenvalue of block Lanczos (lmain 2, ncycles
+// 100, orthotype 1) from init_vec_singlesubspace (lancguess 9) for 'davlanc',
+// or the power method for 'davpower'.  The caller of spectral_bounds widens
 // them (chebyspec.f90:82-83 multiplies by 0.9 and 1.1); here they are
 // returned as estimated.
-void spectral_bounds_davlanc(Handle& H, int ncycles, double bounds[2]) {
+namespace {
+
+// part[blk] = {v1.v2} (mode 0) or {(e v1 - v2)^2, v2.v2} (mode 1), e = *ev
+__global__ void __launch_bounds__(CB_THREADS) power_dots_kernel(int64_t n, int mode, const double* __restrict__ v1,
+                                                                const double* __restrict__ v2,
+                                                                const double* __restrict__ ev,
+                                                                double* __restrict__ part) {
+  __shared__ double red[2][CB_THREADS / 64];
+  const double e = mode ? ev[0] : 0.0;
+  double s0 = 0.0, s1 = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    if (mode == 0) {
+      s0 += v1[i] * v2[i];
+    } else {
+      const double r = e * v1[i] - v2[i];
+      s0 += r * r;
+      s1 += v2[i] * v2[i];
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s0 += __shfl_down(s0, off, 64);
+    s1 += __shfl_down(s1, off, 64);
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][wv] = s0;
+    red[1][wv] = s1;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    double t = 0.0;
+    for (int i = 0; i < CB_THREADS / 64; ++i) t += red[threadIdx.x][i];
+    part[(int64_t)blockIdx.x * 2 + threadIdx.x] = t;
+  }
+}
+
+// v1 = v2 / sqrt(*nrm2)
+__global__ void power_next_kernel(int64_t n, const double* __restrict__ v2, const double* __restrict__ nrm2,
+                                  double* __restrict__ v1) {
+  const double inv = 1.0 / sqrt(nrm2[0]);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v1[i] = v2[i] * inv;
+}
+
+}  // namespace
+
+// power_method (source/eigen/power.f90:22-100): guess from power_guess
+// (:307-463, the highest eigenvector on the subdim ISs of highest H_jj),
+// then power_iterations (:467-524): v2 = H v1, E = v1.v2, r = |E v1 - v2|,
+// v2 /= |v2|; stop when r <= thrsh, else v1 = v2.  The eigenpair returned is
+// (E, v1) of the converged iteration, as wreigenpair writes it.
+void power_run(Handle& H, int subdim, int maxit, double thrsh, double* eval, double* vec, int* niter) {
+  const int64_t n = H.d.ndim;
+  ARG_CHECK(maxit >= 1 && thrsh >= 0.0, "maxit must be >= 1 and thrsh >= 0");
+  hipStream_t st = H.st;
+  DevBuf<double> v1, v2, part, sums;
+  v1.alloc(n);
+  v2.alloc(n);
+  const int nblk = reduce_blocks(n);
+  part.alloc((size_t)nblk * 2);
+  sums.alloc(4);
+  subdiag_guess(H, subdim > 0 ? subdim : 800, 1, v1.get(), true, true);
+  double e = 0.0, r = INFINITY;
+  int it = 0;
+  for (it = 1; it <= maxit; ++it) {
+    H.sigma(v1.get(), n, v2.get(), n, 1, Prec::F64, st);
+    hipLaunchKernelGGL(power_dots_kernel, dim3(nblk), dim3(CB_THREADS), 0, st, n, 0, v1.get(), v2.get(), nullptr,
+                       part.get());
+    HIP_CHECK(hipGetLastError());
+    launch_colsum(nblk, 2, part.get(), sums.get(), st);  // sums[0] = E
+    hipLaunchKernelGGL(power_dots_kernel, dim3(nblk), dim3(CB_THREADS), 0, st, n, 1, v1.get(), v2.get(), sums.get(),
+                       part.get());
+    HIP_CHECK(hipGetLastError());
+    launch_colsum(nblk, 2, part.get(), sums.get() + 2, st);  // |r|^2, |v2|^2
+    double h[4];
+    HIP_CHECK(hipMemcpyAsync(h, sums.get(), sizeof(h), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    e = h[0];
+    r = std::sqrt(h[2]);
+    if (r <= thrsh) break;
+    hipLaunchKernelGGL(power_next_kernel, dim3(nblk), dim3(CB_THREADS), 0, st, n, v2.get(), sums.get() + 3, v1.get());
+    HIP_CHECK(hipGetLastError());
+  }
+  if (eval) *eval = e;
+  if (niter) *niter = std::min(it, maxit);
+  if (vec) {
+    HIP_CHECK(hipMemcpyAsync(vec, v1.get(), sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+  }
+  if (!(r <= thrsh)) throw Error(ADC2G_ENOCONV, "power method: convergence not reached...");
+}
+
+void spectral_bounds_run(Handle& H, int method, int ncycles, double bounds[2]) {
+  ARG_CHECK(method == 1 || method == 2, "method must be 1 (davlanc) or 2 (davpower)");
   const int64_t n = H.d.ndim;
   adc2g_dav_params dp{};
   dp.nstates = 1;
-  dp.blocksize = 5;
-  dp.maxvec = 20;
+  dp.blocksize = method == 1 ? 5 : 2;   // davlanc: dmain_f 5, maxsubdim_f 20; davpower: 2, 10
+  dp.maxvec = method == 1 ? 20 : 10;
   dp.maxiter = 100;
   dp.tol = 1e-8;
   dp.precon = 2;
@@ -140,10 +234,18 @@ void spectral_bounds_davlanc(Handle& H, int ncycles, double bounds[2]) {
   dp.subdim = 800;
   double lo = 0.0;
   davidson_run(H, dp, &lo, nullptr, nullptr, nullptr);
+  if (method == 2) {
+    // upper bound from the power method (specbounds.f90:218-229)
+    double hi = 0.0;
+    power_run(H, 800, 350, 1e-8, &hi, nullptr, nullptr);
+    bounds[0] = lo;
+    bounds[1] = hi;
+    return;
+  }
   const int m = ncycles > 0 ? ncycles : 100;
   DevBuf<double> g;
   g.alloc((size_t)n * 2);
-  subdiag_guess(H, 800, 2, g.get(), true);
+  subdiag_guess(H, 800, 2, g.get(), true, false);
   std::vector<double> q0((size_t)n * 2);
   HIP_CHECK(hipMemcpyAsync(q0.data(), g.get(), sizeof(double) * q0.size(), hipMemcpyDeviceToHost, H.st));
   HIP_CHECK(hipStreamSynchronize(H.st));

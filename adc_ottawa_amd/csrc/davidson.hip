@@ -96,8 +96,9 @@ __global__ void scatter_rows_kernel(int nr, int nc, const int64_t* __restrict__ 
 // scattered into V (n x bs, ld n).  The columns come from FP64 sigma-vectors
 // of unit vectors on the selected ISs.  With descending, the ISs of highest
 // H_jj (init_vec_singlesubspace, source/eigen/block_lanczos.f90:748-923, which
-// also keeps the lowest eigenvectors of that block).
-void subdiag_guess(Handle& H, int subdim, int bs, double* V, bool descending) {
+// also keeps the lowest eigenvectors of that block); with top, the highest bs
+// eigenvectors instead (power_guess, source/eigen/power.f90:307-463).
+void subdiag_guess(Handle& H, int subdim, int bs, double* V, bool descending, bool top) {
   const int64_t n = H.d.ndim;
   hipStream_t st = H.st;
   std::vector<double> hii(n);
@@ -154,7 +155,9 @@ void subdiag_guess(Handle& H, int subdim, int bs, double* V, bool descending) {
     throw Error(ADC2G_EEIG, "the diagonalisation of the subspace Hamiltonian failed");
   DevBuf<double> Ud;
   Ud.alloc((size_t)m * bs);
-  HIP_CHECK(hipMemcpyAsync(Ud.get(), U.data(), sizeof(double) * m * bs, hipMemcpyHostToDevice, st));
+  // the lowest bs eigenvectors, or with top the highest ones (power_guess)
+  const size_t first = top ? (size_t)(m - bs) : 0;
+  HIP_CHECK(hipMemcpyAsync(Ud.get(), U.data() + first * m, sizeof(double) * m * bs, hipMemcpyHostToDevice, st));
   HIP_CHECK(hipMemsetAsync(V, 0, sizeof(double) * n * bs, st));
   const int64_t tot = (int64_t)m * bs;
   hipLaunchKernelGGL(scatter_rows_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, m, bs, idx.get(),
@@ -339,7 +342,7 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
     ARG_CHECK(dev <= 1e-8, "the initial block must have orthonormal columns (max |G - I| = " + std::to_string(dev) +
                                ")");
   } else if (p.guess == ADC2G_GUESS_SUBDIAG) {
-    subdiag_guess(H, p.subdim > 0 ? p.subdim : 800, bs, D.V.get(), false);
+    subdiag_guess(H, p.subdim > 0 ? p.subdim : 800, bs, D.V.get(), false, false);
   } else {
     // initvec_ondiag: unit vectors on the bs smallest |H_jj| (stable order for ties)
     std::vector<double> hii(n);

@@ -54,10 +54,16 @@ void mp2_run(Handle& H, double* e_mp2, double* d2, double* rho);
 // Sub-space diagonalisation guess: the lowest bs eigenvectors of the stored
 // matrix on the subdim ISs of lowest (descending: highest) H_jj, into V
 // (ndim x bs, device).
-void subdiag_guess(Handle& H, int subdim, int bs, double* V, bool descending);
+void subdiag_guess(Handle& H, int subdim, int bs, double* V, bool descending, bool top);
 
-// Spectral bounds of spectral_bounds_davlanc (source/propagation/specbounds.f90:85-160).
-void spectral_bounds_davlanc(Handle& H, int ncycles, double bounds[2]);
+// Power method for the highest eigenpair (power_method, source/eigen/power.f90);
+// vec (host, ndim) and niter may be NULL.  Throws ADC2G_ENOCONV when the
+// residual stays above thrsh after maxit iterations.
+void power_run(Handle& H, int subdim, int maxit, double thrsh, double* eval, double* vec, int* niter);
+
+// Spectral bounds of spectral_bounds (source/propagation/specbounds.f90):
+// method 1 'davlanc' (:85-160), 2 'davpower' (:166-238).
+void spectral_bounds_run(Handle& H, int method, int ncycles, double bounds[2]);
 
 int precon_nparams();
 void precon_harvest(Handle& H, const adc2g_dav_params& p, int64_t max_records, int64_t* added);

@@ -264,13 +264,27 @@ module adc2g_mod
        type(c_ptr), value :: rho
      end function adc2g_mp2
 
-     ! spectral_bounds 'davlanc' (ncycles 0 = 100); widen by 0.9 / 1.1 as chebyspec.f90:82-83
-     integer(c_int32_t) function adc2g_spectral_bounds(h, ncycles, bounds) bind(c, name='adc2g_spectral_bounds')
+     ! spectral_bounds: method 1 'davlanc' (ncycles 0 = 100), 2 'davpower';
+     ! widen by 0.9 / 1.1 as chebyspec.f90:82-83 when both are positive
+     integer(c_int32_t) function adc2g_spectral_bounds(h, method, ncycles, bounds) &
+          bind(c, name='adc2g_spectral_bounds')
        import :: c_int32_t, c_double, c_ptr
        type(c_ptr), value :: h
-       integer(c_int32_t), value :: ncycles
+       integer(c_int32_t), value :: method, ncycles
        real(c_double), intent(out) :: bounds(2)
      end function adc2g_spectral_bounds
+
+     ! power method (power_method): vec = c_loc of ndim doubles or c_null_ptr
+     integer(c_int32_t) function adc2g_power(h, subdim, maxit, thrsh, eval, vec, niter) &
+          bind(c, name='adc2g_power')
+       import :: c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: subdim, maxit
+       real(c_double), value :: thrsh
+       real(c_double), intent(out) :: eval
+       type(c_ptr), value :: vec
+       integer(c_int32_t), intent(out) :: niter
+     end function adc2g_power
 
      integer(c_int32_t) function adc2g_precon_nparams() bind(c, name='adc2g_precon_nparams')
        import :: c_int32_t

@@ -100,7 +100,7 @@ SYMBOLS = [
     "adc2g_sigma_profile", "adc2g_precon_absorb", "adc2g_coupling_form", "adc2g_cheby_auto",
     "adc2g_block_lanczos", "adc2g_relax", "adc2g_write_hmlt", "adc2g_davidson_vectors",
     "adc2g_sil_autocorr_orders", "adc2g_create_cvs", "adc2g_mp2",
-    "adc2g_spectral_bounds",
+    "adc2g_spectral_bounds", "adc2g_power",
 ]
 
 _lib = None
@@ -145,7 +145,8 @@ def load(path: str = LIBPATH):
     lib.adc2g_davidson_vectors.argtypes = [vp, C.POINTER(DavParams), vp, vp, vp, C.POINTER(DavStats)]
     lib.adc2g_create_cvs.argtypes = [vp, i32, vp, C.POINTER(vp)]
     lib.adc2g_mp2.argtypes = [vp, vp, vp, vp]
-    lib.adc2g_spectral_bounds.argtypes = [vp, i32, vp]
+    lib.adc2g_spectral_bounds.argtypes = [vp, i32, i32, vp]
+    lib.adc2g_power.argtypes = [vp, i32, i32, C.c_double, vp, vp, C.POINTER(i32)]
     lib.adc2g_sil_autocorr_orders.argtypes = [vp, C.POINTER(SilParams), vp, i32, vp, vp, vp, i64,
                                               C.POINTER(i64), C.POINTER(SilStats)]
     lib.adc2g_relax.argtypes = [vp, C.POINTER(RelaxParams), vp, vp, vp, vp, vp, C.POINTER(RelaxStats)]
@@ -371,11 +372,20 @@ class Operator:
         check(_lib.adc2g_mp2(self._h, C.byref(e), C.byref(d), _ptr(rho) if density else None))
         return e.value, d.value, rho
 
-    def spectral_bounds(self, ncycles=0):
-        """(Emin, Emax) as spectral_bounds 'davlanc' estimates them."""
+    def spectral_bounds(self, method="davlanc", ncycles=0):
+        """(Emin, Emax) as spectral_bounds estimates them ('davlanc' or 'davpower')."""
         b = np.empty(2)
-        check(_lib.adc2g_spectral_bounds(self._h, int(ncycles), _ptr(b)))
+        check(_lib.adc2g_spectral_bounds(self._h, {"davlanc": 1, "davpower": 2}[method], int(ncycles), _ptr(b)))
         return float(b[0]), float(b[1])
+
+    def power(self, subdim=0, maxit=0, thrsh=0.0, vector=False):
+        """Highest eigenpair by the power method: (E, v or None, iterations)."""
+        e = C.c_double()
+        it = C.c_int32()
+        vec = np.empty(self.ndim) if vector else None
+        check(_lib.adc2g_power(self._h, int(subdim), int(maxit), float(thrsh), C.byref(e),
+                               _ptr(vec) if vector else None, C.byref(it)))
+        return e.value, vec, it.value
 
     def cvs(self, core):
         """CVS view on the core orbitals ``core`` (1-based occupied indices);

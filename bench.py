@@ -287,7 +287,7 @@ def cheby_measure(op, sz, order, seed=0):
     bounds = (lo - pad, hi + pad)
     t0 = time.perf_counter()
     try:
-        dl = op.spectral_bounds()
+        dl = op.spectral_bounds("davlanc")
         log(f"spectral bounds (davlanc): {dl[0]:.6f} .. {dl[1]:.3f} Eh in {time.perf_counter() - t0:.2f}s")
     except lib.Adc2gError as e:   # reported; the padded Lanczos bounds are used either way
         dl = str(e)

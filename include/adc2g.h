@@ -282,19 +282,34 @@ int32_t adc2g_sil_autocorr_orders(adc2g_handle* h, const adc2g_sil_params* p, co
 int32_t adc2g_cheby_auto(adc2g_handle* h, const double* bounds, const double* q0, int32_t order,
                          int32_t precision, double* auto_out);
 
-/* Spectral bounds of the matrix as spectral_bounds(..., 'davlanc')
- * (source/propagation/specbounds.f90:85-160) estimates them:
- *  - bounds[0] is the lowest root of block Davidson (1 root, block 5,
- *    subspace 20, tol 1e-8, Olsen, deflation, sub-diagonalisation guess on 800
- *    ISs);
- *  - bounds[1] is the highest eigenvalue of block Lanczos (block 2, ncycles
- *    cycles, 0 meaning the reference's 100, partial re-orthogonalisation)
- *    started from the lowest eigenvectors of the 800 ISs of highest H_jj
- *    (init_vec_singlesubspace, source/eigen/block_lanczos.f90:748-923).
+/* Spectral bounds of the matrix as spectral_bounds
+ * (source/propagation/specbounds.f90) estimates them.
+ *  - bounds[0] is the lowest root of block Davidson (1 root, tol 1e-8, Olsen,
+ *    deflation, sub-diagonalisation guess on 800 ISs; block/subspace 5/20 for
+ *    method 1, 2/10 for method 2).
+ *  - bounds[1], method 1 'davlanc' (:85-160): the highest eigenvalue of block
+ *    Lanczos (block 2, ncycles cycles, 0 meaning the reference's 100, partial
+ *    re-orthogonalisation) started from the lowest eigenvectors of the 800
+ *    ISs of highest H_jj (init_vec_singlesubspace,
+ *    source/eigen/block_lanczos.f90:748-923).
+ *  - bounds[1], method 2 'davpower' (:166-238): the power method
+ *    (adc2g_power with its defaults).
  * The reference's Chebyshev driver then multiplies them by 0.9 and 1.1
  * (chebyspec.f90:82-83), which contains the spectrum only for positive
- * bounds; here they are returned unwidened. */
-int32_t adc2g_spectral_bounds(adc2g_handle* h, int32_t ncycles, double* bounds);
+ * bounds; here they are returned unwidened.  ADC2G_ENOCONV when the
+ * Davidson or the power iterations do not converge (the reference stops). */
+int32_t adc2g_spectral_bounds(adc2g_handle* h, int32_t method, int32_t ncycles, double* bounds);
+
+/* Highest eigenpair by the power method (power_method,
+ * source/eigen/power.f90:22-100).  The guess is the highest eigenvector of the
+ * stored matrix on the subdim (<= 0: 800, the reference's fixed value) ISs of
+ * highest H_jj (power_guess, :307-463).  Up to maxit (<= 0: 350) iterations
+ * of v2 = H v1, E = v1.v2, r = |E v1 - v2|, v1 = v2 / |v2|, until
+ * r <= thrsh (<= 0: 1e-8) (power_iterations, :467-524).  eval and vec (host,
+ * ndim, may be NULL) receive (E, v1); niter (may be NULL) the iterations.
+ * ADC2G_ENOCONV if r stays above thrsh. */
+int32_t adc2g_power(adc2g_handle* h, int32_t subdim, int32_t maxit, double thrsh, double* eval,
+                    double* vec, int32_t* niter);
 
 /* Block Lanczos pseudospectrum: run_block_lanczos
  * (source/eigen/block_lanczos.f90:947-1256) with hxq_incore (:1325-1361)

@@ -110,3 +110,34 @@ def test_spectral_bounds_davlanc(form):
     scale = np.abs(lam).max()
     assert abs(lo - lam[0]) < 1e-9 * scale
     assert abs(hi - lam[-1]) < 1e-7 * scale
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_power_method_vs_oracle(form):
+    """power_method with a 20-IS guess (about 200 iterations): the oracle's
+    iteration count (within 2), the largest eigenvalue to 1e-12 relative and
+    the top eigenvector up to sign; a case that does not converge in 350
+    iterations returns ADC2G_ENOCONV; 'davpower' bounds equal the dense
+    extremes."""
+    from oracle import power
+    print(f"[case] power o=3 v=8 naux=10 coupling={form}", flush=True)
+    sys = make_system(3, 8, 10, seed=5)
+    Hm = elements.build_blocks(sys, apply_minc=True).dense()
+    H = elements.build_blocks(sys, apply_minc=False).dense()
+    lam, U = np.linalg.eigh(H)
+    g = power.guess(np.diag(H), lambda s: Hm[np.ix_(s, s)], subdim=20)
+    e_ref, _, it_ref, ok = power.power_method(lambda x: H @ x, g)
+    assert ok
+    with lib.Operator.from_system(sys, coupling=form) as op:
+        e, vec, it = op.power(subdim=20, vector=True)
+        lo, hi = op.spectral_bounds("davpower")
+    scale = np.abs(lam).max()
+    assert abs(it - it_ref) <= 2
+    assert abs(e - lam[-1]) < 1e-12 * scale
+    assert min(np.abs(vec - U[:, -1]).max(), np.abs(vec + U[:, -1]).max()) < 1e-7
+    assert abs(lo - lam[0]) < 1e-9 * scale and abs(hi - lam[-1]) < 1e-9 * scale
+    sys2 = make_system(4, 6, 8, seed=3)
+    with lib.Operator.from_system(sys2) as op2:
+        with pytest.raises(lib.Adc2gError) as ex:
+            op2.power(subdim=20)
+        assert ex.value.code == 12

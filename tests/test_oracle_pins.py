@@ -24,7 +24,7 @@ import pytest
 import scipy.sparse.linalg as sla
 
 from adc_ottawa_amd.synth import make_system
-from oracle import cheby, csf, davidson, dfsigma, elements, lanczos, mlp, mp2, relax, schur, sil
+from oracle import cheby, csf, davidson, dfsigma, elements, lanczos, mlp, mp2, power, relax, schur, sil
 
 
 def test_csf_counts_and_bijectivity():
@@ -312,6 +312,22 @@ def test_oracle_mp2():
     off = ~np.eye(o, dtype=bool)
     assert np.abs(rho[:o, :o][off] - 2.0 * blk["oo"][off]).max() < 1e-15
     assert np.abs(np.diag(rho)[:o] - 2.0 - np.diag(blk["oo"])).max() < 1e-15
+
+
+def test_oracle_power_method():
+    """The power-method restatement converges to the largest eigenvalue of the
+    dense matrix (o=3, v=8 with a 20-IS guess: about 200 iterations); with
+    the whole space as the guess sub-space it starts on the exact top
+    eigenvector and stops at the first iteration."""
+    sys = make_system(3, 8, 10, seed=5)
+    Hm = elements.build_blocks(sys, apply_minc=True).dense()
+    H = elements.build_blocks(sys, apply_minc=False).dense()
+    lam = np.linalg.eigvalsh(H)
+    hs = lambda s: Hm[np.ix_(s, s)]
+    e, v, it, ok = power.power_method(lambda x: H @ x, power.guess(np.diag(H), hs, subdim=20))
+    assert ok and 100 < it < 350 and abs(e - lam[-1]) < 1e-12 * abs(lam[-1])
+    e, v, it, ok = power.power_method(lambda x: H @ x, power.guess(np.diag(H), hs, subdim=H.shape[0]))
+    assert ok and it == 1
 
 
 def test_oracle_subdiag_guess():
