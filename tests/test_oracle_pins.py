@@ -24,7 +24,7 @@ import pytest
 import scipy.sparse.linalg as sla
 
 from adc_ottawa_amd.synth import make_system
-from oracle import cheby, csf, davidson, dfsigma, elements, lanczos, mlp, mp2, power, relax, schur, sil
+from oracle import cheby, csf, davidson, dfsigma, elements, fvec, lanczos, mlp, mp2, power, relax, schur, sil
 
 
 def test_csf_counts_and_bijectivity():
@@ -427,3 +427,37 @@ def test_bf16_rounding():
     # ties go to the even significand; bf16 keeps 8 significant bits
     assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.0 + 2 ** -6 and r[3] == 1.0 + 2 ** -7
     assert np.all(np.abs(r - x) <= np.abs(x) * 2 ** -8)
+
+
+@pytest.mark.parametrize("o,v,naux,seed", [(3, 8, 10, 5), (4, 6, 12, 2)])
+def test_oracle_fvector(o, v, naux, seed):
+    """F-vector: the literal restatement of every term function equals the
+    compact amplitude form (symmetric and non-symmetric dpl); dpl = identity
+    (the number operator) gives f = 0; the density-like intermediates equal
+    the MP2 density blocks (tauA = -uu^T/4, tauB = oo^T/4, tau21 + tau22 =
+    -ou^T/2)."""
+    sys = make_system(o, v, naux, seed=seed)
+    nb = o + v
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(nb, nb))
+    for dpl in (d + d.T, d):
+        L = fvec.literal_terms(sys, dpl)
+        C = fvec.compact_terms(sys, dpl)
+        for k in ("F0", "tauA", "FA", "tauB", "FB", "tau21", "F21", "tau22", "F22", "tau23", "F23", "tau24",
+                  "F24", "tauC", "FC", "F27", "F28", "F29", "F210", "FI", "FII", "FIII", "FIV1", "FIV2"):
+            assert np.abs(L[k] - C[k]).max() <= 1e-13 * max(np.abs(L[k]).max(), 1e-300), k
+        assert np.abs(L["F25"] + L["F26"] - C["F25+F26"]).max() < 1e-13 * np.abs(L["F25"]).max()
+        f = fvec.assemble(sys, dpl, L)
+        assert np.abs(f - fvec.fvector_compact(sys, dpl)).max() < 1e-13 * np.abs(f).max()
+        # the zeroth-order 1h1p term dominates and the 2h2p block is first order
+        n1 = o * v
+        assert np.abs(f[:n1] + np.sqrt(2.0) * dpl[o:, :o].reshape(-1)).max() < 0.2 * np.abs(f[:n1]).max()
+    eye = np.eye(nb)
+    assert np.abs(fvec.fvector(sys, eye)).max() < 1e-15
+    assert np.abs(fvec.fvector_compact(sys, eye)).max() < 1e-15
+    _, parts = mp2.density(sys)
+    L = fvec.literal_terms(sys, d)
+    sc = np.abs(parts["uu"]).max()
+    assert np.abs(L["tauA"] + parts["uu"].T / 4.0).max() < 1e-13 * sc
+    assert np.abs(L["tauB"] - parts["oo"].T / 4.0).max() < 1e-13 * np.abs(parts["oo"]).max()
+    assert np.abs(L["tau21"] + L["tau22"] + parts["ou"].T / 2.0).max() < 1e-13 * np.abs(parts["ou"]).max()
