@@ -302,6 +302,14 @@ int32_t adc2g_mp2(adc2g_handle* h, double* e_mp2, double* d2, double* rho) {
   });
 }
 
+int32_t adc2g_fvector(adc2g_handle* h, const double* dpl, double* f) {
+  return guarded([&] {
+    ARG_CHECK(h && dpl && f, "NULL argument");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    fvector_run(h->h, dpl, f);
+  });
+}
+
 int32_t adc2g_spectral_bounds(adc2g_handle* h, int32_t method, int32_t ncycles, double* bounds) {
   return guarded([&] {
     ARG_CHECK(h && bounds, "NULL argument");

@@ -50,6 +50,8 @@ void hmlt_write(Handle& H, const char* dia_path, const char* off_path, double mi
 // MP2 energy, D2 diagnostic and ground-state density (rho: nbas x nbas
 // column-major); any output may be NULL.
 void mp2_run(Handle& H, double* e_mp2, double* d2, double* rho);
+// ADC(2) F-vector of the nbas x nbas column-major operator matrix dpl (host) into f (host, ndim)
+void fvector_run(Handle& H, const double* dpl, double* f);
 
 // Sub-space diagonalisation guess: the lowest bs eigenvectors of the stored
 // matrix on the subdim ISs of lowest (descending: highest) H_jj, into V

@@ -264,6 +264,14 @@ module adc2g_mod
        type(c_ptr), value :: rho
      end function adc2g_mp2
 
+     ! F-vector of get_modifiedtm_adc2 for the nbas x nbas operator matrix dpl
+     integer(c_int32_t) function adc2g_fvector(h, dpl, f) bind(c, name='adc2g_fvector')
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: dpl(*)
+       real(c_double), intent(out) :: f(*)
+     end function adc2g_fvector
+
      ! spectral_bounds: method 1 'davlanc' (ncycles 0 = 100), 2 'davpower';
      ! widen by 0.9 / 1.1 as chebyspec.f90:82-83 when both are positive
      integer(c_int32_t) function adc2g_spectral_bounds(h, method, ncycles, bounds) &

@@ -99,7 +99,7 @@ SYMBOLS = [
     "adc2g_precon_get", "adc2g_precon_set", "adc2g_precon_eval", "adc2g_last_error",
     "adc2g_sigma_profile", "adc2g_precon_absorb", "adc2g_coupling_form", "adc2g_cheby_auto",
     "adc2g_block_lanczos", "adc2g_relax", "adc2g_write_hmlt", "adc2g_davidson_vectors",
-    "adc2g_sil_autocorr_orders", "adc2g_create_cvs", "adc2g_mp2",
+    "adc2g_sil_autocorr_orders", "adc2g_create_cvs", "adc2g_mp2", "adc2g_fvector",
     "adc2g_spectral_bounds", "adc2g_power",
 ]
 
@@ -145,6 +145,7 @@ def load(path: str = LIBPATH):
     lib.adc2g_davidson_vectors.argtypes = [vp, C.POINTER(DavParams), vp, vp, vp, C.POINTER(DavStats)]
     lib.adc2g_create_cvs.argtypes = [vp, i32, vp, C.POINTER(vp)]
     lib.adc2g_mp2.argtypes = [vp, vp, vp, vp]
+    lib.adc2g_fvector.argtypes = [vp, vp, vp]
     lib.adc2g_spectral_bounds.argtypes = [vp, i32, i32, vp]
     lib.adc2g_power.argtypes = [vp, i32, i32, C.c_double, vp, vp, C.POINTER(i32)]
     lib.adc2g_sil_autocorr_orders.argtypes = [vp, C.POINTER(SilParams), vp, i32, vp, vp, vp, i64,
@@ -371,6 +372,16 @@ class Operator:
         rho = np.empty((nb, nb), order="F") if density else None
         check(_lib.adc2g_mp2(self._h, C.byref(e), C.byref(d), _ptr(rho) if density else None))
         return e.value, d.value, rho
+
+    def fvector(self, dpl):
+        """f_J = <Psi_J|D|Psi_0> (ndim) of get_modifiedtm_adc2 for the nbas x nbas operator matrix dpl."""
+        nb = self.nocc + self.nvirt
+        d = np.asfortranarray(dpl, dtype=np.float64)
+        if d.shape != (nb, nb):
+            raise ValueError(f"dpl must be {nb} x {nb}")
+        f = np.empty(self.ndim)
+        check(_lib.adc2g_fvector(self._h, _ptr(d), _ptr(f)))
+        return f
 
     def spectral_bounds(self, method="davlanc", ncycles=0):
         """(Emin, Emax) as spectral_bounds estimates them ('davlanc' or 'davpower')."""

@@ -377,6 +377,20 @@ int32_t adc2g_relax(adc2g_handle* h, const adc2g_relax_params* p, const double* 
  * (8 v^2 o v bytes).  Not available on a CVS view. */
 int32_t adc2g_mp2(adc2g_handle* h, double* e_mp2, double* d2, double* rho);
 
+/* ---- F-vector ---------------------------------------------------------------- */
+
+/* The ADC(2) F-vector f_J = <Psi_J|D|Psi_0> of a one-electron operator with
+ * the nbas x nbas column-major matrix dpl (the reference's dpl(p,q) in the MO
+ * basis, used with the reference's index order), into f (ndim, the
+ * configuration order of adc2g_sigma).  Replaces get_modifiedtm_adc2
+ * (source/adclib/get_moment.f90:153-442) with the term functions of
+ * source/adclib/dipole_ph.f90: 1h1p f = -sqrt(2) (dpl(a,k) + F0 + F25..F210 +
+ * FA + FB + FC + F21..F24), 2h2p FI, FII, FIII, FIV1, FIV2 (closed shell,
+ * C1 symmetry; the CVS branches of :372-373 are not taken).  Needs the
+ * (vv|ov) block and eight n1 x n1 matrices in device memory.  Not available
+ * on a CVS view. */
+int32_t adc2g_fvector(adc2g_handle* h, const double* dpl, double* f);
+
 /* ---- on-disk matrix ------------------------------------------------------ */
 
 /* Write the matrix in the reference's on-disk format so that its readers
