@@ -23,7 +23,11 @@
 !            written to lancname, transition moments fvec.r_n printed;
 !   chebyord > 0: Chebyshev autocorrelation of fvec with the bounds
 !            cheby_emin / cheby_emax, file 'chebyauto';
-!   hmlt = .true.: the matrix files 'hmlt.diai' / 'hmlt.offi'.
+!   hmlt = .true.: the matrix files 'hmlt.diai' / 'hmlt.offi';
+!   dpl_file = 'd.bin': the initial state is the F-vector of this nbas x nbas
+!            column-major operator matrix (contract_dipole_initial_state with
+!            statenumber 0, adc2_autospec.f90:131-199 -> get_modifiedtm_adc2),
+!            computed by the library, instead of the vector in fvec_file.
 !=======================================================================
 program adc2g_driver
 
@@ -40,10 +44,11 @@ program adc2g_driver
   real(c_double) :: stepsize = 10.0d0, cheby_emin = 0.0d0, cheby_emax = 1.0d0
   character(len=256) :: eps_file = 'eps.bin', b_file = 'B.bin', davname = 'davstates'
   character(len=256) :: fvec_file = 'fvec.bin', auto_file = 'auto', lancname = 'lancstates'
+  character(len=256) :: dpl_file = ''
   namelist /adc2g/ nocc, nvirt, naux, eps_file, b_file, nstates, blocksize, maxvec, maxiter, tol, &
        precon, precision, deflate, davname, sil, fvec_file, auto_file, tout, tfinal, krydim, autotol, &
        solver, guess, subdim, stepsize, rlx_krydim, autoord, lanczos, lmain, ncycles, lancname, &
-       chebyord, cheby_emin, cheby_emax, hmlt
+       chebyord, cheby_emin, cheby_emax, hmlt, dpl_file
 
   type(c_ptr) :: h
   type(adc2g_opts) :: opts
@@ -55,7 +60,7 @@ program adc2g_driver
   type(adc2g_relax_stats) :: rs
   type(adc2g_lanczos_params) :: lp
   type(adc2g_lanczos_stats) :: ls
-  real(c_double), allocatable :: eps(:), B(:), eigval(:), fvec(:), q0(:,:), cauto(:)
+  real(c_double), allocatable :: eps(:), B(:), eigval(:), fvec(:), q0(:,:), cauto(:), dpl(:,:)
   real(c_double), allocatable, target :: eigvec(:,:), lener(:), ritz(:,:), tmom(:), fvec_t(:)
   complex(c_double_complex), allocatable :: auto0(:)
   complex(c_double_complex), allocatable, target :: auto1(:), auto2(:)
@@ -125,9 +130,18 @@ program adc2g_driver
 
   if (sil .or. lanczos .or. chebyord > 0) then
      allocate(fvec(ndim))
-     open(newunit=u, file=trim(fvec_file), access='stream', form='unformatted', status='old')
-     read(u) fvec
-     close(u)
+     if (len_trim(dpl_file) > 0) then
+        allocate(dpl(nbas, nbas))
+        open(newunit=u, file=trim(dpl_file), access='stream', form='unformatted', status='old')
+        read(u) dpl
+        close(u)
+        call check(adc2g_fvector(h, dpl, fvec))
+        write(*, '(a,es22.14)') ' F-vector: |f| =', sqrt(sum(fvec**2))
+     else
+        open(newunit=u, file=trim(fvec_file), access='stream', form='unformatted', status='old')
+        read(u) fvec
+        close(u)
+     end if
   end if
 
   if (lanczos) then

@@ -371,6 +371,23 @@ def mp2_measure(op):
             "e_mp2_Eh": float(e), "d2": float(d2), "density_trace": float(np.trace(rho))}
 
 
+def fvector_measure(op):
+    """ADC(2) F-vector (adc2g_fvector) of a seeded symmetric one-electron
+    operator matrix on the workload's system, FP64; the number operator
+    (identity) must give f = 0."""
+    nb = op.nocc + op.nvirt
+    d = np.random.default_rng(7).normal(0.0, 0.3, (nb, nb))
+    d = d + d.T
+    op.fvector(np.eye(nb))  # first call: kernel loads
+    t0 = time.perf_counter()
+    f = op.fvector(d)
+    t1 = time.perf_counter()
+    f0 = op.fvector(np.eye(nb))
+    log(f"fvector: {t1 - t0:.3f}s, |f| {np.linalg.norm(f):.4e}, |f(I)|max {np.abs(f0).max():.1e}")
+    return {"seconds": round(t1 - t0, 4), "norm": float(np.linalg.norm(f)),
+            "number_operator_max": float(np.abs(f0).max())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -393,6 +410,7 @@ def main():
     ap.add_argument("--relax", type=float, default=1e-3,
                     help="relax the lowest state in imaginary time to this residual (0 skips)")
     ap.add_argument("--skip-mp2", action="store_true", help="skip the MP2 energy/density timing")
+    ap.add_argument("--skip-fvector", action="store_true", help="skip the F-vector timing")
     ap.add_argument("--dav-roots", type=int, default=20, help="Davidson comparison: roots (block >= roots)")
     ap.add_argument("--dav-block", type=int, default=20, help="Davidson comparison: block size")
     ap.add_argument("--dav-maxvec", type=int, default=200, help="Davidson comparison: maximum subspace")
@@ -474,6 +492,7 @@ def main():
     lanc = guarded("lanczos", lambda: lanczos_measure(op, sz, args.lanczos)) if args.lanczos else None
     rlx = guarded("relax", lambda: relax_measure(op, args.relax)) if args.relax > 0 else None
     mp2r = guarded("mp2", lambda: mp2_measure(op)) if not args.skip_mp2 else None
+    fvr = guarded("fvector", lambda: fvector_measure(op)) if not args.skip_fvector else None
 
     n1 = op.n1
     bytes_per_sigma = 8.0 * (2 * op.ndim) + 8.0 * 2 * n1 * n1  # read x, write sigma, pair tensors
@@ -503,6 +522,7 @@ def main():
         "block_lanczos": lanc,
         "relaxation": rlx,
         "mp2": mp2r,
+        "fvector": fvr,
         "operator_build_seconds": round(t_build, 3),
     }
     op.close()
