@@ -24,10 +24,12 @@
 !   chebyord > 0: Chebyshev autocorrelation of fvec with the bounds
 !            cheby_emin / cheby_emax, file 'chebyauto';
 !   hmlt = .true.: the matrix files 'hmlt.diai' / 'hmlt.offi';
-!   dpl_file = 'd.bin': the initial state is the F-vector of this nbas x nbas
-!            column-major operator matrix (contract_dipole_initial_state with
-!            statenumber 0, adc2_autospec.f90:131-199 -> get_modifiedtm_adc2),
-!            computed by the library, instead of the vector in fvec_file.
+!   dpl_file = 'd.bin': the F-vector of this nbas x nbas column-major operator
+!            matrix (get_modifiedtm_adc2), computed by the library, gives the
+!            transition moments and oscillator strengths of the states
+!            (adc2_ener.f90:138-146) and is the initial state instead of the
+!            vector in fvec_file (contract_dipole_initial_state with
+!            statenumber 0, adc2_autospec.f90:131-199).
 !=======================================================================
 program adc2g_driver
 
@@ -116,9 +118,26 @@ program adc2g_driver
      write(*, '(a,i6,a,i8,a,f10.3,a)') ' Davidson: iterations', ds%niter, ', sigma-vectors', ds%nsigma, &
           ', time', ds%time_total, ' s'
   end if
-  do i = 1, nstates
-     write(*, '(i4,2x,f18.12,2x,f14.7)') i, eigval(i), eigval(i) * adc2g_eh2ev
-  end do
+  if (len_trim(dpl_file) > 0) then
+     ! F-vector of the operator matrix (get_modifiedtm_adc2); with it the
+     ! transition moments tm = x_n . f and oscillator strengths 2/3 E tm^2
+     ! of adc2_ener (adc2_ener.f90:138-146, tm of get_moment.f90:16-28)
+     allocate(fvec(ndim), dpl(nbas, nbas))
+     open(newunit=u, file=trim(dpl_file), access='stream', form='unformatted', status='old')
+     read(u) dpl
+     close(u)
+     call check(adc2g_fvector(h, dpl, fvec))
+     write(*, '(a,es22.14)') ' F-vector: |f| =', sqrt(sum(fvec**2))
+     write(*, '(a)') ' state        E [Eh]              E [eV]        transition moment    osc. strength'
+     do i = 1, nstates
+        write(*, '(i4,2x,f18.12,2x,f14.7,2x,es20.12,2x,es20.12)') i, eigval(i), eigval(i) * adc2g_eh2ev, &
+             dot_product(eigvec(:, i), fvec), 2.0d0 / 3.0d0 * eigval(i) * dot_product(eigvec(:, i), fvec)**2
+     end do
+  else
+     do i = 1, nstates
+        write(*, '(i4,2x,f18.12,2x,f14.7)') i, eigval(i), eigval(i) * adc2g_eh2ev
+     end do
+  end if
   call adc2g_write_davstates(trim(davname), ndim, nstates, eigval, eigvec)
 
   if (hmlt) then
@@ -128,20 +147,11 @@ program adc2g_driver
      write(*, '(a,i12,a,i6)') ' hmlt: off-diagonal elements', noffd, ', records', nbuf
   end if
 
-  if (sil .or. lanczos .or. chebyord > 0) then
+  if ((sil .or. lanczos .or. chebyord > 0) .and. len_trim(dpl_file) == 0) then
      allocate(fvec(ndim))
-     if (len_trim(dpl_file) > 0) then
-        allocate(dpl(nbas, nbas))
-        open(newunit=u, file=trim(dpl_file), access='stream', form='unformatted', status='old')
-        read(u) dpl
-        close(u)
-        call check(adc2g_fvector(h, dpl, fvec))
-        write(*, '(a,es22.14)') ' F-vector: |f| =', sqrt(sum(fvec**2))
-     else
-        open(newunit=u, file=trim(fvec_file), access='stream', form='unformatted', status='old')
-        read(u) fvec
-        close(u)
-     end if
+     open(newunit=u, file=trim(fvec_file), access='stream', form='unformatted', status='old')
+     read(u) fvec
+     close(u)
   end if
 
   if (lanczos) then

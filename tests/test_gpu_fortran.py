@@ -147,9 +147,10 @@ def test_fortran_driver_consumers(tmp_path):
 
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built")
 def test_fortran_driver_fvector_initial_state(tmp_path):
-    """dpl_file: the SIL initial state is the F-vector of the operator matrix
-    (contract_dipole_initial_state, statenumber 0); the 'auto' file equals the
-    oracle's propagation of the oracle's F-vector."""
+    """dpl_file: the F-vector of the operator matrix gives the states'
+    transition moments and oscillator strengths (adc2_ener) and is the SIL
+    initial state (contract_dipole_initial_state, statenumber 0); both and the
+    'auto' file equal the oracle's values for the oracle's F-vector."""
     print("[case] fortran driver F-vector initial state o=3 v=7 naux=9", flush=True)
     o, v, naux = 3, 7, 9
     sys_ = make_system(o, v, naux, seed=11)
@@ -159,7 +160,7 @@ def test_fortran_driver_fvector_initial_state(tmp_path):
     d = d + d.T
     np.asfortranarray(d).T.astype("<f8").tofile(tmp_path / "d.bin")  # column-major bytes
     nml = (f"&adc2g nocc={o} nvirt={v} naux={naux} eps_file='eps.bin' b_file='B.bin' nstates=2 "
-           f"blocksize=4 maxvec=20 tol=1d-8 precon=1 davname='davstates' sil=.true. dpl_file='d.bin' "
+           f"blocksize=4 maxvec=20 tol=1d-10 precon=1 davname='davstates' sil=.true. dpl_file='d.bin' "
            f"auto_file='auto' tout=0.1d0 tfinal=1.0d0 krydim=12 autotol=1d-10 /\n")
     r = subprocess.run([DRIVER], input=nml, capture_output=True, text=True, cwd=tmp_path, timeout=120)
     print(r.stdout)
@@ -168,6 +169,16 @@ def test_fortran_driver_fvector_initial_state(tmp_path):
     assert "F-vector" in r.stdout
     df = dfsigma.DF(sys_)
     H = df.sigma(np.eye(f.size))
+    # transition moments and oscillator strengths of adc2_ener (sign of tm
+    # follows the eigenvector's sign, so compare |tm|)
+    w, U = np.linalg.eigh(H)
+    rows = [ln.split() for ln in r.stdout.splitlines() if len(ln.split()) == 5 and ln.split()[0] in ("1", "2")]
+    assert len(rows) == 2
+    for i, row in enumerate(rows):
+        tm = U[:, i] @ f
+        assert abs(float(row[1]) - w[i]) < 1e-9
+        assert abs(abs(float(row[3])) - abs(tm)) < 1e-7 * np.linalg.norm(f)
+        assert abs(float(row[4]) - 2.0 / 3.0 * w[i] * tm ** 2) < 1e-7 * np.dot(f, f) * max(abs(w[i]), 1.0)
     lines = open(tmp_path / "auto").read().splitlines()
     data = np.array([[float(t) for t in ln.split()] for ln in lines[1:]])
     _, ref = sil.propagate(lambda x: -1j * (H @ x), f, tout=0.1, tfinal=1.0, kdim=12, autotol=1e-10)
