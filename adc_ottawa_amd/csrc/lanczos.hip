@@ -1,7 +1,8 @@
 // Block Lanczos pseudospectrum on the GPU sigma-vector: run_block_lanczos
 // (source/eigen/block_lanczos.f90:947-1256) without re-orthogonalisation
-// (orthotype 0, the default) or with partial re-orthogonalisation
-// (orthotype 1: pro, update_omega, pro_mgs, localro, :1430-1706), with
+// (orthotype 0, the default), with partial re-orthogonalisation
+// (orthotype 1: pro, update_omega, pro_mgs, localro, :1430-1706) or with the
+// modified partial re-orthogonalisation (orthotype 2: mpro, mpro_mgs, :1710-1890), with
 // hxq_incore (:1325-1361) replaced by the sigma-vector of the block.  Per cycle j (B_0 = 0):
 //   U = H Q_j - Q_{j-1} B_{j-1}^T,  A_j = Q_j^T U,  R = U - Q_j A_j,
 //   R = Q_{j+1} B_j  (B_j upper triangular).
@@ -25,6 +26,7 @@
 // the device in column chunks, normalised, dotted with travec and copied out
 // on request (ritzvecs_incore, :2237-2300).
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cfloat>
 #include <chrono>
@@ -393,8 +395,8 @@ void lanczos_run(Handle& H, const adc2g_lanczos_params& p, const double* q0_h, d
   const int nvec = b * m;
   hipStream_t st = H.st;
 
-  ARG_CHECK(p.orthotype == 0 || p.orthotype == 1, "orthotype must be 0 or 1");
-  const bool pro = p.orthotype == 1;
+  ARG_CHECK(p.orthotype >= 0 && p.orthotype <= 2, "orthotype must be 0, 1 or 2");
+  const bool pro = p.orthotype >= 1;  // the omega recurrence (PRO and MPRO)
   DevBuf<double> L, U, X2, small, Pw, part, sums, scale;
   L.alloc((size_t)n * nvec);
   U.alloc((size_t)n * b);
@@ -471,6 +473,73 @@ void lanczos_run(Handle& H, const adc2g_lanczos_params& p, const double* q0_h, d
   };
   int npro = 0;
 
+  // modified partial re-orthogonalisation (orthotype 2; mpro, mpro_mgs,
+  // lowerbound, upperbound, :1710-1890): Q_{j+1} against the intervals of
+  // stored blocks around the estimates that reach sqrt(eps), bounded where
+  // the estimates fall below eps^(3/4); the intervals, widened by one block
+  // each side, are used once more in the next cycle.
+  const double eta = std::pow(epmach, 0.75);
+  bool l2nd = false;
+  std::vector<std::array<int, 2>> ivs;
+  double* Qn_cur = nullptr;  // Q_{j+1} of the current cycle
+  auto lowerbound = [&](int j, int k) -> int {
+    if (k == 1 || k == 2) return 1;
+    int lb = k - 1;
+    for (int i = k - 2; i <= 1; ++i) {  // do i=k-2,1: runs only for k = 3
+      if (std::fabs(OM(j + 1, i)) > eta) lb = i;
+      else break;
+    }
+    return lb;
+  };
+  auto upperbound = [&](int j, int k) -> int {
+    int ub = k + 1;
+    for (int i = k + 1; i <= j - 1; ++i) {
+      if (std::fabs(OM(j + 1, i)) >= eta) ub = i;
+      else break;
+    }
+    return ub;
+  };
+  auto mpro_mgs = [&]() {
+    for (const auto& iv : ivs)
+      for (int k = iv[0]; k <= iv[1]; ++k) project_out(L.get() + (size_t)n * b * (k - 1), Qn_cur);
+    normalise_cols(Qn_cur);
+  };
+  auto mpro = [&](int jj) -> bool {
+    if (l2nd) {
+      ++npro;
+      mpro_mgs();
+      for (const auto& iv : ivs)
+        for (int l = iv[0]; l <= iv[1]; ++l) OM(jj + 1, l) = OM(l, jj + 1) = om_eps;
+      ivs.clear();
+      l2nd = false;
+      return true;
+    }
+    ivs.clear();
+    bool lro = false;
+    int k = 1;
+    for (;;) {  // the goto-10 scan of mpro
+      if (OM(jj + 1, k) >= orthlim) {
+        lro = true;
+        const int lo = lowerbound(jj, k), hi = upperbound(jj, k);
+        ivs.push_back({lo, hi});
+        k += hi;
+      } else {
+        ++k;
+      }
+      if (!(k < jj - 1)) break;
+    }
+    if (!lro) return false;
+    ++npro;
+    mpro_mgs();
+    for (auto& iv : ivs) {
+      for (int l = iv[0]; l <= iv[1]; ++l) OM(jj + 1, l) = OM(l, jj + 1) = om_eps;
+      if (iv[0] > 1) --iv[0];
+      ++iv[1];
+    }
+    l2nd = true;
+    return true;
+  };
+
   Timer tm;
   double t_sigma = 0.0;
   const double u = 0.5 * DBL_EPSILON;
@@ -517,6 +586,7 @@ void lanczos_run(Handle& H, const adc2g_lanczos_params& p, const double* q0_h, d
     gemm_nn(H, n, b, b, U.get(), n, small.get(), b, X2.get(), n, 1.0, 0.0);
     // two Cholesky QR passes: U = X2 R2^-1, Q_{j+1} = U R3^-1
     double* Qn = L.get() + (size_t)n * b * (j + 1);
+    Qn_cur = Qn;
     for (int pass = 0; pass < 2; ++pass) {
       const double* src = pass == 0 ? X2.get() : U.get();
       double* dst = pass == 0 ? U.get() : Qn;
@@ -560,9 +630,13 @@ void lanczos_run(Handle& H, const adc2g_lanczos_params& p, const double* q0_h, d
           x += bnorm[jj] * OM(jj - 1, k) + (anorm[jj] + anorm[k]) * OM(jj, k);
           OM(jj + 1, k) = OM(k, jj + 1) = inv * x;
         }
-        for (int k = 1; k < jj; ++k)
-          if (OM(jj + 1, k) >= orthlim) lro = true;
-        if (lro) {
+        if (p.orthotype == 2) {
+          lro = mpro(jj);
+        } else {
+          for (int k = 1; k < jj; ++k)
+            if (OM(jj + 1, k) >= orthlim) lro = true;
+        }
+        if (lro && p.orthotype == 1) {
           // pro_mgs (:1643-1706): Q_j and Q_{j+1} against the stored blocks 1..j-1,
           // Q_j normalised, then localro
           ++npro;
@@ -575,9 +649,11 @@ void lanczos_run(Handle& H, const adc2g_lanczos_params& p, const double* q0_h, d
           for (int k = 1; k <= jj; ++k) OM(jj + 1, k) = OM(jj, k) = om_eps;
         }
       }
-      // localro (:1430-1470): Q_{j+1} against Q_j, normalised
-      project_out(Pw.get(), Qn);
-      normalise_cols(Qn);
+      if (p.orthotype == 1) {
+        // localro (:1430-1470): Q_{j+1} against Q_j, normalised
+        project_out(Pw.get(), Qn);
+        normalise_cols(Qn);
+      }
     }
     for (int c = 0; c < b; ++c)
       for (int r = 0; r < b; ++r) {

@@ -78,7 +78,7 @@ module adc2g_mod
      integer(c_int32_t) :: ncycles = 0
      integer(c_int32_t) :: precision = 0
      integer(c_int32_t) :: check_orth = 0
-     integer(c_int32_t) :: orthotype = 0      ! 0 none, 1 partial re-orthogonalisation
+     integer(c_int32_t) :: orthotype = 0      ! 0 none, 1 partial (pro), 2 modified partial (mpro)
      integer(c_int32_t) :: reserved = 0
   end type adc2g_lanczos_params
 

@@ -131,14 +131,15 @@ typedef struct {
   int32_t ncycles;       /* iter: blocks generated; nvec = blocksize * ncycles */
   int32_t precision;     /* ADC2G_PREC_* of the sigma-vectors */
   int32_t check_orth;    /* 1: stats->orth_loss = max |L^T L - I| over the Lanczos vectors */
-  int32_t orthotype;     /* 0: none (default); 1: partial re-orthogonalisation (pro) */
+  int32_t orthotype;     /* 0: none (default); 1: partial re-orthogonalisation (pro);
+                            2: modified partial re-orthogonalisation (mpro) */
   int32_t reserved;
 } adc2g_lanczos_params;
 
 typedef struct {
   int32_t nvec;          /* Lanczos states = blocksize * ncycles */
   int32_t nsigma;        /* sigma-vectors applied */
-  int32_t npro;          /* partial re-orthogonalisations (orthotype 1) */
+  int32_t npro;          /* re-orthogonalisations (orthotype 1 and 2) */
   int32_t reserved;
   double time_sigma;     /* s in the sigma-vectors (device events) */
   double time_tdiag;     /* s in the eigen-decomposition of T (host) */
@@ -317,8 +318,11 @@ int32_t adc2g_power(adc2g_handle* h, int32_t subdim, int32_t maxit, double thrsh
  * re-orthogonalisation.  orthotype 1 adds partial re-orthogonalisation: pro
  * with the omega recurrence (:1474-1639), pro_mgs against the stored blocks
  * (:1643-1706) and localro (:1430-1470), with block Gram-Schmidt steps in
- * place of the vector-by-vector ones.  orthotype 2 (mpro, :1710-1845) is not
- * provided.
+ * place of the vector-by-vector ones.  orthotype 2 is the modified partial
+ * re-orthogonalisation (mpro, mpro_mgs, lowerbound, upperbound, :1710-1890):
+ * the same omega recurrence, Q_{j+1} alone against the intervals of stored
+ * blocks around the estimates that reach sqrt(eps), the widened intervals once
+ * more in the next cycle, no localro.
  *   U = H Q_j - Q_{j-1} B_{j-1}^T,  A_j = Q_j^T U,  R = U - Q_j A_j = Q_{j+1} B_j
  * with B_j upper triangular.  The reference factors R with dgeqrf/dorgqr; here a
  * shifted Cholesky QR with two refinement passes gives the factor with a

@@ -42,6 +42,76 @@ def _localro(q1, q2):
     return q2 / np.sqrt((q2 * q2).sum(axis=0))
 
 
+def _lowerbound(om, j, k, eta):
+    """lowerbound (:1848-1868).  Its loop ``do i=k-2,1`` runs only when
+    k - 2 <= 1, so for k >= 4 the bound is k - 1."""
+    if k in (1, 2):
+        return 1
+    lb = k - 1
+    for i in range(k - 2, 2):              # do i=k-2,1 (step +1)
+        if abs(om[j + 1, i]) > eta:
+            lb = i
+        else:
+            break
+    return lb
+
+
+def _upperbound(om, j, k, eta):
+    """upperbound (:1872-1890)."""
+    ub = k + 1
+    for i in range(k + 1, j):              # do i=k+1,j-1
+        if abs(om[j + 1, i]) >= eta:
+            ub = i
+        else:
+            break
+    return ub
+
+
+def _mpro(om, jj, b, L, q, eps, orthlim, l2nd, intervals):
+    """mpro (:1710-1781) and mpro_mgs (:1785-1845) at cycle jj (1-based):
+    Q_{j+1} (``q``, modified in place) orthogonalised vector by vector against
+    the stored blocks of the intervals, its columns normalised; the
+    intervals found are widened by one block each side and used again in the
+    next cycle.  Returns (lro, l2nd, intervals, number of re-orthogonalisations)."""
+    eta = EPMACH ** 0.75
+
+    def mgs(ivs):
+        for lo, hi in ivs:
+            for k in range(lo, hi + 1):
+                for m in range(b):
+                    l = L[:, (k - 1) * b + m]
+                    q[:] -= np.outer(l, q.T @ l)
+        q[:] /= np.sqrt((q * q).sum(axis=0))
+
+    if l2nd:
+        mgs(intervals)
+        for lo, hi in intervals:
+            for l in range(lo, hi + 1):
+                om[jj + 1, l] = om[l, jj + 1] = eps
+        return True, False, [], 1
+    ivs, k, lro = [], 1, False
+    while True:                            # the goto-10 loop
+        if om[jj + 1, k] >= orthlim:
+            lro = True
+            lo, hi = _lowerbound(om, jj, k, eta), _upperbound(om, jj, k, eta)
+            ivs.append([lo, hi])
+            k = k + hi
+        else:
+            k += 1
+        if not k < jj - 1:
+            break
+    if not lro:
+        return False, False, [], 0
+    mgs(ivs)
+    for iv in ivs:
+        for l in range(iv[0], iv[1] + 1):
+            om[jj + 1, l] = om[l, jj + 1] = eps
+        if iv[0] > 1:
+            iv[0] -= 1
+        iv[1] += 1
+    return True, True, ivs, 1
+
+
 def block_lanczos(matvec, q0: np.ndarray, ncycles: int, orthotype: int = 0) -> dict:
     """``matvec(X)`` returns H X for an (n, b) block; q0 holds orthonormal columns.
 
@@ -54,6 +124,12 @@ def block_lanczos(matvec, q0: np.ndarray, ncycles: int, orthotype: int = 0) -> d
     Otherwise localro alone runs (:1192).  The stored Lanczos vectors (Ritz
     vectors, later re-orthogonalisations) are the blocks as they were at the
     start of their cycle; the recurrence uses the modified working Q_j.
+
+    orthotype 2 is the modified partial re-orthogonalisation (mpro,
+    :1710-1781): the same omega recurrence; Q_{j+1} alone is orthogonalised
+    against the intervals of stored blocks around the estimates that reach
+    sqrt(eps) (bounds where the estimates fall below eps^(3/4)), and against
+    the widened intervals once more in the next cycle; no localro.
 
     Returns T, eigval (ascending), Y (eigenvectors of T), ritz (normalised
     L Y), L (the Lanczos vectors), Qnext / Bnext (the block and the triangle
@@ -74,6 +150,7 @@ def block_lanczos(matvec, q0: np.ndarray, ncycles: int, orthotype: int = 0) -> d
     bnorm = np.zeros(ncycles + 1)
     amats, bmats = [], []
     npro = 0
+    l2nd, intervals = False, []
     for j in range(ncycles):
         c0 = j * b
         L[:, c0:c0 + b] = q_cur
@@ -86,7 +163,7 @@ def block_lanczos(matvec, q0: np.ndarray, ncycles: int, orthotype: int = 0) -> d
         q_prev, q_cur = q_cur.copy(), q_next
         jj = j + 1                                   # the reference's 1-based cycle
         lro = False
-        if orthotype == 1 and 1 < jj < ncycles:
+        if orthotype in (1, 2) and 1 < jj < ncycles:
             sv = lambda m: np.linalg.svd(m, compute_uv=False)
             if jj == 2:
                 anorm[1] = sv(amats[0])[0]
@@ -105,7 +182,10 @@ def block_lanczos(matvec, q0: np.ndarray, ncycles: int, orthotype: int = 0) -> d
                 if k > 1:
                     x += bnorm[k] * om[jj, k - 1]
                 om[jj + 1, k] = om[k, jj + 1] = inv * x
-            if any(om[jj + 1, k] >= orthlim for k in range(1, jj)):
+            if orthotype == 2:
+                lro, l2nd, intervals, did = _mpro(om, jj, b, L, q_cur, eps, orthlim, l2nd, intervals)
+                npro += did
+            elif any(om[jj + 1, k] >= orthlim for k in range(1, jj)):
                 lro = True
                 npro += 1
                 for k in range(1, jj):

@@ -135,3 +135,26 @@ def test_lanczos_partial_reorthogonalisation_vs_oracle(form):
     # with re-orthogonalisation the top eigenvalue appears once (no ghost copy)
     assert (np.abs(ev - lam[-1]) < 1e-6).sum() == 1
     assert ev0.size == ev.size
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_lanczos_modified_partial_reorthogonalisation_vs_oracle(form):
+    """orthotype 2 (mpro) on the run that loses orthogonality without
+    re-orthogonalisation: the oracle's number of re-orthogonalisations,
+    orthogonal stored vectors, every eigenvalue of T to 1e-10 relative and
+    the top eigenvalue once."""
+    print(f"[case] lanczos mpro o=3 v=8 naux=10 b=3 m=40 coupling={form}", flush=True)
+    sys = make_system(3, 8, 10, seed=5)
+    H = elements.build_blocks(sys, apply_minc=False).dense()
+    n = H.shape[0]
+    q0 = np.zeros((n, 3))
+    q0[[0, 5, 11], [0, 1, 2]] = 1.0
+    ref = lanczos.block_lanczos(lambda X: H @ X, q0, 40, orthotype=2)
+    lam = np.linalg.eigvalsh(H)
+    with lib.Operator.from_system(sys, coupling=form) as op:
+        ev, _, st = op.block_lanczos(q0, 40, orthotype=2, check_orth=True)
+    scale = np.abs(lam).max()
+    assert st["npro"] == ref["npro"] > 0
+    assert st["orth_loss"] < 1e-10
+    assert np.abs(ev - ref["eigval"]).max() < 1e-10 * scale
+    assert (np.abs(ev - lam[-1]) < 1e-6).sum() == 1

@@ -241,6 +241,29 @@ def test_oracle_block_lanczos_partial_reorthogonalisation():
     ev = out1["eigval"]
     assert np.abs(ev[:3] - lam[:3]).max() < 1e-10 and np.abs(ev[-3:] - lam[-3:]).max() < 1e-10
     assert (np.abs(ev - lam[-1]) < 1e-6).sum() == 1
+    # orthotype 2 (mpro): re-orthogonalisations happen, the stored vectors stay
+    # orthogonal below sqrt(eps) and the extreme eigenvalues appear once
+    out2 = lanczos.block_lanczos(lambda X: H @ X, q0, 40, orthotype=2)
+    L2, ev2 = out2["L"], out2["eigval"]
+    assert out2["npro"] > 0
+    assert np.abs(L2.T @ L2 - np.eye(L2.shape[1])).max() < 1e-10
+    assert np.abs(ev2[:3] - lam[:3]).max() < 1e-10 and np.abs(ev2[-3:] - lam[-3:]).max() < 1e-10
+    assert (np.abs(ev2 - lam[-1]) < 1e-6).sum() == 1
+
+
+def test_oracle_mpro_bounds():
+    """lowerbound/upperbound of mpro as the reference's loops run: the lower
+    loop (do i=k-2,1) only runs for k = 3."""
+    om = np.zeros((12, 12))
+    j, eta = 9, lanczos.EPMACH ** 0.75
+    om[j + 1, 1:9] = [1e-20, 1, 1, 1, 1, 1, 1e-20, 1]
+    assert lanczos._lowerbound(om, j, 1, eta) == 1 and lanczos._lowerbound(om, j, 2, eta) == 1
+    assert lanczos._lowerbound(om, j, 3, eta) == 2          # om(j+1,1) below eta
+    om[j + 1, 1] = 1.0
+    assert lanczos._lowerbound(om, j, 3, eta) == 1
+    assert lanczos._lowerbound(om, j, 6, eta) == 5          # no loop for k >= 4
+    assert lanczos._upperbound(om, j, 2, eta) == 6          # 3..6 above eta, 7 below
+    assert lanczos._upperbound(om, j, 7, eta) == 8          # starts at k + 1 whatever om(j+1,k+1)
 
 
 def test_oracle_block_lanczos_full_krylov():
