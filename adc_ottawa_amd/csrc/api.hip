@@ -310,6 +310,18 @@ int32_t adc2g_fvector(adc2g_handle* h, const double* dpl, double* f) {
   });
 }
 
+int32_t adc2g_lanczos_guess(adc2g_handle* h, int32_t kind, int32_t subdim, int32_t blocksize, double* q0) {
+  return guarded([&] {
+    ARG_CHECK(h && q0, "NULL argument");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    const int64_t n = h->h.d.ndim;
+    adc2g::DevBuf<double> g;
+    g.alloc((size_t)n * (blocksize > 0 ? blocksize : 1));
+    lanczos_guess(h->h, kind, subdim > 0 ? subdim : 800, blocksize, g.get());
+    HIP_CHECK(hipMemcpy(q0, g.get(), sizeof(double) * n * blocksize, hipMemcpyDeviceToHost));
+  });
+}
+
 int32_t adc2g_spectral_bounds(adc2g_handle* h, int32_t method, int32_t ncycles, double* bounds) {
   return guarded([&] {
     ARG_CHECK(h && bounds, "NULL argument");

@@ -217,9 +217,48 @@ void power_run(Handle& H, int subdim, int maxit, double thrsh, double* eval, dou
   if (!(r <= thrsh)) throw Error(ADC2G_ENOCONV, "power method: convergence not reached...");
 }
 
+// init_vec_doublesubspace (block_lanczos.f90:443-744): the first bs/2
+// columns are the lowest eigenvectors of the subdim ISs of lowest H_jj, the
+// others eigenvectors bs/2+1 .. bs (ascending order) of the subdim ISs of
+// highest H_jj, as the reference's column loop takes them;
+// init_vec_singlesubspace (:748-923): the lowest bs eigenvectors of the
+// highest-H_jj block.
+void lanczos_guess(Handle& H, int kind, int subdim, int bs, double* V) {
+  ARG_CHECK(kind == 8 || kind == 9, "kind must be 8 (double subspace) or 9 (single subspace)");
+  ARG_CHECK(bs >= 1 && subdim >= 1, "blocksize and subdim must be positive");
+  if (kind == 9) {
+    subdiag_cols(H, subdim, true, 0, bs, V);
+    return;
+  }
+  ARG_CHECK(bs % 2 == 0, "the double-subspace guess needs an even blocksize");
+  subdiag_cols(H, subdim, false, 0, bs / 2, V);
+  subdiag_cols(H, subdim, true, bs / 2, bs / 2, V + (size_t)H.d.ndim * (bs / 2));
+}
+
 void spectral_bounds_run(Handle& H, int method, int ncycles, double bounds[2]) {
-  ARG_CHECK(method == 1 || method == 2, "method must be 1 (davlanc) or 2 (davpower)");
+  ARG_CHECK(method >= 1 && method <= 3, "method must be 1 (davlanc), 2 (davpower) or 3 (lanczos)");
   const int64_t n = H.d.ndim;
+  if (method == 3) {
+    // spectral_bounds_lanczos (specbounds.f90:52-105): block Lanczos with
+    // lancguess 8, lmain 2, ncycles 50 and the default orthotype 0; the
+    // lowest and highest Lanczos state energies
+    const int m = ncycles > 0 ? ncycles : 50;
+    DevBuf<double> g;
+    g.alloc((size_t)n * 2);
+    lanczos_guess(H, 8, 800, 2, g.get());
+    std::vector<double> q0((size_t)n * 2);
+    HIP_CHECK(hipMemcpyAsync(q0.data(), g.get(), sizeof(double) * q0.size(), hipMemcpyDeviceToHost, H.st));
+    HIP_CHECK(hipStreamSynchronize(H.st));
+    adc2g_lanczos_params lp{};
+    lp.blocksize = 2;
+    lp.ncycles = m;
+    lp.orthotype = 0;
+    std::vector<double> ev((size_t)2 * m);
+    lanczos_run(H, lp, q0.data(), ev.data(), nullptr, nullptr, nullptr, nullptr, nullptr);
+    bounds[0] = ev.front();
+    bounds[1] = ev.back();
+    return;
+  }
   adc2g_dav_params dp{};
   dp.nstates = 1;
   dp.blocksize = method == 1 ? 5 : 2;   // davlanc: dmain_f 5, maxsubdim_f 20; davpower: 2, 10
@@ -245,7 +284,7 @@ void spectral_bounds_run(Handle& H, int method, int ncycles, double bounds[2]) {
   const int m = ncycles > 0 ? ncycles : 100;
   DevBuf<double> g;
   g.alloc((size_t)n * 2);
-  subdiag_guess(H, 800, 2, g.get(), true, false);
+  lanczos_guess(H, 9, 800, 2, g.get());
   std::vector<double> q0((size_t)n * 2);
   HIP_CHECK(hipMemcpyAsync(q0.data(), g.get(), sizeof(double) * q0.size(), hipMemcpyDeviceToHost, H.st));
   HIP_CHECK(hipStreamSynchronize(H.st));

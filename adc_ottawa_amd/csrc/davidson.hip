@@ -99,6 +99,13 @@ __global__ void scatter_rows_kernel(int nr, int nc, const int64_t* __restrict__ 
 // also keeps the lowest eigenvectors of that block); with top, the highest bs
 // eigenvectors instead (power_guess, source/eigen/power.f90:307-463).
 void subdiag_guess(Handle& H, int subdim, int bs, double* V, bool descending, bool top) {
+  subdiag_cols(H, subdim, descending, top ? -1 : 0, bs, V);
+}
+
+// The same sub-diagonalisation with eigenvectors first .. first+count-1
+// (ascending eigenvalue order; first = -1: the highest count) scattered into
+// V (n x count, ld n).
+void subdiag_cols(Handle& H, int subdim, bool descending, int first, int bs, double* V) {
   const int64_t n = H.d.ndim;
   hipStream_t st = H.st;
   std::vector<double> hii(n);
@@ -113,7 +120,7 @@ void subdiag_guess(Handle& H, int subdim, int bs, double* V, bool descending, bo
   int64_t sd = std::min<int64_t>(subdim, n);
   while (sd < n && std::fabs(hii[order[sd - 1]] - hii[order[sd]]) < 1e-6) ++sd;
   ARG_CHECK(sd <= 4096, "the sub-diagonalisation guess is limited to 4096 ISs");
-  ARG_CHECK(bs <= sd, "blocksize exceeds the sub-diagonalisation dimension");
+  ARG_CHECK(bs <= sd && first + bs <= sd, "blocksize exceeds the sub-diagonalisation dimension");
   const int m = (int)sd;
   DevBuf<int64_t> idx;
   idx.alloc(m);
@@ -155,9 +162,9 @@ void subdiag_guess(Handle& H, int subdim, int bs, double* V, bool descending, bo
     throw Error(ADC2G_EEIG, "the diagonalisation of the subspace Hamiltonian failed");
   DevBuf<double> Ud;
   Ud.alloc((size_t)m * bs);
-  // the lowest bs eigenvectors, or with top the highest ones (power_guess)
-  const size_t first = top ? (size_t)(m - bs) : 0;
-  HIP_CHECK(hipMemcpyAsync(Ud.get(), U.data() + first * m, sizeof(double) * m * bs, hipMemcpyHostToDevice, st));
+  // eigenvectors first.. (the lowest bs for 0; the highest for -1, power_guess)
+  const size_t f0 = first < 0 ? (size_t)(m - bs) : (size_t)first;
+  HIP_CHECK(hipMemcpyAsync(Ud.get(), U.data() + f0 * m, sizeof(double) * m * bs, hipMemcpyHostToDevice, st));
   HIP_CHECK(hipMemsetAsync(V, 0, sizeof(double) * n * bs, st));
   const int64_t tot = (int64_t)m * bs;
   hipLaunchKernelGGL(scatter_rows_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, m, bs, idx.get(),

@@ -57,6 +57,11 @@ void fvector_run(Handle& H, const double* dpl, double* f);
 // matrix on the subdim ISs of lowest (descending: highest) H_jj, into V
 // (ndim x bs, device).
 void subdiag_guess(Handle& H, int subdim, int bs, double* V, bool descending, bool top);
+// eigenvectors first .. first+count-1 of that block (first = -1: the highest count)
+void subdiag_cols(Handle& H, int subdim, bool descending, int first, int count, double* V);
+// block Lanczos start block on the device (V: n x bs): kind 8
+// init_vec_doublesubspace, kind 9 init_vec_singlesubspace
+void lanczos_guess(Handle& H, int kind, int subdim, int bs, double* V);
 
 // Power method for the highest eigenpair (power_method, source/eigen/power.f90);
 // vec (host, ndim) and niter may be NULL.  Throws ADC2G_ENOCONV when the

@@ -18,9 +18,11 @@
 !            with stepsize, rlx_krydim and the same nstates / maxiter / tol;
 !   guess = 0 unit vectors | 1 sub-diagonalisation (subdim ISs) for Davidson;
 !   autoord = 0..2: SIL autocorrelation orders, files auto_file, 'auto1', 'auto2';
-!   lanczos = .true.: block Lanczos (lancdiag_block) with lmain unit vectors on
-!            the 1h1p entries of largest |fvec| and ncycles blocks, Lanczos states
-!            written to lancname, transition moments fvec.r_n printed;
+!   lanczos = .true.: block Lanczos (lancdiag_block) with lmain start vectors and
+!            ncycles blocks, Lanczos states written to lancname, transition
+!            moments fvec.r_n printed; lancguess 1 (default) unit vectors on
+!            the 1h1p entries of largest |fvec|, 8 the double-subspace and 9
+!            the single-subspace guess (800 ISs); orthotype 0, 1 (pro), 2 (mpro);
 !   chebyord > 0: Chebyshev autocorrelation of fvec with the bounds
 !            cheby_emin / cheby_emax, file 'chebyauto';
 !   hmlt = .true.: the matrix files 'hmlt.diai' / 'hmlt.offi';
@@ -40,7 +42,7 @@ program adc2g_driver
   integer :: nocc = 0, nvirt = 0, naux = 0, nstates = 5, blocksize = 0, maxvec = -1, maxiter = 300
   integer :: precon = 2, precision = 0, krydim = 10
   integer :: solver = 1, guess = 0, subdim = 800, rlx_krydim = 10, autoord = 0
-  integer :: lmain = 0, ncycles = 0, chebyord = 0
+  integer :: lmain = 0, ncycles = 0, chebyord = 0, lancguess = 1, orthotype = 0
   logical :: deflate = .true., sil = .false., lanczos = .false., hmlt = .false.
   real(c_double) :: tol = 1.0d-7, tout = 0.1d0, tfinal = 1.0d0, autotol = 1.0d-5
   real(c_double) :: stepsize = 10.0d0, cheby_emin = 0.0d0, cheby_emax = 1.0d0
@@ -50,7 +52,7 @@ program adc2g_driver
   namelist /adc2g/ nocc, nvirt, naux, eps_file, b_file, nstates, blocksize, maxvec, maxiter, tol, &
        precon, precision, deflate, davname, sil, fvec_file, auto_file, tout, tfinal, krydim, autotol, &
        solver, guess, subdim, stepsize, rlx_krydim, autoord, lanczos, lmain, ncycles, lancname, &
-       chebyord, cheby_emin, cheby_emax, hmlt, dpl_file
+       chebyord, cheby_emin, cheby_emax, hmlt, dpl_file, lancguess, orthotype
 
   type(c_ptr) :: h
   type(adc2g_opts) :: opts
@@ -158,17 +160,23 @@ program adc2g_driver
      ! init_vec_unit: unit vectors on the lmain 1h1p entries of largest |fvec|
      nvec = lmain * ncycles
      allocate(q0(ndim, lmain), lener(nvec), ritz(ndim, nvec), tmom(nvec), fvec_t(ndim), used(n1))
-     q0 = 0.0d0
-     used = .false.
-     do i = 1, lmain
-        k = maxloc(abs(fvec(1:n1)), dim=1, mask=.not. used)
-        used(k) = .true.
-        q0(k, i) = 1.0d0
-     end do
+     if (lancguess == 8 .or. lancguess == 9) then
+        ! init_vec_doublesubspace / init_vec_singlesubspace (block_lanczos.f90:443-923)
+        call check(adc2g_lanczos_guess(h, int(lancguess, c_int32_t), 800_c_int32_t, int(lmain, c_int32_t), q0))
+     else
+        q0 = 0.0d0
+        used = .false.
+        do i = 1, lmain
+           k = maxloc(abs(fvec(1:n1)), dim=1, mask=.not. used)
+           used(k) = .true.
+           q0(k, i) = 1.0d0
+        end do
+     end if
      fvec_t = fvec
      lp%blocksize = lmain
      lp%ncycles = ncycles
      lp%precision = precision
+     lp%orthotype = orthotype
      rc = adc2g_block_lanczos(h, lp, q0, lener, c_null_ptr, c_loc(fvec_t), c_loc(tmom), c_loc(ritz), ls)
      call check(rc)
      call adc2g_write_davstates(trim(lancname), ndim, nvec, lener, ritz)

@@ -272,7 +272,16 @@ module adc2g_mod
        real(c_double), intent(out) :: f(*)
      end function adc2g_fvector
 
-     ! spectral_bounds: method 1 'davlanc' (ncycles 0 = 100), 2 'davpower';
+     ! block Lanczos start block: kind 8 init_vec_doublesubspace, 9 init_vec_singlesubspace
+     integer(c_int32_t) function adc2g_lanczos_guess(h, kind, subdim, blocksize, q0) &
+          bind(c, name='adc2g_lanczos_guess')
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: kind, subdim, blocksize
+       real(c_double), intent(out) :: q0(*)
+     end function adc2g_lanczos_guess
+
+     ! spectral_bounds: method 1 'davlanc' (ncycles 0 = 100), 2 'davpower', 3 'lanczos' (0 = 50);
      ! widen by 0.9 / 1.1 as chebyspec.f90:82-83 when both are positive
      integer(c_int32_t) function adc2g_spectral_bounds(h, method, ncycles, bounds) &
           bind(c, name='adc2g_spectral_bounds')

@@ -99,7 +99,7 @@ SYMBOLS = [
     "adc2g_precon_get", "adc2g_precon_set", "adc2g_precon_eval", "adc2g_last_error",
     "adc2g_sigma_profile", "adc2g_precon_absorb", "adc2g_coupling_form", "adc2g_cheby_auto",
     "adc2g_block_lanczos", "adc2g_relax", "adc2g_write_hmlt", "adc2g_davidson_vectors",
-    "adc2g_sil_autocorr_orders", "adc2g_create_cvs", "adc2g_mp2", "adc2g_fvector",
+    "adc2g_sil_autocorr_orders", "adc2g_create_cvs", "adc2g_mp2", "adc2g_fvector", "adc2g_lanczos_guess",
     "adc2g_spectral_bounds", "adc2g_power",
 ]
 
@@ -146,6 +146,7 @@ def load(path: str = LIBPATH):
     lib.adc2g_create_cvs.argtypes = [vp, i32, vp, C.POINTER(vp)]
     lib.adc2g_mp2.argtypes = [vp, vp, vp, vp]
     lib.adc2g_fvector.argtypes = [vp, vp, vp]
+    lib.adc2g_lanczos_guess.argtypes = [vp, i32, i32, i32, vp]
     lib.adc2g_spectral_bounds.argtypes = [vp, i32, i32, vp]
     lib.adc2g_power.argtypes = [vp, i32, i32, C.c_double, vp, vp, C.POINTER(i32)]
     lib.adc2g_sil_autocorr_orders.argtypes = [vp, C.POINTER(SilParams), vp, i32, vp, vp, vp, i64,
@@ -384,10 +385,17 @@ class Operator:
         return f
 
     def spectral_bounds(self, method="davlanc", ncycles=0):
-        """(Emin, Emax) as spectral_bounds estimates them ('davlanc' or 'davpower')."""
+        """(Emin, Emax) as spectral_bounds estimates them ('davlanc', 'davpower' or 'lanczos')."""
         b = np.empty(2)
-        check(_lib.adc2g_spectral_bounds(self._h, {"davlanc": 1, "davpower": 2}[method], int(ncycles), _ptr(b)))
+        check(_lib.adc2g_spectral_bounds(self._h, {"davlanc": 1, "davpower": 2, "lanczos": 3}[method],
+                                         int(ncycles), _ptr(b)))
         return float(b[0]), float(b[1])
+
+    def lanczos_guess(self, kind, blocksize, subdim=0):
+        """Block Lanczos start block: kind 8 init_vec_doublesubspace, 9 init_vec_singlesubspace."""
+        q0 = np.empty((self.ndim, blocksize), order="F")
+        check(_lib.adc2g_lanczos_guess(self._h, int(kind), int(subdim), int(blocksize), _ptr(q0)))
+        return q0
 
     def power(self, subdim=0, maxit=0, thrsh=0.0, vector=False):
         """Highest eigenpair by the power method: (E, v or None, iterations)."""

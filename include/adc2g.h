@@ -285,7 +285,11 @@ int32_t adc2g_cheby_auto(adc2g_handle* h, const double* bounds, const double* q0
 
 /* Spectral bounds of the matrix as spectral_bounds
  * (source/propagation/specbounds.f90) estimates them.
- *  - bounds[0] is the lowest root of block Davidson (1 root, tol 1e-8, Olsen,
+ *  - method 3 'lanczos' (spectral_bounds_lanczos, :52-105): the lowest and
+ *    highest eigenvalues of block Lanczos (block 2, ncycles cycles, 0 meaning
+ *    the reference's 50, orthotype 0) started from the double-subspace guess
+ *    (adc2g_lanczos_guess kind 8, 800 ISs).
+ *  - methods 1 and 2: bounds[0] is the lowest root of block Davidson (1 root, tol 1e-8, Olsen,
  *    deflation, sub-diagonalisation guess on 800 ISs; block/subspace 5/20 for
  *    method 1, 2/10 for method 2).
  *  - bounds[1], method 1 'davlanc' (:85-160): the highest eigenvalue of block
@@ -300,6 +304,22 @@ int32_t adc2g_cheby_auto(adc2g_handle* h, const double* bounds, const double* q0
  * bounds; here they are returned unwidened.  ADC2G_ENOCONV when the
  * Davidson or the power iterations do not converge (the reference stops). */
 int32_t adc2g_spectral_bounds(adc2g_handle* h, int32_t method, int32_t ncycles, double* bounds);
+
+/* Start block of block Lanczos from sub-diagonalisations of the matrix, into
+ * q0 (host, ndim x blocksize, column-major):
+ *  - kind 8, init_vec_doublesubspace (source/eigen/block_lanczos.f90:443-744):
+ *    columns 1..bs/2 the lowest eigenvectors of the subdim ISs of lowest
+ *    H_jj; columns bs/2+1..bs eigenvectors bs/2+1..bs (ascending eigenvalue
+ *    order, as the reference's column loop takes them) of the subdim ISs of
+ *    highest H_jj; blocksize even;
+ *  - kind 9, init_vec_singlesubspace (:748-923): the lowest blocksize
+ *    eigenvectors of the subdim ISs of highest H_jj.
+ * subdim 0 means the reference's 800; it grows while the next H_jj is within
+ * 1e-6 of the last (at most 4096).  The sub-matrix elements come from
+ * sigma-vectors of unit vectors, with |h| <= 1e-12 dropped as the stored
+ * matrix drops them.  The two halves of kind 8 are orthonormal when the two
+ * IS sets are disjoint (ndim >= 2 subdim) or when they are the whole space. */
+int32_t adc2g_lanczos_guess(adc2g_handle* h, int32_t kind, int32_t subdim, int32_t blocksize, double* q0);
 
 /* Highest eigenpair by the power method (power_method,
  * source/eigen/power.f90:22-100).  The guess is the highest eigenvector of the

@@ -214,6 +214,43 @@ def block_lanczos(matvec, q0: np.ndarray, ncycles: int, orthotype: int = 0) -> d
             "npro": npro}
 
 
+def subspace_guess(hii, hsub, blocksize, kind=8, subdim=800, minc=1e-12):
+    """init_vec_doublesubspace (kind 8, :443-744) / init_vec_singlesubspace
+    (kind 9, :748-923).  A subspace is the subdim ISs of lowest (ascending)
+    or highest (descending) H_jj, extended while the next H_jj is within
+    1e-6 of the last; ``hsub(sel)`` returns H[sel][:, sel] (|h| <= minc
+    dropped, the diagonal from hii).  Kind 8: columns 1..bs/2 are the lowest
+    eigenvectors of the ascending subspace, columns bs/2+1..bs eigenvectors
+    bs/2+1..bs (ascending eigenvalue order, as the reference's column loop
+    takes them) of the descending one.  Kind 9: the lowest bs eigenvectors of
+    the descending subspace."""
+    hii = np.asarray(hii, dtype=np.float64)
+    n = hii.size
+
+    def block(descending):
+        order = np.argsort(-hii if descending else hii, kind="stable")
+        sd = min(subdim, n)
+        while sd < n and abs(hii[order[sd - 1]] - hii[order[sd]]) < 1e-6:
+            sd += 1
+        sel = order[:sd]
+        A = np.array(hsub(sel), dtype=np.float64)
+        A[np.abs(A) <= minc] = 0.0
+        A[np.arange(sd), np.arange(sd)] = hii[sel]
+        return sel, np.linalg.eigh(A)[1]
+
+    V = np.zeros((n, blocksize))
+    if kind == 9:
+        sel, U = block(True)
+        V[sel, :] = U[:, :blocksize]
+        return V
+    h = blocksize // 2
+    sel, U = block(False)
+    V[sel, :h] = U[:, :h]
+    sel, U = block(True)
+    V[sel, h:] = U[:, h:blocksize]
+    return V
+
+
 def tranmom(ritz: np.ndarray, travec: np.ndarray) -> np.ndarray:
     """travec . r_n for every Ritz vector: the tm(ndim, vec, mtm) products that
     get_tranmom_1 (source/adclib/fspacetrial.f90:1076-1156, called at

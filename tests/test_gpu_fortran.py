@@ -184,3 +184,39 @@ def test_fortran_driver_fvector_initial_state(tmp_path):
     _, ref = sil.propagate(lambda x: -1j * (H @ x), f, tout=0.1, tfinal=1.0, kdim=12, autotol=1e-10)
     assert data.shape == (ref.size, 4)
     assert np.abs(data[:, 1] + 1j * data[:, 2] - ref).max() < 1e-10 * np.dot(f, f) + 1e-13
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built")
+def test_fortran_driver_lanczos_double_subspace(tmp_path):
+    """lancguess=8 (init_vec_doublesubspace, 800 ISs) with orthotype=2 at the
+    water size: the Lanczos-state energies equal the oracle's block Lanczos
+    from the oracle's guess."""
+    from oracle import lanczos
+    print("[case] fortran driver lancguess 8 water o=5 v=36 naux=164", flush=True)
+    o, v, naux = 5, 36, 164
+    sys_ = make_system(o, v, naux, seed=0)
+    sys_.eps.astype("<f8").tofile(tmp_path / "eps.bin")
+    sys_.B.astype("<f8").tofile(tmp_path / "B.bin")
+    df = dfsigma.DF(sys_)
+    n = df.n1 + df.D2.size
+    f = np.zeros(n)
+    f[: df.n1] = unit_dipole(o, v, seed=2)
+    f.astype("<f8").tofile(tmp_path / "f.bin")
+    nml = (f"&adc2g nocc={o} nvirt={v} naux={naux} eps_file='eps.bin' b_file='B.bin' nstates=1 "
+           f"blocksize=4 maxvec=16 tol=1d-6 precon=2 davname='davstates' fvec_file='f.bin' "
+           f"lanczos=.true. lmain=2 ncycles=6 lancguess=8 orthotype=2 lancname='lancstates' /\n")
+    r = subprocess.run([DRIVER], input=nml, capture_output=True, text=True, cwd=tmp_path, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+    def hsub(sel):
+        X = np.zeros((n, sel.size))
+        X[sel, np.arange(sel.size)] = 1.0
+        return df.sigma(X)[sel]
+
+    q0 = lanczos.subspace_guess(df.diag(), hsub, 2, kind=8, subdim=800)
+    ref = lanczos.block_lanczos(df.sigma, q0, 6, orthotype=2)
+    recs = read_davstates(tmp_path / "lancstates", n)
+    ev = np.array([x[1] for x in recs])
+    assert ev.size == 12
+    assert np.abs(ev - ref["eigval"]).max() < 1e-9 * np.abs(ref["eigval"]).max()

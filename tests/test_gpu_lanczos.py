@@ -158,3 +158,47 @@ def test_lanczos_modified_partial_reorthogonalisation_vs_oracle(form):
     assert st["orth_loss"] < 1e-10
     assert np.abs(ev - ref["eigval"]).max() < 1e-10 * scale
     assert (np.abs(ev - lam[-1]) < 1e-6).sum() == 1
+
+
+@pytest.mark.parametrize("kind,b", [(8, 2), (8, 4), (9, 3)])
+def test_lanczos_guess_vs_oracle(kind, b):
+    """init_vec_doublesubspace (8) / init_vec_singlesubspace (9) with a
+    20-IS subspace: the oracle's columns up to sign, orthonormal."""
+    print(f"[case] lanczos guess kind={kind} b={b} o=3 v=8 naux=10", flush=True)
+    sys = make_system(3, 8, 10, seed=5)
+    Hm = elements.build_blocks(sys, apply_minc=True).dense()
+    ref = lanczos.subspace_guess(np.diag(Hm), lambda s: Hm[np.ix_(s, s)], b, kind=kind, subdim=20)
+    with lib.Operator.from_system(sys) as op:
+        q0 = op.lanczos_guess(kind, b, subdim=20)
+        with pytest.raises(lib.Adc2gError):
+            op.lanczos_guess(8, 3, subdim=20)         # odd block for the double subspace
+    assert _match_up_to_sign(q0, ref) < 1e-10
+    assert np.abs(q0.T @ q0 - np.eye(b)).max() < 1e-12
+
+
+def test_spectral_bounds_lanczos_water_vs_oracle():
+    """spectral_bounds 'lanczos' at the water size (ndim 16470, so the two
+    800-IS subspaces are disjoint): the oracle's guess and 50-cycle block
+    Lanczos (orthotype 0) give the same lowest and highest eigenvalues of T to
+    1e-9 relative."""
+    from oracle import dfsigma
+    print("[case] spectral bounds lanczos water o=5 v=36 naux=164", flush=True)
+    sys = make_system(5, 36, 164, seed=0)
+    df = dfsigma.DF(sys)
+    n = df.n1 + df.D2.size
+    hii = df.diag()
+
+    def hsub(sel):
+        X = np.zeros((n, sel.size))
+        X[sel, np.arange(sel.size)] = 1.0
+        return df.sigma(X)[sel]
+
+    q0 = lanczos.subspace_guess(hii, hsub, 2, kind=8, subdim=800)
+    ref = lanczos.block_lanczos(df.sigma, q0, 50)
+    with lib.Operator.from_system(sys) as op:
+        lo, hi = op.spectral_bounds("lanczos")
+    scale = np.abs(ref["eigval"]).max()
+    print(f"[bounds lanczos] GPU {lo:.12f} {hi:.12f} oracle {ref['eigval'][0]:.12f} {ref['eigval'][-1]:.12f}",
+          flush=True)
+    assert abs(lo - ref["eigval"][0]) < 1e-9 * scale
+    assert abs(hi - ref["eigval"][-1]) < 1e-9 * scale

@@ -266,6 +266,33 @@ def test_oracle_mpro_bounds():
     assert lanczos._upperbound(om, j, 7, eta) == 8          # starts at k + 1 whatever om(j+1,k+1)
 
 
+def test_oracle_lanczos_subspace_guess():
+    """init_vec_doublesubspace: with subdim = ndim both subspaces are the whole
+    space, so the columns are eigenvectors 1..bs/2 and bs/2+1..bs of H; with
+    small disjoint subspaces the columns are orthonormal, supported on the
+    selected ISs, and the Ritz values of the block lie in the spectrum."""
+    rng = np.random.default_rng(3)
+    n, b = 40, 4
+    G = rng.standard_normal((n, n))
+    H = G + G.T + np.diag(np.linspace(-20, 20, n))
+    lam, U = np.linalg.eigh(H)
+    hii = np.diag(H).copy()
+    V = lanczos.subspace_guess(hii, lambda s: H[np.ix_(s, s)], b, kind=8, subdim=n)
+    for c in range(b):
+        assert min(np.abs(V[:, c] - U[:, c]).max(), np.abs(V[:, c] + U[:, c]).max()) < 1e-10
+    V = lanczos.subspace_guess(hii, lambda s: H[np.ix_(s, s)], b, kind=8, subdim=10)
+    assert np.abs(V.T @ V - np.eye(b)).max() < 1e-12
+    lo, hi = np.argsort(hii)[:10], np.argsort(-hii)[:10]
+    assert np.abs(np.delete(V[:, :2], lo, axis=0)).max() == 0.0
+    assert np.abs(np.delete(V[:, 2:], hi, axis=0)).max() == 0.0
+    # the second half holds eigenvectors 3 and 4 of the high block, not its lowest
+    A = H[np.ix_(hi, hi)]
+    w = np.linalg.eigvalsh(A)
+    assert np.allclose(np.sort(np.diag(V[:, 2:].T @ H @ V[:, 2:])), w[2:4])
+    V9 = lanczos.subspace_guess(hii, lambda s: H[np.ix_(s, s)], 2, kind=9, subdim=10)
+    assert np.allclose(np.sort(np.diag(V9.T @ H @ V9)), w[:2])
+
+
 def test_oracle_block_lanczos_full_krylov():
     """With blocksize * ncycles = ndim the Krylov space is the whole space and
     the spectrum of T is the spectrum of H."""
