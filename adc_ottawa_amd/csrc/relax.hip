@@ -152,6 +152,102 @@ struct SigmaClock {
 
 }  // namespace
 
+#define RL_LOOP(i) \
+  for (int64_t i = blockIdx.x * (int64_t)RL_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RL_THREADS)
+
+// ---- Bulirsch-Stoer (bslib.f90: BSStep, ModMidpoint, PolyExtrapol, AbsBSError)
+// for the relaxation right-hand side f(psi) = -H psi.  The reference works in
+// complex arithmetic, but with a real start vector and f real every
+// imaginary part stays exactly zero, so the vectors here are real.
+
+// psi1 = saved, psi2 = saved + h dt
+__global__ void __launch_bounds__(RL_THREADS) bs_mm_start_kernel(int64_t n, const double* __restrict__ saved,
+                                                                 const double* __restrict__ dt, double h,
+                                                                 double* __restrict__ psi1, double* __restrict__ psi2) {
+  RL_LOOP(i) {
+    psi1[i] = saved[i];
+    psi2[i] = saved[i] + h * dt[i];
+  }
+}
+
+// est = -w;  swap = psi1 + 2h est;  psi1 = psi2;  psi2 = swap
+__global__ void __launch_bounds__(RL_THREADS) bs_mm_step_kernel(int64_t n, const double* __restrict__ w, double h2,
+                                                                double* __restrict__ psi1, double* __restrict__ psi2) {
+  RL_LOOP(i) {
+    const double swap = psi1[i] + h2 * (-w[i]);
+    psi1[i] = psi2[i];
+    psi2[i] = swap;
+  }
+}
+
+// out = 0.5 (psi1 + psi2 + h est), est = -w
+__global__ void __launch_bounds__(RL_THREADS) bs_mm_final_kernel(int64_t n, const double* __restrict__ w, double h,
+                                                                 const double* __restrict__ psi1,
+                                                                 const double* __restrict__ psi2,
+                                                                 double* __restrict__ out) {
+  RL_LOOP(i) out[i] = 0.5 * (psi1[i] + psi2[i] + h * (-w[i]));
+}
+
+constexpr int BS_MAXORDER = 16;
+struct BSFactors {
+  double f[BS_MAXORDER + 1], g[BS_MAXORDER + 1];
+};
+
+// PolyExtrapol for one column (tableau columns 1..IntOrder-1 at T + (c-1)*n)
+// fused with AbsBSError: part[blk] = max |DeltaPsi| over the block
+__global__ void __launch_bounds__(RL_THREADS) bs_extrapol_kernel(int64_t n, int column, int int_order, BSFactors fac,
+                                                                 double* __restrict__ psi, double* __restrict__ T,
+                                                                 double* __restrict__ part) {
+  double mx = 0.0;
+  RL_LOOP(i) {
+    double x = psi[i];
+    double dpsi = x;
+    if (column == 1) {
+      T[i] = x;
+    } else {
+      double diff = x;
+      for (int q = 1; q < column; ++q) {
+        double* Tq = T + (size_t)n * (q - 1);
+        const double interim = Tq[i];
+        Tq[i] = dpsi;
+        const double delta = diff - interim;
+        dpsi = fac.f[q] * delta;
+        diff = fac.g[q] * delta;
+        x = x + dpsi;
+      }
+      if (column < int_order) T[(size_t)n * (column - 1) + i] = dpsi;
+      psi[i] = x;
+    }
+    mx = fmax(mx, fabs(dpsi));
+  }
+  __shared__ double red[RL_THREADS / 64];
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < RL_THREADS / 64; ++k) t = fmax(t, red[k]);
+    part[blockIdx.x] = t;
+  }
+}
+
+#undef RL_LOOP
+
+// BSStep with its Fortran save variables as members (they persist from call
+// to call, as in the reference).
+struct BSEngine {
+  int optimal_column = 0;
+  int nsteps[BS_MAXORDER + 2] = {};
+  double int_accuracy = 0.0, new_time = 0.0;
+  double expense[BS_MAXORDER + 2] = {};
+  double correction[BS_MAXORDER + 1][BS_MAXORDER + 1] = {};
+  double old_accuracy = -1.0;
+  int old_int_order = -1;
+  bool first_call = true;
+  double old_sq[BS_MAXORDER + 1] = {};
+};
+
 void relax_run(Handle& H, const adc2g_relax_params& p, const double* guess_h, double* eigval, double* eigvec,
                double* resid_out, int32_t* niter_out, adc2g_relax_stats* stats) {
   using clk = std::chrono::steady_clock;
@@ -169,6 +265,10 @@ void relax_run(Handle& H, const adc2g_relax_params& p, const double* guess_h, do
   const Prec pr = prec_from_int(p.precision);
   hipStream_t st = H.st;
   const int nblk = reduce_blocks(n);
+  const bool bs_mode = p.integrator == 2;
+  ARG_CHECK(p.integrator >= 0 && p.integrator <= 2,
+            "integrator must be 0 or 1 (XSIL) or 2 (Bulirsch-Stoer); RKF45 (3) is not provided");
+  ARG_CHECK(!bs_mode || (std::isfinite(p.toler) && p.toler > 0.0), "toler must be positive for Bulirsch-Stoer");
 
   DevBuf<double> L, r, psi, hpsi, guess, conv, part, sc, coef, Sx;
   L.alloc((size_t)n * maxvec);
@@ -244,6 +344,163 @@ void relax_run(Handle& H, const adc2g_relax_params& p, const double* guess_h, do
   std::vector<double> alpha(kd), beta(kd), b2h(kd), sx((size_t)maxvec * kb);
   auto SH = [&](std::vector<double>& m, int i, int j) -> double& { return m[i + (size_t)maxvec * j]; };
 
+  // Bulirsch-Stoer state (bs_rlx: intorder 16, nextstep 0, time 0) and
+  // vectors: saved psi, dtpsi (as H psi), the midpoint pair, the tableau
+  BSEngine bse;
+  int bs_order = BS_MAXORDER;
+  double bs_next = 0.0, bs_time = 0.0;
+  DevBuf<double> bs_saved, bs_w, bs_p1, bs_p2, bs_tab, bs_dt;
+  std::vector<double> bs_part(nblk);
+  if (bs_mode) {
+    bs_saved.alloc(n);
+    bs_w.alloc(n);
+    bs_dt.alloc(n);
+    bs_p1.alloc(n);
+    bs_p2.alloc(n);
+    bs_tab.alloc((size_t)n * BS_MAXORDER);
+  }
+  // one BSStep call; psi holds Psi, bs_w holds H psi (DtPsi = -bs_w)
+  auto bs_step = [&](double int_period, double abs_time, double large_step, double tol_error,
+                     double& actual_out) -> int {
+    const double RelativeMinStep = 1.0e-10, MaxScale = 0.1, ErrorSafety = 0.25, ReductionSafety = 0.7,
+                 MinReduction = 0.5, MaxReduction = 1.0e-5, Tiny = 1.0e-25;
+    int& io = bs_order;
+    if (io < 2 || io > BS_MAXORDER) return 1;
+    const double min_large = RelativeMinStep * int_period;
+    double next_large = -Tiny;
+    bse.new_time = -Tiny;
+    if (tol_error != bse.old_accuracy || io != bse.old_int_order) {
+      bse.old_accuracy = tol_error;
+      bse.int_accuracy = ErrorSafety * tol_error;
+      bse.old_int_order = io;
+      for (int q = 1; q <= io + 1; ++q) bse.nsteps[q] = 2 * q;
+      bse.expense[1] = bse.nsteps[1] + 1;
+      for (int q = 2; q <= io + 1; ++q) bse.expense[q] = bse.expense[q - 1] + bse.nsteps[q];
+      for (int q = 2; q <= io; ++q)
+        for (int q1 = 1; q1 < q; ++q1)
+          bse.correction[q1][q] =
+              std::pow(bse.int_accuracy, (bse.expense[q1 + 1] - bse.expense[q + 1]) /
+                                             ((2 * q1 + 1) * (bse.expense[q + 1] - bse.expense[1] + 1.0)));
+      int oc = 2;
+      while (oc <= io - 1) {
+        if (bse.expense[oc + 1] > bse.expense[oc] * bse.correction[oc - 1][oc]) break;
+        ++oc;
+      }
+      bse.optimal_column = oc;
+      io = oc;
+    }
+    double actual = large_step;
+    bool reduced = false;
+    double min_work = 1.0e35;
+    HIP_CHECK(hipMemcpyAsync(bs_saved.get(), psi.get(), sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    // DtPsi = -H psi, kept apart from bs_w, which the midpoint steps reuse
+    HIP_CHECK(hipMemcpyAsync(bs_dt.get(), bs_w.get(), sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    if (actual != next_large || abs_time != bse.new_time) {
+      bse.first_call = true;
+      bse.optimal_column = io;
+    }
+    double err[BS_MAXORDER] = {};
+    int q = 0, q1 = 0;
+    for (;;) {  // 300
+      double reduction = 0.0;
+      bool conv = false;
+      q1 = 0;
+      for (q = 1; q <= io; ++q) {
+        bse.new_time = abs_time + actual;
+        const double sq = (actual / bse.nsteps[q]) * (actual / bse.nsteps[q]);
+        if (actual < min_large && bse.new_time < int_period) return 2;
+        // ModMidpoint: dt = -H saved is -bs_dt
+        const int ns_q = bse.nsteps[q];
+        const double h = actual / ns_q, h2 = 2.0 * h;
+        hipLaunchKernelGGL(bs_mm_start_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, bs_saved.get(), bs_dt.get(),
+                           -h, bs_p1.get(), bs_p2.get());
+        HIP_CHECK(hipGetLastError());
+        sigma1(bs_p2.get(), bs_w.get());
+        for (int k = 2; k <= ns_q; ++k) {
+          hipLaunchKernelGGL(bs_mm_step_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, bs_w.get(), h2, bs_p1.get(),
+                             bs_p2.get());
+          HIP_CHECK(hipGetLastError());
+          sigma1(bs_p2.get(), bs_w.get());
+        }
+        hipLaunchKernelGGL(bs_mm_final_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, bs_w.get(), h, bs_p1.get(),
+                           bs_p2.get(), psi.get());
+        HIP_CHECK(hipGetLastError());
+        // PolyExtrapol + AbsBSError
+        bse.old_sq[q] = sq;
+        BSFactors fac{};
+        for (int pp = 1; pp < q; ++pp) {
+          const double hl = 1.0 / (bse.old_sq[q - pp] - sq);
+          fac.f[pp] = sq * hl;
+          fac.g[pp] = bse.old_sq[q - pp] * hl;
+        }
+        hipLaunchKernelGGL(bs_extrapol_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, q, io, fac, psi.get(),
+                           bs_tab.get(), part.get());
+        HIP_CHECK(hipGetLastError());
+        double max_error = 0.0;
+        q1 = 0;
+        if (q >= 2) {
+          HIP_CHECK(hipMemcpyAsync(bs_part.data(), part.get(), sizeof(double) * nblk, hipMemcpyDeviceToHost, st));
+          HIP_CHECK(hipStreamSynchronize(st));
+          clock.collect();
+          for (double x : bs_part) max_error = std::max(max_error, x);
+          q1 = q - 1;
+          err[q1] = std::pow(max_error / bse.int_accuracy, 1.0 / (2 * q1 + 1.0));
+        }
+        if (q >= 2 && (q >= bse.optimal_column - 1 || bse.first_call)) {
+          if (max_error < tol_error) {
+            conv = true;
+            break;
+          }
+          const int oc = bse.optimal_column;
+          if (q == io || q == oc + 1) {
+            reduction = ReductionSafety / err[q1];
+            break;
+          } else if (q == oc) {
+            if (bse.correction[oc - 1][oc] < err[q1]) {
+              reduction = 1.0 / err[q1];
+              break;
+            }
+          } else if (oc == io) {
+            if (bse.correction[q1][io - 1] < err[q1]) {
+              reduction = ReductionSafety * bse.correction[q1][io - 1] / err[q1];
+              break;
+            }
+          } else if (bse.correction[q1][oc] < err[q1]) {
+            reduction = bse.correction[q1][oc - 1] / err[q1];
+            break;
+          }
+        }
+      }
+      if (conv) break;
+      reduction = std::max(std::min(reduction, MinReduction), MaxReduction);  // 400
+      actual *= reduction;
+      reduced = true;
+    }
+    bse.first_call = false;  // 500
+    double scale = 0.0;
+    for (int q2 = 1; q2 <= q1; ++q2) {
+      const double wf = std::max(err[q2], MaxScale);
+      const double work = wf * bse.expense[q2 + 1];
+      if (work < min_work) {
+        scale = wf;
+        min_work = work;
+        bse.optimal_column = q2 + 1;
+      }
+    }
+    next_large = actual / scale;
+    const int oc = bse.optimal_column;
+    if (oc >= q && oc < io && !reduced) {
+      const double wf = std::max(scale / bse.correction[oc - 1][oc], MaxScale);
+      if (bse.expense[oc + 1] * wf <= min_work) {
+        next_large = actual / wf;
+        bse.optimal_column = oc + 1;
+      }
+    }
+    bs_next = next_large;
+    actual_out = actual;
+    return 0;
+  };
+
   int nconv = 0;
   int status = ADC2G_OK;
   std::string fail_msg;
@@ -258,7 +515,43 @@ void relax_run(Handle& H, const adc2g_relax_params& p, const double* guess_h, do
     if (p.verbose) std::fprintf(stderr, "relax state %d step %5d  E = %.12f  residual = %.4e\n", s + 1, 0, E, res);
     int it_done = p.maxiter;
     bool converged = false;
-    for (int it = 1; it <= p.maxiter; ++it) {
+    if (bs_mode) {
+      // bs_rlx (relaxation.f90:315-495): int(tfinal/intperiod) timesteps with
+      // tfinal = maxiter*step, each covered by BSStep calls
+      const double intperiod = p.step, tfinal = (double)p.maxiter * p.step, tiny = 1e-9;
+      const int nt = (int)(tfinal / intperiod);
+      it_done = nt;
+      for (int it = 1; it <= nt; ++it) {
+        double inttime = 0.0;
+        for (;;) {
+          double stepsize;
+          if (inttime == 0.0) stepsize = intperiod;
+          else if (inttime + bs_next > intperiod) stepsize = intperiod - inttime;
+          else stepsize = bs_next;
+          sigma1(psi.get(), bs_w.get());  // dtpsi = -H psi
+          double true_step = 0.0;
+          const int code = bs_step(intperiod, bs_time, stepsize, p.toler, true_step);
+          if (code != 0)
+            throw Error(ADC2G_EBREAK, std::string("Failure in the Bulirsch-Stoer integrator: ") +
+                                          (code == 1 ? "Illegal integration order" : "Stepsize underflow"));
+          bs_time += true_step;
+          inttime += true_step;
+          if (!(std::fabs(intperiod - inttime) > std::fabs(tiny * intperiod))) break;
+        }
+        normalise(psi.get(), w_d + 2);
+        residual(psi.get(), E, res);
+        if (!std::isfinite(E) || !std::isfinite(res))
+          throw Error(ADC2G_EBREAK, "relaxation: non-finite energy at state " + std::to_string(s + 1));
+        if (p.verbose)
+          std::fprintf(stderr, "relax state %d step %5d  E = %.12f  residual = %.4e\n", s + 1, it, E, res);
+        if (res <= p.tol) {
+          converged = true;
+          it_done = it;
+          break;
+        }
+      }
+    }
+    for (int it = 1; !bs_mode && it <= p.maxiter; ++it) {
       if (ncurr * kb == maxvec) {  // collapse_subspace
         for (int blk = 1; blk < maxvec / kb; ++blk)
           HIP_CHECK(hipMemcpyAsync(L.get() + (size_t)n * kb * (blk - 1), L.get() + (size_t)n * kb * blk,

@@ -14,8 +14,9 @@
 !          nstates=5 blocksize=5 maxvec=40 tol=1d-7 precon=1 davname='davstates'
 !          sil=.true. fvec_file='f.bin' tout=0.1d0 tfinal=2d0 krydim=20 autotol=1d-10 /
 ! Further keys:
-!   solver = 1 Davidson (davdiag_block) | 2 relaxation (XSIL, relaxation.f90:36-133)
+!   solver = 1 Davidson (davdiag_block) | 2 relaxation (relaxation.f90:36-133)
 !            with stepsize, rlx_krydim and the same nstates / maxiter / tol;
+!            integrator = 1 XSIL (default) | 2 Bulirsch-Stoer with siltol;
 !   guess = 0 unit vectors | 1 sub-diagonalisation (subdim ISs) for Davidson;
 !   autoord = 0..2: SIL autocorrelation orders, files auto_file, 'auto1', 'auto2';
 !   lanczos = .true.: block Lanczos (lancdiag_block) with lmain start vectors and
@@ -42,17 +43,17 @@ program adc2g_driver
   integer :: nocc = 0, nvirt = 0, naux = 0, nstates = 5, blocksize = 0, maxvec = -1, maxiter = 300
   integer :: precon = 2, precision = 0, krydim = 10
   integer :: solver = 1, guess = 0, subdim = 800, rlx_krydim = 10, autoord = 0
-  integer :: lmain = 0, ncycles = 0, chebyord = 0, lancguess = 1, orthotype = 0
+  integer :: lmain = 0, ncycles = 0, chebyord = 0, lancguess = 1, orthotype = 0, integrator = 1
   logical :: deflate = .true., sil = .false., lanczos = .false., hmlt = .false.
   real(c_double) :: tol = 1.0d-7, tout = 0.1d0, tfinal = 1.0d0, autotol = 1.0d-5
-  real(c_double) :: stepsize = 10.0d0, cheby_emin = 0.0d0, cheby_emax = 1.0d0
+  real(c_double) :: stepsize = 10.0d0, cheby_emin = 0.0d0, cheby_emax = 1.0d0, siltol = 1.0d-5
   character(len=256) :: eps_file = 'eps.bin', b_file = 'B.bin', davname = 'davstates'
   character(len=256) :: fvec_file = 'fvec.bin', auto_file = 'auto', lancname = 'lancstates'
   character(len=256) :: dpl_file = ''
   namelist /adc2g/ nocc, nvirt, naux, eps_file, b_file, nstates, blocksize, maxvec, maxiter, tol, &
        precon, precision, deflate, davname, sil, fvec_file, auto_file, tout, tfinal, krydim, autotol, &
        solver, guess, subdim, stepsize, rlx_krydim, autoord, lanczos, lmain, ncycles, lancname, &
-       chebyord, cheby_emin, cheby_emax, hmlt, dpl_file, lancguess, orthotype
+       chebyord, cheby_emin, cheby_emax, hmlt, dpl_file, lancguess, orthotype, integrator, siltol
 
   type(c_ptr) :: h
   type(adc2g_opts) :: opts
@@ -101,6 +102,8 @@ program adc2g_driver
      rp%step = stepsize
      rp%tol = tol
      rp%precision = precision
+     rp%integrator = integrator
+     rp%toler = siltol
      rc = adc2g_relax(h, rp, c_null_ptr, eigval, c_loc(eigvec), c_null_ptr, c_null_ptr, rs)
      call check(rc)
      write(*, '(a,i8,a,f10.3,a)') ' Relaxation: sigma-vectors', rs%nsigma, ', time', rs%time_total, ' s'

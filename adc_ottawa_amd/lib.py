@@ -83,7 +83,8 @@ class LanczosStats(C.Structure):
 class RelaxParams(C.Structure):
     _fields_ = [("nstates", C.c_int32), ("krydim", C.c_int32), ("maxdim", C.c_int32),
                 ("maxiter", C.c_int32), ("step", C.c_double), ("tol", C.c_double),
-                ("precision", C.c_int32), ("verbose", C.c_int32)]
+                ("precision", C.c_int32), ("verbose", C.c_int32), ("integrator", C.c_int32),
+                ("reserved", C.c_int32), ("toler", C.c_double)]
 
 
 class RelaxStats(C.Structure):
@@ -340,15 +341,17 @@ class Operator:
         return ev, extras, stats
 
     def relax(self, nstates, guess=None, krydim=10, maxdim=-1, maxiter=100, step=10.0, tol=1e-7,
-              precision="fp64", vectors=True, verbose=False):
-        """Imaginary-time relaxation (XSIL) of the nstates lowest states.
+              precision="fp64", vectors=True, verbose=False, integrator="xsil", toler=1e-5):
+        """Imaginary-time relaxation of the nstates lowest states, with the XSIL
+        or ('bs') the Bulirsch-Stoer integrator (error tolerance toler).
 
         guess: (ndim, nstates) or None (unit vectors on the smallest |H_jj|).
         Returns (energies, vectors or None, residuals, iterations, stats)."""
         g = None
         if guess is not None:
             g = np.asfortranarray(np.asarray(guess, dtype=np.float64).reshape(self.ndim, nstates))
-        p = RelaxParams(nstates, krydim, maxdim, maxiter, step, tol, PREC[precision], 1 if verbose else 0)
+        p = RelaxParams(nstates, krydim, maxdim, maxiter, step, tol, PREC[precision], 1 if verbose else 0,
+                        {"xsil": 1, "bs": 2}[integrator], 0, toler)
         ev = np.empty(nstates)
         vec = np.empty((self.ndim, nstates), order="F") if vectors else None
         res = np.empty(nstates)

@@ -188,3 +188,224 @@ def ondiag_guess(hii: np.ndarray, nstates: int) -> np.ndarray:
     g = np.zeros((hii.size, nstates))
     g[idx, np.arange(nstates)] = 1.0
     return g
+
+
+# ---------------------------------------------------------------- Bulirsch-Stoer
+class BSStepper:
+    """BSStep (source/propagation/bslib.f90) with ModMidpoint, PolyExtrapol
+    and AbsBSError, for a real right-hand side f(psi) = -H psi (the
+    relaxation case, where the complex arithmetic keeps zero imaginary
+    parts).  The Fortran ``save`` variables (OptimalColumn, NumberOfSteps,
+    IntAccuracy, NewTime, Expense, Correction, OldAccuracy, OldIntOrder,
+    CallIsFirstCall and PolyExtrapol's OldSquaredStepSize) live on the
+    object, so they persist from call to call as in the reference."""
+
+    MAXORDER = 16
+
+    def __init__(self):
+        M = self.MAXORDER
+        self.optimal_column = 0
+        self.nsteps = np.zeros(M + 2, dtype=np.int64)      # 1-based
+        self.int_accuracy = 0.0
+        self.new_time = 0.0
+        self.expense = np.zeros(M + 2)
+        self.correction = np.zeros((M + 1, M + 1))          # 1-based
+        self.old_accuracy = -1.0
+        self.old_int_order = -1
+        self.first_call = True
+        self.old_sq = np.zeros(M + 1)                        # PolyExtrapol save
+        self.nfunc = 0
+
+    def _midpoint(self, f, saved, dtpsi, large, nsteps):
+        h = large / nsteps
+        psi1 = saved.copy()
+        psi2 = saved + h * dtpsi
+        est = f(psi2)
+        self.nfunc += 1
+        for _ in range(2, nsteps + 1):
+            swap = psi1 + 2.0 * h * est
+            psi1 = psi2
+            psi2 = swap
+            est = f(psi2)
+            self.nfunc += 1
+        return 0.5 * (psi1 + psi2 + h * est)
+
+    def _extrapol(self, column, psi, tableau, sq, int_order):
+        self.old_sq[column] = sq
+        delta_psi = psi.copy()
+        if column == 1:
+            tableau[:, 1] = psi
+            return psi, delta_psi
+        diff = psi.copy()
+        for p in range(1, column):
+            help_ = 1.0 / (self.old_sq[column - p] - sq)
+            factor = sq * help_
+            old_factor = self.old_sq[column - p] * help_
+            interim = tableau[:, p].copy()
+            tableau[:, p] = delta_psi
+            delta = diff - interim
+            delta_psi = factor * delta
+            diff = old_factor * delta
+            psi = psi + delta_psi
+        if column < int_order:
+            tableau[:, column] = delta_psi
+        return psi, delta_psi
+
+    def step(self, f, psi, dtpsi, int_period, abs_time, int_order, large_step, tol_error):
+        """Returns (psi, int_order, actual_large_step, next_large_step, error_code)."""
+        RelativeMinStep, MaxScale, ErrorSafety = 1.0e-10, 0.1, 0.25
+        ReductionSafety, MinReduction, MaxReduction, Tiny = 0.7, 0.5, 1.0e-5, 1.0e-25
+        if int_order < 2 or int_order > self.MAXORDER:
+            return psi, int_order, 0.0, 0.0, 1
+        min_large = RelativeMinStep * int_period
+        next_large = -Tiny
+        self.new_time = -Tiny
+        if tol_error != self.old_accuracy or int_order != self.old_int_order:
+            self.old_accuracy = tol_error
+            self.int_accuracy = ErrorSafety * tol_error
+            self.old_int_order = int_order
+            for p in range(1, int_order + 2):
+                self.nsteps[p] = 2 * p
+            self.expense[1] = self.nsteps[1] + 1
+            for p in range(2, int_order + 2):
+                self.expense[p] = self.expense[p - 1] + self.nsteps[p]
+            for p in range(2, int_order + 1):
+                for p1 in range(1, p):
+                    self.correction[p1, p] = self.int_accuracy ** (
+                        (self.expense[p1 + 1] - self.expense[p + 1])
+                        / ((2 * p1 + 1) * (self.expense[p + 1] - self.expense[1] + 1.0)))
+            oc = 2
+            while oc <= int_order - 1:                       # Do OptimalColumn = 2,IntOrder-1
+                if self.expense[oc + 1] > self.expense[oc] * self.correction[oc - 1, oc]:
+                    break
+                oc += 1
+            self.optimal_column = oc
+            int_order = oc
+        actual = large_step
+        reduced = False
+        min_work = 1.0e35
+        n = psi.size
+        saved = psi.copy()
+        tableau = np.zeros((n, int_order + 1))               # AuxPsi(:,4:), columns 1..IntOrder-1
+        if actual != next_large or abs_time != self.new_time:
+            self.first_call = True
+            self.optimal_column = int_order
+        err = np.zeros(self.MAXORDER)
+        while True:                                           # 300
+            reduction = 0.0
+            converged = False
+            p1 = 0
+            for p in range(1, int_order + 1):
+                self.new_time = abs_time + actual
+                sq = (actual / self.nsteps[p]) ** 2
+                if actual < min_large and self.new_time < int_period:
+                    return psi, int_order, actual, next_large, 2
+                psi = self._midpoint(lambda x: f(x), saved, dtpsi, actual, int(self.nsteps[p]))
+                psi, delta_psi = self._extrapol(p, psi, tableau, sq, int_order)
+                max_error = 0.0
+                p1 = 0
+                if p >= 2:
+                    max_error = float(np.abs(delta_psi).max())
+                    p1 = p - 1
+                    err[p1] = (max_error / self.int_accuracy) ** (1.0 / (2 * p1 + 1.0))
+                if p >= 2 and (p >= self.optimal_column - 1 or self.first_call):
+                    if max_error < tol_error:
+                        converged = True
+                        break
+                    oc = self.optimal_column
+                    if p == int_order or p == oc + 1:
+                        reduction = ReductionSafety / err[p1]
+                        break
+                    elif p == oc:
+                        if self.correction[oc - 1, oc] < err[p1]:
+                            reduction = 1.0 / err[p1]
+                            break
+                    elif oc == int_order:
+                        if self.correction[p1, int_order - 1] < err[p1]:
+                            reduction = ReductionSafety * self.correction[p1, int_order - 1] / err[p1]
+                            break
+                    elif self.correction[p1, oc] < err[p1]:
+                        reduction = self.correction[p1, oc - 1] / err[p1]
+                        break
+            if converged:
+                break
+            reduction = max(min(reduction, MinReduction), MaxReduction)   # 400
+            actual = actual * reduction
+            reduced = True
+        self.first_call = False                               # 500
+        scale = 0.0
+        for p2 in range(1, p1 + 1):
+            wf = max(err[p2], MaxScale)
+            work = wf * self.expense[p2 + 1]
+            if work < min_work:
+                scale = wf
+                min_work = work
+                self.optimal_column = p2 + 1
+        next_large = actual / scale
+        oc = self.optimal_column
+        if oc >= p and oc < int_order and not reduced:
+            wf = max(scale / self.correction[oc - 1, oc], MaxScale)
+            if self.expense[oc + 1] * wf <= min_work:
+                next_large = actual / wf
+                self.optimal_column = oc + 1
+        return psi, int_order, actual, next_large, 0
+
+
+def bs_relax(matvec, guess: np.ndarray, maxiter=100, step=10.0, eps=1e-7, toler=1e-5, log=None):
+    """bs_rlx (source/propagation/relaxation.f90:315-495): each state from its
+    initial vector, timesteps of length ``step`` integrated by BSStep (maximum
+    order 16) until the whole step is covered; then the no-op orthogonalisation
+    loop (vec_conv(:,s)), normalisation and the residual test.  tfinal =
+    maxiter * step and int(tfinal / step) timesteps (evaluated in floating
+    point, as the reference does).  Returns dict(eigval, eigvec, niter,
+    residual, nfunc)."""
+    guess = np.asarray(guess, dtype=np.float64)
+    n, ns = guess.shape
+    f = lambda x: -matvec(x)
+    bs = BSStepper()
+    intorder = 16
+    tfinal = maxiter * step
+    nt = int(tfinal / step)
+    tiny = 1e-9
+    time = 0.0
+    E = np.zeros(ns)
+    V = np.zeros((n, ns))
+    niter = np.zeros(ns, dtype=np.int64)
+    resid = np.zeros(ns)
+    nextstep = 0.0
+    for s in range(ns):
+        psi = guess[:, s].copy()
+        it_done = nt
+        ok = False
+        for i in range(1, nt + 1):
+            inttime = 0.0
+            while True:
+                if inttime == 0.0:
+                    stepsize = step
+                elif inttime + nextstep > step:
+                    stepsize = step - inttime
+                else:
+                    stepsize = nextstep
+                dtpsi = f(psi)
+                psi, intorder, true_step, nextstep, code = bs.step(f, psi, dtpsi, step, time, intorder,
+                                                                   stepsize, toler)
+                if code != 0:
+                    raise RuntimeError(f"Bulirsch-Stoer error {code}")
+                time += true_step
+                inttime += true_step
+                if not abs(step - inttime) > abs(tiny * step):
+                    break
+            psi = psi / np.sqrt(psi @ psi)
+            hpsi = matvec(psi)
+            e = psi @ hpsi
+            r = np.linalg.norm(hpsi - e * psi)
+            if log is not None:
+                log.append((s, i, e, r))
+            if r <= eps:
+                ok = True
+                it_done = i
+                break
+        E[s], V[:, s], niter[s], resid[s] = e, psi, it_done, r
+        if not ok:
+            break
+    return {"eigval": E, "eigvec": V, "niter": niter, "residual": resid, "nfunc": bs.nfunc}

@@ -107,3 +107,24 @@ def test_relax_benzene_consistency():
     assert abs(np.linalg.norm(v) - 1.0) < 1e-12
     assert abs(e - ev[0]) < 1e-12 * abs(e)
     assert abs(r - res[0]) < 1e-9 * r
+
+
+@pytest.mark.parametrize("form", ("df", "int"))
+def test_relax_bulirsch_stoer_vs_oracle(form):
+    """integrator 2 (bs_rlx with BSStep): the oracle's number of timesteps
+    per state (within 1), energies to 1e-9 relative and the vectors up to
+    sign; both states reach the lowest eigenpair, as in the reference."""
+    print(f"[case] relax bulirsch-stoer o=3 v=8 naux=10 coupling={form}", flush=True)
+    sys = make_system(3, 8, 10, seed=5)
+    H = elements.build_blocks(sys, apply_minc=False).dense()
+    g = relax.ondiag_guess(np.diag(H), 2)
+    ref = relax.bs_relax(lambda x: H @ x, g, maxiter=300, step=2.0, eps=1e-6, toler=1e-8)
+    with lib.Operator.from_system(sys, coupling=form) as op:
+        E, V, res, nit, st = op.relax(2, maxiter=300, step=2.0, tol=1e-6, integrator="bs", toler=1e-8)
+    print(f"[bs] GPU niter {list(nit)} sigma {st['nsigma']}, oracle niter {list(ref['niter'])}", flush=True)
+    assert np.abs(nit - ref["niter"]).max() <= 1
+    assert np.abs(E - ref["eigval"]).max() < 1e-9 * np.abs(ref["eigval"]).max()
+    for s in range(2):
+        v, w = V[:, s], ref["eigvec"][:, s]
+        assert min(np.abs(v - w).max(), np.abs(v + w).max()) < 1e-5
+    assert np.all(res <= 1e-6)

@@ -511,3 +511,34 @@ def test_oracle_fvector(o, v, naux, seed):
     assert np.abs(L["tauA"] + parts["uu"].T / 4.0).max() < 1e-13 * sc
     assert np.abs(L["tauB"] - parts["oo"].T / 4.0).max() < 1e-13 * np.abs(parts["oo"]).max()
     assert np.abs(L["tau21"] + L["tau22"] + parts["ou"].T / 2.0).max() < 1e-13 * np.abs(parts["ou"]).max()
+
+
+def test_oracle_bulirsch_stoer_relaxation():
+    """bs_rlx: one Bulirsch-Stoer timestep equals exp(-H dtau) psi from
+    scipy.linalg.expm to 1e-8 (tolerance 1e-10), and the relaxation reaches
+    the lowest eigenpair; the second state reaches it too, since the
+    reference's orthogonalisation loop projects on vec_conv(:,s), which is
+    zero."""
+    from scipy.linalg import expm
+    sys = make_system(3, 8, 10, seed=5)
+    H = elements.build_blocks(sys, apply_minc=False).dense()
+    lam, U = np.linalg.eigh(H)
+    g = relax.ondiag_guess(np.diag(H), 2)
+    bs = relax.BSStepper()
+    f = lambda x: -(H @ x)
+    psi, t, order, nxt, inttime, step = g[:, 0].copy(), 0.0, 16, 0.0, 0.0, 2.0
+    while True:
+        ss = step if inttime == 0.0 else (step - inttime if inttime + nxt > step else nxt)
+        psi, order, tr, nxt, code = bs.step(f, psi, f(psi), step, t, order, ss, 1e-10)
+        assert code == 0
+        t += tr
+        inttime += tr
+        if not abs(step - inttime) > 1e-9 * step:
+            break
+    ex = expm(-H * step) @ g[:, 0]
+    assert np.abs(psi - ex).max() < 1e-8 * np.abs(ex).max()
+    out = relax.bs_relax(lambda x: H @ x, g, maxiter=300, step=2.0, eps=1e-6, toler=1e-8)
+    assert np.abs(out["eigval"] - lam[0]).max() < 1e-9 * abs(lam[0])
+    for s in range(2):
+        v = out["eigvec"][:, s]  # residual 1e-6 over a gap of 0.023
+        assert min(np.abs(v - U[:, 0]).max(), np.abs(v + U[:, 0]).max()) < 1e-4
