@@ -25,6 +25,7 @@
 //    normalisation is kept;
 //  * E = psi.H psi and |H psi - E psi| (residual_1vec, :1610-1651).
 #include <algorithm>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -232,6 +233,75 @@ __global__ void __launch_bounds__(RL_THREADS) bs_extrapol_kernel(int64_t n, int 
   }
 }
 
+// ---- RKF45 (rkf45rlxlib.f90: r8_fehl, r8_rkf45) for f(y) = -H y; the
+// stage combinations keep the reference's grouping (no contraction)
+
+// f = -w
+__global__ void __launch_bounds__(RL_THREADS) rk_neg_kernel(int64_t n, const double* __restrict__ w,
+                                                            double* __restrict__ f) {
+  RL_LOOP(i) f[i] = -w[i];
+}
+
+// x = y + ch * (stage combination of yp, f1..f5)
+__global__ void __launch_bounds__(RL_THREADS) rk_stage_kernel(int64_t n, int stage, double ch,
+                                                              const double* __restrict__ y,
+                                                              const double* __restrict__ yp,
+                                                              const double* __restrict__ f1,
+                                                              const double* __restrict__ f2,
+                                                              const double* __restrict__ f3,
+                                                              const double* __restrict__ f4,
+                                                              const double* __restrict__ f5, double* __restrict__ x) {
+#pragma clang fp contract(off)
+  RL_LOOP(i) {
+    double c;
+    switch (stage) {
+      case 1: c = yp[i]; break;
+      case 2: c = yp[i] + 3.0 * f1[i]; break;
+      case 3: c = 1932.0 * yp[i] + (7296.0 * f2[i] - 7200.0 * f1[i]); break;
+      case 4: c = (8341.0 * yp[i] - 845.0 * f3[i]) + (29440.0 * f2[i] - 32832.0 * f1[i]); break;
+      case 5: c = (-6080.0 * yp[i] + (9295.0 * f3[i] - 5643.0 * f4[i])) + (41040.0 * f1[i] - 28352.0 * f2[i]); break;
+      default:
+        c = (902880.0 * yp[i] + (3855735.0 * f3[i] - 1371249.0 * f4[i])) + (3953664.0 * f2[i] + 277020.0 * f5[i]);
+        break;
+    }
+    x[i] = y[i] + ch * c;
+  }
+}
+
+// part[blk] = max_k ee_k / et_k, et = |y| + |s| + ae
+__global__ void __launch_bounds__(RL_THREADS) rk_err_kernel(int64_t n, double ae, const double* __restrict__ y,
+                                                            const double* __restrict__ s,
+                                                            const double* __restrict__ yp,
+                                                            const double* __restrict__ f2,
+                                                            const double* __restrict__ f3,
+                                                            const double* __restrict__ f4,
+                                                            const double* __restrict__ f5, double* __restrict__ part) {
+#pragma clang fp contract(off)
+  double mx = 0.0;
+  RL_LOOP(i) {
+    const double et = fabs(y[i]) + fabs(s[i]) + ae;
+    const double ee = fabs((-2090.0 * yp[i] + (21970.0 * f3[i] - 15048.0 * f4[i])) +
+                           (22528.0 * f2[i] - 27360.0 * f5[i]));
+    mx = fmax(mx, ee / et);
+  }
+  __shared__ double red[RL_THREADS / 64];
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < RL_THREADS / 64; ++k) t = fmax(t, red[k]);
+    part[blockIdx.x] = t;
+  }
+}
+
+// r8_rkf45's save variables
+struct RKState {
+  double abserr_save = -1.0, h = -1.0, relerr_save = -1.0;
+  int flag_save = -1000, init = -1000, kflag = -1000, kop = -1, nfe = -1;
+};
+
 #undef RL_LOOP
 
 // BSStep with its Fortran save variables as members (they persist from call
@@ -266,9 +336,11 @@ void relax_run(Handle& H, const adc2g_relax_params& p, const double* guess_h, do
   hipStream_t st = H.st;
   const int nblk = reduce_blocks(n);
   const bool bs_mode = p.integrator == 2;
-  ARG_CHECK(p.integrator >= 0 && p.integrator <= 2,
-            "integrator must be 0 or 1 (XSIL) or 2 (Bulirsch-Stoer); RKF45 (3) is not provided");
-  ARG_CHECK(!bs_mode || (std::isfinite(p.toler) && p.toler > 0.0), "toler must be positive for Bulirsch-Stoer");
+  const bool rk_mode = p.integrator == 3;
+  ARG_CHECK(p.integrator >= 0 && p.integrator <= 3,
+            "integrator must be 0 or 1 (XSIL), 2 (Bulirsch-Stoer) or 3 (RKF45)");
+  ARG_CHECK(!(bs_mode || rk_mode) || (std::isfinite(p.toler) && p.toler > 0.0),
+            "toler must be positive for Bulirsch-Stoer and RKF45");
 
   DevBuf<double> L, r, psi, hpsi, guess, conv, part, sc, coef, Sx;
   L.alloc((size_t)n * maxvec);
@@ -501,6 +573,171 @@ void relax_run(Handle& H, const adc2g_relax_params& p, const double* guess_h, do
     return 0;
   };
 
+  // RKF45 state and vectors (y is psi; yp persists, as the reference's dtpsi)
+  RKState rks;
+  DevBuf<double> rk_yp, rk_f[5], rk_x, rk_w;
+  std::vector<double> rk_part(nblk);
+  if (rk_mode) {
+    rk_yp.alloc(n);
+    for (auto& b : rk_f) b.alloc(n);
+    rk_x.alloc(n);
+    rk_w.alloc(n);
+    HIP_CHECK(hipMemsetAsync(rk_yp.get(), 0, sizeof(double) * n, st));
+  }
+  auto rk_f_of = [&](const double* x, double* out) {  // out = -H x
+    sigma1(x, rk_w.get());
+    hipLaunchKernelGGL(rk_neg_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, rk_w.get(), out);
+    HIP_CHECK(hipGetLastError());
+  };
+  auto rk_stage = [&](int stage, double ch, double* out) {
+    hipLaunchKernelGGL(rk_stage_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, stage, ch, psi.get(), rk_yp.get(),
+                       rk_f[0].get(), rk_f[1].get(), rk_f[2].get(), rk_f[3].get(), rk_f[4].get(), out);
+    HIP_CHECK(hipGetLastError());
+  };
+  // r8_fehl: the solution s ends in rk_f[0] (the reference passes f1 as s)
+  auto rk_fehl = [&](double h) {
+    rk_stage(1, h / 4.0, rk_x.get());
+    rk_f_of(rk_x.get(), rk_f[0].get());
+    rk_stage(2, 3.0 * h / 32.0, rk_x.get());
+    rk_f_of(rk_x.get(), rk_f[1].get());
+    rk_stage(3, h / 2197.0, rk_x.get());
+    rk_f_of(rk_x.get(), rk_f[2].get());
+    rk_stage(4, h / 4104.0, rk_x.get());
+    rk_f_of(rk_x.get(), rk_f[3].get());
+    rk_stage(5, h / 20520.0, rk_x.get());  // the reference stores this point in f1, used once
+    rk_f_of(rk_x.get(), rk_f[4].get());
+    rk_stage(6, h / 7618050.0, rk_f[0].get());
+  };
+  // one r8_rkf45 call (flag 1 or 2 as rkf45_rlx uses it); returns the flag
+  auto rk_step = [&](double& t, double tout, double relerr, double abserr, int flag) -> int {
+    const double eps = DBL_EPSILON;
+    if (relerr < 0.0 || abserr < 0.0 || flag == 0 || flag > 8 || flag < -2) return 8;
+    int mflag = std::abs(flag);
+    if (mflag != 1) {
+      if (t == tout && rks.kflag != 3) return 8;
+      if (mflag == 2) {
+        if (rks.kflag == 3) {
+          flag = rks.flag_save;
+          mflag = std::abs(flag);
+        } else if (rks.init == 0) {
+          flag = rks.flag_save;
+        } else if (rks.kflag == 4) {
+          rks.nfe = 0;
+        } else if (rks.kflag == 5 && abserr == 0.0) {
+          return 5;
+        } else if (rks.kflag == 6 && relerr <= rks.relerr_save && abserr <= rks.abserr_save) {
+          return 6;
+        }
+      }
+    }
+    rks.flag_save = flag;
+    rks.kflag = 0;
+    rks.relerr_save = relerr;
+    rks.abserr_save = abserr;
+    if (relerr < 2.0 * eps + 1.0e-12) {
+      rks.kflag = 3;
+      return 3;
+    }
+    double dt = tout - t;
+    if (mflag == 1) {
+      rks.init = 0;
+      rks.kop = 0;
+      rk_f_of(psi.get(), rk_yp.get());
+      rks.nfe = 1;
+      if (t == tout) return 2;
+    }
+    if (rks.init == 0) {
+      // the initial step from the components, in order (host copies of y, yp)
+      rks.init = 1;
+      std::vector<double> hy(n), hyp(n);
+      HIP_CHECK(hipMemcpyAsync(hy.data(), psi.get(), sizeof(double) * n, hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipMemcpyAsync(hyp.data(), rk_yp.get(), sizeof(double) * n, hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipStreamSynchronize(st));
+      clock.collect();
+      double h = std::fabs(dt), toln = 0.0;
+      for (int64_t k = 0; k < n; ++k) {
+        const double tol = relerr * std::fabs(hy[k]) + abserr;
+        if (tol > 0.0) {
+          toln = tol;
+          const double ypk = std::fabs(hyp[k]);
+          if (tol < ypk * std::pow(h, 5)) h = std::pow(tol / ypk, 0.2);
+        }
+      }
+      if (toln <= 0.0) h = 0.0;
+      rks.h = std::max(h, 26.0 * eps * std::max(std::fabs(t), std::fabs(dt)));
+      rks.flag_save = flag >= 0 ? 2 : -2;
+    }
+    rks.h = dt >= 0.0 ? std::fabs(rks.h) : -std::fabs(rks.h);
+    if (2.0 * std::fabs(dt) <= std::fabs(rks.h)) ++rks.kop;
+    if (rks.kop == 10000) {
+      rks.kop = 0;
+      return 7;
+    }
+    if (std::fabs(dt) <= 26.0 * eps * std::fabs(t)) {
+      t = tout;
+      rk_stage(1, dt, rk_x.get());  // y + dt yp
+      HIP_CHECK(hipMemcpyAsync(psi.get(), rk_x.get(), sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+      rk_f_of(psi.get(), rk_yp.get());
+      ++rks.nfe;
+      return 2;
+    }
+    bool output = false;
+    const double scale = 2.0 / relerr, ae = scale * abserr;
+    for (;;) {
+      bool hfaild = false;
+      const double hmin = 26.0 * eps * std::fabs(t);
+      dt = tout - t;
+      if (!(2.0 * std::fabs(rks.h) <= std::fabs(dt))) {
+        if (std::fabs(dt) <= std::fabs(rks.h)) {
+          output = true;
+          rks.h = dt;
+        } else {
+          rks.h = 0.5 * dt;
+        }
+      }
+      double esttol = 0.0;
+      for (;;) {
+        if (rks.nfe > 30000) {
+          rks.kflag = 4;
+          return 4;
+        }
+        rk_fehl(rks.h);
+        rks.nfe += 5;
+        hipLaunchKernelGGL(rk_err_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, ae, psi.get(), rk_f[0].get(),
+                           rk_yp.get(), rk_f[1].get(), rk_f[2].get(), rk_f[3].get(), rk_f[4].get(), part.get());
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(rk_part.data(), part.get(), sizeof(double) * nblk, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        clock.collect();
+        double eeoet = 0.0;
+        for (double x : rk_part) eeoet = std::max(eeoet, x);
+        esttol = std::fabs(rks.h) * eeoet * scale / 752400.0;
+        if (esttol <= 1.0) break;
+        hfaild = true;
+        output = false;
+        const double sc = esttol < 59049.0 ? 0.9 / std::pow(esttol, 0.2) : 0.1;
+        rks.h = sc * rks.h;
+        if (std::fabs(rks.h) < hmin) {
+          rks.kflag = 6;
+          return 6;
+        }
+      }
+      t = t + rks.h;
+      HIP_CHECK(hipMemcpyAsync(psi.get(), rk_f[0].get(), sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+      rk_f_of(psi.get(), rk_yp.get());
+      ++rks.nfe;
+      double sc = 0.0001889568 < esttol ? 0.9 / std::pow(esttol, 0.2) : 5.0;
+      if (hfaild) sc = std::min(sc, 1.0);
+      const double hh = std::max(sc * std::fabs(rks.h), hmin);
+      rks.h = rks.h >= 0.0 ? hh : -hh;
+      if (output) {
+        t = tout;
+        return 2;
+      }
+      if (flag <= 0) return -2;
+    }
+  };
+
   int nconv = 0;
   int status = ADC2G_OK;
   std::string fail_msg;
@@ -551,7 +788,29 @@ void relax_run(Handle& H, const adc2g_relax_params& p, const double* guess_h, do
         }
       }
     }
-    for (int it = 1; !bs_mode && it <= p.maxiter; ++it) {
+    if (rk_mode) {
+      // rkf45_rlx (relaxation.f90:501-621): tout = i*step, relerr = abserr =
+      // siltol, flag 1 at the start of the state
+      double t = 0.0;
+      int flag = 1;
+      for (int it = 1; it <= p.maxiter; ++it) {
+        flag = rk_step(t, it * p.step, p.toler, p.toler, flag);
+        if (flag != 2)
+          throw Error(ADC2G_EBREAK, "Error in the RKF45 integrator. Error code: " + std::to_string(flag));
+        normalise(psi.get(), w_d + 2);
+        residual(psi.get(), E, res);
+        if (!std::isfinite(E) || !std::isfinite(res))
+          throw Error(ADC2G_EBREAK, "relaxation: non-finite energy at state " + std::to_string(s + 1));
+        if (p.verbose)
+          std::fprintf(stderr, "relax state %d step %5d  E = %.12f  residual = %.4e\n", s + 1, it, E, res);
+        if (res <= p.tol) {
+          converged = true;
+          it_done = it;
+          break;
+        }
+      }
+    }
+    for (int it = 1; !bs_mode && !rk_mode && it <= p.maxiter; ++it) {
       if (ncurr * kb == maxvec) {  // collapse_subspace
         for (int blk = 1; blk < maxvec / kb; ++blk)
           HIP_CHECK(hipMemcpyAsync(L.get() + (size_t)n * kb * (blk - 1), L.get() + (size_t)n * kb * blk,

@@ -16,7 +16,7 @@
 ! Further keys:
 !   solver = 1 Davidson (davdiag_block) | 2 relaxation (relaxation.f90:36-133)
 !            with stepsize, rlx_krydim and the same nstates / maxiter / tol;
-!            integrator = 1 XSIL (default) | 2 Bulirsch-Stoer with siltol;
+!            integrator = 1 XSIL (default) | 2 Bulirsch-Stoer | 3 RKF45, with siltol;
 !   guess = 0 unit vectors | 1 sub-diagonalisation (subdim ISs) for Davidson;
 !   autoord = 0..2: SIL autocorrelation orders, files auto_file, 'auto1', 'auto2';
 !   lanczos = .true.: block Lanczos (lancdiag_block) with lmain start vectors and

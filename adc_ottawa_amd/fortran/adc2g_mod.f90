@@ -94,9 +94,9 @@ module adc2g_mod
      real(c_double)     :: tol = 1.0d-7
      integer(c_int32_t) :: precision = 0
      integer(c_int32_t) :: verbose = 0
-     integer(c_int32_t) :: integrator = 1      ! 1 XSIL, 2 Bulirsch-Stoer
+     integer(c_int32_t) :: integrator = 1      ! 1 XSIL, 2 Bulirsch-Stoer, 3 RKF45
      integer(c_int32_t) :: reserved = 0
-     real(c_double)     :: toler = 1.0d-5      ! siltol: Bulirsch-Stoer error tolerance
+     real(c_double)     :: toler = 1.0d-5      ! siltol: error tolerance of integrators 2 and 3
   end type adc2g_relax_params
 
   type, bind(c) :: adc2g_relax_stats

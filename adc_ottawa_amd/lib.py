@@ -342,8 +342,9 @@ class Operator:
 
     def relax(self, nstates, guess=None, krydim=10, maxdim=-1, maxiter=100, step=10.0, tol=1e-7,
               precision="fp64", vectors=True, verbose=False, integrator="xsil", toler=1e-5):
-        """Imaginary-time relaxation of the nstates lowest states, with the XSIL
-        or ('bs') the Bulirsch-Stoer integrator (error tolerance toler).
+        """Imaginary-time relaxation of the nstates lowest states, with the XSIL,
+        the Bulirsch-Stoer ('bs') or the RKF45 ('rkf45') integrator (error
+        tolerance toler for the last two).
 
         guess: (ndim, nstates) or None (unit vectors on the smallest |H_jj|).
         Returns (energies, vectors or None, residuals, iterations, stats)."""
@@ -351,7 +352,7 @@ class Operator:
         if guess is not None:
             g = np.asfortranarray(np.asarray(guess, dtype=np.float64).reshape(self.ndim, nstates))
         p = RelaxParams(nstates, krydim, maxdim, maxiter, step, tol, PREC[precision], 1 if verbose else 0,
-                        {"xsil": 1, "bs": 2}[integrator], 0, toler)
+                        {"xsil": 1, "bs": 2, "rkf45": 3}[integrator], 0, toler)
         ev = np.empty(nstates)
         vec = np.empty((self.ndim, nstates), order="F") if vectors else None
         res = np.empty(nstates)

@@ -157,9 +157,10 @@ typedef struct {
   double tol;            /* davtol: convergence when |H psi - E psi| <= tol */
   int32_t precision;     /* ADC2G_PREC_* of the sigma-vectors */
   int32_t verbose;       /* 1: one line per timestep on stderr (wrtable_1vec) */
-  int32_t integrator;    /* integrator: 0 or 1 XSIL (default), 2 Bulirsch-Stoer (bs_rlx) */
+  int32_t integrator;    /* integrator: 0 or 1 XSIL (default), 2 Bulirsch-Stoer (bs_rlx),
+                            3 RKF45 (rkf45_rlx) */
   int32_t reserved;
-  double toler;          /* siltol: Bulirsch-Stoer error tolerance (integrator 2) */
+  double toler;          /* siltol: error tolerance of integrators 2 and 3 */
 } adc2g_relax_params;
 
 typedef struct {

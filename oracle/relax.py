@@ -409,3 +409,213 @@ def bs_relax(matvec, guess: np.ndarray, maxiter=100, step=10.0, eps=1e-7, toler=
         if not ok:
             break
     return {"eigval": E, "eigvec": V, "niter": niter, "residual": resid, "nfunc": bs.nfunc}
+
+
+# ---------------------------------------------------------------- RKF45
+class RKF45:
+    """r8_rkf45 with r8_fehl (source/propagation/rkf45rlxlib.f90:87-795, the
+    Watts-Shampine RKF45 driver) for dy/dt = f(y) = -H y.  The Fortran
+    ``save`` variables (abserr_save, h, flag_save, init, kflag, kop, nfe,
+    relerr_save) are members, so they persist from call to call."""
+
+    MAXNFE = 30000
+
+    def __init__(self):
+        self.abserr_save = -1.0
+        self.h = -1.0
+        self.flag_save = -1000
+        self.init = -1000
+        self.kflag = -1000
+        self.kop = -1
+        self.nfe = -1
+        self.relerr_save = -1.0
+        self.calls = 0
+
+    def _f(self, f, y):
+        self.calls += 1
+        return f(y)
+
+    def fehl(self, f, y, h, yp):
+        ch = h / 4.0
+        f1 = self._f(f, y + ch * yp)
+        ch = 3.0 * h / 32.0
+        f2 = self._f(f, y + ch * (yp + 3.0 * f1))
+        ch = h / 2197.0
+        f3 = self._f(f, y + ch * (1932.0 * yp + (7296.0 * f2 - 7200.0 * f1)))
+        ch = h / 4104.0
+        f4 = self._f(f, y + ch * ((8341.0 * yp - 845.0 * f3) + (29440.0 * f2 - 32832.0 * f1)))
+        ch = h / 20520.0
+        f1n = y + ch * ((-6080.0 * yp + (9295.0 * f3 - 5643.0 * f4)) + (41040.0 * f1 - 28352.0 * f2))
+        f5 = self._f(f, f1n)
+        ch = h / 7618050.0
+        s = y + ch * ((902880.0 * yp + (3855735.0 * f3 - 1371249.0 * f4)) + (3953664.0 * f2 + 277020.0 * f5))
+        return s, f2, f3, f4, f5
+
+    def step(self, f, y, yp, t, tout, relerr, abserr, flag):
+        """One call; returns (y, yp, t, flag).  y and yp are the caller's
+        arrays (yp is only recomputed where the reference recomputes it)."""
+        eps = np.finfo(np.float64).eps
+        if relerr < 0.0 or abserr < 0.0 or flag == 0 or flag > 8 or flag < -2:
+            return y, yp, t, 8
+        mflag = abs(flag)
+        if mflag != 1:
+            if t == tout and self.kflag != 3:
+                return y, yp, t, 8
+            if mflag == 2:
+                if self.kflag == 3:
+                    flag = self.flag_save
+                    mflag = abs(flag)
+                elif self.init == 0:
+                    flag = self.flag_save
+                elif self.kflag == 4:
+                    self.nfe = 0
+                elif self.kflag == 5 and abserr == 0.0:
+                    raise RuntimeError("R8_RKF45: KFLAG = 5 and ABSERR = 0")
+                elif self.kflag == 6 and relerr <= self.relerr_save and abserr <= self.abserr_save:
+                    raise RuntimeError("R8_RKF45: KFLAG = 6")
+            else:
+                if flag == 3:
+                    flag = self.flag_save
+                    if self.kflag == 3:
+                        mflag = abs(flag)
+                elif flag == 4:
+                    self.nfe = 0
+                    flag = self.flag_save
+                    if self.kflag == 3:
+                        mflag = abs(flag)
+                elif flag == 5 and abserr > 0.0:
+                    flag = self.flag_save
+                    if self.kflag == 3:
+                        mflag = abs(flag)
+                else:
+                    raise RuntimeError("R8_RKF45: integration cannot be continued")
+        self.flag_save = flag
+        self.kflag = 0
+        self.relerr_save = relerr
+        self.abserr_save = abserr
+        relerr_min = 2.0 * eps + 1.0e-12
+        if relerr < relerr_min:
+            self.kflag = 3
+            return y, yp, t, 3
+        dt = tout - t
+        if mflag == 1:
+            self.init = 0
+            self.kop = 0
+            yp = self._f(f, y)
+            self.nfe = 1
+            if t == tout:
+                return y, yp, t, 2
+        if self.init == 0:
+            self.init = 1
+            h = abs(dt)
+            toln = 0.0
+            for k in range(y.size):                      # sequential, as written
+                tol = relerr * abs(y[k]) + abserr
+                if tol > 0.0:
+                    toln = tol
+                    ypk = abs(yp[k])
+                    if tol < ypk * h ** 5:
+                        h = (tol / ypk) ** 0.2
+            if toln <= 0.0:
+                h = 0.0
+            self.h = max(h, 26.0 * eps * max(abs(t), abs(dt)))
+            self.flag_save = 2 if flag >= 0 else -2
+        self.h = abs(self.h) if dt >= 0 else -abs(self.h)
+        if 2.0 * abs(dt) <= abs(self.h):
+            self.kop += 1
+        if self.kop == 10000:
+            self.kop = 0
+            return y, yp, t, 7
+        if abs(dt) <= 26.0 * eps * abs(t):
+            y = y + dt * yp
+            yp = self._f(f, y)
+            self.nfe += 1
+            return y, yp, tout, 2
+        output = False
+        scale = 2.0 / relerr
+        ae = scale * abserr
+        while True:
+            hfaild = False
+            hmin = 26.0 * eps * abs(t)
+            dt = tout - t
+            if not 2.0 * abs(self.h) <= abs(dt):
+                if abs(dt) <= abs(self.h):
+                    output = True
+                    self.h = dt
+                else:
+                    self.h = 0.5 * dt
+            while True:
+                if self.nfe > self.MAXNFE:
+                    self.kflag = 4
+                    return y, yp, t, 4
+                s, f2, f3, f4, f5 = self.fehl(f, y, self.h, yp)
+                self.nfe += 5
+                et = np.abs(y) + np.abs(s) + ae
+                if np.any(et <= 0.0):
+                    return y, yp, t, 5
+                ee = np.abs((-2090.0 * yp + (21970.0 * f3 - 15048.0 * f4)) + (22528.0 * f2 - 27360.0 * f5))
+                eeoet = float(np.max(ee / et))
+                esttol = abs(self.h) * eeoet * scale / 752400.0
+                if esttol <= 1.0:
+                    break
+                hfaild = True
+                output = False
+                sc = 0.9 / esttol ** 0.2 if esttol < 59049.0 else 0.1
+                self.h = sc * self.h
+                if abs(self.h) < hmin:
+                    self.kflag = 6
+                    return y, yp, t, 6
+            t = t + self.h
+            y = s
+            yp = self._f(f, y)
+            self.nfe += 1
+            sc = 0.9 / esttol ** 0.2 if 0.0001889568 < esttol else 5.0
+            if hfaild:
+                sc = min(sc, 1.0)
+            hh = max(sc * abs(self.h), hmin)
+            self.h = hh if self.h >= 0 else -hh
+            if output:
+                return y, yp, tout, 2
+            if flag <= 0:
+                return y, yp, t, -2
+
+
+def rkf45_relax(matvec, guess: np.ndarray, maxiter=100, step=10.0, eps=1e-7, toler=1e-5, log=None):
+    """rkf45_rlx (source/propagation/relaxation.f90:501-621): each state from
+    its initial vector, r8_rkf45 called with tout = i*step (relerr = abserr =
+    toler), flag 1 at the start of each state; then the no-op orthogonalisation
+    loop, normalisation of y (the derivative yp is not recomputed, as in the
+    reference) and the residual test."""
+    guess = np.asarray(guess, dtype=np.float64)
+    n, ns = guess.shape
+    f = lambda x: -matvec(x)
+    rk = RKF45()
+    E = np.zeros(ns)
+    V = np.zeros((n, ns))
+    niter = np.zeros(ns, dtype=np.int64)
+    resid = np.zeros(ns)
+    yp = np.zeros(n)
+    for s in range(ns):
+        psi = guess[:, s].copy()
+        t = 0.0
+        flag = 1
+        ok = False
+        it_done = maxiter
+        for i in range(1, maxiter + 1):
+            psi, yp, t, flag = rk.step(f, psi, yp, t, i * step, toler, toler, flag)
+            if flag != 2:
+                raise RuntimeError(f"Error in the RKF45 integrator. Error code: {flag}")
+            psi = psi / np.sqrt(psi @ psi)
+            hpsi = matvec(psi)
+            e = psi @ hpsi
+            r = np.linalg.norm(hpsi - e * psi)
+            if log is not None:
+                log.append((s, i, e, r))
+            if r <= eps:
+                ok = True
+                it_done = i
+                break
+        E[s], V[:, s], niter[s], resid[s] = e, psi, it_done, r
+        if not ok:
+            break
+    return {"eigval": E, "eigvec": V, "niter": niter, "residual": resid, "nfunc": rk.calls}

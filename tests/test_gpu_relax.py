@@ -128,3 +128,28 @@ def test_relax_bulirsch_stoer_vs_oracle(form):
         v, w = V[:, s], ref["eigvec"][:, s]
         assert min(np.abs(v - w).max(), np.abs(v + w).max()) < 1e-5
     assert np.all(res <= 1e-6)
+
+
+@pytest.mark.parametrize("form", ("df", "int"))
+def test_relax_rkf45_vs_oracle(form):
+    """integrator 3 (rkf45_rlx with r8_rkf45): the oracle's number of
+    timesteps per state (within 1), energies to 1e-9 relative and vectors up
+    to sign; the case that exceeds 30000 derivative evaluations returns the
+    reference's error (code 4 in the message)."""
+    print(f"[case] relax rkf45 o=3 v=8 naux=10 coupling={form}", flush=True)
+    sys = make_system(3, 8, 10, seed=5)
+    H = elements.build_blocks(sys, apply_minc=False).dense()
+    g = relax.ondiag_guess(np.diag(H), 2)
+    ref = relax.rkf45_relax(lambda x: H @ x, g, maxiter=300, step=5.0, eps=1e-6, toler=1e-7)
+    with lib.Operator.from_system(sys, coupling=form) as op:
+        E, V, res, nit, st = op.relax(2, maxiter=300, step=5.0, tol=1e-6, integrator="rkf45", toler=1e-7)
+        with pytest.raises(lib.Adc2gError, match="Error code: 4"):
+            op.relax(2, maxiter=300, step=2.0, tol=1e-6, integrator="rkf45", toler=1e-8)
+    print(f"[rkf45] GPU niter {list(nit)} sigma {st['nsigma']}, oracle niter {list(ref['niter'])} "
+          f"calls {ref['nfunc']}", flush=True)
+    assert np.abs(nit - ref["niter"]).max() <= 1
+    assert np.abs(E - ref["eigval"]).max() < 1e-9 * np.abs(ref["eigval"]).max()
+    for s in range(2):
+        v, w = V[:, s], ref["eigvec"][:, s]
+        assert min(np.abs(v - w).max(), np.abs(v + w).max()) < 1e-5
+    assert np.all(res <= 1e-6)

@@ -542,3 +542,28 @@ def test_oracle_bulirsch_stoer_relaxation():
     for s in range(2):
         v = out["eigvec"][:, s]  # residual 1e-6 over a gap of 0.023
         assert min(np.abs(v - U[:, 0]).max(), np.abs(v + U[:, 0]).max()) < 1e-4
+
+
+def test_oracle_rkf45_relaxation():
+    """rkf45_rlx: one r8_rkf45 call to tout = 2 equals exp(-2H) psi from
+    scipy.linalg.expm to 1e-9 (tolerances 1e-10), the relaxation reaches the
+    lowest eigenpair for both states (the orthogonalisation loop is a no-op as
+    in bs_rlx), and a run that needs more than 30000 derivative evaluations
+    stops with the reference's error code 4."""
+    from scipy.linalg import expm
+    sys = make_system(3, 8, 10, seed=5)
+    H = elements.build_blocks(sys, apply_minc=False).dense()
+    lam, U = np.linalg.eigh(H)
+    g = relax.ondiag_guess(np.diag(H), 2)
+    rk = relax.RKF45()
+    y, _, t, flag = rk.step(lambda x: -(H @ x), g[:, 0].copy(), np.zeros(H.shape[0]), 0.0, 2.0, 1e-10, 1e-10, 1)
+    ex = expm(-2.0 * H) @ g[:, 0]
+    assert flag == 2 and t == 2.0
+    assert np.abs(y - ex).max() < 1e-9 * np.abs(ex).max()
+    out = relax.rkf45_relax(lambda x: H @ x, g, maxiter=300, step=5.0, eps=1e-6, toler=1e-7)
+    assert np.abs(out["eigval"] - lam[0]).max() < 1e-9 * abs(lam[0])
+    for s in range(2):
+        v = out["eigvec"][:, s]
+        assert min(np.abs(v - U[:, 0]).max(), np.abs(v + U[:, 0]).max()) < 1e-4
+    with pytest.raises(RuntimeError, match="Error code: 4"):
+        relax.rkf45_relax(lambda x: H @ x, g, maxiter=300, step=2.0, eps=1e-6, toler=1e-8)
