@@ -215,8 +215,19 @@ void pair_gemm(Handle& H, const double* A, const double* B, double* C, double be
 }  // namespace
 
 void fvector_run(Handle& H, const double* dpl_host, double* f_host) {
-  ARG_CHECK(!H.parent, "the F-vector belongs to the full operator, not to a CVS view");
   ARG_CHECK(dpl_host && f_host, "NULL argument");
+  if (H.parent) {
+    // CVS view: get_modifiedtm_adc2 on the CVS configurations (lcvs skips the
+    // I and II blocks, which the CVS space does not have; every other element
+    // is the same function of its configuration), i.e. the full-space vector
+    // at the view's configurations
+    std::vector<double> full(H.parent->d.ndim);
+    fvector_run(*H.parent, dpl_host, full.data());
+    std::vector<int64_t> vm(H.d.ndim);
+    HIP_CHECK(hipMemcpy(vm.data(), H.vmap.get(), sizeof(int64_t) * vm.size(), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < H.d.ndim; ++i) f_host[i] = full[vm[i]];
+    return;
+  }
   const int o = H.d.o, v = H.d.v, naux = H.d.naux, nbas = o + v;
   const int64_t n1 = H.d.n1, nn = n1 * n1, vv = (int64_t)v * v, oo = (int64_t)o * o;
   hipStream_t st = H.st;

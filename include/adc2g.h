@@ -414,9 +414,11 @@ int32_t adc2g_mp2(adc2g_handle* h, double* e_mp2, double* d2, double* rho);
  * (source/adclib/get_moment.f90:153-442) with the term functions of
  * source/adclib/dipole_ph.f90: 1h1p f = -sqrt(2) (dpl(a,k) + F0 + F25..F210 +
  * FA + FB + FC + F21..F24), 2h2p FI, FII, FIII, FIV1, FIV2 (closed shell,
- * C1 symmetry; the CVS branches of :372-373 are not taken).  Needs the
- * (vv|ov) block and eight n1 x n1 matrices in device memory.  Not available
- * on a CVS view. */
+ * C1 symmetry).  Needs the (vv|ov) block and eight n1 x n1 matrices in
+ * device memory.  On a CVS view (adc2g_create_cvs) f has the view's ndim and
+ * holds the elements of the CVS configurations, as get_modifiedtm_adc2 with
+ * lcvs computes them (:372-373 skip the I and II blocks, which the CVS space
+ * does not have). */
 int32_t adc2g_fvector(adc2g_handle* h, const double* dpl, double* f);
 
 /* ---- on-disk matrix ------------------------------------------------------ */

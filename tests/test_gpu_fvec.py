@@ -73,8 +73,22 @@ def test_fvector_bad_arguments():
     with lib.Operator.from_system(sys) as op:
         with pytest.raises(ValueError):
             op.fvector(np.eye(5))
-        with op.cvs([1]) as view:
-            with pytest.raises(lib.Adc2gError):
-                view.fvector(np.eye(11))
         f = op.fvector(np.eye(11))
         assert np.abs(f).max() < 1e-14
+
+
+@pytest.mark.parametrize("core", ([1], [1, 3]))
+def test_fvector_cvs_view_vs_oracle(core):
+    """On a CVS view the F-vector holds the CVS configurations' elements
+    (get_modifiedtm_adc2 with lcvs, whose space has no I/II blocks): the
+    oracle's full vector at oracle/csf.py's CVS indices, to 1e-12."""
+    from oracle import csf
+    print(f"[case] fvector cvs core={core} o=3 v=8 naux=10", flush=True)
+    sys = make_system(3, 8, 10, seed=5)
+    d = _dpl(11, 9)
+    ref = fvec.fvector(sys, d)[csf.cvs_indices(3, 8, [c - 1 for c in core])]
+    with lib.Operator.from_system(sys) as op:
+        with op.cvs(core) as view:
+            got = view.fvector(d)
+    assert got.shape == ref.shape
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
