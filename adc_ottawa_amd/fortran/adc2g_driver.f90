@@ -27,6 +27,9 @@
 !   chebyord > 0: Chebyshev autocorrelation of fvec with the bounds
 !            cheby_emin / cheby_emax, file 'chebyauto';
 !   hmlt = .true.: the matrix files 'hmlt.diai' / 'hmlt.offi';
+!   ncore = n, core = i1, ..., in: CVS on these 1-based core orbitals (lcvs);
+!            every later step (solvers, F-vector, propagation) uses the CVS
+!            configurations;
 !   dpl_file = 'd.bin': the F-vector of this nbas x nbas column-major operator
 !            matrix (get_modifiedtm_adc2), computed by the library, gives the
 !            transition moments and oscillator strengths of the states
@@ -44,6 +47,7 @@ program adc2g_driver
   integer :: precon = 2, precision = 0, krydim = 10
   integer :: solver = 1, guess = 0, subdim = 800, rlx_krydim = 10, autoord = 0
   integer :: lmain = 0, ncycles = 0, chebyord = 0, lancguess = 1, orthotype = 0, integrator = 1
+  integer :: ncore = 0, core(16) = 0
   logical :: deflate = .true., sil = .false., lanczos = .false., hmlt = .false.
   real(c_double) :: tol = 1.0d-7, tout = 0.1d0, tfinal = 1.0d0, autotol = 1.0d-5
   real(c_double) :: stepsize = 10.0d0, cheby_emin = 0.0d0, cheby_emax = 1.0d0, siltol = 1.0d-5
@@ -53,9 +57,10 @@ program adc2g_driver
   namelist /adc2g/ nocc, nvirt, naux, eps_file, b_file, nstates, blocksize, maxvec, maxiter, tol, &
        precon, precision, deflate, davname, sil, fvec_file, auto_file, tout, tfinal, krydim, autotol, &
        solver, guess, subdim, stepsize, rlx_krydim, autoord, lanczos, lmain, ncycles, lancname, &
-       chebyord, cheby_emin, cheby_emax, hmlt, dpl_file, lancguess, orthotype, integrator, siltol
+       chebyord, cheby_emin, cheby_emax, hmlt, dpl_file, lancguess, orthotype, integrator, siltol, &
+       ncore, core
 
-  type(c_ptr) :: h
+  type(c_ptr) :: h, hfull
   type(adc2g_opts) :: opts
   type(adc2g_dav_params) :: dp
   type(adc2g_dav_stats) :: ds
@@ -88,6 +93,13 @@ program adc2g_driver
   rc = adc2g_create(int(nocc, c_int32_t), int(nvirt, c_int32_t), int(naux, c_int32_t), eps, B, opts, h)
   call check(rc)
   deallocate(B)
+  hfull = c_null_ptr
+  if (ncore > 0) then
+     ! CVS (lcvs): every later step works on the core-hole configurations
+     hfull = h
+     call check(adc2g_create_cvs(hfull, int(ncore, c_int32_t), int(core(1:ncore), c_int32_t), h))
+     write(*, '(a,16i4)') ' CVS core orbitals:', core(1:ncore)
+  end if
   rc = adc2g_dims(h, ndim, n1, counts)
   call check(rc)
   write(*, '(a,i12,a,i8,a,5i10)') ' ADC(2)-s space: ndim =', ndim, '  1h1p =', n1, '  2h2p types =', counts
@@ -220,6 +232,7 @@ program adc2g_driver
   end if
 
   rc = adc2g_destroy(h)
+  if (c_associated(hfull)) rc = adc2g_destroy(hfull)
 
 contains
 

@@ -220,3 +220,41 @@ def test_fortran_driver_lanczos_double_subspace(tmp_path):
     ev = np.array([x[1] for x in recs])
     assert ev.size == 12
     assert np.abs(ev - ref["eigval"]).max() < 1e-9 * np.abs(ref["eigval"]).max()
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built")
+def test_fortran_driver_cvs(tmp_path):
+    """ncore/core: CVS on core orbital 1.  The Davidson eigenvalues equal the
+    lowest eigenvalues of the CVS sub-matrix, and the printed transition
+    moments and oscillator strengths come from the CVS F-vector (the
+    oracle's F-vector at the CVS configurations)."""
+    from oracle import csf
+    print("[case] fortran driver cvs core=1 o=3 v=7 naux=9", flush=True)
+    o, v, naux = 3, 7, 9
+    sys_ = make_system(o, v, naux, seed=11)
+    sys_.eps.astype("<f8").tofile(tmp_path / "eps.bin")
+    sys_.B.astype("<f8").tofile(tmp_path / "B.bin")
+    d = np.random.default_rng(4).normal(0.0, 0.3, (o + v, o + v))
+    d = d + d.T
+    np.asfortranarray(d).T.astype("<f8").tofile(tmp_path / "d.bin")
+    nml = (f"&adc2g nocc={o} nvirt={v} naux={naux} eps_file='eps.bin' b_file='B.bin' nstates=2 "
+           f"blocksize=4 maxvec=20 tol=1d-10 precon=1 davname='davstates' dpl_file='d.bin' "
+           f"ncore=1 core=1 /\n")
+    r = subprocess.run([DRIVER], input=nml, capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    idx = csf.cvs_indices(o, v, [0])
+    df = dfsigma.DF(sys_)
+    Hf = df.sigma(np.eye(df.n1 + df.D2.size))
+    Hc = Hf[np.ix_(idx, idx)]
+    w, U = np.linalg.eigh(Hc)
+    f = fvec.fvector(sys_, d)[idx]
+    recs = read_davstates(tmp_path / "davstates", idx.size)
+    ev = np.array([x[1] for x in recs])
+    assert np.abs(ev - w[:2]).max() < 1e-10
+    rows = [ln.split() for ln in r.stdout.splitlines() if len(ln.split()) == 5 and ln.split()[0] in ("1", "2")]
+    assert len(rows) == 2
+    for i, row in enumerate(rows):
+        tm = U[:, i] @ f
+        assert abs(abs(float(row[3])) - abs(tm)) < 1e-7 * np.linalg.norm(f)
+        assert abs(float(row[4]) - 2.0 / 3.0 * w[i] * tm ** 2) < 1e-7 * np.dot(f, f) * max(abs(w[i]), 1.0)
