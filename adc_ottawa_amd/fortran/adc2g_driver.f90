@@ -27,6 +27,8 @@
 !   chebyord > 0: Chebyshev autocorrelation of fvec with the bounds
 !            cheby_emin / cheby_emax, file 'chebyauto';
 !   hmlt = .true.: the matrix files 'hmlt.diai' / 'hmlt.offi';
+!   ehf: the HF energy added to the state energies of 'davstates.dat' (the
+!            table2 file, written for the full space);
 !   ncore = n, core = i1, ..., in: CVS on these 1-based core orbitals (lcvs);
 !            every later step (solvers, F-vector, propagation) uses the CVS
 !            configurations;
@@ -50,7 +52,7 @@ program adc2g_driver
   integer :: ncore = 0, core(16) = 0
   logical :: deflate = .true., sil = .false., lanczos = .false., hmlt = .false.
   real(c_double) :: tol = 1.0d-7, tout = 0.1d0, tfinal = 1.0d0, autotol = 1.0d-5
-  real(c_double) :: stepsize = 10.0d0, cheby_emin = 0.0d0, cheby_emax = 1.0d0, siltol = 1.0d-5
+  real(c_double) :: stepsize = 10.0d0, cheby_emin = 0.0d0, cheby_emax = 1.0d0, siltol = 1.0d-5, ehf = 0.0d0
   character(len=256) :: eps_file = 'eps.bin', b_file = 'B.bin', davname = 'davstates'
   character(len=256) :: fvec_file = 'fvec.bin', auto_file = 'auto', lancname = 'lancstates'
   character(len=256) :: dpl_file = ''
@@ -58,7 +60,7 @@ program adc2g_driver
        precon, precision, deflate, davname, sil, fvec_file, auto_file, tout, tfinal, krydim, autotol, &
        solver, guess, subdim, stepsize, rlx_krydim, autoord, lanczos, lmain, ncycles, lancname, &
        chebyord, cheby_emin, cheby_emax, hmlt, dpl_file, lancguess, orthotype, integrator, siltol, &
-       ncore, core
+       ncore, core, ehf
 
   type(c_ptr) :: h, hfull
   type(adc2g_opts) :: opts
@@ -70,7 +72,8 @@ program adc2g_driver
   type(adc2g_relax_stats) :: rs
   type(adc2g_lanczos_params) :: lp
   type(adc2g_lanczos_stats) :: ls
-  real(c_double), allocatable :: eps(:), B(:), eigval(:), fvec(:), q0(:,:), cauto(:), dpl(:,:)
+  real(c_double), allocatable :: eps(:), B(:), eigval(:), fvec(:), q0(:,:), cauto(:), dpl(:,:), tmv(:), osc(:)
+  real(c_double) :: emp2 = 0.0d0, d2mp = 0.0d0
   real(c_double), allocatable, target :: eigvec(:,:), lener(:), ritz(:,:), tmom(:), fvec_t(:)
   complex(c_double_complex), allocatable :: auto0(:)
   complex(c_double_complex), allocatable, target :: auto1(:), auto2(:)
@@ -145,6 +148,11 @@ program adc2g_driver
      close(u)
      call check(adc2g_fvector(h, dpl, fvec))
      write(*, '(a,es22.14)') ' F-vector: |f| =', sqrt(sum(fvec**2))
+     allocate(tmv(nstates), osc(nstates))
+     do i = 1, nstates
+        tmv(i) = dot_product(eigvec(:, i), fvec)
+        osc(i) = 2.0d0 / 3.0d0 * eigval(i) * tmv(i)**2
+     end do
      write(*, '(a)') ' state        E [Eh]              E [eV]        transition moment    osc. strength'
      do i = 1, nstates
         write(*, '(i4,2x,f18.12,2x,f14.7,2x,es20.12,2x,es20.12)') i, eigval(i), eigval(i) * adc2g_eh2ev, &
@@ -156,6 +164,11 @@ program adc2g_driver
      end do
   end if
   call adc2g_write_davstates(trim(davname), ndim, nstates, eigval, eigvec)
+  if (ncore == 0) then
+     ! table2 / wrstateinfo (misc.f90:648-852): 'davstates.dat'
+     call check(adc2g_mp2(h, emp2, d2mp, c_null_ptr))
+     call write_state_table('davstates.dat')
+  end if
 
   if (hmlt) then
      rc = adc2g_write_hmlt(h, 'hmlt.diai'//c_null_char, 'hmlt.offi'//c_null_char, 1.0d-12, 0_c_int32_t, &
@@ -235,6 +248,126 @@ program adc2g_driver
   if (c_associated(hfull)) rc = adc2g_destroy(hfull)
 
 contains
+
+  ! (lo, hi), 1-based, of pair p (0-based) of the ordered pairs lo < hi <= n
+  subroutine pair_of(p, n, lo, hi)
+    integer(c_int64_t), intent(in) :: p
+    integer, intent(in) :: n
+    integer, intent(out) :: lo, hi
+    integer(c_int64_t) :: base
+    lo = 1
+    base = 0
+    do while (base + (n - lo) <= p)
+       base = base + (n - lo)
+       lo = lo + 1
+    end do
+    hi = lo + int(p - base) + 1
+  end subroutine pair_of
+
+  ! kpq columns 3..6 (k, l, a, b as MO indices; l = -1 for 1h1p) of
+  ! configuration idx (1-based) in the order of select_atom_is/select_atom_d
+  subroutine config_of(idx, k, l, a, b)
+    integer(c_int64_t), intent(in) :: idx
+    integer, intent(out) :: k, l, a, b
+    integer(c_int64_t) :: u, npv, off(6)
+    integer :: t
+    npv = int(nvirt, c_int64_t) * (nvirt - 1) / 2
+    off(1) = n1
+    do t = 1, 5
+       off(t + 1) = off(t) + counts(t)
+    end do
+    if (idx <= n1) then
+       u = idx - 1
+       a = nocc + int(u / nocc) + 1
+       k = int(mod(u, int(nocc, c_int64_t))) + 1
+       l = -1
+       b = -1
+    else if (idx <= off(2)) then          ! t1: i outer, a inner
+       u = idx - off(1) - 1
+       k = int(u / nvirt) + 1
+       l = k
+       a = nocc + int(mod(u, int(nvirt, c_int64_t))) + 1
+       b = a
+    else if (idx <= off(3)) then          ! t2: (a<b) outer, i inner
+       u = idx - off(2) - 1
+       k = int(mod(u, int(nocc, c_int64_t))) + 1
+       l = k
+       call pair_of(u / nocc, nvirt, a, b)
+       a = nocc + a
+       b = nocc + b
+    else if (idx <= off(4)) then          ! t3: (i<j) outer, a inner
+       u = idx - off(3) - 1
+       a = nocc + int(mod(u, int(nvirt, c_int64_t))) + 1
+       b = a
+       call pair_of(u / nvirt, nocc, k, l)
+    else                                  ! t4, t5: (i<j) outer, (a<b) inner
+       u = idx - merge(off(5), off(4), idx > off(5)) - 1
+       call pair_of(u / npv, nocc, k, l)
+       call pair_of(mod(u, npv), nvirt, a, b)
+       a = nocc + a
+       b = nocc + b
+    end if
+  end subroutine config_of
+
+  ! table2 with wrstateinfo (source/utilities/misc.f90:648-852): per state the
+  ! energy, excitation energy, transition moment and oscillator strength (when
+  ! an operator matrix was given) and the configurations among the 50 largest
+  ! |C| with |C| >= 0.05, in the reference's formats
+  subroutine write_state_table(fname)
+    character(len=*), intent(in) :: fname
+    integer :: st, ks, io, kk, ll, aa, bb
+    integer(c_int64_t) :: j, lim
+    integer(c_int64_t), allocatable :: order(:)
+    real(c_double), allocatable :: c2(:)
+    character(len=120) :: fmat
+    character(len=2) :: spin
+    lim = n1 + counts(1) + counts(2) + counts(3) + counts(4)
+    open(newunit=io, file=fname, form='formatted', status='replace')
+    write(io, '(/,2x,a,2x,F14.8)') 'Ground state MP2 energy:', ehf + emp2
+    allocate(order(ndim), c2(ndim))
+    do st = 1, nstates
+       c2 = eigvec(:, st)**2
+       ! the indices of the 50 largest C^2, descending
+       do ks = 1, int(min(50_c_int64_t, ndim))
+          order(ks) = maxloc(c2, dim=1, kind=c_int64_t)
+          c2(order(ks)) = -1.0d0
+       end do
+       if (st < 10) then
+          fmat = '(2/,2x,a,3x,i1,a1,i1,2x,a,5x,F14.8)'
+       else
+          fmat = '(2/,2x,a,2x,i2,a1,i1,2x,a,5x,F14.8)'
+       end if
+       write(io, fmat) 'State', st, '.', 1, 'Energy:', eigval(st) + ehf + emp2
+       if (eigval(st) * adc2g_eh2ev < 10.0d0) then
+          write(io, '(2x,a,2x,F10.5)') 'Excitation Energy (eV):', eigval(st) * adc2g_eh2ev
+       else
+          write(io, '(2x,a,3x,F10.5)') 'Excitation Energy (eV):', eigval(st) * adc2g_eh2ev
+       end if
+       if (allocated(tmv)) then
+          write(io, '(2x,a,F10.5)') 'Transition Dipole Moment:', tmv(st)
+          write(io, '(2x,a,5x,F10.5)') 'Oscillator Strength:', osc(st)
+       end if
+       write(io, '(/,2x,a,/)') 'Dominant Configurations:'
+       write(io, '(2x,29a)') ('*', ks = 1, 29)
+       write(io, '(3x,a)') 'j   k -> a  b        C_jkab'
+       write(io, '(2x,29a)') ('*', ks = 1, 29)
+       do ks = 1, int(min(50_c_int64_t, ndim))
+          j = order(ks)
+          if (abs(eigvec(j, st)) < 0.05d0) cycle
+          call config_of(j, kk, ll, aa, bb)
+          if (ll == -1) then
+             write(io, '(3x,i2,4x,a2,1x,i2,9x,F8.5)') kk, '->', aa, eigvec(j, st)
+          else if (kk /= ll .and. aa /= bb) then
+             spin = merge('I ', 'II', j <= lim)
+             write(io, '(3x,2(i2,1x),a2,2(1x,i2),2x,a2,2x,F8.5)') kk, ll, '->', aa, bb, spin, eigvec(j, st)
+          else
+             write(io, '(3x,2(i2,1x),a2,2(1x,i2),6x,F8.5)') kk, ll, '->', aa, bb, eigvec(j, st)
+          end if
+       end do
+       write(io, '(2x,29a)') ('*', ks = 1, 29)
+    end do
+    close(io)
+  end subroutine write_state_table
 
   subroutine check(code)
     integer(c_int32_t), intent(in) :: code

@@ -258,3 +258,68 @@ def test_fortran_driver_cvs(tmp_path):
         tm = U[:, i] @ f
         assert abs(abs(float(row[3])) - abs(tm)) < 1e-7 * np.linalg.norm(f)
         assert abs(float(row[4]) - 2.0 / 3.0 * w[i] * tm ** 2) < 1e-7 * np.dot(f, f) * max(abs(w[i]), 1.0)
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built")
+def test_fortran_driver_state_table(tmp_path):
+    """'davstates.dat' as table2/wrstateinfo write it: the MP2 ground-state
+    line, per state the total and excitation energies, the transition moment
+    and oscillator strength, and the dominant configurations (|C| >= 0.05 among
+    the 50 largest) whose printed MO indices name configurations with exactly
+    the printed coefficients (oracle/csf.py order, dense eigenvectors)."""
+    from oracle import csf, mp2
+    print("[case] fortran driver state table o=3 v=7 naux=9", flush=True)
+    o, v, naux = 3, 7, 9
+    sys_ = make_system(o, v, naux, seed=11)
+    sys_.eps.astype("<f8").tofile(tmp_path / "eps.bin")
+    sys_.B.astype("<f8").tofile(tmp_path / "B.bin")
+    d = np.random.default_rng(4).normal(0.0, 0.3, (o + v, o + v))
+    d = d + d.T
+    np.asfortranarray(d).T.astype("<f8").tofile(tmp_path / "d.bin")
+    nml = (f"&adc2g nocc={o} nvirt={v} naux={naux} eps_file='eps.bin' b_file='B.bin' nstates=3 "
+           f"blocksize=6 maxvec=30 tol=1d-10 precon=1 davname='davstates' dpl_file='d.bin' ehf=-1.5d0 /\n")
+    r = subprocess.run([DRIVER], input=nml, capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    text = open(tmp_path / "davstates.dat").read()
+    print(text[:1500])
+    df = dfsigma.DF(sys_)
+    H = df.sigma(np.eye(df.n1 + df.D2.size))
+    w, U = np.linalg.eigh(H)
+    e_mp2 = mp2.energy(sys_)
+    f = fvec.fvector(sys_, d)
+    lines = text.splitlines()
+    gs = [ln for ln in lines if "Ground state MP2 energy:" in ln]
+    assert len(gs) == 1 and abs(float(gs[0].split()[-1]) - (-1.5 + e_mp2)) < 1e-7
+    typ, a, b, k, l = csf.table(o, v)
+    key = {}
+    for j in range(typ.size):
+        if typ[j] == 0:
+            key[(k[j] + 1, -1, a[j] + o + 1, -1)] = j
+        else:
+            key[(k[j] + 1, l[j] + 1, a[j] + o + 1, b[j] + o + 1, "II" if typ[j] == 5 else "I")] = j
+    states = text.split("State")[1:]
+    assert len(states) == 3
+    for i, blk in enumerate(states):
+        bl = blk.splitlines()
+        assert abs(float(bl[0].split()[-1]) - (w[i] - 1.5 + e_mp2)) < 1e-7
+        ex = [x for x in bl if "Excitation Energy (eV):" in x][0]
+        assert abs(float(ex.split()[-1]) - w[i] * 27.2113845) < 1e-4
+        tm = U[:, i] @ f
+        tdm = [x for x in bl if "Transition Dipole Moment:" in x][0]
+        assert abs(abs(float(tdm.split(":")[1])) - abs(tm)) < 1e-5
+        rows = [x.split() for x in bl if "->" in x and "C_jkab" not in x]
+        assert rows, blk
+        big = np.sort(np.abs(U[:, i])[np.abs(U[:, i]) >= 0.05])[::-1][:50]
+        assert len(rows) == big.size
+        sgn = None
+        for row in rows:
+            if len(row) == 4:                      # k -> a C
+                j = key[(int(row[0]), -1, int(row[2]), -1)]
+            elif len(row) == 7:                    # k l -> a b spin C
+                j = key[(int(row[0]), int(row[1]), int(row[3]), int(row[4]), row[5])]
+            else:                                  # k l -> a b C
+                kk, ll, aa, bb = int(row[0]), int(row[1]), int(row[3]), int(row[4])
+                j = key[(kk, ll, aa, bb, "I")]
+            c = float(row[-1])
+            sgn = sgn if sgn is not None else np.sign(c * U[j, i])
+            assert abs(c - sgn * U[j, i]) < 2e-5
