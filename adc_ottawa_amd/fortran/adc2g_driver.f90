@@ -84,6 +84,10 @@ program adc2g_driver
   real(c_double) :: bounds(2)
 
   read(*, nml=adc2g)
+  if (ncore < 0 .or. ncore > size(core)) then
+     write(*, '(a,i0,a,i0)') ' adc2g input error: ncore must be between 0 and ', size(core), ', got ', ncore
+     stop 1
+  end if
   nbas = nocc + nvirt
   allocate(eps(nbas), B(int(naux, c_int64_t) * nbas * nbas))
   open(newunit=u, file=trim(eps_file), access='stream', form='unformatted', status='old')

@@ -61,3 +61,16 @@ def test_error_codes_match_reference_sil_codes():
     for name, code in [("ADC2G_ESIL_ORDER", 1), ("ADC2G_ESIL_UNDERFLOW", 2), ("ADC2G_ESIL_DIAG", 3),
                        ("ADC2G_ESIL_RESTART", 4)]:
         assert re.search(rf"#define {name} {code}\b", src)
+
+
+DRIVER = os.path.join(ROOT, "adc_ottawa_amd", "adc2g_driver")
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built")
+def test_fortran_driver_rejects_ncore_beyond_core_array(tmp_path):
+    # the namelist's core(16) holds at most 16 orbitals: a larger ncore stops
+    # with a message before any file is read or any library call is made
+    inp = "&adc2g nocc=20 nvirt=40 naux=10 ncore=17 /\n"
+    r = subprocess.run([DRIVER], input=inp, capture_output=True, text=True, cwd=tmp_path, timeout=60)
+    assert r.returncode != 0
+    assert "ncore must be between 0 and 16" in r.stdout
