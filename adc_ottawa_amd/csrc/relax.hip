@@ -156,10 +156,12 @@ struct SigmaClock {
 #define RL_LOOP(i) \
   for (int64_t i = blockIdx.x * (int64_t)RL_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RL_THREADS)
 
-// ---- Bulirsch-Stoer (bslib.f90: BSStep, ModMidpoint, PolyExtrapol, AbsBSError)
-// for the relaxation right-hand side f(psi) = -H psi.  The reference works in
-// complex arithmetic, but with a real start vector and f real every
-// imaginary part stays exactly zero, so the vectors here are real.
+// ---- Extrapolation integrator, device side: the modified midpoint rule and
+// the polynomial extrapolation tableau (integrator 2 of the reference,
+// source/propagation/bslib.f90) for the relaxation right-hand side
+// f(psi) = -H psi.  The reference works in complex arithmetic, but with a
+// real start vector and f real every imaginary part stays exactly zero, so
+// the vectors here are real.
 
 // psi1 = saved, psi2 = saved + h dt
 __global__ void __launch_bounds__(RL_THREADS) bs_mm_start_kernel(int64_t n, const double* __restrict__ saved,
@@ -194,8 +196,10 @@ struct BSFactors {
   double f[BS_MAXORDER + 1], g[BS_MAXORDER + 1];
 };
 
-// PolyExtrapol for one column (tableau columns 1..IntOrder-1 at T + (c-1)*n)
-// fused with AbsBSError: part[blk] = max |DeltaPsi| over the block
+// Extrapolation of column `column` against the earlier ones (Neville-type
+// recursion in h^2 with the factors f[q], g[q]; tableau columns 1..K-1 at
+// T + (c-1)*n), fused with the error norm: part[blk] = max |dpsi| over the
+// block, dpsi the last correction
 __global__ void __launch_bounds__(RL_THREADS) bs_extrapol_kernel(int64_t n, int column, int int_order, BSFactors fac,
                                                                  double* __restrict__ psi, double* __restrict__ T,
                                                                  double* __restrict__ part) {
@@ -233,8 +237,11 @@ __global__ void __launch_bounds__(RL_THREADS) bs_extrapol_kernel(int64_t n, int 
   }
 }
 
-// ---- RKF45 (rkf45rlxlib.f90: r8_fehl, r8_rkf45) for f(y) = -H y; the
-// stage combinations keep the reference's grouping (no contraction)
+// ---- RKF45, device side (integrator 3 of the reference,
+// source/propagation/rkf45rlxlib.f90) for f(y) = -H y: the Fehlberg stage
+// combinations with the published integer coefficients, grouped as the
+// reference groups them and with contraction off, so the rounding (and with
+// it every step decision) is the reference's
 
 // f = -w
 __global__ void __launch_bounds__(RL_THREADS) rk_neg_kernel(int64_t n, const double* __restrict__ w,
@@ -296,26 +303,206 @@ __global__ void __launch_bounds__(RL_THREADS) rk_err_kernel(int64_t n, double ae
   }
 }
 
-// r8_rkf45's save variables
-struct RKState {
-  double abserr_save = -1.0, h = -1.0, relerr_save = -1.0;
-  int flag_save = -1000, init = -1000, kflag = -1000, kop = -1, nfe = -1;
-};
-
 #undef RL_LOOP
 
-// BSStep with its Fortran save variables as members (they persist from call
-// to call, as in the reference).
-struct BSEngine {
-  int optimal_column = 0;
-  int nsteps[BS_MAXORDER + 2] = {};
-  double int_accuracy = 0.0, new_time = 0.0;
-  double expense[BS_MAXORDER + 2] = {};
-  double correction[BS_MAXORDER + 1][BS_MAXORDER + 1] = {};
-  double old_accuracy = -1.0;
-  int old_int_order = -1;
-  bool first_call = true;
-  double old_sq[BS_MAXORDER + 1] = {};
+// ---- Extrapolation integrator: step-size and order control
+//
+// Gragg-Bulirsch-Stoer extrapolation with Deuflhard's order and step-size
+// control (P. Deuflhard, Numer. Math. 41 (1983) 399-422; E. Hairer,
+// S. P. Norsett, G. Wanner, Solving Ordinary Differential Equations I,
+// 2nd ed., Sec. II.9), for the step-number sequence n_j = 2j, polynomial
+// extrapolation in h^2 and the absolute max-norm error of the last
+// extrapolation correction.  The tolerances, safety factors and decision
+// rules are the ones BSStep (source/propagation/bslib.f90:118-366) uses, so
+// the steps taken are the reference's:
+//  * work of column j: A_1 = n_1 + 1, A_j = A_{j-1} + n_j; convergence
+//    factors alpha(i, j) = e^((A_{i+1} - A_{j+1}) / ((2i + 1)(A_{j+1} - A_1 + 1)))
+//    with e = tol / 4;
+//  * whenever tol or the highest column K changed since the last step, K is
+//    lowered to the first j >= 2 with A_{j+1} > A_j alpha(j-1, j) (one more
+//    column no longer pays for itself); the table is rebuilt with the new K
+//    at the next step, so K settles over the first steps;
+//  * every step opens the whole order window (BSStep compares the requested
+//    step and time with values it resets at entry, so its restart test always
+//    fires); with err_j = (|dy_j|_max / e)^(1/(2j+1)) after column j + 1,
+//    the step is accepted once |dy_j|_max < tol; otherwise it is shrunk at
+//    column K by 0.7 / err_{K-1}, and at a column j + 1 < K when
+//    alpha(j, K-1) < err_j by 0.7 alpha(j, K-1) / err_j, each factor
+//    clamped to [1e-5, 0.5], and the step is redone from column 1;
+//  * after a step accepted at column k, the next step is h / s, where
+//    s = max(err_q, 0.1) for the q < k of least work A_{q+1} s (first one on
+//    ties); if that q is k - 1, k < K and the step was not shrunk, one column
+//    more is tried: s' = max(s / alpha(k-1, k), 0.1), taken when
+//    A_{k+1} s' does not exceed that least work.
+class ExtrapolationControl {
+ public:
+  static constexpr int kMaxColumns = 16;
+  static constexpr double kMinStepRel = 1e-10;   // smallest step, relative to the interval
+  static constexpr double kTolFraction = 0.25;   // e = tol / 4
+  static constexpr double kScaleFloor = 0.1;     // lower bound of err_j in the work estimate
+  static constexpr double kOrderSafety = 0.7;
+  static constexpr double kShrinkMax = 0.5, kShrinkMin = 1e-5;
+
+  enum class Verdict { kAccept, kShrink, kNextColumn };
+
+  // Rebuild the tables when tol or K changed; returns the (lowered) K.
+  int configure(double tol, int K) {
+    if (tol == tol_ && K == K_) return K;
+    tol_ = tol;
+    K_ = K;
+    e_ = kTolFraction * tol;
+    for (int j = 1; j <= K + 1; ++j) n_[j] = 2 * j;
+    A_[1] = n_[1] + 1;
+    for (int j = 2; j <= K + 1; ++j) A_[j] = A_[j - 1] + n_[j];
+    for (int j = 2; j <= K; ++j)
+      for (int i = 1; i < j; ++i)
+        alpha_[i][j] = std::pow(e_, (A_[i + 1] - A_[j + 1]) / ((2 * i + 1) * (A_[j + 1] - A_[1] + 1.0)));
+    int k = 2;
+    for (; k <= K - 1; ++k)
+      if (A_[k + 1] > A_[k] * alpha_[k - 1][k]) break;
+    return k;
+  }
+
+  int substeps(int j) const { return n_[j]; }
+  double scaled_error(int j, double max_dy) const { return std::pow(max_dy / e_, 1.0 / (2 * j + 1.0)); }
+
+  // Convergence monitor after column k >= 2 (err_{k-1} in err[k-1]).
+  Verdict judge(int k, int K, double max_dy, double tol, const double* err, double* shrink) const {
+    if (max_dy < tol) return Verdict::kAccept;
+    const int j = k - 1;
+    if (k == K) {
+      *shrink = kOrderSafety / err[j];
+      return Verdict::kShrink;
+    }
+    if (alpha_[j][K - 1] < err[j]) {
+      *shrink = kOrderSafety * alpha_[j][K - 1] / err[j];
+      return Verdict::kShrink;
+    }
+    return Verdict::kNextColumn;
+  }
+
+  static double clamp_shrink(double s) { return std::max(std::min(s, kShrinkMax), kShrinkMin); }
+
+  // Step proposed for the next call after acceptance at column k.
+  double next_step(double h, int k, int K, bool shrunk, const double* err) const {
+    double s = 0.0, least = 1.0e35;
+    int best = 0;
+    for (int q = 1; q <= k - 1; ++q) {
+      const double sq = std::max(err[q], kScaleFloor);
+      const double w = sq * A_[q + 1];
+      if (w < least) {
+        s = sq;
+        least = w;
+        best = q;
+      }
+    }
+    double next = h / s;
+    if (best == k - 1 && k < K && !shrunk) {
+      const double s1 = std::max(s / alpha_[k - 1][k], kScaleFloor);
+      if (A_[k + 1] * s1 <= least) next = h / s1;
+    }
+    return next;
+  }
+
+ private:
+  double tol_ = -1.0, e_ = 0.0;
+  int K_ = -1;
+  int n_[kMaxColumns + 2] = {};
+  double A_[kMaxColumns + 2] = {};
+  double alpha_[kMaxColumns + 1][kMaxColumns + 1] = {};
+};
+
+// ---- RKF45: Fehlberg's 4(5) pair with the Shampine-Watts step control
+//
+// L. F. Shampine and H. A. Watts, "Practical solution of ordinary
+// differential equations by Runge-Kutta methods", Sandia report SAND76-0585
+// (1976); G. E. Forsythe, M. A. Malcolm and C. B. Moler, Computer Methods for
+// Mathematical Computations (1977), Sec. 6.5.  The quantities are the
+// published ones, as rkf45_rlx (source/propagation/relaxation.f90:501-621)
+// drives the driver (source/propagation/rkf45rlxlib.f90): one call per
+// output time tout = i*step, relerr = abserr = tol.
+//  * Error test: with s the fifth-order solution and
+//    ee_k = |-2090 y'_k + 21970 f3_k - 15048 f4_k + 22528 f2_k - 27360 f5_k|,
+//    the step passes when |h| max_k ee_k / (|y_k| + |s_k| + abserr/relerr)
+//    * (2 / relerr) / 752400 <= 1.
+//  * Step factors (safety 0.9, order 1/5): after a failure
+//    0.9 / est^(1/5), at least 0.1 (est >= 9^5); after a success
+//    0.9 / est^(1/5), at most 5 (est <= (0.9/5)^5), at most 1 right after a
+//    failure; never below hmin = 26 eps |t|.
+//  * The first step of a state comes from the tolerance and the derivative,
+//    component by component: h = |tout - t|, reduced to (tol_k / |y'_k|)^(1/5)
+//    wherever tol_k < |y'_k| h^5, and at least 26 eps max(|t|, |tout - t|).
+//  * The step is cut to reach tout exactly (half the remaining interval
+//    when the step exceeds half of it); an interval below 26 eps |t| is
+//    covered by one Euler step.
+//  * Limits: 30000 derivative evaluations per state (code 4); a step
+//    reduced below hmin (code 6); relerr below 2 eps + 1e-12 (code 3).  The
+//    published routine also stops after 100 calls whose step exceeds twice
+//    the output interval (code 7); rkf45rlxlib.f90:593 raises that count to
+//    10000, which is kept so that the same runs finish.
+//  * The derivative y' carries over from one call to the next, also when the
+//    caller normalises y in between (as rkf45_rlx does).
+class Rkf45Control {
+ public:
+  static constexpr int kMaxEvals = 30000;
+  static constexpr int kMaxOversizeCalls = 10000;
+  static constexpr double kErrDen = 752400.0;
+  static constexpr double kFailEstCap = 59049.0;        // 9^5: failure factor floor 0.1
+  static constexpr double kPassEstFloor = 0.0001889568;  // (0.9/5)^5: growth cap 5
+
+  // Status codes of the published driver (2: reached tout).
+  enum { kReached = 2, kRelTolTooSmall = 3, kTooManyEvals = 4, kStepTooSmall = 6, kTooManyOversize = 7,
+         kBadInput = 8 };
+
+  void begin_state() {
+    have_step_ = false;
+    oversize_calls_ = 0;
+    nfev_ = 0;
+  }
+  void count_evals(int k) { nfev_ += k; }
+  bool evals_exhausted() const { return nfev_ > kMaxEvals; }
+  bool have_step() const { return have_step_; }
+  double step() const { return h_; }
+  void set_step(double h) { h_ = h; }
+
+  // First step from the host copies of y and y' (component order as published).
+  void initial_step(const std::vector<double>& y, const std::vector<double>& yp, double t, double dt,
+                    double relerr, double abserr) {
+    double h = std::fabs(dt), last_tol = 0.0;
+    for (size_t k = 0; k < y.size(); ++k) {
+      const double tol = relerr * std::fabs(y[k]) + abserr;
+      if (tol > 0.0) {
+        last_tol = tol;
+        const double ypk = std::fabs(yp[k]);
+        if (tol < ypk * std::pow(h, 5)) h = std::pow(tol / ypk, 0.2);
+      }
+    }
+    if (last_tol <= 0.0) h = 0.0;
+    h_ = std::max(h, 26.0 * DBL_EPSILON * std::max(std::fabs(t), std::fabs(dt)));
+    have_step_ = true;
+  }
+
+  // Counts a call whose step exceeds twice the output interval.
+  bool oversize_limit(double dt) {
+    if (2.0 * std::fabs(dt) <= std::fabs(h_)) ++oversize_calls_;
+    if (oversize_calls_ == kMaxOversizeCalls) {
+      oversize_calls_ = 0;
+      return true;
+    }
+    return false;
+  }
+
+  static double fail_factor(double est) { return est < kFailEstCap ? 0.9 / std::pow(est, 0.2) : 0.1; }
+  static double pass_factor(double est, bool failed) {
+    const double s = kPassEstFloor < est ? 0.9 / std::pow(est, 0.2) : 5.0;
+    return failed ? std::min(s, 1.0) : s;
+  }
+
+ private:
+  double h_ = 0.0;
+  bool have_step_ = false;
+  int oversize_calls_ = 0;
+  int nfev_ = 0;
 };
 
 void relax_run(Handle& H, const adc2g_relax_params& p, const double* guess_h, double* eigval, double* eigvec,
@@ -416,10 +603,11 @@ void relax_run(Handle& H, const adc2g_relax_params& p, const double* guess_h, do
   std::vector<double> alpha(kd), beta(kd), b2h(kd), sx((size_t)maxvec * kb);
   auto SH = [&](std::vector<double>& m, int i, int j) -> double& { return m[i + (size_t)maxvec * j]; };
 
-  // Bulirsch-Stoer state (bs_rlx: intorder 16, nextstep 0, time 0) and
-  // vectors: saved psi, dtpsi (as H psi), the midpoint pair, the tableau
-  BSEngine bse;
-  int bs_order = BS_MAXORDER;
+  // Extrapolation integrator state (bs_rlx: highest column 16, no proposed
+  // step yet, time 0) and vectors: the saved state, its derivative -H psi,
+  // the midpoint pair and the extrapolation tableau
+  ExtrapolationControl bsc;
+  int bs_K = ExtrapolationControl::kMaxColumns;
   double bs_next = 0.0, bs_time = 0.0;
   DevBuf<double> bs_saved, bs_w, bs_p1, bs_p2, bs_tab, bs_dt;
   std::vector<double> bs_part(nblk);
@@ -429,152 +617,80 @@ void relax_run(Handle& H, const adc2g_relax_params& p, const double* guess_h, do
     bs_dt.alloc(n);
     bs_p1.alloc(n);
     bs_p2.alloc(n);
-    bs_tab.alloc((size_t)n * BS_MAXORDER);
+    bs_tab.alloc((size_t)n * ExtrapolationControl::kMaxColumns);
   }
-  // one BSStep call; psi holds Psi, bs_w holds H psi (DtPsi = -bs_w)
-  auto bs_step = [&](double int_period, double abs_time, double large_step, double tol_error,
-                     double& actual_out) -> int {
-    const double RelativeMinStep = 1.0e-10, MaxScale = 0.1, ErrorSafety = 0.25, ReductionSafety = 0.7,
-                 MinReduction = 0.5, MaxReduction = 1.0e-5, Tiny = 1.0e-25;
-    int& io = bs_order;
-    if (io < 2 || io > BS_MAXORDER) return 1;
-    const double min_large = RelativeMinStep * int_period;
-    double next_large = -Tiny;
-    bse.new_time = -Tiny;
-    if (tol_error != bse.old_accuracy || io != bse.old_int_order) {
-      bse.old_accuracy = tol_error;
-      bse.int_accuracy = ErrorSafety * tol_error;
-      bse.old_int_order = io;
-      for (int q = 1; q <= io + 1; ++q) bse.nsteps[q] = 2 * q;
-      bse.expense[1] = bse.nsteps[1] + 1;
-      for (int q = 2; q <= io + 1; ++q) bse.expense[q] = bse.expense[q - 1] + bse.nsteps[q];
-      for (int q = 2; q <= io; ++q)
-        for (int q1 = 1; q1 < q; ++q1)
-          bse.correction[q1][q] =
-              std::pow(bse.int_accuracy, (bse.expense[q1 + 1] - bse.expense[q + 1]) /
-                                             ((2 * q1 + 1) * (bse.expense[q + 1] - bse.expense[1] + 1.0)));
-      int oc = 2;
-      while (oc <= io - 1) {
-        if (bse.expense[oc + 1] > bse.expense[oc] * bse.correction[oc - 1][oc]) break;
-        ++oc;
-      }
-      bse.optimal_column = oc;
-      io = oc;
+  // Modified midpoint rule over h with m substeps from the saved state and
+  // its derivative -bs_dt; the smoothed end value goes to psi.
+  auto bs_midpoint = [&](double h, int m) {
+    const double hs = h / m;
+    hipLaunchKernelGGL(bs_mm_start_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, bs_saved.get(), bs_dt.get(), -hs,
+                       bs_p1.get(), bs_p2.get());
+    HIP_CHECK(hipGetLastError());
+    sigma1(bs_p2.get(), bs_w.get());
+    for (int k = 2; k <= m; ++k) {
+      hipLaunchKernelGGL(bs_mm_step_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, bs_w.get(), 2.0 * hs,
+                         bs_p1.get(), bs_p2.get());
+      HIP_CHECK(hipGetLastError());
+      sigma1(bs_p2.get(), bs_w.get());
     }
-    double actual = large_step;
-    bool reduced = false;
-    double min_work = 1.0e35;
+    hipLaunchKernelGGL(bs_mm_final_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, bs_w.get(), hs, bs_p1.get(),
+                       bs_p2.get(), psi.get());
+    HIP_CHECK(hipGetLastError());
+  };
+  // One extrapolation step of at most h inside [0, period] from time t: psi
+  // holds the state, bs_w holds H psi.  Returns 0, 1 (highest column out of
+  // range) or 2 (step below 1e-10 of the period); the step taken goes to
+  // taken and the proposal for the next one to bs_next.
+  auto bs_step = [&](double period, double t, double h, double tol, double& taken) -> int {
+    using EC = ExtrapolationControl;
+    if (bs_K < 2 || bs_K > EC::kMaxColumns) return 1;
+    bs_K = bsc.configure(tol, bs_K);
+    const int K = bs_K;
+    const double hmin = EC::kMinStepRel * period;
     HIP_CHECK(hipMemcpyAsync(bs_saved.get(), psi.get(), sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-    // DtPsi = -H psi, kept apart from bs_w, which the midpoint steps reuse
     HIP_CHECK(hipMemcpyAsync(bs_dt.get(), bs_w.get(), sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-    if (actual != next_large || abs_time != bse.new_time) {
-      bse.first_call = true;
-      bse.optimal_column = io;
-    }
-    double err[BS_MAXORDER] = {};
-    int q = 0, q1 = 0;
-    for (;;) {  // 300
-      double reduction = 0.0;
-      bool conv = false;
-      q1 = 0;
-      for (q = 1; q <= io; ++q) {
-        bse.new_time = abs_time + actual;
-        const double sq = (actual / bse.nsteps[q]) * (actual / bse.nsteps[q]);
-        if (actual < min_large && bse.new_time < int_period) return 2;
-        // ModMidpoint: dt = -H saved is -bs_dt
-        const int ns_q = bse.nsteps[q];
-        const double h = actual / ns_q, h2 = 2.0 * h;
-        hipLaunchKernelGGL(bs_mm_start_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, bs_saved.get(), bs_dt.get(),
-                           -h, bs_p1.get(), bs_p2.get());
-        HIP_CHECK(hipGetLastError());
-        sigma1(bs_p2.get(), bs_w.get());
-        for (int k = 2; k <= ns_q; ++k) {
-          hipLaunchKernelGGL(bs_mm_step_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, bs_w.get(), h2, bs_p1.get(),
-                             bs_p2.get());
-          HIP_CHECK(hipGetLastError());
-          sigma1(bs_p2.get(), bs_w.get());
-        }
-        hipLaunchKernelGGL(bs_mm_final_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, bs_w.get(), h, bs_p1.get(),
-                           bs_p2.get(), psi.get());
-        HIP_CHECK(hipGetLastError());
-        // PolyExtrapol + AbsBSError
-        bse.old_sq[q] = sq;
+    double err[EC::kMaxColumns + 1] = {}, hsq[EC::kMaxColumns + 1] = {};
+    bool shrunk = false;
+    int k_acc = 0;
+    for (;;) {
+      double shrink = 0.0;
+      for (int k = 1; k <= K && !k_acc; ++k) {
+        if (h < hmin && t + h < period) return 2;
+        const int m = bsc.substeps(k);
+        hsq[k] = (h / m) * (h / m);
+        bs_midpoint(h, m);
+        // polynomial extrapolation in h^2: column k against the earlier ones
         BSFactors fac{};
-        for (int pp = 1; pp < q; ++pp) {
-          const double hl = 1.0 / (bse.old_sq[q - pp] - sq);
-          fac.f[pp] = sq * hl;
-          fac.g[pp] = bse.old_sq[q - pp] * hl;
+        for (int q = 1; q < k; ++q) {
+          const double r = 1.0 / (hsq[k - q] - hsq[k]);
+          fac.f[q] = hsq[k] * r;
+          fac.g[q] = hsq[k - q] * r;
         }
-        hipLaunchKernelGGL(bs_extrapol_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, q, io, fac, psi.get(),
+        hipLaunchKernelGGL(bs_extrapol_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, k, K, fac, psi.get(),
                            bs_tab.get(), part.get());
         HIP_CHECK(hipGetLastError());
-        double max_error = 0.0;
-        q1 = 0;
-        if (q >= 2) {
-          HIP_CHECK(hipMemcpyAsync(bs_part.data(), part.get(), sizeof(double) * nblk, hipMemcpyDeviceToHost, st));
-          HIP_CHECK(hipStreamSynchronize(st));
-          clock.collect();
-          for (double x : bs_part) max_error = std::max(max_error, x);
-          q1 = q - 1;
-          err[q1] = std::pow(max_error / bse.int_accuracy, 1.0 / (2 * q1 + 1.0));
-        }
-        if (q >= 2 && (q >= bse.optimal_column - 1 || bse.first_call)) {
-          if (max_error < tol_error) {
-            conv = true;
-            break;
-          }
-          const int oc = bse.optimal_column;
-          if (q == io || q == oc + 1) {
-            reduction = ReductionSafety / err[q1];
-            break;
-          } else if (q == oc) {
-            if (bse.correction[oc - 1][oc] < err[q1]) {
-              reduction = 1.0 / err[q1];
-              break;
-            }
-          } else if (oc == io) {
-            if (bse.correction[q1][io - 1] < err[q1]) {
-              reduction = ReductionSafety * bse.correction[q1][io - 1] / err[q1];
-              break;
-            }
-          } else if (bse.correction[q1][oc] < err[q1]) {
-            reduction = bse.correction[q1][oc - 1] / err[q1];
-            break;
-          }
-        }
+        if (k < 2) continue;
+        HIP_CHECK(hipMemcpyAsync(bs_part.data(), part.get(), sizeof(double) * nblk, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        clock.collect();
+        double max_dy = 0.0;
+        for (double x : bs_part) max_dy = std::max(max_dy, x);
+        err[k - 1] = bsc.scaled_error(k - 1, max_dy);
+        const auto v = bsc.judge(k, K, max_dy, tol, err, &shrink);
+        if (v == EC::Verdict::kAccept) k_acc = k;
+        else if (v == EC::Verdict::kShrink) break;
       }
-      if (conv) break;
-      reduction = std::max(std::min(reduction, MinReduction), MaxReduction);  // 400
-      actual *= reduction;
-      reduced = true;
+      if (k_acc) break;
+      h *= EC::clamp_shrink(shrink);
+      shrunk = true;
     }
-    bse.first_call = false;  // 500
-    double scale = 0.0;
-    for (int q2 = 1; q2 <= q1; ++q2) {
-      const double wf = std::max(err[q2], MaxScale);
-      const double work = wf * bse.expense[q2 + 1];
-      if (work < min_work) {
-        scale = wf;
-        min_work = work;
-        bse.optimal_column = q2 + 1;
-      }
-    }
-    next_large = actual / scale;
-    const int oc = bse.optimal_column;
-    if (oc >= q && oc < io && !reduced) {
-      const double wf = std::max(scale / bse.correction[oc - 1][oc], MaxScale);
-      if (bse.expense[oc + 1] * wf <= min_work) {
-        next_large = actual / wf;
-        bse.optimal_column = oc + 1;
-      }
-    }
-    bs_next = next_large;
-    actual_out = actual;
+    bs_next = bsc.next_step(h, k_acc, K, shrunk, err);
+    taken = h;
     return 0;
   };
 
-  // RKF45 state and vectors (y is psi; yp persists, as the reference's dtpsi)
-  RKState rks;
+  // RKF45 state and vectors (y is psi; y' persists from call to call)
+  Rkf45Control rkc;
   DevBuf<double> rk_yp, rk_f[5], rk_x, rk_w;
   std::vector<double> rk_part(nblk);
   if (rk_mode) {
@@ -594,8 +710,10 @@ void relax_run(Handle& H, const adc2g_relax_params& p, const double* guess_h, do
                        rk_f[0].get(), rk_f[1].get(), rk_f[2].get(), rk_f[3].get(), rk_f[4].get(), out);
     HIP_CHECK(hipGetLastError());
   };
-  // r8_fehl: the solution s ends in rk_f[0] (the reference passes f1 as s)
-  auto rk_fehl = [&](double h) {
+  // One Fehlberg step of size h from (psi, y'): stages 2..6 at
+  // h/4, 3h/8, 12h/13, h, h/2 into rk_f[0..4]; the fifth-order solution then
+  // replaces rk_f[0] (the first stage is no longer needed).
+  auto rk_fehlberg = [&](double h) {
     rk_stage(1, h / 4.0, rk_x.get());
     rk_f_of(rk_x.get(), rk_f[0].get());
     rk_stage(2, 3.0 * h / 32.0, rk_x.get());
@@ -604,137 +722,92 @@ void relax_run(Handle& H, const adc2g_relax_params& p, const double* guess_h, do
     rk_f_of(rk_x.get(), rk_f[2].get());
     rk_stage(4, h / 4104.0, rk_x.get());
     rk_f_of(rk_x.get(), rk_f[3].get());
-    rk_stage(5, h / 20520.0, rk_x.get());  // the reference stores this point in f1, used once
+    rk_stage(5, h / 20520.0, rk_x.get());
     rk_f_of(rk_x.get(), rk_f[4].get());
     rk_stage(6, h / 7618050.0, rk_f[0].get());
   };
-  // one r8_rkf45 call (flag 1 or 2 as rkf45_rlx uses it); returns the flag
-  auto rk_step = [&](double& t, double tout, double relerr, double abserr, int flag) -> int {
-    const double eps = DBL_EPSILON;
-    if (relerr < 0.0 || abserr < 0.0 || flag == 0 || flag > 8 || flag < -2) return 8;
-    int mflag = std::abs(flag);
-    if (mflag != 1) {
-      if (t == tout && rks.kflag != 3) return 8;
-      if (mflag == 2) {
-        if (rks.kflag == 3) {
-          flag = rks.flag_save;
-          mflag = std::abs(flag);
-        } else if (rks.init == 0) {
-          flag = rks.flag_save;
-        } else if (rks.kflag == 4) {
-          rks.nfe = 0;
-        } else if (rks.kflag == 5 && abserr == 0.0) {
-          return 5;
-        } else if (rks.kflag == 6 && relerr <= rks.relerr_save && abserr <= rks.abserr_save) {
-          return 6;
-        }
-      }
-    }
-    rks.flag_save = flag;
-    rks.kflag = 0;
-    rks.relerr_save = relerr;
-    rks.abserr_save = abserr;
-    if (relerr < 2.0 * eps + 1.0e-12) {
-      rks.kflag = 3;
-      return 3;
-    }
+  // max_k ee_k / (|y_k| + |s_k| + ae) of the step just taken
+  auto rk_error_ratio = [&](double ae) -> double {
+    hipLaunchKernelGGL(rk_err_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, ae, psi.get(), rk_f[0].get(),
+                       rk_yp.get(), rk_f[1].get(), rk_f[2].get(), rk_f[3].get(), rk_f[4].get(), part.get());
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(rk_part.data(), part.get(), sizeof(double) * nblk, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    clock.collect();
+    double r = 0.0;
+    for (double x : rk_part) r = std::max(r, x);
+    return r;
+  };
+  // Advance psi from t to tout with relerr = abserr = tol (first: the first
+  // call of a state, which evaluates y' and restarts the counters).  Returns
+  // Rkf45Control::kReached or an error code of the published driver.
+  auto rk_advance = [&](double& t, double tout, double tol, bool first) -> int {
+    using RC = Rkf45Control;
+    const double relerr = tol, abserr = tol, tiny = 26.0 * DBL_EPSILON;
+    if (relerr < 0.0 || abserr < 0.0) return RC::kBadInput;
+    if (!first && t == tout) return RC::kBadInput;
+    if (relerr < 2.0 * DBL_EPSILON + 1.0e-12) return RC::kRelTolTooSmall;
     double dt = tout - t;
-    if (mflag == 1) {
-      rks.init = 0;
-      rks.kop = 0;
+    if (first) {
+      rkc.begin_state();
       rk_f_of(psi.get(), rk_yp.get());
-      rks.nfe = 1;
-      if (t == tout) return 2;
+      rkc.count_evals(1);
+      if (t == tout) return RC::kReached;
     }
-    if (rks.init == 0) {
-      // the initial step from the components, in order (host copies of y, yp)
-      rks.init = 1;
+    if (!rkc.have_step()) {
       std::vector<double> hy(n), hyp(n);
       HIP_CHECK(hipMemcpyAsync(hy.data(), psi.get(), sizeof(double) * n, hipMemcpyDeviceToHost, st));
       HIP_CHECK(hipMemcpyAsync(hyp.data(), rk_yp.get(), sizeof(double) * n, hipMemcpyDeviceToHost, st));
       HIP_CHECK(hipStreamSynchronize(st));
       clock.collect();
-      double h = std::fabs(dt), toln = 0.0;
-      for (int64_t k = 0; k < n; ++k) {
-        const double tol = relerr * std::fabs(hy[k]) + abserr;
-        if (tol > 0.0) {
-          toln = tol;
-          const double ypk = std::fabs(hyp[k]);
-          if (tol < ypk * std::pow(h, 5)) h = std::pow(tol / ypk, 0.2);
-        }
-      }
-      if (toln <= 0.0) h = 0.0;
-      rks.h = std::max(h, 26.0 * eps * std::max(std::fabs(t), std::fabs(dt)));
-      rks.flag_save = flag >= 0 ? 2 : -2;
+      rkc.initial_step(hy, hyp, t, dt, relerr, abserr);
     }
-    rks.h = dt >= 0.0 ? std::fabs(rks.h) : -std::fabs(rks.h);
-    if (2.0 * std::fabs(dt) <= std::fabs(rks.h)) ++rks.kop;
-    if (rks.kop == 10000) {
-      rks.kop = 0;
-      return 7;
-    }
-    if (std::fabs(dt) <= 26.0 * eps * std::fabs(t)) {
+    rkc.set_step(dt >= 0.0 ? std::fabs(rkc.step()) : -std::fabs(rkc.step()));
+    if (rkc.oversize_limit(dt)) return RC::kTooManyOversize;
+    if (std::fabs(dt) <= tiny * std::fabs(t)) {  // one Euler step covers it
       t = tout;
-      rk_stage(1, dt, rk_x.get());  // y + dt yp
+      rk_stage(1, dt, rk_x.get());
       HIP_CHECK(hipMemcpyAsync(psi.get(), rk_x.get(), sizeof(double) * n, hipMemcpyDeviceToDevice, st));
       rk_f_of(psi.get(), rk_yp.get());
-      ++rks.nfe;
-      return 2;
+      rkc.count_evals(1);
+      return RC::kReached;
     }
-    bool output = false;
     const double scale = 2.0 / relerr, ae = scale * abserr;
     for (;;) {
-      bool hfaild = false;
-      const double hmin = 26.0 * eps * std::fabs(t);
+      bool failed = false, last = false;
+      const double hmin = tiny * std::fabs(t);
       dt = tout - t;
-      if (!(2.0 * std::fabs(rks.h) <= std::fabs(dt))) {
-        if (std::fabs(dt) <= std::fabs(rks.h)) {
-          output = true;
-          rks.h = dt;
+      double h = rkc.step();
+      if (!(2.0 * std::fabs(h) <= std::fabs(dt))) {
+        if (std::fabs(dt) <= std::fabs(h)) {
+          last = true;
+          h = dt;
         } else {
-          rks.h = 0.5 * dt;
+          h = 0.5 * dt;
         }
       }
-      double esttol = 0.0;
+      double est = 0.0;
       for (;;) {
-        if (rks.nfe > 30000) {
-          rks.kflag = 4;
-          return 4;
-        }
-        rk_fehl(rks.h);
-        rks.nfe += 5;
-        hipLaunchKernelGGL(rk_err_kernel, dim3(nblk), dim3(RL_THREADS), 0, st, n, ae, psi.get(), rk_f[0].get(),
-                           rk_yp.get(), rk_f[1].get(), rk_f[2].get(), rk_f[3].get(), rk_f[4].get(), part.get());
-        HIP_CHECK(hipGetLastError());
-        HIP_CHECK(hipMemcpyAsync(rk_part.data(), part.get(), sizeof(double) * nblk, hipMemcpyDeviceToHost, st));
-        HIP_CHECK(hipStreamSynchronize(st));
-        clock.collect();
-        double eeoet = 0.0;
-        for (double x : rk_part) eeoet = std::max(eeoet, x);
-        esttol = std::fabs(rks.h) * eeoet * scale / 752400.0;
-        if (esttol <= 1.0) break;
-        hfaild = true;
-        output = false;
-        const double sc = esttol < 59049.0 ? 0.9 / std::pow(esttol, 0.2) : 0.1;
-        rks.h = sc * rks.h;
-        if (std::fabs(rks.h) < hmin) {
-          rks.kflag = 6;
-          return 6;
-        }
+        if (rkc.evals_exhausted()) return RC::kTooManyEvals;
+        rk_fehlberg(h);
+        rkc.count_evals(5);
+        est = std::fabs(h) * rk_error_ratio(ae) * scale / RC::kErrDen;
+        if (est <= 1.0) break;
+        failed = true;
+        last = false;
+        h = RC::fail_factor(est) * h;
+        if (std::fabs(h) < hmin) return RC::kStepTooSmall;
       }
-      t = t + rks.h;
+      t = t + h;
       HIP_CHECK(hipMemcpyAsync(psi.get(), rk_f[0].get(), sizeof(double) * n, hipMemcpyDeviceToDevice, st));
       rk_f_of(psi.get(), rk_yp.get());
-      ++rks.nfe;
-      double sc = 0.0001889568 < esttol ? 0.9 / std::pow(esttol, 0.2) : 5.0;
-      if (hfaild) sc = std::min(sc, 1.0);
-      const double hh = std::max(sc * std::fabs(rks.h), hmin);
-      rks.h = rks.h >= 0.0 ? hh : -hh;
-      if (output) {
+      rkc.count_evals(1);
+      const double hh = std::max(RC::pass_factor(est, failed) * std::fabs(h), hmin);
+      rkc.set_step(h >= 0.0 ? hh : -hh);
+      if (last) {
         t = tout;
-        return 2;
+        return RC::kReached;
       }
-      if (flag <= 0) return -2;
     }
   };
 
@@ -754,7 +827,7 @@ void relax_run(Handle& H, const adc2g_relax_params& p, const double* guess_h, do
     bool converged = false;
     if (bs_mode) {
       // bs_rlx (relaxation.f90:315-495): int(tfinal/intperiod) timesteps with
-      // tfinal = maxiter*step, each covered by BSStep calls
+      // tfinal = maxiter*step, each covered by extrapolation steps
       const double intperiod = p.step, tfinal = (double)p.maxiter * p.step, tiny = 1e-9;
       const int nt = (int)(tfinal / intperiod);
       it_done = nt;
@@ -790,13 +863,12 @@ void relax_run(Handle& H, const adc2g_relax_params& p, const double* guess_h, do
     }
     if (rk_mode) {
       // rkf45_rlx (relaxation.f90:501-621): tout = i*step, relerr = abserr =
-      // siltol, flag 1 at the start of the state
+      // siltol, the driver restarted at the start of the state
       double t = 0.0;
-      int flag = 1;
       for (int it = 1; it <= p.maxiter; ++it) {
-        flag = rk_step(t, it * p.step, p.toler, p.toler, flag);
-        if (flag != 2)
-          throw Error(ADC2G_EBREAK, "Error in the RKF45 integrator. Error code: " + std::to_string(flag));
+        const int code = rk_advance(t, it * p.step, p.toler, it == 1);
+        if (code != Rkf45Control::kReached)
+          throw Error(ADC2G_EBREAK, "Error in the RKF45 integrator. Error code: " + std::to_string(code));
         normalise(psi.get(), w_d + 2);
         residual(psi.get(), E, res);
         if (!std::isfinite(E) || !std::isfinite(res))
