@@ -1,8 +1,10 @@
 // C ABI (include/adc2g.h): argument checks, exception -> error code mapping,
 // host <-> device staging for the host-pointer entry points.
+#include <cmath>
 #include <cstring>
 #include <memory>
 
+#include "coupled.hpp"
 #include "handle.hpp"
 #include "solvers.hpp"
 
@@ -111,6 +113,22 @@ int32_t adc2g_phph(adc2g_handle* h, double* h11) {
     HIP_CHECK(hipSetDevice(h->h.device));
     const size_t n = (size_t)h->h.d.n1 * h->h.d.n1;
     HIP_CHECK(hipMemcpyAsync(h11, h->h.H11.get(), n * sizeof(double), hipMemcpyDeviceToHost, h->h.st));
+    HIP_CHECK(hipStreamSynchronize(h->h.st));
+  });
+}
+
+int32_t adc2g_coupled_schur(adc2g_handle* h, double sigma, int32_t precision, double* a0) {
+  return guarded([&] {
+    ARG_CHECK(h && a0, "NULL argument");
+    ARG_CHECK(!h->h.parent, "the Schur complement belongs to the full operator, not to a CVS view");
+    ARG_CHECK(std::isfinite(sigma), "sigma must be finite");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    CoupledPrecon cp;
+    cp.init(h->h, prec_from_int(precision));
+    ARG_CHECK(sigma < cp.dmin, "sigma must lie below every 2h2p diagonal element");
+    cp.form_schur(sigma, cp.prec);
+    const size_t n = (size_t)h->h.d.n1 * h->h.d.n1;
+    HIP_CHECK(hipMemcpyAsync(a0, cp.Qm.get(), n * sizeof(double), hipMemcpyDeviceToHost, h->h.st));
     HIP_CHECK(hipStreamSynchronize(h->h.st));
   });
 }

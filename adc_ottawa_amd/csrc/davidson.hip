@@ -29,14 +29,16 @@
 // Mixed precision: with precision != FP64 the first phase uses low-precision
 // sigma-vectors without convergence counting; when every residual is below
 // lowprec_tol, or the largest one has not improved by 10% for 40 iterations,
-// the solver restarts from the Ritz vectors with FP64 sigma-vectors
-// ("restarted refinement"), so convergence is always judged on FP64
-// residuals.
+// the sigma-vectors of the whole subspace are recomputed in FP64 and the
+// iteration continues on them (refinement), so convergence is always judged
+// on FP64 residuals.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <memory>
 #include <numeric>
 
+#include "coupled.hpp"
 #include "linalg.hpp"
 #include "precon.hpp"
 #include "skinny.hpp"
@@ -307,9 +309,13 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
   const auto t0 = std::chrono::steady_clock::now();
   ARG_CHECK(p.nstates >= 1, "nstates must be >= 1");
   ARG_CHECK(p.tol > 0, "tol must be > 0");
-  ARG_CHECK(p.precon >= 1 && p.precon <= 3, "precon must be 1 (DPR), 2 (Olsen) or 3 (learned)");
-  ARG_CHECK(!(p.precon == 3 && H.parent), "the learned preconditioner works on the full operator only");
-  if (p.precon == 3 && !precon_ready(H))
+  ARG_CHECK(p.precon >= ADC2G_PRE_DPR && p.precon <= ADC2G_PRE_LEARNED_COUPLED,
+            "precon must be 1 (DPR), 2 (Olsen), 3 (learned), 4 (coupled) or 5 (learned on the coupled step)");
+  const bool learned = p.precon == ADC2G_PRE_LEARNED || p.precon == ADC2G_PRE_LEARNED_COUPLED;
+  const bool coupled = p.precon == ADC2G_PRE_COUPLED || p.precon == ADC2G_PRE_LEARNED_COUPLED;
+  ARG_CHECK(!((learned || coupled) && H.parent),
+            "the learned and coupled preconditioners work on the full operator only");
+  if (learned && !precon_ready(H))
     throw Error(ADC2G_ESTATE, "learned preconditioner requested but no network is loaded");
   const int nstates = p.nstates;
   // davinitialise (block_davidson.f90:195-205)
@@ -368,6 +374,18 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
     launch_unit_vectors(bs, didx.get(), D.V.get(), n, st);
     HIP_CHECK(hipStreamSynchronize(st));
   }
+  // coupled (Schur-complement) step: A0 = H11 - C P C^T at the shift chosen
+  // for the nstates wanted roots, eigen-decomposed once per run
+  std::unique_ptr<CoupledPrecon> cpl;
+  double t_precon = 0.0;
+  if (coupled) {
+    const auto tc = std::chrono::steady_clock::now();
+    cpl.reset(new CoupledPrecon);
+    cpl->init(H, Prec::F32);
+    cpl->verbose = p.verbose;
+    cpl->choose_sigma(nstates, 6);
+    t_precon += std::chrono::duration<double>(std::chrono::steady_clock::now() - tc).count();
+  }
   int currdim = bs, bs_curr = bs, mv_curr = mv, ns_curr = nstates;
   int nconv = 0, nconv_prev = 0;
   int64_t nmult_ref = 0;
@@ -377,7 +395,9 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
 
   std::vector<double> theta(bs), norm(bs), stored_norm(bs, 0.0);
   std::vector<double> prev_norm(bs, INFINITY);
-  std::vector<int> stall(bs, 0);
+  // learned step: per root, iterations without a 10 % residual drop, and the
+  // sticky switch to the base step once two such iterations occurred
+  std::vector<int> stall(bs, 0), sticky(bs, 0);
   double low_best = INFINITY;
   int low_stall = 0;
   bool converged = false;
@@ -419,8 +439,20 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
                     norm[i] < p.tol ? 'y' : 'n');
       std::fflush(stdout);
     }
-    if (trace) davtrace_iteration(H, *trace, it, bs_curr, ns_curr, theta.data(), norm.data(), D.X.get(),
-                                  D.Res.get(), n);
+    if (trace) {
+      // records of the active roots with the network's base step (the
+      // coupled step when the run uses it, else DPR inside the sampler)
+      const double* base = nullptr;
+      if (coupled && ns_curr > 0) {
+        for (int c0 = 0; c0 < ns_curr; c0 += cpl->nc) {
+          const int nb = std::min(cpl->nc, ns_curr - c0);
+          cpl->apply(nb, theta.data() + c0, D.Res.get() + n * c0, n, D.T.get() + n * c0, n);
+        }
+        base = D.T.get();
+      }
+      davtrace_iteration(H, *trace, it, bs_curr, ns_curr, theta.data(), norm.data(), D.X.get(), D.Res.get(), n,
+                         base);
+    }
 
     bool newly = false;
     if (lowphase) {
@@ -436,11 +468,11 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
         ++low_stall;
       }
       if (refine && (mx < lowtol || low_stall >= 40)) {
-        // restarted refinement: FP64 sigma-vectors from the current Ritz vectors
+        // refinement: the subspace is kept and its sigma-vectors W = H V are
+        // recomputed in FP64, so the iteration continues where the
+        // low-precision phase left it, now with exact Rayleigh matrices
         lowphase = false;
         cur = Prec::F64;
-        HIP_CHECK(hipMemcpyAsync(D.V.get(), D.X.get(), sizeof(double) * n * bs_curr, hipMemcpyDeviceToDevice, st));
-        currdim = bs_curr;
         D.sigma(0, currdim, cur);
         D.rayleigh_cols(0, currdim);
         std::fill(prev_norm.begin(), prev_norm.end(), INFINITY);
@@ -479,6 +511,9 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
           if (have_t) D.swap_cols(D.T, indx, src);
           std::swap(theta[indx], theta[src]);
           std::swap(norm[indx], norm[src]);
+          std::swap(prev_norm[indx], prev_norm[src]);
+          std::swap(stall[indx], stall[src]);
+          std::swap(sticky[indx], sticky[src]);
           stored_norm[indx] = norm[indx];
         }
         const int dd = nconv - nconv_prev;
@@ -507,16 +542,32 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
       launch_colsum(nb2, 2 * bs_curr, D.part.get(), D.nrm.get(), st);
       launch_olsen_step(n, bs_curr, D.thd.get(), H.hii.get(), D.X.get(), D.Res.get(), D.nrm.get(), D.T.get(), n, st);
     } else {
-      // learned step with DPR fallback for roots whose residual stalls
-      std::vector<int> use_dpr(bs_curr, 0);
-      for (int i = 0; i < bs_curr; ++i) {
-        if (norm[i] > 0.9 * prev_norm[i]) ++stall[i]; else stall[i] = 0;
-        if (stall[i] >= 2) {
-          use_dpr[i] = 1;
-          ++nfallback;
+      const auto tc = std::chrono::steady_clock::now();
+      if (coupled) {
+        if (it % 10 == 0) cpl->track(theta.data(), ns_curr);
+        for (int c0 = 0; c0 < bs_curr; c0 += cpl->nc) {
+          const int nb = std::min(cpl->nc, bs_curr - c0);
+          cpl->apply(nb, theta.data() + c0, D.Res.get() + n * c0, n, D.T.get() + n * c0, n);
         }
       }
-      precon_learned_step(H, n, bs_curr, D.thd.get(), norm.data(), use_dpr.data(), D.Res.get(), D.T.get(), n);
+      if (learned) {
+        // learned step on the base step (DPR or coupled); a root whose
+        // residual missed a 10 % drop in two iterations keeps the base step
+        // from then on
+        std::vector<int> use_base(bs_curr, 0);
+        for (int i = 0; i < bs_curr; ++i) {
+          if (norm[i] > 0.9 * prev_norm[i]) ++stall[i]; else stall[i] = 0;
+          if (stall[i] >= 2) sticky[i] = 1;
+          if (sticky[i]) {
+            use_base[i] = 1;
+            ++nfallback;
+          }
+        }
+        precon_learned_step(H, n, bs_curr, D.thd.get(), norm.data(), use_base.data(), D.Res.get(), D.T.get(), n,
+                            coupled ? D.T.get() : nullptr);
+      }
+      HIP_CHECK(hipStreamSynchronize(st));
+      t_precon += std::chrono::duration<double>(std::chrono::steady_clock::now() - tc).count();
     }
     for (int i = 0; i < bs_curr; ++i) prev_norm[i] = norm[i];
     // new block, orthogonal to the subspace and to the converged (deflated) vectors
@@ -562,6 +613,9 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
     stats->time_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     stats->time_sigma = D.t_sigma;
     stats->resnorm_max = rmax;
+    stats->time_precon = t_precon;
+    stats->nprecon_builds = cpl ? cpl->builds : 0;
+    stats->precon_sigma = cpl ? cpl->sigma : 0.0;
   }
 }
 

@@ -120,8 +120,15 @@ void dispatch(GemmArgs g, GemmWorkspace& ws, hipStream_t st, int splitk_hint) {
   const bool need_partial = !g.direct;
   if (need_partial) {
     const size_t need = (size_t)g.batch * split * (size_t)g.M * g.N * sizeof(Acc);
-    ws.buf.alloc(need);
-    g.partial = ws.buf.get();
+    if (ws.ext) {
+      if (need > ws.ext_size)
+        throw Error(ADC2G_ENOMEM, "GEMM partials need " + std::to_string(need) + " bytes; the scratch leaves " +
+                                      std::to_string(ws.ext_size));
+      g.partial = ws.ext;
+    } else {
+      ws.buf.alloc(need);
+      g.partial = ws.buf.get();
+    }
   }
   switch (shape) {
     case 2: launch<T, 64, 64, BK>(g, st); break;

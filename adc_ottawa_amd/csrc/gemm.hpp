@@ -611,6 +611,9 @@ enum class Prec { F64, F32, BF16 };
 
 struct GemmWorkspace {
   DevBuf<char> buf;
+  // when set, split-K partials go here (no allocation; too small -> ADC2G_ENOMEM)
+  char* ext = nullptr;
+  size_t ext_size = 0;
 };
 
 void gemm(GemmArgs g, Prec prec, GemmWorkspace& ws, hipStream_t st, int splitk_hint = 0);

@@ -309,6 +309,20 @@ size_t Handle::sigma_workspace_bytes(int nc, Prec p) const {
   return b;
 }
 
+bool Handle::lowprec_ready(Prec p) const {
+  return form == FORM_INT ? is[prec_index(p)].ready : (p == Prec::F64 ? fs[0].ready : fs[prec_index(p)].ready);
+}
+
+// Vectors per pass that fit the caller's scratch: the workspace of the pass
+// (256-byte aligned regions) plus at least 32 MiB, or the per-vector
+// workspace if larger, for the GEMM split-K partials.
+int Handle::scratch_chunk(Prec p) const {
+  const size_t per = sigma_workspace_bytes(1, p) + 6 * 256;
+  const size_t reserve = std::max<size_t>((size_t)32 << 20, per);
+  if (scratch_bytes < per + reserve) return 0;
+  return (int)std::min<size_t>((size_t)chunk, (scratch_bytes - reserve) / per);
+}
+
 // W = H X for nvec columns (device pointers, ld = ldx / ldw), in chunks of at
 // most `chunk` vectors.  The 1h1p-1h1p block H11 x1 is always FP64; every
 // coupling contraction runs in the GEMM precision p (FP32 accumulation for
@@ -321,20 +335,57 @@ void Handle::sigma(const double* X, int64_t ldx, double* W, int64_t ldw, int nve
     return;
   }
   if (!s) s = st;
-  ensure_lowprec(p);
-  const int nc_max = std::min(chunk, nvec);
   const size_t es = prec_size(p);
   const size_t n1 = d.n1, naux = d.naux;
-  X1c.alloc(n1 * nc_max * es);
-  ZG.alloc(n1 * n1 * nc_max * es);
-  if (form == FORM_INT) {
-    X1t.alloc(n1 * nc_max * es);
+  const size_t wes = (p == Prec::F64) ? 8 : 4;  // DF W accumulator type
+  int nc_max = std::min(chunk, nvec);
+  if (scratch) {
+    // caller-provided scratch: no allocation; the operands must be resident
+    ARG_CHECK(lowprec_ready(p), "the operands of this precision are not resident (create the handle with "
+                                "keep_lowprec = 1 or run one sigma-vector of this precision first)");
+    nc_max = std::min(nvec, scratch_chunk(p));
+    ARG_CHECK(nc_max >= 1, "one vector needs " + std::to_string(sigma_workspace_bytes(1, p)) +
+                               " bytes of scratch; the scratch holds " + std::to_string(scratch_bytes));
+    char* q = scratch;
+    auto carve = [&](size_t bytes) {
+      char* r = q;
+      q += (bytes + 255) / 256 * 256;
+      return (void*)r;
+    };
+    ws_x1c = carve(n1 * nc_max * es);
+    ws_zg = carve(n1 * n1 * nc_max * es);
+    if (form == FORM_INT) {
+      ws_x1t = carve(n1 * nc_max * es);
+    } else {
+      ws_yvv = carve(naux * n1 * nc_max * es);
+      ws_yoo = carve(naux * n1 * nc_max * es);
+      ws_wup = carve(naux * n1 * nc_max * wes);
+      ws_wupt = carve(naux * n1 * nc_max * es);
+    }
+    ARG_CHECK(q <= scratch + scratch_bytes, "scratch layout exceeds the scratch");
+    gws.ext = q;
+    gws.ext_size = scratch_bytes - (size_t)(q - scratch);
   } else {
-    const size_t wes = (p == Prec::F64) ? 8 : 4;  // W accumulator type
-    Yvv.alloc(naux * n1 * nc_max * es);
-    Yoo.alloc(naux * n1 * nc_max * es);
-    Wup.alloc(naux * n1 * nc_max * wes);
-    WupT.alloc(naux * n1 * nc_max * es);
+    ensure_lowprec(p);
+    gws.ext = nullptr;
+    gws.ext_size = 0;
+    X1c.alloc(n1 * nc_max * es);
+    ZG.alloc(n1 * n1 * nc_max * es);
+    ws_x1c = X1c.get();
+    ws_zg = ZG.get();
+    if (form == FORM_INT) {
+      X1t.alloc(n1 * nc_max * es);
+      ws_x1t = X1t.get();
+    } else {
+      Yvv.alloc(naux * n1 * nc_max * es);
+      Yoo.alloc(naux * n1 * nc_max * es);
+      Wup.alloc(naux * n1 * nc_max * wes);
+      WupT.alloc(naux * n1 * nc_max * es);
+      ws_yvv = Yvv.get();
+      ws_yoo = Yoo.get();
+      ws_wup = Wup.get();
+      ws_wupt = WupT.get();
+    }
   }
   for (int c0 = 0; c0 < nvec; c0 += nc_max) {
     const int nc = std::min(nc_max, nvec - c0);
@@ -343,6 +394,18 @@ void Handle::sigma(const double* X, int64_t ldx, double* W, int64_t ldw, int nve
     else
       sigma_df(X + ldx * c0, ldx, W + ldw * c0, ldw, nc, p, s);
   }
+}
+
+void Handle::sigma_part(const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, Prec p, int which,
+                        hipStream_t s) {
+  part = parent ? SIGMA_ALL : which;
+  try {
+    sigma(X, ldx, W, ldw, nvec, p, s);
+  } catch (...) {
+    part = SIGMA_ALL;
+    throw;
+  }
+  part = SIGMA_ALL;
 }
 
 namespace {
@@ -364,6 +427,24 @@ void pack_chunk(Prec p, const double* Xc, int64_t ldx, int nc, const DimsK& dk, 
   }
 }
 
+void pack_x1_only(Prec p, const double* Xc, int64_t ldx, int nc, const DimsK& dk, void* X1c, void* X1t,
+                  hipStream_t s) {
+  switch (p) {
+    case Prec::F64:
+      launch_pack_x1<double>(Xc, ldx, nc, dk, (double*)X1c, s);
+      launch_pack_x1t<double>(Xc, ldx, nc, dk, (double*)X1t, s);
+      break;
+    case Prec::F32:
+      launch_pack_x1<float>(Xc, ldx, nc, dk, (float*)X1c, s);
+      launch_pack_x1t<float>(Xc, ldx, nc, dk, (float*)X1t, s);
+      break;
+    case Prec::BF16:
+      launch_pack_x1<__bf16>(Xc, ldx, nc, dk, (__bf16*)X1c, s);
+      launch_pack_x1t<__bf16>(Xc, ldx, nc, dk, (__bf16*)X1t, s);
+      break;
+  }
+}
+
 void unpack_chunk(Prec p, const void* G, const double* Xc, int64_t ldx, double* Wc, int64_t ldw, int nc,
                   const DimsK& dk, const double* eps, const PairLayout& lay, hipStream_t s) {
   switch (p) {
@@ -374,6 +455,47 @@ void unpack_chunk(Prec p, const void* G, const double* Xc, int64_t ldx, double* 
 }
 
 }  // namespace
+
+// INT form, down half of one chunk: W1 += C x2 for the nc pair tensors in ZG
+// (virtual-major layout, GEMM precision p), W1 the 1h1p rows (ld ldw):
+//   s1(a,k)  += sum_c sum_bl V[a][c][bl] Z[c][vec][k][bl]   (batched over c, summed)
+//   s1(a,k)  -= sum_{m,bl} Z[a][vec][m][bl] O[k][m][bl]
+void Handle::int_down(const void* Z, int nc, Prec p, double* Wc, int64_t ldw, hipStream_t s) {
+  const int o = d.o, v = d.v;
+  const int64_t n1 = d.n1;
+  const IntSet& I = is[prec_index(p)];
+  auto span = [&](int cls, double work) -> std::pair<hipEvent_t, hipEvent_t> {
+    if (!prof_on) return {nullptr, nullptr};
+    hipEvent_t a = prof_event(), b = prof_event();
+    prof.push_back({cls, a, b, work});
+    return {a, b};
+  };
+  {  // sigma1(a, k + o*vec) += sum_{c,bl} Z[c][vec][k][bl] V[a][c][bl], computed as its
+     // transpose (rows (vec,k), columns a) so that v = 200 fills a 208-column tile
+    GemmArgs g;
+    g.M = o * nc; g.N = v; g.K = (int)n1; g.batch = v; g.reduce_batch = 1;
+    g.A = Z; g.sam = n1; g.sak = 1; g.sab = (int64_t)nc * o * n1;
+    g.B = I.V.get(); g.sbn = (int64_t)v * n1; g.sbk = 1; g.sbb = n1;
+    g.map.kind = MAP_SIG1T; g.map.C = Wc; g.map.beta = 1.0;
+    g.map.q[0] = o; g.map.q[1] = ldw;
+    if (v > 192 && v <= 208) g.tile = 6;
+    g.tag = 1;
+    auto e = span(0, 2.0 * v * (double)n1 * n1 * nc);
+    g.ev_begin = e.first; g.ev_end = e.second;
+    gemm(g, p, gws, s);
+  }
+  {  // sigma1(a*nc + vec, k) -= sum_{m,bl} Z[a][vec][m][bl] O[k][m][bl]
+    GemmArgs g;
+    g.M = v * nc; g.N = o; g.K = (int)(o * n1);
+    g.A = Z; g.sam = (int64_t)o * n1; g.sak = 1;
+    g.B = I.O.get(); g.sbn = (int64_t)o * n1; g.sbk = 1;
+    g.map.kind = MAP_SIG1O; g.map.C = Wc; g.map.alpha = -1.0; g.map.beta = 1.0;
+    g.map.q[0] = o; g.map.q[1] = ldw; g.map.q[2] = nc;
+    auto e = span(3, 2.0 * o * (double)n1 * n1 * nc);
+    g.ev_begin = e.first; g.ev_end = e.second;
+    gemm(g, p, gws, s);
+  }
+}
 
 // INT form, one chunk of nc vectors.  With the pair tensor stored
 // virtual-major, Z[c][vec][k][(b,l)] = Z(c,b,k,l) of vector vec:
@@ -402,11 +524,16 @@ void Handle::sigma_int(const double* Xc, int64_t ldx, double* Wc, int64_t ldw, i
   };
   const double flops_v = 2.0 * v * (double)n1 * n1 * nc;  // each virtual contraction
   const double flops_o = 2.0 * o * (double)n1 * n1 * nc;  // each occupied contraction
-  auto ep = span(1, (double)nc * (d.n2 * 8.0 + (double)n1 * n1 * esd));
-  if (ep.first) HIP_CHECK(hipEventRecord(ep.first, s));
-  pack_chunk(p, Xc, ldx, nc, dk, ZG.get(), X1c.get(), X1t.get(), lay, s);
-  if (ep.second) HIP_CHECK(hipEventRecord(ep.second, s));
-  {  // sigma1 = H11 x1 (FP64)
+  const bool down = part != SIGMA_UP, up = part != SIGMA_DOWN;
+  if (down) {
+    auto ep = span(1, (double)nc * (d.n2 * 8.0 + (double)n1 * n1 * esd));
+    if (ep.first) HIP_CHECK(hipEventRecord(ep.first, s));
+    pack_chunk(p, Xc, ldx, nc, dk, ws_zg, ws_x1c, ws_x1t, lay, s);
+    if (ep.second) HIP_CHECK(hipEventRecord(ep.second, s));
+  } else {
+    pack_x1_only(p, Xc, ldx, nc, dk, ws_x1c, ws_x1t, s);
+  }
+  if (down) {  // sigma1 = H11 x1 (FP64)
     GemmArgs g;
     g.M = (int)n1; g.N = nc; g.K = (int)n1;
     g.A = H11.get(); g.sam = n1; g.sak = 1;
@@ -414,31 +541,7 @@ void Handle::sigma_int(const double* Xc, int64_t ldx, double* Wc, int64_t ldw, i
     g.map.C = Wc; g.map.scm = 1; g.map.scn = ldw;
     gemm(g, Prec::F64, gws, s);
   }
-  {  // sigma1(a, k + o*vec) += sum_{c,bl} Z[c][vec][k][bl] V[a][c][bl], computed as its
-     // transpose (rows (vec,k), columns a) so that v = 200 fills a 208-column tile
-    GemmArgs g;
-    g.M = o * nc; g.N = v; g.K = (int)n1; g.batch = v; g.reduce_batch = 1;
-    g.A = ZG.get(); g.sam = n1; g.sak = 1; g.sab = (int64_t)nc * o * n1;
-    g.B = I.V.get(); g.sbn = (int64_t)v * n1; g.sbk = 1; g.sbb = n1;
-    g.map.kind = MAP_SIG1T; g.map.C = Wc; g.map.beta = 1.0;
-    g.map.q[0] = o; g.map.q[1] = ldw;
-    if (v > 192 && v <= 208) g.tile = 6;
-    g.tag = 1;
-    auto e = span(0, flops_v);
-    g.ev_begin = e.first; g.ev_end = e.second;
-    gemm(g, p, gws, s);
-  }
-  {  // sigma1(a*nc + vec, k) -= sum_{m,bl} Z[a][vec][m][bl] O[k][m][bl]
-    GemmArgs g;
-    g.M = v * nc; g.N = o; g.K = (int)(o * n1);
-    g.A = ZG.get(); g.sam = (int64_t)o * n1; g.sak = 1;
-    g.B = I.O.get(); g.sbn = (int64_t)o * n1; g.sbk = 1;
-    g.map.kind = MAP_SIG1O; g.map.C = Wc; g.map.alpha = -1.0; g.map.beta = 1.0;
-    g.map.q[0] = o; g.map.q[1] = ldw; g.map.q[2] = nc;
-    auto e = span(3, flops_o);
-    g.ev_begin = e.first; g.ev_end = e.second;
-    gemm(g, p, gws, s);
-  }
+  if (down) int_down(ws_zg, nc, p, Wc, ldw, s);
   // occupied up term: streaming kernel (nocc <= 32) or GEMM.  BF16 writes G
   // with the virtual contraction first and adds this term afterwards (no G
   // read in the GEMM epilogue); FP64/FP32 write this term first and let the
@@ -449,26 +552,27 @@ void Handle::sigma_int(const double* Xc, int64_t ldx, double* Wc, int64_t ldw, i
     bool ok = false;
     switch (p) {
       case Prec::F64:
-        ok = launch_occ_up<double>(Xc, ldx, nc, dk, (const double*)I.Ot.get(), (double*)ZG.get(), accumulate, s);
+        ok = launch_occ_up<double>(Xc, ldx, nc, dk, (const double*)I.Ot.get(), (double*)ws_zg, accumulate, s);
         break;
       case Prec::F32:
-        ok = launch_occ_up<float>(Xc, ldx, nc, dk, (const float*)I.Ot.get(), (float*)ZG.get(), accumulate, s);
+        ok = launch_occ_up<float>(Xc, ldx, nc, dk, (const float*)I.Ot.get(), (float*)ws_zg, accumulate, s);
         break;
       case Prec::BF16:
-        ok = launch_occ_up<__bf16>(Xc, ldx, nc, dk, (const __bf16*)I.Ot.get(), (__bf16*)ZG.get(), accumulate, s);
+        ok = launch_occ_up<__bf16>(Xc, ldx, nc, dk, (const __bf16*)I.Ot.get(), (__bf16*)ws_zg, accumulate, s);
         break;
     }
     if (e.second) HIP_CHECK(hipEventRecord(e.second, s));
     if (!ok && prof_on) prof.pop_back();  // nothing was launched between the two events
     return ok;
   };
+  if (!up) return;
   const bool occ_last = (p == Prec::BF16) && d.o <= 32;
   if (!occ_last && !occ_kernel(false)) {  // G[(c*nc + vec)][(k,bl)] = -sum_m x1(c,m) Ot[m][k][bl]   (overwrites Z)
     GemmArgs g;
     g.M = v * nc; g.N = (int)(o * n1); g.K = o;
-    g.A = X1c.get(); g.sam = o; g.sak = 1;
+    g.A = ws_x1c; g.sam = o; g.sak = 1;
     g.B = I.Ot.get(); g.sbk = (int64_t)o * n1; g.sbn = 1;
-    g.map.C = ZG.get(); g.map.scm = (int64_t)o * n1; g.map.scn = 1; g.map.alpha = -1.0;
+    g.map.C = ws_zg; g.map.scm = (int64_t)o * n1; g.map.scn = 1; g.map.alpha = -1.0;
     g.out_prec = out_lp;
     auto e = span(3, flops_o);
     g.ev_begin = e.first; g.ev_end = e.second;
@@ -479,9 +583,9 @@ void Handle::sigma_int(const double* Xc, int64_t ldx, double* Wc, int64_t ldw, i
     g.tile = 2;  // 64x64: K = v is short and the epilogue reads and writes G; measured
                  // ahead of 64x128 and 128x128 at o=16, v=200, 20 vectors
     g.M = o * nc; g.N = (int)n1; g.K = v; g.batch = v;
-    g.A = X1t.get(); g.sam = v; g.sak = 1; g.sab = 0;
+    g.A = ws_x1t; g.sam = v; g.sak = 1; g.sab = 0;
     g.B = I.Vt.get(); g.sbn = v; g.sbk = 1; g.sbb = n1 * v;
-    g.map.C = ZG.get(); g.map.scm = n1; g.map.scn = 1; g.map.scb = (int64_t)nc * o * n1;
+    g.map.C = ws_zg; g.map.scm = n1; g.map.scn = 1; g.map.scb = (int64_t)nc * o * n1;
     g.map.beta = occ_last ? 0.0 : 1.0;
     g.out_prec = out_lp;
     g.tag = 1;
@@ -492,7 +596,7 @@ void Handle::sigma_int(const double* Xc, int64_t ldx, double* Wc, int64_t ldw, i
   if (occ_last) occ_kernel(true);
   auto eu = span(2, (double)nc * (3.0 * d.n2 * 8.0 + (double)n1 * n1 * esd));
   if (eu.first) HIP_CHECK(hipEventRecord(eu.first, s));
-  unpack_chunk(p, ZG.get(), Xc, ldx, Wc, ldw, nc, dk, eps.get(), lay, s);
+  unpack_chunk(p, ws_zg, Xc, ldx, Wc, ldw, nc, dk, eps.get(), lay, s);
   if (eu.second) HIP_CHECK(hipEventRecord(eu.second, s));
 }
 
@@ -525,7 +629,7 @@ void Handle::sigma_df(const double* Xc, int64_t ldx, double* Wc, int64_t ldw, in
   const double flops_w = 2.0 * (v + o) * (double)naux * n1 * nc;  // each of the sigma1 and W contraction pairs
   auto ep = span(1, (double)nc * (d.n2 * 8.0 + (double)n1 * n1 * esd));
   if (ep.first) HIP_CHECK(hipEventRecord(ep.first, s));
-  pack_chunk(p, Xc, ldx, nc, dk, ZG.get(), X1c.get(), nullptr, lay, s);
+  pack_chunk(p, Xc, ldx, nc, dk, ws_zg, ws_x1c, nullptr, lay, s);
   if (ep.second) HIP_CHECK(hipEventRecord(ep.second, s));
   // sigma1 = H11 x1 (FP64)
   {
@@ -541,8 +645,8 @@ void Handle::sigma_df(const double* Xc, int64_t ldx, double* Wc, int64_t ldw, in
     GemmArgs g;
     g.M = naux; g.N = (int)(n1 * nc); g.K = (int)n1;
     g.A = BovP; g.sam = n1; g.sak = 1;
-    g.B = ZG.get(); g.sbk = 1; g.sbn = n1;
-    g.map.kind = MAP_DOWNY; g.map.C = Yvv.get(); g.map.C2 = Yoo.get();
+    g.B = ws_zg; g.sbk = 1; g.sbn = n1;
+    g.map.kind = MAP_DOWNY; g.map.C = ws_yvv; g.map.C2 = ws_yoo;
     g.map.q[0] = o; g.map.q[1] = nc; g.map.q[2] = n1; g.map.q[3] = naux;
     g.out_prec = out_lp;
     g.tag = 1;
@@ -557,7 +661,7 @@ void Handle::sigma_df(const double* Xc, int64_t ldx, double* Wc, int64_t ldw, in
     GemmArgs g;
     g.M = v; g.N = o * nc; g.K = v * naux;
     g.A = BvvP; g.sam = 1; g.sak = v;
-    g.B = Yvv.get(); g.sbk = (int64_t)o * nc; g.sbn = 1;
+    g.B = ws_yvv; g.sbk = (int64_t)o * nc; g.sbn = 1;
     g.map.kind = MAP_SIG1V; g.map.C = Wc; g.map.beta = 1.0;
     g.map.q[0] = o; g.map.q[1] = ldw;
     gemm(g, p, gws, s);
@@ -566,7 +670,7 @@ void Handle::sigma_df(const double* Xc, int64_t ldx, double* Wc, int64_t ldw, in
   {
     GemmArgs g;
     g.M = v * nc; g.N = o; g.K = o * naux;
-    g.A = Yoo.get(); g.sam = (int64_t)o * naux; g.sak = 1;
+    g.A = ws_yoo; g.sam = (int64_t)o * naux; g.sak = 1;
     g.B = BooP; g.sbk = o; g.sbn = 1;
     g.map.kind = MAP_SIG1O; g.map.C = Wc; g.map.alpha = -1.0; g.map.beta = 1.0;
     g.map.q[0] = o; g.map.q[1] = ldw; g.map.q[2] = nc;
@@ -577,8 +681,8 @@ void Handle::sigma_df(const double* Xc, int64_t ldx, double* Wc, int64_t ldw, in
     GemmArgs g;
     g.M = v * naux; g.N = o * nc; g.K = v;
     g.A = BvvP; g.sam = v; g.sak = 1;
-    g.B = X1c.get(); g.sbk = (int64_t)o * nc; g.sbn = 1;
-    g.map.kind = MAP_WUPV; g.map.C = Wup.get();
+    g.B = ws_x1c; g.sbk = (int64_t)o * nc; g.sbn = 1;
+    g.map.kind = MAP_WUPV; g.map.C = ws_wup;
     g.map.q[0] = o; g.map.q[1] = v; g.map.q[2] = n1; g.map.q[3] = nc;
     g.out_prec = out_w;
     gemm(g, p, gws, s);
@@ -586,9 +690,9 @@ void Handle::sigma_df(const double* Xc, int64_t ldx, double* Wc, int64_t ldw, in
   {
     GemmArgs g;
     g.M = v * nc; g.N = o * naux; g.K = o;
-    g.A = X1c.get(); g.sam = o; g.sak = 1;
+    g.A = ws_x1c; g.sam = o; g.sak = 1;
     g.B = BooP; g.sbk = 1; g.sbn = o;
-    g.map.kind = MAP_WUPO; g.map.C = Wup.get(); g.map.alpha = -1.0; g.map.beta = 1.0;
+    g.map.kind = MAP_WUPO; g.map.C = ws_wup; g.map.alpha = -1.0; g.map.beta = 1.0;
     g.map.q[0] = o; g.map.q[1] = v; g.map.q[2] = n1; g.map.q[3] = nc;
     g.out_prec = out_w;
     gemm(g, p, gws, s);
@@ -596,13 +700,13 @@ void Handle::sigma_df(const double* Xc, int64_t ldx, double* Wc, int64_t ldw, in
   // Wt[(vec,c,k)][P] = W[P][(vec,c,k)] in the GEMM precision
   switch (p) {
     case Prec::F64:
-      launch_transpose<double, double>((const double*)Wup.get(), naux, n1 * nc, (double*)WupT.get(), s);
+      launch_transpose<double, double>((const double*)ws_wup, naux, n1 * nc, (double*)ws_wupt, s);
       break;
     case Prec::F32:
-      launch_transpose<float, float>((const float*)Wup.get(), naux, n1 * nc, (float*)WupT.get(), s);
+      launch_transpose<float, float>((const float*)ws_wup, naux, n1 * nc, (float*)ws_wupt, s);
       break;
     case Prec::BF16:
-      launch_transpose<float, __bf16>((const float*)Wup.get(), naux, n1 * nc, (__bf16*)WupT.get(), s);
+      launch_transpose<float, __bf16>((const float*)ws_wup, naux, n1 * nc, (__bf16*)ws_wupt, s);
       break;
   }
   if (e1.second) HIP_CHECK(hipEventRecord(e1.second, s));
@@ -610,9 +714,9 @@ void Handle::sigma_df(const double* Xc, int64_t ldx, double* Wc, int64_t ldw, in
   {
     GemmArgs g;
     g.M = (int)(n1 * nc); g.N = (int)n1; g.K = naux;
-    g.A = WupT.get(); g.sam = naux; g.sak = 1;
+    g.A = ws_wupt; g.sam = naux; g.sak = 1;
     g.B = BovTP; g.sbk = 1; g.sbn = naux;
-    g.map.C = ZG.get(); g.map.scm = n1; g.map.scn = 1;
+    g.map.C = ws_zg; g.map.scm = n1; g.map.scn = 1;
     g.out_prec = out_lp;
     g.tag = 1;
     auto e = span(0, 2.0 * naux * (double)n1 * n1 * nc);
@@ -621,7 +725,7 @@ void Handle::sigma_df(const double* Xc, int64_t ldx, double* Wc, int64_t ldw, in
   }
   auto eu = span(2, (double)nc * (3.0 * d.n2 * 8.0 + (double)n1 * n1 * esd));
   if (eu.first) HIP_CHECK(hipEventRecord(eu.first, s));
-  unpack_chunk(p, ZG.get(), Xc, ldx, Wc, ldw, nc, dk, eps.get(), lay, s);
+  unpack_chunk(p, ws_zg, Xc, ldx, Wc, ldw, nc, dk, eps.get(), lay, s);
   if (eu.second) HIP_CHECK(hipEventRecord(eu.second, s));
 }
 

@@ -55,6 +55,14 @@ struct Handle {
   DevBuf<char> Wup;         // W[P][vec][(c,k)] (accumulator type: FP64, or FP32 for FP32/BF16)
   DevBuf<char> WupT;        // W transposed, [(vec,c,k)][P], GEMM precision
   DevBuf<char> ZG;          // Z then G, in the chunk's GEMM precision
+  // workspaces of the current sigma pass: the buffers above, or regions of
+  // a caller's scratch (sigma with `scratch` set allocates nothing)
+  void *ws_x1c = nullptr, *ws_x1t = nullptr, *ws_zg = nullptr, *ws_yvv = nullptr, *ws_yoo = nullptr,
+       *ws_wup = nullptr, *ws_wupt = nullptr;
+  char* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  bool lowprec_ready(Prec p) const;
+  int scratch_chunk(Prec p) const;
   GemmWorkspace gws;
   DevBuf<double> skinny_ws;  // per-block partials of the tall-skinny products
   int chunk = 32;           // vectors per sigma pass
@@ -99,8 +107,18 @@ struct Handle {
   void build_phph(const double* dB, int nbas);
   void ensure_lowprec(Prec p);
   void sigma(const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, Prec p, hipStream_t s);
+  // Half sigma-vectors for the coupled preconditioner: SIGMA_DOWN writes only
+  // the 1h1p rows W1 = H11 x1 + C x2, SIGMA_UP only the 2h2p rows
+  // W2 = C^T x1 + D x2 (the other rows of W are left undefined).  The INT form
+  // skips the contractions of the other half; the DF form and CVS views
+  // compute the whole sigma-vector.
+  enum { SIGMA_ALL = 0, SIGMA_DOWN = 1, SIGMA_UP = 2 };
+  int part = SIGMA_ALL;
+  void sigma_part(const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, Prec p, int which, hipStream_t s);
   void sigma_df(const double* X, int64_t ldx, double* W, int64_t ldw, int nc, Prec p, hipStream_t s);
   void sigma_int(const double* X, int64_t ldx, double* W, int64_t ldw, int nc, Prec p, hipStream_t s);
+  // INT form: W1 += C x2 for the nc pair tensors Z (virtual-major layout)
+  void int_down(const void* Z, int nc, Prec p, double* W1, int64_t ldw, hipStream_t s);
   size_t sigma_workspace_bytes(int nc, Prec p) const;
 };
 

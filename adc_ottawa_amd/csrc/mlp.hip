@@ -172,6 +172,7 @@ struct InferArgs {
   const double* hii;
   const double* Res;
   double* T;
+  const double* base;   // base step (ld as Res), or NULL for the DPR step
   const float* P;       // bf16-rounded weights as fp32
   const __bf16* P16;    // bf16 weights
   int* nonfinite;       // count of elements that fell back to DPR
@@ -223,11 +224,12 @@ __global__ void __launch_bounds__(256) infer_kernel(InferArgs a) {
     const bool dpr_only = a.use_dpr[col] != 0;
     const double* res = a.Res + a.ld * col;
     double* T = a.T + a.ld * col;
+    const double* bstep = a.base ? a.base + a.ld * col : nullptr;
     if (dpr_only) {
       if (li < 4) {
         for (int q = 0; q < 4; ++q) {
           const int64_t j = e0 + 4 * g + q;
-          if (li == q && j < a.n) T[j] = res[j] / guard_den2(th - a.hii[j]);
+          if (li == q && j < a.n) T[j] = bstep ? bstep[j] : res[j] / guard_den2(th - a.hii[j]);
         }
       }
       continue;
@@ -278,7 +280,7 @@ __global__ void __launch_bounds__(256) infer_kernel(InferArgs a) {
       if (li == 3) m = part[3];
       m += sb3;
       if (j < a.n) {
-        const double dstep = res[j] / guard_den2(th - a.hii[j]);
+        const double dstep = bstep ? bstep[j] : res[j] / guard_den2(th - a.hii[j]);
         if (isfinite(m)) {
           T[j] = dstep * (1.0 + (double)m);
         } else {
@@ -334,6 +336,7 @@ struct SampleArgs {
   const double* Xs;      // converged eigenvectors (ld)
   const int* match;      // column of Xs for each root
   const double* Res;
+  const double* bstep;   // base step of each root (ld), or NULL for the DPR step
   float* feat;
   int* cls;
   float* dn;
@@ -356,7 +359,7 @@ __global__ void sample_kernel(SampleArgs a) {
     }
     const double th = a.theta[r];
     const double rj = a.Res[j + a.ld * r];
-    const double dj = rj / guard_den2(th - a.hii[j]);
+    const double dj = a.bstep ? a.bstep[j + a.ld * r] : rj / guard_den2(th - a.hii[j]);
     const double ej = (a.X[j + a.ld * r] - a.Xs[j + a.ld * a.match[r]] / a.s[r]) / a.enorm[r];
     a.feat[3 * out] = asinhf((float)(a.hii[j] - th));
     a.feat[3 * out + 1] = (float)th;
@@ -566,7 +569,7 @@ void precon_train(Handle& H, const Handle& data, int epochs, int batch, double l
 }
 
 void precon_learned_step(Handle& H, int64_t n, int nc, const double* theta_d, const double* rnorm_h,
-                         const int* use_dpr_h, const double* Res, double* T, int64_t ld) {
+                         const int* use_dpr_h, const double* Res, double* T, int64_t ld, const double* base) {
   PreconNet& N = *H.net;
   DevBuf<double> rn;
   DevBuf<int> ud;
@@ -578,7 +581,7 @@ void precon_learned_step(Handle& H, int64_t n, int nc, const double* theta_d, co
   a.n = n; a.nc = nc; a.ld = ld;
   for (int t = 0; t < 6; ++t) a.off[t] = H.d.off[t];
   a.theta = theta_d; a.rnorm = rn.get(); a.use_dpr = ud.get();
-  a.hii = H.hii.get(); a.Res = Res; a.T = T;
+  a.hii = H.hii.get(); a.Res = Res; a.T = T; a.base = base;
   a.P = N.Pb.get(); a.P16 = N.P16.get(); a.nonfinite = N.nonfinite.get();
   const int64_t waves = std::min<int64_t>(((n + 15) / 16) * nc, (int64_t)2048 * 4);
   hipLaunchKernelGGL(infer_kernel, dim3((int)((waves + 3) / 4)), dim3(256), 0, H.st, a);
@@ -587,7 +590,7 @@ void precon_learned_step(Handle& H, int64_t n, int nc, const double* theta_d, co
 }
 
 void davtrace_iteration(Handle& H, DavTrace& tr, int it, int bs_curr, int ns_curr, const double* theta_h,
-                        const double* norm_h, const double* X, const double* Res, int64_t n) {
+                        const double* norm_h, const double* X, const double* Res, int64_t n, const double* base) {
   (void)bs_curr;
   if (tr.added >= tr.max_records) return;
   const int nr = ns_curr;  // active (not yet deflated) roots
@@ -641,7 +644,7 @@ void davtrace_iteration(Handle& H, DavTrace& tr, int it, int bs_curr, int ns_cur
   a.seed = tr.seed + (uint64_t)it * 0x9E3779B97F4A7C15ull;
   a.theta = tr.dbuf.get(); a.rnorm = tr.dbuf.get() + nr; a.s = tr.dbuf.get() + 2 * nr;
   a.enorm = tr.dbuf.get() + 3 * nr; a.active = tr.active.get();
-  a.hii = H.hii.get(); a.X = X; a.Xs = tr.Xs.get(); a.match = tr.match.get(); a.Res = Res;
+  a.hii = H.hii.get(); a.X = X; a.Xs = tr.Xs.get(); a.match = tr.match.get(); a.Res = Res; a.bstep = base;
   a.feat = N.feat.get(); a.cls = N.cls.get(); a.dn = N.dn.get(); a.en = N.en.get();
   a.base = N.nrec;
   hipLaunchKernelGGL(sample_kernel, dim3(grid_for(cnt)), dim3(256), 0, H.st, a);
@@ -677,9 +680,11 @@ void davtrace_iteration(Handle& H, DavTrace& tr, int it, int bs_curr, int ns_cur
 }
 
 void precon_harvest(Handle& H, const adc2g_dav_params& p, int64_t max_records, int64_t* added) {
-  // pass 1: FP64 DPR Davidson (with deflation) to convergence -> x*
+  // pass 1: FP64 Davidson with the base step (DPR, or the coupled step for
+  // the coupled preconditioners), with deflation, to convergence -> x*
   adc2g_dav_params q = p;
-  q.precon = ADC2G_PRE_DPR;
+  q.precon = (p.precon == ADC2G_PRE_COUPLED || p.precon == ADC2G_PRE_LEARNED_COUPLED) ? ADC2G_PRE_COUPLED
+                                                                                       : ADC2G_PRE_DPR;
   q.deflate = 1;
   q.precision = ADC2G_PREC_FP64;
   q.verbose = 0;

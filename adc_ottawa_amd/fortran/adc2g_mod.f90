@@ -19,7 +19,8 @@ module adc2g_mod
 
   integer(c_int32_t), parameter :: ADC2G_OK = 0
   integer(c_int32_t), parameter :: ADC2G_PREC_FP64 = 0, ADC2G_PREC_FP32 = 1, ADC2G_PREC_BF16 = 2
-  integer(c_int32_t), parameter :: ADC2G_PRE_DPR = 1, ADC2G_PRE_OLSEN = 2, ADC2G_PRE_LEARNED = 3
+  integer(c_int32_t), parameter :: ADC2G_PRE_DPR = 1, ADC2G_PRE_OLSEN = 2, ADC2G_PRE_LEARNED = 3, &
+       ADC2G_PRE_COUPLED = 4, ADC2G_PRE_LEARNED_COUPLED = 5
   integer(c_int32_t), parameter :: ADC2G_COUPLING_AUTO = 0, ADC2G_COUPLING_DF = 1, ADC2G_COUPLING_INT = 2
   integer(c_int32_t), parameter :: ADC2G_GUESS_ONDIAG = 0, ADC2G_GUESS_SUBDIAG = 1, ADC2G_GUESS_VECTORS = 2
   real(c_double), parameter :: adc2g_eh2ev = 27.2113845d0       ! parameters.f90:192
@@ -54,6 +55,9 @@ module adc2g_mod
      integer(c_int64_t) :: nmult_ref
      integer(c_int32_t) :: niter_lowprec, nfallback
      real(c_double)     :: time_total, time_sigma, resnorm_max
+     real(c_double)     :: time_precon
+     integer(c_int32_t) :: nprecon_builds, reserved
+     real(c_double)     :: precon_sigma
   end type adc2g_dav_stats
 
   type, bind(c) :: adc2g_sil_params

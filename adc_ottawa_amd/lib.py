@@ -23,7 +23,7 @@ ERRORS = {1: "SIL illegal integration order", 2: "SIL stepsize underflow",
           15: "call out of order", 16: "block Lanczos breakdown"}
 
 PREC = {"fp64": 0, "fp32": 1, "bf16": 2}
-PRECON = {"dpr": 1, "olsen": 2, "learned": 3}
+PRECON = {"dpr": 1, "olsen": 2, "learned": 3, "coupled": 4, "learned_coupled": 5}
 
 
 class Adc2gError(RuntimeError):
@@ -54,7 +54,8 @@ GUESS = {"ondiag": 0, "subdiag": 1, "vectors": 2}
 class DavStats(C.Structure):
     _fields_ = [("niter", C.c_int32), ("nsigma", C.c_int32), ("nmult_ref", C.c_int64),
                 ("niter_lowprec", C.c_int32), ("nfallback", C.c_int32), ("time_total", C.c_double),
-                ("time_sigma", C.c_double), ("resnorm_max", C.c_double)]
+                ("time_sigma", C.c_double), ("resnorm_max", C.c_double), ("time_precon", C.c_double),
+                ("nprecon_builds", C.c_int32), ("reserved", C.c_int32), ("precon_sigma", C.c_double)]
 
 
 class SilParams(C.Structure):
@@ -101,7 +102,7 @@ SYMBOLS = [
     "adc2g_sigma_profile", "adc2g_precon_absorb", "adc2g_coupling_form", "adc2g_cheby_auto",
     "adc2g_block_lanczos", "adc2g_relax", "adc2g_write_hmlt", "adc2g_davidson_vectors",
     "adc2g_sil_autocorr_orders", "adc2g_create_cvs", "adc2g_mp2", "adc2g_fvector", "adc2g_lanczos_guess",
-    "adc2g_spectral_bounds", "adc2g_power",
+    "adc2g_spectral_bounds", "adc2g_power", "adc2g_coupled_schur",
 ]
 
 _lib = None
@@ -124,6 +125,7 @@ def load(path: str = LIBPATH):
     lib.adc2g_dims.argtypes = [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]
     lib.adc2g_diagonal.argtypes = [vp, vp]
     lib.adc2g_phph.argtypes = [vp, vp]
+    lib.adc2g_coupled_schur.argtypes = [vp, C.c_double, i32, vp]
     lib.adc2g_sigma.argtypes = [vp, i32, vp, vp, i32]
     lib.adc2g_sigma_device.argtypes = [vp, i32, vp, vp, i32, vp]
     lib.adc2g_matxvec_treal.argtypes = [vp, vp, vp, i32]
@@ -228,6 +230,12 @@ class Operator:
     def phph(self):
         out = np.empty((self.n1, self.n1))
         check(_lib.adc2g_phph(self._h, _ptr(out)))
+        return out
+
+    def coupled_schur(self, sigma, precision="fp64"):
+        """A0 = H11 - C (D - sigma)^-1 C^T (n1 x n1) of the coupled preconditioner."""
+        out = np.empty((self.n1, self.n1), order="F")
+        check(_lib.adc2g_coupled_schur(self._h, float(sigma), PREC[precision], _ptr(out)))
         return out
 
     def sigma(self, X, precision="fp64"):
@@ -429,9 +437,10 @@ class Operator:
         return {n: tuple(out[3 * i:3 * i + 3]) for i, n in enumerate(names)}
 
     # ---- learned preconditioner
-    def precon_harvest(self, nstates, max_records=1 << 40, **kw):
+    def precon_harvest(self, nstates, max_records=1 << 40, base="dpr", **kw):
+        """Training records with the network's base step: "dpr" or "coupled"."""
         p = DavParams(nstates=nstates, blocksize=kw.get("blocksize", 0), maxvec=kw.get("maxvec", -1),
-                      maxiter=kw.get("maxiter", 200), tol=kw.get("tol", 1e-7), precon=1)
+                      maxiter=kw.get("maxiter", 200), tol=kw.get("tol", 1e-7), precon=PRECON[base])
         added = C.c_int64()
         check(_lib.adc2g_precon_harvest(self._h, C.byref(p), max_records, C.byref(added)))
         return added.value

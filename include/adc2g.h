@@ -57,6 +57,10 @@ extern "C" {
 #define ADC2G_PRE_DPR 1         /* diagonal preconditioned residue */
 #define ADC2G_PRE_OLSEN 2       /* Olsen */
 #define ADC2G_PRE_LEARNED 3     /* MLP correction on top of DPR, DPR fallback */
+#define ADC2G_PRE_COUPLED 4     /* exact inverse of H - theta with the 2h2p diagonal shifted by a fixed
+                                   sigma, through the 1h1p Schur complement (no counterpart in the
+                                   reference, whose steps are diagonal) */
+#define ADC2G_PRE_LEARNED_COUPLED 5  /* MLP correction on top of the coupled step, coupled-step fallback */
 
 /* Evaluation of the 1h1p<->2h2p coupling (same matrix either way) */
 #define ADC2G_COUPLING_AUTO 0   /* INT when it needs fewer flops and its integrals fit, else DF */
@@ -106,6 +110,10 @@ typedef struct {
   double time_total;     /* seconds, wall */
   double time_sigma;     /* seconds inside sigma-vectors */
   double resnorm_max;    /* max FP64 residual norm of the returned roots */
+  double time_precon;    /* seconds in the coupled preconditioner (set-up and steps) */
+  int32_t nprecon_builds;/* Schur complements formed by the coupled preconditioner */
+  int32_t reserved;
+  double precon_sigma;   /* shift of the 2h2p diagonal the coupled preconditioner used */
 } adc2g_dav_stats;
 
 typedef struct {
@@ -221,6 +229,15 @@ int32_t adc2g_diagonal(adc2g_handle* h, double* hii);
  * assembled by get_offdiag_adc2_save_omp (source/adclib/get_matrix.f90:848-884)
  * plus the diagonal of get_diag_adc2_save. */
 int32_t adc2g_phph(adc2g_handle* h, double* h11);
+
+/* The 1h1p Schur complement of the coupled preconditioner (ADC2G_PRE_COUPLED;
+ * no counterpart in the reference): a0 (n1 x n1, column-major) receives
+ * A0 = H11 - C (D - sigma)^-1 C^T, with C the 1h1p<->2h2p coupling block and
+ * D the 2h2p diagonal, for a shift sigma below every 2h2p diagonal element.
+ * The contractions with C run in the given precision (ADC2G_PREC_*), the sum
+ * in FP64.  The number of eigenvalues of A0 below sigma equals the number of
+ * eigenvalues of H below sigma (Sylvester's law of inertia). */
+int32_t adc2g_coupled_schur(adc2g_handle* h, double sigma, int32_t precision, double* a0);
 
 /* ---- sigma-vectors ----------------------------------------------------- */
 
@@ -440,10 +457,12 @@ int32_t adc2g_write_hmlt(adc2g_handle* h, const char* dia_path, const char* off_
 
 /* ---- learned preconditioner -------------------------------------------- */
 
-/* Run FP64 Davidson on this handle's system with the reference DPR step and
- * append (features, DPR step, exact correction) records of every unconverged
- * root and iteration to the handle's training set (kept in HBM); at most
- * max_records are kept per call.  Returns the number added. */
+/* Run FP64 Davidson on this handle's system with the base step of the
+ * network (the reference DPR step, or the coupled step when p->precon is
+ * ADC2G_PRE_COUPLED or ADC2G_PRE_LEARNED_COUPLED) and append (features, base
+ * step, exact correction) records of every unconverged root and iteration to
+ * the handle's training set (kept in HBM); at most max_records are kept per
+ * call.  Returns the number added. */
 int32_t adc2g_precon_harvest(adc2g_handle* h, const adc2g_dav_params* p, int64_t max_records,
                              int64_t* added);
 

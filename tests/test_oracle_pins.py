@@ -17,14 +17,17 @@ in ``oracle/`` is checked here against an independent fact:
   quadrature and the full-Krylov spectrum;
 * relaxation: each timestep against exp(-H dtau) from scipy.linalg.expm, and
   the relaxed states against a dense eigendecomposition;
-* network: analytic gradients match central finite differences.
+* network: analytic gradients match central finite differences;
+* coupled preconditioner: the step solves the shifted system it stands for,
+  and the Schur complement's inertia counts the eigenvalues of H below the
+  shift.
 """
 import numpy as np
 import pytest
 import scipy.sparse.linalg as sla
 
 from adc_ottawa_amd.synth import make_system
-from oracle import cheby, csf, davidson, dfsigma, elements, fvec, lanczos, mlp, mp2, power, relax, schur, sil
+from oracle import cheby, coupled, csf, davidson, dfsigma, elements, fvec, lanczos, mlp, mp2, power, relax, schur, sil
 
 
 def test_csf_counts_and_bijectivity():
@@ -567,3 +570,36 @@ def test_oracle_rkf45_relaxation():
         assert min(np.abs(v - U[:, 0]).max(), np.abs(v + U[:, 0]).max()) < 1e-4
     with pytest.raises(RuntimeError, match="Error code: 4"):
         relax.rkf45_relax(lambda x: H @ x, g, maxiter=300, step=2.0, eps=1e-6, toler=1e-8)
+
+
+def test_oracle_coupled():
+    """The coupled step solves M(theta) t = r, M = [[H11 - theta, C], [C^T, D - sigma]],
+    and #eig(A0(sigma)) < sigma equals #eig(H) < sigma for several sigma below
+    min D (Sylvester's law of inertia), against a dense eigh of H.  A dropped
+    P, a transposed C or a wrong sign in A0 fails the inertia count."""
+    sys_ = make_system(3, 8, 10, seed=4)
+    df = dfsigma.DF(sys_)
+    C = coupled.coupling_matrix(df)
+    H11, D = df.H11, df.D2
+    n1 = df.n1
+    H = np.block([[H11, C], [C.T, np.diag(D)]])
+    assert np.abs(H - df.sigma(np.eye(H.shape[0]))).max() < 1e-12 * np.abs(H).max()
+    w = np.linalg.eigvalsh(H)
+    dmin = D.min()
+    counts = []
+    for sig in np.linspace(w[0] - 0.5, dmin - 1e-3, 9):
+        A0 = coupled.schur_complement(H11, C, D, sig)
+        assert coupled.count_below(A0, sig) == int(np.sum(w < sig))
+        counts.append(int(np.sum(w < sig)))
+    assert max(counts) > 1          # the shifts do separate roots
+    rng = np.random.default_rng(2)
+    R = rng.standard_normal((H.shape[0], 3))
+    theta = np.array([w[0] + 0.1, w[1] - 0.05, w[2]])
+    sig = dmin - 0.2
+    T = coupled.coupled_step(H11, C, D, sig, theta, R)
+    for i in range(3):
+        M = H - theta[i] * np.eye(H.shape[0])
+        M[n1:, n1:] += (theta[i] - sig) * np.eye(H.shape[0] - n1)
+        assert np.abs(M @ T[:, i] - R[:, i]).max() < 1e-10 * np.abs(R[:, i]).max()
+    with pytest.raises(ValueError):
+        coupled.schur_complement(H11, C, D, dmin + 1e-3)
