@@ -163,6 +163,17 @@ int32_t adc2g_sigma_device(adc2g_handle* h, int32_t nvec, const double* dX, doub
   });
 }
 
+int32_t adc2g_sigma_device_noalloc(adc2g_handle* h, int32_t nvec, const double* dX, double* dW, int32_t precision,
+                                   void* stream) {
+  return guarded([&] {
+    ARG_CHECK(h && dX && dW, "NULL argument");
+    ARG_CHECK(nvec >= 0, "nvec must be >= 0");
+    Handle& H = h->h;
+    HIP_CHECK(hipSetDevice(H.device));
+    sigma_noalloc(H, nvec, dX, dW, prec_from_int(precision), (hipStream_t)stream);
+  });
+}
+
 int32_t adc2g_matxvec_treal(adc2g_handle* h, const double* v1, double* v2, int32_t precision) {
   return guarded([&] {
     ARG_CHECK(h && v1 && v2, "NULL argument");

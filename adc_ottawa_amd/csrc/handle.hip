@@ -101,6 +101,7 @@ void Handle::create(int o, int v, int naux, const double* eps_h, const double* B
   const size_t budget = (size_t)12 << 30;
   chunk = 64;
   while (chunk > 1 && sigma_workspace_bytes(chunk, Prec::F64) > budget) chunk /= 2;
+  resolve_scratch(device);
 }
 
 static int prec_index(Prec p) { return p == Prec::F64 ? 0 : (p == Prec::F32 ? 1 : 2); }

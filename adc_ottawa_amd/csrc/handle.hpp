@@ -124,5 +124,11 @@ struct Handle {
 
 Prec prec_from_int(int p);
 void precon_net_free(Handle& H);
+// The no-allocation sigma-vector (noalloc.hip): the device's scratch address
+// is resolved once per device (at handle creation); sigma_noalloc runs
+// H.sigma with every workspace inside that scratch.
+void resolve_scratch(int device);
+size_t scratch_capacity();
+void sigma_noalloc(Handle& H, int nvec, const double* dX, double* dW, Prec p, hipStream_t s);
 
 }  // namespace adc2g

@@ -102,7 +102,7 @@ SYMBOLS = [
     "adc2g_sigma_profile", "adc2g_precon_absorb", "adc2g_coupling_form", "adc2g_cheby_auto",
     "adc2g_block_lanczos", "adc2g_relax", "adc2g_write_hmlt", "adc2g_davidson_vectors",
     "adc2g_sil_autocorr_orders", "adc2g_create_cvs", "adc2g_mp2", "adc2g_fvector", "adc2g_lanczos_guess",
-    "adc2g_spectral_bounds", "adc2g_power", "adc2g_coupled_schur",
+    "adc2g_spectral_bounds", "adc2g_power", "adc2g_coupled_schur", "adc2g_sigma_device_noalloc",
 ]
 
 _lib = None
@@ -128,6 +128,7 @@ def load(path: str = LIBPATH):
     lib.adc2g_coupled_schur.argtypes = [vp, C.c_double, i32, vp]
     lib.adc2g_sigma.argtypes = [vp, i32, vp, vp, i32]
     lib.adc2g_sigma_device.argtypes = [vp, i32, vp, vp, i32, vp]
+    lib.adc2g_sigma_device_noalloc.argtypes = [vp, i32, vp, vp, i32, vp]
     lib.adc2g_matxvec_treal.argtypes = [vp, vp, vp, i32]
     lib.adc2g_davidson.argtypes = [vp, C.POINTER(DavParams), vp, vp, C.POINTER(DavStats)]
     lib.adc2g_sil_autocorr.argtypes = [vp, C.POINTER(SilParams), vp, vp, i64, C.POINTER(i64),

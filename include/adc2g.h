@@ -249,6 +249,21 @@ int32_t adc2g_sigma(adc2g_handle* h, int32_t nvec, const double* X, double* W, i
 int32_t adc2g_sigma_device(adc2g_handle* h, int32_t nvec, const double* dX, double* dW,
                            int32_t precision, void* stream);
 
+/* The same as adc2g_sigma_device, allocating nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  Every workspace (pair tensors, packed
+ * 1h1p blocks, GEMM split-K partials) lies in one static device array of
+ * 256 MiB per GPU that the library holds until the process ends; the vectors
+ * go through it in passes of as many columns as fit (at o=16, v=200: 2 FP64,
+ * 4 FP32 or 9 BF16 vectors per pass).  ADC2G_ENOMEM when one vector's
+ * workspace does not fit (n1^2 elements of the precision plus its partials,
+ * e.g. FP64 beyond n1 of about 4000); ADC2G_EARG for a CVS view or a
+ * precision whose operands are not resident yet (create the handle with
+ * keep_lowprec = 1, or run one adc2g_sigma_device of that precision first).
+ * Calls on one GPU share the array: calls that overlap on one GPU race. */
+int32_t adc2g_sigma_device_noalloc(adc2g_handle* h, int32_t nvec, const double* dX, double* dW,
+                                   int32_t precision, void* stream);
+
 /* v2 = -i H v1 for one complex vector (interleaved re/im, length ndim):
  * matxvec_treal (source/propagation/tdselib.f90:323-437). */
 int32_t adc2g_matxvec_treal(adc2g_handle* h, const double* v1, double* v2, int32_t precision);
