@@ -24,8 +24,9 @@ Pinned in tests/test_oracle_pins.py by independent routes:
   and the oo/uu blocks equal the closed-shell textbook forms
   P_ij = -2 sum t_ik^ab (2 t_jk^ab - t_jk^ba) and
   P_ab = 2 sum t_ij^ac (2 t_ij^bc - t_ij^cb).
-The ou block has no such check here: parity unpinned beyond the GPU-oracle
-comparison.
+* every block, the ou block included, and the energy against exact
+  Rayleigh-Schroedinger perturbation theory in the determinant space of the
+  same Hamiltonian (tests/detspace.py, test_oracle_mp2_density_brute_force).
 """
 from __future__ import annotations
 

@@ -1,0 +1,137 @@
+"""Brute-force determinant-space perturbation theory for tiny closed shells.
+Test helper (a checker of the oracle, independent of the reference's
+formulas).
+
+The Hamiltonian of a synthetic system (orbital energies eps, integrals
+(pq|rs) = sum_P B[P,p,q] B[P,r,s]) is built in second quantisation over the
+Sz = 0 determinants of 2*nocc electrons in 2*nbas spin orbitals, with the
+one-electron part chosen so that the Fock matrix is diag(eps):
+    h_pq = eps_p delta_pq - sum_j [2 (pq|jj) - (pj|jq)],
+    H = sum_{pq,s} h_pq a+_ps a_qs + 1/2 sum_{pqrs,s,t} (pq|rs) a+_ps a+_rt a_st a_qs.
+With H0 = sum_p eps_p n_p (diagonal in determinants) and V = H - H0,
+Rayleigh-Schroedinger perturbation theory in intermediate normalisation is
+done exactly by linear algebra in the determinant space:
+    Psi1 = R V Phi0,  Psi2 = R (V - E1) Psi1,  R = Q (E0 - H0)^-1 Q,
+    E2 = <Phi0|V|Psi1>,
+    rho2 = <Psi1|g|Psi1> + <Phi0|g|Psi2> + <Psi2|g|Phi0> - <Psi1|Psi1> <Phi0|g|Phi0>
+for the spin-summed one-particle density g_pq = sum_s a+_ps a_qs.
+"""
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+
+
+def _sign_before(det: int, k: int) -> int:
+    return -1 if bin(det & ((1 << k) - 1)).count("1") % 2 else 1
+
+
+def _annihilate(det, k):
+    if not (det >> k) & 1:
+        return None, 0
+    return det ^ (1 << k), _sign_before(det, k)
+
+
+def _create(det, k):
+    if (det >> k) & 1:
+        return None, 0
+    return det | (1 << k), _sign_before(det, k)
+
+
+class DetSpace:
+    """Sz = 0 determinants of nocc alpha and nocc beta electrons in nbas
+    spatial orbitals; spin orbital 2p + s (s = 0 alpha, 1 beta)."""
+
+    def __init__(self, nocc: int, nbas: int):
+        self.nocc, self.nbas = nocc, nbas
+        dets = []
+        for a in itertools.combinations(range(nbas), nocc):
+            for b in itertools.combinations(range(nbas), nocc):
+                d = 0
+                for p in a:
+                    d |= 1 << (2 * p)
+                for p in b:
+                    d |= 1 << (2 * p + 1)
+                dets.append(d)
+        self.dets = dets
+        self.index = {d: i for i, d in enumerate(dets)}
+        self.ref = self.index[sum((1 << (2 * p)) | (1 << (2 * p + 1)) for p in range(nocc))]
+
+    def one_body(self, p: int, q: int) -> np.ndarray:
+        """Matrix of g_pq = sum_s a+_ps a_qs."""
+        n = len(self.dets)
+        M = np.zeros((n, n))
+        for j, d in enumerate(self.dets):
+            for s in (0, 1):
+                d1, s1 = _annihilate(d, 2 * q + s)
+                if d1 is None:
+                    continue
+                d2, s2 = _create(d1, 2 * p + s)
+                if d2 is None:
+                    continue
+                M[self.index[d2], j] += s1 * s2
+        return M
+
+    def hamiltonian(self, h: np.ndarray, eri: np.ndarray) -> np.ndarray:
+        nb = self.nbas
+        n = len(self.dets)
+        H = np.zeros((n, n))
+        for p in range(nb):
+            for q in range(nb):
+                if h[p, q] != 0.0:
+                    H += h[p, q] * self.one_body(p, q)
+        for j, d in enumerate(self.dets):
+            for q, s, r, t in itertools.product(range(nb), (0, 1), range(nb), (0, 1)):
+                # a+_ps a+_rt a_st a_qs: annihilate q_s then s_t, create r_t then p_s
+                d1, c1 = _annihilate(d, 2 * q + s)
+                if d1 is None:
+                    continue
+                for ss in range(nb):
+                    d2, c2 = _annihilate(d1, 2 * ss + t)
+                    if d2 is None:
+                        continue
+                    d3, c3 = _create(d2, 2 * r + t)
+                    if d3 is None:
+                        continue
+                    for p in range(nb):
+                        val = eri[p, q, r, ss]
+                        if val == 0.0:
+                            continue
+                        d4, c4 = _create(d3, 2 * p + s)
+                        if d4 is None:
+                            continue
+                        H[self.index[d4], j] += 0.5 * val * c1 * c2 * c3 * c4
+        return H
+
+
+def mp_second_order(eps: np.ndarray, eri: np.ndarray, nocc: int):
+    """(E2, rho0, rho2) of the closed shell by exact RSPT in the determinant space."""
+    nb = eps.size
+    o = nocc
+    h = np.diag(eps).astype(np.float64)
+    h -= 2.0 * np.einsum("pqjj->pq", eri[:, :, :o, :o]) - np.einsum("pjjq->pq", eri[:, :o, :o, :])
+    ds = DetSpace(o, nb)
+    H = ds.hamiltonian(h, eri)
+    occ_energy = np.array([sum(eps[k // 2] for k in range(2 * nb) if (d >> k) & 1) for d in ds.dets])
+    E0 = occ_energy[ds.ref]
+    V = H - np.diag(occ_energy)
+    n = len(ds.dets)
+    R = np.zeros(n)
+    mask = np.arange(n) != ds.ref
+    R[mask] = 1.0 / (E0 - occ_energy[mask])
+    phi0 = np.zeros(n)
+    phi0[ds.ref] = 1.0
+    psi1 = R * (V @ phi0)
+    E1 = phi0 @ V @ phi0
+    psi2 = R * (V @ psi1 - E1 * psi1)
+    E2 = phi0 @ V @ psi1
+    rho0 = np.zeros((nb, nb))
+    rho2 = np.zeros((nb, nb))
+    n11 = psi1 @ psi1
+    for p in range(nb):
+        for q in range(nb):
+            g = ds.one_body(p, q)
+            rho0[p, q] = phi0 @ g @ phi0
+            rho2[p, q] = psi1 @ g @ psi1 + phi0 @ g @ psi2 + psi2 @ g @ phi0 - n11 * rho0[p, q]
+    return E2, rho0, rho2

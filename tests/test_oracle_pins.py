@@ -603,3 +603,27 @@ def test_oracle_coupled():
         assert np.abs(M @ T[:, i] - R[:, i]).max() < 1e-10 * np.abs(R[:, i]).max()
     with pytest.raises(ValueError):
         coupled.schur_complement(H11, C, D, dmin + 1e-3)
+
+
+@pytest.mark.parametrize("o,v,naux,seed", [(2, 3, 6, 3), (2, 4, 5, 8), (3, 3, 6, 1)])
+def test_oracle_mp2_density_brute_force(o, v, naux, seed):
+    """The MP2 energy and every block of the second-order density -- the oo,
+    uu and ou blocks of rho_mp2 (source/adclib/mp2.f90:421-596) -- against
+    exact Rayleigh-Schroedinger perturbation theory in the determinant space
+    of the same Hamiltonian (tests/detspace.py: Psi1 = R V Phi0,
+    Psi2 = R (V - E1) Psi1, rho2 from <Psi|g|Psi>), which shares nothing with
+    the restated formulas.  The ou block comes from the singles of Psi2, so a
+    wrong denominator, sign or exchange term in rhogs2_ou fails here."""
+    from tests.detspace import mp_second_order
+    sys_ = make_system(o, v, naux, seed=seed)
+    V = mp2.integrals(sys_)
+    E2, rho0, rho2 = mp_second_order(sys_.eps, V, o)
+    assert abs(E2 - mp2.energy(sys_, V)) < 1e-13 * abs(E2)
+    assert np.abs(rho0 - np.diag([2.0] * o + [0.0] * v)).max() == 0.0
+    _, blk = mp2.density(sys_, V)
+    scale = np.abs(rho2).max()
+    assert np.abs(rho2[:o, :o] - blk["oo"]).max() < 1e-12 * scale
+    assert np.abs(rho2[o:, o:] - blk["uu"]).max() < 1e-12 * scale
+    assert np.abs(rho2[:o, o:] - blk["ou"]).max() < 1e-12 * scale
+    assert np.abs(rho2[o:, :o] - blk["ou"].T).max() < 1e-12 * scale
+    assert np.abs(blk["ou"]).max() > 1e-3 * scale     # the ou block is not trivially zero
