@@ -551,13 +551,18 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
         }
       }
       if (learned) {
-        // learned step on the base step (DPR or coupled); a root whose
-        // residual missed a 10 % drop in two iterations keeps the base step
-        // from then on
+        // learned step on the base step (DPR or coupled); a root that falls
+        // short of the base step's usual progress keeps the base step from
+        // then on: with DPR (slow, a few % per iteration) when its residual
+        // missed a 10 % drop in two iterations, with the coupled step (close
+        // to shift-and-invert, a factor of several per iteration) when it
+        // missed halving once
+        const double need = coupled ? 0.5 : 0.9;
+        const int patience = coupled ? 1 : 2;
         std::vector<int> use_base(bs_curr, 0);
         for (int i = 0; i < bs_curr; ++i) {
-          if (norm[i] > 0.9 * prev_norm[i]) ++stall[i]; else stall[i] = 0;
-          if (stall[i] >= 2) sticky[i] = 1;
+          if (norm[i] > need * prev_norm[i]) ++stall[i]; else stall[i] = 0;
+          if (stall[i] >= patience) sticky[i] = 1;
           if (sticky[i]) {
             use_base[i] = 1;
             ++nfallback;

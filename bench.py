@@ -39,6 +39,7 @@ METRIC = "sigma-vector GB/s & time-to-20-roots (≤1e-7 Eh): learned vs diagonal
 PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # dense MFMA TFLOP/s, MI355X
 HBM_PEAK = 8000.0  # GB/s
 WEIGHTS = os.path.join(ROOT, "adc_ottawa_amd", "precon_weights.npy")
+WEIGHTS_COUPLED = os.path.join(ROOT, "adc_ottawa_amd", "precon_weights_coupled.npy")
 # HBM bytes per launch of the coupling contractions, measured with rocprofv3
 # FETCH_SIZE / WRITE_SIZE counter runs of this benchmark (scaled by a copy of
 # known size); written next to the rocprof summaries it comes from.
@@ -154,6 +155,57 @@ def stream_stats(prof):
     return out
 
 
+def host_info():
+    """The host the CPU timings ran on: logical CPUs and the model name."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"logical_cpus": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "model": model}
+
+
+def cpu_time_to_roots(nroots=5):
+    """Water-sized system (o=5, v=36, naux=164, the reference's CPU-runnable
+    case): time to nroots roots (block nroots, subspace 4*block, residual
+    1e-7, no deflation) of the reference's algorithm on the host -- the
+    Davidson restatement (oracle/davidson.py: all currdim sigma-vectors every
+    iteration, QR of the subspace) with DPR on the full stored matrix through
+    the C/OpenMP hxvec_incore restatement -- beside the GPU Davidson on the
+    same system with DPR, Olsen (the reference's default) and the coupled
+    step."""
+    from oracle import cpu_baseline as cb  # the checker / baseline, never the product
+    from oracle import davidson as odav
+    s0 = make_system(5, 36, 164, seed=0)
+    out = {"system": "o=5 v=36 naux=164 seed 0", "roots": nroots, "host": host_info()}
+    with lib.Operator.from_system(s0) as op0:
+        h11, hii = op0.phph(), op0.diagonal()
+        for name, pc in (("dpr", "dpr"), ("olsen", "olsen"), ("coupled", "coupled")):
+            t0 = time.perf_counter()
+            ev, _, st = op0.davidson(nroots, blocksize=nroots, maxvec=4 * nroots, tol=1e-7, precon=pc,
+                                     deflate=False, maxiter=2000, vectors=False)
+            out["gpu_" + name] = {"seconds": round(time.perf_counter() - t0, 4), "iterations": st["niter"],
+                                  "sigma_vectors": st["nsigma"]}
+            if name == "dpr":
+                ev_gpu = ev
+    coo = cb.COO(s0, h11=h11)
+    sigma = lambda V: coo.hxvec(hii, V)[0]  # noqa: E731
+    for name, pc in (("dpr", 1),):
+        t0 = time.perf_counter()
+        r = odav.davidson(sigma, hii, nroots, blocksize=nroots, maxvec=4 * nroots, tol=1e-7, precon=pc,
+                          deflate=False, maxiter=2000)
+        wall = time.perf_counter() - t0
+        out["cpu_" + name] = {"seconds": round(wall, 3), "iterations": r["niter"], "sigma_vectors": r["nmult"],
+                              "max_abs_err_vs_gpu_Eh": float(np.abs(r["eigval"] - ev_gpu).max())}
+        log(f"cpu time-to-{nroots}-roots ({name}, water size): {wall:.2f}s, {r['niter']} iterations")
+    return out
+
+
 def cpu_baseline(sys_, op, nvec):
     from oracle import cpu_baseline as cb  # the checker / baseline, never the product
     cb.build()
@@ -173,7 +225,7 @@ def cpu_baseline(sys_, op, nvec):
     log(f"cpu baseline: sample {coo.rows} rows, nnz {coo.nnz} of {coo.full_nnz}, build {tb:.1f}s, "
         f"hxvec offdiag {t_off:.2f}s diag {t_diag:.2f}s -> full block {t_full:.1f}s")
     return {"value": round(nvec / t_full, 4), "unit": "sigma-vectors/s", "cores": min(cores, nvec),
-            "kind": "port",
+            "kind": "port", "host": host_info(),
             "sample": (f"reference stored-matrix path restated in C/OpenMP (COO build as get_offdiag_adc2_save_omp, "
                        f"sigma as hxvec_incore, OpenMP over the {nvec} columns): COO of the {coo.rows} 1h1p rows "
                        f"with virtual index < {na} ({coo.nnz} of {coo.full_nnz} stored elements), hxvec_incore "
@@ -183,20 +235,32 @@ def cpu_baseline(sys_, op, nvec):
 
 
 def davidson_compare(op, nroots, only=None, blocksize=20, maxvec=200):
+    """Time to nroots roots (residual 1e-7, deflation) with the reference's
+    steps (DPR; Olsen, the reference's default precon), the coupled
+    (Schur-complement) step, the learned networks on DPR and on the coupled
+    step, each in FP64 and with an FP32 first phase refined in FP64."""
     res = {}
     runs = [("fp64_dpr", dict(precon="dpr", precision="fp64")),
-            ("bf16_dpr_refined", dict(precon="dpr", precision="bf16", lowprec_tol=5e-2)),
+            ("fp64_olsen", dict(precon="olsen", precision="fp64")),
             ("fp32_dpr_refined", dict(precon="dpr", precision="fp32", lowprec_tol=1e-4)),
-            ("fp64_dpr_subdiag_guess", dict(precon="dpr", precision="fp64", guess="subdiag", subdim=800))]
+            ("fp64_coupled", dict(precon="coupled", precision="fp64")),
+            ("fp32_coupled_refined", dict(precon="coupled", precision="fp32", lowprec_tol=1e-4))]
     have_net = os.path.exists(WEIGHTS)
+    have_cnet = os.path.exists(WEIGHTS_COUPLED)
     if have_net:
-        op.set_precon_params(np.load(WEIGHTS))
-        runs += [("fp64_learned", dict(precon="learned", precision="fp64")),
-                 ("bf16_learned_refined", dict(precon="learned", precision="bf16", lowprec_tol=5e-2))]
+        runs += [("fp64_learned", dict(precon="learned", precision="fp64", weights=WEIGHTS))]
+    if have_cnet:
+        runs += [("fp64_learned_coupled", dict(precon="learned_coupled", precision="fp64", weights=WEIGHTS_COUPLED)),
+                 ("fp32_learned_coupled_refined", dict(precon="learned_coupled", precision="fp32",
+                                                       lowprec_tol=1e-4, weights=WEIGHTS_COUPLED))]
     if only:
         runs = [r for r in runs if r[0] in only]
     ref = None
     for name, kw in runs:
+        kw = dict(kw)
+        w = kw.pop("weights", None)
+        if w is not None:
+            op.set_precon_params(np.load(w))
         t0 = time.perf_counter()
         try:
             ev, _, st = op.davidson(nroots, blocksize=blocksize, maxvec=maxvec, tol=1e-7, maxiter=1000,
@@ -209,6 +273,9 @@ def davidson_compare(op, nroots, only=None, blocksize=20, maxvec=200):
         rec = {"seconds": round(wall, 3), "iterations": st["niter"], "sigma_vectors": st["nsigma"],
                "lowprec_iterations": st["niter_lowprec"], "fallbacks": st["nfallback"],
                "sigma_seconds": round(st["time_sigma"], 3), "max_resnorm": st["resnorm_max"]}
+        if st["nprecon_builds"]:
+            rec.update({"precon_seconds": round(st["time_precon"], 3), "schur_complements": st["nprecon_builds"],
+                        "shift_Eh": round(st["precon_sigma"], 6)})
         if ref is None:
             ref = ev
             rec["eigenvalues_Eh"] = [float(x) for x in ev]
@@ -223,7 +290,7 @@ def davidson_compare(op, nroots, only=None, blocksize=20, maxvec=200):
         for name in res:
             if "seconds" in res[name]:
                 res[name]["speedup_vs_fp64_dpr"] = round(base / res[name]["seconds"], 3)
-    res["learned_weights"] = have_net
+    res["learned_weights"] = {"dpr": have_net, "coupled": have_cnet}
     res["roots"], res["blocksize"], res["maxvec"] = nroots, blocksize, maxvec
     return res
 
@@ -467,12 +534,17 @@ def main():
     if args.copy_calibration:
         copy_calibration(hip)
 
-    cpu = None
+    cpu = cpu_roots = None
     if not args.skip_cpu:
         try:
             cpu = cpu_baseline(sys_, op, nvec)
         except Exception as e:  # the baseline is reported, never required
             log(f"cpu baseline failed: {e}")
+        try:
+            cpu_roots = cpu_time_to_roots()
+        except Exception as e:  # noqa: BLE001 - reported, never fatal
+            log(f"cpu time-to-roots failed: {e}")
+            cpu_roots = {"error": str(e)}
     dav = None
     if not args.skip_davidson:
         dav = davidson_compare(op, args.dav_roots, [r for r in args.dav_runs.split(",") if r],
@@ -517,6 +589,7 @@ def main():
         "sigma_GBps_effective": round(bytes_per_sigma * value / 1e9, 1),
         "detail": extra,
         "davidson_20_roots": dav,
+        "cpu_time_to_5_roots_water": cpu_roots,
         "sil_autocorrelation": sil,
         "chebyshev_autocorrelation": cheb,
         "block_lanczos": lanc,

@@ -117,3 +117,37 @@ def test_trained_network_vs_dpr():
           f"({st_l['time_total']:.2f}s, {st_l['nfallback']} DPR fallbacks): iteration gain {gain:.2f}")
     assert np.abs(ev_d - ev_l).max() < 1e-8
     assert gain > 0.3
+
+
+WEIGHTS_COUPLED = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "adc_ottawa_amd",
+                               "precon_weights_coupled.npy")
+
+
+@pytest.mark.skipif(not os.path.exists(WEIGHTS_COUPLED), reason="no trained weights committed")
+def test_trained_coupled_network_benzene():
+    """The committed network on the coupled step, on an unseen o=16, v=200
+    system (seed 5002, not among the training seeds 1000-1199) with 20 roots:
+    the same eigenvalues as the plain FP64 DPR Davidson, at least 3 times
+    faster than it in wall time with the FP32 first phase refined in FP64, and
+    within 10 % of the iterations of its base step (the coupled step without
+    the network); the measured ratios are printed."""
+    w = np.load(WEIGHTS_COUPLED)
+    print("[case] learned on coupled vs DPR o=16 v=200 naux=864 seed=5002, 20 roots", flush=True)
+    sys_ = make_system(16, 200, 864, seed=5002)
+    kw = dict(blocksize=20, maxvec=200, tol=1e-7, maxiter=1000, deflate=True, vectors=False)
+    with lib.Operator.from_system(sys_) as o:
+        o.set_precon_params(w)
+        ev_c, _, st_c = o.davidson(20, precon="coupled", **kw)
+        ev_n, _, st_n = o.davidson(20, precon="learned_coupled", **kw)
+        ev_l, _, st_l = o.davidson(20, precon="learned_coupled", precision="fp32", lowprec_tol=1e-4, **kw)
+        ev_d, _, st_d = o.davidson(20, precon="dpr", **kw)
+    speedup = st_d["time_total"] / st_l["time_total"]
+    gain = st_c["niter"] / st_n["niter"]
+    print(f"DPR {st_d['niter']} it {st_d['time_total']:.2f}s; coupled {st_c['niter']} it {st_c['time_total']:.2f}s; "
+          f"learned on coupled {st_n['niter']} it {st_n['time_total']:.2f}s ({st_n['nfallback']} base-step "
+          f"fallbacks); learned on coupled + FP32 first phase {st_l['niter']} it {st_l['time_total']:.2f}s: "
+          f"{speedup:.2f}x DPR, iteration gain of the network over its base {gain:.2f}")
+    for ev in (ev_c, ev_n, ev_l):
+        assert np.abs(ev - ev_d).max() < 1e-8
+    assert speedup >= 3.0
+    assert gain >= 0.9
