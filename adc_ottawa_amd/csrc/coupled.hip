@@ -61,9 +61,10 @@ __global__ void cp_scale2_kernel(int64_t n1, int64_t n, int nb, const double* __
 }
 
 // T[n1 + j, c] = (R[n1 + j, c] - U[n1 + j, c]) / (hii[n1 + j] - sigma)
+// (T may be R: each element is read before it is written, by the same thread)
 __global__ void cp_final2_kernel(int64_t n1, int64_t n, const double* __restrict__ hii, double sigma,
-                                 const double* __restrict__ R, int64_t ldr, const double* __restrict__ U, int64_t ldu,
-                                 double* __restrict__ T, int64_t ldt) {
+                                 const double* R, int64_t ldr, const double* __restrict__ U, int64_t ldu,
+                                 double* T, int64_t ldt) {
   const int c = blockIdx.y;
   for (int64_t i = n1 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     T[i + ldt * c] = (R[i + ldr * c] - U[i + ldu * c]) / (hii[i] - sigma);
@@ -397,6 +398,8 @@ bool CoupledPrecon::track(const double* theta, int nact) {
 }
 
 // T[:, c] = M(theta_c)^-1 R[:, c] for nb <= nc columns (theta on the host).
+// T may be R (with ldt = ldr): R's 1h1p rows are consumed into Bm before T's
+// are written, and its 2h2p rows element by element in cp_final2_kernel.
 void CoupledPrecon::apply(int nb, const double* theta_h, const double* R, int64_t ldr, double* T, int64_t ldt) {
   Handle& Hd = *H;
   hipStream_t st = Hd.st;

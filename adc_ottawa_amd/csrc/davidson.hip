@@ -188,8 +188,13 @@ struct Dav {
   double t_sigma = 0;
   int nsigma = 0;
   int n_second_pass = 0;
+  // the correction block shares the residual block's memory (a step that
+  // reads each residual column before writing its correction, and needs the
+  // residual no more afterwards: the coupled step without the network)
+  bool t_alias = false;
 
   explicit Dav(Handle& h) : H(h) {}
+  double* t() const { return t_alias ? Res.get() : T.get(); }
   ~Dav() {
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -202,7 +207,7 @@ struct Dav {
     X.alloc((size_t)n * bs);
     WY.alloc((size_t)n * bs);
     Res.alloc((size_t)n * bs);
-    T.alloc((size_t)n * bs);
+    if (!t_alias) T.alloc((size_t)n * bs);
     Rm.alloc((size_t)mv * mv);
     Rt.alloc((size_t)mv * mv);
     Zs.alloc((size_t)mv * mv);
@@ -266,8 +271,8 @@ struct Dav {
   // the DGKS test asks for it; within the block, orthonormalisation through
   // the Gram matrix, repeated when that matrix was ill-conditioned.
   int append_block(int c0, int r, const double* Xc, int ncv) {
-    cgs_pass(V.get(), c0, Xc, ncv, T.get(), r);
-    gemm_tn(H, r, r, n, T.get(), n, T.get(), n, Gm.get(), r, 1.0, 0.0);
+    cgs_pass(V.get(), c0, Xc, ncv, t(), r);
+    gemm_tn(H, r, r, n, t(), n, t(), n, Gm.get(), r, 1.0, 0.0);
     if (c0 + ncv > 0) {
       launch_dgks_check(r, c0, S1.get(), mv, ncv, S2.get(), bs, Gm.get(), r, info.get() + 1, H.st);
       int need = 0;
@@ -275,18 +280,18 @@ struct Dav {
       HIP_CHECK(hipStreamSynchronize(H.st));
       if (need) {
         ++n_second_pass;
-        cgs_pass(V.get(), c0, Xc, ncv, T.get(), r);
-        gemm_tn(H, r, r, n, T.get(), n, T.get(), n, Gm.get(), r, 1.0, 0.0);
+        cgs_pass(V.get(), c0, Xc, ncv, t(), r);
+        gemm_tn(H, r, r, n, t(), n, t(), n, Gm.get(), r, 1.0, 0.0);
       }
     }
     double* slot = V.get() + n * c0;
     double gc = 0.0;
-    const int k1 = orth_from_gram(T.get(), r, slot, &gc);
+    const int k1 = orth_from_gram(t(), r, slot, &gc);
     if (k1 == 0) return 0;
     if (!(gc < 1e3)) {
       gemm_tn(H, k1, k1, n, slot, n, slot, n, Gm.get(), k1, 1.0, 0.0);
-      const int k2 = orth_from_gram(slot, k1, T.get(), &gc);
-      if (k2 > 0) HIP_CHECK(hipMemcpyAsync(slot, T.get(), sizeof(double) * n * k2, hipMemcpyDeviceToDevice, H.st));
+      const int k2 = orth_from_gram(slot, k1, t(), &gc);
+      if (k2 > 0) HIP_CHECK(hipMemcpyAsync(slot, t(), sizeof(double) * n * k2, hipMemcpyDeviceToDevice, H.st));
       return k2;
     }
     return k1;
@@ -330,6 +335,7 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
   Dav D(H);
   D.bs = bs;
   D.mv = mv;
+  D.t_alias = coupled && !learned && trace == nullptr;
   D.alloc();
   const int64_t n = D.n;
   hipStream_t st = H.st;
@@ -413,7 +419,7 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
     // and, for DPR, the correction step
     const bool fused_dpr = (p.precon == 1);
     int nb = ritz_fused(n, currdim, bs_curr, D.V.get(), D.W.get(), n, D.Zs.get(), mv, D.thd.get(), H.hii.get(),
-                        fused_dpr, D.X.get(), D.WY.get(), D.Res.get(), D.T.get(), D.rpart, st);
+                        fused_dpr, D.X.get(), D.WY.get(), D.Res.get(), fused_dpr ? D.t() : nullptr, D.rpart, st);
     const bool have_t = fused_dpr && nb > 0;
     if (nb == 0) {
       gemm_nn(H, n, bs_curr, currdim, D.V.get(), n, D.Zs.get(), mv, D.X.get(), n, 1.0, 0.0);
@@ -446,9 +452,9 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
       if (coupled && ns_curr > 0) {
         for (int c0 = 0; c0 < ns_curr; c0 += cpl->nc) {
           const int nb = std::min(cpl->nc, ns_curr - c0);
-          cpl->apply(nb, theta.data() + c0, D.Res.get() + n * c0, n, D.T.get() + n * c0, n);
+          cpl->apply(nb, theta.data() + c0, D.Res.get() + n * c0, n, D.t() + n * c0, n);
         }
-        base = D.T.get();
+        base = D.t();
       }
       davtrace_iteration(H, *trace, it, bs_curr, ns_curr, theta.data(), norm.data(), D.X.get(), D.Res.get(), n,
                          base);
@@ -535,19 +541,19 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
     // new directions (dpr / olsen / learned)
     HIP_CHECK(hipMemcpyAsync(D.thd.get(), theta.data(), sizeof(double) * bs_curr, hipMemcpyHostToDevice, st));
     if (p.precon == 1) {
-      if (!have_t) launch_dpr(n, bs_curr, D.thd.get(), H.hii.get(), D.Res.get(), D.T.get(), n, st);
+      if (!have_t) launch_dpr(n, bs_curr, D.thd.get(), H.hii.get(), D.Res.get(), D.t(), n, st);
     } else if (p.precon == 2) {
       const int nb2 = launch_olsen_dots(n, bs_curr, D.thd.get(), H.hii.get(), D.X.get(), D.Res.get(), n,
                                         D.part.get(), st);
       launch_colsum(nb2, 2 * bs_curr, D.part.get(), D.nrm.get(), st);
-      launch_olsen_step(n, bs_curr, D.thd.get(), H.hii.get(), D.X.get(), D.Res.get(), D.nrm.get(), D.T.get(), n, st);
+      launch_olsen_step(n, bs_curr, D.thd.get(), H.hii.get(), D.X.get(), D.Res.get(), D.nrm.get(), D.t(), n, st);
     } else {
       const auto tc = std::chrono::steady_clock::now();
       if (coupled) {
         if (it % 10 == 0) cpl->track(theta.data(), ns_curr);
         for (int c0 = 0; c0 < bs_curr; c0 += cpl->nc) {
           const int nb = std::min(cpl->nc, bs_curr - c0);
-          cpl->apply(nb, theta.data() + c0, D.Res.get() + n * c0, n, D.T.get() + n * c0, n);
+          cpl->apply(nb, theta.data() + c0, D.Res.get() + n * c0, n, D.t() + n * c0, n);
         }
       }
       if (learned) {
@@ -568,8 +574,8 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
             ++nfallback;
           }
         }
-        precon_learned_step(H, n, bs_curr, D.thd.get(), norm.data(), use_base.data(), D.Res.get(), D.T.get(), n,
-                            coupled ? D.T.get() : nullptr);
+        precon_learned_step(H, n, bs_curr, D.thd.get(), norm.data(), use_base.data(), D.Res.get(), D.t(), n,
+                            coupled ? D.t() : nullptr);
       }
       HIP_CHECK(hipStreamSynchronize(st));
       t_precon += std::chrono::duration<double>(std::chrono::steady_clock::now() - tc).count();

@@ -478,11 +478,15 @@ def main():
                     help="relax the lowest state in imaginary time to this residual (0 skips)")
     ap.add_argument("--skip-mp2", action="store_true", help="skip the MP2 energy/density timing")
     ap.add_argument("--skip-fvector", action="store_true", help="skip the F-vector timing")
-    ap.add_argument("--dav-roots", type=int, default=20, help="Davidson comparison: roots (block >= roots)")
-    ap.add_argument("--dav-block", type=int, default=20, help="Davidson comparison: block size")
-    ap.add_argument("--dav-maxvec", type=int, default=200, help="Davidson comparison: maximum subspace")
+    ap.add_argument("--dav-roots", type=int, default=0,
+                    help="Davidson comparison: roots (default 20; 40 for the large workload)")
+    ap.add_argument("--dav-block", type=int, default=0,
+                    help="Davidson comparison: block size, >= roots (default: the number of roots)")
+    ap.add_argument("--dav-maxvec", type=int, default=0,
+                    help="Davidson comparison: maximum subspace (default 10 blocks; 2 for the large workload)")
     ap.add_argument("--dav-runs", default="",
-                    help="comma-separated subset of the Davidson comparison runs (default: all)")
+                    help="comma-separated subset of the Davidson comparison runs (default: all; for the large "
+                         "workload the coupled step in FP64 and with the FP32 first phase)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run device copies of 1 GiB (scale factor for rocprofv3 --pmc byte counters)")
     args = ap.parse_args()
@@ -547,8 +551,17 @@ def main():
             cpu_roots = {"error": str(e)}
     dav = None
     if not args.skip_davidson:
-        dav = davidson_compare(op, args.dav_roots, [r for r in args.dav_runs.split(",") if r],
-                               blocksize=args.dav_block, maxvec=args.dav_maxvec)
+        # the large workload's 40 roots: the block must hold every root
+        # (block_davidson.f90:195, 221-222), and two blocks of 40 FP64
+        # vectors of ndim 72,018,000 are what HBM holds beside the operator
+        # (V and W 46 GB each, the Ritz, W y and residual blocks 23 GB each);
+        # the diagonal steps do not converge there in 1000 iterations
+        large = args.workload == "large"
+        nroots = args.dav_roots or (40 if large else 20)
+        block = args.dav_block or nroots
+        maxvec = args.dav_maxvec or (2 * block if large else 10 * block)
+        runs = args.dav_runs or ("fp64_coupled,fp32_coupled_refined" if large else "")
+        dav = davidson_compare(op, nroots, [r for r in runs.split(",") if r], blocksize=block, maxvec=maxvec)
 
     sil = sil_compare(op, sz, args.sil, args.sil_tol) if args.sil > 0 else None
     def guarded(name, fn):
