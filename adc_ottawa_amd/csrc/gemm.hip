@@ -63,11 +63,11 @@ void dispatch(GemmArgs g, GemmWorkspace& ws, hipStream_t st, int splitk_hint) {
   constexpr int BK = (sizeof(T) == 8) ? 16 : (sizeof(T) == 4 ? 32 : 64);
   // tile shape: 128x32 for N <= 32, 64x64 when M or N is small; otherwise
   // the shape with the least padded output (within 2%), larger tiles first
-  static const int SBM[7] = {0, 128, 64, 128, 128, 64, 64};
-  static const int SBN[7] = {0, 128, 64, 32, 64, 128, 208};
+  static const int SBM[9] = {0, 128, 64, 128, 128, 64, 64, 32, 32};
+  static const int SBN[9] = {0, 128, 64, 32, 64, 128, 208, 208, 128};
   int shape = g.tile;
-  // the 208-column tile loads its operands along K only
-  if (shape == 6 && !(g.sak == 1 && g.sbk == 1)) shape = 0;
+  // the 208-column tiles load their operands along K only
+  if ((shape == 6 || shape == 7) && !(g.sak == 1 && g.sbk == 1)) shape = 0;
   if (shape <= 0) {
     auto padded = [&](int sh) {
       const int bm = SBM[sh], bn = SBN[sh];
@@ -87,6 +87,12 @@ void dispatch(GemmArgs g, GemmWorkspace& ws, hipStream_t st, int splitk_hint) {
           break;
         }
     }
+  }
+  // thin products (M <= 32 rows, e.g. the coupling contractions of one or two
+  // vectors): 32-row tiles, so that no MFMA works on padding rows
+  if (g.M <= 32) {
+    if (shape == 6) shape = 7;
+    else if (shape == 2 && g.N >= 128) shape = 8;
   }
   const int BMs = SBM[shape], BNs = SBN[shape];
   const int tiles = ((g.M + BMs - 1) / BMs) * ((g.N + BNs - 1) / BNs) * g.batch;
@@ -136,6 +142,8 @@ void dispatch(GemmArgs g, GemmWorkspace& ws, hipStream_t st, int splitk_hint) {
     case 4: launch<T, 128, 64, BK>(g, st); break;
     case 5: launch<T, 64, 128, BK>(g, st); break;
     case 6: launch<T, 64, 208, BK>(g, st); break;
+    case 7: launch<T, 32, 208, BK>(g, st); break;
+    case 8: launch<T, 32, 128, BK>(g, st); break;
     default: launch<T, 128, 128, BK>(g, st);
   }
   if (need_partial) {
@@ -147,11 +155,11 @@ void dispatch(GemmArgs g, GemmWorkspace& ws, hipStream_t st, int splitk_hint) {
     if (g.splitk >= 8 && total <= (int64_t)1 << 22) {  // many splits, few outputs
       const int wb = (int)((total + 63) / 64);
       if (g.out_prec == 1)
-        hipLaunchKernelGGL((splitk_reduce_wide_kernel<Acc, float>), dim3(wb), dim3(256), 0, st, g);
+        hipLaunchKernelGGL((splitk_reduce_wide_kernel<Acc, float>), dim3(wb), dim3(1024), 0, st, g);
       else if (g.out_prec == 2)
-        hipLaunchKernelGGL((splitk_reduce_wide_kernel<Acc, __bf16>), dim3(wb), dim3(256), 0, st, g);
+        hipLaunchKernelGGL((splitk_reduce_wide_kernel<Acc, __bf16>), dim3(wb), dim3(1024), 0, st, g);
       else
-        hipLaunchKernelGGL((splitk_reduce_wide_kernel<Acc, double>), dim3(wb), dim3(256), 0, st, g);
+        hipLaunchKernelGGL((splitk_reduce_wide_kernel<Acc, double>), dim3(wb), dim3(1024), 0, st, g);
       HIP_CHECK(hipGetLastError());
       return;
     }

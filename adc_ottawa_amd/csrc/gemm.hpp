@@ -581,11 +581,13 @@ __global__ void splitk_reduce_kernel(GemmArgs g) {
 }
 
 // Split-K reduction for many splits and few outputs: 64 consecutive outputs
-// per workgroup (one per lane), the splits dealt round-robin to the 4 waves and
-// the 4 wave sums added in a fixed order.
+// per workgroup (one per lane), the splits dealt round-robin to the 16 waves,
+// each wave keeping 4 independent sums (4 loads in flight per lane), and the
+// 64 sums added in a fixed order.
 template <class Acc, class OutT>
-__global__ void __launch_bounds__(256) splitk_reduce_wide_kernel(GemmArgs g) {
-  __shared__ double red[4][64];
+__global__ void __launch_bounds__(1024) splitk_reduce_wide_kernel(GemmArgs g) {
+  constexpr int NW = 16, U = 4;
+  __shared__ double red[NW][64];
   const int64_t MN = (int64_t)g.M * g.N;
   const int64_t total = MN * g.batch;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -594,14 +596,26 @@ __global__ void __launch_bounds__(256) splitk_reduce_wide_kernel(GemmArgs g) {
   if (idx < total) {
     const int64_t b = idx / MN, mn = idx - b * MN;
     const Acc* p = (const Acc*)g.partial + b * g.splitk * MN + mn;
-    for (int k = wv; k < g.splitk; k += 4) s += (double)p[k * MN];
+    double acc[U] = {0.0, 0.0, 0.0, 0.0};
+    int k = wv;
+    for (; k + (U - 1) * NW < g.splitk; k += U * NW) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc[u] += (double)p[(int64_t)(k + u * NW) * MN];
+    }
+    // fewer than U splits of this wave remain
+    if (k < g.splitk) acc[0] += (double)p[(int64_t)k * MN];
+    if (k + NW < g.splitk) acc[1] += (double)p[(int64_t)(k + NW) * MN];
+    if (k + 2 * NW < g.splitk) acc[2] += (double)p[(int64_t)(k + 2 * NW) * MN];
+    s = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   }
   red[wv][lane] = s;
   __syncthreads();
   if (wv == 0 && idx < total) {
-    s = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) t += red[w][lane];
     const int64_t b = idx / MN, mn = idx - b * MN;
-    map_store<OutT>(g.map, mn / g.N, mn % g.N, b, s);
+    map_store<OutT>(g.map, mn / g.N, mn % g.N, b, t);
   }
 }
 
