@@ -31,7 +31,11 @@ the CPU baseline.
 Pinned by: the closed forms of ``test_oracle_pins`` (zero-coupling limit,
 ADC(1)/CIS eigenvalues by dense eigh, symmetry), agreement with the
 independent DF formulation in ``dfsigma`` and with the Lowdin-partitioned
-exact eigenvalues of ``schur``.
+exact eigenvalues of ``schur``; and against the exact intermediate-state
+representation in the determinant space (tests/detspace.py isr_orders,
+test_oracle_isr_brute_force): H11 order by order through second order
+element by element, C and D through the 2h2p-basis invariants C C^T and
+C D C^T.
 """
 from __future__ import annotations
 

@@ -35,9 +35,13 @@ f = 0 exactly, since <Psi_J|N|Psi_0> = N <Psi_J|Psi_0> = 0 (this fixes the
 relative signs and factors of F21..F24 and of every 2h2p function); and the
 density-like intermediates against the MP2 density of oracle/mp2.py
 (tauA(c,a) = -uu(a,c)/4 and tauB(m,k) = oo(k,m)/4, both pinned by the
-textbook closed-shell MP2 density; tau21 + tau22 = -ou^T/2).  The
-second-order terms with dpl_ov (F25..F210, FC) are pinned only by the two
-restatements agreeing: parity unpinned beyond that.
+textbook closed-shell MP2 density; tau21 + tau22 = -ou^T/2); and every term
+against the exact intermediate-state representation in the determinant
+space of the same Hamiltonian (tests/detspace.py isr_orders,
+test_oracle_isr_brute_force): the 1h1p block order by order through second
+order element by element (the reference's 1h1p configurations carry the sign
+-1 against C_ak = g_ak/sqrt(2)), the 2h2p block through the invariants of the
+2h2p basis (C f2, |f2|^2).
 """
 from __future__ import annotations
 

@@ -15,6 +15,17 @@ done exactly by linear algebra in the determinant space:
     E2 = <Phi0|V|Psi1>,
     rho2 = <Psi1|g|Psi1> + <Phi0|g|Psi2> + <Psi2|g|Phi0> - <Psi1|Psi1> <Phi0|g|Phi0>
 for the spin-summed one-particle density g_pq = sum_s a+_ps a_qs.
+
+Intermediate-state representation (ISR) of the 1h1p class, also exact in the
+determinant space: for H(lam) = H0 + lam V with exact ground state Psi0(lam)
+(lowest eigenvector, <Phi0|Psi0> > 0), the precursor states
+C_ak Psi0 (C_ak = g_ak / sqrt(2), the singlet 1h1p excitation), made
+orthogonal to Psi0 and then symmetrically (Loewdin) orthonormalised among
+themselves, give the intermediate states Psi~_ak(lam), and
+    f_ak(lam) = <Psi~_ak|D|Psi0>,
+    M_ak,bl(lam) = <Psi~_ak|H(lam) - E0(lam)|Psi~_bl>.
+Their Taylor coefficients in lam are the strict perturbation-theory orders;
+they are read off a polynomial through samples at lam = h*(-n..n).
 """
 from __future__ import annotations
 
@@ -135,3 +146,81 @@ def mp_second_order(eps: np.ndarray, eri: np.ndarray, nocc: int):
             rho0[p, q] = phi0 @ g @ phi0
             rho2[p, q] = psi1 @ g @ psi1 + phi0 @ g @ psi2 + psi2 @ g @ phi0 - n11 * rho0[p, q]
     return E2, rho0, rho2
+
+
+def _fock_hamiltonian(eps: np.ndarray, eri: np.ndarray, nocc: int):
+    """(DetSpace, H, diagonal of H0): the Hamiltonian whose Fock matrix is diag(eps)."""
+    nb = eps.size
+    o = nocc
+    h = np.diag(eps).astype(np.float64)
+    h -= 2.0 * np.einsum("pqjj->pq", eri[:, :, :o, :o]) - np.einsum("pjjq->pq", eri[:, :o, :o, :])
+    ds = DetSpace(o, nb)
+    H = ds.hamiltonian(h, eri)
+    occ_energy = np.array([sum(eps[k // 2] for k in range(2 * nb) if (d >> k) & 1) for d in ds.dets])
+    return ds, H, occ_energy
+
+
+def isr_orders(eps: np.ndarray, eri: np.ndarray, nocc: int, dpl: np.ndarray, h: float = 0.02, npts: int = 5):
+    """Strict perturbation orders of the ISR quantities of the ADC(2) classes.
+
+    1h1p class (singlet C_ak = g_ak / sqrt(2), (a, k) virtual-major a*nocc + k):
+      f1[n, a, k] and M11[n] (orders n = 0, 1, 2).
+    2h2p class: precursors g_ak g_bl Psi0 for unordered pairs (a,k) <= (b,l)
+    (they span the singlet 2h2p space), made orthogonal to Psi0 and to the
+    1h1p intermediate states, then Loewdin-orthonormalised.  This basis
+    differs from any other orthonormal singlet 2h2p basis by an orthogonal
+    transformation, so only invariants under it are returned, each as its
+    lam^0, lam^1, lam^2 coefficients (the first two vanish):
+      CC  = C^T C,  CDC = C^T M22 C  (1h1p x 1h1p),  Cf = C^T f2,  ff = f2^T f2,
+    with C = <Psi~_2h2p|H - E0|Psi~_1h1p> and f2 = <Psi~_2h2p|D|Psi0>; their
+    lam^2 coefficients are C1^T C1, C1^T D0 C1, C1^T f2^(1), |f2^(1)|^2 of the
+    first-order coupling C1, zeroth-order 2h2p diagonal D0 and first-order
+    2h2p f-vector f2^(1)."""
+    nb = eps.size
+    o, v = nocc, nb - nocc
+    n1 = o * v
+    ds, H, e0 = _fock_hamiltonian(eps, eri, nocc)
+    V = H - np.diag(e0)
+    D = np.zeros_like(H)
+    G = {}
+    for p in range(nb):
+        for q in range(nb):
+            g = ds.one_body(p, q)
+            D += dpl[p, q] * g
+            G[p, q] = g
+    C1 = [G[o + a, k] / np.sqrt(2.0) for a in range(v) for k in range(o)]
+    E = [G[o + a, k] for a in range(v) for k in range(o)]
+    C2 = [E[i] @ E[j] for i in range(n1) for j in range(i, n1)]
+    lams = h * np.arange(-npts, npts + 1)
+    rec = {k: [] for k in ("f1", "M11", "CC", "CDC", "Cf", "ff")}
+    for lam in lams:
+        Hl = np.diag(e0) + lam * V
+        w, U = np.linalg.eigh(Hl)
+        psi = U[:, 0] * np.sign(U[ds.ref, 0])
+        Hs = Hl - w[0] * np.eye(Hl.shape[0])
+
+        def loewdin(P):
+            sw, sv = np.linalg.eigh(P.T @ P)
+            return P @ (sv @ np.diag(sw ** -0.5) @ sv.T)
+
+        P1 = np.array([g @ psi for g in C1]).T
+        P1 = loewdin(P1 - np.outer(psi, psi @ P1))
+        P2 = np.array([g @ psi for g in C2]).T
+        P2 = P2 - np.outer(psi, psi @ P2) - P1 @ (P1.T @ P2)
+        P2 = loewdin(P2)
+        Cm = P2.T @ Hs @ P1
+        f2 = P2.T @ D @ psi
+        rec["f1"].append(P1.T @ D @ psi)
+        rec["M11"].append(P1.T @ Hs @ P1)
+        rec["CC"].append(Cm.T @ Cm)
+        rec["CDC"].append(Cm.T @ (P2.T @ Hs @ P2) @ Cm)
+        rec["Cf"].append(Cm.T @ f2)
+        rec["ff"].append(np.array([f2 @ f2]))
+    Vm = np.vander(lams, lams.size, increasing=True)
+    out = {}
+    for k, vals in rec.items():
+        arr = np.array(vals)
+        c = np.linalg.solve(Vm, arr.reshape(lams.size, -1))[:3]
+        out[k] = c.reshape((3,) + arr.shape[1:])
+    out["f1"] = out["f1"].reshape(3, v, o)
+    return out

@@ -627,3 +627,66 @@ def test_oracle_mp2_density_brute_force(o, v, naux, seed):
     assert np.abs(rho2[:o, o:] - blk["ou"]).max() < 1e-12 * scale
     assert np.abs(rho2[o:, :o] - blk["ou"].T).max() < 1e-12 * scale
     assert np.abs(blk["ou"]).max() > 1e-3 * scale     # the ou block is not trivially zero
+
+
+def _scaled(sys_, lam):
+    """The same system with every two-electron integral scaled by lam (the
+    fluctuation potential of the Moller-Plesset partition, orbital energies
+    unchanged)."""
+    from adc_ottawa_amd.synth import System
+    return System(sys_.nocc, sys_.nvirt, sys_.naux, sys_.eps.copy(), sys_.B * np.sqrt(lam))
+
+
+def _orders(fn, sys_):
+    """Coefficients of lam^0, lam^1, lam^2 of fn(system with integrals x lam);
+    exact for the ADC(2) quantities, which are polynomials of degree <= 2 in
+    the integrals."""
+    lams = np.array([0.5, 1.0, 2.0])
+    vals = np.array([np.asarray(fn(_scaled(sys_, x)), dtype=np.float64) for x in lams])
+    c = np.linalg.solve(np.vander(lams, 3, increasing=True), vals.reshape(3, -1))
+    return c.reshape((3,) + vals.shape[1:])
+
+
+@pytest.mark.parametrize("o,v,naux,seed", [(2, 3, 6, 3), (2, 4, 5, 8), (3, 3, 6, 1)])
+def test_oracle_isr_brute_force(o, v, naux, seed):
+    """The ADC(2)-s matrix and f-vector of the oracle against the
+    intermediate-state representation built exactly in the determinant space
+    (tests/detspace.py: exact ground state of H0 + lam V, precursor states
+    C_J Psi0, orthogonalised to Psi0 and the lower class, Loewdin within a
+    class; strict perturbation orders read off a polynomial in lam), which
+    shares nothing with the restated formulas:
+    * 1h1p/1h1p block H11 (adc_ph.f90 K, C1, CA, CB, CC) order by order
+      through second order, element by element;
+    * 1h1p f-vector (get_moment.f90:153-442: F0, F21..F210, FA, FB, FC) order
+      by order through second order, element by element, the sign of the
+      reference's 1h1p configurations being -1 against C_ak = g_ak/sqrt(2);
+    * coupling C1 (adc_ph.f90:974-1073), 2h2p diagonal and 2h2p f-vector
+      (FI..FIV2) through the invariants of the 2h2p basis: C C^T, C D C^T,
+      C f2 (same sign convention) and |f2|^2."""
+    from tests.detspace import isr_orders
+    sys_ = make_system(o, v, naux, seed=seed)
+    rng = np.random.default_rng(100 + seed)
+    d = rng.standard_normal((o + v, o + v))
+    d = d + d.T
+    bf = isr_orders(sys_.eps, mp2.integrals(sys_), o, d)
+    n1 = o * v
+    typ, a, _, k, _ = csf.table(o, v)
+    ph = np.where(typ == 0)[0]
+    assert np.array_equal(a[ph] * o + k[ph], np.arange(n1))  # 1h1p configurations virtual-major
+    h11 = _orders(lambda s: elements.build_blocks(s, apply_minc=False).H11, sys_)
+    f1 = _orders(lambda s: fvec.fvector(s, d)[:n1], sys_)
+    tol = (1e-12, 1e-9, 1e-6)   # the fit of the brute-force series limits order 2 to ~1e-8
+    for n in range(3):
+        ref_m, ref_f = bf["M11"][n], -bf["f1"][n].reshape(n1)
+        assert np.abs(h11[n] - ref_m).max() < tol[n] * np.abs(ref_m).max(), f"H11 order {n}"
+        assert np.abs(f1[n] - ref_f).max() < tol[n] * np.abs(ref_f).max(), f"f(1h1p) order {n}"
+    assert np.abs(bf["M11"][2]).max() > 1e-4 and np.abs(bf["f1"][2]).max() > 1e-5  # second order is not trivial
+    blk = elements.build_blocks(sys_, apply_minc=False)
+    C, D2 = blk.C, blk.D
+    f2 = fvec.fvector(sys_, d)[n1:]
+    checks = {"CC": C @ C.T, "CDC": (C * D2[None, :]) @ C.T, "Cf": -(C @ f2), "ff": np.array([f2 @ f2])}
+    for key, ref in checks.items():
+        got = bf[key]
+        scale = np.abs(ref).max()
+        assert np.abs(got[2] - ref).max() < 1e-9 * scale, key
+        assert max(np.abs(got[0]).max(), np.abs(got[1]).max()) < 1e-12 * scale, key
