@@ -181,7 +181,7 @@ struct Dav {
   Handle& H;
   int64_t n;
   int bs, mv;
-  DevBuf<double> V, W, X, WY, Res, T, Rm, Rt, Zs, thd, part, nrm, small, work, M, S1, S2, Gm, cond, rpart, tmpcol;
+  DevBuf<double> V, W, X, WY, Res, T, Rm, Rt, Zs, thd, part, nrm, small, work, M, S1, S2, Gm, cond, rpart, tmpcol, bn;
   DevBuf<int> info;
   DevBuf<int64_t> idx;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -192,6 +192,7 @@ struct Dav {
   // reads each residual column before writing its correction, and needs the
   // residual no more afterwards: the coupled step without the network)
   bool t_alias = false;
+  bool verbose = false;
 
   explicit Dav(Handle& h) : H(h) {}
   double* t() const { return t_alias ? Res.get() : T.get(); }
@@ -220,6 +221,7 @@ struct Dav {
     S1.alloc((size_t)mv * bs);
     S2.alloc((size_t)bs * bs);
     Gm.alloc((size_t)bs * bs);
+    bn.alloc(bs);
     cond.alloc(1);
     info.alloc(4);
     HIP_CHECK(hipEventCreate(&e0));
@@ -254,9 +256,12 @@ struct Dav {
 
   // Orthonormalise Tp[:, :r] within itself into dst (dst != Tp) from its Gram
   // matrix Gm (r x r, ld r); returns the number of directions kept and the
-  // condition number of the kept part of the Gram matrix.
-  int orth_from_gram(double* Tp, int r, double* dst, double* gram_cond) {
-    launch_orth_coeffs(r, Gm.get(), r, M.get(), r, work.get(), info.get(), 1e-12, H.st, cond.get());
+  // condition number of the kept part of the Gram matrix.  With bnorm (the
+  // squared column norms before the projection against the subspace), a
+  // column that kept less than 1e-12 of its norm through the projection is
+  // dropped: it is rounding noise of the subspace, not a new direction.
+  int orth_from_gram(double* Tp, int r, double* dst, double* gram_cond, const double* bnorm = nullptr) {
+    launch_orth_coeffs(r, Gm.get(), r, M.get(), r, work.get(), info.get(), 1e-12, H.st, cond.get(), bnorm);
     int kept = 0;
     HIP_CHECK(hipMemcpyAsync(&kept, info.get(), sizeof(int), hipMemcpyDeviceToHost, H.st));
     HIP_CHECK(hipMemcpyAsync(gram_cond, cond.get(), sizeof(double), hipMemcpyDeviceToHost, H.st));
@@ -273,8 +278,9 @@ struct Dav {
   int append_block(int c0, int r, const double* Xc, int ncv) {
     cgs_pass(V.get(), c0, Xc, ncv, t(), r);
     gemm_tn(H, r, r, n, t(), n, t(), n, Gm.get(), r, 1.0, 0.0);
-    if (c0 + ncv > 0) {
-      launch_dgks_check(r, c0, S1.get(), mv, ncv, S2.get(), bs, Gm.get(), r, info.get() + 1, H.st);
+    const bool projected = c0 + ncv > 0;
+    if (projected) {
+      launch_dgks_check(r, c0, S1.get(), mv, ncv, S2.get(), bs, Gm.get(), r, info.get() + 1, H.st, bn.get());
       int need = 0;
       HIP_CHECK(hipMemcpyAsync(&need, info.get() + 1, sizeof(int), hipMemcpyDeviceToHost, H.st));
       HIP_CHECK(hipStreamSynchronize(H.st));
@@ -286,9 +292,25 @@ struct Dav {
     }
     double* slot = V.get() + n * c0;
     double gc = 0.0;
-    const int k1 = orth_from_gram(t(), r, slot, &gc);
+    if (verbose && projected) {
+      std::vector<double> g((size_t)r * r), b(r);
+      HIP_CHECK(hipMemcpyAsync(g.data(), Gm.get(), sizeof(double) * g.size(), hipMemcpyDeviceToHost, H.st));
+      HIP_CHECK(hipMemcpyAsync(b.data(), bn.get(), sizeof(double) * r, hipMemcpyDeviceToHost, H.st));
+      HIP_CHECK(hipStreamSynchronize(H.st));
+      double mn = INFINITY;
+      for (int j = 0; j < r; ++j) mn = std::min(mn, g[(size_t)j * r + j] / b[j]);
+      std::printf("  expansion: %d columns, smallest |t_after|^2/|t_before|^2 %.3e\n", r, mn);
+    }
+    const int k1 = orth_from_gram(t(), r, slot, &gc, projected ? bn.get() : nullptr);
+    if (verbose) std::printf("  expansion: kept %d of %d, Gram condition %.3e\n", k1, r, gc);
     if (k1 == 0) return 0;
     if (!(gc < 1e3)) {
+      // the combination of nearly dependent columns amplifies their rounding
+      // components along the subspace by up to sqrt(gc): project the
+      // normalised block once more before orthonormalising it again (else V
+      // loses orthogonality, and the Rayleigh matrix grows spurious Ritz
+      // pairs of near-zero vectors that pass the residual test)
+      if (projected) cgs_pass(V.get(), c0, Xc, ncv, slot, k1);
       gemm_tn(H, k1, k1, n, slot, n, slot, n, Gm.get(), k1, 1.0, 0.0);
       const int k2 = orth_from_gram(slot, k1, t(), &gc);
       if (k2 > 0) HIP_CHECK(hipMemcpyAsync(slot, t(), sizeof(double) * n * k2, hipMemcpyDeviceToDevice, H.st));
@@ -336,6 +358,7 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
   D.bs = bs;
   D.mv = mv;
   D.t_alias = coupled && !learned && trace == nullptr;
+  D.verbose = p.verbose != 0;
   D.alloc();
   const int64_t n = D.n;
   hipStream_t st = H.st;

@@ -534,7 +534,7 @@ __global__ void __launch_bounds__(SYEV_THREADS) syevx_kernel(int n, int k, doubl
 // work: r*r (copy) + r*r (U) + r (lambda) + r*r (syev scratch).
 __global__ void __launch_bounds__(SYEV_THREADS) orth_coeffs_kernel(int r, const double* G, int ldg, double* M,
                                                                     int ldm, double* work, int* kept, double tol,
-                                                                    double* cond) {
+                                                                    double* cond, const double* bnorm) {
   __shared__ SyevShared S;
   __shared__ BisectShared Bs;
   __shared__ double dinv[SYEV_NMAX];
@@ -545,9 +545,13 @@ __global__ void __launch_bounds__(SYEV_THREADS) orth_coeffs_kernel(int r, const 
   double* U = work + (int64_t)r * r;
   double* lam = U + (int64_t)r * r;
   double* Zw = lam + r;
+  // a column that kept less than tol of its norm through the projection
+  // that made it (bnorm: squared norms before it) is rounding noise of the
+  // subspace: its row and column of the scaled Gram matrix are zeroed, so it
+  // is dropped
   for (int i = tid; i < r; i += nt) {
     const double g = G[i + (int64_t)i * ldg];
-    dinv[i] = (g > 0.0) ? 1.0 / sqrt(g) : 0.0;
+    dinv[i] = (g > 0.0 && !(bnorm && g <= tol * tol * bnorm[i])) ? 1.0 / sqrt(g) : 0.0;
   }
   __syncthreads();
   for (int64_t idx = tid; idx < (int64_t)r * r; idx += nt) {
@@ -595,7 +599,7 @@ __global__ void __launch_bounds__(SYEV_THREADS) orth_coeffs_kernel(int r, const 
 constexpr double DGKS_KEEP = 1e-6;
 __global__ void dgks_check_kernel(int r, int nb1, const double* __restrict__ S1, int ld1, int nb2,
                                   const double* __restrict__ S2, int ld2, const double* __restrict__ G, int ldg,
-                                  int* __restrict__ need) {
+                                  int* __restrict__ need, double* __restrict__ bnorm) {
   int bad = 0;
   for (int j = threadIdx.x; j < r; j += blockDim.x) {
     double b = 0.0;
@@ -603,6 +607,7 @@ __global__ void dgks_check_kernel(int r, int nb1, const double* __restrict__ S1,
     for (int i = 0; i < nb2; ++i) b += S2[i + (int64_t)j * ld2] * S2[i + (int64_t)j * ld2];
     const double a = G[j + (int64_t)j * ldg];
     b += a;
+    if (bnorm) bnorm[j] = b;
     if (!(a >= DGKS_KEEP * b)) bad = 1;
   }
   bad = __syncthreads_or(bad);
@@ -780,8 +785,9 @@ void launch_olsen_step(int64_t n, int nc, const double* theta, const double* hii
 }
 
 void launch_dgks_check(int r, int nb1, const double* S1, int ld1, int nb2, const double* S2, int ld2, const double* G,
-                       int ldg, int* need, hipStream_t st) {
-  hipLaunchKernelGGL(dgks_check_kernel, dim3(1), dim3(256), 0, st, r, nb1, S1, ld1, nb2, S2, ld2, G, ldg, need);
+                       int ldg, int* need, hipStream_t st, double* bnorm) {
+  hipLaunchKernelGGL(dgks_check_kernel, dim3(1), dim3(256), 0, st, r, nb1, S1, ld1, nb2, S2, ld2, G, ldg, need,
+                     bnorm);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -791,10 +797,10 @@ void launch_unit_vectors(int nc, const int64_t* idx, double* V, int64_t ld, hipS
 }
 
 void launch_orth_coeffs(int r, const double* G, int ldg, double* M, int ldm, double* work, int* kept, double tol,
-                        hipStream_t st, double* cond) {
+                        hipStream_t st, double* cond, const double* bnorm) {
   if (r > SYEV_NMAX) throw Error(ADC2G_EARG, "orth_coeffs: block too large");
   hipLaunchKernelGGL(orth_coeffs_kernel, dim3(1), dim3(SYEV_THREADS), 0, st, r, G, ldg, M, ldm, work, kept, tol,
-                     cond);
+                     cond, bnorm);
   HIP_CHECK(hipGetLastError());
 }
 

@@ -42,15 +42,19 @@ void launch_unit_vectors(int nc, const int64_t* idx, double* V, int64_t ld, hipS
 // with relative eigenvalue below tol.  Returns r_kept through kept[0].
 // Runs on one workgroup; scratch: work (see .hip).
 // cond (optional, device) receives lambda_max / lambda_min of the kept directions.
+// bnorm (optional, device): squared norms of the columns before a projection
+// that made T; a column that kept less than tol of its norm through it is
+// rounding noise and is dropped.
 void launch_orth_coeffs(int r, const double* G, int ldg, double* M, int ldm, double* work, int* kept,
-                        double tol, hipStream_t st, double* cond = nullptr);
+                        double tol, hipStream_t st, double* cond = nullptr, const double* bnorm = nullptr);
 
 // Block Gram-Schmidt re-orthogonalisation test (DGKS form): after one
 // projection T <- T - V1 S1 - V2 S2 of an orthonormal [V1 V2], need[0] = 1 if
 // some column j kept less than 1e-6 of its squared norm (G = T^T T
-// afterwards); above that, one pass leaves it orthogonal to ~1e-13.
+// afterwards); above that, one pass leaves it orthogonal to ~1e-13.  bnorm
+// (optional, device, r) receives the squared norms before the projection.
 void launch_dgks_check(int r, int nb1, const double* S1, int ld1, int nb2, const double* S2, int ld2, const double* G,
-                       int ldg, int* need, hipStream_t st);
+                       int ldg, int* need, hipStream_t st, double* bnorm = nullptr);
 
 // Copy the upper block of a symmetric matrix to the lower: R[j + ld*i] = R[i + ld*j]
 // for i in [c0, c1), j < c0... (rows c0..c1 of the new columns)
