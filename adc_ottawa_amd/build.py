@@ -79,11 +79,30 @@ def build_fortran(verbose: bool = True) -> str:
     return DRIVER
 
 
+FRONTEND_DIR = os.path.join(HERE, "frontend")
+INTS_LIB = os.path.join(FRONTEND_DIR, "libadc2g_ints.so")
+
+
+def build_frontend(verbose: bool = True) -> str:
+    """Compile the molecular front end's AO integral engine (host C, OpenMP)."""
+    src = os.path.join(FRONTEND_DIR, "ints.c")
+    tmp = INTS_LIB + ".tmp"
+    cmd = [os.environ.get("CC", "gcc"), "-O3", "-fPIC", "-shared", "-fopenmp", "-Wall", "-o", tmp, src, "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"front-end build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, INTS_LIB)
+    if verbose:
+        print(f"built {INTS_LIB}", file=sys.stderr)
+    return INTS_LIB
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=min(8, os.cpu_count() or 1))
     ap.add_argument("--no-fortran", action="store_true")
     a = ap.parse_args()
     build(a.jobs)
+    build_frontend()
     if not a.no_fortran:
         build_fortran()

@@ -68,6 +68,17 @@ int32_t adc2g_create_cvs(adc2g_handle* full, int32_t ncore, const int32_t* core,
   });
 }
 
+int32_t adc2g_create_frozen_core(adc2g_handle* full, int32_t nfrozen, const int32_t* frozen, adc2g_handle** out) {
+  return guarded([&] {
+    ARG_CHECK(full && out, "NULL argument");
+    *out = nullptr;
+    HIP_CHECK(hipSetDevice(full->h.device));
+    auto hp = std::make_unique<adc2g_handle>();
+    hp->h.create_frozen_core(full->h, nfrozen, frozen);
+    *out = hp.release();
+  });
+}
+
 int32_t adc2g_destroy(adc2g_handle* h) {
   return guarded([&] {
     if (!h) return;

@@ -769,16 +769,22 @@ __global__ void view_gather_kernel(int64_t nv, int nc, const int64_t* __restrict
 // source/adclib/get_matrix.f90:1323-1726, calls the same element functions), so
 // the view is the principal sub-matrix on these configurations, in the order of
 // the full space.
+// Orbital flags of a view: 1-based occupied indices, each at most once.
+static std::vector<char> view_orbitals(int o, int n, const int32_t* orb, const char* what) {
+  ARG_CHECK(n >= 1 && n < o && orb, std::string("need 1 <= n < nocc ") + what + " orbitals");
+  std::vector<char> is(o, 0);
+  for (int c = 0; c < n; ++c) {
+    ARG_CHECK(orb[c] >= 1 && orb[c] <= o, std::string(what) + " orbitals are occupied orbitals 1..nocc");
+    ARG_CHECK(!is[orb[c] - 1], std::string(what) + " orbital listed twice");
+    is[orb[c] - 1] = 1;
+  }
+  return is;
+}
+
 void Handle::create_cvs(Handle& full, int ncore, const int32_t* core) {
   ARG_CHECK(!full.parent, "a view cannot be made from another view");
   const int o = full.d.o, v = full.d.v;
-  ARG_CHECK(ncore >= 1 && ncore < o && core, "need 1 <= ncore < nocc core orbitals");
-  std::vector<char> isc(o, 0);
-  for (int c = 0; c < ncore; ++c) {
-    ARG_CHECK(core[c] >= 1 && core[c] <= o, "core orbitals are occupied orbitals 1..nocc");
-    ARG_CHECK(!isc[core[c] - 1], "core orbital listed twice");
-    isc[core[c] - 1] = 1;
-  }
+  const std::vector<char> isc = view_orbitals(o, ncore, core, "core");
   const Dims& F = full.d;
   std::vector<int64_t> map;
   map.reserve((size_t)F.ndim / 4);
@@ -786,27 +792,80 @@ void Handle::create_cvs(Handle& full, int ncore, const int32_t* core) {
   for (int a = 0; a < v; ++a)
     for (int i = 0; i < o; ++i)
       if (isc[i]) map.push_back((int64_t)a * o + i);
-  vcnt[0] = vcnt[1] = 0;
+  int64_t cnt[5] = {0, 0, 0, 0, 0};
   // t3 (a = b, i < j): (i, j) outer, a inner
   int64_t pij = 0;
-  int64_t c3 = 0, c4 = 0;
   for (int i = 0; i < o; ++i)
     for (int j = i + 1; j < o; ++j, ++pij)
       if (isc[i] + isc[j] == 1)
-        for (int a = 0; a < v; ++a, ++c3) map.push_back(F.off[2] + pij * v + a);
+        for (int a = 0; a < v; ++a, ++cnt[2]) map.push_back(F.off[2] + pij * v + a);
   // t4 then t5 (a < b, i < j): (i, j) outer, (a, b) inner
   for (int t = 3; t <= 4; ++t) {
     pij = 0;
     for (int i = 0; i < o; ++i)
       for (int j = i + 1; j < o; ++j, ++pij)
         if (isc[i] + isc[j] == 1)
-          for (int64_t pab = 0; pab < F.npv; ++pab) {
-            map.push_back(F.off[t] + pij * F.npv + pab);
-            if (t == 3) ++c4;
-          }
+          for (int64_t pab = 0; pab < F.npv; ++pab, ++cnt[t]) map.push_back(F.off[t] + pij * F.npv + pab);
   }
-  vcnt[2] = c3;
-  vcnt[3] = vcnt[4] = c4;
+  finish_view(full, map, (int64_t)ncore * v, cnt);
+}
+
+// Frozen-core configuration space (lifrzcore in select_atom_is, select_atom_d:
+// source/adclib/select_fano.f90:27-40, 575, 610, 635-642, 665-672): every
+// configuration none of whose holes is a frozen orbital, in the loop order of
+// the full space.  Only the configurations are restricted: the matrix
+// elements, and the sums over occupied orbitals inside them, are those of the
+// full matrix, so the view is the principal sub-matrix on these
+// configurations.  (The reference's MP2 ground state with frozen core sums
+// over the active occupied orbitals only, source/adclib/mp2.f90:84-90: that is
+// the MP2 of the system without the frozen orbitals.)
+void Handle::create_frozen_core(Handle& full, int nfrozen, const int32_t* frozen) {
+  ARG_CHECK(!full.parent, "a view cannot be made from another view");
+  const int o = full.d.o, v = full.d.v;
+  const std::vector<char> isf = view_orbitals(o, nfrozen, frozen, "frozen");
+  const Dims& F = full.d;
+  std::vector<int64_t> map;
+  map.reserve((size_t)F.ndim);
+  for (int a = 0; a < v; ++a)
+    for (int i = 0; i < o; ++i)
+      if (!isf[i]) map.push_back((int64_t)a * o + i);
+  int64_t cnt[5] = {0, 0, 0, 0, 0};
+  // t1 (a = b, i = j): i outer, a inner
+  for (int i = 0; i < o; ++i)
+    for (int a = 0; a < v; ++a)
+      if (!isf[i]) {
+        map.push_back(F.off[0] + (int64_t)i * v + a);
+        ++cnt[0];
+      }
+  // t2 (a < b, i = j): (a, b) outer, i inner
+  for (int64_t pab = 0; pab < F.npv; ++pab)
+    for (int i = 0; i < o; ++i)
+      if (!isf[i]) {
+        map.push_back(F.off[1] + pab * o + i);
+        ++cnt[1];
+      }
+  // t3 (a = b, i < j): (i, j) outer, a inner; t4, t5 (a < b, i < j): (i, j) outer, (a, b) inner
+  int64_t pij = 0;
+  for (int i = 0; i < o; ++i)
+    for (int j = i + 1; j < o; ++j, ++pij)
+      if (!isf[i] && !isf[j])
+        for (int a = 0; a < v; ++a, ++cnt[2]) map.push_back(F.off[2] + pij * v + a);
+  for (int t = 3; t <= 4; ++t) {
+    pij = 0;
+    for (int i = 0; i < o; ++i)
+      for (int j = i + 1; j < o; ++j, ++pij)
+        if (!isf[i] && !isf[j])
+          for (int64_t pab = 0; pab < F.npv; ++pab, ++cnt[t]) map.push_back(F.off[t] + pij * F.npv + pab);
+  }
+  finish_view(full, map, (int64_t)(o - nfrozen) * v, cnt);
+}
+
+// A view on the configurations map[] of the full space (n1 of them 1h1p, then
+// cnt[0..4] of t1..t5, each group in the order of the full space).
+void Handle::finish_view(Handle& full, const std::vector<int64_t>& map, int64_t n1v, const int64_t cnt[5]) {
+  const int o = full.d.o, v = full.d.v;
+  const Dims& F = full.d;
+  for (int t = 0; t < 5; ++t) vcnt[t] = cnt[t];
   parent = &full;
   ++full.nviews;
   device = full.device;
@@ -818,7 +877,7 @@ void Handle::create_cvs(Handle& full, int ncore, const int32_t* core) {
   d.o = o;
   d.v = v;
   d.naux = F.naux;
-  d.n1 = (int64_t)ncore * v;
+  d.n1 = n1v;
   d.ndim = nv;
   d.n2 = nv - d.n1;
   for (int t = 0; t < 5; ++t) d.cnt[t] = vcnt[t];

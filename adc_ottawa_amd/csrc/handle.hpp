@@ -98,6 +98,8 @@ struct Handle {
   DevBuf<double> vX, vW;          // parent-size staging (columns of a chunk)
   int64_t vcnt[5] = {0, 0, 0, 0, 0};
   void create_cvs(Handle& full, int ncore, const int32_t* core);
+  void create_frozen_core(Handle& full, int nfrozen, const int32_t* frozen);
+  void finish_view(Handle& full, const std::vector<int64_t>& map, int64_t n1v, const int64_t cnt[5]);
   void sigma_view(const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, Prec p, hipStream_t s);
 
   Handle() = default;

@@ -262,6 +262,13 @@ module adc2g_mod
        integer(c_int32_t), intent(in) :: core(*)
        type(c_ptr), intent(out) :: out
      end function adc2g_create_cvs
+     integer(c_int32_t) function adc2g_create_frozen_core(full, nfrozen, frozen, out) bind(c, name='adc2g_create_frozen_core')
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: full
+       integer(c_int32_t), value :: nfrozen
+       integer(c_int32_t), intent(in) :: frozen(*)
+       type(c_ptr), intent(out) :: out
+     end function adc2g_create_frozen_core
 
      ! MP2 energy, D2 diagnostic and density (rho: c_loc of nbas x nbas or c_null_ptr)
      integer(c_int32_t) function adc2g_mp2(h, e_mp2, d2, rho) bind(c, name='adc2g_mp2')

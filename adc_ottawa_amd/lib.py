@@ -101,7 +101,7 @@ SYMBOLS = [
     "adc2g_precon_get", "adc2g_precon_set", "adc2g_precon_eval", "adc2g_last_error",
     "adc2g_sigma_profile", "adc2g_precon_absorb", "adc2g_coupling_form", "adc2g_cheby_auto",
     "adc2g_block_lanczos", "adc2g_relax", "adc2g_write_hmlt", "adc2g_davidson_vectors",
-    "adc2g_sil_autocorr_orders", "adc2g_create_cvs", "adc2g_mp2", "adc2g_fvector", "adc2g_lanczos_guess",
+    "adc2g_sil_autocorr_orders", "adc2g_create_cvs", "adc2g_create_frozen_core", "adc2g_mp2", "adc2g_fvector", "adc2g_lanczos_guess",
     "adc2g_spectral_bounds", "adc2g_power", "adc2g_coupled_schur", "adc2g_sigma_device_noalloc",
 ]
 
@@ -148,6 +148,7 @@ def load(path: str = LIBPATH):
                                      C.POINTER(i32)]
     lib.adc2g_davidson_vectors.argtypes = [vp, C.POINTER(DavParams), vp, vp, vp, C.POINTER(DavStats)]
     lib.adc2g_create_cvs.argtypes = [vp, i32, vp, C.POINTER(vp)]
+    lib.adc2g_create_frozen_core.argtypes = [vp, i32, vp, C.POINTER(vp)]
     lib.adc2g_mp2.argtypes = [vp, vp, vp, vp]
     lib.adc2g_fvector.argtypes = [vp, vp, vp]
     lib.adc2g_lanczos_guess.argtypes = [vp, i32, i32, i32, vp]
@@ -424,6 +425,11 @@ class Operator:
         close it before this operator."""
         return CvsOperator(self, core)
 
+    def frozen_core(self, frozen):
+        """Frozen-core view: the configurations with no hole in the orbitals
+        ``frozen`` (1-based occupied indices); close it before this operator."""
+        return FrozenCoreOperator(self, frozen)
+
     def sigma_device(self, nvec, dX, dW, precision="fp64", stream=None):
         """W = H X on device pointers (ints), asynchronous on ``stream``."""
         check(_lib.adc2g_sigma_device(self._h, nvec, C.c_void_p(dX), C.c_void_p(dW), PREC[precision],
@@ -481,15 +487,17 @@ class Operator:
         return m, g, loss.value
 
 
-class CvsOperator(Operator):
-    """CVS view of an Operator (``adc2g_create_cvs``): the principal sub-matrix
-    on the core-hole configurations, with the same solver methods."""
+class _View(Operator):
+    """A configuration-subspace view of an Operator: the principal sub-matrix
+    on the view's configurations, with the same solver methods."""
 
-    def __init__(self, parent, core):
+    _create = None
+
+    def __init__(self, parent, orbitals):
         lib = load()
         self._h = C.c_void_p()
-        core = np.ascontiguousarray(core, dtype=np.int32)
-        check(lib.adc2g_create_cvs(parent.handle, core.size, _ptr(core), C.byref(self._h)))
+        orbs = np.ascontiguousarray(orbitals, dtype=np.int32)
+        check(getattr(lib, self._create)(parent.handle, orbs.size, _ptr(orbs), C.byref(self._h)))
         self._parent = parent
         nd, n1 = C.c_int64(), C.c_int64()
         cnt = (C.c_int64 * 5)()
@@ -497,4 +505,22 @@ class CvsOperator(Operator):
         self.ndim, self.n1, self.counts = nd.value, n1.value, list(cnt)
         self.nocc, self.nvirt, self.naux = parent.nocc, parent.nvirt, parent.naux
         self.coupling = parent.coupling
-        self.core = [int(c) for c in core]
+        self.orbitals = [int(c) for c in orbs]
+
+
+class CvsOperator(_View):
+    """CVS view (``adc2g_create_cvs``) on the core orbitals (1-based occupied
+    indices): the core-hole configurations."""
+
+    _create = "adc2g_create_cvs"
+
+    @property
+    def core(self):
+        return self.orbitals
+
+
+class FrozenCoreOperator(_View):
+    """Frozen-core view (``adc2g_create_frozen_core``) of the frozen orbitals
+    (1-based occupied indices): the configurations with no hole in them."""
+
+    _create = "adc2g_create_frozen_core"

@@ -213,6 +213,19 @@ int32_t adc2g_destroy(adc2g_handle* h);
  * full operator must outlive the view. */
 int32_t adc2g_create_cvs(adc2g_handle* full, int32_t ncore, const int32_t* core, adc2g_handle** out);
 
+/* Frozen-core view of a full operator: every configuration none of whose
+ * holes is one of the nfrozen frozen orbitals (1-based occupied indices in
+ * frozen[]), each group (1h1p, t1..t5) in the order of the full space.
+ * Replaces the lifrzcore selection of select_atom_is / select_atom_d
+ * (source/adclib/select_fano.f90:27-40, 575, 610, 635-642, 665-672), which
+ * restricts only the configurations: the matrix elements, with their sums
+ * over all occupied orbitals, are those of the full matrix, so the view is
+ * the principal sub-matrix on these configurations.  (The reference's MP2
+ * with frozen core, source/adclib/mp2.f90:84-90, sums over the active
+ * occupied orbitals: adc2g_mp2 of a handle made without the frozen orbitals.)
+ * The same solver entry points as a CVS view accept it. */
+int32_t adc2g_create_frozen_core(adc2g_handle* full, int32_t nfrozen, const int32_t* frozen, adc2g_handle** out);
+
 /* Sizes: ndim, n1, and the counts of t1..t5 (counts[0..4]); kpq(1..5,0) of
  * source/adclib/select_fano.f90 with kpq(5,0) repeated for t5. */
 int32_t adc2g_dims(const adc2g_handle* h, int64_t* ndim, int64_t* n1, int64_t counts[5]);

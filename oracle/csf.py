@@ -99,3 +99,16 @@ def cvs_indices(o: int, v: int, core) -> np.ndarray:
             if isc[i] + isc[j] == 1:
                 out.extend(off + p * npv + q for q in range(npv))
     return np.asarray(out, dtype=np.int64)
+
+
+def frozen_core_indices(o: int, v: int, frozen) -> np.ndarray:
+    """Full-space indices of the frozen-core configurations, in the full
+    order: those none of whose holes is a frozen orbital (the lifrzcore
+    cycles of select_atom_is / select_atom_d, source/adclib/select_fano.f90:
+    27-40, 575, 610, 635-642, 665-672).  ``frozen`` holds 0-based occupied
+    indices."""
+    typ, a, b, k, l = table(o, v)
+    isf = np.zeros(o, dtype=bool)
+    isf[np.asarray(frozen, dtype=int)] = True
+    bad = isf[k] | ((l >= 0) & isf[np.maximum(l, 0)])
+    return np.where(~bad)[0].astype(np.int64)
