@@ -36,6 +36,8 @@
 #include <cstdio>
 
 #include "coupled.hpp"
+
+#include <type_traits>
 #include "linalg.hpp"
 #include "solvers.hpp"
 
@@ -167,6 +169,72 @@ __global__ void __launch_bounds__(CP_THREADS) cp_unit_pairs_kernel(int o, int v,
   }
 }
 
+// The same tensor with one workgroup per (c, vec): every V value its o rows
+// read -- R[x] = V[c][a'][x] for x = (b,l) and S[b][m] = V[b][a'][c o + m] --
+// is staged in LDS by coalesced loads (the per-row kernel above gathers the
+// l = k' and b-strided terms one element per cache line), then the o rows
+// of n1 elements are written in order.  Each thread walks its (b, l) by a
+// fixed stride (no integer division per element), and the BF16/FP32 tensors
+// are formed in FP32 arithmetic (their rounding is coarser than its error).
+// LDS: (2 n1 + nbas) doubles.
+template <class T>
+__global__ void __launch_bounds__(CP_THREADS) cp_unit_pairs_lds_kernel(int o, int v, int nb, int j0,
+                                                                       const double* __restrict__ V,
+                                                                       const double* __restrict__ O,
+                                                                       const double* __restrict__ eps, double sigma,
+                                                                       T* __restrict__ Z) {
+  typedef typename std::conditional<std::is_same<T, double>::value, double, float>::type C;
+  extern __shared__ double sm[];
+  const int n1 = o * v;
+  double* R = sm;
+  double* S = sm + n1;
+  double* ev = S + n1;
+  const int vec = blockIdx.x % nb, c = blockIdx.x / nb;
+  const int j = j0 + vec, ap = j / o, kp = j % o;
+  const double* vr = V + ((int64_t)c * v + ap) * n1;
+  for (int x = threadIdx.x; x < n1; x += CP_THREADS) R[x] = vr[x];
+  for (int t = threadIdx.x; t < n1; t += CP_THREADS) {
+    const int b = t / o, m = t - b * o;
+    S[t] = V[((int64_t)b * v + ap) * n1 + (int64_t)c * o + m];
+  }
+  for (int t = threadIdx.x; t < o + v; t += CP_THREADS) ev[t] = eps[t];
+  __syncthreads();
+  const int qs = CP_THREADS / o, rs = CP_THREADS - qs * o;  // the stride CP_THREADS as (b, l) steps
+  const int b0 = threadIdx.x / o, l0 = threadIdx.x - b0 * o;
+  for (int k = 0; k < o; ++k) {
+    T* zr = Z + ((int64_t)(c * nb + vec) * o + k) * n1;
+    const C base = (C)(ev[o + c] - ev[k] - sigma);
+    int b = b0, l = l0;
+    for (int bl = threadIdx.x; bl < n1; bl += CP_THREADS) {
+      double g1 = 0.0, g2 = 0.0, g3 = 0.0, g4 = 0.0;
+      if (k == kp) {
+        g1 += R[bl];          // (c a'|b l)
+        g3 += S[b * o + l];   // (b a'|c l)
+      }
+      if (c == ap) {
+        g1 -= O[((int64_t)k * o + kp) * n1 + bl];                  // (k k'|b l)
+        g4 -= O[((int64_t)l * o + kp) * n1 + (int64_t)b * o + k];  // (l k'|b k)
+      }
+      if (l == kp) {
+        g2 += S[b * o + k];   // (b a'|c k)
+        g4 += R[b * o + k];   // (c a'|b k)
+      }
+      if (b == ap) {
+        g2 -= O[((int64_t)l * o + kp) * n1 + (int64_t)c * o + k];  // (l k'|c k)
+        g3 -= O[((int64_t)k * o + kp) * n1 + (int64_t)c * o + l];  // (k k'|c l)
+      }
+      const C num = (C)(2.0 * g1 + 2.0 * g2 - g3 - g4);
+      zr[bl] = (T)(num / (base + (C)(ev[o + b] - ev[l])));
+      l += rs;
+      b += qs;
+      if (l >= o) {
+        l -= o;
+        ++b;
+      }
+    }
+  }
+}
+
 // A[:, j0 + c] = H11[:, j0 + c] - M[:, c] (n1 x nb; H11 symmetric)
 __global__ void cp_a0_cols_kernel(int n1, int nb, int j0, const double* __restrict__ H11,
                                   const double* __restrict__ M, double* __restrict__ A) {
@@ -250,20 +318,40 @@ void CoupledPrecon::form_schur(double s, Prec bp) {
     const int o = Hd.d.o, v = Hd.d.v;
     for (int j0 = 0; j0 < n1; j0 += nc) {
       const int nb = std::min(nc, n1 - j0);
-      const dim3 grid((unsigned)v * nb * o);
-      switch (bp) {
-        case Prec::F64:
-          hipLaunchKernelGGL(cp_unit_pairs_kernel<double>, grid, blk, 0, st, o, v, nb, j0, V, O, Hd.eps.get(), sigma,
-                             (double*)Hd.ZG.get());
-          break;
-        case Prec::F32:
-          hipLaunchKernelGGL(cp_unit_pairs_kernel<float>, grid, blk, 0, st, o, v, nb, j0, V, O, Hd.eps.get(), sigma,
-                             (float*)Hd.ZG.get());
-          break;
-        case Prec::BF16:
-          hipLaunchKernelGGL(cp_unit_pairs_kernel<__bf16>, grid, blk, 0, st, o, v, nb, j0, V, O, Hd.eps.get(), sigma,
-                             (__bf16*)Hd.ZG.get());
-          break;
+      // staged variant when its LDS fits 64 KB (o = 16, v = 200: 52 KB)
+      const size_t lds = sizeof(double) * ((size_t)2 * n1 + o + v);
+      if (lds <= 65536) {
+        const dim3 grid((unsigned)v * nb);
+        switch (bp) {
+          case Prec::F64:
+            hipLaunchKernelGGL(cp_unit_pairs_lds_kernel<double>, grid, blk, lds, st, o, v, nb, j0, V, O, Hd.eps.get(),
+                               sigma, (double*)Hd.ZG.get());
+            break;
+          case Prec::F32:
+            hipLaunchKernelGGL(cp_unit_pairs_lds_kernel<float>, grid, blk, lds, st, o, v, nb, j0, V, O, Hd.eps.get(),
+                               sigma, (float*)Hd.ZG.get());
+            break;
+          case Prec::BF16:
+            hipLaunchKernelGGL(cp_unit_pairs_lds_kernel<__bf16>, grid, blk, lds, st, o, v, nb, j0, V, O,
+                               Hd.eps.get(), sigma, (__bf16*)Hd.ZG.get());
+            break;
+        }
+      } else {
+        const dim3 grid((unsigned)v * nb * o);
+        switch (bp) {
+          case Prec::F64:
+            hipLaunchKernelGGL(cp_unit_pairs_kernel<double>, grid, blk, 0, st, o, v, nb, j0, V, O, Hd.eps.get(), sigma,
+                               (double*)Hd.ZG.get());
+            break;
+          case Prec::F32:
+            hipLaunchKernelGGL(cp_unit_pairs_kernel<float>, grid, blk, 0, st, o, v, nb, j0, V, O, Hd.eps.get(), sigma,
+                               (float*)Hd.ZG.get());
+            break;
+          case Prec::BF16:
+            hipLaunchKernelGGL(cp_unit_pairs_kernel<__bf16>, grid, blk, 0, st, o, v, nb, j0, V, O, Hd.eps.get(),
+                               sigma, (__bf16*)Hd.ZG.get());
+            break;
+        }
       }
       HIP_CHECK(hipGetLastError());
       HIP_CHECK(hipMemsetAsync(Bm.get(), 0, sizeof(double) * n1 * nb, st));
