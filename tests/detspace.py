@@ -224,3 +224,38 @@ def isr_orders(eps: np.ndarray, eri: np.ndarray, nocc: int, dpl: np.ndarray, h: 
         out[k] = c.reshape((3,) + arr.shape[1:])
     out["f1"] = out["f1"].reshape(3, v, o)
     return out
+
+
+def csf_first_order(eps: np.ndarray, eri: np.ndarray, nocc: int, pair_tensor):
+    """(C1, M22): first-order coupling <Phi_ak|V|Phi_J> and 2h2p block
+    <Phi_J|V - E1|Phi_J'> between the determinant-space states of the
+    singlet CSFs: Phi_ak = g_ak/sqrt(2) Phi0 and, for a 2h2p CSF J,
+    Phi_J proportional to sum_cbkl Z_J(c,b,k,l) g_ck g_bl Phi0 with
+    Z_J = pair_tensor(e_J), normalised.  At first order the intermediate
+    states are these states, so C1 and M22 are the first-order ISR blocks."""
+    nb = eps.size
+    o, v = nocc, nb - nocc
+    ds, H, e0 = _fock_hamiltonian(eps, eri, nocc)
+    V = H - np.diag(e0)
+    g = {(p, q): ds.one_body(p, q) for p in range(nb) for q in range(nb)}
+    phi0 = np.zeros(len(ds.dets))
+    phi0[ds.ref] = 1.0
+    P1 = np.array([g[o + a, k] @ phi0 / np.sqrt(2.0) for a in range(v) for k in range(o)]).T
+    cols = []
+    n2 = _n2(o, v)
+    for J in range(n2):
+        e = np.zeros(n2)
+        e[J] = 1.0
+        Z = pair_tensor(e, o, v)
+        s = np.zeros_like(phi0)
+        for c, b, k, l in zip(*np.nonzero(Z)):
+            s += Z[c, b, k, l] * (g[o + c, k] @ (g[o + b, l] @ phi0))
+        cols.append(s / np.linalg.norm(s))
+    P2 = np.array(cols).T
+    E1 = phi0 @ V @ phi0
+    return P1.T @ V @ P2, P2.T @ V @ P2 - E1 * np.eye(n2), P2.T @ P2
+
+
+def _n2(o: int, v: int) -> int:
+    npv, npo = v * (v - 1) // 2, o * (o - 1) // 2
+    return o * v + npv * o + npo * v + 2 * npo * npv

@@ -690,3 +690,33 @@ def test_oracle_isr_brute_force(o, v, naux, seed):
         scale = np.abs(ref).max()
         assert np.abs(got[2] - ref).max() < 1e-9 * scale, key
         assert max(np.abs(got[0]).max(), np.abs(got[1]).max()) < 1e-12 * scale, key
+
+
+@pytest.mark.parametrize("o,v,naux,seed", [(2, 3, 6, 3), (2, 4, 5, 8), (3, 3, 6, 1)])
+def test_oracle_csf_states_and_adc2x_block(o, v, naux, seed):
+    """The configuration basis itself and the ADC(2)-x block, against the
+    determinant space (tests/detspace.py csf_first_order): each 2h2p CSF is
+    the normalised state sum_cbkl Z_J(c,b,k,l) g_ck g_bl Phi0 of its pair
+    tensor (oracle/dfsigma.py pair_tensor), each 1h1p CSF g_ak/sqrt(2) Phi0.
+    * The 2h2p states are orthonormal, and the first-order coupling
+      <Phi_ak|V|Phi_J> equals the reference's element functions C1..C5
+      (oracle/elements.py) element by element: this fixes the order, the
+      phases and the two a!=b, k!=l spin couplings of the reference's CSFs.
+    * The first-order 2h2p block <Phi_J|V - E1|Phi_J'> (the ADC(2)-x block)
+      equals the pair-space form of oracle/adcx.py element by element, and
+      the reference's element functions restated literally (the diagonal of
+      t1, t2, t3, t5 and the (t5,t1), (t3,t2) blocks) equal both."""
+    from oracle import adcx
+    from tests.detspace import csf_first_order
+    sys_ = make_system(o, v, naux, seed=seed)
+    C1, M1, S = csf_first_order(sys_.eps, mp2.integrals(sys_), o, dfsigma.pair_tensor)
+    assert np.abs(S - np.eye(S.shape[0])).max() < 1e-14
+    blk = elements.build_blocks(sys_, apply_minc=False)
+    assert np.abs(C1 - blk.C).max() < 1e-13 * np.abs(blk.C).max()
+    scale = np.abs(M1).max()
+    assert np.abs(adcx.block22_dense(sys_) - M1).max() < 1e-13 * scale
+    d = adcx.diagonal_first_order(sys_)
+    m = ~np.isnan(d)
+    assert np.abs(d[m] - np.diag(M1)[m]).max() < 1e-13 * scale
+    for r, c, val in adcx.offdiagonal_blocks(sys_).values():
+        assert np.abs(M1[np.ix_(r, c)] - val).max() < 1e-13 * scale
