@@ -79,6 +79,14 @@ int32_t adc2g_create_frozen_core(adc2g_handle* full, int32_t nfrozen, const int3
   });
 }
 
+int32_t adc2g_set_adc2x(adc2g_handle* h, int32_t enable) {
+  return guarded([&] {
+    ARG_CHECK(h, "NULL handle");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    h->h.enable_adcx(enable != 0);
+  });
+}
+
 int32_t adc2g_destroy(adc2g_handle* h) {
   return guarded([&] {
     if (!h) return;

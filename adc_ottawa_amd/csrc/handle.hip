@@ -395,6 +395,7 @@ void Handle::sigma(const double* X, int64_t ldx, double* W, int64_t ldw, int nve
     else
       sigma_df(X + ldx * c0, ldx, W + ldw * c0, ldw, nc, p, s);
   }
+  if (adcx && part == SIGMA_ALL) adcx_apply(X, ldx, W, ldw, nvec, s);
 }
 
 void Handle::sigma_part(const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, Prec p, int which,

@@ -102,6 +102,18 @@ struct Handle {
   void finish_view(Handle& full, const std::vector<int64_t>& map, int64_t n1v, const int64_t cnt[5]);
   void sigma_view(const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, Prec p, hipStream_t s);
 
+  // ADC(2)-x: the first-order 2h2p-2h2p block added to the sigma-vector
+  // (FP64, one vector at a time) and its diagonal to hii.  Integral matrices
+  // (built on enable): ax_vvvv [(c,b)][(d,e)] = (cd|be), ax_oooo [(m,n)][(k,l)]
+  // = (km|ln), ax_ov [(c,k)][(d,m)] = (ck|dm), ax_vo [(c,k)][(d,m)] = (cd|km),
+  // ax_a1 = 2 ax_ov - ax_vo; per-vector layouts of T = 2Z - XZ and of the
+  // three partial results (adcx.hip).
+  bool adcx = false;
+  DevBuf<double> ax_vvvv, ax_oooo, ax_ov, ax_vo, ax_a1, ax_d2;
+  DevBuf<double> ax_t1, ax_t3, ax_t4, ax_g1, ax_g2, ax_g3;
+  void enable_adcx(bool on);
+  void adcx_apply(const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, hipStream_t s);
+
   Handle() = default;
   ~Handle();
 

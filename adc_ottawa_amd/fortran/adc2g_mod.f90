@@ -269,6 +269,11 @@ module adc2g_mod
        integer(c_int32_t), intent(in) :: frozen(*)
        type(c_ptr), intent(out) :: out
      end function adc2g_create_frozen_core
+     integer(c_int32_t) function adc2g_set_adc2x(h, enable) bind(c, name='adc2g_set_adc2x')
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: enable
+     end function adc2g_set_adc2x
 
      ! MP2 energy, D2 diagnostic and density (rho: c_loc of nbas x nbas or c_null_ptr)
      integer(c_int32_t) function adc2g_mp2(h, e_mp2, d2, rho) bind(c, name='adc2g_mp2')

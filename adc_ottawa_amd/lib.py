@@ -101,7 +101,7 @@ SYMBOLS = [
     "adc2g_precon_get", "adc2g_precon_set", "adc2g_precon_eval", "adc2g_last_error",
     "adc2g_sigma_profile", "adc2g_precon_absorb", "adc2g_coupling_form", "adc2g_cheby_auto",
     "adc2g_block_lanczos", "adc2g_relax", "adc2g_write_hmlt", "adc2g_davidson_vectors",
-    "adc2g_sil_autocorr_orders", "adc2g_create_cvs", "adc2g_create_frozen_core", "adc2g_mp2", "adc2g_fvector", "adc2g_lanczos_guess",
+    "adc2g_sil_autocorr_orders", "adc2g_create_cvs", "adc2g_create_frozen_core", "adc2g_set_adc2x", "adc2g_mp2", "adc2g_fvector", "adc2g_lanczos_guess",
     "adc2g_spectral_bounds", "adc2g_power", "adc2g_coupled_schur", "adc2g_sigma_device_noalloc",
 ]
 
@@ -149,6 +149,7 @@ def load(path: str = LIBPATH):
     lib.adc2g_davidson_vectors.argtypes = [vp, C.POINTER(DavParams), vp, vp, vp, C.POINTER(DavStats)]
     lib.adc2g_create_cvs.argtypes = [vp, i32, vp, C.POINTER(vp)]
     lib.adc2g_create_frozen_core.argtypes = [vp, i32, vp, C.POINTER(vp)]
+    lib.adc2g_set_adc2x.argtypes = [vp, i32]
     lib.adc2g_mp2.argtypes = [vp, vp, vp, vp]
     lib.adc2g_fvector.argtypes = [vp, vp, vp]
     lib.adc2g_lanczos_guess.argtypes = [vp, i32, i32, i32, vp]
@@ -424,6 +425,11 @@ class Operator:
         """CVS view on the core orbitals ``core`` (1-based occupied indices);
         close it before this operator."""
         return CvsOperator(self, core)
+
+    def set_adc2x(self, enable=True):
+        """Switch the operator to ADC(2)-x (first-order 2h2p-2h2p block in the
+        sigma-vector and the diagonal) or back to ADC(2)-s."""
+        check(_lib.adc2g_set_adc2x(self._h, 1 if enable else 0))
 
     def frozen_core(self, frozen):
         """Frozen-core view: the configurations with no hole in the orbitals

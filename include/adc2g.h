@@ -226,6 +226,18 @@ int32_t adc2g_create_cvs(adc2g_handle* full, int32_t ncore, const int32_t* core,
  * The same solver entry points as a CVS view accept it. */
 int32_t adc2g_create_frozen_core(adc2g_handle* full, int32_t nfrozen, const int32_t* frozen, adc2g_handle** out);
 
+/* ADC(2)-x: with enable = 1 the operator's sigma-vector (every solver entry
+ * point, and views made afterwards) includes the first-order 2h2p-2h2p block
+ * of get_offdiag_adc2ext_save (source/adclib/get_matrix.f90:2435-2893, element
+ * functions C_1_1 .. C_4ii_4ii of source/adclib/adc_ph.f90:1102-1645), and the
+ * diagonal gains its diagonal (get_diag_adc2ext_direct); enable = 0 returns to
+ * ADC(2)-s.  Full operator only, before any view is made.  The block is
+ * applied in FP64 one vector at a time and needs the v^4 (vv|vv) matrix in
+ * device memory (12.8 GB at v = 200), built on the first enable.  The coupled
+ * and learned-coupled Davidson steps assume a diagonal 2h2p block and are
+ * rejected; the no-allocation sigma-vector does not include the block. */
+int32_t adc2g_set_adc2x(adc2g_handle* h, int32_t enable);
+
 /* Sizes: ndim, n1, and the counts of t1..t5 (counts[0..4]); kpq(1..5,0) of
  * source/adclib/select_fano.f90 with kpq(5,0) repeated for t5. */
 int32_t adc2g_dims(const adc2g_handle* h, int64_t* ndim, int64_t* n1, int64_t counts[5]);

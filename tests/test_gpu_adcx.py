@@ -1,0 +1,86 @@
+"""ADC(2)-x on the GPU (adc2g_set_adc2x) against the oracle.
+
+The oracle's first-order 2h2p-2h2p block (oracle/adcx.py) is pinned element
+by element against the determinant-space first-order block on the CSF
+states, and against the reference's element functions restated literally
+(tests/test_oracle_pins.py).  Here: the GPU sigma-vector with the block,
+for both coupling forms, equals the ADC(2)-s oracle plus the block; the
+diagonal gains the block's diagonal; the Davidson roots equal a dense
+eigendecomposition of the ADC(2)-x matrix; switching back restores ADC(2)-s;
+at the benzene size the operator stays symmetric and linear.
+"""
+import numpy as np
+import pytest
+
+from adc_ottawa_amd import lib
+from adc_ottawa_amd.synth import SIZES, make_system
+from oracle import adcx, elements
+
+pytestmark = pytest.mark.gpu
+
+
+def _dense_x(sys_):
+    blk = elements.build_blocks(sys_, apply_minc=False)
+    n1 = blk.n1
+    H = np.zeros((blk.ndim, blk.ndim))
+    H[:n1, :n1] = blk.H11
+    H[:n1, n1:] = blk.C
+    H[n1:, :n1] = blk.C.T
+    H[n1:, n1:] = np.diag(blk.D) + adcx.block22_dense(sys_)
+    return blk, H
+
+
+@pytest.mark.parametrize("form", ["int", "df"])
+@pytest.mark.parametrize("o,v,naux,seed", [(2, 3, 6, 3), (3, 5, 9, 1), (4, 7, 12, 2)])
+def test_adcx_sigma_and_diagonal(form, o, v, naux, seed):
+    print(f"[case] adcx sigma o={o} v={v} coupling={form}", flush=True)
+    sys_ = make_system(o, v, naux, seed=seed)
+    blk, H = _dense_x(sys_)
+    X = np.random.default_rng(seed).standard_normal((blk.ndim, 3))
+    with lib.Operator.from_system(sys_, coupling=form) as op:
+        W_s = op.sigma(X)
+        d_s = op.diagonal()
+        op.set_adc2x(True)
+        W_x = op.sigma(X)
+        d_x = op.diagonal()
+        with pytest.raises(lib.Adc2gError):
+            op.davidson(2, blocksize=2, maxvec=8, precon="coupled")
+        op.set_adc2x(False)
+        W_back = op.sigma(X)
+        d_back = op.diagonal()
+    ref = H @ X
+    assert np.abs(W_x - ref).max() < 1e-11 * np.abs(ref).max()
+    assert np.abs(d_x - np.diag(H)).max() < 1e-12 * np.abs(np.diag(H)).max()
+    assert np.abs(W_back - W_s).max() == 0.0 and np.abs(d_back - d_s).max() < 1e-14 * np.abs(d_s).max()
+    assert np.abs(W_x - W_s).max() > 1e-6 * np.abs(ref).max()       # the block is not trivially zero
+
+
+def test_adcx_davidson_roots():
+    """ADC(2)-x roots (o=4, v=7) by the GPU Davidson against eigh of the dense matrix."""
+    print("[case] adcx davidson o=4 v=7", flush=True)
+    sys_ = make_system(4, 7, 12, seed=5)
+    _, H = _dense_x(sys_)
+    w = np.linalg.eigvalsh(H)[:5]
+    with lib.Operator.from_system(sys_) as op:
+        op.set_adc2x(True)
+        ev, X, st = op.davidson(5, blocksize=8, maxvec=48, tol=1e-8, precon="olsen", maxiter=500)
+    print("ADC(2)-x roots", ev, "iterations", st["niter"])
+    assert np.abs(ev - w).max() < 1e-9
+    res = np.linalg.norm(H @ X - X * ev, axis=0)
+    assert res.max() < 1e-8 * 1.01
+
+
+def test_adcx_benzene_symmetric_linear():
+    sz = SIZES["benzene"]
+    print(f"[case] adcx benzene {sz}", flush=True)
+    sys_ = make_system(seed=0, **sz)
+    rng = np.random.default_rng(7)
+    with lib.Operator.from_system(sys_) as op:
+        op.set_adc2x(True)
+        X = rng.standard_normal((op.ndim, 2))
+        W = op.sigma(X)
+        W2 = op.sigma(2.0 * X[:, 0] - 3.0 * X[:, 1])
+    a, b = X[:, 1] @ W[:, 0], X[:, 0] @ W[:, 1]
+    print(f"x1.Hx0 = {a:.10e}, x0.Hx1 = {b:.10e}")
+    assert abs(a - b) < 1e-11 * np.abs(W).max() * np.sqrt(op.ndim) * 10
+    assert np.abs(W2 - (2.0 * W[:, 0] - 3.0 * W[:, 1])).max() < 1e-11 * np.abs(W).max() * 5
