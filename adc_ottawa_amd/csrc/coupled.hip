@@ -274,6 +274,16 @@ void CoupledPrecon::init(Handle& Hd, Prec p) {
   const double eo_max = *std::max_element(e.begin(), e.begin() + Hd.d.o);
   const double ev_min = *std::min_element(e.begin() + Hd.d.o, e.end());
   dmin = 2.0 * ev_min - 2.0 * eo_max;
+  if (Hd.adcx) {
+    // ADC(2)-x: the step is that of the matrix with the 2h2p block replaced
+    // by its diagonal (hii: D plus the first-order block's diagonal), whose
+    // minimum bounds the shift
+    std::vector<double> h2(n - n1);
+    HIP_CHECK(hipMemcpyAsync(h2.data(), Hd.hii.get() + n1, sizeof(double) * h2.size(), hipMemcpyDeviceToHost,
+                             Hd.st));
+    HIP_CHECK(hipStreamSynchronize(Hd.st));
+    dmin = *std::min_element(h2.begin(), h2.end());
+  }
   nc = std::min(64, std::max(1, Hd.chunk));
   Xb.alloc((size_t)n * nc);
   Wb.alloc((size_t)n * nc);
@@ -307,9 +317,10 @@ void CoupledPrecon::form_schur(double s, Prec bp) {
   const auto t0 = clk::now();
   sigma = s;
   const dim3 blk(CP_THREADS);
-  if (Hd.form == Handle::FORM_INT) {
+  if (Hd.form == Handle::FORM_INT && !Hd.adcx) {
     // pair tensors of P C^T e_j straight from the integral blocks, then the
-    // down contractions: A0 e_j = H11 e_j - C P C^T e_j
+    // down contractions: A0 e_j = H11 e_j - C P C^T e_j (P from orbital
+    // energies; with ADC(2)-x the half sigma-vectors below use hii)
     Hd.ensure_lowprec(bp);
     const size_t es = bp == Prec::F64 ? 8 : (bp == Prec::F32 ? 4 : 2);
     Hd.ZG.alloc((size_t)n1 * n1 * nc * es);

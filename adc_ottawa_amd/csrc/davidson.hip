@@ -342,7 +342,6 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
   const bool coupled = p.precon == ADC2G_PRE_COUPLED || p.precon == ADC2G_PRE_LEARNED_COUPLED;
   ARG_CHECK(!((learned || coupled) && H.parent),
             "the learned and coupled preconditioners work on the full operator only");
-  ARG_CHECK(!(coupled && H.adcx), "the coupled step assumes the diagonal 2h2p block of ADC(2)-s, not ADC(2)-x");
   if (learned && !precon_ready(H))
     throw Error(ADC2G_ESTATE, "learned preconditioner requested but no network is loaded");
   const int nstates = p.nstates;

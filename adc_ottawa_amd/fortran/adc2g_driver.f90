@@ -32,6 +32,10 @@
 !   ncore = n, core = i1, ..., in: CVS on these 1-based core orbitals (lcvs);
 !            every later step (solvers, F-vector, propagation) uses the CVS
 !            configurations;
+!   nfrozen = n: frozen core (lifrzcore) on the n lowest occupied orbitals:
+!            every later step uses the configurations with no hole in them;
+!   adc2x = .true.: ADC(2)-x (method = adc2-x): the first-order 2h2p-2h2p
+!            block in the matrix (set before the CVS / frozen-core view);
 !   dpl_file = 'd.bin': the F-vector of this nbas x nbas column-major operator
 !            matrix (get_modifiedtm_adc2), computed by the library, gives the
 !            transition moments and oscillator strengths of the states
@@ -49,8 +53,8 @@ program adc2g_driver
   integer :: precon = 2, precision = 0, krydim = 10
   integer :: solver = 1, guess = 0, subdim = 800, rlx_krydim = 10, autoord = 0
   integer :: lmain = 0, ncycles = 0, chebyord = 0, lancguess = 1, orthotype = 0, integrator = 1
-  integer :: ncore = 0, core(16) = 0
-  logical :: deflate = .true., sil = .false., lanczos = .false., hmlt = .false.
+  integer :: ncore = 0, core(16) = 0, nfrozen = 0, ifr
+  logical :: deflate = .true., sil = .false., lanczos = .false., hmlt = .false., adc2x = .false.
   real(c_double) :: tol = 1.0d-7, tout = 0.1d0, tfinal = 1.0d0, autotol = 1.0d-5
   real(c_double) :: stepsize = 10.0d0, cheby_emin = 0.0d0, cheby_emax = 1.0d0, siltol = 1.0d-5, ehf = 0.0d0
   character(len=256) :: eps_file = 'eps.bin', b_file = 'B.bin', davname = 'davstates'
@@ -60,7 +64,7 @@ program adc2g_driver
        precon, precision, deflate, davname, sil, fvec_file, auto_file, tout, tfinal, krydim, autotol, &
        solver, guess, subdim, stepsize, rlx_krydim, autoord, lanczos, lmain, ncycles, lancname, &
        chebyord, cheby_emin, cheby_emax, hmlt, dpl_file, lancguess, orthotype, integrator, siltol, &
-       ncore, core, ehf
+       ncore, core, ehf, nfrozen, adc2x
 
   type(c_ptr) :: h, hfull
   type(adc2g_opts) :: opts
@@ -101,6 +105,17 @@ program adc2g_driver
   call check(rc)
   deallocate(B)
   hfull = c_null_ptr
+  if (adc2x) then
+     call check(adc2g_set_adc2x(h, 1_c_int32_t))
+     write(*, '(a)') ' ADC(2)-x: first-order 2h2p-2h2p block included'
+  end if
+  if (nfrozen > 0 .and. ncore == 0) then
+     ! frozen core (lifrzcore): the configurations with no hole in the frozen orbitals
+     hfull = h
+     call check(adc2g_create_frozen_core(hfull, int(nfrozen, c_int32_t), &
+          int((/ (ifr, ifr = 1, nfrozen) /), c_int32_t), h))
+     write(*, '(a,i4)') ' frozen core orbitals: 1 ..', nfrozen
+  end if
   if (ncore > 0) then
      ! CVS (lcvs): every later step works on the core-hole configurations
      hfull = h
@@ -168,8 +183,8 @@ program adc2g_driver
      end do
   end if
   call adc2g_write_davstates(trim(davname), ndim, nstates, eigval, eigvec)
-  if (ncore == 0) then
-     ! table2 / wrstateinfo (misc.f90:648-852): 'davstates.dat'
+  if (ncore == 0 .and. nfrozen == 0) then
+     ! table2 / wrstateinfo (misc.f90:648-852): 'davstates.dat' (full space)
      call check(adc2g_mp2(h, emp2, d2mp, c_null_ptr))
      call write_state_table('davstates.dat')
   end if

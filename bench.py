@@ -457,6 +457,37 @@ def fvector_measure(op):
             "number_operator_max": float(np.abs(f0).max())}
 
 
+def adc2x_measure(op, nvec=20, roots=0):
+    """ADC(2)-x on the workload's operator (adc2g_set_adc2x): the set-up (the
+    (vv|vv), (oo|oo) and ring matrices and the diagonal), and FP64
+    sigma-vectors of a block of nvec vectors with the first-order 2h2p-2h2p
+    block; with roots > 0 also a Davidson (Olsen, block = roots, subspace 10
+    blocks, 1e-7, deflation).  Switched off again afterwards."""
+    rng = np.random.default_rng(3)
+    X = np.asfortranarray(rng.standard_normal((op.ndim, nvec)))
+    t0 = time.perf_counter()
+    op.set_adc2x(True)
+    t_setup = time.perf_counter() - t0
+    try:
+        op.sigma(X[:, :2])
+        t0 = time.perf_counter()
+        op.sigma(X)
+        t_sig = time.perf_counter() - t0
+        out = {"setup_seconds": round(t_setup, 3), "block": nvec, "ms_per_sigma_vector": round(1e3 * t_sig / nvec, 3)}
+        log(f"adc2x: set-up {t_setup:.2f}s, {1e3 * t_sig / nvec:.2f} ms per sigma-vector")
+        if roots > 0:
+            t0 = time.perf_counter()
+            ev, _, st = op.davidson(roots, blocksize=roots, maxvec=10 * roots, tol=1e-7, precon="olsen",
+                                    deflate=True, maxiter=1000, vectors=False)
+            out["davidson"] = {"roots": roots, "seconds": round(time.perf_counter() - t0, 2),
+                               "iterations": st["niter"], "sigma_vectors": st["nsigma"],
+                               "eigenvalues_Eh": [float(x) for x in ev]}
+            log(f"adc2x davidson: {roots} roots in {out['davidson']['seconds']}s, {st['niter']} iterations")
+    finally:
+        op.set_adc2x(False)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -482,6 +513,9 @@ def main():
                     help="relax the lowest state in imaginary time to this residual (0 skips)")
     ap.add_argument("--skip-mp2", action="store_true", help="skip the MP2 energy/density timing")
     ap.add_argument("--skip-fvector", action="store_true", help="skip the F-vector timing")
+    ap.add_argument("--adc2x-roots", type=int, default=0,
+                    help="ADC(2)-x leg: also a Davidson to this many roots (the sigma-vector timing runs by "
+                         "default; -1 skips the leg)")
     ap.add_argument("--dav-roots", type=int, default=0,
                     help="Davidson comparison: roots (default 20; 40 for the large workload)")
     ap.add_argument("--dav-block", type=int, default=0,
@@ -589,6 +623,9 @@ def main():
     rlx = guarded("relax", lambda: relax_measure(op, args.relax)) if args.relax > 0 else None
     mp2r = guarded("mp2", lambda: mp2_measure(op)) if not args.skip_mp2 else None
     fvr = guarded("fvector", lambda: fvector_measure(op)) if not args.skip_fvector else None
+    # last: the ADC(2)-x leg switches the operator and back
+    adx = guarded("adc2x", lambda: adc2x_measure(op, roots=max(args.adc2x_roots, 0))) \
+        if args.adc2x_roots >= 0 else None
 
     n1 = op.n1
     bytes_per_sigma = 8.0 * (2 * op.ndim) + 8.0 * 2 * n1 * n1  # read x, write sigma, pair tensors
@@ -620,6 +657,7 @@ def main():
         "relaxation": rlx,
         "mp2": mp2r,
         "fvector": fvr,
+        "adc2x": adx,
         "operator_build_seconds": round(t_build, 3),
     }
     op.close()

@@ -323,3 +323,34 @@ def test_fortran_driver_state_table(tmp_path):
             c = float(row[-1])
             sgn = sgn if sgn is not None else np.sign(c * U[j, i])
             assert abs(c - sgn * U[j, i]) < 2e-5
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built")
+def test_fortran_driver_adc2x_frozen_core(tmp_path):
+    """adc2x = .true. with nfrozen = 1: the Davidson eigenvalues equal the
+    lowest eigenvalues of the oracle's ADC(2)-x matrix on the frozen-core
+    configurations (the reference's method = adc2-x with frozen_core)."""
+    from oracle import adcx, csf, elements
+    print("[case] fortran driver adc2x nfrozen=1 o=3 v=6 naux=9", flush=True)
+    o, v, naux = 3, 6, 9
+    sys_ = make_system(o, v, naux, seed=12)
+    sys_.eps.astype("<f8").tofile(tmp_path / "eps.bin")
+    sys_.B.astype("<f8").tofile(tmp_path / "B.bin")
+    nml = (f"&adc2g nocc={o} nvirt={v} naux={naux} eps_file='eps.bin' b_file='B.bin' nstates=3 "
+           f"blocksize=5 maxvec=30 tol=1d-10 precon=2 davname='davstates' adc2x=.true. nfrozen=1 /\n")
+    r = subprocess.run([DRIVER], input=nml, capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ADC(2)-x" in r.stdout
+    blk = elements.build_blocks(sys_, apply_minc=False)
+    n1 = blk.n1
+    H = np.zeros((blk.ndim, blk.ndim))
+    H[:n1, :n1] = blk.H11
+    H[:n1, n1:] = blk.C
+    H[n1:, :n1] = blk.C.T
+    H[n1:, n1:] = np.diag(blk.D) + adcx.block22_dense(sys_)
+    idx = csf.frozen_core_indices(o, v, [0])
+    w = np.linalg.eigvalsh(H[np.ix_(idx, idx)])
+    recs = read_davstates(tmp_path / "davstates", idx.size)
+    ev = np.array([x[1] for x in recs])
+    assert np.abs(ev - w[:3]).max() < 1e-10
