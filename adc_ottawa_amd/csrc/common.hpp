@@ -6,6 +6,8 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
+#include <type_traits>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -36,6 +38,17 @@ void set_last_error(const std::string& msg);
     if (!(cond)) throw ::adc2g::Error(ADC2G_EARG, (msg));     \
   } while (0)
 
+// ADC2G_POISON=1 fills every new floating-point device buffer with 0xff
+// bytes (NaN): a debugging aid that turns a read of memory no kernel has
+// written into NaN results.  Integer (index) buffers are left alone.
+inline bool poison_new_buffers() {
+  static const bool on = [] {
+    const char* e = std::getenv("ADC2G_POISON");
+    return e != nullptr && e[0] == '1';
+  }();
+  return on;
+}
+
 // Device buffer owned by the handle.
 template <class T>
 struct DevBuf {
@@ -62,6 +75,12 @@ struct DevBuf {
                                     hipGetErrorString(e));
     }
     n = count;
+    if constexpr (!std::is_integral<T>::value || std::is_same<T, char>::value) {
+      if (poison_new_buffers()) {
+        (void)hipMemset(p, 0xff, count * sizeof(T));
+        (void)hipDeviceSynchronize();
+      }
+    }
   }
   T* get() const { return p; }
 };

@@ -445,6 +445,10 @@ __device__ void syevx_dev(int n, int k, double* A, int lda, double* w, double* Z
         ML[(int64_t)i * k + j] = m;
       }
       U1[(int64_t)(n - 1) * k + j] = cu;
+      // the last row has no super-diagonals; the back substitution reads
+      // them (times zero), so they must not hold stale NaN or Inf
+      U2[(int64_t)(n - 1) * k + j] = 0.0;
+      U3[(int64_t)(n - 1) * k + j] = 0.0;
       for (int i = 0; i < n; ++i) {
         double& u = U1[(int64_t)i * k + j];
         if (fabs(u) < tiny) u = (u < 0.0) ? -tiny : tiny;
