@@ -1,6 +1,7 @@
-"""ADC(2)-s excitation energies of a molecule from its GAMESS output.
+"""ADC(2)-s (or, with --adc2x, ADC(2)-x) excitation energies of a molecule
+from its GAMESS output.
 
-    python -m adc_ottawa_amd.frontend gamess.log --frozen 11 --roots 10 [--cvs] [--block 20] [--maxvec 60]
+    python -m adc_ottawa_amd.frontend gamess.log --frozen 11 --roots 10 [--cvs] [--adc2x] [--block 20] [--maxvec 60]
 
 Reads the basis and geometry from the GAMESS log, computes the AO integrals,
 redoes the RHF, factorises the MO integrals (pivoted Cholesky to 1e-11 on the diagonal) and
@@ -32,8 +33,9 @@ def main():
     ap.add_argument("--precon", default="olsen", choices=("dpr", "olsen"))
     ap.add_argument("--guess", default="adc1", choices=("adc1", "ondiag", "subdiag"))
     ap.add_argument("--deflate", action="store_true")
+    ap.add_argument("--adc2x", action="store_true", help="ADC(2)-x: the first-order 2h2p-2h2p block")
     a = ap.parse_args()
-    r = run(a.log, a.frozen, a.roots, a.cvs, a.block, a.maxvec, a.tol, a.precon, a.guess, a.deflate)
+    r = run(a.log, a.frozen, a.roots, a.cvs, a.block, a.maxvec, a.tol, a.precon, a.guess, a.deflate, adc2x=a.adc2x)
     print(json.dumps(r, indent=1))
     print("excitation energies (eV): " + " ".join(f"{e:.6f}" for e in r["energies_eV"]), file=sys.stderr)
     return 0

@@ -1,6 +1,8 @@
-"""ADC(2)-s of a molecule through the GPU library: the frozen-core MP2
-energy and the block-Davidson roots of the reference's initial (frozen-core)
-or final (CVS) configuration space, with the reference's ADC(1) guess.
+"""ADC(2)-s or ADC(2)-x of a molecule through the GPU library: the
+frozen-core MP2 energy and the block-Davidson roots of the reference's
+initial (frozen-core) or final (CVS) configuration space, with the
+reference's ADC(1) guess.  ADC(2)-x (the reference's method = adc2-x) adds
+the first-order 2h2p-2h2p block (adc2g_set_adc2x) before the view is made.
 
 The reference's guess = adc1 (initvec_adc1, source/eigen/block_davidson.f90:
 246-289) starts the Davidson from the lowest eigenvectors of the ADC(1)
@@ -46,7 +48,7 @@ def adc1_guess(system, kind: str, orbitals, ndim: int, nblock: int) -> np.ndarra
 
 
 def run(log, frozen, roots, cvs=False, block=0, maxvec=0, tol=1e-7, precon="olsen", guess="adc1", deflate=False,
-        verbose=False):
+        verbose=False, adc2x=False):
     t0 = time.perf_counter()
     mol = gamess.read_gamess_log(log)
     ints = gamess.ao_integrals(mol)
@@ -63,6 +65,9 @@ def run(log, frozen, roots, cvs=False, block=0, maxvec=0, tol=1e-7, precon="olse
     block = block or roots
     maxvec = maxvec or 4 * block
     with lib.Operator.from_system(full) as op:
+        if adc2x:
+            op.set_adc2x(True)
+        out["method"] = "adc2-x" if adc2x else "adc2-s"
         view = op.cvs(orbs) if kind == "cvs" else (op.frozen_core(orbs) if kind == "frozen" else None)
         target = view if view is not None else op
         try:

@@ -100,3 +100,43 @@ def test_large_block_no_spurious_roots():
     print(f"{st['niter']} iterations, roots {got[0]:.4f} .. {got[-1]:.4f} eV", flush=True)
     assert got[0] > 400.0 and st["resnorm_max"] < 1e-7
     assert np.abs(got[:46] - ref[:46]).max() < 1e-4
+
+
+def test_n2_adc2x_frozen_core():
+    """ADC(2)-x (the reference's method = adc2-x) of N2 / aug-cc-pVDZ on the
+    frozen-core space, through the front end: every Davidson eigenpair
+    satisfies the oracle's ADC(2)-x eigen-equation (density-fitted ADC(2)-s
+    sigma-vector plus the first-order 2h2p-2h2p block, both restated in numpy)
+    on the frozen-core configurations, to the Davidson tolerance.  The
+    reference ships no ADC(2)-x output for an example; its element functions
+    pin the block at small sizes (test_oracle_pins)."""
+    from adc_ottawa_amd.frontend import adc
+    from oracle import adcx, csf, dfsigma
+    print("[case] front end n2 ADC(2)-x frozen core", flush=True)
+    log = os.path.join(GOLDEN, "n2_augccpvdz_gamess.txt")
+    mol = gamess.read_gamess_log(log)
+    ints = gamess.ao_integrals(mol)
+    full, _ = scf.mo_system(scf.rhf(mol, ints), ints)
+    o, v, k, orbs = full.nocc, full.nvirt, 8, [1, 2]
+    with lib.Operator.from_system(full) as op:
+        with op.frozen_core(orbs) as view:
+            g = adc1_guess(full, "frozen", orbs, view.ndim, 16)
+            ev_s, _, _ = view.davidson(k, blocksize=16, maxvec=64, tol=1e-7, precon="olsen", maxiter=1000,
+                                       vectors=False, guess=g)
+        op.set_adc2x(True)
+        with op.frozen_core(orbs) as view:
+            ev, vec, st = view.davidson(k, blocksize=16, maxvec=64, tol=1e-7, precon="olsen", maxiter=1000, guess=g)
+    idx = csf.frozen_core_indices(o, v, [0, 1])
+    assert vec.shape == (idx.size, k)
+    X = np.zeros((op.ndim, k))
+    X[idx] = vec
+    n1 = o * v
+    W = dfsigma.DF(full).sigma(X)
+    W[n1:] += adcx.block22(full, X[n1:])
+    res = np.linalg.norm(W[idx] - vec * ev, axis=0)
+    print(f"ADC(2)-x roots {np.round(ev * EH2EV, 5)} eV, {st['niter']} iterations, oracle residuals <= {res.max():.1e}; "
+          f"ADC(2)-s roots {np.round(ev_s * EH2EV, 5)} eV", flush=True)
+    assert res.max() < 1e-6 and np.abs(vec.T @ vec - np.eye(k)).max() < 1e-10
+    assert np.abs(ev - ev_s).max() > 1e-3        # the block changes the roots
+    r = adc.run(log, 2, k, block=16, maxvec=64, adc2x=True)
+    assert r["method"] == "adc2-x" and np.abs(np.sort(r["energies_eV"]) - np.sort(ev) * EH2EV).max() < 1e-5
