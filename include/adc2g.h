@@ -235,8 +235,9 @@ int32_t adc2g_create_frozen_core(adc2g_handle* full, int32_t nfrozen, const int3
  * applied in FP64 one vector at a time and needs the v^4 (vv|vv) matrix in
  * device memory (12.8 GB at v = 200), built on the first enable.  With it the
  * coupled Davidson step is that of the matrix whose 2h2p block is replaced by
- * its diagonal (an approximation of M(theta), formed from half sigma-vectors);
- * the no-allocation sigma-vector does not include the block. */
+ * its diagonal (an approximation of M(theta), formed from half sigma-vectors).
+ * The FP32 and BF16 sigma-vectors add the block in FP64 too; the
+ * no-allocation sigma-vector rejects an ADC(2)-x handle (ADC2G_EARG). */
 int32_t adc2g_set_adc2x(adc2g_handle* h, int32_t enable);
 
 /* Sizes: ndim, n1, and the counts of t1..t5 (counts[0..4]); kpq(1..5,0) of
