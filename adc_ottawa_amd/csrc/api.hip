@@ -233,6 +233,34 @@ int32_t adc2g_sil_autocorr(adc2g_handle* h, const adc2g_sil_params* p, const dou
   });
 }
 
+int32_t adc2g_set_dipole_is(adc2g_handle* h, int32_t component, const double* diag, int64_t nnz,
+                            const int64_t* row, const int64_t* col, const double* val, double d00,
+                            const double* d0j) {
+  return guarded([&] {
+    ARG_CHECK(h, "NULL handle");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    h->h.set_dipole_is(component, diag, nnz, row, col, val, d00, d0j);
+  });
+}
+
+int32_t adc2g_matxvec_treal_laser(adc2g_handle* h, double time, int32_t npulse, const adc2g_pulse* pulses,
+                                  const double* v1, double* v2) {
+  return guarded([&] {
+    ARG_CHECK(h && v1 && v2, "NULL argument");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    matxvec_treal_laser_host(h->h, time, npulse, pulses, v1, v2);
+  });
+}
+
+int32_t adc2g_sil_laser(adc2g_handle* h, const adc2g_laser_params* p, int32_t npulse, const adc2g_pulse* pulses,
+                        double* psi, double* norms, int64_t nrec_cap, int64_t* nrec_out, adc2g_sil_stats* stats) {
+  return guarded([&] {
+    ARG_CHECK(h && p && psi && norms, "NULL argument");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    sil_laser_run(h->h, *p, npulse, pulses, psi, norms, nrec_cap, nrec_out, stats);
+  });
+}
+
 int32_t adc2g_sil_autocorr_orders(adc2g_handle* h, const adc2g_sil_params* p, const double* fvec, int32_t autoord,
                                   double* auto0, double* auto1, double* auto2, int64_t nrec_cap, int64_t* nrec_out,
                                   adc2g_sil_stats* stats) {

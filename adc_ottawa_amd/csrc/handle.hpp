@@ -114,6 +114,21 @@ struct Handle {
   void enable_adcx(bool on);
   void adcx_apply(const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, hipStream_t s);
 
+  // Intermediate-state representations of the dipole components for laser
+  // propagation (laser.hip): diagonal, off-diagonal part in CSR with both
+  // triangles, ground-state element and column.
+  struct DipoleIS {
+    bool set = false;
+    double d00 = 0.0;
+    DevBuf<double> diag, val, d0j;
+    DevBuf<int64_t> rowptr;
+    DevBuf<int32_t> col;
+  };
+  DipoleIS dip[3];
+  DevBuf<double> laser_part;  // partial sums of the ground-state row
+  void set_dipole_is(int c, const double* diag, int64_t nnz, const int64_t* row, const int64_t* col,
+                     const double* val, double d00, const double* d0j);
+
   Handle() = default;
   ~Handle();
 

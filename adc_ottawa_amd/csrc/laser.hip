@@ -1,0 +1,290 @@
+// Laser-driven propagation: the time-dependent operator of propagate_laser_adc2
+// (source/propagation/proplib_adc2.f90:37-630).
+//
+// The wavefunction has ndim + 1 components: the intermediate states, then the
+// MP2 ground state Psi0.  With E(t) the sum of the pulses (efield,
+// source/propagation/tdselib.f90:850-1029) and D_c the intermediate-state
+// representation of dipole component c, matxvec_treal_laser
+// (tdselib.f90:569-720) applies
+//   IS rows:      -i H v + i sum_c E_c [(D_c + d00_c) v + d0j_c v0]
+//   ground row:            i sum_c E_c (d00_c v0 + d0j_c . v)
+// (<Psi0|H|Psi_J> = 0 and H00 = 0 for the ground-state-shifted H).  D_c comes
+// from the caller as the reference stores it (SCRATCH/dipole_<c>, read by
+// opxvec_ext, tdselib.f90:1031-1153): the diagonal, then each off-diagonal
+// element once, applied to (i, j) and (j, i).  Here it is a CSR matrix with
+// both triangles, one thread per row; vectors are planar complex (re column,
+// im column) with leading dimension ndim + 1, so -i H v is one two-column
+// sigma-vector of the first ndim rows.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "linalg.hpp"
+#include "solvers.hpp"
+
+namespace adc2g {
+
+namespace {
+
+struct LaserComp {
+  double E, d00;
+  const double* diag;
+  const double* val;
+  const double* d0j;
+  const int64_t* rowptr;
+  const int32_t* col;
+};
+
+struct LaserDev {
+  LaserComp c[3];
+  int nc;
+};
+
+__device__ double laser_block_sum(double x, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wv] = x;
+  __syncthreads();
+  double t = 0.0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += sh[i];
+  return t;
+}
+
+// intermediate-state rows: out = -i hv + i sum_c E_c [(D_c + d00_c) v + d0j_c v0]
+__global__ void laser_is_kernel(int64_t nd, int64_t n, const double* __restrict__ hv, const double* __restrict__ v,
+                                double* __restrict__ out, LaserDev L) {
+  const double v0re = v[nd], v0im = v[n + nd];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nd; i += (int64_t)gridDim.x * blockDim.x) {
+    double sre = 0.0, sim = 0.0;
+    for (int q = 0; q < L.nc; ++q) {
+      const LaserComp& c = L.c[q];
+      const double dd = c.diag[i] + c.d00;
+      double dre = dd * v[i], dim = dd * v[n + i];
+      for (int64_t p = c.rowptr[i]; p < c.rowptr[i + 1]; ++p) {
+        const int64_t j = c.col[p];
+        const double w = c.val[p];
+        dre += w * v[j];
+        dim += w * v[n + j];
+      }
+      dre += c.d0j[i] * v0re;
+      dim += c.d0j[i] * v0im;
+      sre += c.E * dre;
+      sim += c.E * dim;
+    }
+    // -i (hre + i him) = him - i hre;  i (sre + i sim) = -sim + i sre
+    out[i] = hv[n + i] - sim;
+    out[n + i] = -hv[i] + sre;
+  }
+}
+
+// partial sums of sum_c E_c d0j_c . v over the intermediate states
+__global__ void laser_ground_partial_kernel(int64_t nd, int64_t n, const double* __restrict__ v, LaserDev L,
+                                            double* __restrict__ part) {
+  __shared__ double sh[16];
+  double sre = 0.0, sim = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nd; i += (int64_t)gridDim.x * blockDim.x) {
+    double w = 0.0;
+    for (int q = 0; q < L.nc; ++q) w += L.c[q].E * L.c[q].d0j[i];
+    sre += w * v[i];
+    sim += w * v[n + i];
+  }
+  sre = laser_block_sum(sre, sh);
+  sim = laser_block_sum(sim, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sre;
+    part[2 * blockIdx.x + 1] = sim;
+  }
+}
+
+// ground row: out0 = i (sum_c E_c d00_c v0 + the partial sums)
+__global__ void laser_ground_final_kernel(int nblk, int64_t nd, int64_t n, const double* __restrict__ v, LaserDev L,
+                                          const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double sh[16];
+  double sre = 0.0, sim = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    sre += part[2 * b];
+    sim += part[2 * b + 1];
+  }
+  sre = laser_block_sum(sre, sh);
+  sim = laser_block_sum(sim, sh);
+  if (threadIdx.x == 0) {
+    double e00 = 0.0;
+    for (int q = 0; q < L.nc; ++q) e00 += L.c[q].E * L.c[q].d00;
+    const double gre = sre + e00 * v[nd], gim = sim + e00 * v[n + nd];
+    out[nd] = -gim;
+    out[n + nd] = gre;
+  }
+}
+
+double envelope_value(const adc2g_pulse& p, double t) {
+  const double pi = 3.14159265358979323846;
+  switch (p.envelope) {
+    case 1: {  // cos^2 about tzero, half-width fwhm
+      const double tz = p.envpar[0], fw = p.envpar[1];
+      if (t > tz - fw && t < tz + fw) {
+        const double c = std::cos(pi * (t - tz) / (2.0 * fw));
+        return c * c;
+      }
+      return 0.0;
+    }
+    case 2: {  // sin^2 ramp of length tau, then 1
+      const double tau = p.envpar[0];
+      if (t < 0.0) return 0.0;
+      if (t <= tau) {
+        const double s = std::sin(pi * t / (2.0 * tau));
+        return s * s;
+      }
+      return 1.0;
+    }
+    case 3:  // box on [ti, tf]
+      return (t < p.envpar[0] || t > p.envpar[1]) ? 0.0 : 1.0;
+    case 4: {  // sin^2 from tzero over 2 fwhm
+      const double tz = p.envpar[0], fw = p.envpar[1];
+      if (t > tz && t < tz + 2.0 * fw) {
+        const double s = std::sin(pi * (t - tz) / (2.0 * fw));
+        return s * s;
+      }
+      return 0.0;
+    }
+  }
+  return 0.0;
+}
+
+}  // namespace
+
+void laser_check_pulses(int npulse, const adc2g_pulse* pulses) {
+  ARG_CHECK(npulse >= 0, "npulse must be >= 0");
+  ARG_CHECK(npulse == 0 || pulses, "NULL pulses");
+  for (int i = 0; i < npulse; ++i) {
+    const adc2g_pulse& p = pulses[i];
+    ARG_CHECK(p.carrier == 1 || p.carrier == 2, "pulse carrier must be 1 (cosine) or 2 (sine)");
+    ARG_CHECK(p.envelope >= 1 && p.envelope <= 4, "pulse envelope must be 1 (cos^2), 2 (sin^2 ramp), 3 (box) or 4 (sin^2)");
+    ARG_CHECK((p.envelope != 1 && p.envelope != 4) || p.envpar[1] > 0.0, "the cos^2 / sin^2 envelope needs fwhm > 0");
+    ARG_CHECK(p.envelope != 2 || p.envpar[0] > 0.0, "the sin^2 ramp needs tau > 0");
+  }
+}
+
+void laser_efield(int npulse, const adc2g_pulse* pulses, double t, double E[3]) {
+  E[0] = E[1] = E[2] = 0.0;
+  for (int i = 0; i < npulse; ++i) {
+    const adc2g_pulse& p = pulses[i];
+    const double arg = p.freq * (t - p.t0) - p.phase;
+    const double carrier = (p.carrier == 1) ? std::cos(arg) : std::sin(arg);
+    const double f = p.strength * envelope_value(p, t) * carrier;
+    for (int c = 0; c < 3; ++c) E[c] += p.pulse_vec[c] * f;
+  }
+}
+
+void Handle::set_dipole_is(int c, const double* diag_h, int64_t nnz, const int64_t* row, const int64_t* colh,
+                           const double* valh, double d00, const double* d0j_h) {
+  ARG_CHECK(c >= 0 && c <= 2, "dipole component must be 0 (x), 1 (y) or 2 (z)");
+  ARG_CHECK(nnz >= 0, "nnz must be >= 0");
+  DipoleIS& D = dip[c];
+  if (nnz == 0 && !diag_h && !d0j_h && d00 == 0.0) {
+    D.set = false;
+    D.diag.release();
+    D.val.release();
+    D.d0j.release();
+    D.rowptr.release();
+    D.col.release();
+    D.d00 = 0.0;
+    return;
+  }
+  ARG_CHECK(diag_h && d0j_h, "the diagonal and the ground-state column are required");
+  ARG_CHECK(nnz == 0 || (row && colh && valh), "NULL off-diagonal arrays");
+  const int64_t nd = d.ndim;
+  ARG_CHECK(nd < ((int64_t)1 << 31), "the configuration space is too large for 32-bit column indices");
+  std::vector<int64_t> rp(nd + 1, 0);
+  for (int64_t k = 0; k < nnz; ++k) {
+    ARG_CHECK(row[k] >= 0 && row[k] < nd && colh[k] >= 0 && colh[k] < nd,
+              "off-diagonal element " + std::to_string(k) + " has an index outside 0 .. ndim-1");
+    ARG_CHECK(row[k] != colh[k], "off-diagonal element " + std::to_string(k) + " lies on the diagonal");
+    ++rp[row[k] + 1];
+    ++rp[colh[k] + 1];
+  }
+  for (int64_t i = 0; i < nd; ++i) rp[i + 1] += rp[i];
+  std::vector<int64_t> fill(rp.begin(), rp.end() - 1);
+  std::vector<int32_t> cl(std::max<int64_t>(2 * nnz, 1));
+  std::vector<double> vl(std::max<int64_t>(2 * nnz, 1));
+  for (int64_t k = 0; k < nnz; ++k) {
+    int64_t p = fill[row[k]]++;
+    cl[p] = (int32_t)colh[k];
+    vl[p] = valh[k];
+    p = fill[colh[k]]++;
+    cl[p] = (int32_t)row[k];
+    vl[p] = valh[k];
+  }
+  HIP_CHECK(hipSetDevice(device));
+  D.diag.alloc(nd);
+  D.d0j.alloc(nd);
+  D.rowptr.alloc(nd + 1);
+  D.col.alloc(cl.size());
+  D.val.alloc(vl.size());
+  HIP_CHECK(hipMemcpyAsync(D.diag.get(), diag_h, sizeof(double) * nd, hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipMemcpyAsync(D.d0j.get(), d0j_h, sizeof(double) * nd, hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipMemcpyAsync(D.rowptr.get(), rp.data(), sizeof(int64_t) * (nd + 1), hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipMemcpyAsync(D.col.get(), cl.data(), sizeof(int32_t) * cl.size(), hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipMemcpyAsync(D.val.get(), vl.data(), sizeof(double) * vl.size(), hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+  D.d00 = d00;
+  D.set = true;
+}
+
+void laser_apply(Handle& H, const double E[3], const double* v, double* hv, double* out, Prec p, hipStream_t s) {
+  const int64_t nd = H.d.ndim, n = nd + 1;
+  H.sigma(v, n, hv, n, 2, p, s);
+  LaserDev L{};
+  L.nc = 0;
+  for (int c = 0; c < 3; ++c) {
+    if (E[c] == 0.0) continue;
+    const Handle::DipoleIS& D = H.dip[c];
+    ARG_CHECK(D.set, std::string("the field has a ") + "xyz"[c] +
+                         " component but that dipole component is not set (adc2g_set_dipole_is)");
+    LaserComp& q = L.c[L.nc++];
+    q.E = E[c];
+    q.d00 = D.d00;
+    q.diag = D.diag.get();
+    q.val = D.val.get();
+    q.d0j = D.d0j.get();
+    q.rowptr = D.rowptr.get();
+    q.col = D.col.get();
+  }
+  hipLaunchKernelGGL(laser_is_kernel, dim3(grid_for(nd)), dim3(256), 0, s, nd, n, hv, v, out, L);
+  HIP_CHECK(hipGetLastError());
+  const int nb = L.nc > 0 ? reduce_blocks(nd) : 0;
+  H.laser_part.alloc(2 * (size_t)std::max(nb, 1));
+  if (nb > 0) {
+    hipLaunchKernelGGL(laser_ground_partial_kernel, dim3(nb), dim3(256), 0, s, nd, n, v, L, H.laser_part.get());
+    HIP_CHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(laser_ground_final_kernel, dim3(1), dim3(256), 0, s, nb, nd, n, v, L, H.laser_part.get(), out);
+  HIP_CHECK(hipGetLastError());
+}
+
+void matxvec_treal_laser_host(Handle& H, double t, int npulse, const adc2g_pulse* pulses, const double* v1,
+                              double* v2) {
+  laser_check_pulses(npulse, pulses);
+  const int64_t n = H.d.ndim + 1;
+  double E[3];
+  laser_efield(npulse, pulses, t, E);
+  std::vector<double> planar(2 * n);
+  for (int64_t i = 0; i < n; ++i) {
+    planar[i] = v1[2 * i];
+    planar[n + i] = v1[2 * i + 1];
+  }
+  DevBuf<double> dv, dh, dout;
+  dv.alloc(2 * n);
+  dh.alloc(2 * n);
+  dout.alloc(2 * n);
+  HIP_CHECK(hipMemcpyAsync(dv.get(), planar.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice, H.st));
+  laser_apply(H, E, dv.get(), dh.get(), dout.get(), Prec::F64, H.st);
+  HIP_CHECK(hipMemcpyAsync(planar.data(), dout.get(), sizeof(double) * 2 * n, hipMemcpyDeviceToHost, H.st));
+  HIP_CHECK(hipStreamSynchronize(H.st));
+  for (int64_t i = 0; i < n; ++i) {
+    v2[2 * i] = planar[i];
+    v2[2 * i + 1] = planar[n + i];
+  }
+}
+
+}  // namespace adc2g

@@ -21,6 +21,7 @@
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <functional>
 
 #include "linalg.hpp"
 #include "solvers.hpp"
@@ -156,8 +157,12 @@ struct Sil {
   int64_t nsigma = 0;
   int nb = 0;
 
-  Sil(Handle& h, int kd, Prec p) : H(h), kdim(kd), prec(p) {
-    n = H.d.ndim;
+  // the operator -i H v when set (laser propagation: ndim + 1 rows), else
+  // the handle's sigma-vector
+  std::function<void(const double*, double*)> op;
+
+  Sil(Handle& h, int kd, Prec p, int64_t nrows = 0) : H(h), kdim(kd), prec(p) {
+    n = nrows > 0 ? nrows : H.d.ndim;
     K.alloc((size_t)(kdim + 1) * 2 * n);
     psi.alloc(2 * n);
     hv.alloc(2 * n);
@@ -177,6 +182,11 @@ struct Sil {
 
   // out = -i H v
   void func(const double* v, double* out) {
+    if (op) {
+      op(v, out);
+      ++nsigma;
+      return;
+    }
     H.sigma(v, n, hv.get(), n, 2, prec, H.st);
     hipLaunchKernelGGL(minus_i_kernel, dim3(grid_for(n)), dim3(256), 0, H.st, n, hv.get(), out);
     HIP_CHECK(hipGetLastError());
@@ -386,6 +396,85 @@ void sil_run(Handle& H, const adc2g_sil_params& p, const double* fvec, double* a
                    (long long)nrec, i * intperiod, (long long)nsteps, (long long)S.nsigma);
       std::fflush(stderr);
     }
+  }
+  if (nrec_out) *nrec_out = nrec + 1;
+  if (stats) {
+    stats->nsteps = nsteps;
+    stats->nsigma = S.nsigma;
+    stats->max_order = maxorder;
+    stats->reserved = 0;
+    stats->time_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+}
+
+// propagate_wavepacket_sil (source/propagation/proplib_adc2.f90:486-630):
+// integration period tout; in each period SIL steps from the current time
+// until the period is covered (|intperiod - inttime| <= 1e-9 intperiod), each
+// with -i H(t) at its start time; a period in which |dpsi/dt| <= 1e-10 is
+// skipped (time advances by the whole period); the norm is recorded at the
+// start of every period and at the end.
+void sil_laser_run(Handle& H, const adc2g_laser_params& p, int npulse, const adc2g_pulse* pulses, double* psi_io,
+                   double* norms, int64_t nrec_cap, int64_t* nrec_out, adc2g_sil_stats* stats) {
+  const auto t0 = std::chrono::steady_clock::now();
+  ARG_CHECK(p.tout > 0 && p.tfinal >= 0, "tout must be > 0 and tfinal >= 0");
+  ARG_CHECK(p.kdim >= 1 && p.kdim <= 30, "krydim must be in 1..30");
+  ARG_CHECK(p.proptol > 0, "proptol must be > 0");
+  if (p.kdim < 2) throw Error(ADC2G_ESIL_ORDER, "IntOrder is too small");
+  laser_check_pulses(npulse, pulses);
+  const double intperiod = p.tout;
+  const int64_t nrec = (int64_t)(p.tfinal / intperiod);
+  ARG_CHECK(nrec_cap >= nrec + 1, "norms holds fewer than nrec+1 values");
+  const Prec prec = prec_from_int(p.precision);
+  const int64_t n = H.d.ndim + 1;
+  Sil S(H, p.kdim, prec, n);
+  DevBuf<double> hv;
+  hv.alloc(2 * (size_t)n);
+  double time = 0.0;
+  S.op = [&](const double* v, double* out) {
+    double E[3];
+    laser_efield(npulse, pulses, time, E);
+    laser_apply(H, E, v, hv.get(), out, prec, H.st);
+  };
+  std::vector<double> planar(2 * n);
+  for (int64_t i = 0; i < n; ++i) {
+    planar[i] = psi_io[2 * i];
+    planar[n + i] = psi_io[2 * i + 1];
+  }
+  HIP_CHECK(hipMemcpyAsync(S.Kp(0), planar.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice, H.st));
+  const double tiny = 1e-9, tinier = 1e-10;
+  int64_t nsteps = 0;
+  int maxorder = 0;
+  for (int64_t i = 1; i <= nrec; ++i) {
+    double inttime = 0.0;
+    while (true) {
+      const double stepsize = intperiod - inttime;
+      S.func(S.Kp(0), S.Kp(1));
+      if (inttime == 0.0) norms[i - 1] = std::sqrt(S.cdot(S.Kp(0), S.Kp(0)).real());
+      if (std::sqrt(S.cdot(S.Kp(1), S.Kp(1)).real()) > tinier) {
+        int order = 0;
+        const double truestep = S.step(stepsize, p.proptol, order);
+        ++nsteps;
+        maxorder = std::max(maxorder, order);
+        time += truestep;
+        inttime += truestep;
+        if (std::fabs(intperiod - inttime) > std::fabs(tiny * intperiod)) continue;
+      } else {
+        time += intperiod;
+      }
+      break;
+    }
+    if (p.verbose && (i % 100 == 0 || i == nrec)) {
+      std::fprintf(stderr, "sil laser: period %lld of %lld (t = %.2f a.u.), %lld steps\n", (long long)i,
+                   (long long)nrec, time, (long long)nsteps);
+      std::fflush(stderr);
+    }
+  }
+  norms[nrec] = std::sqrt(S.cdot(S.Kp(0), S.Kp(0)).real());
+  HIP_CHECK(hipMemcpyAsync(planar.data(), S.Kp(0), sizeof(double) * 2 * n, hipMemcpyDeviceToHost, H.st));
+  HIP_CHECK(hipStreamSynchronize(H.st));
+  for (int64_t i = 0; i < n; ++i) {
+    psi_io[2 * i] = planar[i];
+    psi_io[2 * i + 1] = planar[n + i];
   }
   if (nrec_out) *nrec_out = nrec + 1;
   if (stats) {

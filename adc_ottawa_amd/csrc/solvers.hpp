@@ -28,6 +28,18 @@ void sil_run(Handle& H, const adc2g_sil_params& p, const double* fvec, double* a
              int64_t* nrec_out, adc2g_sil_stats* stats, int autoord = 0, double* auto1 = nullptr,
              double* auto2 = nullptr);
 
+// Laser-driven propagation (laser.hip): E(t) of the pulses (host), the
+// operator -i H(t) on planar complex vectors of ndim + 1 rows (v, out: re
+// column then im column, leading dimension ndim + 1; hv: scratch of the same
+// shape), and the SIL driver.
+void laser_efield(int npulse, const adc2g_pulse* pulses, double t, double E[3]);
+void laser_check_pulses(int npulse, const adc2g_pulse* pulses);
+void laser_apply(Handle& H, const double E[3], const double* v, double* hv, double* out, Prec p, hipStream_t s);
+void matxvec_treal_laser_host(Handle& H, double t, int npulse, const adc2g_pulse* pulses, const double* v1,
+                              double* v2);
+void sil_laser_run(Handle& H, const adc2g_laser_params& p, int npulse, const adc2g_pulse* pulses, double* psi,
+                   double* norms, int64_t nrec_cap, int64_t* nrec_out, adc2g_sil_stats* stats);
+
 // Chebyshev order-domain autocorrelation C_0..C_2K (auto_out, host) of the
 // host vector q0 with the spectrum scaled by bounds[0..1] = (Emin, Emax).
 void cheby_run(Handle& H, const double bounds[2], const double* q0, int order, Prec p, double* auto_out);

@@ -69,6 +69,69 @@ class SilStats(C.Structure):
                 ("reserved", C.c_int32), ("time_total", C.c_double)]
 
 
+class Pulse(C.Structure):
+    """adc2g_pulse: one laser pulse of efield (tdselib.f90:850-1029)."""
+    _fields_ = [("carrier", C.c_int32), ("envelope", C.c_int32), ("pulse_vec", C.c_double * 3),
+                ("strength", C.c_double), ("freq", C.c_double), ("t0", C.c_double), ("phase", C.c_double),
+                ("envpar", C.c_double * 2)]
+
+
+CARRIER = {"cos": 1, "sin": 2}
+ENVELOPE = {"cos2": 1, "sin2ramp": 2, "box": 3, "sin2": 4}
+
+
+def pulses_array(pulses):
+    """ctypes array of adc2g_pulse from dicts with keys carrier ('cos'/'sin'),
+    envelope ('cos2', 'sin2ramp', 'box', 'sin2'), pulse_vec (3), strength,
+    freq, t0, phase and envpar (2)."""
+    arr = (Pulse * max(len(pulses), 1))()
+    for i, q in enumerate(pulses):
+        arr[i].carrier = CARRIER[q.get("carrier", "cos")]
+        arr[i].envelope = ENVELOPE[q["envelope"]]
+        for c in range(3):
+            arr[i].pulse_vec[c] = float(q["pulse_vec"][c])
+        arr[i].strength = float(q["strength"])
+        arr[i].freq = float(q.get("freq", 0.0))
+        arr[i].t0 = float(q.get("t0", 0.0))
+        arr[i].phase = float(q.get("phase", 0.0))
+        ep = list(q.get("envpar", (0.0, 0.0))) + [0.0, 0.0]
+        arr[i].envpar[0], arr[i].envpar[1] = float(ep[0]), float(ep[1])
+    return arr
+
+
+class LaserParams(C.Structure):
+    _fields_ = [("tout", C.c_double), ("tfinal", C.c_double), ("kdim", C.c_int32), ("precision", C.c_int32),
+                ("proptol", C.c_double), ("verbose", C.c_int32), ("reserved", C.c_int32)]
+
+
+def read_is_operator(path, ndim):
+    """An intermediate-state operator file as the reference writes them for
+    SCRATCH/dipole_<c> and SCRATCH/cap and opxvec_ext reads them
+    (source/propagation/tdselib.f90:1031-1153): Fortran sequential records
+    (4-byte length markers) buf_size; the ndim diagonal elements; then
+    records (oij(buf_size), indxi(buf_size), indxj(buf_size), nlim) with
+    1-based indices, each off-diagonal element once.  Returns (diag, row,
+    col, val) with 0-based indices, for Operator.set_dipole_is."""
+    raw = open(path, "rb").read()
+    recs, pos = [], 0
+    while pos < len(raw):
+        ln = int(np.frombuffer(raw, "<i4", 1, pos)[0])
+        recs.append(raw[pos + 4: pos + 4 + ln])
+        pos += 8 + ln
+    bs = int(np.frombuffer(recs[0], "<i4", 1)[0])
+    diag = np.frombuffer(recs[1], "<f8").copy()
+    if diag.size != ndim:
+        raise ValueError(f"{path}: {diag.size} diagonal elements, expected {ndim}")
+    rows, cols, vals = [], [], []
+    for r in recs[2:]:
+        nlim = int(np.frombuffer(r, "<i4", 1, 16 * bs)[0])
+        vals.append(np.frombuffer(r, "<f8", nlim, 0))
+        rows.append(np.frombuffer(r, "<i4", nlim, 8 * bs).astype(np.int64) - 1)
+        cols.append(np.frombuffer(r, "<i4", nlim, 12 * bs).astype(np.int64) - 1)
+    cat = (lambda x, t: np.concatenate(x).astype(t) if x else np.zeros(0, t))  # noqa: E731
+    return diag, cat(rows, np.int64), cat(cols, np.int64), cat(vals, np.float64)
+
+
 class LanczosParams(C.Structure):
     _fields_ = [("blocksize", C.c_int32), ("ncycles", C.c_int32), ("precision", C.c_int32),
                 ("check_orth", C.c_int32), ("orthotype", C.c_int32), ("reserved", C.c_int32)]
@@ -103,6 +166,7 @@ SYMBOLS = [
     "adc2g_block_lanczos", "adc2g_relax", "adc2g_write_hmlt", "adc2g_davidson_vectors",
     "adc2g_sil_autocorr_orders", "adc2g_create_cvs", "adc2g_create_frozen_core", "adc2g_set_adc2x", "adc2g_mp2", "adc2g_fvector", "adc2g_lanczos_guess",
     "adc2g_spectral_bounds", "adc2g_power", "adc2g_coupled_schur", "adc2g_sigma_device_noalloc",
+    "adc2g_set_dipole_is", "adc2g_matxvec_treal_laser", "adc2g_sil_laser",
 ]
 
 _lib = None
@@ -160,6 +224,10 @@ def load(path: str = LIBPATH):
     lib.adc2g_relax.argtypes = [vp, C.POINTER(RelaxParams), vp, vp, vp, vp, vp, C.POINTER(RelaxStats)]
     lib.adc2g_block_lanczos.argtypes = [vp, C.POINTER(LanczosParams), vp, vp, vp, vp, vp, vp,
                                         C.POINTER(LanczosStats)]
+    lib.adc2g_set_dipole_is.argtypes = [vp, i32, vp, i64, vp, vp, vp, C.c_double, vp]
+    lib.adc2g_matxvec_treal_laser.argtypes = [vp, C.c_double, i32, vp, vp, vp]
+    lib.adc2g_sil_laser.argtypes = [vp, C.POINTER(LaserParams), i32, vp, vp, vp, i64, C.POINTER(i64),
+                                    C.POINTER(SilStats)]
     for s in SYMBOLS:
         getattr(lib, s).restype = C.c_int32
     _lib = lib
@@ -430,6 +498,56 @@ class Operator:
         """Switch the operator to ADC(2)-x (first-order 2h2p-2h2p block in the
         sigma-vector and the diagonal) or back to ADC(2)-s."""
         check(_lib.adc2g_set_adc2x(self._h, 1 if enable else 0))
+
+    def set_dipole_is(self, component, diag, row, col, val, d00, d0j):
+        """Intermediate-state representation of dipole component 0/1/2 (x/y/z):
+        diagonal (ndim), off-diagonal elements once each (0-based row, col,
+        val), ground-state element d00 and column d0j (ndim).  diag = d0j =
+        None with no elements and d00 = 0 clears the component."""
+        row = np.ascontiguousarray(row, dtype=np.int64)
+        col = np.ascontiguousarray(col, dtype=np.int64)
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        if not (row.size == col.size == val.size):
+            raise ValueError("row, col and val must have the same length")
+        if diag is None and d0j is None:
+            if row.size or d00 != 0.0:
+                raise ValueError("diag and d0j are required unless the component is cleared")
+            check(_lib.adc2g_set_dipole_is(self._h, int(component), None, 0, None, None, None, 0.0, None))
+            return
+        diag = np.ascontiguousarray(diag, dtype=np.float64)
+        d0j = np.ascontiguousarray(d0j, dtype=np.float64)
+        if diag.shape != (self.ndim,) or d0j.shape != (self.ndim,):
+            raise ValueError("diag and d0j must have ndim elements")
+        check(_lib.adc2g_set_dipole_is(self._h, int(component), _ptr(diag), row.size, _ptr(row), _ptr(col),
+                                       _ptr(val), float(d00), _ptr(d0j)))
+
+    def matxvec_treal_laser(self, t, pulses, v1):
+        """-i H(t) v1 for complex v1 of ndim + 1 components (ground state last)."""
+        v1 = np.ascontiguousarray(v1, dtype=np.complex128)
+        if v1.shape != (self.ndim + 1,):
+            raise ValueError("v1 must have ndim + 1 components")
+        v2 = np.empty_like(v1)
+        arr = pulses_array(pulses)
+        check(_lib.adc2g_matxvec_treal_laser(self._h, float(t), len(pulses), C.cast(arr, C.c_void_p), _ptr(v1),
+                                             _ptr(v2)))
+        return v2
+
+    def sil_laser(self, psi0, pulses, tout, tfinal, kdim=20, proptol=1e-8, precision="fp64", verbose=False):
+        """SIL propagation of psi0 (ndim + 1 complex components) under the
+        pulses; returns (psi(tfinal), norms at i * tout, stats)."""
+        psi = np.array(psi0, dtype=np.complex128)
+        if psi.shape != (self.ndim + 1,):
+            raise ValueError("psi0 must have ndim + 1 components")
+        nrec = int(tfinal / tout)
+        norms = np.empty(nrec + 1)
+        p = LaserParams(tout, tfinal, kdim, PREC[precision], proptol, 1 if verbose else 0, 0)
+        arr = pulses_array(pulses)
+        n = C.c_int64()
+        st = SilStats()
+        check(_lib.adc2g_sil_laser(self._h, C.byref(p), len(pulses), C.cast(arr, C.c_void_p), _ptr(psi),
+                                   _ptr(norms), norms.size, C.byref(n), C.byref(st)))
+        stats = {f: getattr(st, f) for f, _ in SilStats._fields_}
+        return psi, norms[: n.value], stats
 
     def frozen_core(self, frozen):
         """Frozen-core view: the configurations with no hole in the orbitals

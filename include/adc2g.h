@@ -332,6 +332,67 @@ int32_t adc2g_sil_autocorr_orders(adc2g_handle* h, const adc2g_sil_params* p, co
                                   int32_t autoord, double* auto0, double* auto1, double* auto2,
                                   int64_t nrec_cap, int64_t* nrec_out, adc2g_sil_stats* stats);
 
+/* ---- Laser-driven propagation (propagate_laser_adc2,
+ * source/propagation/proplib_adc2.f90:37-630, with matxvec_treal_laser,
+ * source/propagation/tdselib.f90:569-720).  The wavefunction has ndim + 1
+ * complex components: the intermediate states in the handle's configuration
+ * order, then the MP2 ground state (statenumber = 0 starts from
+ * (0, ..., 0, 1)).  Complex arrays are interleaved (re, im). */
+
+/* One pulse of efield (tdselib.f90:850-897): E(t) = pulse_vec * strength *
+ * envelope(t) * carrier(t).  carrier: 1 cos(freq (t - t0) - phase), 2 sin of
+ * the same.  envelope: 1 cos^2 (envpar = tzero, fwhm: cos^2(pi (t - tzero) /
+ * (2 fwhm)) for |t - tzero| < fwhm), 2 sin^2 ramp (envpar[0] = tau:
+ * sin^2(pi t / (2 tau)) for 0 <= t <= tau, 1 after), 3 box (envpar = ti, tf:
+ * 1 for ti <= t <= tf), 4 sin^2 (envpar = tzero, fwhm: sin^2(pi (t - tzero) /
+ * (2 fwhm)) for tzero < t < tzero + 2 fwhm); 0 outside (tdselib.f90:904-1029). */
+typedef struct {
+  int32_t carrier;      /* ipulse: 1 cosine, 2 sine */
+  int32_t envelope;     /* ienvelope: 1 cos^2, 2 sin^2 ramp, 3 box, 4 sin^2 */
+  double pulse_vec[3];  /* polarisation (x, y, z) */
+  double strength, freq, t0, phase;
+  double envpar[2];
+} adc2g_pulse;
+
+typedef struct {
+  double tout;        /* output interval = integration period (intperiod = tout) */
+  double tfinal;      /* records at i * tout, i = 0 .. int(tfinal / tout) */
+  int32_t kdim;       /* maximum Krylov order of a SIL step, 2..30 */
+  int32_t precision;  /* ADC2G_PREC_* of the sigma-vectors */
+  double proptol;     /* SIL error tolerance */
+  int32_t verbose;
+  int32_t reserved;
+} adc2g_laser_params;
+
+/* The intermediate-state representation of dipole component c (0 x, 1 y,
+ * 2 z) that the reference reads from SCRATCH/dipole_<c> (opxvec_ext,
+ * tdselib.f90:1031-1153: the diagonal, then each off-diagonal element once
+ * and applied to both (i, j) and (j, i)): diag (ndim), nnz off-diagonal
+ * elements (row, col, val) with 0-based row != col, the ground-state element
+ * d00 and the ground-state column d0j (ndim).  Replaces any earlier set of c;
+ * nnz = 0 and diag = d0j = NULL with d00 = 0 clears it. */
+int32_t adc2g_set_dipole_is(adc2g_handle* h, int32_t component, const double* diag, int64_t nnz,
+                            const int64_t* row, const int64_t* col, const double* val, double d00,
+                            const double* d0j);
+
+/* v2 = -i H(t) v1 (ndim + 1 complex components, interleaved): the -i H of
+ * matxvec_treal on the intermediate states plus, for E(t) = sum of the pulses,
+ * + i sum_c E_c [(D_c + d00_c) v1 + d0j_c v1_0] on the intermediate states and
+ * + i sum_c E_c (d00_c v1_0 + d0j_c . v1) on the ground state
+ * (matxvec_treal_laser, tdselib.f90:569-720, without a CAP). */
+int32_t adc2g_matxvec_treal_laser(adc2g_handle* h, double time, int32_t npulse, const adc2g_pulse* pulses,
+                                  const double* v1, double* v2);
+
+/* SIL propagation under the pulses (propagate_wavepacket_sil,
+ * proplib_adc2.f90:486-630): psi (ndim + 1 complex, interleaved) in and
+ * out; the periods i = 1 .. int(tfinal / tout) each integrated by SIL steps
+ * with H at the step's start time (SILStep's Func(Time, ...)), a period
+ * skipped when |dpsi/dt| <= 1e-10; norms[i] = |psi| at i * tout for
+ * i = 0 .. nrec (the wrstepinfo values).  norms holds nrec_cap values. */
+int32_t adc2g_sil_laser(adc2g_handle* h, const adc2g_laser_params* p, int32_t npulse,
+                        const adc2g_pulse* pulses, double* psi, double* norms, int64_t nrec_cap,
+                        int64_t* nrec_out, adc2g_sil_stats* stats);
+
 /* Chebyshev order-domain autocorrelation function (chebyshev_auto,
  * source/propagation/chebyspec.f90:245-319, with the scaled product of
  * matxvec_chebyshev_incore, source/propagation/tdselib.f90:1316-1398):

@@ -1,0 +1,135 @@
+"""Laser-driven propagation on the GPU (adc2g_set_dipole_is,
+adc2g_matxvec_treal_laser, adc2g_sil_laser) against the oracle
+(oracle/laser.py, pinned by the constant-field exponential and the envelope
+widths in tests/test_oracle_pins.py).
+
+The reference computes the intermediate-state dipole matrices in its own
+D-matrix code and stores them in SCRATCH/dipole_<c>; here they are random
+sparse symmetric operators in that layout (diagonal + each off-diagonal
+element once), with random ground-state elements.  Tolerances: the operator
+to 1e-12 relative (FP64, summation order); propagations to 1e-8 (SIL step
+decisions follow the same scalars)."""
+import numpy as np
+import pytest
+import scipy.linalg
+
+from adc_ottawa_amd import lib
+from adc_ottawa_amd.synth import make_system
+from oracle import dfsigma, laser
+
+pytestmark = pytest.mark.gpu
+
+
+def _inputs(o, v, naux, seed, density=4):
+    sys_ = make_system(o, v, naux, seed=seed)
+    df = dfsigma.DF(sys_)
+    nd = df.n1 + df.D2.size
+    H = df.sigma(np.eye(nd))
+    H = 0.5 * (H + H.T)
+    rng = np.random.default_rng(seed)
+    dips, raw = [], []
+    for c in range(3):
+        row = rng.integers(0, nd, density * nd)
+        col = rng.integers(0, nd, density * nd)
+        keep = row != col
+        row, col = row[keep], col[keep]
+        val = rng.normal(0.0, 0.05, row.size)
+        diag = rng.normal(0.0, 0.2, nd)
+        d0j = rng.normal(0.0, 0.1, nd)
+        d00 = float(rng.normal(0.0, 0.5))
+        dips.append((laser.dense_dipole(diag, row, col, val, nd), d00, d0j))
+        raw.append((diag, row, col, val, d00, d0j))
+    return sys_, H, dips, raw
+
+
+PULSES = [{"envelope": "cos2", "envpar": (3.0, 2.5), "carrier": "cos", "freq": 0.6, "t0": 1.0, "phase": 0.3,
+           "pulse_vec": (0.6, 0.0, 0.8), "strength": 0.15},
+          {"envelope": "sin2", "envpar": (0.5, 2.0), "carrier": "sin", "freq": 1.1,
+           "pulse_vec": (0.0, 1.0, 0.0), "strength": 0.05}]
+
+
+@pytest.mark.parametrize("o,v,naux,seed", [(2, 4, 6, 1), (3, 5, 8, 2)])
+def test_laser_operator_vs_oracle(o, v, naux, seed):
+    print(f"[case] laser operator o={o} v={v}", flush=True)
+    sys_, H, dips, raw = _inputs(o, v, naux, seed)
+    nd = H.shape[0]
+    rng = np.random.default_rng(seed + 10)
+    x = rng.normal(size=nd + 1) + 1j * rng.normal(size=nd + 1)
+    with lib.Operator.from_system(sys_) as op:
+        for c, (diag, row, col, val, d00, d0j) in enumerate(raw):
+            op.set_dipole_is(c, diag, row, col, val, d00, d0j)
+        for t in (0.0, 0.7, 2.9, 4.4):
+            E = laser.efield(PULSES, t)
+            got = op.matxvec_treal_laser(t, PULSES, x)
+            want = laser.matxvec_treal_laser(lambda z: H @ z, dips, E, x)
+            assert np.abs(got - want).max() < 1e-12 * np.abs(want).max(), t
+        # a field component whose dipole is not set is an error
+        op.set_dipole_is(1, None, [], [], [], 0.0, None)
+        with pytest.raises(lib.Adc2gError):
+            op.matxvec_treal_laser(2.0, PULSES, x)
+        got = op.matxvec_treal_laser(2.0, PULSES[:1], x)   # x and z only
+        want = laser.matxvec_treal_laser(lambda z: H @ z, dips, laser.efield(PULSES[:1], 2.0), x)
+        assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+
+
+def test_sil_laser_vs_oracle():
+    print("[case] sil laser o=3 v=5", flush=True)
+    sys_, H, dips, raw = _inputs(3, 5, 8, 5)
+    nd = H.shape[0]
+    psi0 = np.zeros(nd + 1, dtype=complex)
+    psi0[nd] = 1.0
+    with lib.Operator.from_system(sys_) as op:
+        for c, (diag, row, col, val, d00, d0j) in enumerate(raw):
+            op.set_dipole_is(c, diag, row, col, val, d00, d0j)
+        psi, norms, st = op.sil_laser(psi0, PULSES, tout=0.5, tfinal=6.0, kdim=16, proptol=1e-9)
+    ref_psi, ref_norms = laser.propagate(lambda z: H @ z, dips, PULSES, psi0, 0.5, 6.0, 16, 1e-9)
+    print(f"{st['nsteps']} steps, {st['nsigma']} operator products, |psi_IS| = {np.linalg.norm(psi[:nd]):.4f}")
+    assert norms.size == ref_norms.size == 13
+    assert np.abs(norms - ref_norms).max() < 1e-10
+    assert np.abs(psi - ref_psi).max() < 1e-8
+    assert np.linalg.norm(psi[:nd]) > 1e-3
+
+
+def test_sil_laser_constant_field_exact():
+    """Constant field (box envelope, zero frequency) from the ground state:
+    exp(-i (H - E.D) t) psi0 of the dense (ndim + 1) matrix, independent of
+    the oracle's propagator."""
+    print("[case] sil laser constant field o=2 v=4", flush=True)
+    sys_, H, dips, raw = _inputs(2, 4, 6, 7)
+    nd = H.shape[0]
+    pulse = [{"envelope": "box", "envpar": (0.0, 100.0), "carrier": "cos", "freq": 0.0,
+              "pulse_vec": (0.48, 0.6, 0.64), "strength": 0.3}]
+    psi0 = np.zeros(nd + 1, dtype=complex)
+    psi0[nd] = 1.0
+    with lib.Operator.from_system(sys_) as op:
+        for c, (diag, row, col, val, d00, d0j) in enumerate(raw):
+            op.set_dipole_is(c, diag, row, col, val, d00, d0j)
+        psi, norms, _ = op.sil_laser(psi0, pulse, tout=0.5, tfinal=3.0, kdim=20, proptol=1e-12)
+    want = scipy.linalg.expm(-1j * 3.0 * laser.full_matrix(H, dips, laser.efield(pulse, 0.0))) @ psi0
+    assert np.abs(psi - want).max() < 1e-9
+    assert np.abs(norms - 1.0).max() < 1e-10
+
+
+def test_sil_laser_benzene_size():
+    """o=16, v=200 (ndim 5.1 million) with a sparse x dipole of 20 million
+    stored elements: the wavefunction leaves the ground state and the norm is
+    conserved through two periods of a cos^2 pulse."""
+    print("[case] sil laser benzene size", flush=True)
+    sys_ = make_system(16, 200, 864, seed=0)
+    rng = np.random.default_rng(0)
+    with lib.Operator.from_system(sys_) as op:
+        nd = op.ndim
+        nnz = 20_000_000
+        row = rng.integers(0, nd, nnz)
+        col = (row + 1 + rng.integers(0, nd - 1, nnz)) % nd
+        op.set_dipole_is(0, rng.normal(0.0, 0.1, nd), row, col, rng.normal(0.0, 1e-3, nnz), 0.3,
+                         rng.normal(0.0, 1e-3, nd))
+        psi0 = np.zeros(nd + 1, dtype=complex)
+        psi0[nd] = 1.0
+        pulse = [{"envelope": "cos2", "envpar": (1.0, 1.0), "carrier": "cos", "freq": 0.5,
+                  "pulse_vec": (1.0, 0.0, 0.0), "strength": 0.05}]
+        psi, norms, st = op.sil_laser(psi0, pulse, tout=1.0, tfinal=2.0, kdim=20, proptol=1e-8)
+    print(f"{st['nsteps']} steps, {st['nsigma']} operator products in {st['time_total']:.2f} s, "
+          f"|psi_IS| = {np.linalg.norm(psi[:nd]):.3e}", flush=True)
+    assert np.abs(norms - 1.0).max() < 1e-8
+    assert np.linalg.norm(psi[:nd]) > 1e-6

@@ -720,3 +720,101 @@ def test_oracle_csf_states_and_adc2x_block(o, v, naux, seed):
     assert np.abs(d[m] - np.diag(M1)[m]).max() < 1e-13 * scale
     for r, c, val in adcx.offdiagonal_blocks(sys_).values():
         assert np.abs(M1[np.ix_(r, c)] - val).max() < 1e-13 * scale
+
+
+def _laser_inputs(o=2, v=4, naux=6, seed=3):
+    """A small ADC(2)-s matrix and random IS dipole operators in the
+    reference's file layout (diagonal + each off-diagonal element once)."""
+    from oracle import laser
+    sys_ = make_system(o, v, naux, seed=seed)
+    df = dfsigma.DF(sys_)
+    nd = df.n1 + df.D2.size
+    H = df.sigma(np.eye(nd))
+    H = 0.5 * (H + H.T)
+    rng = np.random.default_rng(seed)
+    dips, raw = [], []
+    for c in range(3):
+        row = rng.integers(0, nd, 4 * nd)
+        col = rng.integers(0, nd, 4 * nd)
+        keep = row != col
+        row, col = row[keep], col[keep]
+        val = rng.normal(0.0, 0.05, row.size)
+        diag = rng.normal(0.0, 0.2, nd)
+        d0j = rng.normal(0.0, 0.1, nd)
+        d00 = float(rng.normal(0.0, 0.5))
+        dips.append((laser.dense_dipole(diag, row, col, val, nd), d00, d0j))
+        raw.append((diag, row, col, val, d00, d0j))
+    return sys_, H, dips, raw
+
+
+def test_oracle_laser_envelopes():
+    """efield: the cos^2 and sin^2 envelopes have half maximum at +-fwhm/2
+    about their peak (so fwhm is their full width at half maximum) and vanish
+    outside their support; the sin^2 ramp is 1/2 at tau/2 and 1 from tau on;
+    the box is 1 on [ti, tf] only; the carrier peaks (cos) or vanishes (sin)
+    where freq (t - t0) = phase; pulses add along their polarisation."""
+    from oracle import laser
+    c2 = {"envelope": "cos2", "envpar": (50.0, 20.0), "pulse_vec": (1, 0, 0), "strength": 1.0}
+    assert laser.envelope(c2, 50.0) == 1.0
+    assert abs(laser.envelope(c2, 40.0) - 0.5) < 1e-15 and abs(laser.envelope(c2, 60.0) - 0.5) < 1e-15
+    assert laser.envelope(c2, 30.0) == 0.0 and laser.envelope(c2, 70.0) == 0.0
+    s2 = {"envelope": "sin2", "envpar": (10.0, 20.0)}
+    assert abs(laser.envelope(s2, 30.0) - 1.0) < 1e-15
+    assert abs(laser.envelope(s2, 20.0) - 0.5) < 1e-15 and abs(laser.envelope(s2, 40.0) - 0.5) < 1e-15
+    assert laser.envelope(s2, 10.0) == 0.0 and laser.envelope(s2, 50.0) == 0.0
+    r = {"envelope": "sin2ramp", "envpar": (30.0, 0.0)}
+    assert laser.envelope(r, -1.0) == 0.0 and abs(laser.envelope(r, 15.0) - 0.5) < 1e-15
+    assert laser.envelope(r, 30.0) == 1.0 and laser.envelope(r, 100.0) == 1.0
+    b = {"envelope": "box", "envpar": (5.0, 9.0)}
+    assert [laser.envelope(b, t) for t in (4.99, 5.0, 7.0, 9.0, 9.01)] == [0.0, 1.0, 1.0, 1.0, 0.0]
+    w, t0, ph = 0.7, 3.0, 0.4
+    pc = {"envelope": "box", "envpar": (0.0, 100.0), "carrier": "cos", "freq": w, "t0": t0, "phase": ph,
+          "pulse_vec": (0.6, 0.8, 0.0), "strength": 0.05}
+    ps = dict(pc, carrier="sin", pulse_vec=(0.0, 0.0, 1.0))
+    tp = t0 + ph / w
+    assert np.allclose(laser.efield([pc], tp), [0.03, 0.04, 0.0], atol=1e-15)
+    assert np.allclose(laser.efield([ps], tp), 0.0, atol=1e-15)
+    assert np.allclose(laser.efield([pc, ps], 7.3), laser.efield([pc], 7.3) + laser.efield([ps], 7.3), atol=1e-16)
+
+
+def test_oracle_laser_constant_field_exact():
+    """A box pulse of zero frequency is a constant field E: from the ground
+    state the propagation equals exp(-i (H - E.D) t) psi0 on the ndim + 1
+    states (dense exponential), the IS dipole entering with the reference's
+    sign (-mu.E), d00 on both blocks and d0j symmetric; the norm stays 1."""
+    import scipy.linalg
+    from oracle import laser
+    sys_, H, dips, _ = _laser_inputs()
+    nd = H.shape[0]
+    pulse = {"envelope": "box", "envpar": (0.0, 100.0), "carrier": "cos", "freq": 0.0,
+             "pulse_vec": (0.48, 0.6, 0.64), "strength": 0.3}
+    E = laser.efield([pulse], 1.0)
+    psi0 = np.zeros(nd + 1, dtype=complex)
+    psi0[nd] = 1.0
+    hmul = (lambda x: H @ x)  # noqa: E731
+    psi, norms = laser.propagate(hmul, dips, [pulse], psi0, 0.5, 3.0, 20, 1e-12)
+    want = scipy.linalg.expm(-1j * 3.0 * laser.full_matrix(H, dips, E)) @ psi0
+    assert np.abs(psi - want).max() < 1e-9
+    assert np.abs(norms - 1.0).max() < 1e-10
+    # the same generator through matxvec_treal_laser
+    v = np.random.default_rng(1).normal(size=nd + 1) + 1j * np.random.default_rng(2).normal(size=nd + 1)
+    assert np.abs(laser.matxvec_treal_laser(hmul, dips, E, v) + 1j * (laser.full_matrix(H, dips, E) @ v)).max() \
+        < 1e-12 * np.abs(v).max() * np.abs(H).max()
+
+
+def test_oracle_laser_time_dependent_norm():
+    """Under a cos^2 pulse (Hermitian H(t)) the norm is conserved, the
+    wavefunction leaves the ground state, and a period before the pulse
+    (E = 0 on the ground state) is skipped unchanged."""
+    from oracle import laser
+    sys_, H, dips, _ = _laser_inputs(seed=4)
+    nd = H.shape[0]
+    pulse = {"envelope": "cos2", "envpar": (3.0, 2.0), "carrier": "cos", "freq": 0.5,
+             "pulse_vec": (1.0, 0.0, 0.0), "strength": 0.2}
+    psi0 = np.zeros(nd + 1, dtype=complex)
+    psi0[nd] = 1.0
+    psi, norms = laser.propagate(lambda x: H @ x, dips, [pulse], psi0, 0.25, 5.0, 16, 1e-10)
+    assert np.abs(norms - 1.0).max() < 1e-8
+    assert np.linalg.norm(psi[:nd]) > 1e-3
+    psi1, _ = laser.propagate(lambda x: H @ x, dips, [pulse], psi0, 0.25, 1.0, 16, 1e-10)
+    assert np.array_equal(psi1, psi0)
