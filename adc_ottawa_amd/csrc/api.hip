@@ -243,6 +243,22 @@ int32_t adc2g_set_dipole_is(adc2g_handle* h, int32_t component, const double* di
   });
 }
 
+int32_t adc2g_set_cap_is(adc2g_handle* h, const double* diag, int64_t nnz, const int64_t* row, const int64_t* col,
+                         const double* val, double w00, const double* w0j) {
+  return guarded([&] {
+    ARG_CHECK(h, "NULL handle");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    h->h.set_cap_is(diag, nnz, row, col, val, w00, w0j);
+  });
+}
+
+int32_t adc2g_set_cap_projector(adc2g_handle* h, int32_t nvec, const double* vecs, int32_t ground) {
+  return guarded([&] {
+    ARG_CHECK(h, "NULL handle");
+    h->h.set_cap_projector(nvec, vecs, ground != 0);
+  });
+}
+
 int32_t adc2g_matxvec_treal_laser(adc2g_handle* h, double time, int32_t npulse, const adc2g_pulse* pulses,
                                   const double* v1, double* v2) {
   return guarded([&] {

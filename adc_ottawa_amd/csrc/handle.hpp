@@ -125,9 +125,19 @@ struct Handle {
     DevBuf<int32_t> col;
   };
   DipoleIS dip[3];
-  DevBuf<double> laser_part;  // partial sums of the ground-state row
+  // the complex absorbing potential W (same layout; d00 = w00, d0j = w0j)
+  // and the projector of the CAP term: orthonormal IS vectors (ndim x
+  // capproj_n) and whether the ground state is projected out
+  DipoleIS cap;
+  DevBuf<double> capproj;
+  int capproj_n = 0;
+  bool capproj_ground = false;
+  DevBuf<double> laser_part, laser_u, laser_c, laser_coef;  // work arrays of the laser operator
   void set_dipole_is(int c, const double* diag, int64_t nnz, const int64_t* row, const int64_t* col,
                      const double* val, double d00, const double* d0j);
+  void set_cap_is(const double* diag, int64_t nnz, const int64_t* row, const int64_t* col, const double* val,
+                  double w00, const double* w0j);
+  void set_cap_projector(int nvec, const double* vecs, bool ground);
 
   Handle() = default;
   ~Handle();

@@ -117,6 +117,68 @@ __global__ void laser_ground_final_kernel(int nblk, int64_t nd, int64_t n, const
   }
 }
 
+// out = scale [(diag + d00) u + off u + d0j u0] on the IS rows (no -i: the
+// CAP term -W u of the reference)
+__global__ void isop_is_kernel(int64_t nd, int64_t n, const double* __restrict__ u, double* __restrict__ out,
+                               LaserComp c, double scale) {
+  const double u0re = u[nd], u0im = u[n + nd];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nd; i += (int64_t)gridDim.x * blockDim.x) {
+    const double dd = c.diag[i] + c.d00;
+    double re = dd * u[i], im = dd * u[n + i];
+    for (int64_t p = c.rowptr[i]; p < c.rowptr[i + 1]; ++p) {
+      const int64_t j = c.col[p];
+      re += c.val[p] * u[j];
+      im += c.val[p] * u[n + j];
+    }
+    out[i] = scale * (re + c.d0j[i] * u0re);
+    out[n + i] = scale * (im + c.d0j[i] * u0im);
+  }
+}
+
+__global__ void isop_ground_partial_kernel(int64_t nd, int64_t n, const double* __restrict__ u, LaserComp c,
+                                           double* __restrict__ part) {
+  __shared__ double sh[16];
+  double sre = 0.0, sim = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nd; i += (int64_t)gridDim.x * blockDim.x) {
+    sre += c.d0j[i] * u[i];
+    sim += c.d0j[i] * u[n + i];
+  }
+  sre = laser_block_sum(sre, sh);
+  sim = laser_block_sum(sim, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sre;
+    part[2 * blockIdx.x + 1] = sim;
+  }
+}
+
+__global__ void isop_ground_final_kernel(int nblk, int64_t nd, int64_t n, const double* __restrict__ u, LaserComp c,
+                                         double scale, const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double sh[16];
+  double sre = 0.0, sim = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    sre += part[2 * b];
+    sim += part[2 * b + 1];
+  }
+  sre = laser_block_sum(sre, sh);
+  sim = laser_block_sum(sim, sh);
+  if (threadIdx.x == 0) {
+    out[nd] = scale * (sre + c.d00 * u[nd]);
+    out[n + nd] = scale * (sim + c.d00 * u[n + nd]);
+  }
+}
+
+__global__ void zero_ground_kernel(int64_t nd, int64_t n, double* __restrict__ x) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    x[nd] = 0.0;
+    x[n + nd] = 0.0;
+  }
+}
+
+__global__ void add_planar_kernel(int64_t m, const double* __restrict__ x, double* __restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += x[i];
+}
+
 double envelope_value(const adc2g_pulse& p, double t) {
   const double pi = 3.14159265358979323846;
   switch (p.envelope) {
@@ -176,11 +238,14 @@ void laser_efield(int npulse, const adc2g_pulse* pulses, double t, double E[3]) 
   }
 }
 
-void Handle::set_dipole_is(int c, const double* diag_h, int64_t nnz, const int64_t* row, const int64_t* colh,
-                           const double* valh, double d00, const double* d0j_h) {
-  ARG_CHECK(c >= 0 && c <= 2, "dipole component must be 0 (x), 1 (y) or 2 (z)");
+namespace {
+
+// an IS operator in the reference's file layout -> D (CSR with both triangles)
+void fill_is_operator(Handle& Hd, Handle::DipoleIS& D, const double* diag_h, int64_t nnz, const int64_t* row,
+                      const int64_t* colh, const double* valh, double d00, const double* d0j_h) {
   ARG_CHECK(nnz >= 0, "nnz must be >= 0");
-  DipoleIS& D = dip[c];
+  const Dims& d = Hd.d;
+  const hipStream_t st = Hd.st;
   if (nnz == 0 && !diag_h && !d0j_h && d00 == 0.0) {
     D.set = false;
     D.diag.release();
@@ -215,7 +280,7 @@ void Handle::set_dipole_is(int c, const double* diag_h, int64_t nnz, const int64
     cl[p] = (int32_t)row[k];
     vl[p] = valh[k];
   }
-  HIP_CHECK(hipSetDevice(device));
+  HIP_CHECK(hipSetDevice(Hd.device));
   D.diag.alloc(nd);
   D.d0j.alloc(nd);
   D.rowptr.alloc(nd + 1);
@@ -229,6 +294,34 @@ void Handle::set_dipole_is(int c, const double* diag_h, int64_t nnz, const int64
   HIP_CHECK(hipStreamSynchronize(st));
   D.d00 = d00;
   D.set = true;
+}
+
+}  // namespace
+
+void Handle::set_dipole_is(int c, const double* diag_h, int64_t nnz, const int64_t* row, const int64_t* colh,
+                           const double* valh, double d00, const double* d0j_h) {
+  ARG_CHECK(c >= 0 && c <= 2, "dipole component must be 0 (x), 1 (y) or 2 (z)");
+  fill_is_operator(*this, dip[c], diag_h, nnz, row, colh, valh, d00, d0j_h);
+}
+
+void Handle::set_cap_is(const double* diag_h, int64_t nnz, const int64_t* row, const int64_t* colh,
+                        const double* valh, double w00, const double* w0j_h) {
+  fill_is_operator(*this, cap, diag_h, nnz, row, colh, valh, w00, w0j_h);
+}
+
+void Handle::set_cap_projector(int nvec, const double* vecs, bool ground) {
+  ARG_CHECK(nvec >= 0, "nvec must be >= 0");
+  ARG_CHECK(nvec == 0 || vecs, "NULL projector vectors");
+  HIP_CHECK(hipSetDevice(device));
+  if (nvec > 0) {
+    capproj.alloc((size_t)d.ndim * nvec);
+    HIP_CHECK(hipMemcpyAsync(capproj.get(), vecs, sizeof(double) * d.ndim * nvec, hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+  } else {
+    capproj.release();
+  }
+  capproj_n = nvec;
+  capproj_ground = ground;
 }
 
 void laser_apply(Handle& H, const double E[3], const double* v, double* hv, double* out, Prec p, hipStream_t s) {
@@ -259,6 +352,46 @@ void laser_apply(Handle& H, const double E[3], const double* v, double* hv, doub
     HIP_CHECK(hipGetLastError());
   }
   hipLaunchKernelGGL(laser_ground_final_kernel, dim3(1), dim3(256), 0, s, nb, nd, n, v, L, H.laser_part.get(), out);
+  HIP_CHECK(hipGetLastError());
+  if (!H.cap.set) return;
+  // CAP (tdselib.f90:722-830): out += -P (W + w00) P v - P (w0j terms) P v,
+  // P the projector onto the complement of the selected bound states
+  ARG_CHECK(s == H.st, "the CAP term runs on the handle's stream");
+  H.laser_u.alloc(2 * (size_t)n);
+  H.laser_c.alloc(2 * (size_t)n);
+  double* u = H.laser_u.get();
+  double* c = H.laser_c.get();
+  auto project = [&](double* x) {
+    if (H.capproj_n > 0) {
+      H.laser_coef.alloc(2 * (size_t)H.capproj_n);
+      gemm_tn(H, H.capproj_n, 2, nd, H.capproj.get(), nd, x, n, H.laser_coef.get(), H.capproj_n, 1.0, 0.0);
+      gemm_nn(H, nd, 2, H.capproj_n, H.capproj.get(), nd, H.laser_coef.get(), H.capproj_n, x, n, -1.0, 1.0);
+    }
+    if (H.capproj_ground) {
+      hipLaunchKernelGGL(zero_ground_kernel, dim3(1), dim3(64), 0, s, nd, n, x);
+      HIP_CHECK(hipGetLastError());
+    }
+  };
+  HIP_CHECK(hipMemcpyAsync(u, v, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, s));
+  project(u);
+  LaserComp w;
+  w.E = 1.0;
+  w.d00 = H.cap.d00;
+  w.diag = H.cap.diag.get();
+  w.val = H.cap.val.get();
+  w.d0j = H.cap.d0j.get();
+  w.rowptr = H.cap.rowptr.get();
+  w.col = H.cap.col.get();
+  hipLaunchKernelGGL(isop_is_kernel, dim3(grid_for(nd)), dim3(256), 0, s, nd, n, u, c, w, -1.0);
+  HIP_CHECK(hipGetLastError());
+  const int nbc = reduce_blocks(nd);
+  H.laser_part.alloc(2 * (size_t)nbc);
+  hipLaunchKernelGGL(isop_ground_partial_kernel, dim3(nbc), dim3(256), 0, s, nd, n, u, w, H.laser_part.get());
+  HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(isop_ground_final_kernel, dim3(1), dim3(256), 0, s, nbc, nd, n, u, w, -1.0, H.laser_part.get(), c);
+  HIP_CHECK(hipGetLastError());
+  project(c);
+  hipLaunchKernelGGL(add_planar_kernel, dim3(grid_for(2 * n)), dim3(256), 0, s, 2 * n, c, out);
   HIP_CHECK(hipGetLastError());
 }
 

@@ -239,6 +239,139 @@ struct Sil {
     HIP_CHECK(hipGetLastError());
   }
 
+  // exp(A dt) e_0 for the complex (m x m, column-major) matrix A: scaling
+  // and squaring with a Taylor series (terms until below 1e-17 of the sum)
+  static std::vector<cplx> expm_col0(const std::vector<cplx>& A, int m, double dt) {
+    std::vector<cplx> B(A.size());
+    double nrm = 0.0;  // 1-norm
+    for (int q = 0; q < m; ++q) {
+      double c = 0.0;
+      for (int p = 0; p < m; ++p) {
+        B[p + (size_t)q * m] = A[p + (size_t)q * m] * dt;
+        c += std::abs(B[p + (size_t)q * m]);
+      }
+      nrm = std::max(nrm, c);
+    }
+    int sq = 0;
+    while (nrm > 0.5) {
+      nrm *= 0.5;
+      ++sq;
+    }
+    const double sc = std::ldexp(1.0, -sq);
+    for (auto& x : B) x *= sc;
+    auto mul = [m](const std::vector<cplx>& X, const std::vector<cplx>& Y) {
+      std::vector<cplx> Z((size_t)m * m, cplx(0, 0));
+      for (int q = 0; q < m; ++q)
+        for (int k = 0; k < m; ++k) {
+          const cplx y = Y[k + (size_t)q * m];
+          if (y == cplx(0, 0)) continue;
+          for (int p = 0; p < m; ++p) Z[p + (size_t)q * m] += X[p + (size_t)k * m] * y;
+        }
+      return Z;
+    };
+    std::vector<cplx> E((size_t)m * m, cplx(0, 0)), T((size_t)m * m, cplx(0, 0));
+    for (int p = 0; p < m; ++p) E[p + (size_t)p * m] = T[p + (size_t)p * m] = cplx(1, 0);
+    for (int k = 1; k < 40; ++k) {
+      T = mul(T, B);
+      double tn = 0.0, en = 0.0;
+      for (size_t i = 0; i < T.size(); ++i) {
+        T[i] /= (double)k;
+        E[i] += T[i];
+        tn = std::max(tn, std::abs(T[i]));
+        en = std::max(en, std::abs(E[i]));
+      }
+      if (tn <= 1e-17 * en) break;
+    }
+    for (int i = 0; i < sq; ++i) E = mul(E, E);
+    return std::vector<cplx>(E.begin(), E.begin() + m);
+  }
+
+  // One CSILStep (Lanczos-Arnoldi, improved error criterion) on psi =
+  // Kp(0) with dtpsi = -i H psi in Kp(1): the Krylov vectors orthogonalised
+  // against all earlier ones (modified Gram-Schmidt) give the upper Hessenberg
+  // matrix Hs of -i H; after each new order its last column is estimated
+  // from the previous one (sub-diagonal mirrored, diagonal repeated) and the
+  // coefficients c = exp(Hs dt) e_0 compared with those of the previous
+  // order; the order grows until |c - c_old| <= tolerror (at least 2, at most
+  // kdim), else the step shrinks by (tolerror / former error)^(1/order) with
+  // the former error |err * dt * beta_k / k| accumulated over the orders.
+  // psi <- |psi| sum_p c_p K_p.  The exponential is a scaled Taylor series of
+  // Hs (the reference diagonalises Hs; the same exp(Hs dt) e_0).
+  double csil_step(double intperiod, double tolerror, int& order_out) {
+    const double RelativeMinStep = 1.0e-9, Tiny = 1.0e-18;
+    const int MinOrder = 2;
+    const cplx pref(0.0, -1.0);
+    if (kdim < MinOrder) throw Error(ADC2G_ESIL_ORDER, "Illegal integration order");
+    double stepsize = intperiod;
+    const double minstep = std::fabs(RelativeMinStep * intperiod);
+    const double norm = std::sqrt(cdot(Kp(0), Kp(0)).real());
+    scale(Kp(0), cplx(1.0 / norm, 0.0));
+    scale(Kp(1), cplx(1.0 / norm, 0.0));
+    const int ld = kdim + 1;
+    std::vector<cplx> Hs((size_t)ld * ld, cplx(0, 0)), cold(ld, cplx(0, 0)), c1;
+    double former = 1.0, error = 0.0;
+    int order = 0;
+    auto hs = [&](int p, int q) -> cplx& { return Hs[p + (size_t)q * ld]; };
+    auto coeffs = [&](int m, double dt) {
+      std::vector<cplx> A((size_t)m * m);
+      for (int q = 0; q < m; ++q)
+        for (int p = 0; p < m; ++p) A[p + (size_t)q * m] = hs(p, q);
+      return expm_col0(A, m, dt);
+    };
+    while (true) {
+      if (std::fabs(stepsize) < std::fabs(minstep)) throw Error(ADC2G_ESIL_UNDERFLOW, "Stepsize underflow");
+      const int next = order + 1;
+      double* w = Kp(next);
+      for (int p = 0; p <= order; ++p) {
+        const cplx c = cdot(Kp(p), w);
+        hs(p, order) = c;
+        axpy2(w, c, Kp(p), cplx(0, 0), nullptr);
+      }
+      double sub = std::sqrt(cdot(w, w).real());
+      hs(next, order) = pref * sub;
+      cplx cinv(0, 0);
+      if (sub < Tiny) {
+        sub = 0.0;
+      } else {
+        cinv = 1.0 / hs(next, order);
+      }
+      scale(w, cinv);
+      order = next;
+      former = std::fabs(former * stepsize * sub / (double)order);
+      hs(order - 1, order) = hs(order, order - 1);
+      hs(order, order) = hs(order - 1, order - 1);
+      c1 = coeffs(order + 1, stepsize);
+      error = 0.0;
+      if (order > 1) {
+        for (int p = 0; p <= order; ++p) error += std::norm(cold[p] - c1[p]);
+        error = std::sqrt(error);
+      }
+      for (int p = 0; p <= order; ++p) cold[p] = c1[p];
+      if ((error > tolerror && order < kdim) || order < MinOrder) {
+        func(Kp(order), Kp(order + 1));
+        continue;
+      }
+      break;
+    }
+    if (!(error <= tolerror)) {
+      stepsize = std::min(stepsize, stepsize * std::pow(tolerror / former, 1.0 / (double)order));
+      c1 = coeffs(order + 1, stepsize);
+    }
+    std::vector<double> ch(2 * (order + 1));
+    for (int p = 0; p <= order; ++p) {
+      ch[2 * p] = norm * c1[p].real();
+      ch[2 * p + 1] = norm * c1[p].imag();
+    }
+    HIP_CHECK(hipMemcpyAsync(coef.get(), ch.data(), sizeof(double) * ch.size(), hipMemcpyHostToDevice, H.st));
+    hipLaunchKernelGGL(combine_kernel, dim3(grid_for(n)), dim3(256), 0, H.st, n, order + 1, coef.get(), K.get(), 2 * n,
+                       tmp.get());
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(Kp(0), tmp.get(), sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, H.st));
+    HIP_CHECK(hipStreamSynchronize(H.st));
+    order_out = order;
+    return stepsize;
+  }
+
   // One SILStep on psi (dtpsi = -iH psi already in Kp(1)); returns stepsize.
   double step(double intperiod, double tolerror, int& order_out) {
     const double RelativeMinStep = 1.0e-10;
@@ -430,6 +563,8 @@ void sil_laser_run(Handle& H, const adc2g_laser_params& p, int npulse, const adc
   DevBuf<double> hv;
   hv.alloc(2 * (size_t)n);
   double time = 0.0;
+  // with a CAP the operator is not Hermitian: Lanczos-Arnoldi steps
+  const bool csil = H.cap.set;
   S.op = [&](const double* v, double* out) {
     double E[3];
     laser_efield(npulse, pulses, time, E);
@@ -452,7 +587,7 @@ void sil_laser_run(Handle& H, const adc2g_laser_params& p, int npulse, const adc
       if (inttime == 0.0) norms[i - 1] = std::sqrt(S.cdot(S.Kp(0), S.Kp(0)).real());
       if (std::sqrt(S.cdot(S.Kp(1), S.Kp(1)).real()) > tinier) {
         int order = 0;
-        const double truestep = S.step(stepsize, p.proptol, order);
+        const double truestep = csil ? S.csil_step(stepsize, p.proptol, order) : S.step(stepsize, p.proptol, order);
         ++nsteps;
         maxorder = std::max(maxorder, order);
         time += truestep;

@@ -166,7 +166,8 @@ SYMBOLS = [
     "adc2g_block_lanczos", "adc2g_relax", "adc2g_write_hmlt", "adc2g_davidson_vectors",
     "adc2g_sil_autocorr_orders", "adc2g_create_cvs", "adc2g_create_frozen_core", "adc2g_set_adc2x", "adc2g_mp2", "adc2g_fvector", "adc2g_lanczos_guess",
     "adc2g_spectral_bounds", "adc2g_power", "adc2g_coupled_schur", "adc2g_sigma_device_noalloc",
-    "adc2g_set_dipole_is", "adc2g_matxvec_treal_laser", "adc2g_sil_laser",
+    "adc2g_set_dipole_is", "adc2g_matxvec_treal_laser", "adc2g_sil_laser", "adc2g_set_cap_is",
+    "adc2g_set_cap_projector",
 ]
 
 _lib = None
@@ -226,6 +227,8 @@ def load(path: str = LIBPATH):
                                         C.POINTER(LanczosStats)]
     lib.adc2g_set_dipole_is.argtypes = [vp, i32, vp, i64, vp, vp, vp, C.c_double, vp]
     lib.adc2g_matxvec_treal_laser.argtypes = [vp, C.c_double, i32, vp, vp, vp]
+    lib.adc2g_set_cap_is.argtypes = [vp, vp, i64, vp, vp, vp, C.c_double, vp]
+    lib.adc2g_set_cap_projector.argtypes = [vp, i32, vp, i32]
     lib.adc2g_sil_laser.argtypes = [vp, C.POINTER(LaserParams), i32, vp, vp, vp, i64, C.POINTER(i64),
                                     C.POINTER(SilStats)]
     for s in SYMBOLS:
@@ -520,6 +523,35 @@ class Operator:
             raise ValueError("diag and d0j must have ndim elements")
         check(_lib.adc2g_set_dipole_is(self._h, int(component), _ptr(diag), row.size, _ptr(row), _ptr(col),
                                        _ptr(val), float(d00), _ptr(d0j)))
+
+    def set_cap_is(self, diag, row, col, val, w00, w0j):
+        """The CAP's intermediate-state representation (as set_dipole_is);
+        diag = w0j = None with no elements and w00 = 0 removes the CAP."""
+        row = np.ascontiguousarray(row, dtype=np.int64)
+        col = np.ascontiguousarray(col, dtype=np.int64)
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        if not (row.size == col.size == val.size):
+            raise ValueError("row, col and val must have the same length")
+        if diag is None and w0j is None:
+            if row.size or w00 != 0.0:
+                raise ValueError("diag and w0j are required unless the CAP is removed")
+            check(_lib.adc2g_set_cap_is(self._h, None, 0, None, None, None, 0.0, None))
+            return
+        diag = np.ascontiguousarray(diag, dtype=np.float64)
+        w0j = np.ascontiguousarray(w0j, dtype=np.float64)
+        if diag.shape != (self.ndim,) or w0j.shape != (self.ndim,):
+            raise ValueError("diag and w0j must have ndim elements")
+        check(_lib.adc2g_set_cap_is(self._h, _ptr(diag), row.size, _ptr(row), _ptr(col), _ptr(val), float(w00),
+                                    _ptr(w0j)))
+
+    def set_cap_projector(self, vecs=None, ground=False):
+        """Project the CAP term onto the complement of the orthonormal IS
+        vectors vecs (ndim x nvec) and, with ground, of the ground state."""
+        if vecs is None or np.size(vecs) == 0:
+            check(_lib.adc2g_set_cap_projector(self._h, 0, None, 1 if ground else 0))
+            return
+        V = np.asfortranarray(np.asarray(vecs, dtype=np.float64).reshape(self.ndim, -1))
+        check(_lib.adc2g_set_cap_projector(self._h, V.shape[1], _ptr(V), 1 if ground else 0))
 
     def matxvec_treal_laser(self, t, pulses, v1):
         """-i H(t) v1 for complex v1 of ndim + 1 components (ground state last)."""

@@ -375,18 +375,37 @@ int32_t adc2g_set_dipole_is(adc2g_handle* h, int32_t component, const double* di
                             const int64_t* row, const int64_t* col, const double* val, double d00,
                             const double* d0j);
 
+/* The complex absorbing potential: its intermediate-state representation W
+ * (SCRATCH/cap, the same layout as the dipoles), ground-state element w00 and
+ * column w0j.  With a CAP set, adc2g_matxvec_treal_laser adds -W v, the
+ * -i (-i W) v of tdselib.f90:722-830, and adc2g_sil_laser integrates with the
+ * short-iterative Lanczos-Arnoldi method (propagate_wavepacket_csil,
+ * proplib_adc2.f90:659-830; CSILStep with the improved error criterion,
+ * csillib.f90:120-520).  Clearing as for adc2g_set_dipole_is. */
+int32_t adc2g_set_cap_is(adc2g_handle* h, const double* diag, int64_t nnz, const int64_t* row,
+                         const int64_t* col, const double* val, double w00, const double* w0j);
+
+/* The projector of the CAP term (iprojcap, tdselib.f90:735-830): the CAP
+ * acts as P W P with P the projector onto the complement of nvec orthonormal
+ * intermediate-state vectors (vecs: ndim x nvec, column-major: the selected
+ * bound states the reference reads from its eigenvector file) and, with
+ * ground = 1, of the ground state.  nvec = 0 and ground = 0 remove it. */
+int32_t adc2g_set_cap_projector(adc2g_handle* h, int32_t nvec, const double* vecs, int32_t ground);
+
 /* v2 = -i H(t) v1 (ndim + 1 complex components, interleaved): the -i H of
  * matxvec_treal on the intermediate states plus, for E(t) = sum of the pulses,
  * + i sum_c E_c [(D_c + d00_c) v1 + d0j_c v1_0] on the intermediate states and
  * + i sum_c E_c (d00_c v1_0 + d0j_c . v1) on the ground state
- * (matxvec_treal_laser, tdselib.f90:569-720, without a CAP). */
+ * (matxvec_treal_laser, tdselib.f90:569-720), and the CAP term when a CAP is
+ * set. */
 int32_t adc2g_matxvec_treal_laser(adc2g_handle* h, double time, int32_t npulse, const adc2g_pulse* pulses,
                                   const double* v1, double* v2);
 
 /* SIL propagation under the pulses (propagate_wavepacket_sil,
- * proplib_adc2.f90:486-630): psi (ndim + 1 complex, interleaved) in and
- * out; the periods i = 1 .. int(tfinal / tout) each integrated by SIL steps
- * with H at the step's start time (SILStep's Func(Time, ...)), a period
+ * proplib_adc2.f90:486-630; with a CAP the Lanczos-Arnoldi steps of
+ * propagate_wavepacket_csil, :659-830): psi (ndim + 1 complex, interleaved)
+ * in and out; the periods i = 1 .. int(tfinal / tout) each integrated by
+ * steps with H at the step's start time (Func(Time, ...)), a period
  * skipped when |dpsi/dt| <= 1e-10; norms[i] = |psi| at i * tout for
  * i = 0 .. nrec (the wrstepinfo values).  norms holds nrec_cap values. */
 int32_t adc2g_sil_laser(adc2g_handle* h, const adc2g_laser_params* p, int32_t npulse,

@@ -133,3 +133,82 @@ def test_sil_laser_benzene_size():
           f"|psi_IS| = {np.linalg.norm(psi[:nd]):.3e}", flush=True)
     assert np.abs(norms - 1.0).max() < 1e-8
     assert np.linalg.norm(psi[:nd]) > 1e-6
+
+
+def _cap(nd, seed):
+    """PSD CAP on the ndim + 1 states in the reference's layout, and a
+    projector onto the complement of two orthonormal IS vectors."""
+    rng = np.random.default_rng(seed)
+    B = rng.normal(0.0, 0.08, (nd + 1, 4))
+    Wf = B @ B.T
+    w00 = Wf[nd, nd]
+    W = Wf[:nd, :nd] - w00 * np.eye(nd)
+    iu = np.triu_indices(nd, 1)
+    R, _ = np.linalg.qr(rng.normal(size=(nd, 2)))
+    return (W, w00, Wf[:nd, nd].copy()), (np.diag(W).copy(), iu[0], iu[1], W[iu]), R
+
+
+def _setup(op, raw, capraw, cap, R, ground):
+    for c, (diag, row, col, val, d00, d0j) in enumerate(raw):
+        op.set_dipole_is(c, diag, row, col, val, d00, d0j)
+    diag, row, col, val = capraw
+    op.set_cap_is(diag, row, col, val, cap[1], cap[2])
+    op.set_cap_projector(R, ground)
+
+
+@pytest.mark.parametrize("ground", [False, True])
+def test_cap_operator_vs_oracle(ground):
+    print(f"[case] cap operator ground={ground}", flush=True)
+    sys_, H, dips, raw = _inputs(3, 5, 8, 8)
+    nd = H.shape[0]
+    cap, capraw, R = _cap(nd, 8)
+    rng = np.random.default_rng(9)
+    x = rng.normal(size=nd + 1) + 1j * rng.normal(size=nd + 1)
+    with lib.Operator.from_system(sys_) as op:
+        _setup(op, raw, capraw, cap, R, ground)
+        for t in (0.0, 2.2):
+            got = op.matxvec_treal_laser(t, PULSES, x)
+            want = laser.matxvec_treal_laser(lambda z: H @ z, dips, laser.efield(PULSES, t), x, cap, (R, ground))
+            assert np.abs(got - want).max() < 1e-12 * np.abs(want).max(), t
+        op.set_cap_projector(None, False)
+        got = op.matxvec_treal_laser(2.2, PULSES, x)
+        want = laser.matxvec_treal_laser(lambda z: H @ z, dips, laser.efield(PULSES, 2.2), x, cap, (None, False))
+        assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+
+
+def test_csil_laser_vs_oracle():
+    print("[case] csil laser with CAP o=3 v=5", flush=True)
+    sys_, H, dips, raw = _inputs(3, 5, 8, 5)
+    nd = H.shape[0]
+    cap, capraw, R = _cap(nd, 5)
+    psi0 = np.zeros(nd + 1, dtype=complex)
+    psi0[nd] = 1.0
+    with lib.Operator.from_system(sys_) as op:
+        _setup(op, raw, capraw, cap, R, False)
+        psi, norms, st = op.sil_laser(psi0, PULSES, tout=0.5, tfinal=6.0, kdim=16, proptol=1e-9)
+    ref_psi, ref_norms = laser.propagate(lambda z: H @ z, dips, PULSES, psi0, 0.5, 6.0, 16, 1e-9, cap, (R, False))
+    print(f"{st['nsteps']} steps, {st['nsigma']} operator products, norm {norms[-1]:.6f}")
+    assert np.abs(norms - ref_norms).max() < 1e-10
+    assert np.abs(psi - ref_psi).max() < 1e-8
+    assert norms[-1] < norms[0]
+
+
+def test_csil_constant_field_exact():
+    """Constant field with a CAP: exp(-i (H - E.D - i P W P) t) psi0 of the
+    dense (ndim + 1) matrix, independent of the oracle's propagator."""
+    print("[case] csil constant field o=2 v=4", flush=True)
+    sys_, H, dips, raw = _inputs(2, 4, 6, 11)
+    nd = H.shape[0]
+    cap, capraw, R = _cap(nd, 11)
+    pulse = [{"envelope": "box", "envpar": (0.0, 100.0), "carrier": "cos", "freq": 0.0,
+              "pulse_vec": (0.0, 0.6, 0.8), "strength": 0.2}]
+    rng = np.random.default_rng(12)
+    psi0 = rng.normal(size=nd + 1) + 1j * rng.normal(size=nd + 1)
+    psi0 /= np.linalg.norm(psi0)
+    with lib.Operator.from_system(sys_) as op:
+        _setup(op, raw, capraw, cap, R, True)
+        psi, norms, _ = op.sil_laser(psi0, pulse, tout=0.5, tfinal=3.0, kdim=24, proptol=1e-12)
+    G = laser.full_matrix(H, dips, laser.efield(pulse, 0.0)) - 1j * laser.cap_matrix(cap, (R, True), nd)
+    want = scipy.linalg.expm(-1j * 3.0 * G) @ psi0
+    assert np.abs(psi - want).max() < 1e-9
+    assert np.all(np.diff(norms) <= 1e-12)

@@ -818,3 +818,45 @@ def test_oracle_laser_time_dependent_norm():
     assert np.linalg.norm(psi[:nd]) > 1e-3
     psi1, _ = laser.propagate(lambda x: H @ x, dips, [pulse], psi0, 0.25, 1.0, 16, 1e-10)
     assert np.array_equal(psi1, psi0)
+
+
+def _cap_inputs(nd, seed):
+    """A positive semi-definite CAP on the ndim + 1 states (W_full = B B^T)
+    split into the reference's W (IS block without w00), w00 and w0j, and a
+    projector onto the complement of two orthonormal IS vectors."""
+    rng = np.random.default_rng(seed)
+    B = rng.normal(0.0, 0.08, (nd + 1, 4))
+    Wf = B @ B.T
+    w00 = Wf[nd, nd]
+    cap = (Wf[:nd, :nd] - w00 * np.eye(nd), w00, Wf[:nd, nd].copy())
+    R, _ = np.linalg.qr(rng.normal(size=(nd, 2)))
+    return cap, Wf, R
+
+
+@pytest.mark.parametrize("ground", [False, True])
+def test_oracle_csil_cap_constant_field_exact(ground):
+    """With a CAP the generator is H - E.D - i P W P (non-Hermitian): the
+    Lanczos-Arnoldi propagation under a constant field equals the dense
+    exp(-i (H - E.D - i P W P) t) psi0, and the norm never grows (P W P is
+    positive semi-definite)."""
+    import scipy.linalg
+    from oracle import laser
+    sys_, H, dips, _ = _laser_inputs(seed=6)
+    nd = H.shape[0]
+    cap, Wf, R = _cap_inputs(nd, 6)
+    proj = (R, ground)
+    pulse = {"envelope": "box", "envpar": (0.0, 100.0), "carrier": "cos", "freq": 0.0,
+             "pulse_vec": (0.0, 0.6, 0.8), "strength": 0.2}
+    E = laser.efield([pulse], 1.0)
+    rng = np.random.default_rng(3)
+    psi0 = rng.normal(size=nd + 1) + 1j * rng.normal(size=nd + 1)
+    psi0 /= np.linalg.norm(psi0)
+    psi, norms = laser.propagate(lambda x: H @ x, dips, [pulse], psi0, 0.5, 3.0, 24, 1e-12, cap, proj)
+    G = laser.full_matrix(H, dips, E) - 1j * laser.cap_matrix(cap, proj, nd)
+    want = scipy.linalg.expm(-1j * 3.0 * G) @ psi0
+    assert np.abs(psi - want).max() < 1e-9
+    assert np.all(np.diff(norms) <= 1e-12) and norms[-1] < norms[0] - 1e-4
+    # the generator through matxvec_treal_laser
+    v = rng.normal(size=nd + 1) + 1j * rng.normal(size=nd + 1)
+    got = laser.matxvec_treal_laser(lambda x: H @ x, dips, E, v, cap, proj)
+    assert np.abs(got + 1j * (G @ v)).max() < 1e-12 * np.abs(G).max() * np.abs(v).max() * 10
