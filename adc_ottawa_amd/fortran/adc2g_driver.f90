@@ -42,6 +42,17 @@
 !            (adc2_ener.f90:138-146) and is the initial state instead of the
 !            vector in fvec_file (contract_dipole_initial_state with
 !            statenumber 0, adc2_autospec.f90:131-199).
+!   laser = .true.: laser-driven propagation from the ground state
+!            (propagate_laser_adc2, proplib_adc2.f90:37-830) over tout /
+!            tfinal with krydim and proptol: the IS dipole operators from
+!            the reference's files dipis_file(1:3) (SCRATCH/dipole_x/y/z; ''
+!            leaves a component unset) with d0j_file(1:3) (ndim float64) and
+!            d00(1:3); an optional CAP from cap_file with w0j_file and w00
+!            (cap_ground = .true. projects the ground state out of it); npulse
+!            pulses from carrier(:), envelope(:), pulse_vec(3,:), strength(:),
+!            freq(:), t0(:), phase(:), envpar(2,:).  Prints Time / Norm at
+!            every tout (wrstepinfo) and writes the final wavefunction
+!            (ndim + 1 complex, ground state last) to psi_file.
 !=======================================================================
 program adc2g_driver
 
@@ -60,11 +71,20 @@ program adc2g_driver
   character(len=256) :: eps_file = 'eps.bin', b_file = 'B.bin', davname = 'davstates'
   character(len=256) :: fvec_file = 'fvec.bin', auto_file = 'auto', lancname = 'lancstates'
   character(len=256) :: dpl_file = ''
+  ! laser / CAP propagation
+  integer, parameter :: maxpulse = 8
+  logical :: laser = .false., cap_ground = .false.
+  integer :: npulse = 0, carrier(maxpulse) = 1, envelope(maxpulse) = 1
+  real(c_double) :: pulse_vec(3, maxpulse) = 0.0d0, strength(maxpulse) = 0.0d0, freq(maxpulse) = 0.0d0
+  real(c_double) :: t0(maxpulse) = 0.0d0, phase(maxpulse) = 0.0d0, envpar(2, maxpulse) = 0.0d0
+  real(c_double) :: d00(3) = 0.0d0, w00 = 0.0d0, proptol = 1.0d-8
+  character(len=256) :: dipis_file(3) = '', d0j_file(3) = '', cap_file = '', w0j_file = '', psi_file = 'psi.bin'
   namelist /adc2g/ nocc, nvirt, naux, eps_file, b_file, nstates, blocksize, maxvec, maxiter, tol, &
        precon, precision, deflate, davname, sil, fvec_file, auto_file, tout, tfinal, krydim, autotol, &
        solver, guess, subdim, stepsize, rlx_krydim, autoord, lanczos, lmain, ncycles, lancname, &
        chebyord, cheby_emin, cheby_emax, hmlt, dpl_file, lancguess, orthotype, integrator, siltol, &
-       ncore, core, ehf, nfrozen, adc2x
+       ncore, core, ehf, nfrozen, adc2x, laser, cap_ground, npulse, carrier, envelope, pulse_vec, &
+       strength, freq, t0, phase, envpar, d00, w00, proptol, dipis_file, d0j_file, cap_file, w0j_file, psi_file
 
   type(c_ptr) :: h, hfull
   type(adc2g_opts) :: opts
@@ -263,10 +283,112 @@ program adc2g_driver
      write(*, '(a,i10,a,i10)') ' SIL: steps', ss%nsteps, ', complex sigma-vectors', ss%nsigma
   end if
 
+  if (laser) call run_laser()
+
   rc = adc2g_destroy(h)
   if (c_associated(hfull)) rc = adc2g_destroy(hfull)
 
 contains
+
+  ! propagate_laser_adc2 from the ground state (statenumber = 0)
+  subroutine run_laser()
+    type(adc2g_laser_params) :: lpar
+    type(adc2g_pulse) :: pulses(maxpulse)
+    real(c_double), allocatable :: odiag(:), oval(:), vec0(:), norms(:)
+    integer(c_int64_t), allocatable :: orow(:), ocol(:)
+    complex(c_double_complex), allocatable :: psi(:)
+    real(c_double) :: dummy(1)
+    integer(c_int64_t) :: onnz
+    integer :: c, ip
+    if (npulse < 0 .or. npulse > maxpulse) then
+       write(*, '(a,i0)') ' adc2g input error: npulse must be between 0 and ', maxpulse
+       stop 1
+    end if
+    allocate(vec0(ndim))
+    do c = 1, 3
+       if (len_trim(dipis_file(c)) == 0) cycle
+       call read_is_operator(trim(dipis_file(c)), ndim, odiag, orow, ocol, oval, onnz)
+       open(newunit=u, file=trim(d0j_file(c)), access='stream', form='unformatted', status='old')
+       read(u) vec0
+       close(u)
+       call check(adc2g_set_dipole_is(h, int(c - 1, c_int32_t), odiag, onnz, orow, ocol, oval, d00(c), vec0))
+       write(*, '(a,i2,a,i12)') ' laser: dipole component', c, ', off-diagonal elements', onnz
+    end do
+    if (len_trim(cap_file) > 0) then
+       call read_is_operator(trim(cap_file), ndim, odiag, orow, ocol, oval, onnz)
+       open(newunit=u, file=trim(w0j_file), access='stream', form='unformatted', status='old')
+       read(u) vec0
+       close(u)
+       call check(adc2g_set_cap_is(h, odiag, onnz, orow, ocol, oval, w00, vec0))
+       if (cap_ground) call check(adc2g_set_cap_projector(h, 0_c_int32_t, dummy, 1_c_int32_t))
+       write(*, '(a,i12)') ' laser: CAP off-diagonal elements', onnz
+    end if
+    do ip = 1, npulse
+       pulses(ip)%carrier = int(carrier(ip), c_int32_t)
+       pulses(ip)%envelope = int(envelope(ip), c_int32_t)
+       pulses(ip)%pulse_vec = pulse_vec(:, ip)
+       pulses(ip)%strength = strength(ip)
+       pulses(ip)%freq = freq(ip)
+       pulses(ip)%t0 = t0(ip)
+       pulses(ip)%phase = phase(ip)
+       pulses(ip)%envpar = envpar(:, ip)
+    end do
+    lpar%tout = tout
+    lpar%tfinal = tfinal
+    lpar%kdim = int(krydim, c_int32_t)
+    lpar%precision = int(precision, c_int32_t)
+    lpar%proptol = proptol
+    allocate(psi(ndim + 1), norms(int(tfinal / tout) + 1))
+    psi = (0.0d0, 0.0d0)
+    psi(ndim + 1) = (1.0d0, 0.0d0)
+    call check(adc2g_sil_laser(h, lpar, int(npulse, c_int32_t), pulses, psi, norms, &
+         int(size(norms), c_int64_t), nout, ss))
+    do i = 1, int(nout)
+       write(*, '(a,f14.6,a,f12.9)') ' Time:', (i - 1) * tout, '   Norm:', norms(i)
+    end do
+    write(*, '(a,i10,a,i10)') ' laser: steps', ss%nsteps, ', operator products', ss%nsigma
+    open(newunit=u, file=trim(psi_file), access='stream', form='unformatted', status='replace')
+    write(u) psi
+    close(u)
+  end subroutine run_laser
+
+  ! An IS operator file as the reference writes SCRATCH/dipole_<c> and
+  ! SCRATCH/cap and opxvec_ext reads them (tdselib.f90:1031-1153): buf_size,
+  ! the diagonal, then records (oij, indxi, indxj, nlim), 1-based indices.
+  subroutine read_is_operator(path, nd, diag, row, col, val, nnz)
+    character(len=*), intent(in) :: path
+    integer(c_int64_t), intent(in) :: nd
+    real(c_double), allocatable, intent(out) :: diag(:), val(:)
+    integer(c_int64_t), allocatable, intent(out) :: row(:), col(:)
+    integer(c_int64_t), intent(out) :: nnz
+    real(c_double), allocatable :: oij(:), tv(:)
+    integer, allocatable :: ii(:), jj(:)
+    integer(c_int64_t), allocatable :: tr(:), tc(:)
+    integer :: uu, bs, nlim, ios
+    open(newunit=uu, file=path, form='unformatted', access='sequential', status='old')
+    read(uu) bs
+    allocate(diag(nd), oij(bs), ii(bs), jj(bs), row(bs), col(bs), val(bs))
+    read(uu) diag
+    nnz = 0
+    do
+       read(uu, iostat=ios) oij, ii, jj, nlim
+       if (ios /= 0) exit
+       if (nnz + nlim > size(row)) then
+          allocate(tr(2 * (nnz + nlim)), tc(2 * (nnz + nlim)), tv(2 * (nnz + nlim)))
+          tr(1:nnz) = row(1:nnz)
+          tc(1:nnz) = col(1:nnz)
+          tv(1:nnz) = val(1:nnz)
+          call move_alloc(tr, row)
+          call move_alloc(tc, col)
+          call move_alloc(tv, val)
+       end if
+       row(nnz + 1:nnz + nlim) = ii(1:nlim) - 1
+       col(nnz + 1:nnz + nlim) = jj(1:nlim) - 1
+       val(nnz + 1:nnz + nlim) = oij(1:nlim)
+       nnz = nnz + nlim
+    end do
+    close(uu)
+  end subroutine read_is_operator
 
   ! (lo, hi), 1-based, of pair p (0-based) of the ordered pairs lo < hi <= n
   subroutine pair_of(p, n, lo, hi)

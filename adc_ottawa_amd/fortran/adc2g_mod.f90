@@ -75,6 +75,25 @@ module adc2g_mod
      real(c_double)     :: time_total
   end type adc2g_sil_stats
 
+  ! laser pulse of efield (source/propagation/tdselib.f90:850-1029):
+  ! carrier = ipulse, envelope = ienvelope, envpar as the reference's
+  type, bind(c) :: adc2g_pulse
+     integer(c_int32_t) :: carrier = 1, envelope = 1
+     real(c_double)     :: pulse_vec(3) = 0.0d0
+     real(c_double)     :: strength = 0.0d0, freq = 0.0d0, t0 = 0.0d0, phase = 0.0d0
+     real(c_double)     :: envpar(2) = 0.0d0
+  end type adc2g_pulse
+
+  ! laser propagation (propagate_laser_adc2, proplib_adc2.f90:37-830)
+  type, bind(c) :: adc2g_laser_params
+     real(c_double)     :: tout, tfinal
+     integer(c_int32_t) :: kdim = 20
+     integer(c_int32_t) :: precision = 0
+     real(c_double)     :: proptol = 1.0d-8
+     integer(c_int32_t) :: verbose = 0
+     integer(c_int32_t) :: reserved = 0
+  end type adc2g_laser_params
+
   ! block Lanczos (lancdiag_block, source/eigen/block_lanczos.f90:28-81):
   ! blocksize = lmain, ncycles = iter of the lanczos section
   type, bind(c) :: adc2g_lanczos_params
@@ -274,6 +293,65 @@ module adc2g_mod
        type(c_ptr), value :: h
        integer(c_int32_t), value :: enable
      end function adc2g_set_adc2x
+
+     ! laser / CAP propagation: IS operators in the layout of SCRATCH/dipole_<c>
+     ! and SCRATCH/cap (0-based row/col, each off-diagonal element once);
+     ! wavefunctions of ndim+1 complex components, ground state last
+     integer(c_int32_t) function adc2g_set_dipole_is(h, component, diag, nnz, row, col, val, d00, d0j) &
+          bind(c, name='adc2g_set_dipole_is')
+       import :: c_int32_t, c_int64_t, c_double, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: component
+       real(c_double), intent(in) :: diag(*)
+       integer(c_int64_t), value :: nnz
+       integer(c_int64_t), intent(in) :: row(*), col(*)
+       real(c_double), intent(in) :: val(*)
+       real(c_double), value :: d00
+       real(c_double), intent(in) :: d0j(*)
+     end function adc2g_set_dipole_is
+     integer(c_int32_t) function adc2g_set_cap_is(h, diag, nnz, row, col, val, w00, w0j) &
+          bind(c, name='adc2g_set_cap_is')
+       import :: c_int32_t, c_int64_t, c_double, c_ptr
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: diag(*)
+       integer(c_int64_t), value :: nnz
+       integer(c_int64_t), intent(in) :: row(*), col(*)
+       real(c_double), intent(in) :: val(*)
+       real(c_double), value :: w00
+       real(c_double), intent(in) :: w0j(*)
+     end function adc2g_set_cap_is
+     integer(c_int32_t) function adc2g_set_cap_projector(h, nvec, vecs, ground) &
+          bind(c, name='adc2g_set_cap_projector')
+       import :: c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: nvec
+       real(c_double), intent(in) :: vecs(*)
+       integer(c_int32_t), value :: ground
+     end function adc2g_set_cap_projector
+     integer(c_int32_t) function adc2g_matxvec_treal_laser(h, time, npulse, pulses, v1, v2) &
+          bind(c, name='adc2g_matxvec_treal_laser')
+       import :: c_int32_t, c_double, c_double_complex, c_ptr, adc2g_pulse
+       type(c_ptr), value :: h
+       real(c_double), value :: time
+       integer(c_int32_t), value :: npulse
+       type(adc2g_pulse), intent(in) :: pulses(*)
+       complex(c_double_complex), intent(in) :: v1(*)
+       complex(c_double_complex), intent(out) :: v2(*)
+     end function adc2g_matxvec_treal_laser
+     integer(c_int32_t) function adc2g_sil_laser(h, p, npulse, pulses, psi, norms, nrec_cap, nrec_out, stats) &
+          bind(c, name='adc2g_sil_laser')
+       import :: c_int32_t, c_int64_t, c_double, c_double_complex, c_ptr, adc2g_pulse, adc2g_laser_params, &
+            adc2g_sil_stats
+       type(c_ptr), value :: h
+       type(adc2g_laser_params), intent(in) :: p
+       integer(c_int32_t), value :: npulse
+       type(adc2g_pulse), intent(in) :: pulses(*)
+       complex(c_double_complex), intent(inout) :: psi(*)
+       real(c_double), intent(out) :: norms(*)
+       integer(c_int64_t), value :: nrec_cap
+       integer(c_int64_t), intent(out) :: nrec_out
+       type(adc2g_sil_stats), intent(out) :: stats
+     end function adc2g_sil_laser
 
      ! MP2 energy, D2 diagnostic and density (rho: c_loc of nbas x nbas or c_null_ptr)
      integer(c_int32_t) function adc2g_mp2(h, e_mp2, d2, rho) bind(c, name='adc2g_mp2')

@@ -354,3 +354,83 @@ def test_fortran_driver_adc2x_frozen_core(tmp_path):
     recs = read_davstates(tmp_path / "davstates", idx.size)
     ev = np.array([x[1] for x in recs])
     assert np.abs(ev - w[:3]).max() < 1e-10
+
+
+def _write_is_operator(path, diag, row, col, val, bs=16):
+    """The reference's SCRATCH/dipole_<c> / SCRATCH/cap layout (Fortran
+    sequential records: buf_size, the diagonal, then (oij, indxi, indxj, nlim)
+    with 1-based indices), the last record partly filled."""
+    def rec(payload):
+        n = struct.pack("<i", len(payload))
+        return n + payload + n
+    out = rec(struct.pack("<i", bs)) + rec(np.asarray(diag, "<f8").tobytes())
+    for k0 in range(0, len(row), bs):
+        k1 = min(len(row), k0 + bs)
+        o = np.zeros(bs)
+        i = np.zeros(bs, "<i4")
+        j = np.zeros(bs, "<i4")
+        o[: k1 - k0], i[: k1 - k0], j[: k1 - k0] = val[k0:k1], row[k0:k1] + 1, col[k0:k1] + 1
+        out += rec(o.astype("<f8").tobytes() + i.tobytes() + j.tobytes() + struct.pack("<i", k1 - k0))
+    with open(path, "wb") as f:
+        f.write(out)
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built")
+def test_fortran_driver_laser_cap(tmp_path):
+    """laser = .true.: the driver reads IS dipole and CAP operators in the
+    reference's file layout, propagates the ground state under two pulses
+    with the CAP (Lanczos-Arnoldi steps, ground state projected out of the
+    CAP) and prints the norms; norms and final wavefunction equal the
+    oracle's propagation (oracle/laser.py) of the same operators."""
+    from oracle import laser
+    print("[case] fortran driver laser + CAP o=3 v=5 naux=8", flush=True)
+    o, v, naux = 3, 5, 8
+    sys_ = make_system(o, v, naux, seed=21)
+    sys_.eps.astype("<f8").tofile(tmp_path / "eps.bin")
+    sys_.B.astype("<f8").tofile(tmp_path / "B.bin")
+    df = dfsigma.DF(sys_)
+    nd = df.n1 + df.D2.size
+    H = df.sigma(np.eye(nd))
+    H = 0.5 * (H + H.T)
+    rng = np.random.default_rng(21)
+    dips = [None, None, None]
+    d00 = [0.3, 0.0, -0.2]
+    for c in (0, 2):
+        row = rng.integers(0, nd, 3 * nd)
+        col = (row + 1 + rng.integers(0, nd - 1, row.size)) % nd
+        val = rng.normal(0.0, 0.05, row.size)
+        diag = rng.normal(0.0, 0.2, nd)
+        d0j = rng.normal(0.0, 0.1, nd)
+        _write_is_operator(tmp_path / f"dip{c}", diag, row, col, val)
+        d0j.astype("<f8").tofile(tmp_path / f"d0j{c}.bin")
+        dips[c] = (laser.dense_dipole(diag, row, col, val, nd), d00[c], d0j)
+    B = rng.normal(0.0, 0.08, (nd + 1, 3))
+    Wf = B @ B.T
+    w00 = float(Wf[nd, nd])
+    W = Wf[:nd, :nd] - w00 * np.eye(nd)
+    iu = np.triu_indices(nd, 1)
+    _write_is_operator(tmp_path / "cap", np.diag(W), iu[0], iu[1], W[iu], bs=1000)
+    Wf[:nd, nd].astype("<f8").tofile(tmp_path / "w0j.bin")
+    pulses = [{"envelope": "cos2", "envpar": (2.0, 2.0), "carrier": "cos", "freq": 0.7, "t0": 0.5, "phase": 0.2,
+               "pulse_vec": (0.8, 0.0, 0.6), "strength": 0.2},
+              {"envelope": "box", "envpar": (1.0, 3.0), "carrier": "sin", "freq": 1.3,
+               "pulse_vec": (0.0, 0.0, 1.0), "strength": 0.05}]
+    nml = (f"&adc2g nocc={o} nvirt={v} naux={naux} eps_file='eps.bin' b_file='B.bin' nstates=2 blocksize=2 "
+           f"maxvec=20 tol=1d-6 laser=.true. tout=0.5d0 tfinal=4d0 krydim=16 proptol=1d-9 "
+           f"dipis_file='dip0','','dip2' d0j_file='d0j0.bin','','d0j2.bin' d00=0.3d0,0d0,-0.2d0 "
+           f"cap_file='cap' w0j_file='w0j.bin' w00={w00!r} cap_ground=.true. npulse=2 "
+           f"carrier=1,2 envelope=1,3 pulse_vec=0.8d0,0d0,0.6d0,0d0,0d0,1d0 strength=0.2d0,0.05d0 "
+           f"freq=0.7d0,1.3d0 t0=0.5d0,0d0 phase=0.2d0,0d0 envpar=2d0,2d0,1d0,3d0 psi_file='psi.bin' /\n")
+    r = subprocess.run([DRIVER], input=nml, capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    print(r.stdout[-1500:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    norms = np.array([float(x.split()[3]) for x in r.stdout.splitlines() if x.strip().startswith("Time:")])
+    psi = np.fromfile(tmp_path / "psi.bin", dtype="<c16")
+    psi0 = np.zeros(nd + 1, dtype=complex)
+    psi0[nd] = 1.0
+    ref_psi, ref_norms = laser.propagate(lambda z: H @ z, dips, pulses, psi0, 0.5, 4.0, 16, 1e-9,
+                                         (W, w00, Wf[:nd, nd]), (None, True))
+    assert norms.size == ref_norms.size == 9
+    assert np.abs(norms - ref_norms).max() < 2e-9     # printed with 9 decimals
+    assert np.abs(psi - ref_psi).max() < 1e-8
+    assert norms[-1] < 1.0 - 1e-6
