@@ -457,6 +457,40 @@ def fvector_measure(op):
             "number_operator_max": float(np.abs(f0).max())}
 
 
+def laser_measure(op, tfinal, nnz=20_000_000):
+    """Laser-driven propagation of the ground state (adc2g_sil_laser) under
+    a cos^2 pulse along x, with a synthetic intermediate-state dipole of nnz
+    off-diagonal elements (the reference computes it in its ISR code; its
+    sparsity here is an assumption), then the same with a diagonal CAP
+    (Lanczos-Arnoldi steps).  tout = 1 a.u."""
+    rng = np.random.default_rng(5)
+    nd = op.ndim
+    row = rng.integers(0, nd, nnz)
+    col = (row + 1 + rng.integers(0, nd - 1, nnz)) % nd
+    op.set_dipole_is(0, rng.normal(0.0, 0.1, nd), row, col, rng.normal(0.0, 1e-3, nnz), 0.3,
+                     rng.normal(0.0, 1e-3, nd))
+    del row, col
+    psi0 = np.zeros(nd + 1, dtype=complex)
+    psi0[nd] = 1.0
+    pulse = [{"envelope": "cos2", "envpar": (1.0, 1.0), "carrier": "cos", "freq": 0.5,
+              "pulse_vec": (1.0, 0.0, 0.0), "strength": 0.05}]
+    out = {"tfinal_au": tfinal, "dipole_offdiagonal_elements": nnz}
+    for name, cap in (("sil", False), ("csil_cap", True)):
+        if cap:
+            op.set_cap_is(np.abs(rng.normal(0.0, 1e-2, nd)), [], [], [], 0.0, np.zeros(nd))
+        t0 = time.perf_counter()
+        psi, norms, st = op.sil_laser(psi0, pulse, tout=1.0, tfinal=tfinal, kdim=20, proptol=1e-8)
+        dt = time.perf_counter() - t0
+        out[name] = {"seconds": round(dt, 3), "steps": st["nsteps"], "operator_products": st["nsigma"],
+                     "ms_per_product": round(1e3 * dt / max(st["nsigma"], 1), 3),
+                     "final_norm": float(norms[-1]), "excited_norm": float(np.linalg.norm(psi[:nd]))}
+        log(f"laser {name}: {tfinal} a.u. in {dt:.2f}s, {st['nsteps']} steps, {st['nsigma']} operator products, "
+            f"norm {norms[-1]:.9f}")
+    op.set_cap_is(None, [], [], [], 0.0, None)
+    op.set_dipole_is(0, None, [], [], [], 0.0, None)
+    return out
+
+
 def adc2x_measure(op, nvec=20, roots=0):
     """ADC(2)-x on the workload's operator (adc2g_set_adc2x): the set-up (the
     (vv|vv), (oo|oo) and ring matrices and the diagonal), and FP64
@@ -513,6 +547,8 @@ def main():
                     help="relax the lowest state in imaginary time to this residual (0 skips)")
     ap.add_argument("--skip-mp2", action="store_true", help="skip the MP2 energy/density timing")
     ap.add_argument("--skip-fvector", action="store_true", help="skip the F-vector timing")
+    ap.add_argument("--laser", type=float, default=2.0,
+                    help="laser / CAP propagation leg over this many a.u. (0 skips)")
     ap.add_argument("--adc2x-roots", type=int, default=0,
                     help="ADC(2)-x leg: also a Davidson to this many roots (the sigma-vector timing runs by "
                          "default; -1 skips the leg)")
@@ -623,6 +659,7 @@ def main():
     rlx = guarded("relax", lambda: relax_measure(op, args.relax)) if args.relax > 0 else None
     mp2r = guarded("mp2", lambda: mp2_measure(op)) if not args.skip_mp2 else None
     fvr = guarded("fvector", lambda: fvector_measure(op)) if not args.skip_fvector else None
+    las = guarded("laser", lambda: laser_measure(op, args.laser)) if args.laser > 0 else None
     # last: the ADC(2)-x leg switches the operator and back
     adx = guarded("adc2x", lambda: adc2x_measure(op, roots=max(args.adc2x_roots, 0))) \
         if args.adc2x_roots >= 0 else None
@@ -657,6 +694,7 @@ def main():
         "relaxation": rlx,
         "mp2": mp2r,
         "fvector": fvr,
+        "laser_propagation": las,
         "adc2x": adx,
         "operator_build_seconds": round(t_build, 3),
     }
