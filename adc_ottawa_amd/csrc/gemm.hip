@@ -27,7 +27,7 @@ bool direct_map(const GemmArgs& g) {
 // Threads per workgroup: FP64 128x128 tiles run 8 waves (each 64x32) so that
 // two workgroups per CU keep four waves per SIMD; the rest run 4 waves.
 template <class T, int BM, int BN> constexpr int threads_for() {
-  return (sizeof(T) == 8 && BM == 128 && BN == 128) ? 512 : 256;
+  return (sizeof(T) == 8 && ((BM == 128 && BN == 128) || BM == 32)) ? 512 : 256;
 }
 
 template <class T, int BM, int BN, int BK, class OutT, int TAG, int MK>
@@ -55,6 +55,50 @@ void launch(const GemmArgs& g, hipStream_t st) {
   }
   HIP_CHECK(hipGetLastError());
   if (g.ev_end) HIP_CHECK(hipEventRecord(g.ev_end, st));
+}
+
+// Split-K / batch partials: workspace for batch * splitk * M * N accumulators
+template <class Acc>
+void* partial_buffer(const GemmArgs& g, GemmWorkspace& ws) {
+  const size_t need = (size_t)g.batch * g.splitk * (size_t)g.M * g.N * sizeof(Acc);
+  if (ws.ext) {
+    if (need > ws.ext_size)
+      throw Error(ADC2G_ENOMEM, "GEMM partials need " + std::to_string(need) + " bytes; the scratch leaves " +
+                                    std::to_string(ws.ext_size));
+    return ws.ext;
+  }
+  ws.buf.alloc(need);
+  return ws.buf.get();
+}
+
+// Fold the partials (over splits, and over batches when they are terms of one
+// sum) through the map.
+template <class Acc>
+void reduce_partials(GemmArgs g, hipStream_t st) {
+  if (g.reduce_batch) {  // the batch partials are further splits of one sum
+    g.splitk *= g.batch;
+    g.batch = 1;
+  }
+  const int64_t total = (int64_t)g.M * g.N * g.batch;
+  if (g.splitk >= 8 && total <= (int64_t)1 << 22) {  // many splits, few outputs
+    const int wb = (int)((total + 63) / 64);
+    if (g.out_prec == 1)
+      hipLaunchKernelGGL((splitk_reduce_wide_kernel<Acc, float>), dim3(wb), dim3(1024), 0, st, g);
+    else if (g.out_prec == 2)
+      hipLaunchKernelGGL((splitk_reduce_wide_kernel<Acc, __bf16>), dim3(wb), dim3(1024), 0, st, g);
+    else
+      hipLaunchKernelGGL((splitk_reduce_wide_kernel<Acc, double>), dim3(wb), dim3(1024), 0, st, g);
+    HIP_CHECK(hipGetLastError());
+    return;
+  }
+  const int blocks = (int)std::min<int64_t>((total + 255) / 256, 4096);
+  if (g.out_prec == 1)
+    hipLaunchKernelGGL((splitk_reduce_kernel<Acc, float>), dim3(blocks), dim3(256), 0, st, g);
+  else if (g.out_prec == 2)
+    hipLaunchKernelGGL((splitk_reduce_kernel<Acc, __bf16>), dim3(blocks), dim3(256), 0, st, g);
+  else
+    hipLaunchKernelGGL((splitk_reduce_kernel<Acc, double>), dim3(blocks), dim3(256), 0, st, g);
+  HIP_CHECK(hipGetLastError());
 }
 
 template <class T>
@@ -124,18 +168,7 @@ void dispatch(GemmArgs g, GemmWorkspace& ws, hipStream_t st, int splitk_hint) {
   // otherwise by the reduction kernel from the partial
   g.direct = (split == 1 && !g.reduce_batch && direct_map<T>(g)) ? 1 : 0;
   const bool need_partial = !g.direct;
-  if (need_partial) {
-    const size_t need = (size_t)g.batch * split * (size_t)g.M * g.N * sizeof(Acc);
-    if (ws.ext) {
-      if (need > ws.ext_size)
-        throw Error(ADC2G_ENOMEM, "GEMM partials need " + std::to_string(need) + " bytes; the scratch leaves " +
-                                      std::to_string(ws.ext_size));
-      g.partial = ws.ext;
-    } else {
-      ws.buf.alloc(need);
-      g.partial = ws.buf.get();
-    }
-  }
+  if (need_partial) g.partial = partial_buffer<Acc>(g, ws);
   switch (shape) {
     case 2: launch<T, 64, 64, BK>(g, st); break;
     case 3: launch<T, 128, 32, BK>(g, st); break;
@@ -146,32 +179,7 @@ void dispatch(GemmArgs g, GemmWorkspace& ws, hipStream_t st, int splitk_hint) {
     case 8: launch<T, 32, 128, BK>(g, st); break;
     default: launch<T, 128, 128, BK>(g, st);
   }
-  if (need_partial) {
-    if (g.reduce_batch) {  // the batch partials are further splits of one sum
-      g.splitk *= g.batch;
-      g.batch = 1;
-    }
-    const int64_t total = (int64_t)g.M * g.N * g.batch;
-    if (g.splitk >= 8 && total <= (int64_t)1 << 22) {  // many splits, few outputs
-      const int wb = (int)((total + 63) / 64);
-      if (g.out_prec == 1)
-        hipLaunchKernelGGL((splitk_reduce_wide_kernel<Acc, float>), dim3(wb), dim3(1024), 0, st, g);
-      else if (g.out_prec == 2)
-        hipLaunchKernelGGL((splitk_reduce_wide_kernel<Acc, __bf16>), dim3(wb), dim3(1024), 0, st, g);
-      else
-        hipLaunchKernelGGL((splitk_reduce_wide_kernel<Acc, double>), dim3(wb), dim3(1024), 0, st, g);
-      HIP_CHECK(hipGetLastError());
-      return;
-    }
-    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 4096);
-    if (g.out_prec == 1)
-      hipLaunchKernelGGL((splitk_reduce_kernel<Acc, float>), dim3(blocks), dim3(256), 0, st, g);
-    else if (g.out_prec == 2)
-      hipLaunchKernelGGL((splitk_reduce_kernel<Acc, __bf16>), dim3(blocks), dim3(256), 0, st, g);
-    else
-      hipLaunchKernelGGL((splitk_reduce_kernel<Acc, double>), dim3(blocks), dim3(256), 0, st, g);
-    HIP_CHECK(hipGetLastError());
-  }
+  if (need_partial) reduce_partials<Acc>(g, st);
 }
 
 }  // namespace

@@ -280,7 +280,10 @@ gemm_kernel(GemmArgs g) {
   constexpr int TM = WM / 16;
   static_assert(BN % 16 == 0 && BM % 32 == 0, "tile shape");
   constexpr int EA = BM * BK / NT;     // elements per thread per tile
-  constexpr int EB = BN * BK / NT;
+  // (B: rounded up; when BN * BK is not a multiple of NT, as for 208-column
+  // tiles on 512 threads, the K-contiguous load and store skip rows >= BN)
+  constexpr int EB = (BN * BK + NT - 1) / NT;
+  static_assert((BM * BK) % NT == 0, "A tile must divide over the threads");
   static_assert(TN >= 1 && EA >= 1 && EB >= 1, "tile too small for the thread count");
   static_assert(BK % KM == 0, "BK must be a multiple of the MFMA K");
 
@@ -391,7 +394,7 @@ gemm_kernel(GemmArgs g) {
 #pragma unroll
       for (int e = 0; e < EB; ++e) {
         const int n = n0 + kc_r + e * RSA;
-        rb[e] = (n < g.N && k < g.K) ? p[(int64_t)(e * RSA) * g.sbn] : (T)0.0f;
+        rb[e] = (kc_r + e * RSA < BN && n < g.N && k < g.K) ? p[(int64_t)(e * RSA) * g.sbn] : (T)0.0f;
       }
     } else {
       const int n = n0 + nb_r;
@@ -442,7 +445,8 @@ gemm_kernel(GemmArgs g) {
     } else if (b_kc) {
       const int kp = kpos(kc_c);
 #pragma unroll
-      for (int e = 0; e < EB; ++e) sB[buf][(kc_r + e * RSA) * LDR + kp] = rb[e];
+      for (int e = 0; e < EB; ++e)
+        if (kc_r + e * RSA < BN) sB[buf][(kc_r + e * RSA) * LDR + kp] = rb[e];
     } else {
 #pragma unroll
       for (int e = 0; e < EB; ++e) sB[buf][nb_r * LDR + kpos(nb_c + e * CSB)] = rb[e];
