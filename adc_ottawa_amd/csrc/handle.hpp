@@ -120,6 +120,7 @@ struct Handle {
   struct DipoleIS {
     bool set = false;
     double d00 = 0.0;
+    double per_row = 0.0;  // stored off-diagonal elements per row (both triangles)
     DevBuf<double> diag, val, d0j;
     DevBuf<int64_t> rowptr;
     DevBuf<int32_t> col;
@@ -132,7 +133,7 @@ struct Handle {
   DevBuf<double> capproj;
   int capproj_n = 0;
   bool capproj_ground = false;
-  DevBuf<double> laser_part, laser_u, laser_c, laser_coef;  // work arrays of the laser operator
+  DevBuf<double> laser_part, laser_u, laser_c, laser_coef, laser_vz;  // work arrays of the laser operator
   void set_dipole_is(int c, const double* diag, int64_t nnz, const int64_t* row, const int64_t* col,
                      const double* val, double d00, const double* d0j);
   void set_cap_is(const double* diag, int64_t nnz, const int64_t* row, const int64_t* col, const double* val,

@@ -17,6 +17,7 @@
 // sigma-vector of the first ndim rows.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "linalg.hpp"
@@ -51,30 +52,63 @@ __device__ double laser_block_sum(double x, double* sh) {
   return t;
 }
 
-// intermediate-state rows: out = -i hv + i sum_c E_c [(D_c + d00_c) v + d0j_c v0]
+typedef double cplx2 __attribute__((ext_vector_type(2)));
+
+// (re, im) interleaved copy of the first nd rows of a planar complex vector:
+// the CSR products gather one 16-byte element per stored entry
+__global__ void interleave_kernel(int64_t nd, int64_t n, const double* __restrict__ v, cplx2* __restrict__ vz) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nd; i += (int64_t)gridDim.x * blockDim.x)
+    vz[i] = (cplx2){v[i], v[n + i]};
+}
+
+// sum over row i of the off-diagonal part: the LN lanes of a group take every
+// LN-th stored element (coalesced reads of col / val) and combine their sums
+// with shuffles inside the group, so every lane of the group ends with the
+// row's sum
+template <int LN>
+__device__ __forceinline__ void csr_row(const LaserComp& c, int64_t i, int s, const cplx2* __restrict__ vz,
+                                        double& re, double& im) {
+  re = 0.0;
+  im = 0.0;
+  const int64_t p1 = c.rowptr[i + 1];
+  for (int64_t p = c.rowptr[i] + s; p < p1; p += LN) {
+    const double w = c.val[p];
+    const cplx2 z = vz[c.col[p]];
+    re += w * z[0];
+    im += w * z[1];
+  }
+#pragma unroll
+  for (int off = LN / 2; off > 0; off >>= 1) {
+    re += __shfl_xor(re, off, 64);
+    im += __shfl_xor(im, off, 64);
+  }
+}
+
+// intermediate-state rows, LN lanes per row:
+// out = -i hv + i sum_c E_c [(D_c + d00_c) v + d0j_c v0]
+template <int LN>
 __global__ void laser_is_kernel(int64_t nd, int64_t n, const double* __restrict__ hv, const double* __restrict__ v,
-                                double* __restrict__ out, LaserDev L) {
+                                const cplx2* __restrict__ vz, double* __restrict__ out, LaserDev L) {
   const double v0re = v[nd], v0im = v[n + nd];
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nd; i += (int64_t)gridDim.x * blockDim.x) {
+  const int s = threadIdx.x % LN;
+  const int64_t stride = (int64_t)gridDim.x * (blockDim.x / LN);
+  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LN; i < nd; i += stride) {
     double sre = 0.0, sim = 0.0;
     for (int q = 0; q < L.nc; ++q) {
       const LaserComp& c = L.c[q];
+      double dre, dim;
+      csr_row<LN>(c, i, s, vz, dre, dim);
       const double dd = c.diag[i] + c.d00;
-      double dre = dd * v[i], dim = dd * v[n + i];
-      for (int64_t p = c.rowptr[i]; p < c.rowptr[i + 1]; ++p) {
-        const int64_t j = c.col[p];
-        const double w = c.val[p];
-        dre += w * v[j];
-        dim += w * v[n + j];
-      }
-      dre += c.d0j[i] * v0re;
-      dim += c.d0j[i] * v0im;
+      dre += dd * v[i] + c.d0j[i] * v0re;
+      dim += dd * v[n + i] + c.d0j[i] * v0im;
       sre += c.E * dre;
       sim += c.E * dim;
     }
     // -i (hre + i him) = him - i hre;  i (sre + i sim) = -sim + i sre
-    out[i] = hv[n + i] - sim;
-    out[n + i] = -hv[i] + sre;
+    if (s == 0) {
+      out[i] = hv[n + i] - sim;
+      out[n + i] = -hv[i] + sre;
+    }
   }
 }
 
@@ -118,20 +152,21 @@ __global__ void laser_ground_final_kernel(int nblk, int64_t nd, int64_t n, const
 }
 
 // out = scale [(diag + d00) u + off u + d0j u0] on the IS rows (no -i: the
-// CAP term -W u of the reference)
-__global__ void isop_is_kernel(int64_t nd, int64_t n, const double* __restrict__ u, double* __restrict__ out,
-                               LaserComp c, double scale) {
+// CAP term -W u of the reference), LN lanes per row
+template <int LN>
+__global__ void isop_is_kernel(int64_t nd, int64_t n, const double* __restrict__ u, const cplx2* __restrict__ uz,
+                               double* __restrict__ out, LaserComp c, double scale) {
   const double u0re = u[nd], u0im = u[n + nd];
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nd; i += (int64_t)gridDim.x * blockDim.x) {
+  const int s = threadIdx.x % LN;
+  const int64_t stride = (int64_t)gridDim.x * (blockDim.x / LN);
+  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LN; i < nd; i += stride) {
+    double re, im;
+    csr_row<LN>(c, i, s, uz, re, im);
     const double dd = c.diag[i] + c.d00;
-    double re = dd * u[i], im = dd * u[n + i];
-    for (int64_t p = c.rowptr[i]; p < c.rowptr[i + 1]; ++p) {
-      const int64_t j = c.col[p];
-      re += c.val[p] * u[j];
-      im += c.val[p] * u[n + j];
+    if (s == 0) {
+      out[i] = scale * (re + dd * u[i] + c.d0j[i] * u0re);
+      out[n + i] = scale * (im + dd * u[n + i] + c.d0j[i] * u0im);
     }
-    out[i] = scale * (re + c.d0j[i] * u0re);
-    out[n + i] = scale * (im + c.d0j[i] * u0im);
   }
 }
 
@@ -177,6 +212,18 @@ __global__ void zero_ground_kernel(int64_t nd, int64_t n, double* __restrict__ x
 __global__ void add_planar_kernel(int64_t m, const double* __restrict__ x, double* __restrict__ y) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
     y[i] += x[i];
+}
+
+// one launch of a row kernel with LN = 1, 4, 8, 16 or 32 lanes per row, from
+// the stored off-diagonal elements per row (grid-stride over the rows)
+template <class F>
+void launch_rows(double per_row, int64_t nd, F&& f) {
+  auto go = [&](auto ln) { f(ln, dim3(grid_for(nd * decltype(ln)::value))); };
+  if (per_row <= 1.0) go(std::integral_constant<int, 1>());
+  else if (per_row <= 6.0) go(std::integral_constant<int, 4>());
+  else if (per_row <= 12.0) go(std::integral_constant<int, 8>());
+  else if (per_row <= 24.0) go(std::integral_constant<int, 16>());
+  else go(std::integral_constant<int, 32>());
 }
 
 double envelope_value(const adc2g_pulse& p, double t) {
@@ -254,6 +301,7 @@ void fill_is_operator(Handle& Hd, Handle::DipoleIS& D, const double* diag_h, int
     D.rowptr.release();
     D.col.release();
     D.d00 = 0.0;
+    D.per_row = 0.0;
     return;
   }
   ARG_CHECK(diag_h && d0j_h, "the diagonal and the ground-state column are required");
@@ -293,6 +341,7 @@ void fill_is_operator(Handle& Hd, Handle::DipoleIS& D, const double* diag_h, int
   HIP_CHECK(hipMemcpyAsync(D.val.get(), vl.data(), sizeof(double) * vl.size(), hipMemcpyHostToDevice, st));
   HIP_CHECK(hipStreamSynchronize(st));
   D.d00 = d00;
+  D.per_row = (double)(2 * nnz) / (double)nd;
   D.set = true;
 }
 
@@ -329,11 +378,13 @@ void laser_apply(Handle& H, const double E[3], const double* v, double* hv, doub
   H.sigma(v, n, hv, n, 2, p, s);
   LaserDev L{};
   L.nc = 0;
+  double per_row = 0.0;  // lanes per row from the stored elements per row of the active components
   for (int c = 0; c < 3; ++c) {
     if (E[c] == 0.0) continue;
     const Handle::DipoleIS& D = H.dip[c];
     ARG_CHECK(D.set, std::string("the field has a ") + "xyz"[c] +
                          " component but that dipole component is not set (adc2g_set_dipole_is)");
+    per_row = std::max(per_row, D.per_row);
     LaserComp& q = L.c[L.nc++];
     q.E = E[c];
     q.d00 = D.d00;
@@ -343,7 +394,15 @@ void laser_apply(Handle& H, const double E[3], const double* v, double* hv, doub
     q.rowptr = D.rowptr.get();
     q.col = D.col.get();
   }
-  hipLaunchKernelGGL(laser_is_kernel, dim3(grid_for(nd)), dim3(256), 0, s, nd, n, hv, v, out, L);
+  H.laser_vz.alloc(2 * (size_t)nd);
+  cplx2* vz = (cplx2*)H.laser_vz.get();
+  if (per_row > 0.0) {
+    hipLaunchKernelGGL(interleave_kernel, dim3(grid_for(nd)), dim3(256), 0, s, nd, n, v, vz);
+    HIP_CHECK(hipGetLastError());
+  }
+  launch_rows(per_row, nd, [&](auto ln, dim3 grid) {
+    hipLaunchKernelGGL(laser_is_kernel<decltype(ln)::value>, grid, dim3(256), 0, s, nd, n, hv, v, vz, out, L);
+  });
   HIP_CHECK(hipGetLastError());
   const int nb = L.nc > 0 ? reduce_blocks(nd) : 0;
   H.laser_part.alloc(2 * (size_t)std::max(nb, 1));
@@ -382,7 +441,14 @@ void laser_apply(Handle& H, const double E[3], const double* v, double* hv, doub
   w.d0j = H.cap.d0j.get();
   w.rowptr = H.cap.rowptr.get();
   w.col = H.cap.col.get();
-  hipLaunchKernelGGL(isop_is_kernel, dim3(grid_for(nd)), dim3(256), 0, s, nd, n, u, c, w, -1.0);
+  cplx2* uz = (cplx2*)H.laser_vz.get();
+  if (H.cap.per_row > 0.0) {
+    hipLaunchKernelGGL(interleave_kernel, dim3(grid_for(nd)), dim3(256), 0, s, nd, n, u, uz);
+    HIP_CHECK(hipGetLastError());
+  }
+  launch_rows(H.cap.per_row, nd, [&](auto ln, dim3 grid) {
+    hipLaunchKernelGGL(isop_is_kernel<decltype(ln)::value>, grid, dim3(256), 0, s, nd, n, u, uz, c, w, -1.0);
+  });
   HIP_CHECK(hipGetLastError());
   const int nbc = reduce_blocks(nd);
   H.laser_part.alloc(2 * (size_t)nbc);
