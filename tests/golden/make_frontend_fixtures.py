@@ -3,7 +3,9 @@
     python tests/golden/make_frontend_fixtures.py
 
 Reads (as text) examples/cs2_sto2g and examples/n2_augccpvdz under
-/root/reference and writes, per example:
+/root/reference and writes, per example (for examples/h2o_autospec and
+examples/allene_lanczos, which ship no ADC output, only the first file, as
+h2o_augccpvdz_gamess.txt and allene_ccpvdz_gamess.txt):
 * <name>_gamess.txt: the GAMESS log sections the front end reads (the
   number of electrons, the 'COORDINATES (BOHR)' table, the 'ATOMIC BASIS SET'
   block) and GAMESS's final RHF and nuclear-repulsion energies -- input data,
@@ -25,6 +27,8 @@ import re
 REF = "/root/reference/examples"
 OUT = os.path.dirname(os.path.abspath(__file__))
 EXAMPLES = {"cs2_sto2g": ("diff.cs2.log", "cs2_adc2.inp"), "n2_augccpvdz": ("diff.adc_z.log", "adc_z.inp")}
+# examples that ship only the GAMESS run (no ADC output): the log sections only
+GAMESS_ONLY = {"h2o_augccpvdz": "h2o_autospec", "allene_ccpvdz": "allene_lanczos"}
 
 
 def trimmed_log(path):
@@ -111,6 +115,10 @@ def main():
         with open(os.path.join(OUT, f"{name}_reference.json"), "w") as f:
             json.dump(ref, f, indent=1)
         print(name, {k: (v if k != "davidson_eV" else {s: len(e) for s, e in v.items()}) for k, v in ref.items()})
+    for name, src in GAMESS_ONLY.items():
+        with open(os.path.join(OUT, f"{name}_gamess.txt"), "w") as f:
+            f.write(trimmed_log(os.path.join(REF, src, "gamess.log")))
+        print(name, "GAMESS log sections")
 
 
 if __name__ == "__main__":

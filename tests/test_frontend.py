@@ -71,7 +71,7 @@ def test_integrals_rotation_invariant():
     assert abs(energies[0] - energies[1]) < 1e-10
 
 
-@pytest.mark.parametrize("name", ["cs2_sto2g", "n2_augccpvdz"])
+@pytest.mark.parametrize("name", ["cs2_sto2g", "n2_augccpvdz", "h2o_augccpvdz", "allene_ccpvdz"])
 def test_rhf_matches_gamess(name):
     log = os.path.join(GOLDEN, f"{name}_gamess.txt")
     lines = open(log).read().splitlines()

@@ -140,3 +140,39 @@ def test_n2_adc2x_frozen_core():
     assert np.abs(ev - ev_s).max() > 1e-3        # the block changes the roots
     r = adc.run(log, 2, k, block=16, maxvec=64, adc2x=True)
     assert r["method"] == "adc2-x" and np.abs(np.sort(r["energies_eV"]) - np.sort(ev) * EH2EV).max() < 1e-5
+
+
+def test_h2o_frozen_core_reference_settings():
+    """H2O / aug-cc-pVDZ (the reference's examples/h2o_autospec GAMESS run,
+    which ships no ADC output): the front end's RHF equals GAMESS's, and the
+    block Davidson with that example's diag_section (3 roots, block 5,
+    subspace 15, tol 1e-6, guess = subdiag,800, the default Olsen step) on
+    the frozen-core space (frozen_core = initial: the O 1s) gives the three
+    lowest eigenvalues of the oracle's matrix on those configurations
+    (ARPACK on the density-fitted restatement)."""
+    import scipy.sparse.linalg as sla
+    from oracle import csf, dfsigma
+    print("[case] front end h2o aug-cc-pVDZ frozen core, subdiag guess", flush=True)
+    log = os.path.join(GOLDEN, "h2o_augccpvdz_gamess.txt")
+    mol = gamess.read_gamess_log(log)
+    ints = gamess.ao_integrals(mol)
+    hf = scf.rhf(mol, ints)
+    assert abs(hf.energy - _gamess_rhf(log)) < 1e-7
+    full, _ = scf.mo_system(hf, ints)
+    with lib.Operator.from_system(full) as op, op.frozen_core([1]) as view:
+        ev, _, st = view.davidson(3, blocksize=5, maxvec=15, tol=1e-6, precon="olsen", maxiter=200,
+                                  vectors=False, guess="subdiag", subdim=800)
+    idx = csf.frozen_core_indices(full.nocc, full.nvirt, [0])
+    df = dfsigma.DF(full)
+    nd = df.n1 + df.D2.size
+
+    def mv(x):
+        X = np.zeros(nd)
+        X[idx] = x
+        return df.sigma(X)[idx]
+
+    A = sla.LinearOperator((idx.size, idx.size), matvec=mv, dtype=float)
+    w = np.sort(sla.eigsh(A, k=3, which="SA", tol=1e-12)[0])
+    print(f"H2O frozen-core roots {np.round(np.sort(ev) * EH2EV, 5)} eV, {st['niter']} iterations; "
+          f"ARPACK {np.round(w * EH2EV, 5)} eV", flush=True)
+    assert np.abs(np.sort(ev) - w).max() < 1e-8
