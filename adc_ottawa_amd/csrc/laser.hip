@@ -363,6 +363,18 @@ void Handle::set_cap_projector(int nvec, const double* vecs, bool ground) {
   ARG_CHECK(nvec == 0 || vecs, "NULL projector vectors");
   HIP_CHECK(hipSetDevice(device));
   if (nvec > 0) {
+    // the projector 1 - R R^T needs orthonormal columns (the reference's
+    // eigenvectors are)
+    double dev = 0.0;
+    for (int a = 0; a < nvec; ++a)
+      for (int b = 0; b <= a; ++b) {
+        double g = 0.0;
+        const double* x = vecs + (size_t)d.ndim * a;
+        const double* y = vecs + (size_t)d.ndim * b;
+        for (int64_t i = 0; i < d.ndim; ++i) g += x[i] * y[i];
+        dev = std::max(dev, std::fabs(g - (a == b ? 1.0 : 0.0)));
+      }
+    ARG_CHECK(dev <= 1e-8, "the projector vectors must be orthonormal (max |R^T R - I| = " + std::to_string(dev) + ")");
     capproj.alloc((size_t)d.ndim * nvec);
     HIP_CHECK(hipMemcpyAsync(capproj.get(), vecs, sizeof(double) * d.ndim * nvec, hipMemcpyHostToDevice, st));
     HIP_CHECK(hipStreamSynchronize(st));

@@ -170,6 +170,8 @@ def test_cap_operator_vs_oracle(ground):
             got = op.matxvec_treal_laser(t, PULSES, x)
             want = laser.matxvec_treal_laser(lambda z: H @ z, dips, laser.efield(PULSES, t), x, cap, (R, ground))
             assert np.abs(got - want).max() < 1e-12 * np.abs(want).max(), t
+        with pytest.raises(lib.Adc2gError):
+            op.set_cap_projector(2.0 * R, ground)      # not orthonormal
         op.set_cap_projector(None, False)
         got = op.matxvec_treal_laser(2.2, PULSES, x)
         want = laser.matxvec_treal_laser(lambda z: H @ z, dips, laser.efield(PULSES, 2.2), x, cap, (None, False))
