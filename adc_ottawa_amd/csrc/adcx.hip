@@ -19,6 +19,7 @@
 // symmetry of Z.)  One vector at a time, FP64; the vvvv matrix is v^4 doubles.
 #include "handle.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace adc2g {
@@ -62,31 +63,45 @@ __device__ __forceinline__ double ax_t(const double* x, const DimsK& d, int c, i
   return 2.0 * ax_z(x, d, c, b, k, l) - ax_z(x, d, b, c, k, l);
 }
 
-// the three layouts of T for one vector (n1^2 elements each)
-__global__ void ax_layouts_kernel(const double* __restrict__ x, DimsK d, double* __restrict__ T1,
-                                  double* __restrict__ T3, double* __restrict__ T4) {
+// the three layouts of T for the nb vectors of a chunk (vector j =
+// blockIdx.y), stored transposed -- row (j, column index), contiguous over
+// the contraction index -- so that one matrix product with both operands
+// K-contiguous serves the whole chunk:
+//   T1t[(j,b,l)][(d,m)] = T(d,b,m,l), T3t[(j,b,k)][(d,m)] = T(d,b,k,m),
+//   T4t[(j,k,l)][(d,e)] = T(d,e,k,l), and T4n[(d,e)][(j,k,l)] = T(d,e,k,l)
+//   for the product with the (oo|oo) matrix
+__global__ void ax_layouts_kernel(const double* __restrict__ X, int64_t ldx, DimsK d, int nb,
+                                  double* __restrict__ T1, double* __restrict__ T3, double* __restrict__ T4,
+                                  double* __restrict__ T4n) {
   const int o = d.o, v = d.v;
+  const int j = blockIdx.y;
+  const double* x = X + ldx * j;
   const int64_t n1 = d.n1, nn = n1 * n1, oo = (int64_t)o * o;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < nn;
        idx += (int64_t)gridDim.x * blockDim.x) {
     const int64_t r = idx / n1, q = idx - r * n1;
     const int dd = (int)(r / o), m = (int)(r - (int64_t)dd * o);
     const int b = (int)(q / o), l = (int)(q - (int64_t)b * o);
-    T1[idx] = ax_t(x, d, dd, b, m, l);
-    T3[idx] = ax_t(x, d, dd, b, l, m);
+    const int64_t o13 = ((int64_t)j * n1 + q) * n1 + r;
+    T1[o13] = ax_t(x, d, dd, b, m, l);
+    T3[o13] = ax_t(x, d, dd, b, l, m);
     const int64_t de = idx / oo, kl = idx - de * oo;
     const int d4 = (int)(de / v), e4 = (int)(de - (int64_t)d4 * v);
     const int k4 = (int)(kl / o), l4 = (int)(kl - (int64_t)k4 * o);
-    T4[idx] = ax_t(x, d, d4, e4, k4, l4);
+    const double t4 = ax_t(x, d, d4, e4, k4, l4);
+    T4[((int64_t)j * oo + kl) * (int64_t)v * v + de] = t4;
+    T4n[(de * nb + j) * oo + kl] = t4;
   }
 }
 
-__device__ __forceinline__ double ax_g(const double* G1, const double* G2, const double* G3, const DimsK& d, int c,
-                                       int b, int k, int l) {
+// G(c,b,k,l) of vector j of a chunk of nb (the column-block layouts above)
+__device__ __forceinline__ double ax_g(const double* G1, const double* G2, const double* G3, const DimsK& d, int nb,
+                                       int j, int c, int b, int k, int l) {
   const int o = d.o, v = d.v;
-  const int64_t n1 = d.n1;
-  return G1[((int64_t)c * o + k) * n1 + (int64_t)b * o + l] - G2[((int64_t)c * o + l) * n1 + (int64_t)b * o + k] +
-         G3[((int64_t)c * v + b) * o * o + (int64_t)k * o + l];
+  const int64_t n1 = d.n1, oo = (int64_t)o * o;
+  return G1[(((int64_t)c * o + k) * nb + j) * n1 + (int64_t)b * o + l] -
+         G2[(((int64_t)c * o + l) * nb + j) * n1 + (int64_t)b * o + k] +
+         G3[(((int64_t)c * v + b) * nb + j) * oo + (int64_t)k * o + l];
 }
 
 // 2h2p configuration J (0-based within the 2h2p part) -> type 1..5 and (a, b, k, l)
@@ -118,23 +133,27 @@ __device__ __forceinline__ int ax_decode(int64_t J, const DimsK& d, int& a, int&
   return t;
 }
 
-// w[off1 + J] += (M22 x)_J from G (the CSF projection, 1/3 on t5)
+// w[off1 + J] += (M22 x)_J from G (the CSF projection, 1/3 on t5), vector
+// j = blockIdx.y of the chunk
 __global__ void ax_project_kernel(const double* __restrict__ G1, const double* __restrict__ G2,
-                                  const double* __restrict__ G3, DimsK d, double* __restrict__ w) {
+                                  const double* __restrict__ G3, DimsK d, int nb, double* __restrict__ W,
+                                  int64_t ldw) {
   const int64_t n2 = d.ndim - d.n1;
+  const int j = blockIdx.y;
+  double* w = W + ldw * j;
   for (int64_t J = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; J < n2; J += (int64_t)gridDim.x * blockDim.x) {
     int a, b, k, l;
     const int t = ax_decode(J, d, a, b, k, l);
     double val;
     if (t == 1) {
-      val = -AX_SQ2 * ax_g(G1, G2, G3, d, a, a, k, k);
+      val = -AX_SQ2 * ax_g(G1, G2, G3, d, nb, j, a, a, k, k);
     } else if (t == 2) {
-      val = -(ax_g(G1, G2, G3, d, a, b, k, k) + ax_g(G1, G2, G3, d, b, a, k, k));
+      val = -(ax_g(G1, G2, G3, d, nb, j, a, b, k, k) + ax_g(G1, G2, G3, d, nb, j, b, a, k, k));
     } else if (t == 3) {
-      val = ax_g(G1, G2, G3, d, a, a, k, l) + ax_g(G1, G2, G3, d, a, a, l, k);
+      val = ax_g(G1, G2, G3, d, nb, j, a, a, k, l) + ax_g(G1, G2, G3, d, nb, j, a, a, l, k);
     } else {
-      const double g1 = ax_g(G1, G2, G3, d, a, b, k, l), g2 = ax_g(G1, G2, G3, d, b, a, k, l);
-      const double g3 = ax_g(G1, G2, G3, d, a, b, l, k), g4 = ax_g(G1, G2, G3, d, b, a, l, k);
+      const double g1 = ax_g(G1, G2, G3, d, nb, j, a, b, k, l), g2 = ax_g(G1, G2, G3, d, nb, j, b, a, k, l);
+      const double g3 = ax_g(G1, G2, G3, d, nb, j, a, b, l, k), g4 = ax_g(G1, G2, G3, d, nb, j, b, a, l, k);
       val = (t == 4) ? AX_ISQ2 * (g1 + g2 + g3 + g4) : (AX_SQ32 / 3.0) * (g1 - g2 - g3 + g4);
     }
     w[d.off1 + J] += val;
@@ -231,15 +250,18 @@ __global__ void ax_add_kernel(int64_t n, double s, const double* __restrict__ x,
     y[i] += s * x[i];
 }
 
-void ax_gemm(Handle& H, int M, int N, int K, const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
-             int64_t ldc, double alpha, double beta, hipStream_t s) {
+// C (M x N, ldc) = alpha A (M x K, lda) Bt^T + beta C with Bt N x K (ldbt): both
+// operands contiguous along K (the vector loads and the wide tiles)
+void ax_gemm_bt(Handle& H, int M, int N, int K, const double* A, int64_t lda, const double* Bt, int64_t ldbt,
+                double* C, int64_t ldc, double alpha, double beta, hipStream_t s) {
   GemmArgs g;
   g.M = M; g.N = N; g.K = K;
   g.A = A; g.sam = lda; g.sak = 1;
-  g.B = B; g.sbk = ldb; g.sbn = 1;
+  g.B = Bt; g.sbk = 1; g.sbn = ldbt;
   g.map.C = C; g.map.scm = ldc; g.map.scn = 1; g.map.alpha = alpha; g.map.beta = beta;
   gemm(g, Prec::F64, H.gws, s);
 }
+
 
 }  // namespace
 
@@ -336,23 +358,39 @@ void Handle::adcx_apply(const double* X, int64_t ldx, double* W, int64_t ldw, in
   ARG_CHECK(!scratch, "the no-allocation sigma-vector does not include the ADC(2)-x block");
   const int o = d.o, v = d.v;
   const int64_t n1 = d.n1, nn = n1 * n1, vv = (int64_t)v * v, oo = (int64_t)o * o;
-  ax_t1.alloc((size_t)nn);
-  ax_t3.alloc((size_t)nn);
-  ax_t4.alloc((size_t)nn);
-  ax_g1.alloc((size_t)nn);
-  ax_g2.alloc((size_t)nn);
-  ax_g3.alloc((size_t)nn);
-  for (int j = 0; j < nvec; ++j) {
-    hipLaunchKernelGGL(ax_layouts_kernel, dim3(grid_for(nn)), dim3(256), 0, s, X + ldx * j, dk, ax_t1.get(),
-                       ax_t3.get(), ax_t4.get());
+  // vectors per chunk: seven n1 x n1 arrays per vector within about 4 GB
+  const int64_t per_vec = 7 * nn * (int64_t)sizeof(double);
+  const int cap = (int)std::max<int64_t>(1, std::min<int64_t>(16, ((int64_t)4 << 30) / per_vec));
+  const int nbmax = std::min(nvec, cap);
+  ax_t1.alloc((size_t)nbmax * nn);
+  ax_t3.alloc((size_t)nbmax * nn);
+  ax_t4.alloc((size_t)nbmax * nn);
+  ax_t4n.alloc((size_t)nbmax * nn);
+  ax_g1.alloc((size_t)nbmax * nn);
+  ax_g2.alloc((size_t)nbmax * nn);
+  ax_g3.alloc((size_t)nbmax * nn);
+  for (int j0 = 0; j0 < nvec; j0 += nbmax) {
+    const int nb = std::min(nbmax, nvec - j0);
+    const int64_t w1 = (int64_t)nb * n1, w4 = (int64_t)nb * oo;  // row lengths of the chunk layouts
+    hipLaunchKernelGGL(ax_layouts_kernel, dim3(grid_for(nn), nb), dim3(256), 0, s, X + ldx * j0, ldx, dk, nb,
+                       ax_t1.get(), ax_t3.get(), ax_t4.get(), ax_t4n.get());
     HIP_CHECK(hipGetLastError());
-    ax_gemm(*this, (int)n1, (int)n1, (int)n1, ax_a1.get(), n1, ax_t1.get(), n1, ax_g1.get(), n1, 1.0, 0.0, s);
-    ax_gemm(*this, (int)n1, (int)n1, (int)n1, ax_ov.get(), n1, ax_t1.get(), n1, ax_g2.get(), n1, 1.0, 0.0, s);
-    ax_gemm(*this, (int)n1, (int)n1, (int)n1, ax_vo.get(), n1, ax_t3.get(), n1, ax_g2.get(), n1, 1.0, 1.0, s);
-    ax_gemm(*this, (int)vv, (int)oo, (int)vv, ax_vvvv.get(), vv, ax_t4.get(), oo, ax_g3.get(), oo, 0.5, 0.0, s);
-    ax_gemm(*this, (int)vv, (int)oo, (int)oo, ax_t4.get(), oo, ax_oooo.get(), oo, ax_g3.get(), oo, 0.5, 1.0, s);
-    hipLaunchKernelGGL(ax_project_kernel, dim3(grid_for(d.ndim - n1)), dim3(256), 0, s, ax_g1.get(), ax_g2.get(),
-                       ax_g3.get(), dk, W + ldw * j);
+    ax_gemm_bt(*this, (int)n1, (int)w1, (int)n1, ax_a1.get(), n1, ax_t1.get(), n1, ax_g1.get(), w1, 1.0, 0.0, s);
+    ax_gemm_bt(*this, (int)n1, (int)w1, (int)n1, ax_ov.get(), n1, ax_t1.get(), n1, ax_g2.get(), w1, 1.0, 0.0, s);
+    ax_gemm_bt(*this, (int)n1, (int)w1, (int)n1, ax_vo.get(), n1, ax_t3.get(), n1, ax_g2.get(), w1, 1.0, 1.0, s);
+    ax_gemm_bt(*this, (int)vv, (int)w4, (int)vv, ax_vvvv.get(), vv, ax_t4.get(), vv, ax_g3.get(), w4, 0.5, 0.0, s);
+    {  // G3 += T4 OOOO / 2 for each vector of the chunk (batch = vector); OOOO is symmetric
+       // ((km|ln) = (mk|nl)), so B(mn, kl) = OOOO[(k,l)][(m,n)] is contiguous along K too
+      GemmArgs g;
+      g.M = (int)vv; g.N = (int)oo; g.K = (int)oo; g.batch = nb;
+      g.A = ax_t4n.get(); g.sam = w4; g.sak = 1; g.sab = oo;
+      g.B = ax_oooo.get(); g.sbk = 1; g.sbn = oo; g.sbb = 0;
+      g.map.C = ax_g3.get(); g.map.scm = w4; g.map.scn = 1; g.map.scb = oo;
+      g.map.alpha = 0.5; g.map.beta = 1.0;
+      gemm(g, Prec::F64, gws, s);
+    }
+    hipLaunchKernelGGL(ax_project_kernel, dim3(grid_for(d.ndim - n1), nb), dim3(256), 0, s, ax_g1.get(),
+                       ax_g2.get(), ax_g3.get(), dk, nb, W + ldw * j0, ldw);
     HIP_CHECK(hipGetLastError());
   }
 }

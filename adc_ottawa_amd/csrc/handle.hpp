@@ -110,7 +110,7 @@ struct Handle {
   // three partial results (adcx.hip).
   bool adcx = false;
   DevBuf<double> ax_vvvv, ax_oooo, ax_ov, ax_vo, ax_a1, ax_d2;
-  DevBuf<double> ax_t1, ax_t3, ax_t4, ax_g1, ax_g2, ax_g3;
+  DevBuf<double> ax_t1, ax_t3, ax_t4, ax_t4n, ax_g1, ax_g2, ax_g3;
   void enable_adcx(bool on);
   void adcx_apply(const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, hipStream_t s);
 
