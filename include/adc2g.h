@@ -232,8 +232,9 @@ int32_t adc2g_create_frozen_core(adc2g_handle* full, int32_t nfrozen, const int3
  * functions C_1_1 .. C_4ii_4ii of source/adclib/adc_ph.f90:1102-1645), and the
  * diagonal gains its diagonal (get_diag_adc2ext_direct); enable = 0 returns to
  * ADC(2)-s.  Full operator only, before any view is made.  The block is
- * applied in FP64 one vector at a time and needs the v^4 (vv|vv) matrix in
- * device memory (12.8 GB at v = 200), built on the first enable.  With it the
+ * applied in FP64 in chunks of up to 8 vectors and needs the v^4 (vv|vv)
+ * matrix in device memory (12.8 GB at v = 200), built on the first enable,
+ * plus seven (ov)^2 work arrays per vector of a chunk.  With it the
  * coupled Davidson step is that of the matrix whose 2h2p block is replaced by
  * its diagonal (an approximation of M(theta), formed from half sigma-vectors).
  * The FP32 and BF16 sigma-vectors add the block in FP64 too; the
