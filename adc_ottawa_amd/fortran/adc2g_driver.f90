@@ -50,9 +50,12 @@
 !            d00(1:3); an optional CAP from cap_file with w0j_file and w00
 !            (cap_ground = .true. projects the ground state out of it); npulse
 !            pulses from carrier(:), envelope(:), pulse_vec(3,:), strength(:),
-!            freq(:), t0(:), phase(:), envpar(2,:).  Prints Time / Norm at
-!            every tout (wrstepinfo) and writes the final wavefunction
-!            (ndim + 1 complex, ground state last) to psi_file.
+!            freq(:), t0(:), phase(:), envpar(2,:).  statenumber = 0 starts
+!            from the ground state, statenumber = n from the n-th Davidson
+!            state (initwf_exci_initstate); cap_nproj = n projects the n
+!            lowest Davidson states out of the CAP (iprojcap).  Prints Time /
+!            Norm at every tout (wrstepinfo) and writes the final
+!            wavefunction (ndim + 1 complex, ground state last) to psi_file.
 !=======================================================================
 program adc2g_driver
 
@@ -74,7 +77,7 @@ program adc2g_driver
   ! laser / CAP propagation
   integer, parameter :: maxpulse = 8
   logical :: laser = .false., cap_ground = .false.
-  integer :: npulse = 0, carrier(maxpulse) = 1, envelope(maxpulse) = 1
+  integer :: npulse = 0, carrier(maxpulse) = 1, envelope(maxpulse) = 1, statenumber = 0, cap_nproj = 0
   real(c_double) :: pulse_vec(3, maxpulse) = 0.0d0, strength(maxpulse) = 0.0d0, freq(maxpulse) = 0.0d0
   real(c_double) :: t0(maxpulse) = 0.0d0, phase(maxpulse) = 0.0d0, envpar(2, maxpulse) = 0.0d0
   real(c_double) :: d00(3) = 0.0d0, w00 = 0.0d0, proptol = 1.0d-8
@@ -84,7 +87,8 @@ program adc2g_driver
        solver, guess, subdim, stepsize, rlx_krydim, autoord, lanczos, lmain, ncycles, lancname, &
        chebyord, cheby_emin, cheby_emax, hmlt, dpl_file, lancguess, orthotype, integrator, siltol, &
        ncore, core, ehf, nfrozen, adc2x, laser, cap_ground, npulse, carrier, envelope, pulse_vec, &
-       strength, freq, t0, phase, envpar, d00, w00, proptol, dipis_file, d0j_file, cap_file, w0j_file, psi_file
+       strength, freq, t0, phase, envpar, d00, w00, proptol, dipis_file, d0j_file, cap_file, w0j_file, psi_file, &
+       statenumber, cap_nproj
 
   type(c_ptr) :: h, hfull
   type(adc2g_opts) :: opts
@@ -320,7 +324,18 @@ contains
        read(u) vec0
        close(u)
        call check(adc2g_set_cap_is(h, odiag, onnz, orow, ocol, oval, w00, vec0))
-       if (cap_ground) call check(adc2g_set_cap_projector(h, 0_c_int32_t, dummy, 1_c_int32_t))
+       if (cap_nproj > 0) then
+          ! iprojcap: the cap_nproj lowest Davidson states (and, with
+          ! cap_ground, the ground state) projected out of the CAP
+          if (cap_nproj > nstates) then
+             write(*, '(a)') ' adc2g input error: cap_nproj exceeds nstates'
+             stop 1
+          end if
+          call check(adc2g_set_cap_projector(h, int(cap_nproj, c_int32_t), eigvec(:, 1:cap_nproj), &
+               merge(1_c_int32_t, 0_c_int32_t, cap_ground)))
+       else if (cap_ground) then
+          call check(adc2g_set_cap_projector(h, 0_c_int32_t, dummy, 1_c_int32_t))
+       end if
        write(*, '(a,i12)') ' laser: CAP off-diagonal elements', onnz
     end if
     do ip = 1, npulse
@@ -340,7 +355,14 @@ contains
     lpar%proptol = proptol
     allocate(psi(ndim + 1), norms(int(tfinal / tout) + 1))
     psi = (0.0d0, 0.0d0)
-    psi(ndim + 1) = (1.0d0, 0.0d0)
+    if (statenumber == 0) then
+       psi(ndim + 1) = (1.0d0, 0.0d0)       ! the MP2 ground state
+    else if (statenumber >= 1 .and. statenumber <= nstates) then
+       psi(1:ndim) = cmplx(eigvec(:, statenumber), 0.0d0, c_double_complex)  ! initwf_exci_initstate
+    else
+       write(*, '(a)') ' adc2g input error: statenumber must be between 0 and nstates'
+       stop 1
+    end if
     call check(adc2g_sil_laser(h, lpar, int(npulse, c_int32_t), pulses, psi, norms, &
          int(size(norms), c_int64_t), nout, ss))
     do i = 1, int(nout)

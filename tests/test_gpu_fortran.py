@@ -434,3 +434,55 @@ def test_fortran_driver_laser_cap(tmp_path):
     assert np.abs(norms - ref_norms).max() < 2e-9     # printed with 9 decimals
     assert np.abs(psi - ref_psi).max() < 1e-8
     assert norms[-1] < 1.0 - 1e-6
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built")
+def test_fortran_driver_laser_excited_state(tmp_path):
+    """statenumber = 1 starts the laser propagation from the lowest Davidson
+    state (initwf_exci_initstate) and cap_nproj = 2 projects the two lowest
+    states out of the CAP (iprojcap); norms and final wavefunction equal the
+    oracle's propagation from the eigenvectors the driver wrote."""
+    from oracle import laser
+    print("[case] fortran driver laser from an excited state o=3 v=5 naux=8", flush=True)
+    o, v, naux = 3, 5, 8
+    sys_ = make_system(o, v, naux, seed=22)
+    sys_.eps.astype("<f8").tofile(tmp_path / "eps.bin")
+    sys_.B.astype("<f8").tofile(tmp_path / "B.bin")
+    df = dfsigma.DF(sys_)
+    nd = df.n1 + df.D2.size
+    H = df.sigma(np.eye(nd))
+    H = 0.5 * (H + H.T)
+    rng = np.random.default_rng(22)
+    row = rng.integers(0, nd, 3 * nd)
+    col = (row + 1 + rng.integers(0, nd - 1, row.size)) % nd
+    val = rng.normal(0.0, 0.05, row.size)
+    diag = rng.normal(0.0, 0.2, nd)
+    d0j = rng.normal(0.0, 0.1, nd)
+    _write_is_operator(tmp_path / "dipz", diag, row, col, val)
+    d0j.astype("<f8").tofile(tmp_path / "d0jz.bin")
+    dips = [None, None, (laser.dense_dipole(diag, row, col, val, nd), 0.1, d0j)]
+    wd = np.abs(rng.normal(0.0, 0.05, nd))
+    _write_is_operator(tmp_path / "cap", wd, np.zeros(0, int), np.zeros(0, int), np.zeros(0))
+    np.zeros(nd).astype("<f8").tofile(tmp_path / "w0j.bin")
+    pulses = [{"envelope": "sin2", "envpar": (0.0, 1.5), "carrier": "cos", "freq": 0.9,
+               "pulse_vec": (0.0, 0.0, 1.0), "strength": 0.1}]
+    nml = (f"&adc2g nocc={o} nvirt={v} naux={naux} eps_file='eps.bin' b_file='B.bin' nstates=3 blocksize=4 "
+           f"maxvec=24 tol=1d-9 davname='davstates' laser=.true. tout=0.5d0 tfinal=3d0 krydim=16 proptol=1d-9 "
+           f"dipis_file='','','dipz' d0j_file='','','d0jz.bin' d00=0d0,0d0,0.1d0 "
+           f"cap_file='cap' w0j_file='w0j.bin' w00=0d0 cap_nproj=2 statenumber=1 npulse=1 "
+           f"carrier=1 envelope=4 pulse_vec=0d0,0d0,1d0 strength=0.1d0 freq=0.9d0 envpar=0d0,1.5d0 "
+           f"psi_file='psi.bin' /\n")
+    r = subprocess.run([DRIVER], input=nml, capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    print(r.stdout[-1200:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    recs = read_davstates(tmp_path / "davstates", nd)
+    X = np.stack([x[2] for x in recs], axis=1)
+    norms = np.array([float(x.split()[3]) for x in r.stdout.splitlines() if x.strip().startswith("Time:")])
+    psi = np.fromfile(tmp_path / "psi.bin", dtype="<c16")
+    psi0 = np.zeros(nd + 1, dtype=complex)
+    psi0[:nd] = X[:, 0]
+    ref_psi, ref_norms = laser.propagate(lambda z: H @ z, dips, pulses, psi0, 0.5, 3.0, 16, 1e-9,
+                                         (np.diag(wd), 0.0, np.zeros(nd)), (X[:, :2], False))
+    assert norms.size == ref_norms.size == 7
+    assert np.abs(norms - ref_norms).max() < 2e-9
+    assert np.abs(psi - ref_psi).max() < 1e-8
