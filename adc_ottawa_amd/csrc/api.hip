@@ -269,11 +269,21 @@ int32_t adc2g_matxvec_treal_laser(adc2g_handle* h, double time, int32_t npulse, 
 }
 
 int32_t adc2g_sil_laser(adc2g_handle* h, const adc2g_laser_params* p, int32_t npulse, const adc2g_pulse* pulses,
-                        double* psi, double* norms, int64_t nrec_cap, int64_t* nrec_out, adc2g_sil_stats* stats) {
+                        double* psi, double* norms, double* flux, int64_t nrec_cap, int64_t* nrec_out,
+                        adc2g_sil_stats* stats) {
   return guarded([&] {
     ARG_CHECK(h && p && psi && norms, "NULL argument");
     HIP_CHECK(hipSetDevice(h->h.device));
-    sil_laser_run(h->h, *p, npulse, pulses, psi, norms, nrec_cap, nrec_out, stats);
+    sil_laser_run(h->h, *p, npulse, pulses, psi, norms, flux, nrec_cap, nrec_out, stats);
+  });
+}
+
+int32_t adc2g_set_flux_theta(adc2g_handle* h, const double* diag, int64_t nnz, const int64_t* row,
+                             const int64_t* col, const double* val, double theta00, const double* theta0j) {
+  return guarded([&] {
+    ARG_CHECK(h, "NULL handle");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    h->h.set_flux_theta(diag, nnz, row, col, val, theta00, theta0j);
   });
 }
 

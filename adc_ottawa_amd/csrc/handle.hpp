@@ -130,6 +130,7 @@ struct Handle {
   // and the projector of the CAP term: orthonormal IS vectors (ndim x
   // capproj_n) and whether the ground state is projected out
   DipoleIS cap;
+  DipoleIS flux_theta;  // projector onto the absorbing region for the flux (theta00 = d00)
   DevBuf<double> capproj;
   int capproj_n = 0;
   bool capproj_ground = false;
@@ -139,6 +140,8 @@ struct Handle {
   void set_cap_is(const double* diag, int64_t nnz, const int64_t* row, const int64_t* col, const double* val,
                   double w00, const double* w0j);
   void set_cap_projector(int nvec, const double* vecs, bool ground);
+  void set_flux_theta(const double* diag, int64_t nnz, const int64_t* row, const int64_t* col, const double* val,
+                      double theta00, const double* theta0j);
 
   Handle() = default;
   ~Handle();

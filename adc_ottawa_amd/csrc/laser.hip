@@ -151,21 +151,22 @@ __global__ void laser_ground_final_kernel(int nblk, int64_t nd, int64_t n, const
   }
 }
 
-// out = scale [(diag + d00) u + off u + d0j u0] on the IS rows (no -i: the
-// CAP term -W u of the reference), LN lanes per row
+// out = scale [op (diag u + off u) + add u + d0j u0] on the IS rows, LN
+// lanes per row (no -i: the CAP term -W u of the reference takes op = 1,
+// add = w00; the flux's operators op = 1 + their ground element, add = 0)
 template <int LN>
 __global__ void isop_is_kernel(int64_t nd, int64_t n, const double* __restrict__ u, const cplx2* __restrict__ uz,
-                               double* __restrict__ out, LaserComp c, double scale) {
+                               double* __restrict__ out, LaserComp c, double scale, double op, double add) {
   const double u0re = u[nd], u0im = u[n + nd];
   const int s = threadIdx.x % LN;
   const int64_t stride = (int64_t)gridDim.x * (blockDim.x / LN);
   for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LN; i < nd; i += stride) {
     double re, im;
     csr_row<LN>(c, i, s, uz, re, im);
-    const double dd = c.diag[i] + c.d00;
+    const double dg = c.diag[i];
     if (s == 0) {
-      out[i] = scale * (re + dd * u[i] + c.d0j[i] * u0re);
-      out[n + i] = scale * (im + dd * u[n + i] + c.d0j[i] * u0im);
+      out[i] = scale * (op * (re + dg * u[i]) + add * u[i] + c.d0j[i] * u0re);
+      out[n + i] = scale * (op * (im + dg * u[n + i]) + add * u[n + i] + c.d0j[i] * u0im);
     }
   }
 }
@@ -358,6 +359,11 @@ void Handle::set_cap_is(const double* diag_h, int64_t nnz, const int64_t* row, c
   fill_is_operator(*this, cap, diag_h, nnz, row, colh, valh, w00, w0j_h);
 }
 
+void Handle::set_flux_theta(const double* diag_h, int64_t nnz, const int64_t* row, const int64_t* colh,
+                            const double* valh, double theta00, const double* theta0j_h) {
+  fill_is_operator(*this, flux_theta, diag_h, nnz, row, colh, valh, theta00, theta0j_h);
+}
+
 void Handle::set_cap_projector(int nvec, const double* vecs, bool ground) {
   ARG_CHECK(nvec >= 0, "nvec must be >= 0");
   ARG_CHECK(nvec == 0 || vecs, "NULL projector vectors");
@@ -383,6 +389,55 @@ void Handle::set_cap_projector(int nvec, const double* vecs, bool ground) {
   }
   capproj_n = nvec;
   capproj_ground = ground;
+}
+
+// x <- P x: the selected bound states and, if chosen, the ground state
+// projected out (planar complex, ndim + 1 rows)
+void cap_project(Handle& H, double* x) {
+  const int64_t nd = H.d.ndim, n = nd + 1;
+  if (H.capproj_n > 0) {
+    H.laser_coef.alloc(2 * (size_t)H.capproj_n);
+    gemm_tn(H, H.capproj_n, 2, nd, H.capproj.get(), nd, x, n, H.laser_coef.get(), H.capproj_n, 1.0, 0.0);
+    gemm_nn(H, nd, 2, H.capproj_n, H.capproj.get(), nd, H.laser_coef.get(), H.capproj_n, x, n, -1.0, 1.0);
+  }
+  if (H.capproj_ground) {
+    hipLaunchKernelGGL(zero_ground_kernel, dim3(1), dim3(64), 0, H.st, nd, n, x);
+    HIP_CHECK(hipGetLastError());
+  }
+}
+
+// out = scale [op (D_IS u) + add u + d0j u0 ; d00 u0 + d0j . u] for an IS
+// operator D (diagonal + CSR, ground element d00, column d0j)
+void apply_is_operator(Handle& H, const Handle::DipoleIS& D, const double* u, double* out, double scale, double op,
+                       double add) {
+  const int64_t nd = H.d.ndim, n = nd + 1;
+  const hipStream_t s = H.st;
+  LaserComp w;
+  w.E = 1.0;
+  w.d00 = D.d00;
+  w.diag = D.diag.get();
+  w.val = D.val.get();
+  w.d0j = D.d0j.get();
+  w.rowptr = D.rowptr.get();
+  w.col = D.col.get();
+  H.laser_vz.alloc(2 * (size_t)nd);
+  cplx2* uz = (cplx2*)H.laser_vz.get();
+  if (D.per_row > 0.0) {
+    hipLaunchKernelGGL(interleave_kernel, dim3(grid_for(nd)), dim3(256), 0, s, nd, n, u, uz);
+    HIP_CHECK(hipGetLastError());
+  }
+  launch_rows(D.per_row, nd, [&](auto ln, dim3 grid) {
+    hipLaunchKernelGGL(isop_is_kernel<decltype(ln)::value>, grid, dim3(256), 0, s, nd, n, u, uz, out, w, scale, op,
+                       add);
+  });
+  HIP_CHECK(hipGetLastError());
+  const int nbc = reduce_blocks(nd);
+  H.laser_part.alloc(2 * (size_t)std::max(nbc, reduce_blocks(n)));
+  hipLaunchKernelGGL(isop_ground_partial_kernel, dim3(nbc), dim3(256), 0, s, nd, n, u, w, H.laser_part.get());
+  HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(isop_ground_final_kernel, dim3(1), dim3(256), 0, s, nbc, nd, n, u, w, scale, H.laser_part.get(),
+                     out);
+  HIP_CHECK(hipGetLastError());
 }
 
 void laser_apply(Handle& H, const double E[3], const double* v, double* hv, double* out, Prec p, hipStream_t s) {
@@ -425,52 +480,49 @@ void laser_apply(Handle& H, const double E[3], const double* v, double* hv, doub
   hipLaunchKernelGGL(laser_ground_final_kernel, dim3(1), dim3(256), 0, s, nb, nd, n, v, L, H.laser_part.get(), out);
   HIP_CHECK(hipGetLastError());
   if (!H.cap.set) return;
-  // CAP (tdselib.f90:722-830): out += -P (W + w00) P v - P (w0j terms) P v,
-  // P the projector onto the complement of the selected bound states
+  // CAP (tdselib.f90:722-830): out += -P [(W + w00) u + w0j u0] (IS rows) and
+  // -P (w00 u0 + w0j . u) (ground row), u = P v
   ARG_CHECK(s == H.st, "the CAP term runs on the handle's stream");
+  H.laser_u.alloc(2 * (size_t)n);
+  H.laser_c.alloc(2 * (size_t)n);
+  HIP_CHECK(hipMemcpyAsync(H.laser_u.get(), v, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, s));
+  cap_project(H, H.laser_u.get());
+  apply_is_operator(H, H.cap, H.laser_u.get(), H.laser_c.get(), -1.0, 1.0, H.cap.d00);
+  cap_project(H, H.laser_c.get());
+  hipLaunchKernelGGL(add_planar_kernel, dim3(grid_for(2 * n)), dim3(256), 0, s, 2 * n, H.laser_c.get(), out);
+  HIP_CHECK(hipGetLastError());
+}
+
+// adc2_flux_cap (source/propagation/flux.f90:179-411):
+//   flux = 2 Re <dpsi/dt | P Theta' P psi> + 2 Re <psi | P W' P psi>,
+// Theta the IS representation of the projector onto the absorbing region
+// (SCRATCH/theta, theta00, theta0j) and W that of the CAP.  The reference
+// adds theta00 (w00) times the operator's IS output rather than times the
+// vector on the IS rows (oppsi = oppsi + theta00 * oppsi); Theta' and W' are
+// those operators as written there.
+double laser_flux(Handle& H, const double* psi, const double* dtpsi) {
+  ARG_CHECK(H.cap.set && H.flux_theta.set, "the flux needs a CAP and the Theta operator (adc2g_set_flux_theta)");
+  const int64_t nd = H.d.ndim, n = nd + 1;
+  const hipStream_t s = H.st;
   H.laser_u.alloc(2 * (size_t)n);
   H.laser_c.alloc(2 * (size_t)n);
   double* u = H.laser_u.get();
   double* c = H.laser_c.get();
-  auto project = [&](double* x) {
-    if (H.capproj_n > 0) {
-      H.laser_coef.alloc(2 * (size_t)H.capproj_n);
-      gemm_tn(H, H.capproj_n, 2, nd, H.capproj.get(), nd, x, n, H.laser_coef.get(), H.capproj_n, 1.0, 0.0);
-      gemm_nn(H, nd, 2, H.capproj_n, H.capproj.get(), nd, H.laser_coef.get(), H.capproj_n, x, n, -1.0, 1.0);
-    }
-    if (H.capproj_ground) {
-      hipLaunchKernelGGL(zero_ground_kernel, dim3(1), dim3(64), 0, s, nd, n, x);
-      HIP_CHECK(hipGetLastError());
-    }
+  auto term = [&](const Handle::DipoleIS& D, const double* bra) {
+    HIP_CHECK(hipMemcpyAsync(u, psi, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, s));
+    cap_project(H, u);
+    apply_is_operator(H, D, u, c, 1.0, 1.0 + D.d00, 0.0);
+    cap_project(H, c);
+    const int nbk = launch_cdot_partial(n, bra, c, n, H.laser_part.get(), s);
+    H.laser_coef.alloc(2);
+    launch_colsum(nbk, 2, H.laser_part.get(), H.laser_coef.get(), s);
+    double h[2];
+    HIP_CHECK(hipMemcpyAsync(h, H.laser_coef.get(), sizeof(h), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return 2.0 * h[0];
   };
-  HIP_CHECK(hipMemcpyAsync(u, v, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, s));
-  project(u);
-  LaserComp w;
-  w.E = 1.0;
-  w.d00 = H.cap.d00;
-  w.diag = H.cap.diag.get();
-  w.val = H.cap.val.get();
-  w.d0j = H.cap.d0j.get();
-  w.rowptr = H.cap.rowptr.get();
-  w.col = H.cap.col.get();
-  cplx2* uz = (cplx2*)H.laser_vz.get();
-  if (H.cap.per_row > 0.0) {
-    hipLaunchKernelGGL(interleave_kernel, dim3(grid_for(nd)), dim3(256), 0, s, nd, n, u, uz);
-    HIP_CHECK(hipGetLastError());
-  }
-  launch_rows(H.cap.per_row, nd, [&](auto ln, dim3 grid) {
-    hipLaunchKernelGGL(isop_is_kernel<decltype(ln)::value>, grid, dim3(256), 0, s, nd, n, u, uz, c, w, -1.0);
-  });
-  HIP_CHECK(hipGetLastError());
-  const int nbc = reduce_blocks(nd);
-  H.laser_part.alloc(2 * (size_t)nbc);
-  hipLaunchKernelGGL(isop_ground_partial_kernel, dim3(nbc), dim3(256), 0, s, nd, n, u, w, H.laser_part.get());
-  HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(isop_ground_final_kernel, dim3(1), dim3(256), 0, s, nbc, nd, n, u, w, -1.0, H.laser_part.get(), c);
-  HIP_CHECK(hipGetLastError());
-  project(c);
-  hipLaunchKernelGGL(add_planar_kernel, dim3(grid_for(2 * n)), dim3(256), 0, s, 2 * n, c, out);
-  HIP_CHECK(hipGetLastError());
+  H.laser_part.alloc(2 * (size_t)reduce_blocks(n));
+  return term(H.flux_theta, dtpsi) + term(H.cap, psi);
 }
 
 void matxvec_treal_laser_host(Handle& H, double t, int npulse, const adc2g_pulse* pulses, const double* v1,

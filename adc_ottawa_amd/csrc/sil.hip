@@ -547,13 +547,14 @@ void sil_run(Handle& H, const adc2g_sil_params& p, const double* fvec, double* a
 // skipped (time advances by the whole period); the norm is recorded at the
 // start of every period and at the end.
 void sil_laser_run(Handle& H, const adc2g_laser_params& p, int npulse, const adc2g_pulse* pulses, double* psi_io,
-                   double* norms, int64_t nrec_cap, int64_t* nrec_out, adc2g_sil_stats* stats) {
+                   double* norms, double* flux, int64_t nrec_cap, int64_t* nrec_out, adc2g_sil_stats* stats) {
   const auto t0 = std::chrono::steady_clock::now();
   ARG_CHECK(p.tout > 0 && p.tfinal >= 0, "tout must be > 0 and tfinal >= 0");
   ARG_CHECK(p.kdim >= 1 && p.kdim <= 30, "krydim must be in 1..30");
   ARG_CHECK(p.proptol > 0, "proptol must be > 0");
   if (p.kdim < 2) throw Error(ADC2G_ESIL_ORDER, "IntOrder is too small");
   laser_check_pulses(npulse, pulses);
+  ARG_CHECK(!flux || (H.cap.set && H.flux_theta.set), "the flux needs a CAP and the Theta operator");
   const double intperiod = p.tout;
   const int64_t nrec = (int64_t)(p.tfinal / intperiod);
   ARG_CHECK(nrec_cap >= nrec + 1, "norms holds fewer than nrec+1 values");
@@ -584,7 +585,10 @@ void sil_laser_run(Handle& H, const adc2g_laser_params& p, int npulse, const adc
     while (true) {
       const double stepsize = intperiod - inttime;
       S.func(S.Kp(0), S.Kp(1));
-      if (inttime == 0.0) norms[i - 1] = std::sqrt(S.cdot(S.Kp(0), S.Kp(0)).real());
+      if (inttime == 0.0) {
+        norms[i - 1] = std::sqrt(S.cdot(S.Kp(0), S.Kp(0)).real());
+        if (flux) flux[i - 1] = laser_flux(H, S.Kp(0), S.Kp(1));
+      }
       if (std::sqrt(S.cdot(S.Kp(1), S.Kp(1)).real()) > tinier) {
         int order = 0;
         const double truestep = csil ? S.csil_step(stepsize, p.proptol, order) : S.step(stepsize, p.proptol, order);
@@ -605,6 +609,10 @@ void sil_laser_run(Handle& H, const adc2g_laser_params& p, int npulse, const adc
     }
   }
   norms[nrec] = std::sqrt(S.cdot(S.Kp(0), S.Kp(0)).real());
+  if (flux) {  // the final flux value (proplib_adc2.f90:816-819)
+    S.func(S.Kp(0), S.Kp(1));
+    flux[nrec] = laser_flux(H, S.Kp(0), S.Kp(1));
+  }
   HIP_CHECK(hipMemcpyAsync(planar.data(), S.Kp(0), sizeof(double) * 2 * n, hipMemcpyDeviceToHost, H.st));
   HIP_CHECK(hipStreamSynchronize(H.st));
   for (int64_t i = 0; i < n; ++i) {

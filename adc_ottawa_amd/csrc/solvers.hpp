@@ -38,7 +38,9 @@ void laser_apply(Handle& H, const double E[3], const double* v, double* hv, doub
 void matxvec_treal_laser_host(Handle& H, double t, int npulse, const adc2g_pulse* pulses, const double* v1,
                               double* v2);
 void sil_laser_run(Handle& H, const adc2g_laser_params& p, int npulse, const adc2g_pulse* pulses, double* psi,
-                   double* norms, int64_t nrec_cap, int64_t* nrec_out, adc2g_sil_stats* stats);
+                   double* norms, double* flux, int64_t nrec_cap, int64_t* nrec_out, adc2g_sil_stats* stats);
+// the CAP flux of adc2_flux_cap for planar psi and dpsi/dt (device, ndim + 1 rows)
+double laser_flux(Handle& H, const double* psi, const double* dtpsi);
 
 // Chebyshev order-domain autocorrelation C_0..C_2K (auto_out, host) of the
 // host vector q0 with the spectrum scaled by bounds[0..1] = (Emin, Emax).

@@ -53,9 +53,11 @@
 !            freq(:), t0(:), phase(:), envpar(2,:).  statenumber = 0 starts
 !            from the ground state, statenumber = n from the n-th Davidson
 !            state (initwf_exci_initstate); cap_nproj = n projects the n
-!            lowest Davidson states out of the CAP (iprojcap).  Prints Time /
-!            Norm at every tout (wrstepinfo) and writes the final
-!            wavefunction (ndim + 1 complex, ground state last) to psi_file.
+!            lowest Davidson states out of the CAP (iprojcap); theta_file
+!            (SCRATCH/theta) with theta0j_file and theta00 gives the CAP flux
+!            in 'flux.dat' (lflux).  Prints Time / Norm at every tout
+!            (wrstepinfo) and writes the final wavefunction (ndim + 1 complex,
+!            ground state last) to psi_file.
 !=======================================================================
 program adc2g_driver
 
@@ -82,13 +84,15 @@ program adc2g_driver
   real(c_double) :: t0(maxpulse) = 0.0d0, phase(maxpulse) = 0.0d0, envpar(2, maxpulse) = 0.0d0
   real(c_double) :: d00(3) = 0.0d0, w00 = 0.0d0, proptol = 1.0d-8
   character(len=256) :: dipis_file(3) = '', d0j_file(3) = '', cap_file = '', w0j_file = '', psi_file = 'psi.bin'
+  character(len=256) :: theta_file = '', theta0j_file = ''
+  real(c_double) :: theta00 = 0.0d0
   namelist /adc2g/ nocc, nvirt, naux, eps_file, b_file, nstates, blocksize, maxvec, maxiter, tol, &
        precon, precision, deflate, davname, sil, fvec_file, auto_file, tout, tfinal, krydim, autotol, &
        solver, guess, subdim, stepsize, rlx_krydim, autoord, lanczos, lmain, ncycles, lancname, &
        chebyord, cheby_emin, cheby_emax, hmlt, dpl_file, lancguess, orthotype, integrator, siltol, &
        ncore, core, ehf, nfrozen, adc2x, laser, cap_ground, npulse, carrier, envelope, pulse_vec, &
        strength, freq, t0, phase, envpar, d00, w00, proptol, dipis_file, d0j_file, cap_file, w0j_file, psi_file, &
-       statenumber, cap_nproj
+       statenumber, cap_nproj, theta_file, theta0j_file, theta00
 
   type(c_ptr) :: h, hfull
   type(adc2g_opts) :: opts
@@ -299,6 +303,7 @@ contains
     type(adc2g_laser_params) :: lpar
     type(adc2g_pulse) :: pulses(maxpulse)
     real(c_double), allocatable :: odiag(:), oval(:), vec0(:), norms(:)
+    real(c_double), allocatable, target :: fluxv(:)
     integer(c_int64_t), allocatable :: orow(:), ocol(:)
     complex(c_double_complex), allocatable :: psi(:)
     real(c_double) :: dummy(1)
@@ -363,11 +368,34 @@ contains
        write(*, '(a)') ' adc2g input error: statenumber must be between 0 and nstates'
        stop 1
     end if
-    call check(adc2g_sil_laser(h, lpar, int(npulse, c_int32_t), pulses, psi, norms, &
-         int(size(norms), c_int64_t), nout, ss))
+    allocate(fluxv(size(norms)))
+    if (len_trim(theta_file) > 0) then
+       ! lflux: the projector onto the absorbing region (SCRATCH/theta)
+       call read_is_operator(trim(theta_file), ndim, odiag, orow, ocol, oval, onnz)
+       open(newunit=u, file=trim(theta0j_file), access='stream', form='unformatted', status='old')
+       read(u) vec0
+       close(u)
+       call check(adc2g_set_flux_theta(h, odiag, onnz, orow, ocol, oval, theta00, vec0))
+       call check(adc2g_sil_laser(h, lpar, int(npulse, c_int32_t), pulses, psi, norms, c_loc(fluxv), &
+            int(size(norms), c_int64_t), nout, ss))
+    else
+       call check(adc2g_sil_laser(h, lpar, int(npulse, c_int32_t), pulses, psi, norms, c_null_ptr, &
+            int(size(norms), c_int64_t), nout, ss))
+    end if
     do i = 1, int(nout)
        write(*, '(a,f14.6,a,f12.9)') ' Time:', (i - 1) * tout, '   Norm:', norms(i)
     end do
+    if (len_trim(theta_file) > 0) then
+       ! flux.dat as wrstepinfo writes it (proplib_adc2.f90:133-138, 858)
+       open(newunit=u, file='flux.dat', form='formatted', status='replace')
+       write(u, '(72a)') ('#', i = 1, 72)
+       write(u, '(a)') '# Time (au)    Flux'
+       write(u, '(72a)') ('#', i = 1, 72)
+       do i = 1, int(nout)
+          write(u, '(F10.4,5x,ES15.8)') (i - 1) * tout, fluxv(i)
+       end do
+       close(u)
+    end if
     write(*, '(a,i10,a,i10)') ' laser: steps', ss%nsteps, ', operator products', ss%nsigma
     open(newunit=u, file=trim(psi_file), access='stream', form='unformatted', status='replace')
     write(u) psi

@@ -167,7 +167,7 @@ SYMBOLS = [
     "adc2g_sil_autocorr_orders", "adc2g_create_cvs", "adc2g_create_frozen_core", "adc2g_set_adc2x", "adc2g_mp2", "adc2g_fvector", "adc2g_lanczos_guess",
     "adc2g_spectral_bounds", "adc2g_power", "adc2g_coupled_schur", "adc2g_sigma_device_noalloc",
     "adc2g_set_dipole_is", "adc2g_matxvec_treal_laser", "adc2g_sil_laser", "adc2g_set_cap_is",
-    "adc2g_set_cap_projector",
+    "adc2g_set_cap_projector", "adc2g_set_flux_theta",
 ]
 
 _lib = None
@@ -229,8 +229,9 @@ def load(path: str = LIBPATH):
     lib.adc2g_matxvec_treal_laser.argtypes = [vp, C.c_double, i32, vp, vp, vp]
     lib.adc2g_set_cap_is.argtypes = [vp, vp, i64, vp, vp, vp, C.c_double, vp]
     lib.adc2g_set_cap_projector.argtypes = [vp, i32, vp, i32]
-    lib.adc2g_sil_laser.argtypes = [vp, C.POINTER(LaserParams), i32, vp, vp, vp, i64, C.POINTER(i64),
+    lib.adc2g_sil_laser.argtypes = [vp, C.POINTER(LaserParams), i32, vp, vp, vp, vp, i64, C.POINTER(i64),
                                     C.POINTER(SilStats)]
+    lib.adc2g_set_flux_theta.argtypes = [vp, vp, i64, vp, vp, vp, C.c_double, vp]
     for s in SYMBOLS:
         getattr(lib, s).restype = C.c_int32
     _lib = lib
@@ -564,21 +565,47 @@ class Operator:
                                              _ptr(v2)))
         return v2
 
-    def sil_laser(self, psi0, pulses, tout, tfinal, kdim=20, proptol=1e-8, precision="fp64", verbose=False):
+    def set_flux_theta(self, diag, row, col, val, theta00, theta0j):
+        """The projector Theta onto the absorbing region for the CAP flux (as
+        set_dipole_is); diag = theta0j = None with no elements clears it."""
+        row = np.ascontiguousarray(row, dtype=np.int64)
+        col = np.ascontiguousarray(col, dtype=np.int64)
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        if not (row.size == col.size == val.size):
+            raise ValueError("row, col and val must have the same length")
+        if diag is None and theta0j is None:
+            if row.size or theta00 != 0.0:
+                raise ValueError("diag and theta0j are required unless Theta is cleared")
+            check(_lib.adc2g_set_flux_theta(self._h, None, 0, None, None, None, 0.0, None))
+            return
+        diag = np.ascontiguousarray(diag, dtype=np.float64)
+        theta0j = np.ascontiguousarray(theta0j, dtype=np.float64)
+        if diag.shape != (self.ndim,) or theta0j.shape != (self.ndim,):
+            raise ValueError("diag and theta0j must have ndim elements")
+        check(_lib.adc2g_set_flux_theta(self._h, _ptr(diag), row.size, _ptr(row), _ptr(col), _ptr(val),
+                                        float(theta00), _ptr(theta0j)))
+
+    def sil_laser(self, psi0, pulses, tout, tfinal, kdim=20, proptol=1e-8, precision="fp64", verbose=False,
+                  flux=False):
         """SIL propagation of psi0 (ndim + 1 complex components) under the
-        pulses; returns (psi(tfinal), norms at i * tout, stats)."""
+        pulses; returns (psi(tfinal), norms at i * tout, stats), and the CAP
+        flux at the same times after the norms when flux is set."""
         psi = np.array(psi0, dtype=np.complex128)
         if psi.shape != (self.ndim + 1,):
             raise ValueError("psi0 must have ndim + 1 components")
         nrec = int(tfinal / tout)
         norms = np.empty(nrec + 1)
+        fl = np.empty(nrec + 1) if flux else None
         p = LaserParams(tout, tfinal, kdim, PREC[precision], proptol, 1 if verbose else 0, 0)
         arr = pulses_array(pulses)
         n = C.c_int64()
         st = SilStats()
         check(_lib.adc2g_sil_laser(self._h, C.byref(p), len(pulses), C.cast(arr, C.c_void_p), _ptr(psi),
-                                   _ptr(norms), norms.size, C.byref(n), C.byref(st)))
+                                   _ptr(norms), _ptr(fl) if flux else None, norms.size, C.byref(n),
+                                   C.byref(st)))
         stats = {f: getattr(st, f) for f, _ in SilStats._fields_}
+        if flux:
+            return psi, norms[: n.value], fl[: n.value], stats
         return psi, norms[: n.value], stats
 
     def frozen_core(self, frozen):

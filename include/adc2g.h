@@ -408,10 +408,23 @@ int32_t adc2g_matxvec_treal_laser(adc2g_handle* h, double time, int32_t npulse, 
  * in and out; the periods i = 1 .. int(tfinal / tout) each integrated by
  * steps with H at the step's start time (Func(Time, ...)), a period
  * skipped when |dpsi/dt| <= 1e-10; norms[i] = |psi| at i * tout for
- * i = 0 .. nrec (the wrstepinfo values).  norms holds nrec_cap values. */
+ * i = 0 .. nrec (the wrstepinfo values).  norms holds nrec_cap values.
+ * flux (nrec_cap values, or NULL) receives the CAP flux of adc2_flux_cap
+ * (flux.f90:179-411) at the same times; it needs a CAP and the Theta
+ * operator (adc2g_set_flux_theta). */
 int32_t adc2g_sil_laser(adc2g_handle* h, const adc2g_laser_params* p, int32_t npulse,
-                        const adc2g_pulse* pulses, double* psi, double* norms, int64_t nrec_cap,
-                        int64_t* nrec_out, adc2g_sil_stats* stats);
+                        const adc2g_pulse* pulses, double* psi, double* norms, double* flux,
+                        int64_t nrec_cap, int64_t* nrec_out, adc2g_sil_stats* stats);
+
+/* The intermediate-state representation of the projector Theta onto the
+ * absorbing region (SCRATCH/theta, the same layout as the dipoles), with
+ * theta00 and theta0j, for the flux of adc2_flux_cap (flux.f90:179-411):
+ *   flux = 2 Re <dpsi/dt| P Theta' P |psi> + 2 Re <psi| P W' P |psi>,
+ * where, as the reference writes them, Theta' and W' add theta00 (w00) times
+ * the operator's own IS output, (1 + theta00) Theta_IS u, rather than times u.
+ * Clearing as for adc2g_set_dipole_is. */
+int32_t adc2g_set_flux_theta(adc2g_handle* h, const double* diag, int64_t nnz, const int64_t* row,
+                             const int64_t* col, const double* val, double theta00, const double* theta0j);
 
 /* Chebyshev order-domain autocorrelation function (chebyshev_auto,
  * source/propagation/chebyspec.f90:245-319, with the scaled product of

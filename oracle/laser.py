@@ -20,12 +20,18 @@
   coefficients exp(Hs dt) e_0 through the eigen-decomposition of Hs / (-i)
   and a linear solve (eigchess, lineqz), the error the change of the
   coefficients between orders, the step reduced by (tol / former error)^(1/k).
+* ``flux`` follows adc2_flux_cap, source/propagation/flux.f90:179-411:
+  2 Re <dpsi/dt| P Theta' P |psi> + 2 Re <psi| P W' P |psi>, where Theta'
+  (W') on the intermediate states is (1 + theta00) Theta_IS u + theta0j u0 as
+  the reference writes it (oppsi = oppsi + theta00 * oppsi, :263 and :354)
+  and theta00 u0 + theta0j . u on the ground state.
 * ``propagate`` follows propagate_wavepacket_sil,
   source/propagation/proplib_adc2.f90:486-630 (SIL steps, oracle/sil.py
   silstep), or with a CAP propagate_wavepacket_csil, :659-830 (csilstep):
   periods of length tout, steps with H at each step's start time, a period
   skipped when |dpsi/dt| <= 1e-10, norms at the start of each period and at
-  the end.
+  the end; with Theta the flux at the same times (:766, :816-819), the last
+  one from dpsi/dt at the final time.
 
 Pinned by (tests/test_oracle_pins.py): the envelopes' half-maximum widths
 (the fwhm parameter of the cos^2 and sin^2 envelopes), ramp and box values;
@@ -33,7 +39,12 @@ a constant field (box envelope, zero frequency), where the propagation must
 equal exp(-i (H - E.D) t) psi0 of the (ndim + 1)-dimensional matrix from a
 dense exponential; norm conservation under a time-dependent field; with a
 CAP the constant-field propagation equals exp(-i (H - E.D - i P W P) t) psi0
-and the norm decays.
+and the norm decays; the flux with theta00 = w00 = 0 equals
+d/dt [<psi|P Theta P|psi> - <psi|psi>] by finite differences of that exact
+propagation, and with Theta = 2 (W = c) on the intermediate states and no
+off-diagonal terms it reduces to 2 (1 + theta00) d/dt |psi_IS|^2 +
+theta00 d/dt |psi_0|^2 (2 (1 + w00) c |psi_IS|^2 + 2 w00 |psi_0|^2), which
+fixes where theta00 and w00 enter.
 """
 from __future__ import annotations
 
@@ -129,6 +140,27 @@ def cap_matrix(cap, proj, nd):
     return P @ Wf @ P
 
 
+def _flux_op(op, proj, x):
+    """P Op' P x for op = (M dense IS block, m00, m0j) as adc2_flux_cap
+    applies it (flux.f90:257-272)."""
+    M, m00, m0j = op
+    nd = x.size - 1
+    u = _project(x, proj)
+    o = np.zeros_like(u)
+    o[:nd] = M @ u[:nd]
+    o[:nd] = o[:nd] + m00 * o[:nd]
+    o[nd] = m00 * u[nd] + np.dot(m0j, u[:nd])
+    o[:nd] += m0j * u[nd]
+    return _project(o, proj)
+
+
+def flux(theta, cap, proj, psi, dtpsi):
+    """The CAP flux; theta and cap as (M, m00, m0j)."""
+    val1 = 2.0 * np.vdot(dtpsi, _flux_op(theta, proj, psi)).real
+    val2 = 2.0 * np.vdot(psi, _flux_op(cap, proj, psi)).real
+    return float(val1 + val2)
+
+
 def csilstep(psi, dtpsi, func, intperiod, intorder, tolerror):
     """One CSILStep; returns (psi_new, stepsize, trueorder)."""
     RelativeMinStep, Tiny, MinOrder = 1.0e-9, 1.0e-18, 2
@@ -200,14 +232,16 @@ def full_matrix(H, dips, E):
     return M
 
 
-def propagate(hmul, dips, pulses, psi0, tout, tfinal, kdim, proptol, cap=None, proj=(None, False)):
-    """Returns (psi(tfinal), norms at i * tout, i = 0 .. int(tfinal / tout))."""
+def propagate(hmul, dips, pulses, psi0, tout, tfinal, kdim, proptol, cap=None, proj=(None, False), theta=None):
+    """Returns (psi(tfinal), norms at i * tout, i = 0 .. int(tfinal / tout)),
+    and the flux at the same times after the norms when theta is given."""
     stepper = sil.silstep if cap is None else csilstep
     tiny, tinier = 1e-9, 1e-10
     psi = np.array(psi0, dtype=np.complex128)
     intperiod = tout
     nrec = int(tfinal / intperiod)
     norms = np.zeros(nrec + 1)
+    fl = np.zeros(nrec + 1)
     time = 0.0
     for i in range(1, nrec + 1):
         inttime = 0.0
@@ -218,6 +252,8 @@ def propagate(hmul, dips, pulses, psi0, tout, tfinal, kdim, proptol, cap=None, p
             dtpsi = func(psi)
             if inttime == 0.0:
                 norms[i - 1] = np.sqrt(np.vdot(psi, psi).real)
+                if theta is not None:
+                    fl[i - 1] = flux(theta, cap, proj, psi, dtpsi)
             if np.sqrt(np.vdot(dtpsi, dtpsi).real) > tinier:
                 psi, truestep, _ = stepper(psi, dtpsi, func, step, kdim, proptol)
                 time += truestep
@@ -228,4 +264,8 @@ def propagate(hmul, dips, pulses, psi0, tout, tfinal, kdim, proptol, cap=None, p
                 time += intperiod
             break
     norms[nrec] = np.sqrt(np.vdot(psi, psi).real)
+    if theta is not None:
+        dtpsi = matxvec_treal_laser(hmul, dips, efield(pulses, time), psi, cap, proj)
+        fl[nrec] = flux(theta, cap, proj, psi, dtpsi)
+        return psi, norms, fl
     return psi, norms

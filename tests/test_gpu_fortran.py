@@ -411,6 +411,14 @@ def test_fortran_driver_laser_cap(tmp_path):
     iu = np.triu_indices(nd, 1)
     _write_is_operator(tmp_path / "cap", np.diag(W), iu[0], iu[1], W[iu], bs=1000)
     Wf[:nd, nd].astype("<f8").tofile(tmp_path / "w0j.bin")
+    trow = rng.integers(0, nd, 2 * nd)
+    tcol = (trow + 1 + rng.integers(0, nd - 1, trow.size)) % nd
+    tval = rng.normal(0.0, 0.1, trow.size)
+    tdiag = rng.uniform(0.0, 1.0, nd)
+    theta0j = rng.normal(0.0, 0.1, nd)
+    _write_is_operator(tmp_path / "theta", tdiag, trow, tcol, tval)
+    theta0j.astype("<f8").tofile(tmp_path / "theta0j.bin")
+    theta = (laser.dense_dipole(tdiag, trow, tcol, tval, nd), 0.15, theta0j)
     pulses = [{"envelope": "cos2", "envpar": (2.0, 2.0), "carrier": "cos", "freq": 0.7, "t0": 0.5, "phase": 0.2,
                "pulse_vec": (0.8, 0.0, 0.6), "strength": 0.2},
               {"envelope": "box", "envpar": (1.0, 3.0), "carrier": "sin", "freq": 1.3,
@@ -420,7 +428,8 @@ def test_fortran_driver_laser_cap(tmp_path):
            f"dipis_file='dip0','','dip2' d0j_file='d0j0.bin','','d0j2.bin' d00=0.3d0,0d0,-0.2d0 "
            f"cap_file='cap' w0j_file='w0j.bin' w00={w00!r} cap_ground=.true. npulse=2 "
            f"carrier=1,2 envelope=1,3 pulse_vec=0.8d0,0d0,0.6d0,0d0,0d0,1d0 strength=0.2d0,0.05d0 "
-           f"freq=0.7d0,1.3d0 t0=0.5d0,0d0 phase=0.2d0,0d0 envpar=2d0,2d0,1d0,3d0 psi_file='psi.bin' /\n")
+           f"freq=0.7d0,1.3d0 t0=0.5d0,0d0 phase=0.2d0,0d0 envpar=2d0,2d0,1d0,3d0 psi_file='psi.bin' "
+           f"theta_file='theta' theta0j_file='theta0j.bin' theta00=0.15d0 /\n")
     r = subprocess.run([DRIVER], input=nml, capture_output=True, text=True, cwd=tmp_path, timeout=120)
     print(r.stdout[-1500:])
     assert r.returncode == 0, r.stdout + r.stderr
@@ -428,12 +437,18 @@ def test_fortran_driver_laser_cap(tmp_path):
     psi = np.fromfile(tmp_path / "psi.bin", dtype="<c16")
     psi0 = np.zeros(nd + 1, dtype=complex)
     psi0[nd] = 1.0
-    ref_psi, ref_norms = laser.propagate(lambda z: H @ z, dips, pulses, psi0, 0.5, 4.0, 16, 1e-9,
-                                         (W, w00, Wf[:nd, nd]), (None, True))
+    ref_psi, ref_norms, ref_flux = laser.propagate(lambda z: H @ z, dips, pulses, psi0, 0.5, 4.0, 16, 1e-9,
+                                                   (W, w00, Wf[:nd, nd]), (None, True), theta)
     assert norms.size == ref_norms.size == 9
     assert np.abs(norms - ref_norms).max() < 2e-9     # printed with 9 decimals
     assert np.abs(psi - ref_psi).max() < 1e-8
     assert norms[-1] < 1.0 - 1e-6
+    # flux.dat: three '#' header lines, then time and flux (ES15.8)
+    rows = [x.split() for x in (tmp_path / "flux.dat").read_text().splitlines() if not x.startswith("#")]
+    t, flux = np.array([float(a) for a, _ in rows]), np.array([float(b) for _, b in rows])
+    assert np.allclose(t, 0.5 * np.arange(9))
+    assert np.abs(flux - ref_flux).max() < 1e-9 + 1e-8 * np.abs(ref_flux).max()
+    assert np.abs(flux).max() > 1e-6
 
 
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built")

@@ -195,6 +195,40 @@ def test_csil_laser_vs_oracle():
     assert norms[-1] < norms[0]
 
 
+def test_csil_flux_vs_oracle():
+    """The CAP flux (adc2_flux_cap) at every tout and at the end, with
+    theta00 and w00 nonzero and the bound-state projector, against the
+    oracle's propagation (flux pinned in tests/test_oracle_pins.py)."""
+    print("[case] csil flux o=3 v=5", flush=True)
+    sys_, H, dips, raw = _inputs(3, 5, 8, 13)
+    nd = H.shape[0]
+    cap, capraw, R = _cap(nd, 13)
+    rng = np.random.default_rng(14)
+    row = rng.integers(0, nd, 3 * nd)
+    col = rng.integers(0, nd, 3 * nd)
+    keep = row != col
+    row, col = row[keep], col[keep]
+    val = rng.normal(0.0, 0.1, row.size)
+    diag = rng.uniform(0.0, 1.0, nd)
+    theta0j = rng.normal(0.0, 0.1, nd)
+    theta00 = 0.25
+    theta = (laser.dense_dipole(diag, row, col, val, nd), theta00, theta0j)
+    psi0 = np.zeros(nd + 1, dtype=complex)
+    psi0[nd] = 1.0
+    with lib.Operator.from_system(sys_) as op:
+        _setup(op, raw, capraw, cap, R, False)
+        with pytest.raises(lib.Adc2gError):
+            op.sil_laser(psi0, PULSES, tout=0.5, tfinal=1.0, flux=True)   # no Theta yet
+        op.set_flux_theta(diag, row, col, val, theta00, theta0j)
+        psi, norms, fl, st = op.sil_laser(psi0, PULSES, tout=0.5, tfinal=6.0, kdim=16, proptol=1e-9, flux=True)
+    ref_psi, ref_norms, ref_fl = laser.propagate(lambda z: H @ z, dips, PULSES, psi0, 0.5, 6.0, 16, 1e-9, cap,
+                                                 (R, False), theta)
+    print(f"flux {fl[:4]} ... {fl[-1]:.3e}", flush=True)
+    assert np.abs(norms - ref_norms).max() < 1e-10
+    assert np.abs(fl - ref_fl).max() < 1e-8
+    assert np.abs(fl).max() > 1e-6
+
+
 def test_csil_constant_field_exact():
     """Constant field with a CAP: exp(-i (H - E.D - i P W P) t) psi0 of the
     dense (ndim + 1) matrix, independent of the oracle's propagator."""

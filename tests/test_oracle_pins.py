@@ -860,3 +860,77 @@ def test_oracle_csil_cap_constant_field_exact(ground):
     v = rng.normal(size=nd + 1) + 1j * rng.normal(size=nd + 1)
     got = laser.matxvec_treal_laser(lambda x: H @ x, dips, E, v, cap, proj)
     assert np.abs(got + 1j * (G @ v)).max() < 1e-12 * np.abs(G).max() * np.abs(v).max() * 10
+
+
+def _exact_path(G, psi0, t, h):
+    """psi(t - h), psi(t), psi(t + h) of psi' = -i G psi (dense exponential)."""
+    import scipy.linalg
+    return [scipy.linalg.expm(-1j * s * G) @ psi0 for s in (t - h, t, t + h)]
+
+
+@pytest.mark.parametrize("ground", [False, True])
+def test_oracle_flux_is_rate_of_change(ground):
+    """With theta00 = w00 = 0 the flux is d/dt [<psi|P Theta P|psi> -
+    <psi|psi>] along the exact constant-field propagation with the CAP:
+    2 Re <psi'|P Theta P|psi> is the rate of the population in the absorbing
+    region and 2 <psi|P W P|psi> the loss of norm.  Central differences of the
+    dense exponential (step 1e-4, error ~1e-9) check both terms, the
+    projector on both sides, the conjugated bra and the ground-state rows."""
+    from oracle import laser
+    sys_, H, dips, _ = _laser_inputs(seed=6)
+    nd = H.shape[0]
+    cap, Wf, R = _cap_inputs(nd, 7)
+    cap = (cap[0], 0.0, cap[2])
+    proj = (R, ground)
+    rng = np.random.default_rng(8)
+    T = rng.normal(0.0, 0.3, (nd, nd))
+    theta = (0.5 * (T + T.T), 0.0, rng.normal(0.0, 0.3, nd))
+    Tf = laser.cap_matrix(theta, proj, nd)
+    pulse = {"envelope": "box", "envpar": (0.0, 100.0), "carrier": "cos", "freq": 0.0,
+             "pulse_vec": (0.0, 0.6, 0.8), "strength": 0.2}
+    E = laser.efield([pulse], 1.0)
+    G = laser.full_matrix(H, dips, E) - 1j * laser.cap_matrix(cap, proj, nd)
+    psi0 = rng.normal(size=nd + 1) + 1j * rng.normal(size=nd + 1)
+    psi0 /= np.linalg.norm(psi0)
+    h = 1e-4
+    pm, p0, pp = _exact_path(G, psi0, 1.3, h)
+    pop = [np.vdot(x, Tf @ x).real - np.vdot(x, x).real for x in (pm, pp)]
+    want = (pop[1] - pop[0]) / (2.0 * h)
+    dtpsi = laser.matxvec_treal_laser(lambda x: H @ x, dips, E, p0, cap, proj)
+    got = laser.flux(theta, cap, proj, p0, dtpsi)
+    assert abs(got - want) < 1e-7 * max(1.0, abs(want))
+    loss = -(np.vdot(pp, pp).real - np.vdot(pm, pm).real) / (2.0 * h)
+    assert abs(laser.flux((0.0 * theta[0], 0.0, 0.0 * theta[2]), cap, proj, p0, dtpsi) - loss) < 1e-7
+    assert loss > 0.0
+
+
+def test_oracle_flux_theta00_w00_placement():
+    """Theta = 2 and W = c on the intermediate states, no off-diagonal or
+    ground-IS elements, no projector: the reference's (1 + theta00) Theta u
+    gives 2 (1 + theta00) d/dt |psi_IS|^2 + theta00 d/dt |psi_0|^2 for the
+    first term and 2 (1 + w00) c |psi_IS|^2 + 2 w00 |psi_0|^2 for the
+    second, which differ from (Theta + theta00) u when Theta != 1.  The
+    rates come from central differences of the exact propagation."""
+    from oracle import laser
+    sys_, H, dips, _ = _laser_inputs(seed=9)
+    nd = H.shape[0]
+    cap, Wf, R = _cap_inputs(nd, 9)
+    a, b, c = 0.3, 0.2, 0.7
+    theta = (2.0 * np.eye(nd), a, np.zeros(nd))
+    flux_cap = (c * np.eye(nd), b, np.zeros(nd))
+    pulse = {"envelope": "box", "envpar": (0.0, 100.0), "carrier": "cos", "freq": 0.0,
+             "pulse_vec": (0.48, 0.6, 0.64), "strength": 0.3}
+    E = laser.efield([pulse], 1.0)
+    G = laser.full_matrix(H, dips, E) - 1j * laser.cap_matrix(cap, (None, False), nd)
+    rng = np.random.default_rng(10)
+    psi0 = rng.normal(size=nd + 1) + 1j * rng.normal(size=nd + 1)
+    psi0 /= np.linalg.norm(psi0)
+    h = 1e-4
+    pm, p0, pp = _exact_path(G, psi0, 0.9, h)
+    rate_is = (np.linalg.norm(pp[:nd]) ** 2 - np.linalg.norm(pm[:nd]) ** 2) / (2.0 * h)
+    rate_0 = (abs(pp[nd]) ** 2 - abs(pm[nd]) ** 2) / (2.0 * h)
+    dtpsi = laser.matxvec_treal_laser(lambda x: H @ x, dips, E, p0, cap, (None, False))
+    want = (2.0 * (1.0 + a) * rate_is + a * rate_0
+            + 2.0 * (1.0 + b) * c * np.linalg.norm(p0[:nd]) ** 2 + 2.0 * b * abs(p0[nd]) ** 2)
+    got = laser.flux(theta, flux_cap, (None, False), p0, dtpsi)
+    assert abs(got - want) < 1e-7 * max(1.0, abs(want))
