@@ -269,12 +269,20 @@ int32_t adc2g_matxvec_treal_laser(adc2g_handle* h, double time, int32_t npulse, 
 }
 
 int32_t adc2g_sil_laser(adc2g_handle* h, const adc2g_laser_params* p, int32_t npulse, const adc2g_pulse* pulses,
-                        double* psi, double* norms, double* flux, int64_t nrec_cap, int64_t* nrec_out,
-                        adc2g_sil_stats* stats) {
+                        double* psi, double* norms, double* flux, double* trdens, int64_t nrec_cap,
+                        int64_t* nrec_out, adc2g_sil_stats* stats) {
   return guarded([&] {
     ARG_CHECK(h && p && psi && norms, "NULL argument");
     HIP_CHECK(hipSetDevice(h->h.device));
-    sil_laser_run(h->h, *p, npulse, pulses, psi, norms, flux, nrec_cap, nrec_out, stats);
+    sil_laser_run(h->h, *p, npulse, pulses, psi, norms, flux, trdens, nrec_cap, nrec_out, stats);
+  });
+}
+
+int32_t adc2g_trdens_vo(adc2g_handle* h, int32_t nvec, const double* x, int64_t ldx, double* trdens) {
+  return guarded([&] {
+    ARG_CHECK(h && x && trdens, "NULL argument");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    trdens_vo_run(h->h, nvec, x, ldx, trdens);
   });
 }
 

@@ -135,6 +135,9 @@ struct Handle {
   int capproj_n = 0;
   bool capproj_ground = false;
   DevBuf<double> laser_part, laser_u, laser_c, laser_coef, laser_vz;  // work arrays of the laser operator
+  // transition density (nto.hip): the n1 x n1 matrix M, the MP2 density and work space
+  bool nto_ready = false;
+  DevBuf<double> nto_M, nto_rho, nto_p;
   void set_dipole_is(int c, const double* diag, int64_t nnz, const int64_t* row, const int64_t* col,
                      const double* val, double d00, const double* d0j);
   void set_cap_is(const double* diag, int64_t nnz, const int64_t* row, const int64_t* col, const double* val,

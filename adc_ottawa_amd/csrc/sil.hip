@@ -547,7 +547,8 @@ void sil_run(Handle& H, const adc2g_sil_params& p, const double* fvec, double* a
 // skipped (time advances by the whole period); the norm is recorded at the
 // start of every period and at the end.
 void sil_laser_run(Handle& H, const adc2g_laser_params& p, int npulse, const adc2g_pulse* pulses, double* psi_io,
-                   double* norms, double* flux, int64_t nrec_cap, int64_t* nrec_out, adc2g_sil_stats* stats) {
+                   double* norms, double* flux, double* trdens, int64_t nrec_cap, int64_t* nrec_out,
+                   adc2g_sil_stats* stats) {
   const auto t0 = std::chrono::steady_clock::now();
   ARG_CHECK(p.tout > 0 && p.tfinal >= 0, "tout must be > 0 and tfinal >= 0");
   ARG_CHECK(p.kdim >= 1 && p.kdim <= 30, "krydim must be in 1..30");
@@ -555,6 +556,7 @@ void sil_laser_run(Handle& H, const adc2g_laser_params& p, int npulse, const adc
   if (p.kdim < 2) throw Error(ADC2G_ESIL_ORDER, "IntOrder is too small");
   laser_check_pulses(npulse, pulses);
   ARG_CHECK(!flux || (H.cap.set && H.flux_theta.set), "the flux needs a CAP and the Theta operator");
+  ARG_CHECK(!trdens || !H.parent, "the transition density belongs to the full operator, not to a view");
   const double intperiod = p.tout;
   const int64_t nrec = (int64_t)(p.tfinal / intperiod);
   ARG_CHECK(nrec_cap >= nrec + 1, "norms holds fewer than nrec+1 values");
@@ -580,6 +582,33 @@ void sil_laser_run(Handle& H, const adc2g_laser_params& p, int npulse, const adc
   const double tiny = 1e-9, tinier = 1e-10;
   int64_t nsteps = 0;
   int maxorder = 0;
+  // the TD-NTO input (tdadc2_nto, nto.f90:382-547): the occupied-virtual
+  // transition density of psi_IS / |psi_IS|, complex (re, im) per element
+  const int64_t n1 = H.d.n1;
+  DevBuf<double> tdev, tpart;
+  std::vector<double> thost;
+  if (trdens) {
+    tdev.alloc(2 * (size_t)n1);
+    tpart.alloc(2 * (size_t)reduce_blocks(H.d.ndim) + 2);
+    thost.resize(2 * (size_t)n1);
+  }
+  auto record_trdens = [&](int64_t rec) {
+    const double* x = S.Kp(0);
+    const int nbk = launch_cdot_partial(H.d.ndim, x, x, n, tpart.get(), H.st);
+    launch_colsum(nbk, 2, tpart.get(), tpart.get() + 2 * (size_t)reduce_blocks(H.d.ndim), H.st);
+    double nrm[2];
+    HIP_CHECK(hipMemcpyAsync(nrm, tpart.get() + 2 * (size_t)reduce_blocks(H.d.ndim), sizeof(nrm),
+                             hipMemcpyDeviceToHost, H.st));
+    trdens_vo_device(H, 2, x, n, tdev.get());
+    HIP_CHECK(hipMemcpyAsync(thost.data(), tdev.get(), sizeof(double) * 2 * n1, hipMemcpyDeviceToHost, H.st));
+    HIP_CHECK(hipStreamSynchronize(H.st));
+    const double inv = nrm[0] > 0.0 ? 1.0 / std::sqrt(nrm[0]) : 0.0;
+    double* out = trdens + rec * 2 * n1;
+    for (int64_t r = 0; r < n1; ++r) {
+      out[2 * r] = thost[r] * inv;
+      out[2 * r + 1] = thost[n1 + r] * inv;
+    }
+  };
   for (int64_t i = 1; i <= nrec; ++i) {
     double inttime = 0.0;
     while (true) {
@@ -588,6 +617,7 @@ void sil_laser_run(Handle& H, const adc2g_laser_params& p, int npulse, const adc
       if (inttime == 0.0) {
         norms[i - 1] = std::sqrt(S.cdot(S.Kp(0), S.Kp(0)).real());
         if (flux) flux[i - 1] = laser_flux(H, S.Kp(0), S.Kp(1));
+        if (trdens) record_trdens(i - 1);
       }
       if (std::sqrt(S.cdot(S.Kp(1), S.Kp(1)).real()) > tinier) {
         int order = 0;
@@ -613,6 +643,7 @@ void sil_laser_run(Handle& H, const adc2g_laser_params& p, int npulse, const adc
     S.func(S.Kp(0), S.Kp(1));
     flux[nrec] = laser_flux(H, S.Kp(0), S.Kp(1));
   }
+  if (trdens) record_trdens(nrec);
   HIP_CHECK(hipMemcpyAsync(planar.data(), S.Kp(0), sizeof(double) * 2 * n, hipMemcpyDeviceToHost, H.st));
   HIP_CHECK(hipStreamSynchronize(H.st));
   for (int64_t i = 0; i < n; ++i) {

@@ -38,7 +38,8 @@ void laser_apply(Handle& H, const double E[3], const double* v, double* hv, doub
 void matxvec_treal_laser_host(Handle& H, double t, int npulse, const adc2g_pulse* pulses, const double* v1,
                               double* v2);
 void sil_laser_run(Handle& H, const adc2g_laser_params& p, int npulse, const adc2g_pulse* pulses, double* psi,
-                   double* norms, double* flux, int64_t nrec_cap, int64_t* nrec_out, adc2g_sil_stats* stats);
+                   double* norms, double* flux, double* trdens, int64_t nrec_cap, int64_t* nrec_out,
+                   adc2g_sil_stats* stats);
 // the CAP flux of adc2_flux_cap for planar psi and dpsi/dt (device, ndim + 1 rows)
 double laser_flux(Handle& H, const double* psi, const double* dtpsi);
 
@@ -66,6 +67,11 @@ void hmlt_write(Handle& H, const char* dia_path, const char* off_path, double mi
 void mp2_run(Handle& H, double* e_mp2, double* d2, double* rho);
 // ADC(2) F-vector of the nbas x nbas column-major operator matrix dpl (host) into f (host, ndim)
 void fvector_run(Handle& H, const double* dpl, double* f);
+// occupied-virtual transition density (v x o column-major per vector, at s*n1)
+// of nvec vectors whose 1h1p rows are x[s*ldx + r], r < n1 (device), and of
+// host vectors (ldx >= ndim)
+void trdens_vo_device(Handle& H, int nvec, const double* x, int64_t ldx, double* out);
+void trdens_vo_run(Handle& H, int nvec, const double* x_host, int64_t ldx, double* out_host);
 
 // Sub-space diagonalisation guess: the lowest bs eigenvectors of the stored
 // matrix on the subdim ISs of lowest (descending: highest) H_jj, into V

@@ -55,7 +55,9 @@
 !            state (initwf_exci_initstate); cap_nproj = n projects the n
 !            lowest Davidson states out of the CAP (iprojcap); theta_file
 !            (SCRATCH/theta) with theta0j_file and theta00 gives the CAP flux
-!            in 'flux.dat' (lflux).  Prints Time / Norm at every tout
+!            in 'flux.dat' (lflux); trdens_file receives the TD-NTO input
+!            (lnto: the transition density at every tout, nvirt x nocc
+!            complex per record).  Prints Time / Norm at every tout
 !            (wrstepinfo) and writes the final wavefunction (ndim + 1 complex,
 !            ground state last) to psi_file.
 !=======================================================================
@@ -84,7 +86,7 @@ program adc2g_driver
   real(c_double) :: t0(maxpulse) = 0.0d0, phase(maxpulse) = 0.0d0, envpar(2, maxpulse) = 0.0d0
   real(c_double) :: d00(3) = 0.0d0, w00 = 0.0d0, proptol = 1.0d-8
   character(len=256) :: dipis_file(3) = '', d0j_file(3) = '', cap_file = '', w0j_file = '', psi_file = 'psi.bin'
-  character(len=256) :: theta_file = '', theta0j_file = ''
+  character(len=256) :: theta_file = '', theta0j_file = '', trdens_file = ''
   real(c_double) :: theta00 = 0.0d0
   namelist /adc2g/ nocc, nvirt, naux, eps_file, b_file, nstates, blocksize, maxvec, maxiter, tol, &
        precon, precision, deflate, davname, sil, fvec_file, auto_file, tout, tfinal, krydim, autotol, &
@@ -92,7 +94,7 @@ program adc2g_driver
        chebyord, cheby_emin, cheby_emax, hmlt, dpl_file, lancguess, orthotype, integrator, siltol, &
        ncore, core, ehf, nfrozen, adc2x, laser, cap_ground, npulse, carrier, envelope, pulse_vec, &
        strength, freq, t0, phase, envpar, d00, w00, proptol, dipis_file, d0j_file, cap_file, w0j_file, psi_file, &
-       statenumber, cap_nproj, theta_file, theta0j_file, theta00
+       statenumber, cap_nproj, theta_file, theta0j_file, theta00, trdens_file
 
   type(c_ptr) :: h, hfull
   type(adc2g_opts) :: opts
@@ -304,6 +306,8 @@ contains
     type(adc2g_pulse) :: pulses(maxpulse)
     real(c_double), allocatable :: odiag(:), oval(:), vec0(:), norms(:)
     real(c_double), allocatable, target :: fluxv(:)
+    complex(c_double_complex), allocatable, target :: trd(:,:,:)
+    type(c_ptr) :: fptr, tptr
     integer(c_int64_t), allocatable :: orow(:), ocol(:)
     complex(c_double_complex), allocatable :: psi(:)
     real(c_double) :: dummy(1)
@@ -369,6 +373,8 @@ contains
        stop 1
     end if
     allocate(fluxv(size(norms)))
+    fptr = c_null_ptr
+    tptr = c_null_ptr
     if (len_trim(theta_file) > 0) then
        ! lflux: the projector onto the absorbing region (SCRATCH/theta)
        call read_is_operator(trim(theta_file), ndim, odiag, orow, ocol, oval, onnz)
@@ -376,11 +382,20 @@ contains
        read(u) vec0
        close(u)
        call check(adc2g_set_flux_theta(h, odiag, onnz, orow, ocol, oval, theta00, vec0))
-       call check(adc2g_sil_laser(h, lpar, int(npulse, c_int32_t), pulses, psi, norms, c_loc(fluxv), &
-            int(size(norms), c_int64_t), nout, ss))
-    else
-       call check(adc2g_sil_laser(h, lpar, int(npulse, c_int32_t), pulses, psi, norms, c_null_ptr, &
-            int(size(norms), c_int64_t), nout, ss))
+       fptr = c_loc(fluxv)
+    end if
+    if (len_trim(trdens_file) > 0) then
+       allocate(trd(nvirt, nocc, size(norms)))
+       tptr = c_loc(trd)
+    end if
+    call check(adc2g_sil_laser(h, lpar, int(npulse, c_int32_t), pulses, psi, norms, fptr, tptr, &
+         int(size(norms), c_int64_t), nout, ss))
+    if (len_trim(trdens_file) > 0) then
+       ! lnto: the transition densities of tdadc2_nto at every tout (records
+       ! 0 .. nout - 1, each nvirt x nocc complex), for the NTO analysis
+       open(newunit=u, file=trim(trdens_file), access='stream', form='unformatted', status='replace')
+       write(u) trd(:, :, 1:nout)
+       close(u)
     end if
     do i = 1, int(nout)
        write(*, '(a,f14.6,a,f12.9)') ' Time:', (i - 1) * tout, '   Norm:', norms(i)

@@ -338,9 +338,10 @@ module adc2g_mod
        complex(c_double_complex), intent(in) :: v1(*)
        complex(c_double_complex), intent(out) :: v2(*)
      end function adc2g_matxvec_treal_laser
-     ! flux: c_loc of nrec_cap doubles, or c_null_ptr
-     integer(c_int32_t) function adc2g_sil_laser(h, p, npulse, pulses, psi, norms, flux, nrec_cap, nrec_out, &
-          stats) bind(c, name='adc2g_sil_laser')
+     ! flux: c_loc of nrec_cap doubles, or c_null_ptr; trdens: c_loc of
+     ! nrec_cap * nvirt * nocc complex values (the TD-NTO input), or c_null_ptr
+     integer(c_int32_t) function adc2g_sil_laser(h, p, npulse, pulses, psi, norms, flux, trdens, nrec_cap, &
+          nrec_out, stats) bind(c, name='adc2g_sil_laser')
        import :: c_int32_t, c_int64_t, c_double, c_double_complex, c_ptr, adc2g_pulse, adc2g_laser_params, &
             adc2g_sil_stats
        type(c_ptr), value :: h
@@ -349,7 +350,7 @@ module adc2g_mod
        type(adc2g_pulse), intent(in) :: pulses(*)
        complex(c_double_complex), intent(inout) :: psi(*)
        real(c_double), intent(out) :: norms(*)
-       type(c_ptr), value :: flux
+       type(c_ptr), value :: flux, trdens
        integer(c_int64_t), value :: nrec_cap
        integer(c_int64_t), intent(out) :: nrec_out
        type(adc2g_sil_stats), intent(out) :: stats
@@ -365,6 +366,15 @@ module adc2g_mod
        real(c_double), value :: theta00
        real(c_double), intent(in) :: theta0j(*)
      end function adc2g_set_flux_theta
+     ! trdens(nvirt, nocc, nvec): adc2_trdens_gs_virt_occ of x(ldx, nvec)
+     integer(c_int32_t) function adc2g_trdens_vo(h, nvec, x, ldx, trdens) bind(c, name='adc2g_trdens_vo')
+       import :: c_int32_t, c_int64_t, c_double, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: nvec
+       real(c_double), intent(in) :: x(*)
+       integer(c_int64_t), value :: ldx
+       real(c_double), intent(out) :: trdens(*)
+     end function adc2g_trdens_vo
 
      ! MP2 energy, D2 diagnostic and density (rho: c_loc of nbas x nbas or c_null_ptr)
      integer(c_int32_t) function adc2g_mp2(h, e_mp2, d2, rho) bind(c, name='adc2g_mp2')

@@ -132,6 +132,29 @@ def read_is_operator(path, ndim):
     return diag, cat(rows, np.int64), cat(cols, np.int64), cat(vals, np.float64)
 
 
+def nto(trdens, c_occ=None, c_virt=None, thresh=1e-3):
+    """Natural transition orbitals of an occupied-virtual transition density
+    trdens[a, i] (nvirt x nocc, real or complex), as adc2_nto_gs and
+    tdadc2_nto (nto.f90:183-236, 486-531) form them: the SVD
+    trdens^T = U diag(sigma) V^H of the nocc x nvirt matrix (nocc <= nvirt),
+    the pairs with sigma^2 >= thresh kept.  Returns (sigma (all nocc values),
+    hole (nocc x npair, or nao x npair with the occupied MO coefficients
+    c_occ), particle (nvirt x npair, or nao x npair with c_virt))."""
+    T = np.asarray(trdens)
+    nv, no = T.shape
+    if no > nv:
+        raise ValueError("the NTOs need nocc <= nvirt")
+    U, sigma, Vh = np.linalg.svd(T.T, full_matrices=True)
+    npair = int(np.count_nonzero(sigma ** 2 >= thresh))
+    hole = U[:, :npair]
+    particle = Vh.conj().T[:, :npair]
+    if c_occ is not None:
+        hole = np.asarray(c_occ) @ hole
+    if c_virt is not None:
+        particle = np.asarray(c_virt) @ particle
+    return sigma, hole, particle
+
+
 class LanczosParams(C.Structure):
     _fields_ = [("blocksize", C.c_int32), ("ncycles", C.c_int32), ("precision", C.c_int32),
                 ("check_orth", C.c_int32), ("orthotype", C.c_int32), ("reserved", C.c_int32)]
@@ -167,7 +190,7 @@ SYMBOLS = [
     "adc2g_sil_autocorr_orders", "adc2g_create_cvs", "adc2g_create_frozen_core", "adc2g_set_adc2x", "adc2g_mp2", "adc2g_fvector", "adc2g_lanczos_guess",
     "adc2g_spectral_bounds", "adc2g_power", "adc2g_coupled_schur", "adc2g_sigma_device_noalloc",
     "adc2g_set_dipole_is", "adc2g_matxvec_treal_laser", "adc2g_sil_laser", "adc2g_set_cap_is",
-    "adc2g_set_cap_projector", "adc2g_set_flux_theta",
+    "adc2g_set_cap_projector", "adc2g_set_flux_theta", "adc2g_trdens_vo",
 ]
 
 _lib = None
@@ -229,8 +252,9 @@ def load(path: str = LIBPATH):
     lib.adc2g_matxvec_treal_laser.argtypes = [vp, C.c_double, i32, vp, vp, vp]
     lib.adc2g_set_cap_is.argtypes = [vp, vp, i64, vp, vp, vp, C.c_double, vp]
     lib.adc2g_set_cap_projector.argtypes = [vp, i32, vp, i32]
-    lib.adc2g_sil_laser.argtypes = [vp, C.POINTER(LaserParams), i32, vp, vp, vp, vp, i64, C.POINTER(i64),
+    lib.adc2g_sil_laser.argtypes = [vp, C.POINTER(LaserParams), i32, vp, vp, vp, vp, vp, i64, C.POINTER(i64),
                                     C.POINTER(SilStats)]
+    lib.adc2g_trdens_vo.argtypes = [vp, i32, vp, i64, vp]
     lib.adc2g_set_flux_theta.argtypes = [vp, vp, i64, vp, vp, vp, C.c_double, vp]
     for s in SYMBOLS:
         getattr(lib, s).restype = C.c_int32
@@ -586,27 +610,49 @@ class Operator:
                                         float(theta00), _ptr(theta0j)))
 
     def sil_laser(self, psi0, pulses, tout, tfinal, kdim=20, proptol=1e-8, precision="fp64", verbose=False,
-                  flux=False):
+                  flux=False, trdens=False):
         """SIL propagation of psi0 (ndim + 1 complex components) under the
-        pulses; returns (psi(tfinal), norms at i * tout, stats), and the CAP
-        flux at the same times after the norms when flux is set."""
+        pulses; returns (psi(tfinal), norms at i * tout, stats), with after
+        the norms the CAP flux at the same times when flux is set and the
+        transition densities of psi_IS / |psi_IS| (records x nvirt x nocc,
+        complex; the TD-NTO input) when trdens is set."""
         psi = np.array(psi0, dtype=np.complex128)
         if psi.shape != (self.ndim + 1,):
             raise ValueError("psi0 must have ndim + 1 components")
         nrec = int(tfinal / tout)
         norms = np.empty(nrec + 1)
         fl = np.empty(nrec + 1) if flux else None
+        n1 = self.nocc * self.nvirt
+        td = np.empty((nrec + 1) * n1, dtype=np.complex128) if trdens else None
         p = LaserParams(tout, tfinal, kdim, PREC[precision], proptol, 1 if verbose else 0, 0)
         arr = pulses_array(pulses)
         n = C.c_int64()
         st = SilStats()
         check(_lib.adc2g_sil_laser(self._h, C.byref(p), len(pulses), C.cast(arr, C.c_void_p), _ptr(psi),
-                                   _ptr(norms), _ptr(fl) if flux else None, norms.size, C.byref(n),
-                                   C.byref(st)))
+                                   _ptr(norms), _ptr(fl) if flux else None, _ptr(td) if trdens else None,
+                                   norms.size, C.byref(n), C.byref(st)))
         stats = {f: getattr(st, f) for f, _ in SilStats._fields_}
+        out = [psi, norms[: n.value]]
         if flux:
-            return psi, norms[: n.value], fl[: n.value], stats
-        return psi, norms[: n.value], stats
+            out.append(fl[: n.value])
+        if trdens:
+            out.append(td.reshape(nrec + 1, self.nocc, self.nvirt).transpose(0, 2, 1)[: n.value].copy())
+        return tuple(out) + (stats,)
+
+    def trdens_vo(self, x):
+        """The occupied-virtual ground-to-excited transition density
+        trdens[a, i] (nvirt x nocc) of adc2_trdens_gs_virt_occ for x of
+        ndim components, or (nvec, nvirt, nocc) for x of shape (ndim, nvec)."""
+        x = np.asarray(x, dtype=np.float64)
+        one = x.ndim == 1
+        X = np.asfortranarray(x.reshape(self.ndim, -1))
+        if X.shape[0] != self.ndim:
+            raise ValueError("x must have ndim rows")
+        k = X.shape[1]
+        out = np.empty(k * self.nocc * self.nvirt)
+        check(_lib.adc2g_trdens_vo(self._h, k, _ptr(X), self.ndim, _ptr(out)))
+        T = out.reshape(k, self.nocc, self.nvirt).transpose(0, 2, 1)
+        return T[0].copy() if one else T.copy()
 
     def frozen_core(self, frozen):
         """Frozen-core view: the configurations with no hole in the orbitals

@@ -411,10 +411,26 @@ int32_t adc2g_matxvec_treal_laser(adc2g_handle* h, double time, int32_t npulse, 
  * i = 0 .. nrec (the wrstepinfo values).  norms holds nrec_cap values.
  * flux (nrec_cap values, or NULL) receives the CAP flux of adc2_flux_cap
  * (flux.f90:179-411) at the same times; it needs a CAP and the Theta
- * operator (adc2g_set_flux_theta). */
+ * operator (adc2g_set_flux_theta).  trdens (nrec_cap * 2 * nocc * nvirt
+ * doubles, or NULL) receives at the same times the input of the TD-NTO
+ * analysis of tdadc2_nto (nto.f90:382-547): the occupied-virtual transition
+ * density (adc2g_trdens_vo) of psi_IS / |psi_IS|, complex (re, im) per
+ * element, element (a, i) at a + nvirt * i of each record (zero when psi_IS
+ * is zero).  The reference analyses records i >= 1 with |psi_IS| > 1e-4. */
 int32_t adc2g_sil_laser(adc2g_handle* h, const adc2g_laser_params* p, int32_t npulse,
                         const adc2g_pulse* pulses, double* psi, double* norms, double* flux,
-                        int64_t nrec_cap, int64_t* nrec_out, adc2g_sil_stats* stats);
+                        double* trdens, int64_t nrec_cap, int64_t* nrec_out, adc2g_sil_stats* stats);
+
+/* The occupied-virtual block of the ADC(2) ground-to-excited-state
+ * transition density of adc2_trdens_gs_virt_occ (density_matrix.f90:497-636)
+ * for nvec vectors x (ndim rows each, leading dimension ldx >= ndim; only the
+ * 1h1p rows enter) into trdens (nvirt x nocc column-major per vector, vector s
+ * at s * nocc * nvirt): the trdens(nocc+1:nbas, 1:nocc) that adc2_nto_gs and
+ * tdadc2_nto (nto.f90:76-260, 382-547) decompose into natural transition
+ * orbitals.  Uses the MP2 density (as adc2g_mp2, so the (vv|ov) block must be
+ * in memory) and keeps an n1 x n1 FP64 matrix on the device after the first
+ * call (n1 = nocc * nvirt).  Not available on a CVS or frozen-core view. */
+int32_t adc2g_trdens_vo(adc2g_handle* h, int32_t nvec, const double* x, int64_t ldx, double* trdens);
 
 /* The intermediate-state representation of the projector Theta onto the
  * absorbing region (SCRATCH/theta, the same layout as the dipoles), with

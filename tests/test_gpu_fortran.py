@@ -429,7 +429,7 @@ def test_fortran_driver_laser_cap(tmp_path):
            f"cap_file='cap' w0j_file='w0j.bin' w00={w00!r} cap_ground=.true. npulse=2 "
            f"carrier=1,2 envelope=1,3 pulse_vec=0.8d0,0d0,0.6d0,0d0,0d0,1d0 strength=0.2d0,0.05d0 "
            f"freq=0.7d0,1.3d0 t0=0.5d0,0d0 phase=0.2d0,0d0 envpar=2d0,2d0,1d0,3d0 psi_file='psi.bin' "
-           f"theta_file='theta' theta0j_file='theta0j.bin' theta00=0.15d0 /\n")
+           f"theta_file='theta' theta0j_file='theta0j.bin' theta00=0.15d0 trdens_file='trdens.bin' /\n")
     r = subprocess.run([DRIVER], input=nml, capture_output=True, text=True, cwd=tmp_path, timeout=120)
     print(r.stdout[-1500:])
     assert r.returncode == 0, r.stdout + r.stderr
@@ -449,6 +449,13 @@ def test_fortran_driver_laser_cap(tmp_path):
     assert np.allclose(t, 0.5 * np.arange(9))
     assert np.abs(flux - ref_flux).max() < 1e-9 + 1e-8 * np.abs(ref_flux).max()
     assert np.abs(flux).max() > 1e-6
+    # trdens_file: the TD-NTO input (nvirt x nocc complex per record); the
+    # last record is the density of the final wavefunction
+    from oracle import nto
+    trd = np.fromfile(tmp_path / "trdens.bin", dtype="<c16").reshape(9, o, v).transpose(0, 2, 1)
+    Tr = nto.tdnto(sys_, ref_psi[:nd])[0]
+    assert np.abs(trd[-1] - Tr).max() < 1e-8 * np.abs(Tr).max()
+    assert np.abs(trd[0]).max() == 0.0                 # psi_IS = 0 at t = 0
 
 
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built")

@@ -934,3 +934,65 @@ def test_oracle_flux_theta00_w00_placement():
             + 2.0 * (1.0 + b) * c * np.linalg.norm(p0[:nd]) ** 2 + 2.0 * b * abs(p0[nd]) ** 2)
     got = laser.flux(theta, flux_cap, (None, False), p0, dtpsi)
     assert abs(got - want) < 1e-7 * max(1.0, abs(want))
+
+
+@pytest.mark.parametrize("o,v,naux,seed", [(2, 3, 5, 1), (3, 4, 6, 2)])
+def test_oracle_trdens_is_fvector_transpose(o, v, naux, seed):
+    """The transition density of adc2_trdens_gs_virt_occ is the transpose of
+    the F-vector's dpl_vo terms: with rho replaced by the F-vector's own MP2
+    intermediates, trdens_vo(x)[a, i] = x . f(E_ai) for every (a, i), where
+    f is the 1h1p block of the F-vector oracle (pinned against the exact
+    intermediate-state representation) of the operator whose only element is
+    dpl(a, i) = 1.  A dropped or doubled term, a sign, a transposed
+    integral or a denominator of the wrong pair fails it."""
+    from oracle import fvec, mp2, nto
+    sys = make_system(o, v, naux, seed=seed)
+    V = mp2.integrals(sys)
+    nb = o + v
+    terms = fvec.literal_terms(sys, np.zeros((nb, nb)))
+    rho = np.zeros((nb, nb))
+    rho[o:, o:] = -4.0 * terms["tauA"].T
+    rho[:o, :o] = 4.0 * terms["tauB"].T
+    rng = np.random.default_rng(seed)
+    xs = rng.normal(size=(3, o * v))
+    got = np.stack([nto.trdens_vo(sys, x, rho, V) for x in xs])
+    want = np.zeros_like(got)
+    for a in range(v):
+        for i in range(o):
+            dpl = np.zeros((nb, nb))
+            dpl[o + a, i] = 1.0
+            f1 = fvec.fvector(sys, dpl)[: o * v]
+            want[:, a, i] = xs @ f1
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+    # with rho_mp2 as the reference assembles it only the MP2-density terms change
+    full = nto.trdens_vo(sys, xs[0], None, V)
+    R = mp2.density(sys, V)[0] - rho
+    X = xs[0].reshape(v, o)
+    extra = 0.25 * np.sqrt(2.0) * (R[o:, o:] @ X - X @ R[:o, :o].T)
+    assert np.abs(full - got[0] - extra).max() < 1e-12 * np.abs(full).max()
+
+
+def test_oracle_tdnto():
+    """tdadc2_nto: the SVD reconstructs the normalised complex density
+    (U diag(sigma) V^H = trdens^T), the density is that of the normalised IS
+    part (scaling wf leaves it unchanged), and the pair count follows
+    sigma^2 >= 0.001; a pure 1h1p zeroth-order-dominated state has one
+    dominant pair."""
+    from oracle import nto
+    sys = make_system(3, 5, 6, seed=7)
+    o, v = sys.nocc, sys.nvirt
+    rng = np.random.default_rng(7)
+    nd = o * v + 10
+    wf = rng.normal(size=nd) + 1j * rng.normal(size=nd)
+    T, sigma, U, Vm, npair = nto.tdnto(sys, wf)
+    S = np.zeros((o, v))
+    S[:o, :o] = np.diag(sigma)
+    assert np.abs(U @ S @ Vm.conj().T - T.T).max() < 1e-12
+    T2 = nto.tdnto(sys, 3.7j * wf)[0]
+    assert np.abs(T2 - 1j * T).max() < 1e-12          # the phase of wf carries through
+    assert npair == int(np.sum(sigma ** 2 >= 1e-3))
+    e = np.zeros(nd, dtype=complex)
+    e[2 * o + 1] = 1.0                                 # configuration (a=2, i=1)
+    T1, s1, U1, V1, n1 = nto.tdnto(sys, e)
+    assert s1[0] > 0.9 * np.sqrt(2.0) and s1[1] < 0.2 * s1[0]
+    assert abs(abs(U1[1, 0]) - 1.0) < 0.05 and abs(abs(V1[2, 0]) - 1.0) < 0.05
