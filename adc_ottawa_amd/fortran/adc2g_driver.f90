@@ -36,6 +36,9 @@
 !            every later step uses the configurations with no hole in them;
 !   adc2x = .true.: ADC(2)-x (method = adc2-x): the first-order 2h2p-2h2p
 !            block in the matrix (set before the CVS / frozen-core view);
+!   nto_file = 'trd.bin': the occupied-virtual transition density of every
+!            state (nvirt x nocc doubles per state, adc2_trdens_gs_virt_occ),
+!            the input of the NTOs of adc2_nto_gs (full space only);
 !   dpl_file = 'd.bin': the F-vector of this nbas x nbas column-major operator
 !            matrix (get_modifiedtm_adc2), computed by the library, gives the
 !            transition moments and oscillator strengths of the states
@@ -77,6 +80,7 @@ program adc2g_driver
   real(c_double) :: stepsize = 10.0d0, cheby_emin = 0.0d0, cheby_emax = 1.0d0, siltol = 1.0d-5, ehf = 0.0d0
   character(len=256) :: eps_file = 'eps.bin', b_file = 'B.bin', davname = 'davstates'
   character(len=256) :: fvec_file = 'fvec.bin', auto_file = 'auto', lancname = 'lancstates'
+  character(len=256) :: nto_file = ''
   character(len=256) :: dpl_file = ''
   ! laser / CAP propagation
   integer, parameter :: maxpulse = 8
@@ -91,7 +95,7 @@ program adc2g_driver
   namelist /adc2g/ nocc, nvirt, naux, eps_file, b_file, nstates, blocksize, maxvec, maxiter, tol, &
        precon, precision, deflate, davname, sil, fvec_file, auto_file, tout, tfinal, krydim, autotol, &
        solver, guess, subdim, stepsize, rlx_krydim, autoord, lanczos, lmain, ncycles, lancname, &
-       chebyord, cheby_emin, cheby_emax, hmlt, dpl_file, lancguess, orthotype, integrator, siltol, &
+       chebyord, cheby_emin, cheby_emax, hmlt, dpl_file, nto_file, lancguess, orthotype, integrator, siltol, &
        ncore, core, ehf, nfrozen, adc2x, laser, cap_ground, npulse, carrier, envelope, pulse_vec, &
        strength, freq, t0, phase, envpar, d00, w00, proptol, dipis_file, d0j_file, cap_file, w0j_file, psi_file, &
        statenumber, cap_nproj, theta_file, theta0j_file, theta00, trdens_file
@@ -106,7 +110,7 @@ program adc2g_driver
   type(adc2g_relax_stats) :: rs
   type(adc2g_lanczos_params) :: lp
   type(adc2g_lanczos_stats) :: ls
-  real(c_double), allocatable :: eps(:), B(:), eigval(:), fvec(:), q0(:,:), cauto(:), dpl(:,:), tmv(:), osc(:)
+  real(c_double), allocatable :: eps(:), B(:), eigval(:), fvec(:), q0(:,:), cauto(:), dpl(:,:), tmv(:), osc(:), trdvo(:,:,:)
   real(c_double) :: emp2 = 0.0d0, d2mp = 0.0d0
   real(c_double), allocatable, target :: eigvec(:,:), lener(:), ritz(:,:), tmom(:), fvec_t(:)
   complex(c_double_complex), allocatable :: auto0(:)
@@ -213,6 +217,16 @@ program adc2g_driver
      end do
   end if
   call adc2g_write_davstates(trim(davname), ndim, nstates, eigval, eigvec)
+  if (len_trim(nto_file) > 0) then
+     ! lnto (adc2_nto_gs, nto.f90:76-260): the occupied-virtual transition
+     ! density of every state (nvirt x nocc per state), for its SVD
+     allocate(trdvo(nvirt, nocc, nstates))
+     call check(adc2g_trdens_vo(h, int(nstates, c_int32_t), eigvec, int(ndim, c_int64_t), trdvo))
+     open(newunit=u, file=trim(nto_file), access='stream', form='unformatted', status='replace')
+     write(u) trdvo
+     close(u)
+     write(*, '(a,a)') ' NTO transition densities written to ', trim(nto_file)
+  end if
   if (ncore == 0 .and. nfrozen == 0) then
      ! table2 / wrstateinfo (misc.f90:648-852): 'davstates.dat' (full space)
      call check(adc2g_mp2(h, emp2, d2mp, c_null_ptr))

@@ -48,7 +48,7 @@ def adc1_guess(system, kind: str, orbitals, ndim: int, nblock: int) -> np.ndarra
 
 
 def run(log, frozen, roots, cvs=False, block=0, maxvec=0, tol=1e-7, precon="olsen", guess="adc1", deflate=False,
-        verbose=False, adc2x=False):
+        verbose=False, adc2x=False, nto=False):
     t0 = time.perf_counter()
     mol = gamess.read_gamess_log(log)
     ints = gamess.ao_integrals(mol)
@@ -72,8 +72,20 @@ def run(log, frozen, roots, cvs=False, block=0, maxvec=0, tol=1e-7, precon="olse
         target = view if view is not None else op
         try:
             g = adc1_guess(full, kind, orbs, target.ndim, block) if guess == "adc1" else guess
-            ev, _, st = target.davidson(roots, blocksize=block, maxvec=maxvec, tol=tol, precon=precon,
-                                        deflate=deflate, maxiter=1000, vectors=False, verbose=verbose, guess=g)
+            if nto and view is not None:
+                raise ValueError("the NTOs are computed in the full configuration space (no --frozen or --cvs)")
+            ev, vecs, st = target.davidson(roots, blocksize=block, maxvec=maxvec, tol=tol, precon=precon,
+                                           deflate=deflate, maxiter=1000, vectors=nto, verbose=verbose, guess=g)
+            if nto:
+                # adc2_nto_gs (nto.f90:76-260): the SVD of each state's
+                # occupied-virtual transition density, pairs with sigma^2 >= 0.01;
+                # hole and particle orbitals in the AO basis
+                T = op.trdens_vo(vecs)
+                out["nto"] = []
+                for s_ in range(T.shape[0]):
+                    sig, hole, part = lib.nto(T[s_], hf.C[:, :hf.nocc], hf.C[:, hf.nocc:], thresh=0.01)
+                    out["nto"].append({"sigma2": [float(x) ** 2 for x in sig[: hole.shape[1]]],
+                                       "hole_ao": hole.tolist(), "particle_ao": part.tolist()})
             out["space"] = {"cvs": "cvs", "frozen": "frozen core", None: "full"}[kind]
             out["ndim"] = target.ndim
             out["counts"] = [target.n1] + list(target.counts)

@@ -48,7 +48,7 @@ def test_fortran_driver(tmp_path):
     f.astype("<f8").tofile(tmp_path / "f.bin")
     nml = (f"&adc2g nocc={o} nvirt={v} naux={naux} eps_file='eps.bin' b_file='B.bin' nstates=3 "
            f"blocksize=6 maxvec=30 tol=1d-9 precon=1 davname='davstates' sil=.true. fvec_file='f.bin' "
-           f"auto_file='auto' tout=0.1d0 tfinal=1.0d0 krydim=12 autotol=1d-10 /\n")
+           f"auto_file='auto' tout=0.1d0 tfinal=1.0d0 krydim=12 autotol=1d-10 nto_file='trd.bin' /\n")
     r = subprocess.run([DRIVER], input=nml, capture_output=True, text=True, cwd=tmp_path, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -60,6 +60,12 @@ def test_fortran_driver(tmp_path):
     assert np.abs(ev - w).max() < 1e-10
     X = np.stack([x[2] for x in recs], axis=1)
     assert np.linalg.norm(H @ X - X * ev, axis=0).max() < 1e-8
+    # nto_file: the transition density of every state (adc2_nto_gs input)
+    from oracle import nto
+    trd = np.fromfile(tmp_path / "trd.bin", dtype="<f8").reshape(3, o, v).transpose(0, 2, 1)
+    for s in range(3):
+        ref = nto.trdens_vo(sys_, X[:, s])
+        assert np.abs(trd[s] - ref).max() < 1e-12 * np.abs(ref).max()
     lines = open(tmp_path / "auto").read().splitlines()
     assert lines[0].startswith("#    time[fs]")
     data = np.array([[float(t) for t in ln.split()] for ln in lines[1:]])
