@@ -41,6 +41,23 @@ using namespace adc2g;
 
 extern "C" {
 
+// the entry points that need the orbital structure of the operator
+static void require_orbitals(const adc2g::Handle& h, const char* what) {
+  if (h.stored)
+    throw adc2g::Error(ADC2G_EARG, std::string(what) + " needs the orbital structure: not available on a stored-matrix "
+                                   "handle (adc2g_create_from_hmlt)");
+}
+
+int32_t adc2g_create_from_hmlt(const char* dia_path, const char* off_path, int32_t device, adc2g_handle** out) {
+  return guarded([&] {
+    ARG_CHECK(out != nullptr, "out must not be NULL");
+    *out = nullptr;
+    auto hp = std::make_unique<adc2g_handle>();
+    hp->h.create_stored(dia_path, off_path, device);
+    *out = hp.release();
+  });
+}
+
 int32_t adc2g_create(int32_t nocc, int32_t nvirt, int32_t naux, const double* eps, const double* B,
                      const adc2g_opts* opts, adc2g_handle** out) {
   return guarded([&] {
@@ -60,6 +77,7 @@ int32_t adc2g_create(int32_t nocc, int32_t nvirt, int32_t naux, const double* ep
 int32_t adc2g_create_cvs(adc2g_handle* full, int32_t ncore, const int32_t* core, adc2g_handle** out) {
   return guarded([&] {
     ARG_CHECK(full && out, "NULL argument");
+    require_orbitals(full->h, "a CVS view");
     *out = nullptr;
     HIP_CHECK(hipSetDevice(full->h.device));
     auto hp = std::make_unique<adc2g_handle>();
@@ -71,6 +89,7 @@ int32_t adc2g_create_cvs(adc2g_handle* full, int32_t ncore, const int32_t* core,
 int32_t adc2g_create_frozen_core(adc2g_handle* full, int32_t nfrozen, const int32_t* frozen, adc2g_handle** out) {
   return guarded([&] {
     ARG_CHECK(full && out, "NULL argument");
+    require_orbitals(full->h, "a frozen-core view");
     *out = nullptr;
     HIP_CHECK(hipSetDevice(full->h.device));
     auto hp = std::make_unique<adc2g_handle>();
@@ -82,6 +101,7 @@ int32_t adc2g_create_frozen_core(adc2g_handle* full, int32_t nfrozen, const int3
 int32_t adc2g_set_adc2x(adc2g_handle* h, int32_t enable) {
   return guarded([&] {
     ARG_CHECK(h, "NULL handle");
+    require_orbitals(h->h, "ADC(2)-x");
     HIP_CHECK(hipSetDevice(h->h.device));
     h->h.enable_adcx(enable != 0);
   });
@@ -112,7 +132,8 @@ int32_t adc2g_dims(const adc2g_handle* h, int64_t* ndim, int64_t* n1, int64_t co
 int32_t adc2g_coupling_form(const adc2g_handle* h, int32_t* form) {
   return guarded([&] {
     ARG_CHECK(h && form, "NULL argument");
-    *form = h->h.form == Handle::FORM_INT ? ADC2G_COUPLING_INT : ADC2G_COUPLING_DF;
+    *form = h->h.stored ? ADC2G_COUPLING_STORED
+                        : (h->h.form == Handle::FORM_INT ? ADC2G_COUPLING_INT : ADC2G_COUPLING_DF);
   });
 }
 
@@ -128,6 +149,7 @@ int32_t adc2g_diagonal(adc2g_handle* h, double* hii) {
 int32_t adc2g_phph(adc2g_handle* h, double* h11) {
   return guarded([&] {
     ARG_CHECK(h && h11, "NULL argument");
+    require_orbitals(h->h, "the ph-ph block");
     ARG_CHECK(!h->h.parent, "the ph-ph block belongs to the full operator, not to a CVS view");
     HIP_CHECK(hipSetDevice(h->h.device));
     const size_t n = (size_t)h->h.d.n1 * h->h.d.n1;
@@ -139,6 +161,7 @@ int32_t adc2g_phph(adc2g_handle* h, double* h11) {
 int32_t adc2g_coupled_schur(adc2g_handle* h, double sigma, int32_t precision, double* a0) {
   return guarded([&] {
     ARG_CHECK(h && a0, "NULL argument");
+    require_orbitals(h->h, "the Schur complement");
     ARG_CHECK(!h->h.parent, "the Schur complement belongs to the full operator, not to a CVS view");
     ARG_CHECK(std::isfinite(sigma), "sigma must be finite");
     HIP_CHECK(hipSetDevice(h->h.device));
@@ -186,6 +209,7 @@ int32_t adc2g_sigma_device_noalloc(adc2g_handle* h, int32_t nvec, const double* 
                                    void* stream) {
   return guarded([&] {
     ARG_CHECK(h && dX && dW, "NULL argument");
+    require_orbitals(h->h, "the sigma-vector without allocation");
     ARG_CHECK(nvec >= 0, "nvec must be >= 0");
     Handle& H = h->h;
     HIP_CHECK(hipSetDevice(H.device));
@@ -281,6 +305,7 @@ int32_t adc2g_sil_laser(adc2g_handle* h, const adc2g_laser_params* p, int32_t np
 int32_t adc2g_trdens_vo(adc2g_handle* h, int32_t nvec, const double* x, int64_t ldx, double* trdens) {
   return guarded([&] {
     ARG_CHECK(h && x && trdens, "NULL argument");
+    require_orbitals(h->h, "the transition density");
     HIP_CHECK(hipSetDevice(h->h.device));
     trdens_vo_run(h->h, nvec, x, ldx, trdens);
   });
@@ -308,6 +333,7 @@ int32_t adc2g_sil_autocorr_orders(adc2g_handle* h, const adc2g_sil_params* p, co
 int32_t adc2g_precon_harvest(adc2g_handle* h, const adc2g_dav_params* p, int64_t max_records, int64_t* added) {
   return guarded([&] {
     ARG_CHECK(h && p, "NULL argument");
+    require_orbitals(h->h, "the learned preconditioner");
     ARG_CHECK(!h->h.parent, "the learned preconditioner works on the full operator only");
     HIP_CHECK(hipSetDevice(h->h.device));
     precon_harvest(h->h, *p, max_records, added);
@@ -407,6 +433,7 @@ int32_t adc2g_write_hmlt(adc2g_handle* h, const char* dia_path, const char* off_
 int32_t adc2g_mp2(adc2g_handle* h, double* e_mp2, double* d2, double* rho) {
   return guarded([&] {
     ARG_CHECK(h != nullptr, "NULL handle");
+    require_orbitals(h->h, "MP2");
     HIP_CHECK(hipSetDevice(h->h.device));
     mp2_run(h->h, e_mp2, d2, rho);
   });
@@ -415,6 +442,7 @@ int32_t adc2g_mp2(adc2g_handle* h, double* e_mp2, double* d2, double* rho) {
 int32_t adc2g_fvector(adc2g_handle* h, const double* dpl, double* f) {
   return guarded([&] {
     ARG_CHECK(h && dpl && f, "NULL argument");
+    require_orbitals(h->h, "the F-vector");
     HIP_CHECK(hipSetDevice(h->h.device));
     fvector_run(h->h, dpl, f);
   });

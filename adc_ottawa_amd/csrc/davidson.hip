@@ -340,8 +340,8 @@ void davidson_run(Handle& H, const adc2g_dav_params& p, double* eigval, double* 
             "precon must be 1 (DPR), 2 (Olsen), 3 (learned), 4 (coupled) or 5 (learned on the coupled step)");
   const bool learned = p.precon == ADC2G_PRE_LEARNED || p.precon == ADC2G_PRE_LEARNED_COUPLED;
   const bool coupled = p.precon == ADC2G_PRE_COUPLED || p.precon == ADC2G_PRE_LEARNED_COUPLED;
-  ARG_CHECK(!((learned || coupled) && H.parent),
-            "the learned and coupled preconditioners work on the full operator only");
+  ARG_CHECK(!((learned || coupled) && (H.parent || H.stored)),
+            "the learned and coupled preconditioners work on the full operator built from integrals only");
   if (learned && !precon_ready(H))
     throw Error(ADC2G_ESTATE, "learned preconditioner requested but no network is loaded");
   const int nstates = p.nstates;

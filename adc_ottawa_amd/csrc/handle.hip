@@ -336,6 +336,11 @@ void Handle::sigma(const double* X, int64_t ldx, double* W, int64_t ldw, int nve
     return;
   }
   if (!s) s = st;
+  if (stored) {  // FP64 at every requested precision
+    ARG_CHECK(!scratch, "a stored-matrix handle has no scratch-buffer sigma-vector");
+    sigma_stored(X, ldx, W, ldw, nvec, s);
+    return;
+  }
   const size_t es = prec_size(p);
   const size_t n1 = d.n1, naux = d.naux;
   const size_t wes = (p == Prec::F64) ? 8 : 4;  // DF W accumulator type

@@ -93,6 +93,16 @@ struct Handle {
   // the parent's space, applies the parent's sigma-vector and gathers.  d of
   // a view holds only ndim, n1 and the type counts (vcnt).
   Handle* parent = nullptr;
+  // stored matrix (stored.hip): the reference's hmlt files in CSR form with
+  // both triangles; d.ndim only (no orbital structure), H_jj in hii
+  bool stored = false;
+  int64_t stored_noffd = 0;
+  double stored_per_row = 0.0;
+  DevBuf<int64_t> st_rowptr;
+  DevBuf<int32_t> st_col;
+  DevBuf<double> st_val;
+  void create_stored(const char* dia_path, const char* off_path, int dev);
+  void sigma_stored(const double* X, int64_t ldx, double* W, int64_t ldw, int nvec, hipStream_t s);
   int nviews = 0;                 // views alive on this handle
   DevBuf<int64_t> vmap, vinv;     // view -> parent index, parent -> view index or -1
   DevBuf<double> vX, vW;          // parent-size staging (columns of a chunk)

@@ -59,7 +59,8 @@ __global__ void nto_final_kernel(int o, int v, int nvec, const double* __restric
 
 void nto_prepare(Handle& H) {
   if (H.nto_ready) return;
-  ARG_CHECK(!H.parent, "the transition density belongs to the full operator, not to a CVS or frozen-core view");
+  ARG_CHECK(!H.parent && !H.stored, "the transition density needs the full operator built from integrals, not a view "
+                                    "or a stored matrix");
   const int o = H.d.o, v = H.d.v, nbas = o + v;
   const int64_t n1 = H.d.n1, nn = n1 * n1;
   hipStream_t st = H.st;

@@ -556,7 +556,7 @@ void sil_laser_run(Handle& H, const adc2g_laser_params& p, int npulse, const adc
   if (p.kdim < 2) throw Error(ADC2G_ESIL_ORDER, "IntOrder is too small");
   laser_check_pulses(npulse, pulses);
   ARG_CHECK(!flux || (H.cap.set && H.flux_theta.set), "the flux needs a CAP and the Theta operator");
-  ARG_CHECK(!trdens || !H.parent, "the transition density belongs to the full operator, not to a view");
+  ARG_CHECK(!trdens || (!H.parent && !H.stored), "the transition density needs the full operator built from integrals");
   const double intperiod = p.tout;
   const int64_t nrec = (int64_t)(p.tfinal / intperiod);
   ARG_CHECK(nrec_cap >= nrec + 1, "norms holds fewer than nrec+1 values");

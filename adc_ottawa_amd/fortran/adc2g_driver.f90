@@ -36,6 +36,9 @@
 !            every later step uses the configurations with no hole in them;
 !   adc2x = .true.: ADC(2)-x (method = adc2-x): the first-order 2h2p-2h2p
 !            block in the matrix (set before the CVS / frozen-core view);
+!   dia_file = 'hmlt.diai', off_file = 'hmlt.offi': the matrix from the
+!            reference's files instead of eps/B (rdham_on / rdham_off and
+!            hxvec_incore); Davidson, SIL, Lanczos, Chebyshev and hmlt only;
 !   nto_file = 'trd.bin': the occupied-virtual transition density of every
 !            state (nvirt x nocc doubles per state, adc2_trdens_gs_virt_occ),
 !            the input of the NTOs of adc2_nto_gs (full space only);
@@ -80,7 +83,7 @@ program adc2g_driver
   real(c_double) :: stepsize = 10.0d0, cheby_emin = 0.0d0, cheby_emax = 1.0d0, siltol = 1.0d-5, ehf = 0.0d0
   character(len=256) :: eps_file = 'eps.bin', b_file = 'B.bin', davname = 'davstates'
   character(len=256) :: fvec_file = 'fvec.bin', auto_file = 'auto', lancname = 'lancstates'
-  character(len=256) :: nto_file = ''
+  character(len=256) :: nto_file = '', dia_file = '', off_file = ''
   character(len=256) :: dpl_file = ''
   ! laser / CAP propagation
   integer, parameter :: maxpulse = 8
@@ -95,7 +98,7 @@ program adc2g_driver
   namelist /adc2g/ nocc, nvirt, naux, eps_file, b_file, nstates, blocksize, maxvec, maxiter, tol, &
        precon, precision, deflate, davname, sil, fvec_file, auto_file, tout, tfinal, krydim, autotol, &
        solver, guess, subdim, stepsize, rlx_krydim, autoord, lanczos, lmain, ncycles, lancname, &
-       chebyord, cheby_emin, cheby_emax, hmlt, dpl_file, nto_file, lancguess, orthotype, integrator, siltol, &
+       chebyord, cheby_emin, cheby_emax, hmlt, dpl_file, nto_file, dia_file, off_file, lancguess, orthotype, integrator, siltol, &
        ncore, core, ehf, nfrozen, adc2x, laser, cap_ground, npulse, carrier, envelope, pulse_vec, &
        strength, freq, t0, phase, envpar, d00, w00, proptol, dipis_file, d0j_file, cap_file, w0j_file, psi_file, &
        statenumber, cap_nproj, theta_file, theta0j_file, theta00, trdens_file
@@ -127,17 +130,29 @@ program adc2g_driver
      stop 1
   end if
   nbas = nocc + nvirt
-  allocate(eps(nbas), B(int(naux, c_int64_t) * nbas * nbas))
-  open(newunit=u, file=trim(eps_file), access='stream', form='unformatted', status='old')
-  read(u) eps
-  close(u)
-  open(newunit=u, file=trim(b_file), access='stream', form='unformatted', status='old')
-  read(u) B
-  close(u)
-
-  rc = adc2g_create(int(nocc, c_int32_t), int(nvirt, c_int32_t), int(naux, c_int32_t), eps, B, opts, h)
-  call check(rc)
-  deallocate(B)
+  if (len_trim(dia_file) > 0) then
+     ! the matrix from the reference's hmlt files (rdham_on / rdham_off and
+     ! hxvec_incore): no orbitals, so no view, ADC(2)-x, MP2 or F-vector
+     if (adc2x .or. nfrozen > 0 .or. ncore > 0 .or. len_trim(dpl_file) > 0 .or. laser .or. &
+         len_trim(nto_file) > 0) then
+        write(*, '(a)') ' adc2g input error: dia_file allows Davidson, SIL, Lanczos, Chebyshev and hmlt only'
+        stop 1
+     end if
+     rc = adc2g_create_from_hmlt(trim(dia_file)//c_null_char, trim(off_file)//c_null_char, 0_c_int32_t, h)
+     call check(rc)
+     write(*, '(a,a,a,a)') ' stored matrix from ', trim(dia_file), ' and ', trim(off_file)
+  else
+     allocate(eps(nbas), B(int(naux, c_int64_t) * nbas * nbas))
+     open(newunit=u, file=trim(eps_file), access='stream', form='unformatted', status='old')
+     read(u) eps
+     close(u)
+     open(newunit=u, file=trim(b_file), access='stream', form='unformatted', status='old')
+     read(u) B
+     close(u)
+     rc = adc2g_create(int(nocc, c_int32_t), int(nvirt, c_int32_t), int(naux, c_int32_t), eps, B, opts, h)
+     call check(rc)
+     deallocate(B)
+  end if
   hfull = c_null_ptr
   if (adc2x) then
      call check(adc2g_set_adc2x(h, 1_c_int32_t))
@@ -227,7 +242,7 @@ program adc2g_driver
      close(u)
      write(*, '(a,a)') ' NTO transition densities written to ', trim(nto_file)
   end if
-  if (ncore == 0 .and. nfrozen == 0) then
+  if (ncore == 0 .and. nfrozen == 0 .and. len_trim(dia_file) == 0) then
      ! table2 / wrstateinfo (misc.f90:648-852): 'davstates.dat' (full space)
      call check(adc2g_mp2(h, emp2, d2mp, c_null_ptr))
      call write_state_table('davstates.dat')

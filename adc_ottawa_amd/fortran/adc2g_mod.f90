@@ -134,6 +134,14 @@ module adc2g_mod
   end type adc2g_lanczos_stats
 
   interface
+     ! a stored-matrix handle from the reference's hmlt.dia / hmlt.off files
+     integer(c_int32_t) function adc2g_create_from_hmlt(dia_path, off_path, device, out) &
+          bind(c, name='adc2g_create_from_hmlt')
+       import :: c_int32_t, c_char, c_ptr
+       character(kind=c_char), intent(in) :: dia_path(*), off_path(*)
+       integer(c_int32_t), value :: device
+       type(c_ptr), intent(out) :: out
+     end function adc2g_create_from_hmlt
      integer(c_int32_t) function adc2g_create(nocc, nvirt, naux, eps, B, opts, h) bind(c, name='adc2g_create')
        import :: c_int32_t, c_double, c_ptr, adc2g_opts
        integer(c_int32_t), value :: nocc, nvirt, naux

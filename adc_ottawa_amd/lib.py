@@ -190,7 +190,7 @@ SYMBOLS = [
     "adc2g_sil_autocorr_orders", "adc2g_create_cvs", "adc2g_create_frozen_core", "adc2g_set_adc2x", "adc2g_mp2", "adc2g_fvector", "adc2g_lanczos_guess",
     "adc2g_spectral_bounds", "adc2g_power", "adc2g_coupled_schur", "adc2g_sigma_device_noalloc",
     "adc2g_set_dipole_is", "adc2g_matxvec_treal_laser", "adc2g_sil_laser", "adc2g_set_cap_is",
-    "adc2g_set_cap_projector", "adc2g_set_flux_theta", "adc2g_trdens_vo",
+    "adc2g_set_cap_projector", "adc2g_set_flux_theta", "adc2g_trdens_vo", "adc2g_create_from_hmlt",
 ]
 
 _lib = None
@@ -255,6 +255,7 @@ def load(path: str = LIBPATH):
     lib.adc2g_sil_laser.argtypes = [vp, C.POINTER(LaserParams), i32, vp, vp, vp, vp, vp, i64, C.POINTER(i64),
                                     C.POINTER(SilStats)]
     lib.adc2g_trdens_vo.argtypes = [vp, i32, vp, i64, vp]
+    lib.adc2g_create_from_hmlt.argtypes = [C.c_char_p, C.c_char_p, i32, C.POINTER(C.c_void_p)]
     lib.adc2g_set_flux_theta.argtypes = [vp, vp, i64, vp, vp, vp, C.c_double, vp]
     for s in SYMBOLS:
         getattr(lib, s).restype = C.c_int32
@@ -299,6 +300,24 @@ class Operator:
     @classmethod
     def from_system(cls, sys, **kw):
         return cls(sys.nocc, sys.nvirt, sys.naux, sys.eps, sys.B, **kw)
+
+    @classmethod
+    def from_hmlt(cls, dia_path, off_path, device=0):
+        """A stored-matrix operator from the reference's hmlt.dia / hmlt.off
+        files (adc2g_create_from_hmlt): the solvers, sigma, diagonal and
+        write_hmlt work on it; what needs the orbitals raises Adc2gError."""
+        lib = load()
+        self = cls.__new__(cls)
+        self._h = C.c_void_p()
+        check(lib.adc2g_create_from_hmlt(os.fsencode(dia_path), os.fsencode(off_path), int(device),
+                                         C.byref(self._h)))
+        nd, n1 = C.c_int64(), C.c_int64()
+        cnt = (C.c_int64 * 5)()
+        check(lib.adc2g_dims(self._h, C.byref(nd), C.byref(n1), cnt))
+        self.ndim, self.n1, self.counts = nd.value, n1.value, list(cnt)
+        self.nocc = self.nvirt = self.naux = 0
+        self.coupling = "stored"
+        return self
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

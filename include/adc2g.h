@@ -66,6 +66,7 @@ extern "C" {
 #define ADC2G_COUPLING_AUTO 0   /* INT when it needs fewer flops and its integrals fit, else DF */
 #define ADC2G_COUPLING_DF 1     /* contractions with the factors B[P,p,q] (sums over naux) */
 #define ADC2G_COUPLING_INT 2    /* contractions with (vv|ov), (oo|ov) assembled once from the factors */
+#define ADC2G_COUPLING_STORED 3 /* (reported only) a stored matrix, adc2g_create_from_hmlt */
 
 typedef struct adc2g_handle adc2g_handle;
 
@@ -194,6 +195,26 @@ int32_t adc2g_create(int32_t nocc, int32_t nvirt, int32_t naux, const double* ep
 /* Free everything the handle owns.  An operator with CVS views alive is not
  * destroyed (ADC2G_ESTATE): destroy the views first. */
 int32_t adc2g_destroy(adc2g_handle* h);
+
+/* A handle on a stored matrix: the reference's on-disk files hmlt.dia* and
+ * hmlt.off* (the format adc2g_write_hmlt writes: wrtdg / wrtoffdg,
+ * source/adclib/filetools.f90:24-44), loaded as its in-core Davidson loads
+ * them (rdham_on / rdham_off, source/eigen/block_davidson.f90:588-686), with
+ * the product of hxvec_incore (:844-883): H_jj x_j plus, for every stored
+ * element (i, j, h_ij), h_ij x_j on row i and h_ij x_i on row j.  The
+ * elements go to device memory as CSR with both entries (12 bytes each, two
+ * per stored element).  The sigma-vector of every precision is computed in
+ * FP64.  The solvers (Davidson with the DPR and Olsen steps and the ondiag,
+ * subdiag and vector guesses, SIL, laser propagation, Chebyshev, block
+ * Lanczos, power method, relaxation, spectral bounds), adc2g_dims (ndim; n1
+ * and counts 0), adc2g_diagonal, adc2g_sigma, adc2g_sigma_device,
+ * adc2g_matxvec_treal and adc2g_write_hmlt accept the handle; the entry points
+ * that need the orbitals (views, ADC(2)-x, ph-ph block, Schur complement,
+ * coupled and learned steps, MP2, F-vector, transition density, the
+ * no-allocation sigma-vector) return ADC2G_EARG.  Files with records split
+ * into gfortran sub-records (over 2 GiB) and matrices of 2^31 rows or more
+ * are rejected. */
+int32_t adc2g_create_from_hmlt(const char* dia_path, const char* off_path, int32_t device, adc2g_handle** out);
 
 /* CVS view of an operator (core-valence separation; select_atom_is_cvs,
  * source/adclib/select_fano.f90:112-142, select_atom_d_cvs, :903-1027, with the
