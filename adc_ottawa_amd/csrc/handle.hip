@@ -829,6 +829,7 @@ void Handle::create_frozen_core(Handle& full, int nfrozen, const int32_t* frozen
   ARG_CHECK(!full.parent, "a view cannot be made from another view");
   const int o = full.d.o, v = full.d.v;
   const std::vector<char> isf = view_orbitals(o, nfrozen, frozen, "frozen");
+  frozen_occ = isf;
   const Dims& F = full.d;
   std::vector<int64_t> map;
   map.reserve((size_t)F.ndim);

@@ -93,6 +93,8 @@ struct Handle {
   // the parent's space, applies the parent's sigma-vector and gathers.  d of
   // a view holds only ndim, n1 and the type counts (vcnt).
   Handle* parent = nullptr;
+  // frozen-core view: the frozen occupied orbitals (1 = frozen), empty otherwise
+  std::vector<char> frozen_occ;
   // stored matrix (stored.hip): the reference's hmlt files in CSR form with
   // both triangles; d.ndim only (no orbital structure), H_jj in hii
   bool stored = false;
@@ -147,7 +149,7 @@ struct Handle {
   DevBuf<double> laser_part, laser_u, laser_c, laser_coef, laser_vz;  // work arrays of the laser operator
   // transition density (nto.hip): the n1 x n1 matrix M, the MP2 density and work space
   bool nto_ready = false;
-  DevBuf<double> nto_M, nto_rho, nto_p;
+  DevBuf<double> nto_M, nto_rho, nto_p, nto_x;
   void set_dipole_is(int c, const double* diag, int64_t nnz, const int64_t* row, const int64_t* col,
                      const double* val, double d00, const double* d0j);
   void set_cap_is(const double* diag, int64_t nnz, const int64_t* row, const int64_t* col, const double* val,

@@ -1,7 +1,8 @@
 // MP2 ground-state quantities on the device: the energy (mp2_ener,
 // source/adclib/mp2.f90:68-154), the D2 diagnostic (mp2_d2, :161-276) and the
 // ground-state density matrix (rho_mp2, :301-413, with rhogs2_uu, rhogs2_oo,
-// rhogs2_ou, :421-596), closed shell, no frozen core.
+// rhogs2_ou, :421-596), closed shell; on a frozen-core view with the frozen
+// occupied orbitals out of every occupied sum, as lifrzcore sets occ1.
 //
 // With G[(a,i),(b,j)] = (ai|bj) = sum_P B_P,ai B_P,bj (one GEMM over the
 // density-fitting factors), D_ijab = e_i + e_j - e_a - e_b,
@@ -45,14 +46,22 @@ __device__ double mp_block_sum(double s) {
 }
 
 // X = G / D, Y = (2 G - G_swap) / D; part[blk] = partial energy
+// frz (o flags, or NULL): amplitudes with a frozen occupied index are zero,
+// which restricts every occupied sum to the active orbitals (occ1 = ncore + 1)
 __global__ void __launch_bounds__(MP_THREADS) mp2_amp_kernel(int n1, int o, const double* __restrict__ eps,
-                                                             const double* __restrict__ G, double* __restrict__ X,
-                                                             double* __restrict__ Y, double* __restrict__ part) {
+                                                             const double* __restrict__ G, const char* __restrict__ frz,
+                                                             double* __restrict__ X, double* __restrict__ Y,
+                                                             double* __restrict__ part) {
   const int64_t total = (int64_t)n1 * n1;
   double s = 0.0;
   for (int64_t t = blockIdx.x * (int64_t)MP_THREADS + threadIdx.x; t < total; t += (int64_t)gridDim.x * MP_THREADS) {
     const int m = (int)(t % n1), n = (int)(t / n1);
     const int a = m / o, i = m % o, b = n / o, j = n % o;
+    if (frz && (frz[i] || frz[j])) {
+      X[t] = 0.0;
+      Y[t] = 0.0;
+      continue;
+    }
     const double D = eps[i] + eps[j] - eps[o + a] - eps[o + b];
     const double g = G[t];
     const double gs = G[(int64_t)(b * o + i) + (int64_t)n1 * (a * o + j)];
@@ -135,9 +144,20 @@ void df_block(Handle& H, int M, const double* A, int64_t sak, double* C, hipStre
 
 }  // namespace
 
-void mp2_run(Handle& H, double* e_mp2, double* d2, double* rho) {
-  ARG_CHECK(!H.parent, "MP2 quantities belong to the full operator, not to a CVS view");
+void mp2_run(Handle& Hv, double* e_mp2, double* d2, double* rho) {
+  ARG_CHECK(!Hv.parent || !Hv.frozen_occ.empty(), "MP2 quantities belong to the full operator or a frozen-core "
+                                                  "view, not to a CVS view");
+  // a frozen-core view: the parent's integrals with the frozen occupied
+  // orbitals left out of every occupied sum (lifrzcore, mp2.f90:84-90, 333-337)
+  Handle& H = Hv.parent ? *Hv.parent : Hv;
+  const std::vector<char>* frz = Hv.parent ? &Hv.frozen_occ : nullptr;
   const int o = H.d.o, v = H.d.v, n1 = (int)H.d.n1, nbas = o + v;
+  DevBuf<char> dfrz;
+  if (frz) {
+    dfrz.alloc((size_t)o);
+    HIP_CHECK(hipMemcpyAsync(dfrz.get(), frz->data(), (size_t)o, hipMemcpyHostToDevice, H.st));
+  }
+  auto frozen = [&](int i) { return frz && (*frz)[i]; };
   hipStream_t st = H.st;
   const int64_t nn = (int64_t)n1 * n1;
   DevBuf<double> G, X, Y, part, sums, toto, tvtv;
@@ -149,8 +169,8 @@ void mp2_run(Handle& H, double* e_mp2, double* d2, double* rho) {
   const int nblk = (int)std::min<int64_t>(4096, (nn + MP_THREADS - 1) / MP_THREADS);
   part.alloc(nblk);
   sums.alloc(1);
-  hipLaunchKernelGGL(mp2_amp_kernel, dim3(nblk), dim3(MP_THREADS), 0, st, n1, o, H.eps.get(), G.get(), X.get(),
-                     Y.get(), part.get());
+  hipLaunchKernelGGL(mp2_amp_kernel, dim3(nblk), dim3(MP_THREADS), 0, st, n1, o, H.eps.get(), G.get(),
+                     frz ? dfrz.get() : nullptr, X.get(), Y.get(), part.get());
   HIP_CHECK(hipGetLastError());
   launch_colsum(nblk, 1, part.get(), sums.get(), st);
   double e = 0.0;
@@ -211,6 +231,7 @@ void mp2_run(Handle& H, double* e_mp2, double* d2, double* rho) {
   for (int i = 0; i < o; ++i) R(i, i) = 2.0;
   for (int i = 0; i < o; ++i)
     for (int j = 0; j < o; ++j) {
+      if (frozen(i) || frozen(j)) continue;  // do i = occ1, nocc; do j = occ1, nocc
       R(i, j) = R(i, j) + hoo[i + (size_t)o * j];
       R(j, i) = R(i, j);
     }
@@ -221,6 +242,7 @@ void mp2_run(Handle& H, double* e_mp2, double* d2, double* rho) {
     }
   for (int i = 0; i < o; ++i)
     for (int a = 0; a < v; ++a) {
+      if (frozen(i)) continue;
       const double val = 2.0 * (h1[i + (size_t)o * a] + h2[i + (size_t)o * a]) / (he[o + a] - he[i]);
       R(i, o + a) = val;
       R(o + a, i) = val;

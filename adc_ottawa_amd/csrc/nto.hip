@@ -13,7 +13,10 @@
 // with rhomp2(a,b) and rhomp2(i,j) (:561-589), and the third second-order
 // term (:594-627), whose intermediate lambda is -M X.  M comes from the
 // density-fitting factors once per operator and stays in device memory with
-// rho; each call is then two n1 x n1 x nvec FP64 GEMMs and one kernel.
+// rho; each call is then two n1 x n1 x nvec FP64 GEMMs and one kernel.  On a
+// frozen-core view rho is the frozen-core MP2 density (the frozen occupied
+// orbitals out of its occupied sums) and the view's 1h1p amplitudes enter at
+// their configurations; M keeps every orbital, as the reference's loops do.
 #include <cmath>
 #include <vector>
 
@@ -57,12 +60,27 @@ __global__ void nto_final_kernel(int o, int v, int nvec, const double* __restric
   }
 }
 
+// x (the view's 1h1p rows, ld ldx) scattered into the full 1h1p rows (ld n1)
+__global__ void nto_scatter_kernel(int64_t n1v, int nvec, const int64_t* __restrict__ vmap,
+                                   const double* __restrict__ x, int64_t ldx, double* __restrict__ xf, int64_t n1) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n1v * nvec;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = t / n1v, r = t % n1v;
+    xf[vmap[r] + s * n1] = x[r + s * ldx];
+  }
+}
+
+// a frozen-core view works on its parent's orbitals with the frozen-core MP2
+// density; its 1h1p configurations (the view's first rows) map into the
+// parent's (the frozen ones stay zero), as the reference's loops over kpqf
+// of the frozen-core space leave them
 void nto_prepare(Handle& H) {
   if (H.nto_ready) return;
-  ARG_CHECK(!H.parent && !H.stored, "the transition density needs the full operator built from integrals, not a view "
-                                    "or a stored matrix");
-  const int o = H.d.o, v = H.d.v, nbas = o + v;
-  const int64_t n1 = H.d.n1, nn = n1 * n1;
+  ARG_CHECK(!H.stored, "the transition density needs the operator built from integrals, not a stored matrix");
+  ARG_CHECK(!H.parent || !H.frozen_occ.empty(), "the transition density is not available on a CVS view");
+  Handle& F = H.parent ? *H.parent : H;
+  const int o = F.d.o, v = F.d.v, nbas = o + v;
+  const int64_t n1 = F.d.n1, nn = n1 * n1;
   hipStream_t st = H.st;
   std::vector<double> rho((size_t)nbas * nbas);
   mp2_run(H, nullptr, nullptr, rho.data());
@@ -71,13 +89,13 @@ void nto_prepare(Handle& H) {
   DevBuf<double> G;
   G.alloc(nn);
   GemmArgs g;  // G[(a,i)*n1 + (b,j)] = sum_P Bov[P][(a,i)] Bov[P][(b,j)]
-  g.M = (int)n1; g.N = (int)n1; g.K = H.d.naux;
-  g.A = H.Bov.get(); g.sam = 1; g.sak = n1;
-  g.B = H.Bov.get(); g.sbk = n1; g.sbn = 1;
+  g.M = (int)n1; g.N = (int)n1; g.K = F.d.naux;
+  g.A = F.Bov.get(); g.sam = 1; g.sak = n1;
+  g.B = F.Bov.get(); g.sbk = n1; g.sbn = 1;
   g.map.C = G.get(); g.map.scm = n1; g.map.scn = 1;
   gemm(g, Prec::F64, H.gws, st);
   H.nto_M.alloc(nn);
-  hipLaunchKernelGGL(nto_m_kernel, dim3(grid_for(nn, 256, 65536)), dim3(256), 0, st, o, v, H.eps.get(), G.get(),
+  hipLaunchKernelGGL(nto_m_kernel, dim3(grid_for(nn, 256, 65536)), dim3(256), 0, st, o, v, F.eps.get(), G.get(),
                      H.nto_M.get());
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(st));
@@ -89,8 +107,18 @@ void nto_prepare(Handle& H) {
 void trdens_vo_device(Handle& H, int nvec, const double* x, int64_t ldx, double* out) {
   ARG_CHECK(nvec >= 1 && ldx >= H.d.n1, "nvec must be >= 1 and ldx >= the 1h1p count");
   nto_prepare(H);
-  const int64_t n1 = H.d.n1;
+  const Handle& F = H.parent ? *H.parent : H;
+  const int64_t n1 = F.d.n1;
   hipStream_t st = H.st;
+  if (H.parent) {
+    H.nto_x.alloc((size_t)n1 * nvec);
+    HIP_CHECK(hipMemsetAsync(H.nto_x.get(), 0, sizeof(double) * n1 * nvec, st));
+    hipLaunchKernelGGL(nto_scatter_kernel, dim3(grid_for(H.d.n1 * nvec, 256, 16384)), dim3(256), 0, st, H.d.n1, nvec,
+                       H.vmap.get(), x, ldx, H.nto_x.get(), n1);
+    HIP_CHECK(hipGetLastError());
+    x = H.nto_x.get();
+    ldx = n1;
+  }
   H.nto_p.alloc(2 * (size_t)n1 * nvec);
   double* P1 = H.nto_p.get();
   double* P2 = P1 + n1 * nvec;
@@ -102,7 +130,7 @@ void trdens_vo_device(Handle& H, int nvec, const double* x, int64_t ldx, double*
     g.map.C = pass == 0 ? P1 : P2; g.map.scm = 1; g.map.scn = n1;
     gemm(g, Prec::F64, H.gws, st);
   }
-  hipLaunchKernelGGL(nto_final_kernel, dim3(grid_for(n1 * nvec, 256, 16384)), dim3(256), 0, st, H.d.o, H.d.v, nvec, x,
+  hipLaunchKernelGGL(nto_final_kernel, dim3(grid_for(n1 * nvec, 256, 16384)), dim3(256), 0, st, F.d.o, F.d.v, nvec, x,
                      ldx, P1, P2, H.nto_rho.get(), out);
   HIP_CHECK(hipGetLastError());
 }
@@ -110,15 +138,15 @@ void trdens_vo_device(Handle& H, int nvec, const double* x, int64_t ldx, double*
 void trdens_vo_run(Handle& H, int nvec, const double* x_host, int64_t ldx, double* out_host) {
   ARG_CHECK(x_host && out_host, "NULL argument");
   ARG_CHECK(nvec >= 1 && ldx >= H.d.ndim, "nvec must be >= 1 and ldx >= ndim");
-  const int64_t n1 = H.d.n1;
+  const int64_t n1v = H.d.n1, n1 = H.parent ? H.parent->d.n1 : H.d.n1;
   DevBuf<double> dx, dout;
-  dx.alloc((size_t)n1 * nvec);
+  dx.alloc((size_t)n1v * nvec);
   dout.alloc((size_t)n1 * nvec);
   nto_prepare(H);
   // only the 1h1p rows enter
-  HIP_CHECK(hipMemcpy2DAsync(dx.get(), sizeof(double) * n1, x_host, sizeof(double) * ldx, sizeof(double) * n1, nvec,
+  HIP_CHECK(hipMemcpy2DAsync(dx.get(), sizeof(double) * n1v, x_host, sizeof(double) * ldx, sizeof(double) * n1v, nvec,
                              hipMemcpyHostToDevice, H.st));
-  trdens_vo_device(H, nvec, dx.get(), n1, dout.get());
+  trdens_vo_device(H, nvec, dx.get(), n1v, dout.get());
   HIP_CHECK(hipMemcpyAsync(out_host, dout.get(), sizeof(double) * n1 * nvec, hipMemcpyDeviceToHost, H.st));
   HIP_CHECK(hipStreamSynchronize(H.st));
 }

@@ -556,7 +556,8 @@ void sil_laser_run(Handle& H, const adc2g_laser_params& p, int npulse, const adc
   if (p.kdim < 2) throw Error(ADC2G_ESIL_ORDER, "IntOrder is too small");
   laser_check_pulses(npulse, pulses);
   ARG_CHECK(!flux || (H.cap.set && H.flux_theta.set), "the flux needs a CAP and the Theta operator");
-  ARG_CHECK(!trdens || (!H.parent && !H.stored), "the transition density needs the full operator built from integrals");
+  ARG_CHECK(!trdens || (!H.stored && (!H.parent || !H.frozen_occ.empty())),
+            "the transition density needs the operator built from integrals or a frozen-core view of it");
   const double intperiod = p.tout;
   const int64_t nrec = (int64_t)(p.tfinal / intperiod);
   ARG_CHECK(nrec_cap >= nrec + 1, "norms holds fewer than nrec+1 values");
@@ -584,7 +585,7 @@ void sil_laser_run(Handle& H, const adc2g_laser_params& p, int npulse, const adc
   int maxorder = 0;
   // the TD-NTO input (tdadc2_nto, nto.f90:382-547): the occupied-virtual
   // transition density of psi_IS / |psi_IS|, complex (re, im) per element
-  const int64_t n1 = H.d.n1;
+  const int64_t n1 = H.parent ? H.parent->d.n1 : H.d.n1;  // nvirt x nocc of the full orbital space
   DevBuf<double> tdev, tpart;
   std::vector<double> thost;
   if (trdens) {

@@ -72,15 +72,15 @@ def run(log, frozen, roots, cvs=False, block=0, maxvec=0, tol=1e-7, precon="olse
         target = view if view is not None else op
         try:
             g = adc1_guess(full, kind, orbs, target.ndim, block) if guess == "adc1" else guess
-            if nto and view is not None:
-                raise ValueError("the NTOs are computed in the full configuration space (no --frozen or --cvs)")
+            if nto and kind == "cvs":
+                raise ValueError("the NTOs are not available in the CVS space")
             ev, vecs, st = target.davidson(roots, blocksize=block, maxvec=maxvec, tol=tol, precon=precon,
                                            deflate=deflate, maxiter=1000, vectors=nto, verbose=verbose, guess=g)
             if nto:
                 # adc2_nto_gs (nto.f90:76-260): the SVD of each state's
                 # occupied-virtual transition density, pairs with sigma^2 >= 0.01;
                 # hole and particle orbitals in the AO basis
-                T = op.trdens_vo(vecs)
+                T = target.trdens_vo(vecs)
                 out["nto"] = []
                 for s_ in range(T.shape[0]):
                     sig, hole, part = lib.nto(T[s_], hf.C[:, :hf.nocc], hf.C[:, hf.nocc:], thresh=0.01)

@@ -243,8 +243,9 @@ int32_t adc2g_create_cvs(adc2g_handle* full, int32_t ncore, const int32_t* core,
  * over all occupied orbitals, are those of the full matrix, so the view is
  * the principal sub-matrix on these configurations.  (The reference's MP2
  * with frozen core, source/adclib/mp2.f90:84-90, sums over the active
- * occupied orbitals: adc2g_mp2 of a handle made without the frozen orbitals.)
- * The same solver entry points as a CVS view accept it. */
+ * occupied orbitals: adc2g_mp2 of the view does so, and equals adc2g_mp2 of a
+ * handle made without the frozen orbitals.)  The same solver entry points as
+ * a CVS view accept it, and adc2g_mp2 and adc2g_trdens_vo. */
 int32_t adc2g_create_frozen_core(adc2g_handle* full, int32_t nfrozen, const int32_t* frozen, adc2g_handle** out);
 
 /* ADC(2)-x: with enable = 1 the operator's sigma-vector (every solver entry
@@ -450,7 +451,10 @@ int32_t adc2g_sil_laser(adc2g_handle* h, const adc2g_laser_params* p, int32_t np
  * tdadc2_nto (nto.f90:76-260, 382-547) decompose into natural transition
  * orbitals.  Uses the MP2 density (as adc2g_mp2, so the (vv|ov) block must be
  * in memory) and keeps an n1 x n1 FP64 matrix on the device after the first
- * call (n1 = nocc * nvirt).  Not available on a CVS or frozen-core view. */
+ * call (n1 = nocc * nvirt).  On a frozen-core view the view's 1h1p
+ * amplitudes enter at their configurations, with the frozen-core MP2 density,
+ * as the reference's loops over the frozen-core kpqf; trdens then covers all
+ * nocc x nvirt orbital pairs.  Not available on a CVS view. */
 int32_t adc2g_trdens_vo(adc2g_handle* h, int32_t nvec, const double* x, int64_t ldx, double* trdens);
 
 /* The intermediate-state representation of the projector Theta onto the
@@ -592,7 +596,10 @@ int32_t adc2g_relax(adc2g_handle* h, const adc2g_relax_params* p, const double* 
  *    leave it: 2 + oo(i,i) on the occupied diagonal, oo(i,j) + oo(j,i) on the
  *    occupied off-diagonal, uu and ou assigned once and mirrored.
  * Any output may be NULL.  rho needs the (vv|ov) block in memory
- * (8 v^2 o v bytes).  Not available on a CVS view. */
+ * (8 v^2 o v bytes).  On a frozen-core view: the frozen-core quantities of
+ * lifrzcore (mp2.f90:84-90, 333-337; every occupied sum over the active
+ * orbitals, the frozen rows of rho 2 on the diagonal only), rho over all
+ * nbas orbitals.  Not available on a CVS view. */
 int32_t adc2g_mp2(adc2g_handle* h, double* e_mp2, double* d2, double* rho);
 
 /* ---- F-vector ---------------------------------------------------------------- */

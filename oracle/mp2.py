@@ -69,6 +69,27 @@ def d2(sys, V=None):
     return float(max(np.sqrt(np.linalg.eigvalsh(toto)[-1]), np.sqrt(np.linalg.eigvalsh(tvtv)[-1])))
 
 
+def density_frozen(sys, frozen):
+    """rho_mp2 with lifrzcore (mp2.f90:333-337 and the occ1 loops of
+    rhogs2_uu, rhogs2_oo and rhogs2_ou, :435-449, :491-502, :545-576):
+    every occupied sum runs over the active orbitals, the frozen rows keep
+    2 on the diagonal and nothing else.  That is the density of the system
+    without the frozen orbitals (same integrals, same orbital energies),
+    embedded; frozen holds 0-based occupied indices."""
+    from types import SimpleNamespace
+    o, v = sys.nocc, sys.nvirt
+    frz = sorted(set(int(f) for f in frozen))
+    keep = [i for i in range(o) if i not in frz] + list(range(o, o + v))
+    B = np.asarray(sys.B, dtype=np.float64)
+    red = SimpleNamespace(nocc=o - len(frz), nvirt=v, naux=B.shape[0], eps=np.asarray(sys.eps)[keep],
+                          B=B[:, keep][:, :, keep])
+    rho = np.zeros((o + v, o + v))
+    for f in frz:
+        rho[f, f] = 2.0
+    rho[np.ix_(keep, keep)] = density(red)[0]
+    return rho
+
+
 def density(sys, V=None):
     """rho (nbas x nbas) of rho_mp2, with the reference's assembly."""
     o, v, eo, ev = _split(sys)

@@ -213,3 +213,37 @@ def test_h2o_nto():
             Cn = np.array(r["nto"][s][key])
             assert Cn.shape == (mol.nao, got.size)
             assert np.abs(Cn.T @ S @ Cn - np.eye(got.size)).max() < 1e-8
+
+
+def test_h2o_nto_frozen_core():
+    """--nto with --frozen 1 (lifrzcore): the transition density of each
+    frozen-core Davidson state equals the oracle's (the view's amplitudes at
+    their configurations, the frozen-core MP2 density) to 1e-11, and the
+    front end's singular values follow from it."""
+    from adc_ottawa_amd.frontend.adc import run
+    from oracle import mp2, nto
+    print("[case] front end h2o aug-cc-pVDZ NTOs, frozen core", flush=True)
+    log = os.path.join(GOLDEN, "h2o_augccpvdz_gamess.txt")
+    mol = gamess.read_gamess_log(log)
+    ints = gamess.ao_integrals(mol)
+    hf = scf.rhf(mol, ints)
+    full, _ = scf.mo_system(hf, ints)
+    o, v = full.nocc, full.nvirt
+    with lib.Operator.from_system(full) as op, op.frozen_core([1]) as fc:
+        g = adc1_guess(full, "frozen", [1], fc.ndim, 3)
+        ev, vecs, st = fc.davidson(3, blocksize=3, maxvec=12, tol=1e-8, precon="olsen", maxiter=500, guess=g)
+        T = fc.trdens_vo(vecs)
+        n1v = fc.n1
+    rho = mp2.density_frozen(full, [0])
+    rows = np.array([a * o + i for a in range(v) for i in range(1, o)])
+    assert rows.size == n1v
+    for s in range(3):
+        x = np.zeros(o * v)
+        x[rows] = vecs[:n1v, s]
+        ref = nto.trdens_vo(full, x, rho=rho)
+        assert np.abs(T[s] - ref).max() < 1e-11 * np.abs(ref).max(), s
+    r = run(log, 1, 3, block=3, tol=1e-8, nto=True)
+    for s in range(3):
+        sig = lib.nto(T[s], thresh=0.01)[0]
+        want = sig[sig ** 2 >= 0.01] ** 2
+        assert np.abs(np.array(r["nto"][s]["sigma2"]) - want).max() < 1e-5

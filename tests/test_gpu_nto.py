@@ -34,13 +34,13 @@ def test_trdens_vo_vs_oracle(o, v, naux, seed, form):
     assert np.array_equal(again, got)
 
 
-def test_trdens_vo_rejects_views_and_bad_sizes():
-    print("[case] trdens_vo on a frozen-core view", flush=True)
+def test_trdens_vo_rejects_cvs_views_and_bad_sizes():
+    print("[case] trdens_vo on a CVS view", flush=True)
     sys_ = make_system(3, 5, 8, seed=3)
     with lib.Operator.from_system(sys_) as op:
-        with op.frozen_core([1]) as fc:
+        with op.cvs([1]) as cv:
             with pytest.raises(lib.Adc2gError):
-                fc.trdens_vo(np.ones(fc.ndim))
+                cv.trdens_vo(np.ones(cv.ndim))
         with pytest.raises(ValueError):
             op.trdens_vo(np.ones(op.ndim - 1))
 
@@ -100,3 +100,38 @@ def test_sil_laser_trdens_records():
         Tr = nto.tdnto(sys_, ref_psi[:nd])[0]
         assert np.abs(T[i] - Tr).max() < 1e-8 * np.abs(Tr).max(), i
     assert np.abs(T[-1] - T[0]).max() > 1e-3                # the pulse changed the density
+
+
+def test_frozen_core_mp2_and_trdens():
+    """Frozen-core view (lifrzcore): adc2g_mp2 equals the MP2 of the system
+    without the frozen orbitals (energy, D2) and its density embedded, which
+    is the oracle's frozen-core density; the transition density of view
+    vectors equals the oracle's with the view's 1h1p amplitudes at their
+    configurations and that density."""
+    from oracle import mp2
+    print("[case] frozen-core mp2 and trdens o=5 v=9 frozen 1,3", flush=True)
+    sys_ = make_system(5, 9, 12, seed=8)
+    o, v = sys_.nocc, sys_.nvirt
+    frozen = [1, 3]                                    # 1-based occupied orbitals
+    keep = [i for i in range(o) if i + 1 not in frozen] + list(range(o, o + v))
+    rng = np.random.default_rng(8)
+    with lib.Operator.from_system(sys_) as op, op.frozen_core(frozen) as fc:
+        e, d2, rho = fc.mp2()
+        Xv = rng.normal(size=(fc.ndim, 2))
+        T = fc.trdens_vo(Xv)
+        n1v = fc.n1
+    with lib.Operator(o - 2, v, sys_.naux, sys_.eps[keep], sys_.B[:, keep][:, :, keep]) as red:
+        er, d2r, rhor = red.mp2()
+    assert abs(e - er) < 1e-12 * abs(er) and abs(d2 - d2r) < 1e-12 * d2r
+    emb = np.zeros_like(rho)
+    emb[np.ix_(keep, keep)] = rhor
+    emb[0, 0] = emb[2, 2] = 2.0
+    ref_rho = mp2.density_frozen(sys_, [0, 2])
+    assert np.abs(rho - emb).max() < 1e-12 and np.abs(rho - ref_rho).max() < 1e-12
+    rows = np.array([a * o + i for a in range(v) for i in range(o) if i + 1 not in frozen])
+    assert rows.size == n1v
+    for s in range(2):
+        x = np.zeros(o * v)
+        x[rows] = Xv[:n1v, s]
+        ref = nto.trdens_vo(sys_, x, rho=ref_rho)
+        assert np.abs(T[s] - ref).max() < 1e-12 * np.abs(ref).max(), s

@@ -996,3 +996,23 @@ def test_oracle_tdnto():
     T1, s1, U1, V1, n1 = nto.tdnto(sys, e)
     assert s1[0] > 0.9 * np.sqrt(2.0) and s1[1] < 0.2 * s1[0]
     assert abs(abs(U1[1, 0]) - 1.0) < 0.05 and abs(abs(V1[2, 0]) - 1.0) < 0.05
+
+
+def test_oracle_mp2_density_frozen_core():
+    """The frozen-core MP2 density: with no frozen orbital it is the full
+    density; frozen rows and columns hold only the 2 on the diagonal; the
+    second-order correction stays traceless (trace 2 nocc); the active block
+    differs from the full density's (the frozen orbitals no longer
+    correlate)."""
+    from oracle import mp2
+    sys = make_system(4, 7, 9, seed=12)
+    full = mp2.density(sys)[0]
+    assert np.abs(mp2.density_frozen(sys, []) - full).max() < 1e-14
+    rho = mp2.density_frozen(sys, [0, 2])
+    for f in (0, 2):
+        row = rho[f].copy()
+        assert row[f] == 2.0
+        row[f] = 0.0
+        assert not row.any() and not np.delete(rho[:, f], f).any()
+    assert abs(np.trace(rho) - 2 * sys.nocc) < 1e-12
+    assert np.abs(rho[np.ix_([1, 3], [1, 3])] - full[np.ix_([1, 3], [1, 3])]).max() > 1e-6
