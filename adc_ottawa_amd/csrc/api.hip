@@ -402,6 +402,15 @@ int32_t adc2g_cheby_auto(adc2g_handle* h, const double* bounds, const double* q0
   });
 }
 
+int32_t adc2g_cheby_fdstates(adc2g_handle* h, const double* bounds, const double* q0, int32_t order, int32_t nsel,
+                             const double* coef, int32_t precision, double* vecs, double* energies) {
+  return guarded([&] {
+    ARG_CHECK(h && bounds && q0 && coef && energies, "NULL argument");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    cheby_fdstates_run(h->h, bounds, q0, order, nsel, coef, prec_from_int(precision), vecs, energies);
+  });
+}
+
 int32_t adc2g_block_lanczos(adc2g_handle* h, const adc2g_lanczos_params* p, const double* q0,
                             double* eigval, double* tmat, const double* travec, double* tranmom,
                             double* ritz, adc2g_lanczos_stats* stats) {

@@ -9,9 +9,11 @@
 // One sigma-vector per order; the scaling, the recursion and the three dot
 // products of the order are one streaming kernel; the dot products stay on
 // the device until the end (no host round trip per order).
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
+#include "gemm.hpp"
 #include "linalg.hpp"
 #include "solvers.hpp"
 
@@ -113,6 +115,82 @@ void cheby_run(Handle& H, const double bounds[2], const double* q0_h, int order,
       auto_out[2 * k - 1] = 2.0 * h[3 * k + 2] - auto_out[1];
     }
   }
+}
+
+// Chebyshev filter-diagonalisation eigenstates (calc_eigenstates of
+// fdstates_cfd, source/propagation/fdstates_cfd.f90:207-310): for the nsel
+// selected states, v_i = sum_{k=0..K} coef[k, i] q_k with the same recursion,
+// then v_i / |v_i| and E_i = v_i . H v_i (matxvec returns -H v, negated
+// there).  The q_k go into a ring of KC device columns; every KC orders one
+// GEMM adds Q (n x KC) . coef (KC x nsel) to V, so each q_k is written once
+// and read once more.
+void cheby_fdstates_run(Handle& H, const double bounds[2], const double* q0_h, int order, int nsel, const double* coef_h,
+                        Prec p, double* vecs, double* energies) {
+  ARG_CHECK(order >= 0 && nsel >= 1, "order must be >= 0 and nsel >= 1");
+  ARG_CHECK(std::isfinite(bounds[0]) && std::isfinite(bounds[1]) && bounds[1] > bounds[0],
+            "bounds must be finite with Emax > Emin");
+  const int64_t n = H.d.ndim;
+  const int nk = order + 1;
+  const double de = bounds[1] - bounds[0];
+  const double a = 2.0 / de, b = -(2.0 * bounds[0] / de + 1.0);
+  hipStream_t st = H.st;
+  constexpr int KC = 32;
+  const int ring = std::min(KC, nk < 3 ? 3 : nk);
+  DevBuf<double> Q, w, V, C, part;
+  Q.alloc((size_t)n * ring);
+  w.alloc((size_t)n);
+  V.alloc((size_t)n * nsel);
+  C.alloc((size_t)nk * nsel);
+  part.alloc((size_t)reduce_blocks(n) * 3);
+  HIP_CHECK(hipMemcpyAsync(C.get(), coef_h, sizeof(double) * nk * nsel, hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipMemcpyAsync(Q.get(), q0_h, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  auto slot = [&](int k) { return Q.get() + (size_t)(k % ring) * n; };
+  int k0 = 0;  // first order not yet added to V
+  auto flush = [&](int kend) {  // V += Q[k0..kend) coef[k0..kend, :]
+    GemmArgs g;
+    g.M = (int)n; g.N = nsel; g.K = kend - k0;
+    g.A = slot(k0); g.sam = 1; g.sak = n;
+    g.B = C.get() + k0; g.sbk = 1; g.sbn = nk;
+    g.map.C = V.get(); g.map.scm = 1; g.map.scn = n; g.map.beta = (k0 == 0) ? 0.0 : 1.0;
+    gemm(g, Prec::F64, H.gws, st);
+    k0 = kend;
+  };
+  const int nblk = reduce_blocks(n);
+  for (int k = 1; k <= order; ++k) {
+    if (k % ring == k0 % ring && k > k0) flush(k);  // the slot of q_k still holds an unadded order
+    double* qkm1 = slot(k - 1);
+    double* qkm2 = (k > 1) ? slot(k - 2) : qkm1;
+    H.sigma(qkm1, n, w.get(), n, 1, p, st);
+    const double c = (k > 1) ? 2.0 : 1.0, d = (k > 1) ? 1.0 : 0.0;
+    hipLaunchKernelGGL(cheby_step_kernel, dim3(nblk), dim3(CB_THREADS), 0, st, n, a, b, c, d, w.get(), qkm1, qkm2,
+                       qkm1, slot(k), part.get());
+    HIP_CHECK(hipGetLastError());
+  }
+  flush(nk);
+  std::vector<double> hv((size_t)n * nsel);
+  HIP_CHECK(hipMemcpyAsync(hv.data(), V.get(), sizeof(double) * hv.size(), hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+  for (int i = 0; i < nsel; ++i) {
+    double* v = hv.data() + (size_t)i * n;
+    double s = 0.0;
+    for (int64_t r = 0; r < n; ++r) s += v[r] * v[r];
+    ARG_CHECK(s > 0.0, "a filter-diagonalisation state is zero");
+    const double inv = 1.0 / std::sqrt(s);
+    for (int64_t r = 0; r < n; ++r) v[r] *= inv;
+  }
+  HIP_CHECK(hipMemcpyAsync(V.get(), hv.data(), sizeof(double) * hv.size(), hipMemcpyHostToDevice, st));
+  DevBuf<double> W;
+  W.alloc((size_t)n * nsel);
+  H.sigma(V.get(), n, W.get(), n, nsel, p, st);
+  std::vector<double> hw((size_t)n * nsel);
+  HIP_CHECK(hipMemcpyAsync(hw.data(), W.get(), sizeof(double) * hw.size(), hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+  for (int i = 0; i < nsel; ++i) {
+    double e = 0.0;
+    for (int64_t r = 0; r < n; ++r) e += hv[(size_t)i * n + r] * hw[(size_t)i * n + r];
+    energies[i] = e;
+  }
+  if (vecs) std::copy(hv.begin(), hv.end(), vecs);
 }
 
 // spectral_bounds (source/propagation/specbounds.f90): the lower bound is the

@@ -46,6 +46,11 @@ double laser_flux(Handle& H, const double* psi, const double* dtpsi);
 // Chebyshev order-domain autocorrelation C_0..C_2K (auto_out, host) of the
 // host vector q0 with the spectrum scaled by bounds[0..1] = (Emin, Emax).
 void cheby_run(Handle& H, const double bounds[2], const double* q0, int order, Prec p, double* auto_out);
+// Chebyshev filter-diagonalisation eigenstates of the coefficients coef
+// ((order + 1) x nsel column-major, host): vecs (ndim x nsel, host, or NULL)
+// and energies (nsel)
+void cheby_fdstates_run(Handle& H, const double bounds[2], const double* q0, int order, int nsel, const double* coef,
+                        Prec p, double* vecs, double* energies);
 
 // Block Lanczos pseudospectrum (orthotype 0) from the host block q0
 // (ndim x blocksize); outputs as adc2g_block_lanczos documents them.

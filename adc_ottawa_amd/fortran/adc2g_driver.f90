@@ -24,6 +24,10 @@
 !            moments fvec.r_n printed; lancguess 1 (default) unit vectors on
 !            the 1h1p entries of largest |fvec|, 8 the double-subspace and 9
 !            the single-subspace guess (800 ISs); orthotype 0, 1 (pro), 2 (mpro);
+!   fd_nsel > 0 with fd_coef_file (stream, (chebyord + 1) x fd_nsel doubles,
+!            the k2eig columns of the selected states): the Chebyshev
+!            filter-diagonalisation states (calc_fdstates_cfd) from fvec,
+!            written to 'fdstates' as (i, E_i, v_i) records;
 !   chebyord > 0: Chebyshev autocorrelation of fvec with the bounds
 !            cheby_emin / cheby_emax, file 'chebyauto';
 !   hmlt = .true.: the matrix files 'hmlt.diai' / 'hmlt.offi';
@@ -83,7 +87,8 @@ program adc2g_driver
   real(c_double) :: stepsize = 10.0d0, cheby_emin = 0.0d0, cheby_emax = 1.0d0, siltol = 1.0d-5, ehf = 0.0d0
   character(len=256) :: eps_file = 'eps.bin', b_file = 'B.bin', davname = 'davstates'
   character(len=256) :: fvec_file = 'fvec.bin', auto_file = 'auto', lancname = 'lancstates'
-  character(len=256) :: nto_file = '', dia_file = '', off_file = ''
+  character(len=256) :: nto_file = '', dia_file = '', off_file = '', fd_coef_file = ''
+  integer :: fd_nsel = 0
   character(len=256) :: dpl_file = ''
   ! laser / CAP propagation
   integer, parameter :: maxpulse = 8
@@ -98,7 +103,7 @@ program adc2g_driver
   namelist /adc2g/ nocc, nvirt, naux, eps_file, b_file, nstates, blocksize, maxvec, maxiter, tol, &
        precon, precision, deflate, davname, sil, fvec_file, auto_file, tout, tfinal, krydim, autotol, &
        solver, guess, subdim, stepsize, rlx_krydim, autoord, lanczos, lmain, ncycles, lancname, &
-       chebyord, cheby_emin, cheby_emax, hmlt, dpl_file, nto_file, dia_file, off_file, lancguess, orthotype, integrator, siltol, &
+       chebyord, cheby_emin, cheby_emax, hmlt, dpl_file, nto_file, dia_file, off_file, fd_coef_file, fd_nsel, lancguess, orthotype, integrator, siltol, &
        ncore, core, ehf, nfrozen, adc2x, laser, cap_ground, npulse, carrier, envelope, pulse_vec, &
        strength, freq, t0, phase, envpar, d00, w00, proptol, dipis_file, d0j_file, cap_file, w0j_file, psi_file, &
        statenumber, cap_nproj, theta_file, theta0j_file, theta00, trdens_file
@@ -114,6 +119,7 @@ program adc2g_driver
   type(adc2g_lanczos_params) :: lp
   type(adc2g_lanczos_stats) :: ls
   real(c_double), allocatable :: eps(:), B(:), eigval(:), fvec(:), q0(:,:), cauto(:), dpl(:,:), tmv(:), osc(:), trdvo(:,:,:)
+  real(c_double), allocatable :: fdcoef(:,:), fdvec(:,:), fdener(:)
   real(c_double) :: emp2 = 0.0d0, d2mp = 0.0d0
   real(c_double), allocatable, target :: eigvec(:,:), lener(:), ritz(:,:), tmom(:), fvec_t(:)
   complex(c_double_complex), allocatable :: auto0(:)
@@ -299,6 +305,21 @@ program adc2g_driver
      call check(rc)
      call adc2g_write_chebyauto('chebyauto', bounds, int(chebyord, c_int32_t), cauto)
      write(*, '(a,i6)') ' Chebyshev: orders', chebyord
+     if (fd_nsel > 0) then
+        ! calc_fdstates_cfd: the filter-diagonalisation states of the
+        ! coefficients k2eig(0:chebyord, isel(:)) in fd_coef_file, written
+        ! as SCRATCH/fdstates records (i, E_i, v_i)
+        allocate(fdcoef(0:chebyord, fd_nsel), fdvec(ndim, fd_nsel), fdener(fd_nsel))
+        open(newunit=u, file=trim(fd_coef_file), access='stream', form='unformatted', status='old')
+        read(u) fdcoef
+        close(u)
+        call check(adc2g_cheby_fdstates(h, bounds, fvec, int(chebyord, c_int32_t), int(fd_nsel, c_int32_t), &
+             fdcoef, int(precision, c_int32_t), fdvec, fdener))
+        call adc2g_write_davstates('fdstates', ndim, int(fd_nsel, c_int32_t), fdener, fdvec)
+        do i = 1, fd_nsel
+           write(*, '(a,i4,f18.12,f14.7)') ' FD state', i, fdener(i), fdener(i) * adc2g_eh2ev
+        end do
+     end if
   end if
 
   if (sil) then

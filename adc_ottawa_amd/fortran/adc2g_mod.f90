@@ -224,6 +224,14 @@ module adc2g_mod
        type(adc2g_sil_stats), intent(out) :: stats
      end function adc2g_sil_autocorr_orders
 
+     integer(c_int32_t) function adc2g_cheby_fdstates(h, bounds, q0, order, nsel, coef, precision, vecs, &
+          energies) bind(c, name='adc2g_cheby_fdstates')
+       import :: c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: bounds(2), q0(*), coef(*)
+       integer(c_int32_t), value :: order, nsel, precision
+       real(c_double), intent(out) :: vecs(*), energies(*)
+     end function adc2g_cheby_fdstates
      integer(c_int32_t) function adc2g_cheby_auto(h, bounds, q0, order, precision, auto_out) &
           bind(c, name='adc2g_cheby_auto')
        import :: c_int32_t, c_double, c_ptr

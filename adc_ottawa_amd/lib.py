@@ -191,6 +191,7 @@ SYMBOLS = [
     "adc2g_spectral_bounds", "adc2g_power", "adc2g_coupled_schur", "adc2g_sigma_device_noalloc",
     "adc2g_set_dipole_is", "adc2g_matxvec_treal_laser", "adc2g_sil_laser", "adc2g_set_cap_is",
     "adc2g_set_cap_projector", "adc2g_set_flux_theta", "adc2g_trdens_vo", "adc2g_create_from_hmlt",
+    "adc2g_cheby_fdstates",
 ]
 
 _lib = None
@@ -231,6 +232,7 @@ def load(path: str = LIBPATH):
     lib.adc2g_sigma_profile.argtypes = [vp, i32, vp]
     lib.adc2g_coupling_form.argtypes = [vp, vp]
     lib.adc2g_cheby_auto.argtypes = [vp, vp, vp, i32, i32, vp]
+    lib.adc2g_cheby_fdstates.argtypes = [vp, vp, vp, i32, i32, vp, i32, vp, vp]
     lib.adc2g_precon_absorb.argtypes = [vp, vp, C.POINTER(i64)]
     lib.adc2g_write_hmlt.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_double, i32, C.POINTER(i64),
                                      C.POINTER(i32)]
@@ -436,6 +438,22 @@ class Operator:
         out = np.empty(2 * order + 1)
         check(_lib.adc2g_cheby_auto(self._h, _ptr(b), _ptr(q0), int(order), PREC[precision], _ptr(out)))
         return out
+
+    def cheby_fdstates(self, q0, bounds, coef, precision="fp64"):
+        """Chebyshev filter-diagonalisation eigenstates (calc_fdstates_cfd):
+        coef[k, i] (order + 1 rows, one column per selected state) weights
+        T_k(H_norm) q0; returns (energies, vectors (ndim x nsel), normalised)."""
+        q0 = np.ascontiguousarray(q0, dtype=np.float64)
+        if q0.size != self.ndim:
+            raise ValueError("q0 must have ndim entries")
+        c = np.asfortranarray(np.asarray(coef, dtype=np.float64).reshape(len(coef), -1))
+        nk, nsel = c.shape
+        b = np.ascontiguousarray(bounds, dtype=np.float64)
+        vecs = np.empty((self.ndim, nsel), order="F")
+        e = np.empty(nsel)
+        check(_lib.adc2g_cheby_fdstates(self._h, _ptr(b), _ptr(q0), nk - 1, nsel, _ptr(c), PREC[precision],
+                                        _ptr(vecs), _ptr(e)))
+        return e, vecs
 
     def block_lanczos(self, q0, ncycles, precision="fp64", travec=None, tmat=False, ritz=False,
                       check_orth=False, orthotype=0):

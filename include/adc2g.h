@@ -481,6 +481,17 @@ int32_t adc2g_set_flux_theta(adc2g_handle* h, const double* diag, int64_t nnz, c
 int32_t adc2g_cheby_auto(adc2g_handle* h, const double* bounds, const double* q0, int32_t order,
                          int32_t precision, double* auto_out);
 
+/* Chebyshev filter-diagonalisation eigenstates (calc_eigenstates of
+ * fdstates_cfd, source/propagation/fdstates_cfd.f90:207-310, called by
+ * adc2_fdstates): with the recursion of adc2g_cheby_auto from q0 (ndim) and
+ * the bounds, v_i = sum_{k=0..order} coef[k + (order + 1) i] q_k for the nsel
+ * selected states (coef: the reference's k2eig(0:chebyord, isel(i)) columns),
+ * normalised, and energies[i] = v_i . H v_i.  vecs: ndim x nsel column-major,
+ * or NULL.  precision as for adc2g_cheby_auto (accumulation in FP64). */
+int32_t adc2g_cheby_fdstates(adc2g_handle* h, const double* bounds, const double* q0, int32_t order,
+                             int32_t nsel, const double* coef, int32_t precision, double* vecs,
+                             double* energies);
+
 /* Spectral bounds of the matrix as spectral_bounds
  * (source/propagation/specbounds.f90) estimates them.
  *  - method 3 'lanczos' (spectral_bounds_lanczos, :52-105): the lowest and

@@ -13,6 +13,13 @@ scaled matrix-vector product of matxvec_chebyshev_incore
 Pinned by the closed form C_k = sum_n (v_n . q_0)^2 T_k(x_n), x_n the
 eigenvalues of H_norm (``closed_form``, dense eigendecomposition), which
 tests/test_oracle_pins.py checks on small systems.
+
+``fdstates`` restates calc_eigenstates of fdstates_cfd
+(source/propagation/fdstates_cfd.f90:207-310): v_i = sum_k coef[k, i] q_k with
+the same recursion, normalised, E_i = v_i . H v_i (matxvec returns -H v, which
+the reference negates).  Pinned by T_k(H_norm) q0 = V cos(k arccos x) V^T q0
+from the dense eigendecomposition, and by coefficients that make a filter
+which keeps one eigenvector (the state and its eigenvalue come back).
 """
 from __future__ import annotations
 
@@ -51,3 +58,23 @@ def closed_form(H: np.ndarray, q0: np.ndarray, bounds, nk: int) -> np.ndarray:
     th = np.arccos(np.clip(x, -1.0, 1.0))
     k = np.arange(nk)[:, None]
     return (np.cos(k * th[None, :]) * w[None, :]).sum(axis=1)
+
+
+def fdstates(matvec, q0: np.ndarray, bounds, coef: np.ndarray):
+    """(energies, vectors ndim x nsel) of the filter-diagonalisation states."""
+    emin, emax = float(bounds[0]), float(bounds[1])
+    de = emax - emin
+    a, b = 2.0 / de, -(2.0 * emin / de + 1.0)
+    coef = np.asarray(coef, dtype=np.float64)
+    q0 = np.asarray(q0, dtype=np.float64)
+    V = np.outer(q0, coef[0])
+    qkm1, qkm2 = q0.copy(), None
+    for k in range(1, coef.shape[0]):
+        qk = a * matvec(qkm1) + b * qkm1
+        if k > 1:
+            qk = 2.0 * qk - qkm2
+        V += np.outer(qk, coef[k])
+        qkm2, qkm1 = qkm1, qk
+    V /= np.sqrt((V * V).sum(axis=0))
+    E = np.array([V[:, i] @ matvec(V[:, i]) for i in range(V.shape[1])])
+    return E, V

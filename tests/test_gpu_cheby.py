@@ -141,3 +141,25 @@ def test_power_method_vs_oracle(form):
         with pytest.raises(lib.Adc2gError) as ex:
             op2.power(subdim=20)
         assert ex.value.code == 12
+
+
+@pytest.mark.parametrize("order", [0, 1, 10, 31, 32, 75])
+def test_cheby_fdstates_vs_oracle(order):
+    """Chebyshev filter-diagonalisation eigenstates (calc_fdstates_cfd):
+    states and energies against the oracle's recursion (pinned by the dense
+    eigendecomposition) for orders below, at and across the 32-column ring
+    of the accumulation."""
+    print(f"[case] cheby fdstates o=3 v=8 order={order}", flush=True)
+    sys = make_system(3, 8, 10, seed=5)
+    H = elements.build_blocks(sys, apply_minc=False).dense()
+    lam = np.linalg.eigvalsh(H)
+    span = lam[-1] - lam[0]
+    bounds = (lam[0] - 0.02 * span, lam[-1] + 0.02 * span)
+    rng = np.random.default_rng(order + 1)
+    q0 = rng.standard_normal(H.shape[0])
+    coef = rng.standard_normal((order + 1, 3)) / (1.0 + np.arange(order + 1))[:, None]
+    E_ref, V_ref = cheby.fdstates(lambda x: H @ x, q0, bounds, coef)
+    with lib.Operator.from_system(sys) as op:
+        E, V = op.cheby_fdstates(q0, bounds, coef)
+    assert np.abs(V - V_ref).max() < 1e-10
+    assert np.abs(E - E_ref).max() < 1e-10 * span

@@ -104,8 +104,11 @@ def test_fortran_driver_consumers(tmp_path):
            f"solver=2 stepsize=100d0 maxiter=300 tol=1d-7 davname='rlxstates' "
            f"sil=.true. fvec_file='f.bin' auto_file='auto' tout=0.1d0 tfinal=1.0d0 krydim=12 "
            f"autotol=1d-12 autoord=2 lanczos=.true. lmain=3 ncycles=4 lancname='lancstates' "
-           f"chebyord=20 cheby_emin={emin:.17e} cheby_emax={emax:.17e} hmlt=.true. /\n")
+           f"chebyord=20 cheby_emin={emin:.17e} cheby_emax={emax:.17e} hmlt=.true. "
+           f"fd_coef_file='fdc.bin' fd_nsel=2 /\n")
     nml = nml.replace("e-", "d-").replace("e+", "d+")
+    fdc = np.random.default_rng(4).standard_normal((21, 2)) / (1.0 + np.arange(21))[:, None]
+    np.asfortranarray(fdc).T.astype("<f8").tofile(tmp_path / "fdc.bin")   # column-major (0:20, 2)
     r = subprocess.run([DRIVER], input=nml, capture_output=True, text=True, cwd=tmp_path, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -149,6 +152,12 @@ def test_fortran_driver_consumers(tmp_path):
     near = (np.abs(np.abs(np.triu(H, 1)) - 1e-12) < 1e-12).sum()
     nrec_elems = int(np.frombuffer(data, np.int32, 1, len(data) - 8)[0])
     assert abs(nrec_elems - np.count_nonzero(Hm)) <= near
+    # Chebyshev filter-diagonalisation states (fd_coef_file, fd_nsel)
+    recs = read_davstates(tmp_path / "fdstates", ndim)
+    E_ref, V_ref = cheby.fdstates(lambda x: H @ x, f, (emin, emax), fdc)
+    assert [x[0] for x in recs] == [1, 2]
+    assert np.abs(np.array([x[1] for x in recs]) - E_ref).max() < 1e-10 * span
+    assert np.abs(np.stack([x[2] for x in recs], axis=1) - V_ref).max() < 1e-10
 
 
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built")

@@ -1016,3 +1016,40 @@ def test_oracle_mp2_density_frozen_core():
         assert not row.any() and not np.delete(rho[:, f], f).any()
     assert abs(np.trace(rho) - 2 * sys.nocc) < 1e-12
     assert np.abs(rho[np.ix_([1, 3], [1, 3])] - full[np.ix_([1, 3], [1, 3])]).max() > 1e-6
+
+
+def test_oracle_cheby_fdstates():
+    """calc_eigenstates of fdstates_cfd: the accumulated states equal
+    sum_k coef[k, i] V cos(k arccos x) V^T q0 from the dense
+    eigendecomposition (normalised), and a filter built from the Chebyshev
+    expansion of a narrow Gaussian around one eigenvalue returns that
+    eigenvector and eigenvalue."""
+    from oracle import cheby, dfsigma
+    sys = make_system(2, 4, 6, seed=3)
+    df = dfsigma.DF(sys)
+    nd = df.n1 + df.D2.size
+    H = df.sigma(np.eye(nd))
+    H = 0.5 * (H + H.T)
+    lam, U = np.linalg.eigh(H)
+    bounds = (lam[0] - 0.1, lam[-1] + 0.1)
+    rng = np.random.default_rng(3)
+    q0 = rng.normal(size=nd)
+    K = 60
+    coef = rng.normal(size=(K + 1, 3)) / (1.0 + np.arange(K + 1))[:, None]
+    E, V = cheby.fdstates(lambda x: H @ x, q0, bounds, coef)
+    x = (2.0 * lam - bounds[0] - bounds[1]) / (bounds[1] - bounds[0])
+    T = np.cos(np.arange(K + 1)[:, None] * np.arccos(x)[None, :])          # T_k(x_n)
+    want = U @ ((T.T @ coef) * (U.T @ q0)[:, None])
+    want /= np.linalg.norm(want, axis=0)
+    assert np.abs(V - want).max() < 1e-10
+    assert np.abs(E - np.einsum("ni,ni->i", want, H @ want)).max() < 1e-10
+    # a Gaussian filter at one eigenvalue (its Chebyshev coefficients from Gauss-Chebyshev quadrature)
+    gaps = [np.min(np.abs(np.delete(x, i) - x[i])) for i in range(nd)]
+    m = int(np.argmax(gaps))                 # the most isolated eigenvalue (gap 0.078 in x)
+    K = 800
+    th = np.pi * (np.arange(4000) + 0.5) / 4000
+    g = np.exp(-((np.cos(th) - x[m]) / 0.01) ** 2)
+    c = np.array([(2.0 - (k == 0)) * np.mean(g * np.cos(k * th)) for k in range(K + 1)])
+    E1, V1 = cheby.fdstates(lambda v: H @ v, U[:, m] + 0.3 * q0 / np.linalg.norm(q0), bounds, c[:, None])
+    assert gaps[m] > 0.07
+    assert abs(abs(V1[:, 0] @ U[:, m]) - 1.0) < 1e-8 and abs(E1[0] - lam[m]) < 1e-8
