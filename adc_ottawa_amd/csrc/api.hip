@@ -257,6 +257,16 @@ int32_t adc2g_sil_autocorr(adc2g_handle* h, const adc2g_sil_params* p, const dou
   });
 }
 
+int32_t adc2g_sil_fdstates(adc2g_handle* h, const adc2g_sil_params* p, const double* fvec, int32_t nfbas,
+                           const double* ebound, int32_t iwfunc, int32_t nsel, const double* fbas2eig,
+                           double* vecs, double* energies, adc2g_sil_stats* stats) {
+  return guarded([&] {
+    ARG_CHECK(h && p && fvec && ebound && fbas2eig && energies, "NULL argument");
+    HIP_CHECK(hipSetDevice(h->h.device));
+    sil_fdstates_run(h->h, *p, fvec, nfbas, ebound, iwfunc, nsel, fbas2eig, vecs, energies, stats);
+  });
+}
+
 int32_t adc2g_set_dipole_is(adc2g_handle* h, int32_t component, const double* diag, int64_t nnz,
                             const int64_t* row, const int64_t* col, const double* val, double d00,
                             const double* d0j) {

@@ -167,8 +167,17 @@ void cheby_fdstates_run(Handle& H, const double bounds[2], const double* q0_h, i
     HIP_CHECK(hipGetLastError());
   }
   flush(nk);
+  fd_finish(H, V.get(), nsel, p, vecs, energies);
+}
+
+// The states in V (ndim x nsel, device) normalised, and E_i = v_i . H v_i,
+// as both filter-diagonalisation programs finish (fdstates_cfd.f90:286-299,
+// fdstates_tdfd.f90:589-600)
+void fd_finish(Handle& H, double* Vd, int nsel, Prec p, double* vecs, double* energies) {
+  const int64_t n = H.d.ndim;
+  hipStream_t st = H.st;
   std::vector<double> hv((size_t)n * nsel);
-  HIP_CHECK(hipMemcpyAsync(hv.data(), V.get(), sizeof(double) * hv.size(), hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipMemcpyAsync(hv.data(), Vd, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, st));
   HIP_CHECK(hipStreamSynchronize(st));
   for (int i = 0; i < nsel; ++i) {
     double* v = hv.data() + (size_t)i * n;
@@ -178,10 +187,10 @@ void cheby_fdstates_run(Handle& H, const double bounds[2], const double* q0_h, i
     const double inv = 1.0 / std::sqrt(s);
     for (int64_t r = 0; r < n; ++r) v[r] *= inv;
   }
-  HIP_CHECK(hipMemcpyAsync(V.get(), hv.data(), sizeof(double) * hv.size(), hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipMemcpyAsync(Vd, hv.data(), sizeof(double) * hv.size(), hipMemcpyHostToDevice, st));
   DevBuf<double> W;
   W.alloc((size_t)n * nsel);
-  H.sigma(V.get(), n, W.get(), n, nsel, p, st);
+  H.sigma(Vd, n, W.get(), n, nsel, p, st);
   std::vector<double> hw((size_t)n * nsel);
   HIP_CHECK(hipMemcpyAsync(hw.data(), W.get(), sizeof(double) * hw.size(), hipMemcpyDeviceToHost, st));
   HIP_CHECK(hipStreamSynchronize(st));

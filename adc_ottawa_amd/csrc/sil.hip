@@ -23,6 +23,7 @@
 #include <complex>
 #include <functional>
 
+#include "gemm.hpp"
 #include "linalg.hpp"
 #include "solvers.hpp"
 
@@ -652,6 +653,115 @@ void sil_laser_run(Handle& H, const adc2g_laser_params& p, int npulse, const adc
     psi_io[2 * i + 1] = planar[n + i];
   }
   if (nrec_out) *nrec_out = nrec + 1;
+  if (stats) {
+    stats->nsteps = nsteps;
+    stats->nsigma = S.nsigma;
+    stats->max_order = maxorder;
+    stats->reserved = 0;
+    stats->time_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+}
+
+// calc_fdstates_tdfd (source/propagation/fdstates_tdfd.f90:28-627): from
+// psi0 = fvec / |fvec|, SIL periods of tout (calc_filterstates, :315-461);
+// after period i (and at t = 0) the filter states gain
+//   buffer_j += fac dt g(t) Re(exp(i E_j t) psi(t)),   E_j = E_1 + (j-1) dE,
+// fac 1 at the first and last record and 2 between, g(t) = cos(pi t /
+// (2 tfinal))^iwfunc (update_filterstates, :465-520; windowfunc, :524-535),
+// and the states are v_i = sum_j fbas2eig(isel(i), j) buffer_j (:556-627).
+// The buffers are never formed: v_i gains a_i(t) Re psi - b_i(t) Im psi with
+// a_i + i b_i = fac dt g sum_j f2e[i, j] exp(i E_j t), coefficients known
+// before the propagation.  The planar psi(t) of up to 16 records sit in a
+// ring of device columns and one GEMM per 16 records adds them to V.
+void sil_fdstates_run(Handle& H, const adc2g_sil_params& p, const double* fvec, int nfbas, const double ebound[2],
+                      int iwfunc, int nsel, const double* f2e, double* vecs, double* energies,
+                      adc2g_sil_stats* stats) {
+  const auto t0 = std::chrono::steady_clock::now();
+  ARG_CHECK(p.tout > 0 && p.tfinal >= 0, "tout must be > 0 and tfinal >= 0");
+  ARG_CHECK(p.kdim >= 2 && p.kdim <= 30, "krydim must be in 2..30");
+  ARG_CHECK(p.autotol > 0, "autotol must be > 0");
+  ARG_CHECK(nfbas >= 2 && nsel >= 1 && iwfunc >= 0, "need nfbas >= 2, nsel >= 1 and iwfunc >= 0");
+  ARG_CHECK(std::isfinite(ebound[0]) && std::isfinite(ebound[1]) && ebound[1] > ebound[0],
+            "the energy window must be finite with E_max > E_min");
+  const double dt = p.tout;
+  const int64_t nrec = (int64_t)(p.tfinal / dt);
+  const int64_t nr = nrec + 1;
+  Sil S(H, p.kdim, prec_from_int(p.precision));
+  const int64_t n = S.n;
+  // coefficients of every record: rows 2 r (Re psi) and 2 r + 1 (Im psi), column i
+  std::vector<double> coef((size_t)2 * nr * nsel);
+  const double de = (ebound[1] - ebound[0]) / (double)(nfbas - 1);
+  const double pi = 3.14159265358979323846;
+  for (int64_t r = 0; r < nr; ++r) {
+    const double t = (double)r * dt;
+    const double fac = (r == 0 || r == nrec) ? 1.0 : 2.0;
+    const double g = p.tfinal > 0.0 ? std::pow(std::cos(pi * t / (2.0 * p.tfinal)), iwfunc) : 1.0;
+    for (int i = 0; i < nsel; ++i) {
+      double a = 0.0, b = 0.0;
+      for (int j = 0; j < nfbas; ++j) {
+        const double ej = ebound[0] + (double)j * de;
+        a += f2e[i + (size_t)nsel * j] * std::cos(ej * t);
+        b += f2e[i + (size_t)nsel * j] * std::sin(ej * t);
+      }
+      coef[(size_t)2 * r + (size_t)2 * nr * i] = fac * dt * g * a;
+      coef[(size_t)2 * r + 1 + (size_t)2 * nr * i] = -fac * dt * g * b;
+    }
+  }
+  constexpr int RING = 16;
+  DevBuf<double> Q, V, C;
+  Q.alloc((size_t)2 * n * RING);
+  V.alloc((size_t)n * nsel);
+  C.alloc(coef.size());
+  HIP_CHECK(hipMemcpyAsync(C.get(), coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, H.st));
+  int64_t r0 = 0;  // first record not yet added to V
+  auto flush = [&](int64_t rend) {
+    GemmArgs g;
+    g.M = (int)n; g.N = nsel; g.K = (int)(2 * (rend - r0));
+    g.A = Q.get() + (size_t)2 * n * (r0 % RING); g.sam = 1; g.sak = n;
+    g.B = C.get() + 2 * r0; g.sbk = 1; g.sbn = 2 * nr;
+    g.map.C = V.get(); g.map.scm = 1; g.map.scn = n; g.map.beta = (r0 == 0) ? 0.0 : 1.0;
+    gemm(g, Prec::F64, H.gws, H.st);
+    r0 = rend;
+  };
+  auto record = [&](int64_t r) {
+    if (r > r0 && r % RING == 0) flush(r);
+    HIP_CHECK(hipMemcpyAsync(Q.get() + (size_t)2 * n * (r % RING), S.Kp(0), sizeof(double) * 2 * n,
+                             hipMemcpyDeviceToDevice, H.st));
+  };
+  // psi0 = F / |F| (init_wavepacket, :291-311)
+  std::vector<double> f(fvec, fvec + n);
+  double norm2 = 0.0;
+  for (int64_t i = 0; i < n; ++i) norm2 += f[i] * f[i];
+  ARG_CHECK(norm2 > 0.0, "initial vector is zero");
+  const double inv = 1.0 / std::sqrt(norm2);
+  for (auto& x : f) x *= inv;
+  HIP_CHECK(hipMemcpyAsync(S.Kp(0), f.data(), sizeof(double) * n, hipMemcpyHostToDevice, H.st));
+  HIP_CHECK(hipMemsetAsync(S.Kp(0) + n, 0, sizeof(double) * n, H.st));
+  record(0);
+  const double tiny = 1e-9;
+  int64_t nsteps = 0;
+  int maxorder = 0;
+  for (int64_t i = 1; i <= nrec; ++i) {
+    double inttime = 0.0;
+    while (true) {
+      const double stepsize = dt - inttime;
+      S.func(S.Kp(0), S.Kp(1));
+      int order = 0;
+      const double truestep = S.step(stepsize, p.autotol, order);
+      ++nsteps;
+      maxorder = std::max(maxorder, order);
+      inttime += truestep;
+      if (!(std::fabs(dt - inttime) > std::fabs(tiny * dt))) break;
+    }
+    record(i);
+    if (p.verbose && (i % 100 == 0 || i == nrec)) {
+      std::fprintf(stderr, "fd states: period %lld of %lld, %lld steps\n", (long long)i, (long long)nrec,
+                   (long long)nsteps);
+      std::fflush(stderr);
+    }
+  }
+  flush(nr);
+  fd_finish(H, V.get(), nsel, prec_from_int(p.precision), vecs, energies);
   if (stats) {
     stats->nsteps = nsteps;
     stats->nsigma = S.nsigma;

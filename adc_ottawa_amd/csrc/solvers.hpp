@@ -51,6 +51,16 @@ void cheby_run(Handle& H, const double bounds[2], const double* q0, int order, P
 // and energies (nsel)
 void cheby_fdstates_run(Handle& H, const double bounds[2], const double* q0, int order, int nsel, const double* coef,
                         Prec p, double* vecs, double* energies);
+// the filter-diagonalisation states in V (device, ndim x nsel) normalised,
+// copied to vecs (host, or NULL), energies v . H v
+void fd_finish(Handle& H, double* V, int nsel, Prec p, double* vecs, double* energies);
+// time-domain filter-diagonalisation states (calc_fdstates_tdfd): SIL steps
+// of tout from fvec / |fvec|, the filter states of nfbas energies on
+// [ebound[0], ebound[1]] with the window cos(pi t / (2 tfinal))^iwfunc,
+// combined by f2e (nsel x nfbas column-major, host)
+void sil_fdstates_run(Handle& H, const adc2g_sil_params& p, const double* fvec, int nfbas, const double ebound[2],
+                      int iwfunc, int nsel, const double* f2e, double* vecs, double* energies,
+                      adc2g_sil_stats* stats);
 
 // Block Lanczos pseudospectrum (orthotype 0) from the host block q0
 // (ndim x blocksize); outputs as adc2g_block_lanczos documents them.

@@ -224,6 +224,16 @@ module adc2g_mod
        type(adc2g_sil_stats), intent(out) :: stats
      end function adc2g_sil_autocorr_orders
 
+     integer(c_int32_t) function adc2g_sil_fdstates(h, p, fvec, nfbas, ebound, iwfunc, nsel, fbas2eig, vecs, &
+          energies, stats) bind(c, name='adc2g_sil_fdstates')
+       import :: c_int32_t, c_double, c_ptr, adc2g_sil_params, adc2g_sil_stats
+       type(c_ptr), value :: h
+       type(adc2g_sil_params), intent(in) :: p
+       real(c_double), intent(in) :: fvec(*), ebound(2), fbas2eig(*)
+       integer(c_int32_t), value :: nfbas, iwfunc, nsel
+       real(c_double), intent(out) :: vecs(*), energies(*)
+       type(adc2g_sil_stats), intent(out) :: stats
+     end function adc2g_sil_fdstates
      integer(c_int32_t) function adc2g_cheby_fdstates(h, bounds, q0, order, nsel, coef, precision, vecs, &
           energies) bind(c, name='adc2g_cheby_fdstates')
        import :: c_int32_t, c_double, c_ptr

@@ -191,7 +191,7 @@ SYMBOLS = [
     "adc2g_spectral_bounds", "adc2g_power", "adc2g_coupled_schur", "adc2g_sigma_device_noalloc",
     "adc2g_set_dipole_is", "adc2g_matxvec_treal_laser", "adc2g_sil_laser", "adc2g_set_cap_is",
     "adc2g_set_cap_projector", "adc2g_set_flux_theta", "adc2g_trdens_vo", "adc2g_create_from_hmlt",
-    "adc2g_cheby_fdstates",
+    "adc2g_cheby_fdstates", "adc2g_sil_fdstates",
 ]
 
 _lib = None
@@ -233,6 +233,8 @@ def load(path: str = LIBPATH):
     lib.adc2g_coupling_form.argtypes = [vp, vp]
     lib.adc2g_cheby_auto.argtypes = [vp, vp, vp, i32, i32, vp]
     lib.adc2g_cheby_fdstates.argtypes = [vp, vp, vp, i32, i32, vp, i32, vp, vp]
+    lib.adc2g_sil_fdstates.argtypes = [vp, C.POINTER(SilParams), vp, i32, vp, i32, i32, vp, vp, vp,
+                                       C.POINTER(SilStats)]
     lib.adc2g_precon_absorb.argtypes = [vp, vp, C.POINTER(i64)]
     lib.adc2g_write_hmlt.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_double, i32, C.POINTER(i64),
                                      C.POINTER(i32)]
@@ -454,6 +456,24 @@ class Operator:
         check(_lib.adc2g_cheby_fdstates(self._h, _ptr(b), _ptr(q0), nk - 1, nsel, _ptr(c), PREC[precision],
                                         _ptr(vecs), _ptr(e)))
         return e, vecs
+
+    def sil_fdstates(self, fvec, tout, tfinal, kdim, autotol, ebound, fbas2eig, iwfunc=1, precision="fp64"):
+        """Time-domain filter-diagonalisation states (calc_fdstates_tdfd):
+        fbas2eig (nsel x nfbas) combines the filter states of nfbas energies
+        on ebound; returns (energies, vectors (ndim x nsel), stats)."""
+        fvec = np.ascontiguousarray(fvec, dtype=np.float64)
+        if fvec.size != self.ndim:
+            raise ValueError("fvec must have ndim entries")
+        F = np.asfortranarray(np.atleast_2d(np.asarray(fbas2eig, dtype=np.float64)))
+        nsel, nfbas = F.shape
+        eb = np.ascontiguousarray(ebound, dtype=np.float64)
+        vecs = np.empty((self.ndim, nsel), order="F")
+        e = np.empty(nsel)
+        p = SilParams(tout, tfinal, kdim, PREC[precision], autotol, 0, 0)
+        st = SilStats()
+        check(_lib.adc2g_sil_fdstates(self._h, C.byref(p), _ptr(fvec), nfbas, _ptr(eb), int(iwfunc), nsel, _ptr(F),
+                                      _ptr(vecs), _ptr(e), C.byref(st)))
+        return e, vecs, {f: getattr(st, f) for f, _ in SilStats._fields_}
 
     def block_lanczos(self, q0, ncycles, precision="fp64", travec=None, tmat=False, ritz=False,
                       check_orth=False, orthotype=0):

@@ -355,6 +355,20 @@ int32_t adc2g_sil_autocorr_orders(adc2g_handle* h, const adc2g_sil_params* p, co
                                   int32_t autoord, double* auto0, double* auto1, double* auto2,
                                   int64_t nrec_cap, int64_t* nrec_out, adc2g_sil_stats* stats);
 
+/* Time-domain filter-diagonalisation states (calc_fdstates_tdfd,
+ * source/propagation/fdstates_tdfd.f90:28-627, driven by adc2_fdstates): SIL
+ * periods of p->tout (p->kdim, p->autotol, p->precision; p->tfinal) from
+ * fvec / |fvec|; at t = r tout, r = 0 .. int(tfinal / tout), the filter
+ * state of energy E_j = ebound[0] + j (ebound[1] - ebound[0]) / (nfbas - 1)
+ * gains fac tout g(t) Re(exp(i E_j t) psi(t)) (fac 1 at the first and last
+ * record, 2 between; g = cos(pi t / (2 tfinal))^iwfunc), and state i is
+ * sum_j fbas2eig[i + nsel j] buffer_j (fbas2eig: the selected rows
+ * fbas2eig(isel(:), :) of the reference, nsel x nfbas column-major),
+ * normalised, with energies[i] = v_i . H v_i.  vecs: ndim x nsel or NULL. */
+int32_t adc2g_sil_fdstates(adc2g_handle* h, const adc2g_sil_params* p, const double* fvec, int32_t nfbas,
+                           const double* ebound, int32_t iwfunc, int32_t nsel, const double* fbas2eig,
+                           double* vecs, double* energies, adc2g_sil_stats* stats);
+
 /* ---- Laser-driven propagation (propagate_laser_adc2,
  * source/propagation/proplib_adc2.f90:37-630, with matxvec_treal_laser,
  * source/propagation/tdselib.f90:569-720).  The wavefunction has ndim + 1

@@ -14,11 +14,18 @@
   :338-357), intperiod = tout/2, records i = 1..int(tfinal/intperiod) with the
   t/2 trick a0(2 t) = sum_D psi_D(t)^2 * |F|^2 (:521) at time i*tout.
 
+* ``fdstates_td`` follows calc_fdstates_tdfd of
+  source/propagation/fdstates_tdfd.f90:28-627 (periods of tout, the windowed
+  filter states and their combination).
+
 ``func(v)`` must return -i H v for complex v (``matxvec_treal``,
 source/propagation/tdselib.f90:323-437).
 
 Pinned by: the exact propagator exp(-iHt) from a dense eigendecomposition at
-small sizes, norm conservation and the t/2 identity (test_oracle_pins).
+small sizes, norm conservation and the t/2 identity (test_oracle_pins); the
+filter-diagonalisation states by the same states built from the exact
+psi(t) (``states`` argument), and by a filter basis concentrated on one
+isolated eigenvalue, which returns that eigenvalue.
 """
 from __future__ import annotations
 
@@ -143,3 +150,47 @@ def propagate(func, fvec, tout, tfinal, kdim, autotol, record_orders=False, auto
     if record_orders:
         return times, auto, orders
     return times, auto
+
+
+def fdstates_td(func, hmul, fvec, tout, tfinal, kdim, autotol, ebound, f2e, iwfunc, states=None):
+    """calc_fdstates_tdfd (source/propagation/fdstates_tdfd.f90:28-627):
+    psi0 = fvec / |fvec|, SIL periods of tout (calc_filterstates,
+    :315-461), the filter states buffer_j += fac dt g(t) Re(exp(i E_j t)
+    psi(t)) at every record (update_filterstates, :465-520; windowfunc
+    cos(pi t / (2 tfinal))^iwfunc, :524-535), v_i = sum_j f2e[i, j] buffer_j,
+    normalised, E_i = v_i . H v_i (:556-627).  func(x) = -i H x, hmul(x) =
+    H x.  With ``states`` (a function t -> psi(t)) the wavefunctions come
+    from it instead of the SIL steps.  Returns (energies, vectors)."""
+    f2e = np.atleast_2d(np.asarray(f2e, dtype=np.float64))
+    nsel, nfbas = f2e.shape
+    dt = tout
+    nrec = int(tfinal / dt)
+    de = (ebound[1] - ebound[0]) / (nfbas - 1)
+    E = ebound[0] + de * np.arange(nfbas)
+    psi = fvec.astype(np.complex128) / np.sqrt(np.vdot(fvec, fvec).real)
+    buf = np.zeros((fvec.size, nfbas))
+    tiny = 1e-9
+
+    def update(psi, r):
+        t = r * dt
+        fac = 1.0 if (r == 0 or r == nrec) else 2.0
+        gk = np.cos(np.pi * t / (2.0 * tfinal)) ** iwfunc
+        buf[:] += fac * dt * gk * np.real(np.exp(1j * E[None, :] * t) * psi[:, None])
+
+    update(psi if states is None else states(0.0), 0)
+    for i in range(1, nrec + 1):
+        if states is None:
+            inttime = 0.0
+            while True:
+                stepsize = dt - inttime
+                psi, truestep, _ = silstep(psi, func(psi), func, stepsize, kdim, autotol)
+                inttime += truestep
+                if not (abs(dt - inttime) > abs(tiny * dt)):
+                    break
+            update(psi, i)
+        else:
+            update(states(i * dt), i)
+    V = buf @ f2e.T
+    V /= np.sqrt((V * V).sum(axis=0))
+    Es = np.array([V[:, k] @ hmul(V[:, k]) for k in range(nsel)])
+    return Es, V

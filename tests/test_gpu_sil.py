@@ -98,3 +98,31 @@ def test_sil_higher_orders_vs_oracle_and_exact(small, autoord):
     # the zeroth order equals the plain entry point's
     auto0, _ = op.sil_autocorr(f, tout=0.1, tfinal=3.0, kdim=12, autotol=1e-12)
     assert np.abs(autos[0] - auto0).max() == 0.0
+
+
+@pytest.mark.parametrize("tfinal,iwfunc", [(3.0, 1), (20.0, 2)])
+def test_sil_fdstates_vs_oracle(tfinal, iwfunc):
+    """Time-domain filter-diagonalisation states (calc_fdstates_tdfd):
+    energies and states against the oracle's SIL propagation (pinned by the
+    exact propagator), with fewer records than the 16-record ring and with
+    records across it."""
+    from oracle import dfsigma
+    print(f"[case] sil fdstates o=3 v=5 tfinal={tfinal} iwfunc={iwfunc}", flush=True)
+    sys_ = make_system(3, 5, 8, seed=6)
+    df = dfsigma.DF(sys_)
+    nd = df.n1 + df.D2.size
+    H = df.sigma(np.eye(nd))
+    H = 0.5 * (H + H.T)
+    lam = np.linalg.eigvalsh(H)
+    rng = np.random.default_rng(6)
+    f = np.zeros(nd)
+    f[: df.n1] = rng.normal(size=df.n1)
+    F = rng.normal(size=(3, 9))
+    eb = (lam[0], lam[0] + 0.5 * (lam[-1] - lam[0]))
+    E_ref, V_ref = sil.fdstates_td(lambda x: -1j * (H @ x), lambda x: H @ x, f, 0.5, tfinal, 14, 1e-10, eb, F,
+                                   iwfunc)
+    with lib.Operator.from_system(sys_) as op:
+        E, V, st = op.sil_fdstates(f, 0.5, tfinal, 14, 1e-10, eb, F, iwfunc=iwfunc)
+    print(f"{st['nsteps']} SIL steps, energies {np.round(E, 6)}", flush=True)
+    assert np.abs(V - V_ref).max() < 1e-8
+    assert np.abs(E - E_ref).max() < 1e-8

@@ -1053,3 +1053,33 @@ def test_oracle_cheby_fdstates():
     E1, V1 = cheby.fdstates(lambda v: H @ v, U[:, m] + 0.3 * q0 / np.linalg.norm(q0), bounds, c[:, None])
     assert gaps[m] > 0.07
     assert abs(abs(V1[:, 0] @ U[:, m]) - 1.0) < 1e-8 and abs(E1[0] - lam[m]) < 1e-8
+
+
+def test_oracle_sil_fdstates_td():
+    """calc_fdstates_tdfd: the SIL propagation's filter states equal those
+    built from the exact psi(t) = U exp(-i lam t) U^T psi0 (1e-9), and with
+    a single filter energy at an isolated eigenvalue and a long window the
+    state is that eigenvector (energy to 1e-4 of the gap)."""
+    from oracle import dfsigma, sil
+    sys = make_system(2, 4, 6, seed=5)
+    df = dfsigma.DF(sys)
+    nd = df.n1 + df.D2.size
+    H = df.sigma(np.eye(nd))
+    H = 0.5 * (H + H.T)
+    lam, U = np.linalg.eigh(H)
+    rng = np.random.default_rng(5)
+    f = rng.normal(size=nd)
+    psi0 = f / np.linalg.norm(f)
+    exact = lambda t: U @ (np.exp(-1j * lam * t) * (U.T @ psi0))  # noqa: E731
+    func = lambda x: -1j * (H @ x)  # noqa: E731
+    F = rng.normal(size=(3, 7))
+    eb = (lam[0], lam[-1])
+    E1, V1 = sil.fdstates_td(func, lambda x: H @ x, f, 0.5, 20.0, 20, 1e-12, eb, F, 1)
+    E2, V2 = sil.fdstates_td(func, lambda x: H @ x, f, 0.5, 20.0, 20, 1e-12, eb, F, 1, states=exact)
+    assert np.abs(V1 - V2).max() < 1e-9 and np.abs(E1 - E2).max() < 1e-9
+    gaps = [np.min(np.abs(np.delete(lam, i) - lam[i])) for i in range(nd)]
+    m = int(np.argmax(gaps))
+    Fm = np.array([[1.0, 0.0]])
+    E3, V3 = sil.fdstates_td(func, lambda x: H @ x, f, 1.0, 400.0, 20, 1e-12, (lam[m], lam[m] + 1.0), Fm, 2,
+                             states=exact)
+    assert abs(E3[0] - lam[m]) < 1e-4 * gaps[m] and abs(abs(V3[:, 0] @ U[:, m]) - 1.0) < 1e-3
