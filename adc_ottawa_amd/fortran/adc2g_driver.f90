@@ -28,6 +28,11 @@
 !            the k2eig columns of the selected states): the Chebyshev
 !            filter-diagonalisation states (calc_fdstates_cfd) from fvec,
 !            written to 'fdstates' as (i, E_i, v_i) records;
+!   fdtd_nsel > 0 (with sil): the time-domain filter-diagonalisation states
+!            (calc_fdstates_tdfd) of the SIL periods of tout from fvec,
+!            fdtd_nfbas filter energies on fdtd_ebound, window exponent
+!            fdtd_iwfunc, fbas2eig rows in fdtd_coef_file (stream,
+!            fdtd_nsel x fdtd_nfbas doubles), written to 'fdstates_td';
 !   chebyord > 0: Chebyshev autocorrelation of fvec with the bounds
 !            cheby_emin / cheby_emax, file 'chebyauto';
 !   hmlt = .true.: the matrix files 'hmlt.diai' / 'hmlt.offi';
@@ -88,7 +93,9 @@ program adc2g_driver
   character(len=256) :: eps_file = 'eps.bin', b_file = 'B.bin', davname = 'davstates'
   character(len=256) :: fvec_file = 'fvec.bin', auto_file = 'auto', lancname = 'lancstates'
   character(len=256) :: nto_file = '', dia_file = '', off_file = '', fd_coef_file = ''
-  integer :: fd_nsel = 0
+  integer :: fd_nsel = 0, fdtd_nsel = 0, fdtd_nfbas = 0, fdtd_iwfunc = 1
+  character(len=256) :: fdtd_coef_file = ''
+  real(c_double) :: fdtd_ebound(2) = 0.0d0
   character(len=256) :: dpl_file = ''
   ! laser / CAP propagation
   integer, parameter :: maxpulse = 8
@@ -103,7 +110,8 @@ program adc2g_driver
   namelist /adc2g/ nocc, nvirt, naux, eps_file, b_file, nstates, blocksize, maxvec, maxiter, tol, &
        precon, precision, deflate, davname, sil, fvec_file, auto_file, tout, tfinal, krydim, autotol, &
        solver, guess, subdim, stepsize, rlx_krydim, autoord, lanczos, lmain, ncycles, lancname, &
-       chebyord, cheby_emin, cheby_emax, hmlt, dpl_file, nto_file, dia_file, off_file, fd_coef_file, fd_nsel, lancguess, orthotype, integrator, siltol, &
+       chebyord, cheby_emin, cheby_emax, hmlt, dpl_file, nto_file, dia_file, off_file, fd_coef_file, fd_nsel, &
+       fdtd_nsel, fdtd_nfbas, fdtd_iwfunc, fdtd_coef_file, fdtd_ebound, lancguess, orthotype, integrator, siltol, &
        ncore, core, ehf, nfrozen, adc2x, laser, cap_ground, npulse, carrier, envelope, pulse_vec, &
        strength, freq, t0, phase, envpar, d00, w00, proptol, dipis_file, d0j_file, cap_file, w0j_file, psi_file, &
        statenumber, cap_nproj, theta_file, theta0j_file, theta00, trdens_file
@@ -319,6 +327,7 @@ program adc2g_driver
         do i = 1, fd_nsel
            write(*, '(a,i4,f18.12,f14.7)') ' FD state', i, fdener(i), fdener(i) * adc2g_eh2ev
         end do
+        deallocate(fdcoef, fdvec, fdener)
      end if
   end if
 
@@ -338,6 +347,22 @@ program adc2g_driver
      end if
      call check(rc)
      call adc2g_write_auto(trim(auto_file), nout, tout, auto0)
+     if (fdtd_nsel > 0) then
+        ! calc_fdstates_tdfd: the time-domain filter-diagonalisation states
+        ! of the fbas2eig rows in fdtd_coef_file (fdtd_nsel x fdtd_nfbas),
+        ! written to 'fdstates_td' as (i, E_i, v_i) records
+        allocate(fdcoef(fdtd_nsel, fdtd_nfbas), fdvec(ndim, fdtd_nsel), fdener(fdtd_nsel))
+        open(newunit=u, file=trim(fdtd_coef_file), access='stream', form='unformatted', status='old')
+        read(u) fdcoef
+        close(u)
+        call check(adc2g_sil_fdstates(h, sp, fvec, int(fdtd_nfbas, c_int32_t), fdtd_ebound, &
+             int(fdtd_iwfunc, c_int32_t), int(fdtd_nsel, c_int32_t), fdcoef, fdvec, fdener, ss))
+        call adc2g_write_davstates('fdstates_td', ndim, int(fdtd_nsel, c_int32_t), fdener, fdvec)
+        do i = 1, fdtd_nsel
+           write(*, '(a,i4,f18.12,f14.7)') ' TD-FD state', i, fdener(i), fdener(i) * adc2g_eh2ev
+        end do
+        deallocate(fdcoef, fdvec, fdener)
+     end if
      if (autoord >= 1) call adc2g_write_auto('auto1', nout, tout, auto1)
      if (autoord >= 2) call adc2g_write_auto('auto2', nout, tout, auto2)
      write(*, '(a,i10,a,i10)') ' SIL: steps', ss%nsteps, ', complex sigma-vectors', ss%nsigma

@@ -48,7 +48,10 @@ def test_fortran_driver(tmp_path):
     f.astype("<f8").tofile(tmp_path / "f.bin")
     nml = (f"&adc2g nocc={o} nvirt={v} naux={naux} eps_file='eps.bin' b_file='B.bin' nstates=3 "
            f"blocksize=6 maxvec=30 tol=1d-9 precon=1 davname='davstates' sil=.true. fvec_file='f.bin' "
-           f"auto_file='auto' tout=0.1d0 tfinal=1.0d0 krydim=12 autotol=1d-10 nto_file='trd.bin' /\n")
+           f"auto_file='auto' tout=0.1d0 tfinal=1.0d0 krydim=12 autotol=1d-10 nto_file='trd.bin' "
+           f"fdtd_nsel=2 fdtd_nfbas=5 fdtd_iwfunc=2 fdtd_coef_file='fdt.bin' fdtd_ebound=0.5d0,1.5d0 /\n")
+    fdt = np.random.default_rng(7).standard_normal((2, 5))
+    np.asfortranarray(fdt).T.astype("<f8").tofile(tmp_path / "fdt.bin")     # column-major (2, 5)
     r = subprocess.run([DRIVER], input=nml, capture_output=True, text=True, cwd=tmp_path, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -60,6 +63,12 @@ def test_fortran_driver(tmp_path):
     assert np.abs(ev - w).max() < 1e-10
     X = np.stack([x[2] for x in recs], axis=1)
     assert np.linalg.norm(H @ X - X * ev, axis=0).max() < 1e-8
+    # fdtd_*: the time-domain filter-diagonalisation states
+    fdrec = read_davstates(tmp_path / "fdstates_td", ndim)
+    E_td, V_td = sil.fdstates_td(lambda x: -1j * (H @ x), lambda x: H @ x, f, 0.1, 1.0, 12, 1e-10, (0.5, 1.5),
+                                 fdt, 2)
+    assert np.abs(np.array([x[1] for x in fdrec]) - E_td).max() < 1e-9
+    assert np.abs(np.stack([x[2] for x in fdrec], axis=1) - V_td).max() < 1e-8
     # nto_file: the transition density of every state (adc2_nto_gs input)
     from oracle import nto
     trd = np.fromfile(tmp_path / "trd.bin", dtype="<f8").reshape(3, o, v).transpose(0, 2, 1)
