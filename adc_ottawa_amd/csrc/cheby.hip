@@ -61,6 +61,12 @@ __global__ void __launch_bounds__(CB_THREADS) cheby_step_kernel(int64_t n, doubl
   }
 }
 
+// V[:, i] *= s[i]
+__global__ void fd_scale_kernel(int64_t n, int nsel, const double* __restrict__ s, double* __restrict__ V) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n * nsel; t += (int64_t)gridDim.x * blockDim.x)
+    V[t] *= s[t / n];
+}
+
 }  // namespace
 
 void cheby_run(Handle& H, const double bounds[2], const double* q0_h, int order, Prec p, double* auto_out) {
@@ -172,34 +178,45 @@ void cheby_fdstates_run(Handle& H, const double bounds[2], const double* q0_h, i
 
 // The states in V (ndim x nsel, device) normalised, and E_i = v_i . H v_i,
 // as both filter-diagonalisation programs finish (fdstates_cfd.f90:286-299,
-// fdstates_tdfd.f90:589-600)
+// fdstates_tdfd.f90:589-600): the norms and the Rayleigh quotients are the
+// diagonals of two nsel x nsel FP64 GEMMs (V^T V, V^T H V), so only the
+// result crosses PCIe.
 void fd_finish(Handle& H, double* Vd, int nsel, Prec p, double* vecs, double* energies) {
   const int64_t n = H.d.ndim;
   hipStream_t st = H.st;
-  std::vector<double> hv((size_t)n * nsel);
-  HIP_CHECK(hipMemcpyAsync(hv.data(), Vd, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, st));
-  HIP_CHECK(hipStreamSynchronize(st));
+  DevBuf<double> G, W, sc;
+  G.alloc((size_t)nsel * nsel);
+  sc.alloc((size_t)nsel);
+  auto gram = [&](const double* B) {  // G = V^T B
+    GemmArgs g;
+    g.M = nsel; g.N = nsel; g.K = (int)n;
+    g.A = Vd; g.sam = n; g.sak = 1;
+    g.B = B; g.sbk = 1; g.sbn = n;
+    g.map.C = G.get(); g.map.scm = 1; g.map.scn = nsel;
+    gemm(g, Prec::F64, H.gws, st);
+    std::vector<double> h((size_t)nsel * nsel);
+    HIP_CHECK(hipMemcpyAsync(h.data(), G.get(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    std::vector<double> dg(nsel);
+    for (int i = 0; i < nsel; ++i) dg[i] = h[(size_t)i * nsel + i];
+    return dg;
+  };
+  std::vector<double> nrm = gram(Vd);
   for (int i = 0; i < nsel; ++i) {
-    double* v = hv.data() + (size_t)i * n;
-    double s = 0.0;
-    for (int64_t r = 0; r < n; ++r) s += v[r] * v[r];
-    ARG_CHECK(s > 0.0, "a filter-diagonalisation state is zero");
-    const double inv = 1.0 / std::sqrt(s);
-    for (int64_t r = 0; r < n; ++r) v[r] *= inv;
+    ARG_CHECK(nrm[i] > 0.0, "a filter-diagonalisation state is zero");
+    nrm[i] = 1.0 / std::sqrt(nrm[i]);
   }
-  HIP_CHECK(hipMemcpyAsync(Vd, hv.data(), sizeof(double) * hv.size(), hipMemcpyHostToDevice, st));
-  DevBuf<double> W;
+  HIP_CHECK(hipMemcpyAsync(sc.get(), nrm.data(), sizeof(double) * nsel, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(fd_scale_kernel, dim3(grid_for(n * nsel)), dim3(256), 0, st, n, nsel, sc.get(), Vd);
+  HIP_CHECK(hipGetLastError());
   W.alloc((size_t)n * nsel);
   H.sigma(Vd, n, W.get(), n, nsel, p, st);
-  std::vector<double> hw((size_t)n * nsel);
-  HIP_CHECK(hipMemcpyAsync(hw.data(), W.get(), sizeof(double) * hw.size(), hipMemcpyDeviceToHost, st));
-  HIP_CHECK(hipStreamSynchronize(st));
-  for (int i = 0; i < nsel; ++i) {
-    double e = 0.0;
-    for (int64_t r = 0; r < n; ++r) e += hv[(size_t)i * n + r] * hw[(size_t)i * n + r];
-    energies[i] = e;
+  const std::vector<double> e = gram(W.get());
+  for (int i = 0; i < nsel; ++i) energies[i] = e[i];
+  if (vecs) {
+    HIP_CHECK(hipMemcpyAsync(vecs, Vd, sizeof(double) * n * nsel, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
   }
-  if (vecs) std::copy(hv.begin(), hv.end(), vecs);
 }
 
 // spectral_bounds (source/propagation/specbounds.f90): the lower bound is the

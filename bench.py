@@ -444,6 +444,40 @@ def fvector_measure(op):
             "number_operator_max": float(np.abs(f0).max())}
 
 
+def trdens_measure(op, nvec=20, seed=0):
+    """Occupied-virtual transition density (adc2g_trdens_vo) of nvec seeded
+    vectors: the first call builds M (n1 x n1, one DF GEMM) and the MP2
+    density, later calls are two n1 x n1 x nvec GEMMs and one kernel."""
+    X = np.asfortranarray(np.random.default_rng(seed).standard_normal((op.ndim, nvec)))
+    t0 = time.perf_counter()
+    op.trdens_vo(X[:, :1])
+    t1 = time.perf_counter()
+    T = op.trdens_vo(X)
+    t2 = time.perf_counter()
+    log(f"trdens: first call (M and the MP2 density) {t1 - t0:.2f}s, {nvec} vectors {t2 - t1:.3f}s")
+    return {"first_call_seconds": round(t1 - t0, 3), "seconds": round(t2 - t1, 4), "nvec": nvec,
+            "norm": float(np.linalg.norm(T))}
+
+
+def fdstates_measure(op, sz, order, nsel=10, seed=0):
+    """Chebyshev filter-diagonalisation states (adc2g_cheby_fdstates): order
+    Chebyshev orders from the synthetic dipole vector, nsel states with
+    seeded coefficients, FP64; bounds from the library's spectral_bounds
+    'lanczos' widened by 5 % of the range."""
+    from adc_ottawa_amd.synth import unit_dipole
+    lo, hi = op.spectral_bounds("lanczos")
+    pad = 0.05 * (hi - lo)
+    q0 = np.zeros(op.ndim)
+    q0[: op.n1] = unit_dipole(sz["nocc"], sz["nvirt"], seed=seed)
+    coef = np.random.default_rng(seed).standard_normal((order + 1, nsel)) / (1.0 + np.arange(order + 1))[:, None]
+    t0 = time.perf_counter()
+    e, _ = op.cheby_fdstates(q0, (lo - pad, hi + pad), coef)
+    wall = time.perf_counter() - t0
+    log(f"cheby fdstates: {wall:.2f}s for {order} orders and {nsel} states")
+    return {"seconds": round(wall, 3), "ms_per_order": round(wall / order * 1e3, 3), "order": order,
+            "nsel": nsel, "energies_Eh": [float(x) for x in e]}
+
+
 def laser_measure(op, tfinal, nnz=20_000_000):
     """Laser-driven propagation of the ground state (adc2g_sil_laser) under
     a cos^2 pulse along x, with a synthetic intermediate-state dipole of nnz
@@ -536,6 +570,9 @@ def main():
     ap.add_argument("--skip-fvector", action="store_true", help="skip the F-vector timing")
     ap.add_argument("--laser", type=float, default=2.0,
                     help="laser / CAP propagation leg over this many a.u. (0 skips)")
+    ap.add_argument("--trdens", action="store_true", help="time the transition density of 20 vectors")
+    ap.add_argument("--fdstates", type=int, default=0,
+                    help="Chebyshev filter-diagonalisation states to this order (0 = skip)")
     ap.add_argument("--adc2x-roots", type=int, default=0,
                     help="ADC(2)-x leg: also a Davidson to this many roots (the sigma-vector timing runs by "
                          "default; -1 skips the leg)")
@@ -647,6 +684,8 @@ def main():
     mp2r = guarded("mp2", lambda: mp2_measure(op)) if not args.skip_mp2 else None
     fvr = guarded("fvector", lambda: fvector_measure(op)) if not args.skip_fvector else None
     las = guarded("laser", lambda: laser_measure(op, args.laser)) if args.laser > 0 else None
+    tdr = guarded("trdens", lambda: trdens_measure(op)) if args.trdens else None
+    fds = guarded("fdstates", lambda: fdstates_measure(op, sz, args.fdstates)) if args.fdstates > 0 else None
     # last: the ADC(2)-x leg switches the operator and back
     adx = guarded("adc2x", lambda: adc2x_measure(op, roots=max(args.adc2x_roots, 0))) \
         if args.adc2x_roots >= 0 else None
@@ -682,6 +721,8 @@ def main():
         "mp2": mp2r,
         "fvector": fvr,
         "laser_propagation": las,
+        "transition_density": tdr,
+        "cheby_fdstates": fds,
         "adc2x": adx,
         "operator_build_seconds": round(t_build, 3),
     }
