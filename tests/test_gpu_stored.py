@@ -182,3 +182,45 @@ def test_fortran_driver_stored_matrix(tmp_path):
     bad = nml.replace(" /", " adc2x=.true. /")
     r = subprocess.run([driver], input=bad, capture_output=True, text=True, cwd=tmp_path, timeout=120)
     assert r.returncode != 0 and "dia_file allows" in r.stdout
+
+
+def test_stored_handle_writes_and_runs_lanczos(tmp_path):
+    """A stored-matrix handle writes its own hmlt files back (the same
+    diagonal and elements), and block Lanczos and relaxation on it equal
+    those of the integral operator it came from."""
+    print("[case] stored handle write-back, Lanczos, relaxation o=3 v=6", flush=True)
+    sys_ = make_system(3, 6, 9, seed=9)
+    rng = np.random.default_rng(9)
+    with lib.Operator.from_system(sys_) as op:
+        op.write_hmlt(tmp_path / "a.dia", tmp_path / "a.off", minc=0.0)
+        q0 = np.linalg.qr(rng.normal(size=(op.ndim, 4)))[0]
+        ev_i = op.block_lanczos(q0, 6)[0]
+        e_i = op.relax(2, step=20.0)[0]
+    with lib.Operator.from_hmlt(tmp_path / "a.dia", tmp_path / "a.off") as st:
+        st.write_hmlt(tmp_path / "b.dia", tmp_path / "b.off", minc=0.0)
+        ev_s = st.block_lanczos(q0, 6)[0]
+        e_s = st.relax(2, step=20.0)[0]
+    assert (tmp_path / "a.dia").read_bytes() == (tmp_path / "b.dia").read_bytes()
+
+    def elements(path):
+        data, out, p = open(path, "rb").read(), [], 0
+        while p < len(data):
+            m = int(np.frombuffer(data, np.int32, 1, p)[0])
+            out.append(data[p + 4:p + 4 + m])
+            p += m + 8
+        return out
+    maxbl = int(np.frombuffer(elements(tmp_path / "a.dia")[0], np.int32)[0])
+    sets = []
+    for name in ("a.off", "b.off"):
+        trip = {}
+        for r in elements(tmp_path / name):
+            nlim = int(np.frombuffer(r, np.int32, 1, 16 * maxbl)[0])
+            v = np.frombuffer(r, np.float64, maxbl, 0)[:nlim]
+            i = np.frombuffer(r, np.int32, maxbl, 8 * maxbl)[:nlim]
+            j = np.frombuffer(r, np.int32, maxbl, 12 * maxbl)[:nlim]
+            trip.update({(int(a), int(b)): float(c) for a, b, c in zip(i, j, v)})
+        sets.append(trip)
+    assert sets[0].keys() == sets[1].keys()
+    assert max(abs(sets[0][k] - sets[1][k]) for k in sets[0]) < 1e-13
+    assert np.abs(np.sort(ev_s) - np.sort(ev_i)).max() < 1e-10
+    assert np.abs(np.asarray(e_s) - np.asarray(e_i)).max() < 1e-8
