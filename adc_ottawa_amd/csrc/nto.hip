@@ -15,8 +15,9 @@
 // density-fitting factors once per operator and stays in device memory with
 // rho; each call is then two n1 x n1 x nvec FP64 GEMMs and one kernel.  On a
 // frozen-core view rho is the frozen-core MP2 density (the frozen occupied
-// orbitals out of its occupied sums) and the view's 1h1p amplitudes enter at
-// their configurations; M keeps every orbital, as the reference's loops do.
+// orbitals out of its occupied sums), on a CVS view the full one; the view's
+// 1h1p amplitudes enter at their configurations, and M keeps every orbital,
+// as the reference's loops over the view's kpqf do.
 #include <cmath>
 #include <vector>
 
@@ -77,13 +78,15 @@ __global__ void nto_scatter_kernel(int64_t n1v, int nvec, const int64_t* __restr
 void nto_prepare(Handle& H) {
   if (H.nto_ready) return;
   ARG_CHECK(!H.stored, "the transition density needs the operator built from integrals, not a stored matrix");
-  ARG_CHECK(!H.parent || !H.frozen_occ.empty(), "the transition density is not available on a CVS view");
   Handle& F = H.parent ? *H.parent : H;
   const int o = F.d.o, v = F.d.v, nbas = o + v;
   const int64_t n1 = F.d.n1, nn = n1 * n1;
   hipStream_t st = H.st;
   std::vector<double> rho((size_t)nbas * nbas);
-  mp2_run(H, nullptr, nullptr, rho.data());
+  // a frozen-core view: the frozen-core density; a CVS view: the full one
+  // (rho_mp2 changes only with lifrzcore)
+  const bool cvs = H.parent && H.frozen_occ.empty();
+  mp2_run(cvs ? F : H, nullptr, nullptr, rho.data());
   H.nto_rho.alloc((size_t)nbas * nbas);
   HIP_CHECK(hipMemcpyAsync(H.nto_rho.get(), rho.data(), sizeof(double) * rho.size(), hipMemcpyHostToDevice, st));
   DevBuf<double> G;

@@ -557,8 +557,7 @@ void sil_laser_run(Handle& H, const adc2g_laser_params& p, int npulse, const adc
   if (p.kdim < 2) throw Error(ADC2G_ESIL_ORDER, "IntOrder is too small");
   laser_check_pulses(npulse, pulses);
   ARG_CHECK(!flux || (H.cap.set && H.flux_theta.set), "the flux needs a CAP and the Theta operator");
-  ARG_CHECK(!trdens || (!H.stored && (!H.parent || !H.frozen_occ.empty())),
-            "the transition density needs the operator built from integrals or a frozen-core view of it");
+  ARG_CHECK(!trdens || !H.stored, "the transition density needs the operator built from integrals or a view of it");
   const double intperiod = p.tout;
   const int64_t nrec = (int64_t)(p.tfinal / intperiod);
   ARG_CHECK(nrec_cap >= nrec + 1, "norms holds fewer than nrec+1 values");

@@ -11,8 +11,8 @@ runs on the GPU library: the frozen-core MP2 energy, then the block Davidson
 view of the configurations -- the reference's initial space with frozen_core
 -- or, with --cvs, on the CVS view with the frozen orbitals as core orbitals
 -- the reference's final space with cvs.  Energies are printed in eV
-(1 Eh = 27.2113845 eV, the reference's constant).  With --nto (full or
-frozen-core space) each state's natural transition orbitals as adc2_nto_gs forms them: the
+(1 Eh = 27.2113845 eV, the reference's constant).  With --nto each state's
+natural transition orbitals as adc2_nto_gs forms them: the
 pairs with sigma^2 >= 0.01, hole and particle orbitals in the AO basis.
 """
 from __future__ import annotations
@@ -38,7 +38,7 @@ def main():
     ap.add_argument("--deflate", action="store_true")
     ap.add_argument("--adc2x", action="store_true", help="ADC(2)-x: the first-order 2h2p-2h2p block")
     ap.add_argument("--nto", action="store_true",
-                    help="natural transition orbitals of each state (not with --cvs): sigma^2 and AO coefficients")
+                    help="natural transition orbitals of each state: sigma^2 and AO coefficients")
     a = ap.parse_args()
     r = run(a.log, a.frozen, a.roots, a.cvs, a.block, a.maxvec, a.tol, a.precon, a.guess, a.deflate, adc2x=a.adc2x,
             nto=a.nto)

@@ -72,8 +72,6 @@ def run(log, frozen, roots, cvs=False, block=0, maxvec=0, tol=1e-7, precon="olse
         target = view if view is not None else op
         try:
             g = adc1_guess(full, kind, orbs, target.ndim, block) if guess == "adc1" else guess
-            if nto and kind == "cvs":
-                raise ValueError("the NTOs are not available in the CVS space")
             ev, vecs, st = target.davidson(roots, blocksize=block, maxvec=maxvec, tol=tol, precon=precon,
                                            deflate=deflate, maxiter=1000, vectors=nto, verbose=verbose, guess=g)
             if nto:

@@ -245,7 +245,7 @@ int32_t adc2g_create_cvs(adc2g_handle* full, int32_t ncore, const int32_t* core,
  * with frozen core, source/adclib/mp2.f90:84-90, sums over the active
  * occupied orbitals: adc2g_mp2 of the view does so, and equals adc2g_mp2 of a
  * handle made without the frozen orbitals.)  The same solver entry points as
- * a CVS view accept it, and adc2g_mp2 and adc2g_trdens_vo. */
+ * a CVS view accept it, and adc2g_mp2. */
 int32_t adc2g_create_frozen_core(adc2g_handle* full, int32_t nfrozen, const int32_t* frozen, adc2g_handle** out);
 
 /* ADC(2)-x: with enable = 1 the operator's sigma-vector (every solver entry
@@ -467,8 +467,9 @@ int32_t adc2g_sil_laser(adc2g_handle* h, const adc2g_laser_params* p, int32_t np
  * in memory) and keeps an n1 x n1 FP64 matrix on the device after the first
  * call (n1 = nocc * nvirt).  On a frozen-core view the view's 1h1p
  * amplitudes enter at their configurations, with the frozen-core MP2 density,
- * as the reference's loops over the frozen-core kpqf; trdens then covers all
- * nocc x nvirt orbital pairs.  Not available on a CVS view. */
+ * as the reference's loops over the frozen-core kpqf; on a CVS view the same
+ * with the full MP2 density; trdens then covers all nocc x nvirt orbital
+ * pairs.  Not available on a stored-matrix handle. */
 int32_t adc2g_trdens_vo(adc2g_handle* h, int32_t nvec, const double* x, int64_t ldx, double* trdens);
 
 /* The intermediate-state representation of the projector Theta onto the

@@ -34,15 +34,28 @@ def test_trdens_vo_vs_oracle(o, v, naux, seed, form):
     assert np.array_equal(again, got)
 
 
-def test_trdens_vo_rejects_cvs_views_and_bad_sizes():
+def test_trdens_vo_cvs_view_and_bad_sizes():
+    """On a CVS view the view's 1h1p amplitudes (core holes) enter at their
+    configurations with the full MP2 density (rho_mp2 changes only with
+    frozen core)."""
     print("[case] trdens_vo on a CVS view", flush=True)
     sys_ = make_system(3, 5, 8, seed=3)
+    o, v = sys_.nocc, sys_.nvirt
+    rng = np.random.default_rng(3)
     with lib.Operator.from_system(sys_) as op:
-        with op.cvs([1]) as cv:
-            with pytest.raises(lib.Adc2gError):
-                cv.trdens_vo(np.ones(cv.ndim))
+        with op.cvs([1, 3]) as cv:
+            Xv = rng.normal(size=(cv.ndim, 2))
+            T = cv.trdens_vo(Xv)
+            n1v = cv.n1
         with pytest.raises(ValueError):
             op.trdens_vo(np.ones(op.ndim - 1))
+    rows = np.array([a * o + i for a in range(v) for i in (0, 2)])
+    assert rows.size == n1v
+    for s in range(2):
+        x = np.zeros(o * v)
+        x[rows] = Xv[:n1v, s]
+        ref = nto.trdens_vo(sys_, x)
+        assert np.abs(T[s] - ref).max() < 1e-12 * np.abs(ref).max(), s
 
 
 def test_nto_of_the_gpu_density():
