@@ -224,3 +224,29 @@ def test_stored_handle_writes_and_runs_lanczos(tmp_path):
     assert max(abs(sets[0][k] - sets[1][k]) for k in sets[0]) < 1e-13
     assert np.abs(np.sort(ev_s) - np.sort(ev_i)).max() < 1e-10
     assert np.abs(np.asarray(e_s) - np.asarray(e_i)).max() < 1e-8
+
+
+def test_stored_handle_laser_operator(tmp_path):
+    """The laser operator (adc2g_matxvec_treal_laser) on a stored-matrix
+    handle equals the integral operator's: -i H from the stored elements,
+    the IS dipole terms unchanged."""
+    print("[case] stored handle laser operator o=3 v=5", flush=True)
+    sys_ = make_system(3, 5, 8, seed=10)
+    rng = np.random.default_rng(10)
+    pulses = [{"envelope": "box", "envpar": (0.0, 10.0), "carrier": "cos", "freq": 0.3,
+               "pulse_vec": (1.0, 0.0, 0.0), "strength": 0.1}]
+    with lib.Operator.from_system(sys_) as op:
+        nd = op.ndim
+        row = rng.integers(0, nd, 2 * nd)
+        col = (row + 1 + rng.integers(0, nd - 1, row.size)) % nd
+        val = rng.normal(0.0, 0.05, row.size)
+        diag = rng.normal(0.0, 0.2, nd)
+        d0j = rng.normal(0.0, 0.1, nd)
+        x = rng.normal(size=nd + 1) + 1j * rng.normal(size=nd + 1)
+        op.write_hmlt(tmp_path / "h.dia", tmp_path / "h.off", minc=0.0)
+        op.set_dipole_is(0, diag, row, col, val, 0.2, d0j)
+        want = op.matxvec_treal_laser(1.5, pulses, x)
+    with lib.Operator.from_hmlt(tmp_path / "h.dia", tmp_path / "h.off") as st:
+        st.set_dipole_is(0, diag, row, col, val, 0.2, d0j)
+        got = st.matxvec_treal_laser(1.5, pulses, x)
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
